@@ -1,0 +1,282 @@
+"""Benchmark of the MI355X dose engine (one GPU).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+Headline workload (one step): the plain IMRT CCCS plan of a 128^3 water
+phantom at 2.5 mm with a seeded lung ellipsoid and bone cylinder, 7 coplanar
+6 MV beams (gantry 360*b/7) with a 10x10 cm MLC field, VarianTrueBeamHF
+model: per beam ray trace + TERMA + 72-cone superposition; density resident
+in HBM.  `value` is ms per beam.  The same line carries the plain IMPT field
+(100^3 at 2.5 mm, water with a seeded 2 cm bone slab, 20x20 spots x 10
+layers of 100-160 MeV, unit MU) in Gvoxel-spot/s, the roofline of the
+dominant kernel from HIP events around its launches, and the FP64 oracle
+timed on a bounded sample on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    p.add_argument("--cpu-seconds", type=float, default=15.0,
+                   help="approximate budget of the CPU baseline sample")
+    p.add_argument("--workload", default="all", choices=["all", "cccs", "impt"])
+    return p.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+# ---------------------------------------------------------------------------
+def setup_cccs(torch):
+    from dosecuda_amd import phantoms
+    from dosecuda_amd.beam_models import PhotonBeamModel, square_field_mlc
+    pm = PhotonBeamModel()
+    case = phantoms.imrt_lung_bone_case(seed=1002, n=128, spacing=2.5)
+    dens = phantoms.to_density(case)
+    mlc = square_field_mlc(pm, 10.0, 10.0)
+    d = torch.from_numpy(dens).cuda()
+    return dict(pm=pm, case=case, dens=dens, d_dens=d, mlc=mlc, angles=phantoms.imrt_beams(7),
+                out=torch.zeros_like(d), acc=torch.zeros_like(d))
+
+
+def run_cccs(torch, st):
+    from dosecuda_amd import engine
+    c = st["case"]
+    st["acc"].zero_()
+    for ga in st["angles"]:
+        engine.photon_dose(st["d_dens"], c.spacing, c.origin, c.iso, 100.0, ga, 0.0, 0.0,
+                           st["pm"], st["mlc"], out=st["out"])
+        st["acc"].add_(st["out"])
+
+
+def setup_impt(torch):
+    from dosecuda_amd import engine, phantoms
+    from dosecuda_amd.beam_models import ProtonBeamModel
+    m = ProtonBeamModel()
+    case = phantoms.impt_slab_case(seed=1001, n=100, spacing=2.5)
+    rlsp = phantoms.to_rlsp(case)
+    spots, ids = phantoms.impt_spot_grid(m, seed=1001, n_side=20, pitch=5.0, n_layers=10,
+                                         e_lo=100.0, e_hi=160.0)
+    d = torch.from_numpy(rlsp).cuda()
+    plan = engine.ProtonPlan(0, rlsp.shape, case.spacing, case.origin, case.iso, 0.0, 0.0, m, spots)
+    return dict(m=m, case=case, rlsp=rlsp, d_rlsp=d, spots=spots, plan=plan,
+                wet=torch.empty_like(d), dose=torch.empty_like(d))
+
+
+def run_impt(torch, st):
+    from dosecuda_amd import engine
+    c, m = st["case"], st["m"]
+    engine.proton_wet(st["d_rlsp"], c.spacing, c.origin, c.iso, 0.0, 0.0, m.VSADX, m.VSADY,
+                      out=st["wet"])
+    st["plan"].dose(st["wet"], out=st["dose"])
+
+
+def cone_steps(L0, spacing_mm):
+    """Steps of the FP32 march `for (L = L0; L >= 0; L -= spacing/10)`."""
+    sp = np.float32(spacing_mm) / np.float32(10.0)
+    L, n = np.float32(L0), 0
+    while L >= np.float32(0.0):
+        n += 1
+        L = np.float32(L - sp)
+    return n
+
+
+def timed(torch, fn, st, steps, warmup):
+    for _ in range(warmup):
+        fn(torch, st)
+    torch.cuda.synchronize()
+    from dosecuda_amd import engine
+    engine.prof_reset()
+    engine.prof_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn(torch, st)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    engine.prof_enable(False)
+    return (t1 - t0) * 1000.0 / steps
+
+
+# ---------------------------------------------------------------------------
+def cpu_baseline_cccs(st, budget_s):
+    """FP64 oracle CCCS for beam 0 on a slab of z-planes through the field,
+    scaled by active-voxel count to ms/beam."""
+    from oracle import oracle as O
+    pm, c = st["pm"], st["case"]
+    dens = st["dens"]
+    ga = st["angles"][0]
+    g = O.geom(dens.shape, c.spacing, c.origin, c.iso, ga, 0.0, 0.0, np.float32(pm.primary_source_distance))
+    t0 = time.perf_counter()
+    raw, _ = O.raytrace(g, dens, float(np.float32(0.0012)))
+    tr, el = O.terma(g, pm, st["mlc"], raw.astype(np.float32))
+    t_pre = time.perf_counter() - t0
+    t32, e32 = tr.astype(np.float32), el.astype(np.float32)
+    active = ~(t32 <= 0.01)
+    n_act = int(active.sum())
+    nz = dens.shape[0]
+    planes = 1
+    mask = np.zeros(dens.shape, np.uint8)
+    while True:
+        mask[:] = 0
+        k0 = nz // 2 - planes // 2
+        mask[k0:k0 + planes] = 1
+        t1 = time.perf_counter()
+        O.ccc(g, pm, 100.0, t32, dens, e32, mask)
+        dt = time.perf_counter() - t1
+        n_s = int((active & (mask > 0)).sum())
+        if dt > budget_s / 4 or planes >= nz:
+            break
+        planes = min(nz, planes * 2)
+    per_vox = dt / max(n_s, 1)
+    ms_beam = (t_pre + per_vox * n_act) * 1000.0
+    return ms_beam, f"beam 0 of 7: ray trace + TERMA on the full 128^3 grid, CCCS on {planes} " \
+                    f"z-planes ({n_s} of {n_act} active voxels), scaled by active voxels"
+
+
+def cpu_baseline_impt(st, budget_s):
+    from oracle import oracle as O
+    m, c = st["m"], st["case"]
+    t0 = time.perf_counter()
+    raw, sm, _ = O.proton_wet(st["rlsp"], c.spacing, c.origin, c.iso, 0.0, 0.0, m.VSADX, m.VSADY)
+    t_wet = time.perf_counter() - t0
+    wet32 = sm.astype(np.float32)
+    # pencil beam on a subset of spots (one layer at a time) scaled by pairs
+    spots = st["spots"]
+    ids = np.unique(spots[:, 3])
+    g = O.geom(wet32.shape, c.spacing, c.origin, c.iso, 0.0, 0.0, 0.0,
+               (np.float32(m.VSADX) + np.float32(m.VSADY)) / np.float32(2.0))
+    total_pairs = st["plan"].pairs(wet32)
+    done_pairs, t_p = 0, 0.0
+    for e in ids:
+        sel = spots[spots[:, 3] == e]
+        t1 = time.perf_counter()
+        _, pairs = O.pencil(g, m.VSADX, m.VSADY, wet32, sel, m.divergence_params, m.lut_depths,
+                            m.lut_sigmas, m.lut_idds)
+        t_p += time.perf_counter() - t1
+        done_pairs += pairs
+        if t_p > budget_s:
+            break
+    t_field = t_wet + t_p * total_pairs / max(done_pairs, 1)
+    return total_pairs / t_field / 1e9, f"WET on the full grid, pencil beam on {done_pairs} of " \
+                                         f"{total_pairs} voxel-spot pairs, scaled by pairs"
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of "
+                                   "this build (see DESIGN.md for how the path would shard)",
+                          "n_gpus": a.gpus}))
+        sys.exit(2)
+    import torch
+    from dosecuda_amd import _lib, engine
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no GPU visible"}))
+        sys.exit(1)
+    _lib.load()
+    res = {}
+    cpu = {}
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+    if a.workload in ("all", "impt"):
+        log("setting up IMPT 100^3 field")
+        st = setup_impt(torch)
+        ms = timed(torch, run_impt, st, a.steps, a.warmup)
+        wet = st["wet"].cpu().numpy()
+        pairs = st["plan"].pairs(wet)
+        kt = {k: engine.prof_get(k) for k in ("pad", "raytrace", "smooth", "pencil")}
+        pen_ms = kt["pencil"][0] / max(kt["pencil"][1], 1)
+        res["impt"] = dict(ms_per_field=ms, pairs=pairs, gvoxel_spot_per_s=pairs / (ms * 1e-3) / 1e9,
+                           pencil_ms=pen_ms, pencil_gvoxel_spot_per_s=pairs / (pen_ms * 1e-3) / 1e9,
+                           kernels_ms={k: v[0] / max(v[1], 1) for k, v in kt.items()})
+        log(f"IMPT: {ms:.3f} ms/field, {res['impt']['gvoxel_spot_per_s']:.1f} Gvoxel-spot/s "
+            f"(pencil {pen_ms:.3f} ms)")
+        if not a.no_cpu:
+            v, sample = cpu_baseline_impt(st, a.cpu_seconds / 2)
+            cpu["impt"] = dict(value=v, unit="Gvoxel-spot/s", sample=sample)
+            log(f"IMPT CPU oracle: {v:.4f} Gvoxel-spot/s")
+        st["plan"].close()
+        del st
+
+    if a.workload in ("all", "cccs"):
+        log("setting up CCCS 128^3 x 7 beams")
+        st = setup_cccs(torch)
+        ms = timed(torch, run_cccs, st, a.steps, a.warmup)
+        nb = len(st["angles"])
+        kt = {k: engine.prof_get(k) for k in ("pad", "raytrace", "terma", "pack_tr", "ccc_compact", "ccc")}
+        ccc_ms = kt["ccc"][0] / max(kt["ccc"][1], 1)
+        # algorithmic bytes of one ccc launch: the padded {TERMA, density}
+        # volume once (8 B per padded voxel) + per active voxel: list index,
+        # electron read and dose write (12 B)
+        c = st["case"]
+        from dosecuda_amd import _lib as L
+        n = st["dens"].size
+        npad = (st["dens"].shape[0] + 3) * (st["dens"].shape[1] + 3) * (st["dens"].shape[2] + 3)
+        # active voxels of the last beam
+        terma = torch.empty_like(st["d_dens"])
+        engine.photon_dose(st["d_dens"], c.spacing, c.origin, c.iso, 100.0, st["angles"][-1], 0.0,
+                           0.0, st["pm"], st["mlc"], out=st["out"], terma_out=terma)
+        n_act = int((~(terma <= 0.01)).sum().item())
+        bytes_launch = 8 * npad + 12 * n_act
+        achieved = bytes_launch / (ccc_ms * 1e-3) / 1e9
+        samples = n_act * 12 * sum(cone_steps(L0, c.spacing) for L0 in st["pm"].kernel[:, 5])
+        res["cccs"] = dict(ms_per_step=ms, ms_per_beam=ms / nb, ccc_ms=ccc_ms, n_active=n_act,
+                           cone_samples_per_beam=samples,
+                           gsamples_per_s=samples / (ccc_ms * 1e-3) / 1e9,
+                           kernels_ms={k: v[0] / max(v[1], 1) for k, v in kt.items()},
+                           roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS,
+                                         unit="GB/s", frac=achieved / HBM_PEAK_GBS, traffic=None))
+        log(f"CCCS: {ms / nb:.3f} ms/beam (ccc kernel {ccc_ms:.3f} ms, {n_act} active voxels)")
+        if not a.no_cpu:
+            v, sample = cpu_baseline_cccs(st, a.cpu_seconds)
+            cpu["cccs"] = dict(value=v, unit="ms/beam", sample=sample)
+            log(f"CCCS CPU oracle: {v:.1f} ms/beam")
+
+    cc = res.get("cccs")
+    line = {
+        "metric": "IMPT Gvoxel-spot/s + CCCS ms/beam; surrogate vs plain at 2%/2mm gamma>=99%",
+        "value": cc["ms_per_beam"] if cc else None,
+        "unit": "ms/beam",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": cc["ms_per_step"] if cc else None,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded phantoms; unchanged DoseCUDA beam-model tables)",
+        "config": {"workload": "plain IMRT CCCS, 128^3 @ 2.5 mm water + lung ellipsoid + bone "
+                               "cylinder (seed 1002), 7 coplanar 6 MV beams, 10x10 cm MLC field, "
+                               "VarianTrueBeamHF",
+                   "beams": 7, "grid": [128, 128, 128], "spacing_mm": 2.5},
+        "roofline": cc["roofline"] if cc else None,
+        "cpu_baseline": ({"value": cpu["cccs"]["value"], "unit": "ms/beam", "cores": threads,
+                          "kind": "port", "sample": cpu["cccs"]["sample"]} if "cccs" in cpu else None),
+        "impt": res.get("impt"),
+        "impt_cpu_baseline": ({**cpu["impt"], "cores": threads, "kind": "port"} if "impt" in cpu else None),
+        "cccs_detail": cc,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,150 @@
+// dh_common.h -- shared device/host definitions of the MI355X dose engine.
+//
+// Everything here is compiled with -ffp-contract=off: the FP32 geometry that
+// decides which samples exist (ray exit, rounded neighbour indices) is written
+// as explicit multiplies/adds, and explicit fmaf() where the reference calls
+// fmaf (DoseCUDA/dose_kernels/CudaClasses.cuh:100-104, CudaClasses.cu:201-216).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define DH_PI_F 3.141592654f  // CUDART_PI_F, used by the reference for angles
+#define DH_LOG2E_F 1.44269504088896340736f
+
+// Padded volume: one border voxel below and two above on every axis, so that
+// trilinear corners of any clamped coordinate u in [-1, n] are in range and
+// out-of-image corners read the border value (TextureClasses.cu:22-31).
+struct dh_pad {
+    int px, py, pz;      // nx + 3, ny + 3, nz + 3
+    int64_t pxy;         // px * py
+};
+
+static inline dh_pad dh_make_pad(int nx, int ny, int nz)
+{
+    dh_pad p;
+    p.px = nx + 3; p.py = ny + 3; p.pz = nz + 3;
+    p.pxy = (int64_t)p.px * p.py;
+    return p;
+}
+
+// Per-beam geometry, passed by value as a kernel argument (lands in SGPRs).
+struct dh_geom {
+    int nx, ny, nz;
+    float s, inv_s;
+    float iso[3];        // iso - origin, mm
+    float src[3];        // source position relative to iso, mm
+    float singa, cosga, sinta, costa, sinca, cosca;
+};
+
+// ---------------------------------------------------------------------------
+// FP32 geometry shared by all kernels (host + device).
+// ---------------------------------------------------------------------------
+__host__ __device__ static inline void dh_vox_xyz(const dh_geom &g, int i, int j, int k,
+                                                  float &x, float &y, float &z)
+{   // CudaClasses.cuh:106-110
+    x = (float)i * g.s - g.iso[0];
+    y = (float)j * g.s - g.iso[1];
+    z = (float)k * g.s - g.iso[2];
+}
+
+// Unit vector towards the source (CudaClasses.cuh:41-52).  The norm is the
+// correctly rounded reciprocal square root of the FP32 sum of squares, taken
+// through FP64 so host and device agree bit for bit.
+__host__ __device__ static inline void dh_uvec(const dh_geom &g, float x, float y, float z,
+                                               float &ux, float &uy, float &uz)
+{
+    float dx = g.src[0] - x, dy = g.src[1] - y, dz = g.src[2] - z;
+    float ss = dx * dx + dy * dy + dz * dz;
+    float r = (float)(1.0 / sqrt((double)ss));
+    ux = dx * r; uy = dy * r; uz = dz * r;
+}
+
+// Image -> head with couch, gantry and collimator rotations
+// (IMRTClasses.cu:41-79); sinca = 0, cosca = 1 reduces it exactly to the
+// proton version (CudaClasses.cu:65-95).
+__host__ __device__ static inline void dh_img_to_head(const dh_geom &g, float x, float y, float z,
+                                                      float &hx, float &hy, float &hz)
+{
+    float sx = -g.sinta, cx = g.costa;
+    float xt = x * cx + z * sx;
+    float yt = y;
+    float zt = -x * sx + z * cx;
+    sx = -g.singa; cx = g.cosga;
+    float xg = xt * cx - yt * sx;
+    float yg = xt * sx + yt * cx;
+    float zg = zt;
+    sx = -g.sinca; cx = g.cosca;
+    float xc = xg * cx + zg * sx;
+    float yc = yg;
+    float zc = -xg * sx + zg * cx;
+    hx = -xc; hy = zc; hz = yc;
+}
+
+// Head -> image (IMRTClasses.cu:81-118; CudaClasses.cu:97-126).
+__host__ __device__ static inline void dh_head_to_img(const dh_geom &g, float hx, float hy, float hz,
+                                                      float &x, float &y, float &z)
+{
+    float xz = -hx, yz = hz, zz = hy;
+    float sx = g.sinca, cx = g.cosca;
+    float xc = xz * cx + zz * sx;
+    float yc = yz;
+    float zc = -xz * sx + zz * cx;
+    sx = g.singa; cx = g.cosga;
+    float xg = xc * cx - yc * sx;
+    float yg = xc * sx + yc * cx;
+    float zg = zc;
+    sx = g.sinta; cx = g.costa;
+    x = xg * cx + zg * sx;
+    y = yg;
+    z = -xg * sx + zg * cx;
+}
+
+// Linear index of voxel (i, j, k) in the padded layout.
+__host__ __device__ static inline int64_t dh_pad_index(const dh_pad &p, int i, int j, int k)
+{
+    return (int64_t)(i + 1) + (int64_t)p.px * (j + 1) + p.pxy * (k + 1);
+}
+
+#ifdef __HIPCC__
+// Trilinear sample of a padded scalar volume at texel coordinate (tx,ty,tz)
+// (texel centre of voxel i at t = i + 0.5).  u = t - 0.5 is clamped to
+// [-1, n]; outside that range every corner is border anyway.
+__device__ static inline float dh_trilinear(const float *__restrict__ vol, const dh_pad &p,
+                                            int nx, int ny, int nz,
+                                            float tx, float ty, float tz)
+{
+    float ux = __builtin_amdgcn_fmed3f(tx - 0.5f, -1.0f, (float)nx);
+    float uy = __builtin_amdgcn_fmed3f(ty - 0.5f, -1.0f, (float)ny);
+    float uz = __builtin_amdgcn_fmed3f(tz - 0.5f, -1.0f, (float)nz);
+    float fx0 = floorf(ux), fy0 = floorf(uy), fz0 = floorf(uz);
+    float ax = ux - fx0, ay = uy - fy0, az = uz - fz0;
+    int i0 = (int)fx0, j0 = (int)fy0, k0 = (int)fz0;
+    const float *b = vol + dh_pad_index(p, i0, j0, k0);
+    const float *c = b + p.pxy;
+    float v000 = b[0], v100 = b[1], v010 = b[p.px], v110 = b[p.px + 1];
+    float v001 = c[0], v101 = c[1], v011 = c[p.px], v111 = c[p.px + 1];
+    float a00 = fmaf(ax, v100 - v000, v000);
+    float a10 = fmaf(ax, v110 - v010, v010);
+    float a01 = fmaf(ax, v101 - v001, v001);
+    float a11 = fmaf(ax, v111 - v011, v011);
+    float b0 = fmaf(ay, a10 - a00, a00);
+    float b1 = fmaf(ay, a11 - a01, a01);
+    return fmaf(az, b1 - b0, b0);
+}
+
+// Wave-level helpers (wave64).
+__device__ static inline bool dh_wave_any(bool pred)
+{
+    return __builtin_amdgcn_ballot_w64(pred) != 0ull;
+}
+
+// Neumaier-compensated accumulation of x into (s, c).
+__device__ static inline void dh_neumaier(float &s, float &c, float x)
+{
+    float t = s + x;
+    if (fabsf(s) >= fabsf(x)) c += (s - t) + x;
+    else c += (x - t) + s;
+    s = t;
+}
+#endif

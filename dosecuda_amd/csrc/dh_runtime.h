@@ -1,0 +1,113 @@
+// dh_runtime.h -- host runtime of the dose engine: errors, per-device scratch
+// arena, default streams, launch timing, and the kernel launchers shared by
+// the C ABI (api.cpp) and the batching code.
+#pragma once
+
+#include "dh_common.h"
+#include "../../include/dosehip.h"
+
+#include <stdexcept>
+#include <string>
+
+struct dh_error : std::runtime_error {
+    int code;
+    dh_error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define DH_HIP(expr)                                                          \
+    do {                                                                      \
+        hipError_t e_ = (expr);                                               \
+        if (e_ != hipSuccess) throw dh_error(DH_E_RUNTIME, hipGetErrorString(e_)); \
+    } while (0)
+
+#define DH_REQUIRE(cond, msg)                                                 \
+    do {                                                                      \
+        if (!(cond)) throw dh_error(DH_E_INVALID, (msg));                     \
+    } while (0)
+
+// Selects the device and initialises its state once.
+void dh_device_enter(int dev);
+// Library stream of a device (non-blocking, default priority).
+hipStream_t dh_stream(int dev);
+// Grow-only named scratch buffer on a device.  Growing synchronises the
+// device before releasing the old allocation.  Work that uses the same slot
+// from different streams at the same time races.
+void *dh_scratch(int dev, int slot, size_t bytes);
+int dh_current_device();
+
+enum dh_slot {
+    DH_SLOT_VOL_PAD = 0,   // padded scalar volume for the ray-trace
+    DH_SLOT_WET_RAW,       // unsmoothed WET
+    DH_SLOT_TR_PAD,        // padded {TERMA, density} pairs for the CCCS
+    DH_SLOT_ACTIVE,        // compacted list of CCCS voxels
+    DH_SLOT_COUNTER,       // small integer counters
+    DH_SLOT_IN0,           // host-buffer staging
+    DH_SLOT_IN1,
+    DH_SLOT_OUT0,
+    DH_SLOT_OUT1,
+    DH_SLOT_OUT2,
+    DH_SLOT_TMP0,
+    DH_SLOT_TMP1,
+    DH_SLOT_COUNT_
+};
+
+// Launch timing: when enabled, launches are bracketed by HIP events on the
+// launch stream under a kernel name.
+struct dh_timed {
+    const char *name;
+    hipStream_t s;
+    void *ev0;
+    dh_timed(const char *n, hipStream_t st);
+    ~dh_timed();
+};
+
+// ---------------------------------------------------------------------------
+// Kernel launch interfaces
+// ---------------------------------------------------------------------------
+constexpr int DH_MAX_PAIRS = 128;    // MLC leaf pairs passed by value
+constexpr int DH_MAX_PROFILE = 32;   // off-axis profile points
+constexpr int DH_MAX_SPECTRAL = 16;  // spectral energies
+
+// proton layer record (IMPTClasses.cuh:17-26 plus per-layer constants)
+struct dh_layer_k {
+    int start, n;
+    float r80, cut, A, B, C, pad_;
+};
+
+struct dh_terma_args {
+    dh_geom g;
+    int n_pairs, n_prof, n_spec;
+    float sad, ssd_s, inv_p, inv_s_src, d_mlc, ssw, el_att, esw, el_norm_a, el_norm_b;
+    float4 mlc[DH_MAX_PAIRS];        // scaled leaf edges {xL, xR, yB, yT} of open pairs
+    float prof_r[DH_MAX_PROFILE], prof_i[DH_MAX_PROFILE], prof_s[DH_MAX_PROFILE];
+    float mu_e[DH_MAX_SPECTRAL], pw[DH_MAX_SPECTRAL], sw[DH_MAX_SPECTRAL];
+};
+
+struct dh_ccc_args {
+    dh_geom g;
+    dh_pad p;
+    float ax, ay, az;               // unit vector iso -> source
+    float sp;                       // spacing in cm
+    float mu_cal, mu;
+    int nsteps[6];
+    float L0[6], a[6], b[6], c1[6], c2p[6], bp[6], a2[6];
+    float dir[72 * 3];              // cone directions in image space
+};
+
+void dh_launch_pad(const float *src, float *dst, int nx, int ny, int nz, float border,
+                   hipStream_t s);
+void dh_launch_raytrace(const dh_geom &g, const float *vol_pad, float *wet, hipStream_t s);
+void dh_launch_smooth(const dh_geom &g, const float *wet_raw, float *wet_out, hipStream_t s);
+void dh_launch_pencil(const dh_geom &g, float vsadx, float vsady, int n_layers,
+                      const dh_layer_k *d_layers, const float4 *d_luts, const float *d_spots,
+                      bool fold, const float *wet, float *dose, hipStream_t s);
+void dh_launch_terma(const dh_terma_args &a, const float *wet, float *terma, float *electron,
+                     hipStream_t s);
+void dh_launch_ccc(const dh_ccc_args &a, const float *terma, const float *density,
+                   const float *electron, float2 *tr_pad, int *active, int *counter,
+                   float *dose, hipStream_t s);
+
+// Host-side beam set-up shared by every entry point (dosemodule.cu:148-172,
+// CudaClasses.cu:17-51, IMRTClasses.cu:30-39).
+dh_geom dh_make_geom(const dh_grid &grid, const float iso[3], double gantry, double couch,
+                     double collimator, float src_dist);

@@ -1,0 +1,265 @@
+// impt.hip -- proton path on gfx950: volume padding, radiological-depth ray
+// trace, lateral WET smoothing and the double-Gaussian pencil-beam sum.
+//
+// Layout: every volume is the caller's C-order (nz, ny, nx) float32 array;
+// consecutive lanes of a wave64 own consecutive x voxels, so every gather of
+// a wave touches a few adjacent cache lines.  Sampled volumes are copied once
+// into a padded layout (dh_pad) whose border cells hold the texture border
+// value, which makes trilinear sampling branch-free.
+#include "dh_runtime.h"
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__global__ void __launch_bounds__(kBlock)
+k_pad(const float *__restrict__ src, float *__restrict__ dst, int nx, int ny, int nz,
+      dh_pad p, float border)
+{
+    const uint32_t n = (uint32_t)(p.pxy * p.pz);
+    for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < n; t += gridDim.x * kBlock) {
+        int i = (int)(t % (uint32_t)p.px) - 1;
+        uint32_t r = t / (uint32_t)p.px;
+        int j = (int)(r % (uint32_t)p.py) - 1;
+        int k = (int)(r / (uint32_t)p.py) - 1;
+        float v = border;
+        if (i >= 0 && i < nx && j >= 0 && j < ny && k >= 0 && k < nz)
+            v = src[(uint32_t)i + (uint32_t)nx * ((uint32_t)j + (uint32_t)ny * (uint32_t)k)];
+        dst[t] = v;
+    }
+}
+
+// Radiological depth to the source (CudaClasses.cu:180-228): samples every
+// 1 mm from the voxel centre towards the source until the sample position
+// leaves the volume; WET = -0.05 + 0.1 * sum(max(rho, 0)) cm.  The sample
+// lattice and exit test are the reference's FP32 arithmetic; the running sum
+// is FP64 so long heterogeneous rays do not lose low-order bits.
+__global__ void __launch_bounds__(kBlock)
+k_raytrace(dh_geom g, dh_pad p, const float *__restrict__ vol, float *__restrict__ wet)
+{
+    const uint32_t nvox = (uint32_t)g.nx * g.ny * g.nz;
+    const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+    if (v >= nvox) return;
+    const int i = (int)(v % (uint32_t)g.nx);
+    const uint32_t r = v / (uint32_t)g.nx;
+    const int j = (int)(r % (uint32_t)g.ny);
+    const int k = (int)(r / (uint32_t)g.ny);
+    float x, y, z, ux, uy, uz;
+    dh_vox_xyz(g, i, j, k, x, y, z);
+    dh_uvec(g, x, y, z, ux, uy, uz);
+    const float fnx = (float)g.nx, fny = (float)g.ny, fnz = (float)g.nz;
+    double acc = 0.0;
+    float L = 0.0f;
+    for (;;) {
+        float qx = fmaf(ux, L, x), qy = fmaf(uy, L, y), qz = fmaf(uz, L, z);
+        float tx = fmaf(g.inv_s, qx + g.iso[0], 0.5f);
+        float ty = fmaf(g.inv_s, qy + g.iso[1], 0.5f);
+        float tz = fmaf(g.inv_s, qz + g.iso[2], 0.5f);
+        if (!(tx >= 0.0f && tx < fnx && ty >= 0.0f && ty < fny && tz >= 0.0f && tz < fnz)) break;
+        float d = dh_trilinear(vol, p, g.nx, g.ny, g.nz, tx, ty, tz);
+        acc += (double)fmaxf(d, 0.0f);
+        L += 1.0f;
+    }
+    wet[v] = (float)(-0.05 + 0.1 * acc);
+}
+
+struct smooth_dirs {
+    float c[6], s[6];
+};
+
+// Lateral smoothing (IMPTClasses.cu:139-202): mean of the voxel's WET and the
+// nearest-voxel WETs at 1..9 mm along six beam's-eye-view directions, taking
+// only radii below 10 * WET(cm).  Neighbour indices come from the reference's
+// FP32 head->image rotation and roundf().
+__global__ void __launch_bounds__(kBlock)
+k_smooth(dh_geom g, smooth_dirs dirs, const float *__restrict__ wet_in, float *__restrict__ wet_out)
+{
+    const uint32_t nvox = (uint32_t)g.nx * g.ny * g.nz;
+    const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+    if (v >= nvox) return;
+    const int i = (int)(v % (uint32_t)g.nx);
+    const uint32_t r = v / (uint32_t)g.nx;
+    const int j = (int)(r % (uint32_t)g.ny);
+    const int k = (int)(r / (uint32_t)g.ny);
+    float x, y, z, hx, hy, hz;
+    dh_vox_xyz(g, i, j, k, x, y, z);
+    dh_img_to_head(g, x, y, z, hx, hy, hz);
+    const float c = wet_in[v];
+    double sum = (double)c;
+    int n = 1;
+    const float lim = c * 10.0f;
+    for (int a = 0; a < 6; a++) {
+        for (float dr = 1.0f; dr < lim && dr < 10.0f; dr += 1.0f) {
+            float qx, qy, qz;
+            dh_head_to_img(g, hx + dr * dirs.c[a], hy + dr * dirs.s[a], hz, qx, qy, qz);
+            unsigned ii = (unsigned)(int)roundf((qx + g.iso[0]) / g.s);
+            unsigned jj = (unsigned)(int)roundf((qy + g.iso[1]) / g.s);
+            unsigned kk = (unsigned)(int)roundf((qz + g.iso[2]) / g.s);
+            if (ii < (unsigned)g.nx && jj < (unsigned)g.ny && kk < (unsigned)g.nz) {
+                sum += (double)wet_in[ii + (unsigned)g.nx * (jj + (unsigned)g.ny * kk)];
+                n++;
+            }
+        }
+    }
+    wet_out[v] = (float)(sum / (double)n);
+}
+
+// Pencil-beam sum (IMPTClasses.cu:204-271).  One lane owns one voxel and walks
+// every layer in a fixed order, so there are no atomics and the result is
+// bit-reproducible.  Per layer the workgroup stages the layer's IDD/sigma
+// table in LDS; per (voxel, layer) the range cut, LUT interpolation, sigma_air,
+// nuclear halo and the two Gaussian normalisations are evaluated once; the
+// spot loop then reads wave-uniform spot records through the scalar cache.
+// With `Fold`, the normalisation is folded into the exponent,
+// P*exp(a d^2) = 2^(a' d^2 + log2 P), which requires P, H >= 0 (the host checks
+// that every IDD sample of the plan is non-negative).
+template <bool Fold>
+__global__ void __launch_bounds__(kBlock)
+k_pencil(dh_geom g, int n_layers, const dh_layer_k *__restrict__ layers,
+         const float4 *__restrict__ luts, const float *__restrict__ spots,
+         const float *__restrict__ wet, float *__restrict__ dose)
+{
+    __shared__ float4 s_lut[DH_LUT_LENGTH];
+    const uint32_t nvox = (uint32_t)g.nx * g.ny * g.nz;
+    const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+    const bool valid = v < nvox;
+    const uint32_t vv = valid ? v : nvox - 1;
+    const int i = (int)(vv % (uint32_t)g.nx);
+    const uint32_t r = vv / (uint32_t)g.nx;
+    const int j = (int)(r % (uint32_t)g.ny);
+    const int k = (int)(r / (uint32_t)g.ny);
+    float x, y, z, hx, hy, hz;
+    dh_vox_xyz(g, i, j, k, x, y, z);
+    dh_img_to_head(g, x, y, z, hx, hy, hz);
+    const float dxs = g.src[0] - x, dys = g.src[1] - y, dzs = g.src[2] - z;
+    const float dist = sqrtf(dxs * dxs + dys * dys + dzs * dzs);   // norm3df
+    const float w = wet[vv] * 10.0f;                                // cm -> mm
+    const float hz2 = hz * hz;
+    float tot = 0.0f, comp = 0.0f;
+
+    for (int l = 0; l < n_layers; l++) {
+        const dh_layer_k L = layers[l];
+        __syncthreads();
+        for (int q = threadIdx.x; q < DH_LUT_LENGTH; q += kBlock) s_lut[q] = luts[(size_t)l * DH_LUT_LENGTH + q];
+        __syncthreads();
+        const bool act = valid && !(w > L.cut);
+        if (!dh_wave_any(act)) continue;
+        if (!act) continue;
+
+        // lowerBound + linear interpolation (CudaClasses.cuh:130-144, IMPTClasses.cu:69-92)
+        int lo = 0, hi = DH_LUT_LENGTH;
+        while (lo < hi) {
+            int m = (lo + hi) >> 1;
+            if (w <= s_lut[m].x) hi = m; else lo = m + 1;
+        }
+        float idd, sms;
+        if (lo == DH_LUT_LENGTH) { idd = s_lut[lo - 1].z; sms = s_lut[lo - 1].y; }
+        else if (lo == 0) { idd = s_lut[0].z; sms = s_lut[0].y; }
+        else {
+            const float4 a = s_lut[lo - 1], b = s_lut[lo];
+            const float f = (w - a.x) / (b.x - a.x);
+            idd = fmaf(b.z - a.z, f, a.z);
+            sms = fmaf(b.y - a.y, f, a.y);
+        }
+        // sigmaAir (IMPTClasses.cu:94-102)
+        const float da = dist - w + 0.7f * L.r80;
+        const float st = fmaf(fmaf(L.A, da, L.B), da, L.C) + sms;
+        // nuclearHalo (IMPTClasses.cu:104-121)
+        const float r80 = L.r80;
+        const float wc = fminf(fmaxf(w, 0.1f), r80 - 0.1f);
+        const float t5 = wc - r80 - 5.0f;
+        float sh = 2.85f + (0.0014f * r80 * logf(wc + 3.0f)) + (0.06f * wc) - (7.4e-5f * (wc * wc))
+                 - ((0.22f * r80) / (t5 * t5));
+        const float r3 = r80 + 3.0f;
+        float hw = 0.052f * logf(1.13f + (wc / (11.2f - (0.023f * r80))))
+                 + (0.35f * ((0.0017f * (r80 * r80)) - r80) / ((r3 * r3) - (wc * wc)))
+                 - (1.61e-9f * wc * (r3 * r3));
+        sh = fmaxf(sh, 0.0f);
+        hw = fminf(fmaxf(hw, 0.0f), 0.9f);
+        const float sht = hypotf(st, sh);
+        const float P = (1.0f - hw) * idd / (2.0f * DH_PI_F * (st * st));
+        const float H = hw * idd / (2.0f * DH_PI_F * (sht * sht));
+        const float ap = st != 0.0f ? (-0.5f / (st * st)) * DH_LOG2E_F : -INFINITY;
+        const float ah = sht != 0.0f ? (-0.5f / (sht * sht)) * DH_LOG2E_F : -INFINITY;
+        const float lp = Fold ? log2f(P) : P;
+        const float lh = Fold ? log2f(H) : H;
+
+        const float *sp = spots + (size_t)8 * L.start;
+        for (int s0 = 0; s0 < L.n; s0 += 16) {
+            const int e = min(16, L.n - s0);
+            float part = 0.0f;
+            for (int u = 0; u < e; u++) {
+                const float *rec = sp + 8 * (s0 + u);
+                const float sx = rec[0], sy = rec[1], tx = rec[2], ty = rec[3];
+                const float itt = rec[4], mu = rec[5];
+                // caxDistance (IMPTClasses.cu:123-133): |r|^2 - (tau.r)^2/|tau|^2
+                const float rx = hx - sx, ry = hy - sy;
+                const float r2 = fmaf(rx, rx, fmaf(ry, ry, hz2));
+                const float dt = fmaf(tx, rx, fmaf(ty, ry, -hz));
+                const float qd = dt * itt;
+                const float d2 = fmaf(-qd, dt, r2);
+                float val;
+                if (Fold) {
+                    val = __builtin_amdgcn_exp2f(fmaf(ap, d2, lp)) + __builtin_amdgcn_exp2f(fmaf(ah, d2, lh));
+                } else {
+                    val = fmaf(lp, __builtin_amdgcn_exp2f(ap * d2), lh * __builtin_amdgcn_exp2f(ah * d2));
+                }
+                part = fmaf(mu, val, part);
+            }
+            dh_neumaier(tot, comp, part);
+        }
+    }
+    if (valid) dose[v] = tot + comp;
+}
+
+inline unsigned grid_for(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+void dh_launch_pad(const float *src, float *dst, int nx, int ny, int nz, float border, hipStream_t s)
+{
+    dh_pad p = dh_make_pad(nx, ny, nz);
+    size_t n = (size_t)p.pxy * p.pz;
+    unsigned grid = (unsigned)std::min<size_t>(grid_for(n), 4096);
+    dh_timed t("pad", s);
+    hipLaunchKernelGGL(k_pad, dim3(grid), dim3(kBlock), 0, s, src, dst, nx, ny, nz, p, border);
+    DH_HIP(hipGetLastError());
+}
+
+void dh_launch_raytrace(const dh_geom &g, const float *vol_pad, float *wet, hipStream_t s)
+{
+    size_t n = (size_t)g.nx * g.ny * g.nz;
+    dh_pad p = dh_make_pad(g.nx, g.ny, g.nz);
+    dh_timed t("raytrace", s);
+    hipLaunchKernelGGL(k_raytrace, dim3(grid_for(n)), dim3(kBlock), 0, s, g, p, vol_pad, wet);
+    DH_HIP(hipGetLastError());
+}
+
+void dh_launch_smooth(const dh_geom &g, const float *wet_raw, float *wet_out, hipStream_t s)
+{
+    smooth_dirs d;
+    for (int a = 0; a < 6; a++) {           // IMPTClasses.cu:170
+        float ang = (float)a * DH_PI_F / 3.0f;
+        d.s[a] = sinf(ang);
+        d.c[a] = cosf(ang);
+    }
+    size_t n = (size_t)g.nx * g.ny * g.nz;
+    dh_timed t("smooth", s);
+    hipLaunchKernelGGL(k_smooth, dim3(grid_for(n)), dim3(kBlock), 0, s, g, d, wet_raw, wet_out);
+    DH_HIP(hipGetLastError());
+}
+
+void dh_launch_pencil(const dh_geom &g, float, float, int n_layers, const dh_layer_k *d_layers,
+                      const float4 *d_luts, const float *d_spots, bool fold, const float *wet,
+                      float *dose, hipStream_t s)
+{
+    size_t n = (size_t)g.nx * g.ny * g.nz;
+    dh_timed t("pencil", s);
+    if (fold)
+        hipLaunchKernelGGL(k_pencil<true>, dim3(grid_for(n)), dim3(kBlock), 0, s, g, n_layers,
+                           d_layers, d_luts, d_spots, wet, dose);
+    else
+        hipLaunchKernelGGL(k_pencil<false>, dim3(grid_for(n)), dim3(kBlock), 0, s, g, n_layers,
+                           d_layers, d_luts, d_spots, wet, dose);
+    DH_HIP(hipGetLastError());
+}

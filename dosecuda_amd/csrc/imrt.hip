@@ -1,0 +1,291 @@
+// imrt.hip -- photon path on gfx950: TERMA / electron contamination and the
+// 72-cone collapsed-cone superposition.
+//
+// The CCCS samples TERMA and density at the same point, so both are packed
+// into one padded float2 volume {TERMA (border 0), density (border 0.0012)}:
+// one 8-byte gather per trilinear corner serves both fields.  Voxels with
+// TERMA <= 0.01 are written as zero by a compaction pass that also builds the
+// list of voxels the cone kernel visits (ballot + one atomic per wave), so the
+// cone kernel's waves are full of working lanes.
+#include "dh_runtime.h"
+
+namespace {
+
+constexpr int kBlock = 256;
+
+inline unsigned grid_for(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+// Source-occlusion head transmission (IMRTClasses.cu:120-150) over the open
+// leaf pairs; the scaled leaf edges {xL, xR, yB, yT} are precomputed per pair.
+__device__ static inline float head_transmission(const dh_terma_args &a, float hx, float hy,
+                                                 float hz, float Z, float inv)
+{
+    const float dv = (Z - hz) / (a.d_mlc - hz);
+    float tr = 0.0f;
+    for (int q = 0; q < a.n_pairs; q++) {
+        const float4 m = a.mlc[q];
+        const float t1 = ((m.x - hx) * dv) + hx;
+        const float t2 = ((m.y - hx) * dv) + hx;
+        const float e1 = ((m.z - hy) * dv) + hy;
+        const float e2 = ((m.w - hy) * dv) + hy;
+        const float ex = 0.5f * (erff(t2 * inv) - erff(t1 * inv));
+        const float ey = 0.5f * (erff(e2 * inv) - erff(e1 * inv));
+        tr += ex * ey;
+    }
+    return tr;
+}
+
+// TERMA and electron contamination (IMRTClasses.cu:209-250).
+__global__ void __launch_bounds__(kBlock)
+k_terma(dh_terma_args a, const float *__restrict__ wet, float *__restrict__ terma,
+        float *__restrict__ electron)
+{
+    const dh_geom &g = a.g;
+    const uint32_t nvox = (uint32_t)g.nx * g.ny * g.nz;
+    const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+    if (v >= nvox) return;
+    const int i = (int)(v % (uint32_t)g.nx);
+    const uint32_t r = v / (uint32_t)g.nx;
+    const int j = (int)(r % (uint32_t)g.ny);
+    const int k = (int)(r / (uint32_t)g.ny);
+    float x, y, z, hx, hy, hz;
+    dh_vox_xyz(g, i, j, k, x, y, z);
+    dh_img_to_head(g, x, y, z, hx, hy, hz);
+    const float dps = a.sad - hz, dss = a.ssd_s - hz;
+    // offAxisFactors (IMRTClasses.cu:152-171)
+    const float rc = hypotf(hx, hy) * dps / a.sad;
+    int lo = 0, hi = a.n_prof;
+    while (lo < hi) {
+        int m = (lo + hi) >> 1;
+        if (rc <= a.prof_r[m]) hi = m; else lo = m + 1;
+    }
+    float oaf, soft;
+    if (lo == a.n_prof) { oaf = a.prof_i[lo - 1]; soft = a.prof_s[lo - 1]; }
+    else if (lo == 0) { oaf = a.prof_i[0]; soft = a.prof_s[0]; }
+    else {
+        const float f = (rc - a.prof_r[lo - 1]) / (a.prof_r[lo] - a.prof_r[lo - 1]);
+        oaf = fmaf(a.prof_i[lo] - a.prof_i[lo - 1], f, a.prof_i[lo - 1]);
+        soft = fmaf(a.prof_s[lo] - a.prof_s[lo - 1], f, a.prof_s[lo - 1]);
+    }
+    const float Tp = head_transmission(a, hx, hy, hz, a.sad, a.inv_p);
+    const float Ts = head_transmission(a, hx, hy, hz, a.ssd_s, a.inv_s_src);
+    const float W = wet[v];
+    const float tr = fminf(1.0f, Ts / Tp);
+    const float ip = a.sad / dps, is = a.ssd_s / dss;
+    const float fp = (1.0f - tr) * (ip * ip), fs = tr * is;
+    float t = 0.0f;
+    for (int e = 0; e < a.n_spec; e++) {
+        const float att = expf(-a.mu_e[e] * W * soft);
+        t += a.pw[e] * att * fp + a.sw[e] * att * fs;
+    }
+    const float el = fmaxf(0.0f, (expf(-a.el_att * W) - a.el_norm_a) / a.el_norm_b);
+    terma[v] = oaf * ((1.0f - a.ssw) * Tp * t + a.ssw * Ts * t);
+    electron[v] = a.esw * (0.4f + (0.3f * tr)) * el * Tp;
+}
+
+// Packs {TERMA, density} into the padded float2 layout with their borders.
+__global__ void __launch_bounds__(kBlock)
+k_pack_tr(const float *__restrict__ terma, const float *__restrict__ dens, int nx, int ny, int nz,
+          dh_pad p, float2 *__restrict__ out)
+{
+    const uint32_t n = (uint32_t)(p.pxy * p.pz);
+    for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < n; t += gridDim.x * kBlock) {
+        int i = (int)(t % (uint32_t)p.px) - 1;
+        uint32_t r = t / (uint32_t)p.px;
+        int j = (int)(r % (uint32_t)p.py) - 1;
+        int k = (int)(r / (uint32_t)p.py) - 1;
+        float2 v = make_float2(0.0f, DH_AIR_DENSITY);
+        if (i >= 0 && i < nx && j >= 0 && j < ny && k >= 0 && k < nz) {
+            uint32_t s = (uint32_t)i + (uint32_t)nx * ((uint32_t)j + (uint32_t)ny * (uint32_t)k);
+            v = make_float2(terma[s], dens[s]);
+        }
+        out[t] = v;
+    }
+}
+
+__global__ void k_zero_int(int *p, int n)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = 0;
+}
+
+// TERMA of the voxel centre sampled like the reference's texture fetch
+// (IMRTClasses.cu:267-273).
+__device__ static inline float center_terma(const dh_geom &g, const dh_pad &p,
+                                            const float2 *__restrict__ tr, int i, int j, int k)
+{
+    float x, y, z;
+    dh_vox_xyz(g, i, j, k, x, y, z);
+    const float tx = fmaf(g.inv_s, x + g.iso[0], 0.5f) - 0.5f;
+    const float ty = fmaf(g.inv_s, y + g.iso[1], 0.5f) - 0.5f;
+    const float tz = fmaf(g.inv_s, z + g.iso[2], 0.5f) - 0.5f;
+    const float ux = __builtin_amdgcn_fmed3f(tx, -1.0f, (float)g.nx);
+    const float uy = __builtin_amdgcn_fmed3f(ty, -1.0f, (float)g.ny);
+    const float uz = __builtin_amdgcn_fmed3f(tz, -1.0f, (float)g.nz);
+    const float fx = floorf(ux), fy = floorf(uy), fz = floorf(uz);
+    const float ax = ux - fx, ay = uy - fy, az = uz - fz;
+    const float2 *b = tr + dh_pad_index(p, (int)fx, (int)fy, (int)fz);
+    const float2 *c = b + p.pxy;
+    const float a00 = fmaf(ax, b[1].x - b[0].x, b[0].x);
+    const float a10 = fmaf(ax, b[p.px + 1].x - b[p.px].x, b[p.px].x);
+    const float a01 = fmaf(ax, c[1].x - c[0].x, c[0].x);
+    const float a11 = fmaf(ax, c[p.px + 1].x - c[p.px].x, c[p.px].x);
+    const float b0 = fmaf(ay, a10 - a00, a00), b1 = fmaf(ay, a11 - a01, a01);
+    return fmaf(az, b1 - b0, b0);
+}
+
+// Builds the list of voxels with TERMA > 0.01 and zeroes the dose of the rest.
+__global__ void __launch_bounds__(kBlock)
+k_ccc_compact(dh_geom g, dh_pad p, const float2 *__restrict__ tr, int *__restrict__ list,
+              int *__restrict__ counter, float *__restrict__ dose)
+{
+    const uint32_t nvox = (uint32_t)g.nx * g.ny * g.nz;
+    const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+    bool act = false;
+    if (v < nvox) {
+        const int i = (int)(v % (uint32_t)g.nx);
+        const uint32_t r = v / (uint32_t)g.nx;
+        const int j = (int)(r % (uint32_t)g.ny);
+        const int k = (int)(r / (uint32_t)g.ny);
+        const float T0 = center_terma(g, p, tr, i, j, k);
+        act = !(T0 <= 0.01f);
+        if (!act) dose[v] = 0.0f;
+    }
+    const uint64_t m = __builtin_amdgcn_ballot_w64(act);
+    if (m == 0ull) return;
+    const int lane = __lane_id();
+    const int cnt = __popcll(m);
+    int base = 0;
+    if (lane == __ffsll((unsigned long long)m) - 1) base = atomicAdd(counter, cnt);
+    base = __shfl(base, __ffsll((unsigned long long)m) - 1, 64);
+    if (act) {
+        const uint64_t below = m & ((1ull << lane) - 1ull);
+        list[base + __popcll(below)] = (int)v;
+    }
+}
+
+// Collapsed-cone superposition (IMRTClasses.cu:252-350) for the listed voxels:
+// 6 zenith rows x 12 azimuths; each cone direction is tilted towards the
+// source (kernelTilt, IMRTClasses.cu:173-203) and marched from ray_length down
+// to 0 in FP32 steps of spacing/10 cm (the reference's own step lattice), with
+// the primary (exponential) and scatter (linear) recursions.  Cone results are
+// summed with Neumaier compensation.
+__global__ void __launch_bounds__(kBlock)
+k_ccc(dh_ccc_args a, const float2 *__restrict__ tr, const float *__restrict__ electron,
+      const int *__restrict__ list, const int *__restrict__ counter, float *__restrict__ dose)
+{
+    const dh_geom &g = a.g;
+    const int n = *counter;
+    const float fnx = (float)g.nx, fny = (float)g.ny, fnz = (float)g.nz;
+    for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < n; idx += gridDim.x * kBlock) {
+        const uint32_t v = (uint32_t)list[idx];
+        const int i = (int)(v % (uint32_t)g.nx);
+        const uint32_t r = v / (uint32_t)g.nx;
+        const int j = (int)(r % (uint32_t)g.ny);
+        const int k = (int)(r / (uint32_t)g.ny);
+        float x, y, z, ux, uy, uz;
+        dh_vox_xyz(g, i, j, k, x, y, z);
+        dh_uvec(g, x, y, z, ux, uy, uz);
+        // texel-space origin of the voxel (u = t - 0.5)
+        const float ucx = fmaf(g.inv_s, x + g.iso[0], 0.5f) - 0.5f;
+        const float ucy = fmaf(g.inv_s, y + g.iso[1], 0.5f) - 0.5f;
+        const float ucz = fmaf(g.inv_s, z + g.iso[2], 0.5f) - 0.5f;
+        // Rodrigues tilt taking the iso->source axis onto u
+        const float c = a.ax * ux + a.ay * uy + a.az * uz;
+        const float kx = a.ay * uz - a.az * uy;
+        const float ky = a.az * ux - a.ax * uz;
+        const float kz = a.ax * uy - a.ay * ux;
+        const float inv1c = 1.0f / (1.0f + c);
+        float ds = 0.0f, dcmp = 0.0f;
+        for (int row = 0; row < 6; row++) {
+            const float c1 = a.c1[row], c2p = a.c2p[row], bp = a.bp[row], a2 = a.a2[row];
+            const float am = a.a[row], bm = a.b[row], L0 = a.L0[row];
+            const int ns = a.nsteps[row];
+            for (int az = 0; az < 12; az++) {
+                const float dx0 = a.dir[3 * (row * 12 + az) + 0];
+                const float dy0 = a.dir[3 * (row * 12 + az) + 1];
+                const float dz0 = a.dir[3 * (row * 12 + az) + 2];
+                const float sc = (kx * dx0 + ky * dy0 + kz * dz0) * inv1c;
+                const float dx = (ky * dz0 - kz * dy0) + dx0 * c + kx * sc;
+                const float dy = (kz * dx0 - kx * dz0) + dy0 * c + ky * sc;
+                const float dz = (kx * dy0 - ky * dx0) + dz0 * c + kz * sc;
+                const float tdx = dx * g.inv_s, tdy = dy * g.inv_s, tdz = dz * g.inv_s;
+                float Rp = 0.0f, Rs = 0.0f;
+                float L = L0;
+                for (int st = 0; st < ns; st++) {
+                    const float Lm = L * 10.0f;
+                    float qx = __builtin_amdgcn_fmed3f(fmaf(tdx, Lm, ucx), -1.0f, fnx);
+                    float qy = __builtin_amdgcn_fmed3f(fmaf(tdy, Lm, ucy), -1.0f, fny);
+                    float qz = __builtin_amdgcn_fmed3f(fmaf(tdz, Lm, ucz), -1.0f, fnz);
+                    const float fx = floorf(qx), fy = floorf(qy), fz = floorf(qz);
+                    const float wx = qx - fx, wy = qy - fy, wz = qz - fz;
+                    const float2 *b = tr + dh_pad_index(a.p, (int)fx, (int)fy, (int)fz);
+                    const float2 *cc = b + a.p.pxy;
+                    const float2 v000 = b[0], v100 = b[1], v010 = b[a.p.px], v110 = b[a.p.px + 1];
+                    const float2 v001 = cc[0], v101 = cc[1], v011 = cc[a.p.px], v111 = cc[a.p.px + 1];
+                    const float t00 = fmaf(wx, v100.x - v000.x, v000.x);
+                    const float t10 = fmaf(wx, v110.x - v010.x, v010.x);
+                    const float t01 = fmaf(wx, v101.x - v001.x, v001.x);
+                    const float t11 = fmaf(wx, v111.x - v011.x, v011.x);
+                    const float r00 = fmaf(wx, v100.y - v000.y, v000.y);
+                    const float r10 = fmaf(wx, v110.y - v010.y, v010.y);
+                    const float r01 = fmaf(wx, v101.y - v001.y, v001.y);
+                    const float r11 = fmaf(wx, v111.y - v011.y, v011.y);
+                    const float t0 = fmaf(wy, t10 - t00, t00), t1 = fmaf(wy, t11 - t01, t01);
+                    const float r0 = fmaf(wy, r10 - r00, r00), r1 = fmaf(wy, r11 - r01, r01);
+                    const float T = fmaf(wz, t1 - t0, t0);
+                    const float rho = fmaxf(fmaf(wz, r1 - r0, r0), DH_AIR_DENSITY);
+                    const float ex = __builtin_amdgcn_exp2f(a2 * rho);       // exp(-a * rho * sp)
+                    const float K = T * c1;
+                    Rp = fmaf(Rp - K, ex, K);                                  // Rp e + K (1 - e)
+                    Rs = fmaf(rho, fmaf(T, c2p, -bp * Rs), Rs);               // Rs (1 - bD) + T D c2
+                    L = L - a.sp;
+                }
+                dh_neumaier(ds, dcmp, fmaf(am, Rp, bm * Rs));
+            }
+        }
+        float dv = (ds + dcmp) + electron[v];
+        dose[v] = (!isnan(dv) && dv >= 0.0f) ? a.mu_cal * dv * a.mu : 0.0f;
+    }
+}
+
+}  // namespace
+
+void dh_launch_terma(const dh_terma_args &a, const float *wet, float *terma, float *electron,
+                     hipStream_t s)
+{
+    size_t n = (size_t)a.g.nx * a.g.ny * a.g.nz;
+    dh_timed t("terma", s);
+    hipLaunchKernelGGL(k_terma, dim3(grid_for(n)), dim3(kBlock), 0, s, a, wet, terma, electron);
+    DH_HIP(hipGetLastError());
+}
+
+void dh_launch_ccc(const dh_ccc_args &a, const float *terma, const float *density,
+                   const float *electron, float2 *tr_pad, int *active, int *counter,
+                   float *dose, hipStream_t s)
+{
+    const dh_geom &g = a.g;
+    size_t n = (size_t)g.nx * g.ny * g.nz;
+    size_t np = (size_t)a.p.pxy * a.p.pz;
+    {
+        dh_timed t("pack_tr", s);
+        hipLaunchKernelGGL(k_pack_tr, dim3((unsigned)std::min<size_t>(grid_for(np), 4096)),
+                           dim3(kBlock), 0, s, terma, density, g.nx, g.ny, g.nz, a.p, tr_pad);
+        DH_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, counter, 1);
+    DH_HIP(hipGetLastError());
+    {
+        dh_timed t("ccc_compact", s);
+        hipLaunchKernelGGL(k_ccc_compact, dim3(grid_for(n)), dim3(kBlock), 0, s, g, a.p,
+                           (const float2 *)tr_pad, active, counter, dose);
+        DH_HIP(hipGetLastError());
+    }
+    {
+        dh_timed t("ccc", s);
+        unsigned grid = (unsigned)std::min<size_t>(grid_for(n), 2048);
+        hipLaunchKernelGGL(k_ccc, dim3(grid), dim3(kBlock), 0, s, a, (const float2 *)tr_pad,
+                           electron, (const int *)active, (const int *)counter, dose);
+        DH_HIP(hipGetLastError());
+    }
+}

@@ -1,0 +1,193 @@
+// runtime.cpp -- per-device state (stream, scratch arena), launch timing and
+// the host-side beam set-up.
+#include "dh_runtime.h"
+
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace {
+
+struct dev_state {
+    bool ready = false;
+    hipStream_t stream = nullptr;
+    void *slot_ptr[DH_SLOT_COUNT_] = {};
+    size_t slot_bytes[DH_SLOT_COUNT_] = {};
+};
+
+constexpr int kMaxDevices = 64;
+dev_state g_dev[kMaxDevices];
+std::recursive_mutex g_mu;
+
+struct prof_rec {
+    std::string name;
+    hipEvent_t a, b;
+};
+bool g_prof_on = false;
+std::vector<prof_rec> g_prof;
+std::vector<hipEvent_t> g_event_pool;
+
+hipEvent_t take_event()
+{
+    if (!g_event_pool.empty()) {
+        hipEvent_t e = g_event_pool.back();
+        g_event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    DH_HIP(hipEventCreate(&e));
+    return e;
+}
+
+}  // namespace
+
+int dh_current_device()
+{
+    int d = 0;
+    DH_HIP(hipGetDevice(&d));
+    return d;
+}
+
+void dh_device_enter(int dev)
+{
+    int n = 0;
+    DH_HIP(hipGetDeviceCount(&n));
+    if (dev < 0 || dev >= n || dev >= kMaxDevices) {
+        throw dh_error(DH_E_RUNTIME, hipGetErrorString(hipErrorInvalidDevice));
+    }
+    DH_HIP(hipSetDevice(dev));
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    dev_state &st = g_dev[dev];
+    if (!st.ready) {
+        DH_HIP(hipStreamCreateWithFlags(&st.stream, hipStreamNonBlocking));
+        st.ready = true;
+    }
+}
+
+hipStream_t dh_stream(int dev)
+{
+    dh_device_enter(dev);
+    return g_dev[dev].stream;
+}
+
+void *dh_scratch(int dev, int slot, size_t bytes)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    dev_state &st = g_dev[dev];
+    if (bytes == 0) bytes = 256;
+    if (st.slot_bytes[slot] < bytes) {
+        if (st.slot_ptr[slot]) {
+            DH_HIP(hipDeviceSynchronize());
+            DH_HIP(hipFree(st.slot_ptr[slot]));
+            st.slot_ptr[slot] = nullptr;
+            st.slot_bytes[slot] = 0;
+        }
+        size_t b = (bytes + 4095) & ~(size_t)4095;
+        void *p = nullptr;
+        DH_HIP(hipMalloc(&p, b));
+        st.slot_ptr[slot] = p;
+        st.slot_bytes[slot] = b;
+    }
+    return st.slot_ptr[slot];
+}
+
+void dh_device_release(int dev)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    dev_state &st = g_dev[dev];
+    if (!st.ready) return;
+    DH_HIP(hipSetDevice(dev));
+    DH_HIP(hipDeviceSynchronize());
+    for (int s = 0; s < DH_SLOT_COUNT_; s++) {
+        if (st.slot_ptr[s]) DH_HIP(hipFree(st.slot_ptr[s]));
+        st.slot_ptr[s] = nullptr;
+        st.slot_bytes[s] = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launch timing
+// ---------------------------------------------------------------------------
+dh_timed::dh_timed(const char *n, hipStream_t st) : name(n), s(st), ev0(nullptr)
+{
+    if (!g_prof_on) return;
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    hipEvent_t e = take_event();
+    DH_HIP(hipEventRecord(e, s));
+    ev0 = (void *)e;
+}
+
+dh_timed::~dh_timed()
+{
+    if (!ev0) return;
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    hipEvent_t e = take_event();
+    if (hipEventRecord(e, s) != hipSuccess) return;
+    g_prof.push_back(prof_rec{name, (hipEvent_t)ev0, e});
+}
+
+extern "C" int dh_prof_enable(int on)
+{
+    g_prof_on = on != 0;
+    return DH_OK;
+}
+
+extern "C" int dh_prof_reset(void)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    if (!g_prof.empty()) (void)hipDeviceSynchronize();
+    for (auto &r : g_prof) {
+        g_event_pool.push_back(r.a);
+        g_event_pool.push_back(r.b);
+    }
+    g_prof.clear();
+    return DH_OK;
+}
+
+extern int dh_set_error(int code, const char *msg);
+
+extern "C" int dh_prof_get(const char *kernel, double *total_ms, int64_t *launches)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    double tot = 0.0;
+    int64_t n = 0;
+    for (auto &r : g_prof) {
+        if (kernel && std::strcmp(kernel, r.name.c_str()) != 0) continue;
+        if (hipEventSynchronize(r.b) != hipSuccess) return dh_set_error(DH_E_RUNTIME, "event sync failed");
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, r.a, r.b) != hipSuccess) return dh_set_error(DH_E_RUNTIME, "event timing failed");
+        tot += ms;
+        n++;
+    }
+    if (total_ms) *total_ms = tot;
+    if (launches) *launches = n;
+    return DH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// beam set-up, FP32 as the reference host code computes it
+// ---------------------------------------------------------------------------
+dh_geom dh_make_geom(const dh_grid &grid, const float iso[3], double gantry, double couch,
+                     double collimator, float src_dist)
+{
+    dh_geom g;
+    g.nx = grid.nx; g.ny = grid.ny; g.nz = grid.nz;
+    g.s = grid.spacing;                 // CudaDose(dims, spacing), dosemodule.cu:146
+    g.inv_s = 1.0f / grid.spacing;      // CudaClasses.cuh:101
+    for (int a = 0; a < 3; a++) g.iso[a] = iso[a] - grid.origin[a];   // dosemodule.cu:160-164
+    float adj = fmodf((float)(gantry + 180.0), 360.0f);               // dosemodule.cu:152
+    float ga = adj * DH_PI_F / 180.0f;                                  // CudaClasses.cu:27
+    float ta = (float)couch * DH_PI_F / 180.0f;
+    float ca = (float)collimator * DH_PI_F / 180.0f;                   // IMRTClasses.cu:35
+    g.singa = sinf(ga); g.cosga = cosf(ga);
+    g.sinta = sinf(ta); g.costa = cosf(ta);
+    g.sinca = sinf(ca); g.cosca = cosf(ca);
+    float xg = -src_dist * g.singa;                                     // CudaClasses.cu:36-49
+    float yg = src_dist * g.cosga;
+    g.src[0] = xg * g.costa;
+    g.src[1] = yg;
+    g.src[2] = -xg * g.sinta;
+    return g;
+}

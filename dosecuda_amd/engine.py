@@ -1,0 +1,181 @@
+"""Device-pointer API of the engine on torch-ROCm tensors.
+
+torch supplies device memory and the stream; all computation is
+libdosehip.so.  Every function enqueues on the given (default: torch's
+current) stream and returns without synchronising.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+
+
+def _stream(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def _dev(t: torch.Tensor) -> int:
+    if not (t.is_cuda and t.dtype == torch.float32 and t.is_contiguous()):
+        raise ValueError("expected a contiguous float32 tensor on the GPU")
+    return t.device.index if t.device.index is not None else 0
+
+
+def _g(shape, spacing, origin):
+    return _lib.grid(shape, spacing, origin)
+
+
+def proton_wet(rlsp: torch.Tensor, spacing, origin, iso, gantry, couch, vsadx, vsady,
+               out: torch.Tensor = None, raw_out: torch.Tensor = None, stream=None):
+    """Ray trace + lateral smoothing; returns the smoothed WET (cm)."""
+    dev = _dev(rlsp)
+    out = torch.empty_like(rlsp) if out is None else out
+    L = _lib.load()
+    _lib.check(L.dh_proton_raytrace_dev(dev, _g(rlsp.shape, spacing, origin),
+                                        _lib.proton_beam(iso, gantry, couch, vsadx, vsady),
+                                        _lib.ptr(rlsp), _lib.ptr(out), _lib.ptr(raw_out),
+                                        _stream(stream)))
+    return out
+
+
+class ProtonPlan:
+    """One proton field's spots/layers/LUT rows resident on the device."""
+
+    def __init__(self, device: int, shape, spacing, origin, iso, gantry, couch, model, spot_rows):
+        self.shape = tuple(int(v) for v in shape)
+        self._keep: list = []
+        lut = _lib.proton_lut(model.divergence_params, model.lut_depths, model.lut_sigmas,
+                              model.lut_idds, self._keep)
+        rows = np.ascontiguousarray(np.asarray(spot_rows, dtype=np.float32).reshape(-1, 4))
+        self._h = C.c_void_p()
+        L = _lib.load()
+        _lib.check(L.dh_proton_plan_create(device, _g(shape, spacing, origin),
+                                           _lib.proton_beam(iso, gantry, couch, model.VSADX,
+                                                            model.VSADY),
+                                           lut, _lib.ptr(rows), rows.shape[0], C.byref(self._h)))
+        n = C.c_int32()
+        ne = int(np.asarray(model.lut_depths).shape[0])
+        spots = np.zeros(max(ne, 1), np.int32)
+        cut = np.zeros(max(ne, 1), np.float32)
+        _lib.check(L.dh_proton_plan_layers(self._h, C.byref(n), _lib.ptr(spots), _lib.ptr(cut)))
+        self.n_layers = int(n.value)
+        self.layer_spots = spots[:self.n_layers].copy()
+        self.layer_cut = cut[:self.n_layers].copy()
+
+    def dose(self, wet: torch.Tensor, out: torch.Tensor = None, stream=None):
+        out = torch.empty_like(wet) if out is None else out
+        _lib.check(_lib.load().dh_proton_plan_dose_dev(self._h, _lib.ptr(wet), _lib.ptr(out),
+                                                       _stream(stream)))
+        return out
+
+    def pairs(self, wet32: np.ndarray) -> int:
+        """Voxel-spot pairs that pass the range cut 10*WET <= 1.1*R80
+        (IMPTClasses.cu:222-225) for the given float32 WET."""
+        w = (np.asarray(wet32, dtype=np.float32) * np.float32(10.0)).ravel()
+        tot = 0
+        for n, c in zip(self.layer_spots, self.layer_cut):
+            tot += int(n) * int(np.count_nonzero(~(w > c)))
+        return tot
+
+    def close(self):
+        if self._h:
+            _lib.load().dh_proton_plan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def photon_args(model, keep):
+    vals = {k: getattr(model, k) for k in (
+        "mu_calibration", "primary_source_distance", "scatter_source_distance",
+        "primary_source_size", "scatter_source_size", "mlc_distance", "scatter_source_weight",
+        "electron_attenuation", "electron_source_weight", "electron_fitted_dmax",
+        "jaw_transmission", "mlc_transmission", "has_xjaws", "has_yjaws")}
+    arrs = {k: getattr(model, k) for k in (
+        "profile_radius", "profile_intensities", "profile_softening",
+        "spectrum_attenuation_coefficients", "spectrum_primary_weights",
+        "spectrum_scatter_weights")}
+    return _lib.photon_model(vals, arrs, model.kernel, keep)
+
+
+def photon_dose(density: torch.Tensor, spacing, origin, iso, mu, ga, ca, ta, model, mlc,
+                out=None, terma_out=None, electron_out=None, wet_out=None, stream=None):
+    """Ray trace + TERMA + CCCS for one control point."""
+    dev = _dev(density)
+    out = torch.empty_like(density) if out is None else out
+    keep: list = []
+    m = photon_args(model, keep)
+    rows = np.ascontiguousarray(np.asarray(mlc, dtype=np.float32)[:, :4])
+    _lib.check(_lib.load().dh_photon_dose_dev(
+        dev, _g(density.shape, spacing, origin), _lib.photon_cp(iso, mu, ga, ca, ta), m,
+        _lib.ptr(rows), rows.shape[0], _lib.ptr(density), _lib.ptr(out), _lib.ptr(terma_out),
+        _lib.ptr(electron_out), _lib.ptr(wet_out), _stream(stream)))
+    return out
+
+
+# ---- single stages (stage-wise parity) -----------------------------------
+def stage_raytrace(vol: torch.Tensor, spacing, origin, iso, gantry, couch, src_dist, border,
+                   out=None, stream=None):
+    out = torch.empty_like(vol) if out is None else out
+    c = np.asarray(iso, dtype=np.float32).reshape(3)
+    _lib.check(_lib.load().dh_stage_raytrace_dev(
+        _dev(vol), _g(vol.shape, spacing, origin), _lib.ptr(c), float(gantry), float(couch),
+        float(src_dist), float(border), _lib.ptr(vol), _lib.ptr(out), _stream(stream)))
+    return out
+
+
+def stage_smooth(wet_raw: torch.Tensor, spacing, origin, iso, gantry, couch, vsadx, vsady,
+                 out=None, stream=None):
+    out = torch.empty_like(wet_raw) if out is None else out
+    _lib.check(_lib.load().dh_stage_smooth_dev(
+        _dev(wet_raw), _g(wet_raw.shape, spacing, origin),
+        _lib.proton_beam(iso, gantry, couch, vsadx, vsady), _lib.ptr(wet_raw), _lib.ptr(out),
+        _stream(stream)))
+    return out
+
+
+def stage_terma(wet: torch.Tensor, spacing, origin, iso, mu, ga, ca, ta, model, mlc,
+                stream=None):
+    terma, el = torch.empty_like(wet), torch.empty_like(wet)
+    keep: list = []
+    m = photon_args(model, keep)
+    rows = np.ascontiguousarray(np.asarray(mlc, dtype=np.float32)[:, :4])
+    _lib.check(_lib.load().dh_stage_terma_dev(
+        _dev(wet), _g(wet.shape, spacing, origin), _lib.photon_cp(iso, mu, ga, ca, ta), m,
+        _lib.ptr(rows), rows.shape[0], _lib.ptr(wet), _lib.ptr(terma), _lib.ptr(el),
+        _stream(stream)))
+    return terma, el
+
+
+def stage_ccc(terma: torch.Tensor, density: torch.Tensor, electron: torch.Tensor, spacing,
+              origin, iso, mu, ga, ca, ta, model, out=None, stream=None):
+    out = torch.empty_like(terma) if out is None else out
+    keep: list = []
+    m = photon_args(model, keep)
+    _lib.check(_lib.load().dh_stage_ccc_dev(
+        _dev(terma), _g(terma.shape, spacing, origin), _lib.photon_cp(iso, mu, ga, ca, ta), m,
+        _lib.ptr(terma), _lib.ptr(density), _lib.ptr(electron), _lib.ptr(out), _stream(stream)))
+    return out
+
+
+# ---- launch timing -------------------------------------------------------
+def prof_enable(on=True):
+    _lib.load().dh_prof_enable(1 if on else 0)
+
+
+def prof_reset():
+    _lib.load().dh_prof_reset()
+
+
+def prof_get(kernel: str):
+    ms, n = C.c_double(), C.c_int64()
+    _lib.check(_lib.load().dh_prof_get(kernel.encode(), C.byref(ms), C.byref(n)))
+    return float(ms.value), int(n.value)

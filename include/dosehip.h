@@ -1,0 +1,173 @@
+/*
+ * dosehip.h -- C ABI of the MI355X dose engine (libdosehip.so).
+ *
+ * Plain C types only: pointers, sizes and POD structs.  Every entry point
+ * returns DH_OK (0) or a negative DH_E* code; dh_last_error() then holds a
+ * thread-local message.  Host-buffer entry points are synchronous (they
+ * return when the result is in the caller's buffer).  *_dev entry points take
+ * device pointers and a hipStream_t passed as void*; they only enqueue work.
+ *
+ * Reference interfaces replaced (DoseCUDA/dose_kernels):
+ *   dh_proton_raytrace  <- proton_raytrace (dosemodule.cu:104-201)
+ *                          + proton_raytrace_cuda (IMPTClasses.cu:274-299)
+ *   dh_proton_spot      <- proton_spot (dosemodule.cu:243-374)
+ *                          + proton_spot_cuda (IMPTClasses.cu:301-340)
+ *   dh_photon_dose      <- photon_dose (dosemodule.cu:377-557)
+ *                          + photon_dose_cuda (IMRTClasses.cu:353-405)
+ * The Python module dose_kernels (dosemodule.cu:560-595) becomes the ctypes
+ * module dosecuda_amd/dose_kernels.py, which binds exactly these symbols.
+ */
+#ifndef DOSEHIP_H
+#define DOSEHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DH_OK 0
+#define DH_E_INVALID (-1)   /* bad argument (shape, size, null pointer)   */
+#define DH_E_RUNTIME (-2)   /* HIP runtime error (message from HIP)        */
+#define DH_E_NOMEM   (-3)   /* host allocation failed                      */
+
+#define DH_LUT_LENGTH 400   /* IMPTClasses.cuh:6                           */
+#define DH_AIR_DENSITY 0.0012f /* IMRTClasses.cuh:6                        */
+
+/* Grid as the caller holds it: NumPy C-order (nz, ny, nx) float32 volume,
+ * isotropic spacing spacing[0] mm (dosemodule.cu:146), origin in mm. */
+typedef struct dh_grid {
+    int32_t nz, ny, nx;
+    float spacing;
+    float origin[3];
+} dh_grid;
+
+/* Proton field (dosemodule.cu:114-141): raw caller angles in degrees;
+ * the +180 deg gantry shift and iso - origin are applied inside. */
+typedef struct dh_proton_beam {
+    float iso[3];
+    double gantry_angle;
+    double couch_angle;
+    double vsadx, vsady;     /* model.VSADX / VSADY, mm */
+} dh_proton_beam;
+
+/* Proton beam-model tables (dosemodule.cu:260-266,340-349): row-major
+ * [n_energies][row_stride]; the first DH_LUT_LENGTH columns are used.
+ * divergence_params is [n_energies][5] = energy, R80, A, B, C. */
+typedef struct dh_proton_lut {
+    int32_t n_energies;
+    int32_t row_stride;
+    const float *divergence_params;
+    const float *depths;
+    const float *sigmas;
+    const float *idds;
+} dh_proton_lut;
+
+/* Photon control point (dosemodule.cu:449-466): raw angles in degrees. */
+typedef struct dh_photon_cp {
+    float iso[3];
+    double mu, ga, ca, ta;
+} dh_photon_cp;
+
+/* Photon beam model (dosemodule.cu:386-437, IMRTClasses.cuh:23-57).
+ * kernel is [6 zenith rows][theta_deg, Am, am, Bm, bm, ray_length_cm]. */
+typedef struct dh_photon_model {
+    int32_t n_profile;
+    const float *profile_radius, *profile_intensities, *profile_softening;
+    int32_t n_spectral;
+    const float *spectrum_attenuation_coefficients;
+    const float *spectrum_primary_weights;
+    const float *spectrum_scatter_weights;
+    double mu_calibration, primary_source_distance, scatter_source_distance;
+    double primary_source_size, scatter_source_size, mlc_distance;
+    double scatter_source_weight, electron_attenuation, electron_source_weight;
+    double electron_fitted_dmax, jaw_transmission, mlc_transmission;
+    int32_t has_xjaws, has_yjaws;
+    float kernel[36];
+} dh_photon_model;
+
+/* ---- library ---------------------------------------------------------- */
+const char *dh_last_error(void);
+int dh_version(void);
+int dh_device_count(int *count);
+/* Selects and initialises a device (stream, scratch arena).  Called
+ * implicitly by every entry point with its gpu_id. */
+int dh_init(int gpu_id);
+/* Releases every device buffer the library holds on gpu_id. */
+int dh_release(int gpu_id);
+
+/* ---- drop-in, host buffers -------------------------------------------- */
+/* Smoothed WET (cm), shape of rlsp (IMPTClasses.cu:274-299). */
+int dh_proton_raytrace(int gpu_id, const dh_grid *grid, const dh_proton_beam *beam,
+                       const float *rlsp, float *wet_out);
+/* Pencil-beam dose on the WET grid; spot_rows is [n_spots][4] =
+ * x mm, y mm, MU, energy id (dosemodule.cu:211-225). */
+int dh_proton_spot(int gpu_id, const dh_grid *grid, const dh_proton_beam *beam,
+                   const dh_proton_lut *lut, const float *spot_rows, int32_t n_spots,
+                   const float *wet, float *dose_out);
+/* CCCS dose before the output factor; mlc_rows is [n_pairs][4] =
+ * x1, x2, y_offset, y_width mm at iso.  terma_out / electron_out are
+ * optional (NULL) extra outputs the reference keeps on the device. */
+int dh_photon_dose(int gpu_id, const dh_grid *grid, const dh_photon_cp *cp,
+                   const dh_photon_model *model, const float *mlc_rows, int32_t n_pairs,
+                   const float *density, float *dose_out, float *terma_out,
+                   float *electron_out);
+
+/* ---- device buffers, stream-ordered ---------------------------------- */
+/* Same computations on device pointers.  wet_raw_out (optional) receives
+ * the unsmoothed WET. */
+int dh_proton_raytrace_dev(int gpu_id, const dh_grid *grid, const dh_proton_beam *beam,
+                           const float *d_rlsp, float *d_wet_out, float *d_wet_raw_out,
+                           void *stream);
+/* A proton plan holds one field's sorted spots, layers and per-layer LUT rows
+ * on the device (make_spot_array + importLayers, dosemodule.cu:211-225 and
+ * IMPTClasses.cu:29-67), so repeated dose evaluations upload nothing. */
+typedef struct dh_proton_plan dh_proton_plan;
+int dh_proton_plan_create(int gpu_id, const dh_grid *grid, const dh_proton_beam *beam,
+                          const dh_proton_lut *lut, const float *spot_rows, int32_t n_spots,
+                          dh_proton_plan **out);
+/* Layer count, spots per layer and range cut 1.1*R80 (mm) per layer;
+ * arrays may be NULL or must hold n_energies entries. */
+int dh_proton_plan_layers(const dh_proton_plan *plan, int32_t *n_layers,
+                          int32_t *layer_spots, float *layer_cut);
+int dh_proton_plan_dose_dev(dh_proton_plan *plan, const float *d_wet, float *d_dose_out,
+                            void *stream);
+int dh_proton_plan_destroy(dh_proton_plan *plan);
+int dh_photon_dose_dev(int gpu_id, const dh_grid *grid, const dh_photon_cp *cp,
+                       const dh_photon_model *model, const float *mlc_rows, int32_t n_pairs,
+                       const float *d_density, float *d_dose_out, float *d_terma_out,
+                       float *d_electron_out, float *d_wet_out, void *stream);
+
+/* ---- single stages on device pointers (stage-wise parity) ------------ */
+/* Unsmoothed WET of a volume sampled with the given border value, source
+ * at src_dist from iso (CudaClasses.cu:180-228). */
+int dh_stage_raytrace_dev(int gpu_id, const dh_grid *grid, const float iso[3],
+                          double gantry, double couch, float src_dist, float border,
+                          const float *d_vol, float *d_wet_out, void *stream);
+/* Lateral WET smoothing of a given WET (IMPTClasses.cu:139-202). */
+int dh_stage_smooth_dev(int gpu_id, const dh_grid *grid, const dh_proton_beam *beam,
+                        const float *d_wet_raw, float *d_wet_out, void *stream);
+/* TERMA and electron volumes from a given WET (IMRTClasses.cu:209-250). */
+int dh_stage_terma_dev(int gpu_id, const dh_grid *grid, const dh_photon_cp *cp,
+                       const dh_photon_model *model, const float *mlc_rows, int32_t n_pairs,
+                       const float *d_wet, float *d_terma_out, float *d_electron_out,
+                       void *stream);
+/* Cone superposition from given TERMA, density and electron volumes
+ * (IMRTClasses.cu:252-350). */
+int dh_stage_ccc_dev(int gpu_id, const dh_grid *grid, const dh_photon_cp *cp,
+                     const dh_photon_model *model, const float *d_terma,
+                     const float *d_density, const float *d_electron, float *d_dose_out,
+                     void *stream);
+
+/* ---- kernel timing (HIP events on the launch stream) ------------------ */
+/* When enabled, every kernel launch of the library is bracketed by HIP
+ * events; dh_prof_get sums the elapsed times of the launches of one kernel
+ * name since the last reset (synchronising the device first). */
+int dh_prof_enable(int on);
+int dh_prof_reset(void);
+int dh_prof_get(const char *kernel, double *total_ms, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DOSEHIP_H */

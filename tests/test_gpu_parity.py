@@ -1,0 +1,212 @@
+"""GPU parity: every stage of the HIP engine against the FP64 oracle on the
+same inputs, then the drop-in entry points end to end.
+
+Tolerance (north star): max |GPU - oracle| <= 1e-4 * Dmax of the oracle for
+doses; the WET stages are held to 1e-5 of their maximum.  Each test prints
+its max and RMS error relative to the oracle's maximum.
+Stage tests feed both sides the same float32 input, so a float32 decision
+(range cut, TERMA <= 0.01, smoothing radius) cannot differ between them.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+F = np.float32
+TOL_DOSE = 1e-4
+TOL_WET = 1e-5
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def _cuda(t, a):
+    return t.from_numpy(np.ascontiguousarray(a, dtype=F)).cuda()
+
+
+def _report(name, got, ref, tol):
+    mx, rms = rel_err(got, ref)
+    print(f"[parity] {name}: max {mx:.3e} rms {rms:.3e} (x Dmax={np.abs(ref).max():.4g}) tol {tol:g}",
+          flush=True)
+    assert np.all(np.isfinite(got)), name
+    assert mx <= tol, name
+    return mx, rms
+
+
+def _hetero(shape, seed, lo=0.2, hi=1.8):
+    rng = np.random.default_rng(seed)
+    base = rng.uniform(lo, hi, (shape[0] // 4 + 2, shape[1] // 4 + 2, shape[2] // 4 + 2))
+    # blocky heterogeneity: nearest-neighbour upsampling of a coarse field
+    idx = [np.minimum(np.arange(n) // 4, base.shape[a] - 1) for a, n in enumerate(shape)]
+    return base[np.ix_(*idx)].astype(F)
+
+
+RT_CASES = [
+    ((20, 24, 28), 2.5, 0.0, 0.0, 0.0),
+    ((24, 20, 16), 2.0, 57.0, 10.0, 0.0012),
+    ((18, 22, 26), 3.0, 250.0, 340.0, 0.0),
+]
+
+
+@pytest.mark.parametrize("shape, s, ga, ta, border", RT_CASES)
+def test_raytrace_stage(shape, s, ga, ta, border):
+    t = _torch()
+    from dosecuda_amd import engine
+    rho = _hetero(shape, 11)
+    iso = np.array([(shape[2] - 1) * s / 2 + 1.3, (shape[1] - 1) * s / 2 - 0.7, (shape[0] - 1) * s / 2], F)
+    origin = np.array([-3.0, 2.0, 1.0], F)
+    print(f"[case] raytrace shape={shape} s={s} ga={ga} ta={ta}", flush=True)
+    got = engine.stage_raytrace(_cuda(t, rho), s, origin, iso, ga, ta, 1635.0, border).cpu().numpy()
+    g = O.geom(shape, s, origin, iso, ga, ta, 0.0, 1635.0)
+    ref, _ = O.raytrace(g, rho, border)
+    _report("raytrace", got, ref, TOL_WET)
+
+
+@pytest.mark.parametrize("shape, s, ga, ta, border", RT_CASES)
+def test_smooth_stage(shape, s, ga, ta, border):
+    t = _torch()
+    from dosecuda_amd import engine
+    rng = np.random.default_rng(12)
+    wet = (rng.uniform(0.0, 2.0, shape) + np.linspace(0, 3, shape[1])[None, :, None]).astype(F)
+    iso = np.array([(shape[2] - 1) * s / 2, (shape[1] - 1) * s / 2, (shape[0] - 1) * s / 2], F)
+    print(f"[case] smooth shape={shape} ga={ga} ta={ta}", flush=True)
+    got = engine.stage_smooth(_cuda(t, wet), s, np.zeros(3), iso, ga, ta, 1340.0, 1930.0).cpu().numpy()
+    g = O.geom(shape, s, np.zeros(3), iso, ga, ta, 0.0, (F(1340.0) + F(1930.0)) / F(2.0))
+    ref = O.smooth(g, wet)
+    _report("smooth", got, ref, TOL_WET)
+
+
+def _impt_case(proton_model, shape=(36, 44, 40), s=2.5, n_layers=4, side=8, seed=21, ga=0.0):
+    from dosecuda_amd import phantoms
+    rng = np.random.default_rng(seed)
+    rho = np.full(shape, 1.00653, F)
+    d0 = int(rng.integers(8, 16))
+    rho[:, d0:d0 + 6, :] = 1.5562
+    iso = np.array([(shape[2] - 1) * s / 2, (shape[1] - 1) * s / 2, (shape[0] - 1) * s / 2], F)
+    spots, _ = phantoms.impt_spot_grid(proton_model, seed=seed, n_side=side, pitch=5.0,
+                                       n_layers=n_layers, e_lo=70.0, e_hi=110.0)
+    return rho, iso, spots
+
+
+@pytest.mark.parametrize("ga", [0.0, 135.0])
+def test_pencil_stage(proton_model, ga):
+    t = _torch()
+    from dosecuda_amd import engine
+    m = proton_model
+    rho, iso, spots = _impt_case(m, ga=ga)
+    shape, s = rho.shape, 2.5
+    g = O.geom(shape, s, np.zeros(3), iso, ga, 0.0, 0.0, (F(m.VSADX) + F(m.VSADY)) / F(2.0))
+    raw, _ = O.raytrace(g, rho, 0.0)
+    wet = O.smooth(g, raw.astype(F)).astype(F)
+    print(f"[case] pencil shape={shape} ga={ga} spots={len(spots)}", flush=True)
+    plan = engine.ProtonPlan(0, shape, s, np.zeros(3), iso, ga, 0.0, m, spots)
+    got = plan.dose(_cuda(t, wet)).cpu().numpy()
+    ref, pairs = O.pencil(g, m.VSADX, m.VSADY, wet, spots, m.divergence_params, m.lut_depths,
+                          m.lut_sigmas, m.lut_idds)
+    assert pairs == plan.pairs(wet)
+    _report("pencil", got, ref, TOL_DOSE)
+    plan.close()
+
+
+def _imrt_case(photon_model, shape=(28, 32, 30), s=2.5, seed=31):
+    rng = np.random.default_rng(seed)
+    rho = np.full(shape, 1.00653, F)
+    c = np.array(shape) // 2 + rng.integers(-3, 4, 3)
+    k, j, i = np.ogrid[:shape[0], :shape[1], :shape[2]]
+    rho[((k - c[0]) / 5.0) ** 2 + ((j - c[1]) / 6.0) ** 2 + ((i - c[2]) / 4.0) ** 2 <= 1] = 0.3037
+    rho[(j - c[1] - 4) ** 2 + (i - c[2] + 5) ** 2 <= 9] = 1.5562
+    iso = np.array([(shape[2] - 1) * s / 2, (shape[1] - 1) * s / 2, (shape[0] - 1) * s / 2], F)
+    return rho, iso
+
+
+@pytest.mark.parametrize("ga, ca, ta", [(0.0, 0.0, 0.0), (51.43, 20.0, 0.0), (257.0, 0.0, 10.0)])
+def test_terma_stage(photon_model, ga, ca, ta):
+    t = _torch()
+    from dosecuda_amd import engine
+    from dosecuda_amd.beam_models import square_field_mlc
+    pm = photon_model
+    rho, iso = _imrt_case(pm)
+    shape, s = rho.shape, 2.5
+    mlc = square_field_mlc(pm, 5.0, 4.0)
+    g = O.geom(shape, s, np.zeros(3), iso, ga, ta, ca, F(pm.primary_source_distance))
+    raw, _ = O.raytrace(g, rho, float(F(0.0012)))
+    wet = raw.astype(F)
+    print(f"[case] terma shape={shape} ga={ga} ca={ca} ta={ta}", flush=True)
+    tg, eg = engine.stage_terma(_cuda(t, wet), s, np.zeros(3), iso, 100.0, ga, ca, ta, pm, mlc)
+    tr, er = O.terma(g, pm, mlc, wet)
+    _report("terma", tg.cpu().numpy(), tr, 1e-5)
+    _report("electron", eg.cpu().numpy(), er, 1e-5)
+
+
+@pytest.mark.parametrize("ga, ca, ta", [(0.0, 0.0, 0.0), (102.86, 30.0, 0.0), (308.57, 0.0, 20.0)])
+def test_ccc_stage(photon_model, ga, ca, ta):
+    t = _torch()
+    from dosecuda_amd import engine
+    from dosecuda_amd.beam_models import square_field_mlc
+    pm = photon_model
+    rho, iso = _imrt_case(pm)
+    shape, s = rho.shape, 2.5
+    mlc = square_field_mlc(pm, 5.0, 4.0)
+    g = O.geom(shape, s, np.zeros(3), iso, ga, ta, ca, F(pm.primary_source_distance))
+    raw, _ = O.raytrace(g, rho, float(F(0.0012)))
+    tr, er = O.terma(g, pm, mlc, raw.astype(F))
+    t32, e32 = tr.astype(F), er.astype(F)
+    print(f"[case] ccc shape={shape} ga={ga} ca={ca} ta={ta}", flush=True)
+    got = engine.stage_ccc(_cuda(t, t32), _cuda(t, rho), _cuda(t, e32), s, np.zeros(3), iso,
+                           100.0, ga, ca, ta, pm).cpu().numpy()
+    ref, n = O.ccc(g, pm, 100.0, t32, rho, e32)
+    assert n > 0
+    _report("ccc", got, ref, TOL_DOSE)
+
+
+def test_dropin_impt_end_to_end(proton_model):
+    _torch()
+    from dosecuda_amd import dose_kernels
+    from dosecuda_amd.beam_models import ProtonField, VolumeObject
+    m = proton_model
+    rho, iso, spots = _impt_case(m, shape=(30, 40, 34), n_layers=3, side=6)
+    s = 2.5
+    vol = VolumeObject(rho, origin=(0, 0, 0), spacing=(s, s, s))
+    beam = ProtonField(iso, 0.0, 0.0, spots)
+    print("[case] drop-in impt", flush=True)
+    wet = dose_kernels.proton_raytrace_cuda(m, vol, beam, 0)
+    assert wet.dtype == np.float32 and wet.shape == rho.shape
+    raw, sm, _ = O.proton_wet(rho, s, np.zeros(3), iso, 0.0, 0.0, m.VSADX, m.VSADY)
+    _report("dropin wet", wet, sm, TOL_WET)
+    wv = VolumeObject(wet, origin=(0, 0, 0), spacing=(s, s, s))
+    dose = dose_kernels.proton_spot_cuda(m, vol, wv, beam, 0)
+    ref, _ = O.proton_dose(wet, s, np.zeros(3), iso, 0.0, 0.0, m.VSADX, m.VSADY, spots,
+                           m.divergence_params, m.lut_depths, m.lut_sigmas, m.lut_idds)
+    _report("dropin impt dose", dose, ref, TOL_DOSE)
+
+
+def test_dropin_photon_end_to_end(photon_model):
+    _torch()
+    from dosecuda_amd import dose_kernels
+    from dosecuda_amd.beam_models import PhotonControlPoint, VolumeObject, square_field_mlc
+    pm = photon_model
+    rho, iso = _imrt_case(pm, shape=(26, 30, 28))
+    s = 2.5
+    vol = VolumeObject(rho, origin=(0, 0, 0), spacing=(s, s, s))
+    cp = PhotonControlPoint(iso, 100.0, square_field_mlc(pm, 6.0, 5.0), 30.0, 15.0, 0.0)
+    print("[case] drop-in photon", flush=True)
+    dose = dose_kernels.photon_dose_cuda(pm, vol, cp, 0)
+    assert dose.dtype == np.float32 and dose.shape == rho.shape
+    ref = O.photon_dose(rho, s, np.zeros(3), iso, 100.0, 30.0, 15.0, 0.0, pm, cp.mlc)
+    _report("dropin photon dose", dose, ref, TOL_DOSE)
+
+
+def test_invalid_gpu_id_raises(proton_model):
+    _torch()
+    from dosecuda_amd import dose_kernels
+    from dosecuda_amd.beam_models import ProtonField, VolumeObject
+    vol = VolumeObject(np.ones((4, 4, 4), F), spacing=(2, 2, 2))
+    beam = ProtonField(np.zeros(3), 0.0, 0.0, [[0, 0, 1, 10]])
+    with pytest.raises(RuntimeError, match="CUDA error: "):
+        dose_kernels.proton_raytrace_cuda(proton_model, vol, beam, 97)
