@@ -37,6 +37,9 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=15.0,
                    help="approximate budget of the CPU baseline sample")
     p.add_argument("--workload", default="all", choices=["all", "cccs", "impt"])
+    p.add_argument("--copy-calib-mb", type=int, default=0,
+                   help="after the run, copy a buffer of this many MiB once (a counter "
+                        "calibration kernel of known traffic for rocprofv3 --pmc runs)")
     return p.parse_args()
 
 
@@ -276,6 +279,14 @@ def main():
         "cccs_detail": cc,
     }
     print(json.dumps(line))
+    if a.copy_calib_mb:
+        n = a.copy_calib_mb * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: {a.copy_calib_mb} MiB read + {a.copy_calib_mb} MiB written")
 
 
 if __name__ == "__main__":

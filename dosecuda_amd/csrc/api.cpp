@@ -198,11 +198,12 @@ static dh_proton_tables build_proton_tables(const dh_proton_beam *beam, const dh
     const float vx = (float)beam->vsadx, vy = (float)beam->vsady;
     t.spots.resize((size_t)8 * s.size());
     for (size_t q = 0; q < s.size(); q++) {
-        float tx = s[q].x / vx, ty = s[q].y / vy;            // IMPTClasses.cu:125
-        float tt = tx * tx + ty * ty + (-1.0f) * (-1.0f);
+        // unit spot-axis direction (x, y components), IMPTClasses.cu:125
+        float tx = s[q].x / vx, ty = s[q].y / vy;
+        float n = (float)(1.0 / std::sqrt((double)tx * tx + (double)ty * ty + 1.0));
         float *r = &t.spots[8 * q];
-        r[0] = s[q].x; r[1] = s[q].y; r[2] = tx; r[3] = ty;
-        r[4] = 1.0f / tt; r[5] = s[q].mu; r[6] = 0.0f; r[7] = 0.0f;
+        r[0] = s[q].x; r[1] = s[q].y; r[2] = tx * n; r[3] = ty * n;
+        r[4] = s[q].mu; r[5] = 0.0f; r[6] = 0.0f; r[7] = 0.0f;
     }
     int start = 0;
     for (int e = 0; e < lut->n_energies; e++) {

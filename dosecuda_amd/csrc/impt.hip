@@ -50,15 +50,34 @@ k_raytrace(dh_geom g, dh_pad p, const float *__restrict__ vol, float *__restrict
     const float fnx = (float)g.nx, fny = (float)g.ny, fnz = (float)g.nz;
     double acc = 0.0;
     float L = 0.0f;
+    // Four samples per trip: positions and the exit test use the exact FP32
+    // sequence of the per-sample loop (L = 0, 1, 2, ... is exact in FP32), and
+    // inside-ness is monotone along a ray leaving a box, so a sample counts iff
+    // every earlier one was inside.  Loads of samples past the exit read
+    // clamped, in-range addresses and are discarded.
     for (;;) {
-        float qx = fmaf(ux, L, x), qy = fmaf(uy, L, y), qz = fmaf(uz, L, z);
-        float tx = fmaf(g.inv_s, qx + g.iso[0], 0.5f);
-        float ty = fmaf(g.inv_s, qy + g.iso[1], 0.5f);
-        float tz = fmaf(g.inv_s, qz + g.iso[2], 0.5f);
-        if (!(tx >= 0.0f && tx < fnx && ty >= 0.0f && ty < fny && tz >= 0.0f && tz < fnz)) break;
-        float d = dh_trilinear(vol, p, g.nx, g.ny, g.nz, tx, ty, tz);
-        acc += (double)fmaxf(d, 0.0f);
-        L += 1.0f;
+        float d[4];
+        bool in[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const float Lq = L + (float)q;
+            const float qx = fmaf(ux, Lq, x), qy = fmaf(uy, Lq, y), qz = fmaf(uz, Lq, z);
+            const float tx = fmaf(g.inv_s, qx + g.iso[0], 0.5f);
+            const float ty = fmaf(g.inv_s, qy + g.iso[1], 0.5f);
+            const float tz = fmaf(g.inv_s, qz + g.iso[2], 0.5f);
+            in[q] = tx >= 0.0f && tx < fnx && ty >= 0.0f && ty < fny && tz >= 0.0f && tz < fnz;
+            d[q] = dh_trilinear(vol, p, g.nx, g.ny, g.nz, tx, ty, tz);
+        }
+        float part = 0.0f;
+        bool all = true;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            all = all && in[q];
+            part += all ? fmaxf(d[q], 0.0f) : 0.0f;
+        }
+        acc += (double)part;
+        if (!all) break;
+        L += 4.0f;
     }
     wet[v] = (float)(-0.05 + 0.1 * acc);
 }
@@ -115,7 +134,7 @@ k_smooth(dh_geom g, smooth_dirs dirs, const float *__restrict__ wet_in, float *_
 // that every IDD sample of the plan is non-negative).
 template <bool Fold>
 __global__ void __launch_bounds__(kBlock)
-k_pencil(dh_geom g, int n_layers, const dh_layer_k *__restrict__ layers,
+k_pencil(dh_geom g, float vsadx, float vsady, int n_layers, const dh_layer_k *__restrict__ layers,
          const float4 *__restrict__ luts, const float *__restrict__ spots,
          const float *__restrict__ wet, float *__restrict__ dose)
 {
@@ -134,7 +153,12 @@ k_pencil(dh_geom g, int n_layers, const dh_layer_k *__restrict__ layers,
     const float dxs = g.src[0] - x, dys = g.src[1] - y, dzs = g.src[2] - z;
     const float dist = sqrtf(dxs * dxs + dys * dys + dzs * dzs);   // norm3df
     const float w = wet[vv] * 10.0f;                                // cm -> mm
-    const float hz2 = hz * hz;
+    // The spot axis through (sx, sy, 0) with direction (sx/VSADX, sy/VSADY, -1)
+    // crosses the voxel's BEV depth hz at (sx*cx, sy*cy, hz); the squared
+    // distance to the axis is |u|^2 - (u.t)^2 for the in-plane offset
+    // u = (hx - sx*cx, hy - sy*cy, 0) and the unit axis t: the same quantity as
+    // caxDistance (IMPTClasses.cu:123-133) without its hz^2 cancellation.
+    const float cx = 1.0f - hz / vsadx, cy = 1.0f - hz / vsady;
     float tot = 0.0f, comp = 0.0f;
 
     for (int l = 0; l < n_layers; l++) {
@@ -190,14 +214,11 @@ k_pencil(dh_geom g, int n_layers, const dh_layer_k *__restrict__ layers,
             float part = 0.0f;
             for (int u = 0; u < e; u++) {
                 const float *rec = sp + 8 * (s0 + u);
-                const float sx = rec[0], sy = rec[1], tx = rec[2], ty = rec[3];
-                const float itt = rec[4], mu = rec[5];
-                // caxDistance (IMPTClasses.cu:123-133): |r|^2 - (tau.r)^2/|tau|^2
-                const float rx = hx - sx, ry = hy - sy;
-                const float r2 = fmaf(rx, rx, fmaf(ry, ry, hz2));
-                const float dt = fmaf(tx, rx, fmaf(ty, ry, -hz));
-                const float qd = dt * itt;
-                const float d2 = fmaf(-qd, dt, r2);
+                const float sx = rec[0], sy = rec[1], tx = rec[2], ty = rec[3], mu = rec[4];
+                const float ux = fmaf(-sx, cx, hx), uy = fmaf(-sy, cy, hy);
+                const float r2 = fmaf(ux, ux, uy * uy);
+                const float dt = fmaf(tx, ux, ty * uy);
+                const float d2 = fmaf(-dt, dt, r2);
                 float val;
                 if (Fold) {
                     val = __builtin_amdgcn_exp2f(fmaf(ap, d2, lp)) + __builtin_amdgcn_exp2f(fmaf(ah, d2, lh));
@@ -249,17 +270,17 @@ void dh_launch_smooth(const dh_geom &g, const float *wet_raw, float *wet_out, hi
     DH_HIP(hipGetLastError());
 }
 
-void dh_launch_pencil(const dh_geom &g, float, float, int n_layers, const dh_layer_k *d_layers,
+void dh_launch_pencil(const dh_geom &g, float vsadx, float vsady, int n_layers, const dh_layer_k *d_layers,
                       const float4 *d_luts, const float *d_spots, bool fold, const float *wet,
                       float *dose, hipStream_t s)
 {
     size_t n = (size_t)g.nx * g.ny * g.nz;
     dh_timed t("pencil", s);
     if (fold)
-        hipLaunchKernelGGL(k_pencil<true>, dim3(grid_for(n)), dim3(kBlock), 0, s, g, n_layers,
+        hipLaunchKernelGGL(k_pencil<true>, dim3(grid_for(n)), dim3(kBlock), 0, s, g, vsadx, vsady, n_layers,
                            d_layers, d_luts, d_spots, wet, dose);
     else
-        hipLaunchKernelGGL(k_pencil<false>, dim3(grid_for(n)), dim3(kBlock), 0, s, g, n_layers,
+        hipLaunchKernelGGL(k_pencil<false>, dim3(grid_for(n)), dim3(kBlock), 0, s, g, vsadx, vsady, n_layers,
                            d_layers, d_luts, d_spots, wet, dose);
     DH_HIP(hipGetLastError());
 }

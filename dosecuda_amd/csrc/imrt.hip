@@ -5,8 +5,8 @@
 // into one padded float2 volume {TERMA (border 0), density (border 0.0012)}:
 // one 8-byte gather per trilinear corner serves both fields.  Voxels with
 // TERMA <= 0.01 are written as zero by a compaction pass that also builds the
-// list of voxels the cone kernel visits (ballot + one atomic per wave), so the
-// cone kernel's waves are full of working lanes.
+// list of voxels the cone kernel visits (wave ballots, one atomic per
+// workgroup), so the cone kernel's waves are full of working lanes.
 #include "dh_runtime.h"
 
 namespace {
@@ -135,10 +135,14 @@ __device__ static inline float center_terma(const dh_geom &g, const dh_pad &p,
 }
 
 // Builds the list of voxels with TERMA > 0.01 and zeroes the dose of the rest.
+// One atomic per workgroup: waves ballot, the workgroup prefix-sums the four
+// wave counts in LDS and reserves its slice of the list at once.
 __global__ void __launch_bounds__(kBlock)
 k_ccc_compact(dh_geom g, dh_pad p, const float2 *__restrict__ tr, int *__restrict__ list,
               int *__restrict__ counter, float *__restrict__ dose)
 {
+    __shared__ int s_cnt[kBlock / 64];
+    __shared__ int s_base;
     const uint32_t nvox = (uint32_t)g.nx * g.ny * g.nz;
     const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
     bool act = false;
@@ -152,15 +156,19 @@ k_ccc_compact(dh_geom g, dh_pad p, const float2 *__restrict__ tr, int *__restric
         if (!act) dose[v] = 0.0f;
     }
     const uint64_t m = __builtin_amdgcn_ballot_w64(act);
-    if (m == 0ull) return;
     const int lane = __lane_id();
-    const int cnt = __popcll(m);
-    int base = 0;
-    if (lane == __ffsll((unsigned long long)m) - 1) base = atomicAdd(counter, cnt);
-    base = __shfl(base, __ffsll((unsigned long long)m) - 1, 64);
+    const int wave = threadIdx.x >> 6;
+    if (lane == 0) s_cnt[wave] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int w = 0; w < kBlock / 64; w++) { int c = s_cnt[w]; s_cnt[w] = tot; tot += c; }
+        s_base = tot ? atomicAdd(counter, tot) : 0;
+    }
+    __syncthreads();
     if (act) {
         const uint64_t below = m & ((1ull << lane) - 1ull);
-        list[base + __popcll(below)] = (int)v;
+        list[s_base + s_cnt[wave] + __popcll(below)] = (int)v;
     }
 }
 
@@ -170,6 +178,14 @@ k_ccc_compact(dh_geom g, dh_pad p, const float2 *__restrict__ tr, int *__restric
 // to 0 in FP32 steps of spacing/10 cm (the reference's own step lattice), with
 // the primary (exponential) and scatter (linear) recursions.  Cone results are
 // summed with Neumaier compensation.
+// Work split: a wave holds 16 listed voxels x 4 azimuth groups (lane / 16);
+// group q marches azimuths 3q..3q+2 of all six zenith rows (equal work), so a
+// wave's 16 lanes of one group sample the same cone direction for 16
+// neighbouring voxels (coalesced gathers) and the tail of the voxel list is
+// spread over 4x more, 4x shorter work items.  The four group sums are
+// combined with a fixed xor-shuffle tree, so the result is deterministic.
+constexpr int kVoxPerWave = 16;
+
 __global__ void __launch_bounds__(kBlock)
 k_ccc(dh_ccc_args a, const float2 *__restrict__ tr, const float *__restrict__ electron,
       const int *__restrict__ list, const int *__restrict__ counter, float *__restrict__ dose)
@@ -177,8 +193,14 @@ k_ccc(dh_ccc_args a, const float2 *__restrict__ tr, const float *__restrict__ el
     const dh_geom &g = a.g;
     const int n = *counter;
     const float fnx = (float)g.nx, fny = (float)g.ny, fnz = (float)g.nz;
-    for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < n; idx += gridDim.x * kBlock) {
-        const uint32_t v = (uint32_t)list[idx];
+    const int lane = __lane_id();
+    const int grp = lane >> 4;
+    const int wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
+    const int nwaves = (gridDim.x * kBlock) >> 6;
+    for (int base = wave * kVoxPerWave; base < n; base += nwaves * kVoxPerWave) {
+        const int idx = base + (lane & (kVoxPerWave - 1));
+        const bool valid = idx < n;
+        const uint32_t v = (uint32_t)list[valid ? idx : base];
         const int i = (int)(v % (uint32_t)g.nx);
         const uint32_t r = v / (uint32_t)g.nx;
         const int j = (int)(r % (uint32_t)g.ny);
@@ -201,10 +223,11 @@ k_ccc(dh_ccc_args a, const float2 *__restrict__ tr, const float *__restrict__ el
             const float c1 = a.c1[row], c2p = a.c2p[row], bp = a.bp[row], a2 = a.a2[row];
             const float am = a.a[row], bm = a.b[row], L0 = a.L0[row];
             const int ns = a.nsteps[row];
-            for (int az = 0; az < 12; az++) {
-                const float dx0 = a.dir[3 * (row * 12 + az) + 0];
-                const float dy0 = a.dir[3 * (row * 12 + az) + 1];
-                const float dz0 = a.dir[3 * (row * 12 + az) + 2];
+            for (int az3 = 0; az3 < 3; az3++) {
+                const int cone = row * 12 + grp * 3 + az3;
+                const float dx0 = a.dir[3 * cone + 0];
+                const float dy0 = a.dir[3 * cone + 1];
+                const float dz0 = a.dir[3 * cone + 2];
                 const float sc = (kx * dx0 + ky * dy0 + kz * dz0) * inv1c;
                 const float dx = (ky * dz0 - kz * dy0) + dx0 * c + kx * sc;
                 const float dy = (kz * dx0 - kx * dz0) + dy0 * c + ky * sc;
@@ -244,8 +267,13 @@ k_ccc(dh_ccc_args a, const float2 *__restrict__ tr, const float *__restrict__ el
                 dh_neumaier(ds, dcmp, fmaf(am, Rp, bm * Rs));
             }
         }
-        float dv = (ds + dcmp) + electron[v];
-        dose[v] = (!isnan(dv) && dv >= 0.0f) ? a.mu_cal * dv * a.mu : 0.0f;
+        float part = ds + dcmp;
+        part = part + __shfl_xor(part, 16, 64);
+        part = part + __shfl_xor(part, 32, 64);
+        if (grp == 0 && valid) {
+            float dv = part + electron[v];
+            dose[v] = (!isnan(dv) && dv >= 0.0f) ? a.mu_cal * dv * a.mu : 0.0f;
+        }
     }
 }
 

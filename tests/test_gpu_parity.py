@@ -119,8 +119,10 @@ def _imrt_case(photon_model, shape=(28, 32, 30), s=2.5, seed=31):
     rho = np.full(shape, 1.00653, F)
     c = np.array(shape) // 2 + rng.integers(-3, 4, 3)
     k, j, i = np.ogrid[:shape[0], :shape[1], :shape[2]]
-    rho[((k - c[0]) / 5.0) ** 2 + ((j - c[1]) / 6.0) ** 2 + ((i - c[2]) / 4.0) ** 2 <= 1] = 0.3037
-    rho[(j - c[1] - 4) ** 2 + (i - c[2] + 5) ** 2 <= 9] = 1.5562
+    lung = ((k - c[0]) / 5.0) ** 2 + ((j - c[1]) / 6.0) ** 2 + ((i - c[2]) / 4.0) ** 2 <= 1
+    rho[np.broadcast_to(lung, shape)] = 0.3037
+    bone = (j - c[1] - 4) ** 2 + (i - c[2] + 5) ** 2 <= 9
+    rho[np.broadcast_to(bone, shape)] = 1.5562
     iso = np.array([(shape[2] - 1) * s / 2, (shape[1] - 1) * s / 2, (shape[0] - 1) * s / 2], F)
     return rho, iso
 
@@ -210,3 +212,64 @@ def test_invalid_gpu_id_raises(proton_model):
     beam = ProtonField(np.zeros(3), 0.0, 0.0, [[0, 0, 1, 10]])
     with pytest.raises(RuntimeError, match="CUDA error: "):
         dose_kernels.proton_raytrace_cuda(proton_model, vol, beam, 97)
+
+
+def test_full_impt_field_100cubed(proton_model):
+    """Plain IMPT at full size: 100^3 @ 2.5 mm, water with a seeded 2 cm bone
+    slab, 20x20 spots x 10 layers (100-160 MeV), unit MU, gantry 0."""
+    t = _torch()
+    from dosecuda_amd import engine, phantoms
+    m = proton_model
+    case = phantoms.impt_slab_case(seed=1001, n=100, spacing=2.5)
+    rlsp = phantoms.to_rlsp(case)
+    spots, _ = phantoms.impt_spot_grid(m, seed=1001)
+    print(f"[case] impt 100^3 spots={len(spots)}", flush=True)
+    d = _cuda(t, rlsp)
+    raw = t.empty_like(d)
+    wet = engine.proton_wet(d, case.spacing, case.origin, case.iso, 0.0, 0.0, m.VSADX, m.VSADY,
+                            raw_out=raw)
+    plan = engine.ProtonPlan(0, rlsp.shape, case.spacing, case.origin, case.iso, 0.0, 0.0, m, spots)
+    dose = plan.dose(wet).cpu().numpy()
+    wet = wet.cpu().numpy()
+    raw = raw.cpu().numpy()
+    ref_raw, ref_sm, _ = O.proton_wet(rlsp, case.spacing, case.origin, case.iso, 0.0, 0.0,
+                                      m.VSADX, m.VSADY)
+    _report("impt100 raw wet", raw, ref_raw, TOL_WET)
+    # smoothing decisions come from the GPU's own float32 raw WET
+    g = O.geom(rlsp.shape, case.spacing, case.origin, case.iso, 0.0, 0.0, 0.0,
+               (F(m.VSADX) + F(m.VSADY)) / F(2.0))
+    _report("impt100 smoothed wet", wet, O.smooth(g, raw), TOL_WET)
+    ref, pairs = O.pencil(g, m.VSADX, m.VSADY, wet, spots, m.divergence_params, m.lut_depths,
+                          m.lut_sigmas, m.lut_idds)
+    print(f"[parity] impt100 pairs {pairs}", flush=True)
+    _report("impt100 dose", dose, ref, TOL_DOSE)
+    plan.close()
+
+
+def test_full_cccs_beam_128cubed(photon_model):
+    """One oblique beam (gantry 51.43) of the plain CCCS plan on the 128^3
+    lung/bone phantom, 10x10 cm field, each stage against the oracle."""
+    t = _torch()
+    from dosecuda_amd import engine, phantoms
+    from dosecuda_amd.beam_models import square_field_mlc
+    pm = photon_model
+    case = phantoms.imrt_lung_bone_case(seed=1002, n=128, spacing=2.5)
+    dens = phantoms.to_density(case)
+    mlc = square_field_mlc(pm, 10.0, 10.0)
+    ga = 360.0 / 7
+    print("[case] cccs 128^3 gantry 51.43", flush=True)
+    d = _cuda(t, dens)
+    terma, el, wet = t.empty_like(d), t.empty_like(d), t.empty_like(d)
+    dose = engine.photon_dose(d, case.spacing, case.origin, case.iso, 100.0, ga, 0.0, 0.0, pm, mlc,
+                              terma_out=terma, electron_out=el, wet_out=wet)
+    dose, terma, el, wet = (x.cpu().numpy() for x in (dose, terma, el, wet))
+    g = O.geom(dens.shape, case.spacing, case.origin, case.iso, ga, 0.0, 0.0,
+               F(pm.primary_source_distance))
+    ref_wet, _ = O.raytrace(g, dens, float(F(0.0012)))
+    _report("cccs128 wet", wet, ref_wet, TOL_WET)
+    tr, er = O.terma(g, pm, mlc, wet)
+    _report("cccs128 terma", terma, tr, 1e-5)
+    _report("cccs128 electron", el, er, 1e-5)
+    ref, n = O.ccc(g, pm, 100.0, terma, dens, el)
+    print(f"[parity] cccs128 samples {n}", flush=True)
+    _report("cccs128 dose", dose, ref, TOL_DOSE)
