@@ -133,6 +133,38 @@ __device__ static inline float dh_trilinear(const float *__restrict__ vol, const
     return fmaf(az, b1 - b0, b0);
 }
 
+// XCD-aware workgroup numbering.  The dispatcher hands workgroup b to XCD
+// b % 8; this maps b to a logical index so that each XCD receives one
+// contiguous range of logical workgroups (and so one compact region of the
+// volume, which then stays in that XCD's own L2).  Bijective on [0, nb).
+constexpr int DH_NUM_XCD = 8;
+__device__ static inline int dh_xcd_block(int b, int nb)
+{
+    const int x = b % DH_NUM_XCD, q = b / DH_NUM_XCD;
+    const int base = nb / DH_NUM_XCD, rem = nb % DH_NUM_XCD;
+    return x * base + min(x, rem) + q;
+}
+
+// Trilinear sample at a texel-space point u = t - 0.5 known to lie in
+// [-0.5, n - 0.5) on every axis (no clamping needed).
+__device__ static inline float dh_trilinear_u(const float *__restrict__ vol, const dh_pad &p,
+                                              float ux, float uy, float uz)
+{
+    float fx0 = floorf(ux), fy0 = floorf(uy), fz0 = floorf(uz);
+    float ax = ux - fx0, ay = uy - fy0, az = uz - fz0;
+    const float *b = vol + dh_pad_index(p, (int)fx0, (int)fy0, (int)fz0);
+    const float *c = b + p.pxy;
+    float v000 = b[0], v100 = b[1], v010 = b[p.px], v110 = b[p.px + 1];
+    float v001 = c[0], v101 = c[1], v011 = c[p.px], v111 = c[p.px + 1];
+    float a00 = fmaf(ax, v100 - v000, v000);
+    float a10 = fmaf(ax, v110 - v010, v010);
+    float a01 = fmaf(ax, v101 - v001, v001);
+    float a11 = fmaf(ax, v111 - v011, v011);
+    float b0 = fmaf(ay, a10 - a00, a00);
+    float b1 = fmaf(ay, a11 - a01, a01);
+    return fmaf(az, b1 - b0, b0);
+}
+
 // Wave-level helpers (wave64).
 __device__ static inline bool dh_wave_any(bool pred)
 {

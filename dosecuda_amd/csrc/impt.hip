@@ -34,11 +34,36 @@ k_pad(const float *__restrict__ src, float *__restrict__ dst, int nx, int ny, in
 // leaves the volume; WET = -0.05 + 0.1 * sum(max(rho, 0)) cm.  The sample
 // lattice and exit test are the reference's FP32 arithmetic; the running sum
 // is FP64 so long heterogeneous rays do not lose low-order bits.
+// Number of leading samples L = 0, 1, ... that are provably inside the volume:
+// the exact ray/box exit distance in FP64 from the same FP32 inputs, minus a
+// margin of 1e-3 voxel per axis (the FP32 texel coordinate is within a few ulp
+// of the exact one, far below that margin).
+__device__ static inline int safe_samples(const dh_geom &g, float x, float y, float z,
+                                          float ux, float uy, float uz)
+{
+    const double inv = g.inv_s;
+    const float p[3] = {x, y, z}, u[3] = {ux, uy, uz};
+    const int n[3] = {g.nx, g.ny, g.nz};
+    double Lmin = 1e30;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const double t0 = inv * ((double)p[a] + g.iso[a]) + 0.5;
+        const double du = inv * (double)u[a];
+        if (du == 0.0) continue;
+        const double lim = du > 0.0 ? (double)n[a] : 0.0;
+        // margin: 1e-3 voxel plus the FP32 rounding of a coordinate of size n
+        const double La = (lim - t0) / du - (1e-3 + 2e-6 * n[a]) / fabs(du);
+        Lmin = fmin(Lmin, La);
+    }
+    if (!(Lmin >= 0.0)) return 0;
+    return (int)fmin(floor(Lmin) + 1.0, 1.0e7);
+}
+
 __global__ void __launch_bounds__(kBlock)
 k_raytrace(dh_geom g, dh_pad p, const float *__restrict__ vol, float *__restrict__ wet)
 {
     const uint32_t nvox = (uint32_t)g.nx * g.ny * g.nz;
-    const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t v = (uint32_t)dh_xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (v >= nvox) return;
     const int i = (int)(v % (uint32_t)g.nx);
     const uint32_t r = v / (uint32_t)g.nx;
@@ -49,12 +74,29 @@ k_raytrace(dh_geom g, dh_pad p, const float *__restrict__ vol, float *__restrict
     dh_uvec(g, x, y, z, ux, uy, uz);
     const float fnx = (float)g.nx, fny = (float)g.ny, fnz = (float)g.nz;
     double acc = 0.0;
-    float L = 0.0f;
-    // Four samples per trip: positions and the exit test use the exact FP32
-    // sequence of the per-sample loop (L = 0, 1, 2, ... is exact in FP32), and
-    // inside-ness is monotone along a ray leaving a box, so a sample counts iff
-    // every earlier one was inside.  Loads of samples past the exit read
+    // Bulk: samples known to be inside need no exit test; their texel
+    // coordinate is u = u0 + L * du (continuous in L, no decision rides on it).
+    const int nsafe = safe_samples(g, x, y, z, ux, uy, uz);
+    const float u0x = fmaf(g.inv_s, x + g.iso[0], 0.5f) - 0.5f;
+    const float u0y = fmaf(g.inv_s, y + g.iso[1], 0.5f) - 0.5f;
+    const float u0z = fmaf(g.inv_s, z + g.iso[2], 0.5f) - 0.5f;
+    const float dux = ux * g.inv_s, duy = uy * g.inv_s, duz = uz * g.inv_s;
+    int m = 0;
+    for (; m + 4 <= nsafe; m += 4) {
+        float part = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const float L = (float)(m + q);
+            part += fmaxf(dh_trilinear_u(vol, p, fmaf(dux, L, u0x), fmaf(duy, L, u0y),
+                                         fmaf(duz, L, u0z)), 0.0f);
+        }
+        acc += (double)part;
+    }
+    // Tail: the reference's exact FP32 sample positions and exit test, four
+    // at a time.  Inside-ness is monotone along a ray leaving a box, so a
+    // sample counts iff every earlier one was inside; loads past the exit read
     // clamped, in-range addresses and are discarded.
+    float L = (float)m;
     for (;;) {
         float d[4];
         bool in[4];

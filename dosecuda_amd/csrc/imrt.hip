@@ -195,9 +195,16 @@ k_ccc(dh_ccc_args a, const float2 *__restrict__ tr, const float *__restrict__ el
     const float fnx = (float)g.nx, fny = (float)g.ny, fnz = (float)g.nz;
     const int lane = __lane_id();
     const int grp = lane >> 4;
-    const int wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
-    const int nwaves = (gridDim.x * kBlock) >> 6;
-    for (int base = wave * kVoxPerWave; base < n; base += nwaves * kVoxPerWave) {
+    // Each wave takes one contiguous run of 16-voxel chunks of the list, and
+    // the waves of one XCD take neighbouring runs, so an XCD's gathers stay in
+    // one slab of the volume (its own L2).
+    const int wave = dh_xcd_block(blockIdx.x, gridDim.x) * (kBlock / 64) + (threadIdx.x >> 6);
+    const int nwaves = gridDim.x * (kBlock / 64);
+    const int nchunk = (n + kVoxPerWave - 1) / kVoxPerWave;
+    const int c0 = (int)((int64_t)nchunk * wave / nwaves);
+    const int c1 = (int)((int64_t)nchunk * (wave + 1) / nwaves);
+    for (int chunk = c0; chunk < c1; chunk++) {
+        const int base = chunk * kVoxPerWave;
         const int idx = base + (lane & (kVoxPerWave - 1));
         const bool valid = idx < n;
         const uint32_t v = (uint32_t)list[valid ? idx : base];
