@@ -36,7 +36,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     p.add_argument("--cpu-seconds", type=float, default=15.0,
                    help="approximate budget of the CPU baseline sample")
-    p.add_argument("--workload", default="all", choices=["all", "cccs", "impt"])
+    p.add_argument("--workload", default="all", choices=["all", "cccs", "impt", "influence"])
+    p.add_argument("--influence-steps", type=int, default=3)
     p.add_argument("--copy-calib-mb", type=int, default=0,
                    help="after the run, copy a buffer of this many MiB once (a counter "
                         "calibration kernel of known traffic for rocprofv3 --pmc runs)")
@@ -90,6 +91,43 @@ def run_impt(torch, st):
     engine.proton_wet(st["d_rlsp"], c.spacing, c.origin, c.iso, 0.0, 0.0, m.VSADX, m.VSADY,
                       out=st["wet"])
     st["plan"].dose(st["wet"], out=st["dose"])
+
+
+def setup_influence(torch):
+    from dosecuda_amd import engine, phantoms
+    from dosecuda_amd.beam_models import ProtonBeamModel
+    m = ProtonBeamModel()
+    case = phantoms.spheres_case(seed=1003)
+    rlsp = phantoms.to_rlsp(case)
+    d = torch.from_numpy(rlsp).cuda()
+    fields = phantoms.influence_fields(m, seed=1003)
+    st = dict(m=m, case=case, d_rlsp=d, fields=fields, wets=[], infl=[])
+    for g, spots in fields:
+        w = engine.proton_wet(d, case.spacing, case.origin, case.iso, g, 0.0, m.VSADX, m.VSADY)
+        st["wets"].append(w)
+        st["infl"].append(engine.InfluenceMatrix(0, rlsp.shape, case.spacing, case.origin, case.iso,
+                                                 g, 0.0, m, spots))
+    # size the shared CSR once
+    base = 0
+    for f, w in zip(st["infl"], st["wets"]):
+        base += f.count(w, base)
+    st["nnz"] = base
+    st["col"] = torch.empty(base, dtype=torch.int32, device="cuda")
+    st["val"] = torch.empty(base, dtype=torch.float32, device="cuda")
+    return st
+
+
+def run_influence(torch, st):
+    """One build of the 3-field CSR: per field ray trace + smoothing, peak and
+    count passes (one host sync each), fill into the shared CSR."""
+    from dosecuda_amd import engine
+    c, m = st["case"], st["m"]
+    base = 0
+    for (g, _), f, w in zip(st["fields"], st["infl"], st["wets"]):
+        engine.proton_wet(st["d_rlsp"], c.spacing, c.origin, c.iso, g, 0.0, m.VSADX, m.VSADY, out=w)
+        base += f.count(w, base)
+    for f, w in zip(st["infl"], st["wets"]):
+        f.fill(w, st["col"], st["val"])
 
 
 def cone_steps(L0, spacing_mm):
@@ -253,6 +291,30 @@ def main():
             cpu["cccs"] = dict(value=v, unit="ms/beam", sample=sample)
             log(f"CCCS CPU oracle: {v:.1f} ms/beam")
 
+    if a.workload in ("all", "influence"):
+        log("setting up the batched influence matrix: (160,256,256) @ 2 mm, 3 fields, ~15k spots")
+        st = setup_influence(torch)
+        ms = timed(torch, run_influence, st, a.influence_steps, 1)
+        kt = {k: engine.prof_get(k) for k in ("raytrace", "smooth", "influence_peak",
+                                              "influence_count", "influence_fill")}
+        fill_ms = kt["influence_fill"][0] / max(kt["influence_fill"][1], 1)
+        nvox = st["d_rlsp"].numel()
+        n_rows = sum(len(sp) for _, sp in st["fields"])
+        # algorithmic bytes of one fill launch (one field on average): the
+        # field's entries (u32 column + f32 value) and one read of its WET
+        bytes_launch = (8 * st["nnz"] + 4 * nvox * len(st["fields"])) / len(st["fields"])
+        ach = bytes_launch / (fill_ms * 1e-3) / 1e9
+        res["influence"] = dict(ms_per_build=ms, nnz=st["nnz"], rows=n_rows,
+                                fill_ms=fill_ms, csr_bytes=8 * st["nnz"] + 8 * (n_rows + 1),
+                                kernels_ms={k: v[0] / max(v[1], 1) for k, v in kt.items()},
+                                roofline=dict(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS,
+                                              unit="GB/s", frac=ach / HBM_PEAK_GBS, traffic=None))
+        log(f"influence: {ms:.2f} ms/build, nnz {st['nnz']}, fill {fill_ms:.3f} ms/field "
+            f"({ach:.0f} GB/s = {100 * ach / HBM_PEAK_GBS:.1f}% of HBM)")
+        for f in st["infl"]:
+            f.close()
+        del st
+
     cc = res.get("cccs")
     line = {
         "metric": "IMPT Gvoxel-spot/s + CCCS ms/beam; surrogate vs plain at 2%/2mm gamma>=99%",
@@ -277,6 +339,7 @@ def main():
         "impt": res.get("impt"),
         "impt_cpu_baseline": ({**cpu["impt"], "cores": threads, "kind": "port"} if "impt" in cpu else None),
         "cccs_detail": cc,
+        "influence": res.get("influence"),
     }
     print(json.dumps(line))
     if a.copy_calib_mb:

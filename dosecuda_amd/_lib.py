@@ -90,6 +90,12 @@ SIGNATURES = {
                            C.c_int32, _P, _P, _P, _P],
     "dh_stage_ccc_dev": [_I, C.POINTER(Grid), C.POINTER(PhotonCP), C.POINTER(PhotonModel), _P, _P,
                          _P, _P, _P],
+    "dh_influence_create": [_I, C.POINTER(Grid), C.POINTER(ProtonBeam), C.POINTER(ProtonLut), _P,
+                            C.c_int32, C.c_int32, C.c_int32, C.c_float, C.POINTER(C.c_void_p)],
+    "dh_influence_count": [_P, _P, C.c_int64, C.POINTER(C.c_int64), _P, _P],
+    "dh_influence_fill": [_P, _P, _P, _P, _P],
+    "dh_influence_info": [_P, C.POINTER(C.c_int32), _P, _P],
+    "dh_influence_destroy": [_P],
     "dh_prof_enable": [_I],
     "dh_prof_reset": [],
     "dh_prof_get": [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)],

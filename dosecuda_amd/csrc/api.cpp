@@ -27,20 +27,7 @@ int dh_set_error(int code, const char *msg)
     return code;
 }
 
-template <class F>
-static int guarded(F &&f)
-{
-    try {
-        f();
-        return DH_OK;
-    } catch (const dh_error &e) {
-        return dh_set_error(e.code, e.what());
-    } catch (const std::bad_alloc &) {
-        return dh_set_error(DH_E_NOMEM, "Not enough host memory");
-    } catch (const std::exception &e) {
-        return dh_set_error(DH_E_RUNTIME, e.what());
-    }
-}
+#define guarded dh_guarded
 
 extern "C" const char *dh_last_error(void) { return t_err.c_str(); }
 extern "C" int dh_version(void) { return 10000; }
@@ -161,19 +148,11 @@ extern "C" int dh_stage_raytrace_dev(int gpu_id, const dh_grid *grid, const floa
 // ---------------------------------------------------------------------------
 // proton: spots, layers and the pencil-beam dose
 // ---------------------------------------------------------------------------
-struct dh_proton_tables {
-    int n_layers = 0;
-    bool fold = true;
-    std::vector<dh_layer_k> layers;
-    std::vector<float4> luts;
-    std::vector<float> spots;   // 8 floats per sorted spot
-};
-
 // make_spot_array + importLayers (dosemodule.cu:211-225, IMPTClasses.cu:29-67).
 // The sort is stable, so spots of one layer keep the caller's order (the
 // reference's std::sort leaves that order unspecified).
-static dh_proton_tables build_proton_tables(const dh_proton_beam *beam, const dh_proton_lut *lut,
-                                            const float *spot_rows, int32_t n_spots)
+dh_proton_tables dh_build_proton_tables(const dh_proton_beam *beam, const dh_proton_lut *lut,
+                                        const float *spot_rows, int32_t n_spots)
 {
     DH_REQUIRE(beam && lut, "null argument");
     DH_REQUIRE(n_spots >= 0, "negative spot count");
@@ -184,20 +163,23 @@ static dh_proton_tables build_proton_tables(const dh_proton_beam *beam, const dh
                "null beam-model table");
     DH_REQUIRE(lut->row_stride >= DH_LUT_LENGTH || lut->n_energies == 0,
                "lookup tables need at least 400 columns");
-    struct sp { float x, y, mu; int id; };
+    struct sp { float x, y, mu; int id, row; };
     std::vector<sp> s((size_t)n_spots);
     for (int32_t q = 0; q < n_spots; q++) {
         s[q].x = spot_rows[4 * q];
         s[q].y = spot_rows[4 * q + 1];
         s[q].mu = spot_rows[4 * q + 2];
         s[q].id = static_cast<int>(spot_rows[4 * q + 3]);
+        s[q].row = q;
     }
     std::stable_sort(s.begin(), s.end(), [](const sp &a, const sp &b) { return a.id < b.id; });
 
     dh_proton_tables t;
     const float vx = (float)beam->vsadx, vy = (float)beam->vsady;
     t.spots.resize((size_t)8 * s.size());
+    t.order.resize(s.size());
     for (size_t q = 0; q < s.size(); q++) {
+        t.order[q] = s[q].row;
         // unit spot-axis direction (x, y components), IMPTClasses.cu:125
         float tx = s[q].x / vx, ty = s[q].y / vy;
         float n = (float)(1.0 / std::sqrt((double)tx * tx + (double)ty * ty + 1.0));
@@ -262,7 +244,7 @@ static dh_proton_plan *make_plan(int gpu_id, const dh_grid *grid, const dh_proto
 {
     nvox_of(grid);
     dh_device_enter(gpu_id);
-    dh_proton_tables t = build_proton_tables(beam, lut, spot_rows, n_spots);
+    dh_proton_tables t = dh_build_proton_tables(beam, lut, spot_rows, n_spots);
     dh_proton_plan *p = new dh_proton_plan();
     p->dev = gpu_id;
     p->grid = *grid;

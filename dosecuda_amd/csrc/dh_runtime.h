@@ -6,8 +6,10 @@
 #include "dh_common.h"
 #include "../../include/dosehip.h"
 
+#include <new>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 struct dh_error : std::runtime_error {
     int code;
@@ -24,6 +26,25 @@ struct dh_error : std::runtime_error {
     do {                                                                      \
         if (!(cond)) throw dh_error(DH_E_INVALID, (msg));                     \
     } while (0)
+
+// Records the thread-local message returned by dh_last_error().
+int dh_set_error(int code, const char *msg);
+
+// Runs f, mapping exceptions to DH_E_* codes and the thread-local message.
+template <class F>
+static inline int dh_guarded(F &&f)
+{
+    try {
+        f();
+        return DH_OK;
+    } catch (const dh_error &e) {
+        return dh_set_error(e.code, e.what());
+    } catch (const std::bad_alloc &) {
+        return dh_set_error(DH_E_NOMEM, "Not enough host memory");
+    } catch (const std::exception &e) {
+        return dh_set_error(DH_E_RUNTIME, e.what());
+    }
+}
 
 // Selects the device and initialises its state once.
 void dh_device_enter(int dev);
@@ -106,6 +127,21 @@ void dh_launch_terma(const dh_terma_args &a, const float *wet, float *terma, flo
 void dh_launch_ccc(const dh_ccc_args &a, const float *terma, const float *density,
                    const float *electron, float2 *tr_pad, int *active, int *counter,
                    float *dose, hipStream_t s);
+
+// A proton field's spots sorted by energy id (stable), its layers
+// (importLayers) and one 400-entry LUT row per layer.  spots holds 8 floats
+// per sorted spot: x, y, unit axis direction x/y components, MU; order[q] is
+// the caller's row of sorted spot q.
+struct dh_proton_tables {
+    int n_layers = 0;
+    bool fold = true;
+    std::vector<dh_layer_k> layers;
+    std::vector<float4> luts;
+    std::vector<float> spots;
+    std::vector<int> order;
+};
+dh_proton_tables dh_build_proton_tables(const dh_proton_beam *beam, const dh_proton_lut *lut,
+                                        const float *spot_rows, int32_t n_spots);
 
 // Host-side beam set-up shared by every entry point (dosemodule.cu:148-172,
 // CudaClasses.cu:17-51, IMRTClasses.cu:30-39).

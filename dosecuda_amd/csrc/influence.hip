@@ -1,0 +1,613 @@
+// influence.hip -- batched proton dose-influence matrix in CSR form.
+//
+// Rows are spots (mu = 1), columns voxel indices (u32), values the pencil-beam
+// dose of that spot at that voxel, the same formula as the dense pencil
+// kernel (IMPTClasses.cu:204-271), kept where value >= rel * (row peak).
+//
+// Work decomposition: the spots of a layer are grouped into blocks of S
+// spatially adjacent spots; a block's candidate voxels are those whose
+// beam's-eye-view position lies in the block's lateral box (spot positions
+// +- the truncation radius), swept in image order k, j, i.  The (k, j) rows of
+// a block are cut into C chunks; a work item is (block, chunk) and is pulled by
+// a persistent kernel from an atomic queue (largest first).  For each voxel of
+// a sweep the per-(voxel, layer) factors are computed once and shared by the
+// S spots.  Three passes use the same sweep:
+//   PEAK  : per-spot maximum (atomicMax on the float bits, values >= 0);
+//   COUNT : per (spot, chunk) number of entries >= threshold;
+//   FILL  : entries written at offsets from the exclusive scan of the counts,
+//           ballot-compacted, so each row's columns come out in ascending
+//           voxel order and the output is deterministic.
+#include "dh_runtime.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+enum { INF_PEAK = 0, INF_COUNT = 1, INF_FILL = 2 };
+
+struct dh_infl_block {
+    int layer, first, n, pad_;   // first: index of the block's spot list
+    float bx0, bx1, by0, by1;    // BEV (head) lateral box
+    int k0, k1, j0, j1;          // image rows swept (inclusive)
+};
+
+struct dh_infl_args {
+    dh_geom g;
+    float vsadx, vsady;
+    int n_items, C;
+    const dh_infl_block *blocks;
+    const int *block_spots;      // S entries per block (-1 padded)
+    const dh_layer_k *layers;
+    const float4 *luts;
+    const float *spots;          // 8 floats per spot: x, y, t^x, t^y, mu
+    const float *thr;            // per spot threshold (COUNT / FILL)
+    const float *wet;
+    unsigned *peak_bits;         // per spot (PEAK)
+    int *counts;                 // per spot * C (COUNT)
+    const int64_t *offsets;      // per spot * C (FILL)
+    uint32_t *col;
+    float *val;
+    int *queue;
+};
+
+namespace {
+
+// Interval of i (0 <= i < n) for which lo <= a + b*i <= hi, widened by one
+// voxel on each side (the box is only a pre-filter; the threshold decides).
+__device__ static inline void interval(float a, float b, float lo, float hi, int n, int &ilo, int &ihi)
+{
+    if (fabsf(b) < 1e-12f) {
+        if (a < lo || a > hi) { ilo = 1; ihi = 0; }
+        return;
+    }
+    float t0 = (lo - a) / b, t1 = (hi - a) / b;
+    if (t0 > t1) { float t = t0; t0 = t1; t1 = t; }
+    const int l = (int)floorf(fmaxf(t0, -2.0f)) - 1;
+    const int h = (int)ceilf(fminf(t1, (float)n + 2.0f)) + 1;
+    ilo = max(ilo, max(l, 0));
+    ihi = min(ihi, min(h, n - 1));
+}
+
+template <int S, int MODE>
+__global__ void __launch_bounds__(64)
+k_influence(dh_infl_args a)
+{
+    __shared__ float4 s_lut[DH_LUT_LENGTH];
+    __shared__ int s_item;
+    const dh_geom &g = a.g;
+    const int lane = __lane_id();
+    int cached_layer = -1;
+    for (;;) {
+        if (lane == 0) s_item = atomicAdd(a.queue, 1);
+        __syncthreads();
+        const int item = s_item;
+        __syncthreads();
+        if (item >= a.n_items) break;
+        const int bi = item / a.C, chunk = item % a.C;
+        const dh_infl_block B = a.blocks[bi];
+        if (B.layer != cached_layer) {
+            for (int q = lane; q < DH_LUT_LENGTH; q += 64) s_lut[q] = a.luts[(size_t)B.layer * DH_LUT_LENGTH + q];
+            __syncthreads();
+            cached_layer = B.layer;
+        }
+        const dh_layer_k L = a.layers[B.layer];
+        // spot records of the block (wave-uniform)
+        float sx[S], sy[S], tx[S], ty[S], th[S];
+        int sid[S];
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            const int id = a.block_spots[B.first * S + s];
+            sid[s] = id;
+            const int c = id >= 0 ? id : 0;
+            sx[s] = a.spots[8 * c]; sy[s] = a.spots[8 * c + 1];
+            tx[s] = a.spots[8 * c + 2]; ty[s] = a.spots[8 * c + 3];
+            th[s] = (MODE == INF_PEAK || id < 0) ? INFINITY : a.thr[c];
+        }
+        float pk[S];
+        int cnt[S];
+        int64_t cur[S];
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            pk[s] = 0.0f; cnt[s] = 0;
+            cur[s] = (MODE == INF_FILL && sid[s] >= 0) ? a.offsets[(size_t)sid[s] * a.C + chunk] : 0;
+        }
+        // slot end of each (spot, chunk): no write can leave its slot even if a
+        // fill decision disagreed with the count pass
+        int64_t lim[S];
+#pragma unroll
+        for (int s = 0; s < S; s++)
+            lim[s] = (MODE == INF_FILL && sid[s] >= 0) ? a.offsets[(size_t)sid[s] * a.C + chunk + 1] : 0;
+        // rows of this chunk
+        const int nj = B.j1 - B.j0 + 1, nk = B.k1 - B.k0 + 1;
+        const int64_t nrows = (int64_t)nj * nk;
+        const int64_t r0 = nrows * chunk / a.C, r1 = nrows * (chunk + 1) / a.C;
+        // head coordinates are linear in i: h(i) = h(0) + i * dh
+        float dhx, dhy, dhz;
+        {
+            float x0, y0, z0, x1, y1, z1;
+            dh_img_to_head(g, 0.0f, 0.0f, 0.0f, x0, y0, z0);
+            dh_img_to_head(g, g.s, 0.0f, 0.0f, x1, y1, z1);
+            dhx = x1 - x0; dhy = y1 - y0; dhz = z1 - z0;
+        }
+        for (int64_t rr = r0; rr < r1; rr++) {
+            const int k = B.k0 + (int)(rr / nj), j = B.j0 + (int)(rr % nj);
+            float x, y, z, hx0, hy0, hz0;
+            dh_vox_xyz(g, 0, j, k, x, y, z);
+            dh_img_to_head(g, x, y, z, hx0, hy0, hz0);
+            int ilo = 0, ihi = g.nx - 1;
+            interval(hx0, dhx, B.bx0, B.bx1, g.nx, ilo, ihi);
+            interval(hy0, dhy, B.by0, B.by1, g.nx, ilo, ihi);
+            for (int i0 = ilo; i0 <= ihi; i0 += 64) {
+                const int i = i0 + lane;
+                const bool inb = i <= ihi;
+                const uint32_t v = (uint32_t)min(i, ihi) + (uint32_t)g.nx * ((uint32_t)j + (uint32_t)g.ny * (uint32_t)k);
+                const float w = a.wet[v] * 10.0f;
+                const bool act = inb && !(w > L.cut);
+                if (!dh_wave_any(act)) continue;
+                float px, py, pz, hx, hy, hz;
+                dh_vox_xyz(g, min(i, ihi), j, k, px, py, pz);
+                dh_img_to_head(g, px, py, pz, hx, hy, hz);
+                // per (voxel, layer) factors: the dense kernel's prologue
+                int lo = 0, hi = DH_LUT_LENGTH;
+                while (lo < hi) {
+                    int m = (lo + hi) >> 1;
+                    if (w <= s_lut[m].x) hi = m; else lo = m + 1;
+                }
+                float idd, sms;
+                if (lo == DH_LUT_LENGTH) { idd = s_lut[lo - 1].z; sms = s_lut[lo - 1].y; }
+                else if (lo == 0) { idd = s_lut[0].z; sms = s_lut[0].y; }
+                else {
+                    const float4 p0 = s_lut[lo - 1], p1 = s_lut[lo];
+                    const float f = (w - p0.x) / (p1.x - p0.x);
+                    idd = fmaf(p1.z - p0.z, f, p0.z);
+                    sms = fmaf(p1.y - p0.y, f, p0.y);
+                }
+                const float dxs = g.src[0] - px, dys = g.src[1] - py, dzs = g.src[2] - pz;
+                const float dist = sqrtf(dxs * dxs + dys * dys + dzs * dzs);
+                const float da = dist - w + 0.7f * L.r80;
+                const float st = fmaf(fmaf(L.A, da, L.B), da, L.C) + sms;
+                const float r80 = L.r80;
+                const float wc = fminf(fmaxf(w, 0.1f), r80 - 0.1f);
+                const float t5 = wc - r80 - 5.0f;
+                float sh = 2.85f + (0.0014f * r80 * logf(wc + 3.0f)) + (0.06f * wc) - (7.4e-5f * (wc * wc))
+                         - ((0.22f * r80) / (t5 * t5));
+                const float r3 = r80 + 3.0f;
+                float hw = 0.052f * logf(1.13f + (wc / (11.2f - (0.023f * r80))))
+                         + (0.35f * ((0.0017f * (r80 * r80)) - r80) / ((r3 * r3) - (wc * wc)))
+                         - (1.61e-9f * wc * (r3 * r3));
+                sh = fmaxf(sh, 0.0f);
+                hw = fminf(fmaxf(hw, 0.0f), 0.9f);
+                const float sht = hypotf(st, sh);
+                const float P = (1.0f - hw) * idd / (2.0f * DH_PI_F * (st * st));
+                const float H = hw * idd / (2.0f * DH_PI_F * (sht * sht));
+                const float ap = st != 0.0f ? (-0.5f / (st * st)) * DH_LOG2E_F : -INFINITY;
+                const float ah = sht != 0.0f ? (-0.5f / (sht * sht)) * DH_LOG2E_F : -INFINITY;
+                const float cx = 1.0f - hz / a.vsadx, cy = 1.0f - hz / a.vsady;
+#pragma unroll
+                for (int s = 0; s < S; s++) {
+                    const float ux = fmaf(-sx[s], cx, hx), uy = fmaf(-sy[s], cy, hy);
+                    const float r2 = fmaf(ux, ux, uy * uy);
+                    const float dt = fmaf(tx[s], ux, ty[s] * uy);
+                    const float d2 = fmaf(-dt, dt, r2);
+                    const float val = fmaf(P, __builtin_amdgcn_exp2f(ap * d2), H * __builtin_amdgcn_exp2f(ah * d2));
+                    if (MODE == INF_PEAK) {
+                        pk[s] = act ? fmaxf(pk[s], val) : pk[s];
+                    } else {
+                        const bool keep = act && val >= th[s];
+                        const uint64_t m = __builtin_amdgcn_ballot_w64(keep);
+                        if (MODE == INF_COUNT) {
+                            cnt[s] += __popcll(m);
+                        } else {
+                            const int64_t pos = cur[s] + __popcll(m & ((1ull << lane) - 1ull));
+                            if (keep && pos < lim[s]) {
+                                a.col[pos] = v;
+                                a.val[pos] = val;
+                            }
+                            cur[s] += __popcll(m);
+                        }
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            if (sid[s] < 0) continue;
+            if (MODE == INF_PEAK) {
+                float m = pk[s];
+                for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+                if (lane == 0 && m > 0.0f) atomicMax(&a.peak_bits[sid[s]], __float_as_uint(m));
+            } else if (MODE == INF_COUNT) {
+                if (lane == 0) a.counts[(size_t)sid[s] * a.C + chunk] = cnt[s];
+            }
+        }
+    }
+}
+
+__global__ void k_zero_words(unsigned *p, size_t n)
+{
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) p[t] = 0u;
+}
+
+// Smallest box in the head frame: min/max over the volume corners of the
+// head coordinates (for the divergence scaling of the spot axes).
+static void head_depth_range(const dh_geom &g, float &hz_min, float &hz_max)
+{
+    hz_min = 1e30f; hz_max = -1e30f;
+    for (int c = 0; c < 8; c++) {
+        float x, y, z, hx, hy, hz;
+        dh_vox_xyz(g, (c & 1) ? g.nx - 1 : 0, (c & 2) ? g.ny - 1 : 0, (c & 4) ? g.nz - 1 : 0, x, y, z);
+        dh_img_to_head(g, x, y, z, hx, hy, hz);
+        hz_min = std::min(hz_min, hz);
+        hz_max = std::max(hz_max, hz);
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct dh_influence {
+    int dev = 0;
+    dh_geom g;
+    float vsadx = 0, vsady = 0;
+    int S = 16, C = 8;
+    int n_spots = 0;
+    float rel = 1e-4f;
+    std::vector<dh_layer_k> layers;
+    std::vector<float4> luts;
+    std::vector<float> spots;        // 8 floats per sorted spot
+    std::vector<int> spot_layer;
+    std::vector<float> sig_max, dose_max;   // per-layer bounds
+    std::vector<int> block_layer, block_first, block_n;
+    std::vector<int> block_spots;    // S per block
+    dh_layer_k *d_layers = nullptr;
+    float4 *d_luts = nullptr;
+    float *d_spots = nullptr;
+    int *d_block_spots = nullptr;
+    dh_infl_block *d_blocks = nullptr;
+    float *d_thr = nullptr;
+    unsigned *d_peak = nullptr;
+    int *d_counts = nullptr;
+    int64_t *d_offsets = nullptr;
+    int *d_queue = nullptr;
+    std::vector<float> peak;         // host copy after the peak pass
+    std::vector<float> radius;       // truncation radius per spot (COUNT = FILL)
+    std::vector<int> order;          // caller's spot row of each matrix row
+    std::vector<int64_t> offsets;    // host copy of the exclusive scan
+    int64_t nnz = -1;
+};
+
+// Conservative per-layer bounds on the total sigma and on the dose factor
+// P + H over every voxel the layer can reach (w in [0, 1.1 R80], source
+// distance within the volume's range), from the tables themselves.
+static void layer_bounds(const dh_layer_k &L, const float4 *lut, float dist_min, float dist_max,
+                         float &sig_max, float &dose_max)
+{
+    const float r80 = L.r80;
+    float s_ms_max = -1e30f, s_ms_min = 1e30f, idd_max = 0.0f;
+    for (int q = 0; q < DH_LUT_LENGTH; q++) {
+        if (lut[q].x > L.cut + 1.0f && q > 0) break;
+        s_ms_max = std::max(s_ms_max, lut[q].y);
+        s_ms_min = std::min(s_ms_min, lut[q].y);
+        idd_max = std::max(idd_max, lut[q].z);
+    }
+    float sa_max = -1e30f, sa_min = 1e30f;
+    for (int q = 0; q <= 64; q++) {
+        const float w = L.cut * q / 64.0f;
+        for (int e = 0; e < 2; e++) {
+            const float d = (e ? dist_max : dist_min) - w + 0.7f * r80;
+            const float sa = L.A * d * d + L.B * d + L.C;
+            sa_max = std::max(sa_max, sa);
+            sa_min = std::min(sa_min, sa);
+        }
+    }
+    float sh_max = 0.0f;
+    for (int q = 0; q <= 256; q++) {
+        const float wc = std::min(std::max(L.cut * q / 256.0f, 0.1f), r80 - 0.1f);
+        const float t5 = wc - r80 - 5.0f;
+        float sh = 2.85f + 0.0014f * r80 * logf(wc + 3.0f) + 0.06f * wc - 7.4e-5f * wc * wc
+                 - 0.22f * r80 / (t5 * t5);
+        sh_max = std::max(sh_max, sh);
+    }
+    const float st_max = std::max(0.0f, sa_max + s_ms_max);
+    const float st_min = std::max(1e-3f, sa_min + s_ms_min);
+    sig_max = 1.05f * std::sqrt(st_max * st_max + sh_max * sh_max) + 0.5f;
+    // P + H <= idd / (2 pi st^2) (both normalisations are at most that)
+    dose_max = 1.05f * idd_max / (2.0f * DH_PI_F * st_min * st_min);
+}
+
+dh_influence *dh_influence_make(int dev, const dh_geom &g, float vsadx, float vsady,
+                                const std::vector<dh_layer_k> &layers, const std::vector<float4> &luts,
+                                const std::vector<float> &spots, int n_spots, int S, int C, float rel)
+{
+    DH_REQUIRE(S == 8 || S == 16, "spots per block must be 8 or 16");
+    DH_REQUIRE(C >= 1 && C <= 256, "row chunks per block must be 1..256");
+    dh_influence *f = new dh_influence();
+    f->dev = dev; f->g = g; f->vsadx = vsadx; f->vsady = vsady; f->S = S; f->C = C;
+    f->n_spots = n_spots; f->rel = rel;
+    f->layers = layers; f->luts = luts; f->spots = spots;
+    // distances from the source to the volume (bounding sphere)
+    float dmin = 1e30f, dmax = 0.0f;
+    for (int c = 0; c < 8; c++) {
+        float x, y, z;
+        dh_vox_xyz(g, (c & 1) ? g.nx - 1 : 0, (c & 2) ? g.ny - 1 : 0, (c & 4) ? g.nz - 1 : 0, x, y, z);
+        const float d = std::sqrt((g.src[0] - x) * (g.src[0] - x) + (g.src[1] - y) * (g.src[1] - y) +
+                                  (g.src[2] - z) * (g.src[2] - z));
+        dmin = std::min(dmin, d);
+        dmax = std::max(dmax, d);
+    }
+    const float diag = 0.5f * g.s * std::sqrt((float)(g.nx * g.nx + g.ny * g.ny + g.nz * g.nz));
+    dmin = std::max(0.0f, dmin - diag);     // the source may face any point of the box
+    f->spot_layer.assign(n_spots, -1);
+    for (size_t l = 0; l < layers.size(); l++) {
+        float sm, dm;
+        layer_bounds(layers[l], &luts[l * DH_LUT_LENGTH], dmin, dmax, sm, dm);
+        f->sig_max.push_back(sm);
+        f->dose_max.push_back(dm);
+        // blocks of S spatially adjacent spots: sort the layer's spots by a
+        // Morton key of their position on a 4 mm grid
+        const int a0 = layers[l].start, n = layers[l].n;
+        std::vector<std::pair<uint32_t, int>> key(n);
+        for (int q = 0; q < n; q++) {
+            const int id = a0 + q;
+            f->spot_layer[id] = (int)l;
+            auto qz = [](float v) {
+                int c = (int)floorf(v / 4.0f) + 512;
+                return (uint32_t)std::min(std::max(c, 0), 1023);
+            };
+            uint32_t xq = qz(spots[8 * id]), yq = qz(spots[8 * id + 1]), m = 0;
+            for (int b = 0; b < 10; b++) m |= (((xq >> b) & 1u) << (2 * b)) | (((yq >> b) & 1u) << (2 * b + 1));
+            key[q] = {m, id};
+        }
+        std::stable_sort(key.begin(), key.end(), [](auto &x, auto &y) { return x.first < y.first; });
+        for (int q = 0; q < n; q += S) {
+            f->block_layer.push_back((int)l);
+            f->block_first.push_back(q);
+            f->block_n.push_back(std::min(S, n - q));
+            for (int s = 0; s < S; s++) f->block_spots.push_back(q + s < n ? key[q + s].second : -1);
+        }
+    }
+    const size_t nb = f->block_layer.size();
+    try {
+        DH_HIP(hipSetDevice(dev));
+        if (!layers.empty()) {
+            DH_HIP(hipMalloc(&f->d_layers, sizeof(dh_layer_k) * layers.size()));
+            DH_HIP(hipMalloc(&f->d_luts, sizeof(float4) * luts.size()));
+            DH_HIP(hipMemcpy(f->d_layers, layers.data(), sizeof(dh_layer_k) * layers.size(), hipMemcpyHostToDevice));
+            DH_HIP(hipMemcpy(f->d_luts, luts.data(), sizeof(float4) * luts.size(), hipMemcpyHostToDevice));
+        }
+        DH_HIP(hipMalloc(&f->d_spots, sizeof(float) * std::max<size_t>(spots.size(), 8)));
+        if (!spots.empty())
+            DH_HIP(hipMemcpy(f->d_spots, spots.data(), sizeof(float) * spots.size(), hipMemcpyHostToDevice));
+        DH_HIP(hipMalloc(&f->d_block_spots, sizeof(int) * std::max<size_t>(f->block_spots.size(), 1)));
+        if (nb)
+            DH_HIP(hipMemcpy(f->d_block_spots, f->block_spots.data(), sizeof(int) * f->block_spots.size(), hipMemcpyHostToDevice));
+        DH_HIP(hipMalloc(&f->d_blocks, sizeof(dh_infl_block) * std::max<size_t>(nb, 1)));
+        DH_HIP(hipMalloc(&f->d_thr, sizeof(float) * std::max(n_spots, 1)));
+        DH_HIP(hipMalloc(&f->d_peak, sizeof(unsigned) * std::max(n_spots, 1)));
+        DH_HIP(hipMalloc(&f->d_counts, sizeof(int) * std::max<size_t>((size_t)n_spots * C, 1)));
+        DH_HIP(hipMalloc(&f->d_offsets, sizeof(int64_t) * ((size_t)n_spots * C + 1)));
+        DH_HIP(hipMalloc(&f->d_queue, 256));
+    } catch (...) {
+        void dh_influence_free(dh_influence *);
+        dh_influence_free(f);
+        throw;
+    }
+    return f;
+}
+
+void dh_influence_free(dh_influence *f)
+{
+    if (!f) return;
+    (void)hipSetDevice(f->dev);
+    void *ps[] = {f->d_layers, f->d_luts, f->d_spots, f->d_block_spots, f->d_blocks, f->d_thr,
+                  f->d_peak, f->d_counts, f->d_offsets, f->d_queue};
+    for (void *p : ps)
+        if (p) (void)hipFree(p);
+    delete f;
+}
+
+// Blocks' boxes for a pass: radius per spot r[s] (mm).
+static std::vector<dh_infl_block> make_blocks(const dh_influence *f, const std::vector<float> &r)
+{
+    const dh_geom &g = f->g;
+    float hz_min, hz_max;
+    head_depth_range(g, hz_min, hz_max);
+    const float cmin_x = std::min(1.0f - hz_max / f->vsadx, 1.0f - hz_min / f->vsadx);
+    const float cmax_x = std::max(1.0f - hz_max / f->vsadx, 1.0f - hz_min / f->vsadx);
+    const float cmin_y = std::min(1.0f - hz_max / f->vsady, 1.0f - hz_min / f->vsady);
+    const float cmax_y = std::max(1.0f - hz_max / f->vsady, 1.0f - hz_min / f->vsady);
+    std::vector<dh_infl_block> out(f->block_layer.size());
+    for (size_t b = 0; b < out.size(); b++) {
+        dh_infl_block &B = out[b];
+        B.layer = f->block_layer[b]; B.first = (int)b; B.n = f->block_n[b]; B.pad_ = 0;
+        float x0 = 1e30f, x1 = -1e30f, y0 = 1e30f, y1 = -1e30f;
+        for (int s = 0; s < f->S; s++) {
+            const int id = f->block_spots[b * f->S + s];
+            if (id < 0) continue;
+            const float sx = f->spots[8 * id], sy = f->spots[8 * id + 1];
+            const float R = r[id] * 1.002f + 0.5f * g.s;
+            x0 = std::min(x0, std::min(sx * cmin_x, sx * cmax_x) - R);
+            x1 = std::max(x1, std::max(sx * cmin_x, sx * cmax_x) + R);
+            y0 = std::min(y0, std::min(sy * cmin_y, sy * cmax_y) - R);
+            y1 = std::max(y1, std::max(sy * cmin_y, sy * cmax_y) + R);
+        }
+        B.bx0 = x0; B.bx1 = x1; B.by0 = y0; B.by1 = y1;
+        // image-space bounding rows of the prism box x [hz_min, hz_max]
+        float kmin = 1e30f, kmax = -1e30f, jmin = 1e30f, jmax = -1e30f;
+        for (int c = 0; c < 8; c++) {
+            float px, py, pz;
+            dh_head_to_img(g, (c & 1) ? x1 : x0, (c & 2) ? y1 : y0, (c & 4) ? hz_max : hz_min, px, py, pz);
+            const float fj = (py + g.iso[1]) / g.s, fk = (pz + g.iso[2]) / g.s;
+            jmin = std::min(jmin, fj); jmax = std::max(jmax, fj);
+            kmin = std::min(kmin, fk); kmax = std::max(kmax, fk);
+        }
+        B.j0 = std::max(0, (int)floorf(jmin) - 1); B.j1 = std::min(g.ny - 1, (int)ceilf(jmax) + 1);
+        B.k0 = std::max(0, (int)floorf(kmin) - 1); B.k1 = std::min(g.nz - 1, (int)ceilf(kmax) + 1);
+        if (B.j0 > B.j1 || B.k0 > B.k1) { B.j0 = 0; B.j1 = -1; B.k0 = 0; B.k1 = 0; }
+    }
+    // the persistent queue hands out the most expensive blocks first
+    std::stable_sort(out.begin(), out.end(), [](const dh_infl_block &p, const dh_infl_block &q) {
+        const int64_t cp = (int64_t)(p.j1 - p.j0 + 1) * (p.k1 - p.k0 + 1) * (int64_t)(p.bx1 - p.bx0 + p.by1 - p.by0);
+        const int64_t cq = (int64_t)(q.j1 - q.j0 + 1) * (q.k1 - q.k0 + 1) * (int64_t)(q.bx1 - q.bx0 + q.by1 - q.by0);
+        return cp > cq;
+    });
+    return out;
+}
+
+template <int MODE>
+static void launch(dh_influence *f, const std::vector<dh_infl_block> &blocks, const float *d_wet,
+                   uint32_t *d_col, float *d_val, hipStream_t s)
+{
+    DH_HIP(hipMemcpyAsync(f->d_blocks, blocks.data(), sizeof(dh_infl_block) * blocks.size(),
+                          hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, (unsigned *)f->d_queue, (size_t)1);
+    dh_infl_args a;
+    std::memset(&a, 0, sizeof(a));
+    a.g = f->g; a.vsadx = f->vsadx; a.vsady = f->vsady;
+    a.n_items = (int)blocks.size() * f->C; a.C = f->C;
+    a.blocks = f->d_blocks; a.block_spots = f->d_block_spots; a.layers = f->d_layers;
+    a.luts = f->d_luts; a.spots = f->d_spots; a.thr = f->d_thr; a.wet = d_wet;
+    a.peak_bits = f->d_peak; a.counts = f->d_counts; a.offsets = f->d_offsets;
+    a.col = d_col; a.val = d_val; a.queue = f->d_queue;
+    if (a.n_items == 0) return;
+    int cus = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, f->dev) == hipSuccess) cus = prop.multiProcessorCount;
+    const unsigned grid = (unsigned)std::min(a.n_items, cus * 16);
+    const char *name = MODE == INF_PEAK ? "influence_peak" : MODE == INF_COUNT ? "influence_count" : "influence_fill";
+    dh_timed t(name, s);
+    if (f->S == 8) hipLaunchKernelGGL((k_influence<8, MODE>), dim3(grid), dim3(64), 0, s, a);
+    else hipLaunchKernelGGL((k_influence<16, MODE>), dim3(grid), dim3(64), 0, s, a);
+    DH_HIP(hipGetLastError());
+}
+
+// PEAK + COUNT; returns nnz and fills the host offsets (exclusive scan over
+// (spot, chunk) in row-major order, starting at base).
+int64_t dh_influence_count(dh_influence *f, const float *d_wet, int64_t base, hipStream_t s)
+{
+    const int n = f->n_spots;
+    hipLaunchKernelGGL(k_zero_words, dim3(64), dim3(256), 0, s, f->d_peak, (size_t)std::max(n, 1));
+    // peak pass radius: 3 sigma_max of the layer
+    std::vector<float> r(n, 0.0f);
+    for (int q = 0; q < n; q++) {
+        const int l = f->spot_layer[q];
+        r[q] = l >= 0 ? 3.0f * f->sig_max[l] + f->g.s : 0.0f;
+    }
+    launch<INF_PEAK>(f, make_blocks(f, r), d_wet, nullptr, nullptr, s);
+    std::vector<unsigned> pk(std::max(n, 1));
+    DH_HIP(hipMemcpyAsync(pk.data(), f->d_peak, sizeof(unsigned) * n, hipMemcpyDeviceToHost, s));
+    DH_HIP(hipStreamSynchronize(s));
+    f->peak.assign(n, 0.0f);
+    std::vector<float> thr(n, INFINITY);
+    for (int q = 0; q < n; q++) {
+        float v;
+        std::memcpy(&v, &pk[q], 4);
+        f->peak[q] = v;
+        const int l = f->spot_layer[q];
+        if (l < 0 || !(v > 0.0f)) { r[q] = 0.0f; continue; }
+        thr[q] = f->rel * v;
+        // beyond R the dose bound dose_max * exp(-R^2 / 2 sig^2) is below thr
+        const float ratio = std::max(f->dose_max[l] / thr[q], 1.0f);
+        r[q] = f->sig_max[l] * std::sqrt(2.0f * std::log(ratio)) + f->g.s;
+    }
+    DH_HIP(hipMemcpyAsync(f->d_thr, thr.data(), sizeof(float) * n, hipMemcpyHostToDevice, s));
+    launch<INF_COUNT>(f, make_blocks(f, r), d_wet, nullptr, nullptr, s);
+    std::vector<int> cnt((size_t)n * f->C);
+    if (n) DH_HIP(hipMemcpyAsync(cnt.data(), f->d_counts, sizeof(int) * cnt.size(), hipMemcpyDeviceToHost, s));
+    DH_HIP(hipStreamSynchronize(s));
+    f->offsets.assign(cnt.size() + 1, 0);
+    int64_t acc = base;
+    for (int q = 0; q < n; q++) {
+        const int l = f->spot_layer[q];
+        for (int c = 0; c < f->C; c++) {
+            f->offsets[(size_t)q * f->C + c] = acc;
+            if (l >= 0) acc += cnt[(size_t)q * f->C + c];
+        }
+    }
+    f->offsets[cnt.size()] = acc;
+    f->nnz = acc - base;
+    f->radius = r;
+    return f->nnz;
+}
+
+void dh_influence_fill(dh_influence *f, const float *d_wet, uint32_t *d_col, float *d_val, hipStream_t s)
+{
+    DH_REQUIRE(f->nnz >= 0, "dh_influence_count must run before dh_influence_fill");
+    const size_t n = f->offsets.size();
+    if (n) DH_HIP(hipMemcpyAsync(f->d_offsets, f->offsets.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, s));
+    // the same boxes as the COUNT pass, so every entry lands in its slot
+    launch<INF_FILL>(f, make_blocks(f, f->radius), d_wet, d_col, d_val, s);
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" int dh_influence_create(int gpu_id, const dh_grid *grid, const dh_proton_beam *beam,
+                                   const dh_proton_lut *lut, const float *spot_rows, int32_t n_spots,
+                                   int32_t spots_per_block, int32_t row_chunks, float rel_threshold,
+                                   dh_influence **out)
+{
+    return dh_guarded([&] {
+        DH_REQUIRE(out && grid && beam && lut, "null argument");
+        DH_REQUIRE(rel_threshold > 0.0f && rel_threshold < 1.0f, "threshold must be in (0, 1)");
+        DH_REQUIRE(grid->nx > 0 && grid->ny > 0 && grid->nz > 0 && grid->spacing > 0.0f, "bad grid");
+        DH_REQUIRE((int64_t)grid->nx * grid->ny * grid->nz < (int64_t)UINT32_MAX, "grid too large for u32 columns");
+        dh_device_enter(gpu_id);
+        dh_proton_tables t = dh_build_proton_tables(beam, lut, spot_rows, n_spots);
+        dh_geom g = dh_make_geom(*grid, beam->iso, beam->gantry_angle, beam->couch_angle, 0.0,
+                                 ((float)beam->vsadx + (float)beam->vsady) / 2.0f);
+        g.sinca = 0.0f; g.cosca = 1.0f;
+        dh_influence *f = dh_influence_make(gpu_id, g, (float)beam->vsadx, (float)beam->vsady, t.layers,
+                                            t.luts, t.spots, n_spots, spots_per_block, row_chunks,
+                                            rel_threshold);
+        f->order = t.order;
+        *out = f;
+    });
+}
+
+extern "C" int dh_influence_count(dh_influence *f, const float *d_wet, int64_t base, int64_t *nnz,
+                                  int64_t *row_ptr, void *stream)
+{
+    return dh_guarded([&] {
+        DH_REQUIRE(f && d_wet, "null argument");
+        dh_device_enter(f->dev);
+        const int64_t z = dh_influence_count(f, d_wet, base, (hipStream_t)stream);
+        if (nnz) *nnz = z;
+        if (row_ptr) {
+            for (int q = 0; q < f->n_spots; q++) row_ptr[q] = f->offsets[(size_t)q * f->C];
+            row_ptr[f->n_spots] = base + z;
+        }
+    });
+}
+
+extern "C" int dh_influence_fill(dh_influence *f, const float *d_wet, uint32_t *d_col, float *d_val,
+                                 void *stream)
+{
+    return dh_guarded([&] {
+        DH_REQUIRE(f && d_wet && (f->nnz == 0 || (d_col && d_val)), "null argument");
+        dh_device_enter(f->dev);
+        dh_influence_fill(f, d_wet, d_col, d_val, (hipStream_t)stream);
+    });
+}
+
+extern "C" int dh_influence_info(const dh_influence *f, int32_t *n_rows, int32_t *row_spot,
+                                 float *row_peak)
+{
+    return dh_guarded([&] {
+        DH_REQUIRE(f, "null handle");
+        if (n_rows) *n_rows = f->n_spots;
+        for (int q = 0; q < f->n_spots; q++) {
+            if (row_spot) row_spot[q] = f->order[q];
+            if (row_peak) row_peak[q] = q < (int)f->peak.size() ? f->peak[q] : 0.0f;
+        }
+    });
+}
+
+extern "C" int dh_influence_destroy(dh_influence *f)
+{
+    return dh_guarded([&] { dh_influence_free(f); });
+}

@@ -93,6 +93,72 @@ class ProtonPlan:
             pass
 
 
+class InfluenceMatrix:
+    """CSR dose-influence matrix of one proton field (rows = spots at MU 1,
+    columns = voxel indices, values >= rel * row peak), built on the device.
+
+    count() synchronises once to size the arrays; fill() is stream-ordered.
+    Several fields can share one CSR by passing the running row/entry offsets
+    (`base`) and the shared col/val tensors."""
+
+    def __init__(self, device: int, shape, spacing, origin, iso, gantry, couch, model, spot_rows,
+                 spots_per_block=16, row_chunks=8, rel=1e-4):
+        self._keep: list = []
+        lut = _lib.proton_lut(model.divergence_params, model.lut_depths, model.lut_sigmas,
+                              model.lut_idds, self._keep)
+        rows = np.ascontiguousarray(np.asarray(spot_rows, dtype=np.float32).reshape(-1, 4))
+        self.n_rows = rows.shape[0]
+        self._h = C.c_void_p()
+        L = _lib.load()
+        _lib.check(L.dh_influence_create(device, _g(shape, spacing, origin),
+                                         _lib.proton_beam(iso, gantry, couch, model.VSADX,
+                                                          model.VSADY),
+                                         lut, _lib.ptr(rows), rows.shape[0], int(spots_per_block),
+                                         int(row_chunks), float(rel), C.byref(self._h)))
+        self.nnz = None
+        self.row_ptr = None
+
+    def count(self, wet: torch.Tensor, base: int = 0, stream=None) -> int:
+        nnz = C.c_int64()
+        rp = np.zeros(self.n_rows + 1, np.int64)
+        _lib.check(_lib.load().dh_influence_count(self._h, _lib.ptr(wet), int(base), C.byref(nnz),
+                                                  _lib.ptr(rp), _stream(stream)))
+        self.nnz = int(nnz.value)
+        self.row_ptr = rp
+        return self.nnz
+
+    def fill(self, wet: torch.Tensor, col: torch.Tensor, val: torch.Tensor, stream=None):
+        _lib.check(_lib.load().dh_influence_fill(self._h, _lib.ptr(wet), _lib.ptr(col),
+                                                 _lib.ptr(val), _stream(stream)))
+
+    def info(self):
+        """(caller's spot row of each matrix row, row peak)."""
+        n = C.c_int32()
+        spot = np.zeros(max(self.n_rows, 1), np.int32)
+        peak = np.zeros(max(self.n_rows, 1), np.float32)
+        _lib.check(_lib.load().dh_influence_info(self._h, C.byref(n), _lib.ptr(spot), _lib.ptr(peak)))
+        return spot[:self.n_rows], peak[:self.n_rows]
+
+    def build(self, wet: torch.Tensor, stream=None):
+        """count + fill into fresh tensors; returns (row_ptr int64, col int32, val float32)."""
+        nnz = self.count(wet, 0, stream)
+        col = torch.empty(max(nnz, 1), dtype=torch.int32, device=wet.device)
+        val = torch.empty(max(nnz, 1), dtype=torch.float32, device=wet.device)
+        self.fill(wet, col, val, stream)
+        return torch.from_numpy(self.row_ptr).to(wet.device), col[:nnz], val[:nnz]
+
+    def close(self):
+        if self._h:
+            _lib.load().dh_influence_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def photon_args(model, keep):
     vals = {k: getattr(model, k) for k in (
         "mu_calibration", "primary_source_distance", "scatter_source_distance",

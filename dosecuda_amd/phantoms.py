@@ -120,3 +120,28 @@ def to_density(case: Case):
 
 def default_photon_field(model: PhotonBeamModel, size_cm=10.0):
     return square_field_mlc(model, size_cm, size_cm)
+
+
+def influence_fields(model: ProtonBeamModel, seed=1003, gantries=(0.0, 120.0, 240.0),
+                     layers_per_field=(8, 8, 9), id_max=85, n_min=200, n_max=900, total=15000,
+                     half_width=60.0):
+    """Spot lists of the batched influence workload: 25 layers split 8/8/9
+    over gantry 0/120/240, energy ids from 0..id_max (70.2-197.6 MeV), spots
+    per layer U{200..900} rescaled to about `total`, positions uniform in
+    +-half_width mm (BEV), MU 1.  Returns [(gantry, spot_rows)]."""
+    rng = np.random.default_rng(seed)
+    n_layers = int(sum(layers_per_field))
+    ids = np.sort(rng.choice(np.arange(id_max + 1), size=n_layers, replace=False))
+    counts = rng.integers(n_min, n_max + 1, n_layers).astype(np.float64)
+    counts = np.maximum(1, np.round(counts * total / counts.sum())).astype(int)
+    order = rng.permutation(n_layers)
+    out, pos = [], 0
+    for g, nl in zip(gantries, layers_per_field):
+        rows = []
+        for li in order[pos:pos + nl]:
+            n = int(counts[li])
+            xy = rng.uniform(-half_width, half_width, (n, 2))
+            rows.append(np.column_stack([xy, np.ones(n), np.full(n, ids[li])]))
+        pos += nl
+        out.append((float(g), np.array(np.vstack(rows), dtype=np.single)))
+    return out

@@ -138,6 +138,28 @@ int dh_photon_dose_dev(int gpu_id, const dh_grid *grid, const dh_photon_cp *cp,
                        const float *d_density, float *d_dose_out, float *d_terma_out,
                        float *d_electron_out, float *d_wet_out, void *stream);
 
+/* ---- batched proton dose-influence matrix (CSR) ---------------------- */
+/* No reference counterpart: the reference only sums weighted spots
+ * (IMPTClasses.cu:204-271).  Row q of the matrix is sorted spot q (energy-id
+ * order, stable; dh_influence_info gives the caller's spot row) with MU = 1;
+ * columns are voxel indices i + nx*(j + ny*k) in ascending order; values are
+ * the pencil-beam dose of that spot, kept where >= rel_threshold * row peak.
+ * spots_per_block (8 or 16) and row_chunks (1..256) are tuning knobs.
+ * Usage: create -> count (synchronises; gives nnz and row_ptr, whose entries
+ * start at `base` so fields can share one CSR) -> caller allocates col/val ->
+ * fill (stream-ordered) -> destroy. */
+typedef struct dh_influence dh_influence;
+int dh_influence_create(int gpu_id, const dh_grid *grid, const dh_proton_beam *beam,
+                        const dh_proton_lut *lut, const float *spot_rows, int32_t n_spots,
+                        int32_t spots_per_block, int32_t row_chunks, float rel_threshold,
+                        dh_influence **out);
+int dh_influence_count(dh_influence *f, const float *d_wet, int64_t base, int64_t *nnz,
+                       int64_t *row_ptr, void *stream);
+int dh_influence_fill(dh_influence *f, const float *d_wet, uint32_t *d_col, float *d_val,
+                      void *stream);
+int dh_influence_info(const dh_influence *f, int32_t *n_rows, int32_t *row_spot, float *row_peak);
+int dh_influence_destroy(dh_influence *f);
+
 /* ---- single stages on device pointers (stage-wise parity) ------------ */
 /* Unsmoothed WET of a volume sampled with the given border value, source
  * at src_dist from iso (CudaClasses.cu:180-228). */
