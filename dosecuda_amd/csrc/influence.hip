@@ -47,6 +47,7 @@ struct dh_infl_args {
     unsigned *peak_bits;         // per spot (PEAK)
     int *counts;                 // per spot * C (COUNT)
     const int64_t *offsets;      // per spot * C (FILL)
+    const int2 *active;          // [layer][nz][ny] first/last i with w <= cut (lo > hi: none)
     uint32_t *col;
     float *val;
     int *queue;
@@ -131,14 +132,32 @@ k_influence(dh_infl_args a)
             dh_img_to_head(g, g.s, 0.0f, 0.0f, x1, y1, z1);
             dhx = x1 - x0; dhy = y1 - y0; dhz = z1 - z0;
         }
-        for (int64_t rr = r0; rr < r1; rr++) {
-            const int k = B.k0 + (int)(rr / nj), j = B.j0 + (int)(rr % nj);
-            float x, y, z, hx0, hy0, hz0;
-            dh_vox_xyz(g, 0, j, k, x, y, z);
-            dh_img_to_head(g, x, y, z, hx0, hy0, hz0);
-            int ilo = 0, ihi = g.nx - 1;
-            interval(hx0, dhx, B.bx0, B.bx1, g.nx, ilo, ihi);
-            interval(hy0, dhy, B.by0, B.by1, g.nx, ilo, ihi);
+        const int2 *act_rows = a.active + (size_t)B.layer * g.nz * g.ny;
+        for (int64_t rb = r0; rb < r1; rb += 64) {
+            // lane r prepares row rb + r: box interval intersected with the
+            // layer's active interval on that row
+            int my_lo = 1, my_hi = 0;
+            const int64_t rr = rb + lane;
+            if (rr < r1) {
+                const int k = B.k0 + (int)(rr / nj), j = B.j0 + (int)(rr % nj);
+                const int2 ar = act_rows[(size_t)k * g.ny + j];
+                if (ar.x <= ar.y) {
+                    float x, y, z, hx0, hy0, hz0;
+                    dh_vox_xyz(g, 0, j, k, x, y, z);
+                    dh_img_to_head(g, x, y, z, hx0, hy0, hz0);
+                    int ilo = ar.x, ihi = ar.y;
+                    interval(hx0, dhx, B.bx0, B.bx1, g.nx, ilo, ihi);
+                    interval(hy0, dhy, B.by0, B.by1, g.nx, ilo, ihi);
+                    my_lo = ilo; my_hi = ihi;
+                }
+            }
+            uint64_t rows_left = __builtin_amdgcn_ballot_w64(my_lo <= my_hi);
+            while (rows_left) {
+                const int r = __ffsll((unsigned long long)rows_left) - 1;
+                rows_left &= rows_left - 1;
+                const int ilo = __shfl(my_lo, r, 64), ihi = __shfl(my_hi, r, 64);
+                const int64_t rsel = rb + r;
+                const int k = B.k0 + (int)(rsel / nj), j = B.j0 + (int)(rsel % nj);
             for (int i0 = ilo; i0 <= ihi; i0 += 64) {
                 const int i = i0 + lane;
                 const bool inb = i <= ihi;
@@ -210,6 +229,7 @@ k_influence(dh_infl_args a)
                     }
                 }
             }
+            }
         }
 #pragma unroll
         for (int s = 0; s < S; s++) {
@@ -222,6 +242,32 @@ k_influence(dh_infl_args a)
                 if (lane == 0) a.counts[(size_t)sid[s] * a.C + chunk] = cnt[s];
             }
         }
+    }
+}
+
+// Per layer and image row (k, j): first and last i with 10 * WET <= cut.
+// One wave per row; the ballot of each 64-voxel chunk gives the extremes.
+__global__ void __launch_bounds__(256)
+k_active_rows(int nx, int ny, int nz, int n_layers, const dh_layer_k *__restrict__ layers,
+              const float *__restrict__ wet, int2 *__restrict__ out)
+{
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = __lane_id();
+    if (row >= ny * nz) return;
+    const float *w = wet + (size_t)row * nx;
+    for (int l = 0; l < n_layers; l++) {
+        const float cut = layers[l].cut;
+        int lo = nx, hi = -1;
+        for (int i0 = 0; i0 < nx; i0 += 64) {
+            const int i = i0 + lane;
+            const bool act = i < nx && !(w[min(i, nx - 1)] * 10.0f > cut);
+            const uint64_t m = __builtin_amdgcn_ballot_w64(act);
+            if (m) {
+                lo = min(lo, i0 + __ffsll((unsigned long long)m) - 1);
+                hi = max(hi, i0 + 63 - __clzll((unsigned long long)m));
+            }
+        }
+        if (lane == 0) out[(size_t)l * nz * ny + row] = make_int2(lo, hi);
     }
 }
 
@@ -273,6 +319,7 @@ struct dh_influence {
     int *d_counts = nullptr;
     int64_t *d_offsets = nullptr;
     int *d_queue = nullptr;
+    int2 *d_active = nullptr;
     std::vector<float> peak;         // host copy after the peak pass
     std::vector<float> radius;       // truncation radius per spot (COUNT = FILL)
     std::vector<int> order;          // caller's spot row of each matrix row
@@ -391,6 +438,7 @@ dh_influence *dh_influence_make(int dev, const dh_geom &g, float vsadx, float vs
         DH_HIP(hipMalloc(&f->d_counts, sizeof(int) * std::max<size_t>((size_t)n_spots * C, 1)));
         DH_HIP(hipMalloc(&f->d_offsets, sizeof(int64_t) * ((size_t)n_spots * C + 1)));
         DH_HIP(hipMalloc(&f->d_queue, 256));
+        DH_HIP(hipMalloc(&f->d_active, sizeof(int2) * std::max<size_t>(layers.size(), 1) * g.ny * g.nz));
     } catch (...) {
         void dh_influence_free(dh_influence *);
         dh_influence_free(f);
@@ -404,7 +452,7 @@ void dh_influence_free(dh_influence *f)
     if (!f) return;
     (void)hipSetDevice(f->dev);
     void *ps[] = {f->d_layers, f->d_luts, f->d_spots, f->d_block_spots, f->d_blocks, f->d_thr,
-                  f->d_peak, f->d_counts, f->d_offsets, f->d_queue};
+                  f->d_peak, f->d_counts, f->d_offsets, f->d_queue, f->d_active};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     delete f;
@@ -472,12 +520,13 @@ static void launch(dh_influence *f, const std::vector<dh_infl_block> &blocks, co
     a.blocks = f->d_blocks; a.block_spots = f->d_block_spots; a.layers = f->d_layers;
     a.luts = f->d_luts; a.spots = f->d_spots; a.thr = f->d_thr; a.wet = d_wet;
     a.peak_bits = f->d_peak; a.counts = f->d_counts; a.offsets = f->d_offsets;
+    a.active = f->d_active;
     a.col = d_col; a.val = d_val; a.queue = f->d_queue;
     if (a.n_items == 0) return;
     int cus = 256;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, f->dev) == hipSuccess) cus = prop.multiProcessorCount;
-    const unsigned grid = (unsigned)std::min(a.n_items, cus * 16);
+    const unsigned grid = (unsigned)std::min(a.n_items, cus * 32);
     const char *name = MODE == INF_PEAK ? "influence_peak" : MODE == INF_COUNT ? "influence_count" : "influence_fill";
     dh_timed t(name, s);
     if (f->S == 8) hipLaunchKernelGGL((k_influence<8, MODE>), dim3(grid), dim3(64), 0, s, a);
@@ -487,9 +536,20 @@ static void launch(dh_influence *f, const std::vector<dh_infl_block> &blocks, co
 
 // PEAK + COUNT; returns nnz and fills the host offsets (exclusive scan over
 // (spot, chunk) in row-major order, starting at base).
+static void active_rows(dh_influence *f, const float *d_wet, hipStream_t s)
+{
+    const int nrow = f->g.ny * f->g.nz;
+    if (f->layers.empty()) return;
+    dh_timed t("influence_rows", s);
+    hipLaunchKernelGGL(k_active_rows, dim3((nrow + 3) / 4), dim3(256), 0, s, f->g.nx, f->g.ny, f->g.nz,
+                       (int)f->layers.size(), f->d_layers, d_wet, f->d_active);
+    DH_HIP(hipGetLastError());
+}
+
 int64_t dh_influence_count(dh_influence *f, const float *d_wet, int64_t base, hipStream_t s)
 {
     const int n = f->n_spots;
+    active_rows(f, d_wet, s);
     hipLaunchKernelGGL(k_zero_words, dim3(64), dim3(256), 0, s, f->d_peak, (size_t)std::max(n, 1));
     // peak pass radius: 3 sigma_max of the layer
     std::vector<float> r(n, 0.0f);
@@ -539,7 +599,9 @@ void dh_influence_fill(dh_influence *f, const float *d_wet, uint32_t *d_col, flo
     DH_REQUIRE(f->nnz >= 0, "dh_influence_count must run before dh_influence_fill");
     const size_t n = f->offsets.size();
     if (n) DH_HIP(hipMemcpyAsync(f->d_offsets, f->offsets.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, s));
-    // the same boxes as the COUNT pass, so every entry lands in its slot
+    // the same boxes and active rows as the COUNT pass, so every entry lands
+    // in its slot
+    active_rows(f, d_wet, s);
     launch<INF_FILL>(f, make_blocks(f, f->radius), d_wet, d_col, d_val, s);
 }
 

@@ -102,7 +102,7 @@ class InfluenceMatrix:
     (`base`) and the shared col/val tensors."""
 
     def __init__(self, device: int, shape, spacing, origin, iso, gantry, couch, model, spot_rows,
-                 spots_per_block=16, row_chunks=8, rel=1e-4):
+                 spots_per_block=16, row_chunks=16, rel=1e-4):
         self._keep: list = []
         lut = _lib.proton_lut(model.divergence_params, model.lut_depths, model.lut_sigmas,
                               model.lut_idds, self._keep)
