@@ -309,6 +309,12 @@ def main():
                                 kernels_ms={k: v[0] / max(v[1], 1) for k, v in kt.items()},
                                 roofline=dict(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS,
                                               unit="GB/s", frac=ach / HBM_PEAK_GBS, traffic=None))
+        sweep = []
+        for f in st["infl"]:
+            c, rad = f.stats()
+            sweep.append(dict(c, mean_radius_mm=float(rad.mean())))
+        res["influence"]["sweep"] = sweep
+        log(f"influence sweep counters: {sweep}")
         log(f"influence: {ms:.2f} ms/build, nnz {st['nnz']}, fill {fill_ms:.3f} ms/field "
             f"({ach:.0f} GB/s = {100 * ach / HBM_PEAK_GBS:.1f}% of HBM)")
         for f in st["infl"]:

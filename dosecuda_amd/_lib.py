@@ -96,6 +96,7 @@ SIGNATURES = {
     "dh_influence_fill": [_P, _P, _P, _P, _P],
     "dh_influence_info": [_P, C.POINTER(C.c_int32), _P, _P],
     "dh_influence_destroy": [_P],
+    "dh_influence_stats": [_P, _P, _P],
     "dh_prof_enable": [_I],
     "dh_prof_reset": [],
     "dh_prof_get": [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)],

@@ -51,6 +51,7 @@ struct dh_infl_args {
     uint32_t *col;
     float *val;
     int *queue;
+    unsigned long long *stats;   // optional: lane visits, active visits, chunks, rows
 };
 
 namespace {
@@ -133,6 +134,7 @@ k_influence(dh_infl_args a)
             dhx = x1 - x0; dhy = y1 - y0; dhz = z1 - z0;
         }
         const int2 *act_rows = a.active + (size_t)B.layer * g.nz * g.ny;
+        unsigned long long st_visit = 0, st_act = 0, st_chunk = 0, st_rows = 0;
         for (int64_t rb = r0; rb < r1; rb += 64) {
             // lane r prepares row rb + r: box interval intersected with the
             // layer's active interval on that row
@@ -158,12 +160,16 @@ k_influence(dh_infl_args a)
                 const int ilo = __shfl(my_lo, r, 64), ihi = __shfl(my_hi, r, 64);
                 const int64_t rsel = rb + r;
                 const int k = B.k0 + (int)(rsel / nj), j = B.j0 + (int)(rsel % nj);
+                st_rows++;
             for (int i0 = ilo; i0 <= ihi; i0 += 64) {
                 const int i = i0 + lane;
                 const bool inb = i <= ihi;
                 const uint32_t v = (uint32_t)min(i, ihi) + (uint32_t)g.nx * ((uint32_t)j + (uint32_t)g.ny * (uint32_t)k);
                 const float w = a.wet[v] * 10.0f;
                 const bool act = inb && !(w > L.cut);
+                st_chunk++;
+                st_visit += __popcll(__builtin_amdgcn_ballot_w64(inb));
+                st_act += __popcll(__builtin_amdgcn_ballot_w64(act));
                 if (!dh_wave_any(act)) continue;
                 float px, py, pz, hx, hy, hz;
                 dh_vox_xyz(g, min(i, ihi), j, k, px, py, pz);
@@ -230,6 +236,12 @@ k_influence(dh_infl_args a)
                 }
             }
             }
+        }
+        if (a.stats && lane == 0) {
+            atomicAdd(&a.stats[0], st_visit);
+            atomicAdd(&a.stats[1], st_act);
+            atomicAdd(&a.stats[2], st_chunk);
+            atomicAdd(&a.stats[3], st_rows);
         }
 #pragma unroll
         for (int s = 0; s < S; s++) {
@@ -320,6 +332,7 @@ struct dh_influence {
     int64_t *d_offsets = nullptr;
     int *d_queue = nullptr;
     int2 *d_active = nullptr;
+    unsigned long long *d_stats = nullptr;   // 4 counters per pass
     std::vector<float> peak;         // host copy after the peak pass
     std::vector<float> radius;       // truncation radius per spot (COUNT = FILL)
     std::vector<int> order;          // caller's spot row of each matrix row
@@ -438,6 +451,7 @@ dh_influence *dh_influence_make(int dev, const dh_geom &g, float vsadx, float vs
         DH_HIP(hipMalloc(&f->d_counts, sizeof(int) * std::max<size_t>((size_t)n_spots * C, 1)));
         DH_HIP(hipMalloc(&f->d_offsets, sizeof(int64_t) * ((size_t)n_spots * C + 1)));
         DH_HIP(hipMalloc(&f->d_queue, 256));
+        DH_HIP(hipMalloc(&f->d_stats, sizeof(unsigned long long) * 12));
         DH_HIP(hipMalloc(&f->d_active, sizeof(int2) * std::max<size_t>(layers.size(), 1) * g.ny * g.nz));
     } catch (...) {
         void dh_influence_free(dh_influence *);
@@ -452,7 +466,7 @@ void dh_influence_free(dh_influence *f)
     if (!f) return;
     (void)hipSetDevice(f->dev);
     void *ps[] = {f->d_layers, f->d_luts, f->d_spots, f->d_block_spots, f->d_blocks, f->d_thr,
-                  f->d_peak, f->d_counts, f->d_offsets, f->d_queue, f->d_active};
+                  f->d_peak, f->d_counts, f->d_offsets, f->d_queue, f->d_active, f->d_stats};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     delete f;
@@ -521,6 +535,8 @@ static void launch(dh_influence *f, const std::vector<dh_infl_block> &blocks, co
     a.luts = f->d_luts; a.spots = f->d_spots; a.thr = f->d_thr; a.wet = d_wet;
     a.peak_bits = f->d_peak; a.counts = f->d_counts; a.offsets = f->d_offsets;
     a.active = f->d_active;
+    a.stats = f->d_stats + 4 * MODE;
+    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, (unsigned *)a.stats, (size_t)8);
     a.col = d_col; a.val = d_val; a.queue = f->d_queue;
     if (a.n_items == 0) return;
     int cus = 256;
@@ -666,6 +682,20 @@ extern "C" int dh_influence_info(const dh_influence *f, int32_t *n_rows, int32_t
             if (row_spot) row_spot[q] = f->order[q];
             if (row_peak) row_peak[q] = q < (int)f->peak.size() ? f->peak[q] : 0.0f;
         }
+    });
+}
+
+extern "C" int dh_influence_stats(const dh_influence *f, uint64_t *stats12, float *row_radius)
+{
+    return dh_guarded([&] {
+        DH_REQUIRE(f, "null handle");
+        if (stats12) {
+            DH_HIP(hipSetDevice(f->dev));
+            DH_HIP(hipMemcpy(stats12, f->d_stats, sizeof(uint64_t) * 12, hipMemcpyDeviceToHost));
+        }
+        if (row_radius)
+            for (int q = 0; q < f->n_spots; q++)
+                row_radius[q] = q < (int)f->radius.size() ? f->radius[q] : 0.0f;
     });
 }
 

@@ -139,6 +139,15 @@ class InfluenceMatrix:
         _lib.check(_lib.load().dh_influence_info(self._h, C.byref(n), _lib.ptr(spot), _lib.ptr(peak)))
         return spot[:self.n_rows], peak[:self.n_rows]
 
+    def stats(self):
+        """Sweep counters {pass: (lane visits, in-range visits, chunks, rows)}
+        of the last passes and the per-row truncation radius (mm)."""
+        st = np.zeros(12, np.uint64)
+        rad = np.zeros(max(self.n_rows, 1), np.float32)
+        _lib.check(_lib.load().dh_influence_stats(self._h, _lib.ptr(st), _lib.ptr(rad)))
+        return ({"peak": tuple(int(x) for x in st[0:4]), "count": tuple(int(x) for x in st[4:8]),
+                 "fill": tuple(int(x) for x in st[8:12])}, rad[:self.n_rows])
+
     def build(self, wet: torch.Tensor, stream=None):
         """count + fill into fresh tensors; returns (row_ptr int64, col int32, val float32)."""
         nnz = self.count(wet, 0, stream)

@@ -158,6 +158,9 @@ int dh_influence_count(dh_influence *f, const float *d_wet, int64_t base, int64_
 int dh_influence_fill(dh_influence *f, const float *d_wet, uint32_t *d_col, float *d_val,
                       void *stream);
 int dh_influence_info(const dh_influence *f, int32_t *n_rows, int32_t *row_spot, float *row_peak);
+/* Sweep counters of the last peak/count/fill passes (4 each: lane visits,
+ * in-range visits, 64-voxel chunks, rows) and each row's truncation radius. */
+int dh_influence_stats(const dh_influence *f, uint64_t *stats12, float *row_radius);
 int dh_influence_destroy(dh_influence *f);
 
 /* ---- single stages on device pointers (stage-wise parity) ------------ */
