@@ -165,6 +165,54 @@ __device__ static inline float dh_trilinear_u(const float *__restrict__ vol, con
     return fmaf(az, b1 - b0, b0);
 }
 
+// Per (voxel, layer) factors of the double-Gaussian pencil beam
+// (IMPTClasses.cu:69-121,222-264): LUT lookup at WET w (mm) in a 400-entry
+// {depth, sigma, idd, -} table, sigma_air at source distance dist, nuclear
+// halo, and the two Gaussians' normalisations P, H and exponent scales
+// ap, ah (already multiplied by log2 e, so exp(a d^2) = 2^(ap d^2)).
+struct dh_pb_factors {
+    float P, H, ap, ah, st, sht;
+};
+
+__device__ static inline dh_pb_factors dh_pencil_factors(const float4 *lut, float w, float dist,
+                                                         float r80, float A, float B, float C)
+{
+    int lo = 0, hi = 400;
+    while (lo < hi) {
+        int m = (lo + hi) >> 1;
+        if (w <= lut[m].x) hi = m; else lo = m + 1;
+    }
+    float idd, sms;
+    if (lo == 400) { idd = lut[lo - 1].z; sms = lut[lo - 1].y; }
+    else if (lo == 0) { idd = lut[0].z; sms = lut[0].y; }
+    else {
+        const float4 a = lut[lo - 1], b = lut[lo];
+        const float f = (w - a.x) / (b.x - a.x);
+        idd = fmaf(b.z - a.z, f, a.z);
+        sms = fmaf(b.y - a.y, f, a.y);
+    }
+    const float da = dist - w + 0.7f * r80;                      // sigmaAir
+    const float st = fmaf(fmaf(A, da, B), da, C) + sms;
+    const float wc = fminf(fmaxf(w, 0.1f), r80 - 0.1f);         // nuclearHalo
+    const float t5 = wc - r80 - 5.0f;
+    float sh = 2.85f + (0.0014f * r80 * logf(wc + 3.0f)) + (0.06f * wc) - (7.4e-5f * (wc * wc))
+             - ((0.22f * r80) / (t5 * t5));
+    const float r3 = r80 + 3.0f;
+    float hw = 0.052f * logf(1.13f + (wc / (11.2f - (0.023f * r80))))
+             + (0.35f * ((0.0017f * (r80 * r80)) - r80) / ((r3 * r3) - (wc * wc)))
+             - (1.61e-9f * wc * (r3 * r3));
+    sh = fmaxf(sh, 0.0f);
+    hw = fminf(fmaxf(hw, 0.0f), 0.9f);
+    dh_pb_factors f;
+    f.st = st;
+    f.sht = hypotf(st, sh);
+    f.P = (1.0f - hw) * idd / (2.0f * DH_PI_F * (st * st));
+    f.H = hw * idd / (2.0f * DH_PI_F * (f.sht * f.sht));
+    f.ap = st != 0.0f ? (-0.5f / (st * st)) * DH_LOG2E_F : -INFINITY;
+    f.ah = f.sht != 0.0f ? (-0.5f / (f.sht * f.sht)) * DH_LOG2E_F : -INFINITY;
+    return f;
+}
+
 // Wave-level helpers (wave64).
 __device__ static inline bool dh_wave_any(bool pred)
 {

@@ -212,41 +212,9 @@ k_pencil(dh_geom g, float vsadx, float vsady, int n_layers, const dh_layer_k *__
         if (!dh_wave_any(act)) continue;
         if (!act) continue;
 
-        // lowerBound + linear interpolation (CudaClasses.cuh:130-144, IMPTClasses.cu:69-92)
-        int lo = 0, hi = DH_LUT_LENGTH;
-        while (lo < hi) {
-            int m = (lo + hi) >> 1;
-            if (w <= s_lut[m].x) hi = m; else lo = m + 1;
-        }
-        float idd, sms;
-        if (lo == DH_LUT_LENGTH) { idd = s_lut[lo - 1].z; sms = s_lut[lo - 1].y; }
-        else if (lo == 0) { idd = s_lut[0].z; sms = s_lut[0].y; }
-        else {
-            const float4 a = s_lut[lo - 1], b = s_lut[lo];
-            const float f = (w - a.x) / (b.x - a.x);
-            idd = fmaf(b.z - a.z, f, a.z);
-            sms = fmaf(b.y - a.y, f, a.y);
-        }
-        // sigmaAir (IMPTClasses.cu:94-102)
-        const float da = dist - w + 0.7f * L.r80;
-        const float st = fmaf(fmaf(L.A, da, L.B), da, L.C) + sms;
-        // nuclearHalo (IMPTClasses.cu:104-121)
-        const float r80 = L.r80;
-        const float wc = fminf(fmaxf(w, 0.1f), r80 - 0.1f);
-        const float t5 = wc - r80 - 5.0f;
-        float sh = 2.85f + (0.0014f * r80 * logf(wc + 3.0f)) + (0.06f * wc) - (7.4e-5f * (wc * wc))
-                 - ((0.22f * r80) / (t5 * t5));
-        const float r3 = r80 + 3.0f;
-        float hw = 0.052f * logf(1.13f + (wc / (11.2f - (0.023f * r80))))
-                 + (0.35f * ((0.0017f * (r80 * r80)) - r80) / ((r3 * r3) - (wc * wc)))
-                 - (1.61e-9f * wc * (r3 * r3));
-        sh = fmaxf(sh, 0.0f);
-        hw = fminf(fmaxf(hw, 0.0f), 0.9f);
-        const float sht = hypotf(st, sh);
-        const float P = (1.0f - hw) * idd / (2.0f * DH_PI_F * (st * st));
-        const float H = hw * idd / (2.0f * DH_PI_F * (sht * sht));
-        const float ap = st != 0.0f ? (-0.5f / (st * st)) * DH_LOG2E_F : -INFINITY;
-        const float ah = sht != 0.0f ? (-0.5f / (sht * sht)) * DH_LOG2E_F : -INFINITY;
+        // LUT, sigma_air, halo and normalisations (IMPTClasses.cu:69-121,222-264)
+        const dh_pb_factors fc = dh_pencil_factors(s_lut, w, dist, L.r80, L.A, L.B, L.C);
+        const float P = fc.P, H = fc.H, ap = fc.ap, ah = fc.ah;
         const float lp = Fold ? log2f(P) : P;
         const float lh = Fold ? log2f(H) : H;
 

@@ -174,41 +174,10 @@ k_influence(dh_infl_args a)
                 float px, py, pz, hx, hy, hz;
                 dh_vox_xyz(g, min(i, ihi), j, k, px, py, pz);
                 dh_img_to_head(g, px, py, pz, hx, hy, hz);
-                // per (voxel, layer) factors: the dense kernel's prologue
-                int lo = 0, hi = DH_LUT_LENGTH;
-                while (lo < hi) {
-                    int m = (lo + hi) >> 1;
-                    if (w <= s_lut[m].x) hi = m; else lo = m + 1;
-                }
-                float idd, sms;
-                if (lo == DH_LUT_LENGTH) { idd = s_lut[lo - 1].z; sms = s_lut[lo - 1].y; }
-                else if (lo == 0) { idd = s_lut[0].z; sms = s_lut[0].y; }
-                else {
-                    const float4 p0 = s_lut[lo - 1], p1 = s_lut[lo];
-                    const float f = (w - p0.x) / (p1.x - p0.x);
-                    idd = fmaf(p1.z - p0.z, f, p0.z);
-                    sms = fmaf(p1.y - p0.y, f, p0.y);
-                }
                 const float dxs = g.src[0] - px, dys = g.src[1] - py, dzs = g.src[2] - pz;
                 const float dist = sqrtf(dxs * dxs + dys * dys + dzs * dzs);
-                const float da = dist - w + 0.7f * L.r80;
-                const float st = fmaf(fmaf(L.A, da, L.B), da, L.C) + sms;
-                const float r80 = L.r80;
-                const float wc = fminf(fmaxf(w, 0.1f), r80 - 0.1f);
-                const float t5 = wc - r80 - 5.0f;
-                float sh = 2.85f + (0.0014f * r80 * logf(wc + 3.0f)) + (0.06f * wc) - (7.4e-5f * (wc * wc))
-                         - ((0.22f * r80) / (t5 * t5));
-                const float r3 = r80 + 3.0f;
-                float hw = 0.052f * logf(1.13f + (wc / (11.2f - (0.023f * r80))))
-                         + (0.35f * ((0.0017f * (r80 * r80)) - r80) / ((r3 * r3) - (wc * wc)))
-                         - (1.61e-9f * wc * (r3 * r3));
-                sh = fmaxf(sh, 0.0f);
-                hw = fminf(fmaxf(hw, 0.0f), 0.9f);
-                const float sht = hypotf(st, sh);
-                const float P = (1.0f - hw) * idd / (2.0f * DH_PI_F * (st * st));
-                const float H = hw * idd / (2.0f * DH_PI_F * (sht * sht));
-                const float ap = st != 0.0f ? (-0.5f / (st * st)) * DH_LOG2E_F : -INFINITY;
-                const float ah = sht != 0.0f ? (-0.5f / (sht * sht)) * DH_LOG2E_F : -INFINITY;
+                const dh_pb_factors fc = dh_pencil_factors(s_lut, w, dist, L.r80, L.A, L.B, L.C);
+                const float P = fc.P, H = fc.H, ap = fc.ap, ah = fc.ah;
                 const float cx = 1.0f - hz / a.vsadx, cy = 1.0f - hz / a.vsady;
 #pragma unroll
                 for (int s = 0; s < S; s++) {
@@ -283,6 +252,48 @@ k_active_rows(int nx, int ny, int nz, int n_layers, const dh_layer_k *__restrict
     }
 }
 
+// Per layer, over the voxels the layer reaches (10 * WET <= 1.1 R80): the
+// maxima of sigma_t, sigma_halo_total, P and H (float bits; all >= 0), which
+// bound every spot's lateral dose profile in that layer.
+__global__ void __launch_bounds__(256)
+k_layer_bounds(dh_geom g, int n_layers, const dh_layer_k *__restrict__ layers,
+               const float4 *__restrict__ luts, const float *__restrict__ wet,
+               unsigned *__restrict__ out)
+{
+    __shared__ float4 s_lut[DH_LUT_LENGTH];
+    const uint32_t nvox = (uint32_t)g.nx * g.ny * g.nz;
+    const uint32_t v = blockIdx.x * 256 + threadIdx.x;
+    const bool valid = v < nvox;
+    const uint32_t vv = valid ? v : nvox - 1;
+    const int i = (int)(vv % (uint32_t)g.nx);
+    const uint32_t r = vv / (uint32_t)g.nx;
+    const int j = (int)(r % (uint32_t)g.ny), k = (int)(r / (uint32_t)g.ny);
+    float x, y, z;
+    dh_vox_xyz(g, i, j, k, x, y, z);
+    const float dxs = g.src[0] - x, dys = g.src[1] - y, dzs = g.src[2] - z;
+    const float dist = sqrtf(dxs * dxs + dys * dys + dzs * dzs);
+    const float w = wet[vv] * 10.0f;
+    for (int l = 0; l < n_layers; l++) {
+        const dh_layer_k L = layers[l];
+        __syncthreads();
+        for (int q = threadIdx.x; q < DH_LUT_LENGTH; q += 256) s_lut[q] = luts[(size_t)l * DH_LUT_LENGTH + q];
+        __syncthreads();
+        const bool act = valid && !(w > L.cut);
+        float m[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (act) {
+            const dh_pb_factors f = dh_pencil_factors(s_lut, w, dist, L.r80, L.A, L.B, L.C);
+            m[0] = fabsf(f.st); m[1] = fabsf(f.sht);
+            m[2] = fmaxf(f.P, 0.0f); m[3] = fmaxf(f.H, 0.0f);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            float x2 = m[q];
+            for (int off = 32; off > 0; off >>= 1) x2 = fmaxf(x2, __shfl_xor(x2, off, 64));
+            if (__lane_id() == 0 && x2 > 0.0f) atomicMax(&out[4 * l + q], __float_as_uint(x2));
+        }
+    }
+}
+
 __global__ void k_zero_words(unsigned *p, size_t n)
 {
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) p[t] = 0u;
@@ -335,6 +346,7 @@ struct dh_influence {
     unsigned long long *d_stats = nullptr;   // 4 counters per pass
     std::vector<float> peak;         // host copy after the peak pass
     std::vector<float> radius;       // truncation radius per spot (COUNT = FILL)
+    std::vector<float> bnd;          // per layer: sigma_t, sigma_ht, P, H maxima
     std::vector<int> order;          // caller's spot row of each matrix row
     std::vector<int64_t> offsets;    // host copy of the exclusive scan
     int64_t nnz = -1;
@@ -562,16 +574,44 @@ static void active_rows(dh_influence *f, const float *d_wet, hipStream_t s)
     DH_HIP(hipGetLastError());
 }
 
+// Per-layer maxima of sigma_t, sigma_ht, P, H over the voxels the layer
+// reaches, from the actual WET.
+static void wet_bounds(dh_influence *f, const float *d_wet, hipStream_t s)
+{
+    const size_t nl = f->layers.size();
+    if (!nl) return;
+    unsigned *d_b = (unsigned *)dh_scratch(f->dev, DH_SLOT_TMP1, sizeof(unsigned) * 4 * nl);
+    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(256), 0, s, d_b, 4 * nl);
+    const size_t nvox = (size_t)f->g.nx * f->g.ny * f->g.nz;
+    {
+        dh_timed t("influence_bounds", s);
+        hipLaunchKernelGGL(k_layer_bounds, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, s, f->g,
+                           (int)nl, f->d_layers, f->d_luts, d_wet, d_b);
+        DH_HIP(hipGetLastError());
+    }
+    std::vector<unsigned> b(4 * nl);
+    DH_HIP(hipMemcpyAsync(b.data(), d_b, sizeof(unsigned) * 4 * nl, hipMemcpyDeviceToHost, s));
+    DH_HIP(hipStreamSynchronize(s));
+    f->bnd.assign(4 * nl, 0.0f);
+    for (size_t q = 0; q < 4 * nl; q++) std::memcpy(&f->bnd[q], &b[q], 4);
+}
+
 int64_t dh_influence_count(dh_influence *f, const float *d_wet, int64_t base, hipStream_t s)
 {
     const int n = f->n_spots;
     active_rows(f, d_wet, s);
+    wet_bounds(f, d_wet, s);
     hipLaunchKernelGGL(k_zero_words, dim3(64), dim3(256), 0, s, f->d_peak, (size_t)std::max(n, 1));
-    // peak pass radius: 3 sigma_max of the layer
+    // Peak pass radius.  At lateral distance d every voxel's dose is at most
+    // Pmax e^(-d^2 / 2 st_max^2) + Hmax e^(-d^2 / 2 sht_max^2); a voxel within
+    // one voxel diagonal of the axis at the depth of the layer's largest P + H
+    // bounds the peak from below by (P + H) e^(-dg^2 / 2 st^2), so the peak
+    // is found within the radius where the upper bound drops below half of
+    // Pmax (3 sht_max is a generous version of that).
     std::vector<float> r(n, 0.0f);
     for (int q = 0; q < n; q++) {
         const int l = f->spot_layer[q];
-        r[q] = l >= 0 ? 3.0f * f->sig_max[l] + f->g.s : 0.0f;
+        r[q] = l >= 0 ? 3.0f * f->bnd[4 * l + 1] + f->g.s : 0.0f;
     }
     launch<INF_PEAK>(f, make_blocks(f, r), d_wet, nullptr, nullptr, s);
     std::vector<unsigned> pk(std::max(n, 1));
@@ -586,9 +626,13 @@ int64_t dh_influence_count(dh_influence *f, const float *d_wet, int64_t base, hi
         const int l = f->spot_layer[q];
         if (l < 0 || !(v > 0.0f)) { r[q] = 0.0f; continue; }
         thr[q] = f->rel * v;
-        // beyond R the dose bound dose_max * exp(-R^2 / 2 sig^2) is below thr
-        const float ratio = std::max(f->dose_max[l] / thr[q], 1.0f);
-        r[q] = f->sig_max[l] * std::sqrt(2.0f * std::log(ratio)) + f->g.s;
+        // beyond R both Gaussian terms are below thr / 2:
+        // Pmax e^(-R^2 / 2 st^2) <= thr/2 and Hmax e^(-R^2 / 2 sht^2) <= thr/2
+        const float st = f->bnd[4 * l], sht = f->bnd[4 * l + 1];
+        const float Pm = f->bnd[4 * l + 2], Hm = f->bnd[4 * l + 3];
+        const float rp = Pm > 0.5f * thr[q] ? st * std::sqrt(2.0f * std::log(2.0f * Pm / thr[q])) : 0.0f;
+        const float rh = Hm > 0.5f * thr[q] ? sht * std::sqrt(2.0f * std::log(2.0f * Hm / thr[q])) : 0.0f;
+        r[q] = std::max(rp, rh) + f->g.s;
     }
     DH_HIP(hipMemcpyAsync(f->d_thr, thr.data(), sizeof(float) * n, hipMemcpyHostToDevice, s));
     launch<INF_COUNT>(f, make_blocks(f, r), d_wet, nullptr, nullptr, s);
