@@ -295,7 +295,7 @@ def main():
         log("setting up the batched influence matrix: (160,256,256) @ 2 mm, 3 fields, ~15k spots")
         st = setup_influence(torch)
         ms = timed(torch, run_influence, st, a.influence_steps, 1)
-        kt = {k: engine.prof_get(k) for k in ("raytrace", "smooth", "influence_peak",
+        kt = {k: engine.prof_get(k) for k in ("raytrace", "smooth", "influence_rows", "influence_bounds", "influence_peak",
                                               "influence_count", "influence_fill")}
         fill_ms = kt["influence_fill"][0] / max(kt["influence_fill"][1], 1)
         nvox = st["d_rlsp"].numel()

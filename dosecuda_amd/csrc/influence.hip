@@ -48,6 +48,7 @@ struct dh_infl_args {
     int *counts;                 // per spot * C (COUNT)
     const int64_t *offsets;      // per spot * C (FILL)
     const int2 *active;          // [layer][nz][ny] first/last i with w <= cut (lo > hi: none)
+    const float4 *factors;       // [layer][voxel] {ap, log2 P, ah, log2 H} (P, H >= 0 folded)
     uint32_t *col;
     float *val;
     int *queue;
@@ -76,11 +77,9 @@ template <int S, int MODE>
 __global__ void __launch_bounds__(64)
 k_influence(dh_infl_args a)
 {
-    __shared__ float4 s_lut[DH_LUT_LENGTH];
     __shared__ int s_item;
     const dh_geom &g = a.g;
     const int lane = __lane_id();
-    int cached_layer = -1;
     for (;;) {
         if (lane == 0) s_item = atomicAdd(a.queue, 1);
         __syncthreads();
@@ -89,11 +88,6 @@ k_influence(dh_infl_args a)
         if (item >= a.n_items) break;
         const int bi = item / a.C, chunk = item % a.C;
         const dh_infl_block B = a.blocks[bi];
-        if (B.layer != cached_layer) {
-            for (int q = lane; q < DH_LUT_LENGTH; q += 64) s_lut[q] = a.luts[(size_t)B.layer * DH_LUT_LENGTH + q];
-            __syncthreads();
-            cached_layer = B.layer;
-        }
         const dh_layer_k L = a.layers[B.layer];
         // spot records of the block (wave-uniform)
         float sx[S], sy[S], tx[S], ty[S], th[S];
@@ -174,10 +168,8 @@ k_influence(dh_infl_args a)
                 float px, py, pz, hx, hy, hz;
                 dh_vox_xyz(g, min(i, ihi), j, k, px, py, pz);
                 dh_img_to_head(g, px, py, pz, hx, hy, hz);
-                const float dxs = g.src[0] - px, dys = g.src[1] - py, dzs = g.src[2] - pz;
-                const float dist = sqrtf(dxs * dxs + dys * dys + dzs * dzs);
-                const dh_pb_factors fc = dh_pencil_factors(s_lut, w, dist, L.r80, L.A, L.B, L.C);
-                const float P = fc.P, H = fc.H, ap = fc.ap, ah = fc.ah;
+                // factors of this (voxel, layer), written by the bounds pass
+                const float4 fc = a.factors[(size_t)B.layer * g.nx * g.ny * g.nz + v];
                 const float cx = 1.0f - hz / a.vsadx, cy = 1.0f - hz / a.vsady;
 #pragma unroll
                 for (int s = 0; s < S; s++) {
@@ -185,7 +177,8 @@ k_influence(dh_infl_args a)
                     const float r2 = fmaf(ux, ux, uy * uy);
                     const float dt = fmaf(tx[s], ux, ty[s] * uy);
                     const float d2 = fmaf(-dt, dt, r2);
-                    const float val = fmaf(P, __builtin_amdgcn_exp2f(ap * d2), H * __builtin_amdgcn_exp2f(ah * d2));
+                    const float val = __builtin_amdgcn_exp2f(fmaf(fc.x, d2, fc.y)) +
+                                      __builtin_amdgcn_exp2f(fmaf(fc.z, d2, fc.w));
                     if (MODE == INF_PEAK) {
                         pk[s] = act ? fmaxf(pk[s], val) : pk[s];
                     } else {
@@ -258,7 +251,7 @@ k_active_rows(int nx, int ny, int nz, int n_layers, const dh_layer_k *__restrict
 __global__ void __launch_bounds__(256)
 k_layer_bounds(dh_geom g, int n_layers, const dh_layer_k *__restrict__ layers,
                const float4 *__restrict__ luts, const float *__restrict__ wet,
-               unsigned *__restrict__ out)
+               unsigned *__restrict__ out, float4 *__restrict__ factors)
 {
     __shared__ float4 s_lut[DH_LUT_LENGTH];
     const uint32_t nvox = (uint32_t)g.nx * g.ny * g.nz;
@@ -280,11 +273,15 @@ k_layer_bounds(dh_geom g, int n_layers, const dh_layer_k *__restrict__ layers,
         __syncthreads();
         const bool act = valid && !(w > L.cut);
         float m[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        // inactive voxels get a zero dose factor: exp2(-inf) = 0
+        float4 fo = make_float4(0.0f, -INFINITY, 0.0f, -INFINITY);
         if (act) {
             const dh_pb_factors f = dh_pencil_factors(s_lut, w, dist, L.r80, L.A, L.B, L.C);
             m[0] = fabsf(f.st); m[1] = fabsf(f.sht);
             m[2] = fmaxf(f.P, 0.0f); m[3] = fmaxf(f.H, 0.0f);
+            fo = make_float4(f.ap, log2f(fmaxf(f.P, 0.0f)), f.ah, log2f(fmaxf(f.H, 0.0f)));
         }
+        if (valid) factors[(size_t)l * nvox + v] = fo;
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             float x2 = m[q];
@@ -323,6 +320,7 @@ struct dh_influence {
     dh_geom g;
     float vsadx = 0, vsady = 0;
     int S = 16, C = 8;
+    int cus = 0;
     int n_spots = 0;
     float rel = 1e-4f;
     std::vector<dh_layer_k> layers;
@@ -344,6 +342,7 @@ struct dh_influence {
     int *d_queue = nullptr;
     int2 *d_active = nullptr;
     unsigned long long *d_stats = nullptr;   // 4 counters per pass
+    float4 *d_factors = nullptr;             // [layer][voxel]
     std::vector<float> peak;         // host copy after the peak pass
     std::vector<float> radius;       // truncation radius per spot (COUNT = FILL)
     std::vector<float> bnd;          // per layer: sigma_t, sigma_ht, P, H maxima
@@ -464,6 +463,8 @@ dh_influence *dh_influence_make(int dev, const dh_geom &g, float vsadx, float vs
         DH_HIP(hipMalloc(&f->d_offsets, sizeof(int64_t) * ((size_t)n_spots * C + 1)));
         DH_HIP(hipMalloc(&f->d_queue, 256));
         DH_HIP(hipMalloc(&f->d_stats, sizeof(unsigned long long) * 12));
+        DH_HIP(hipMalloc(&f->d_factors, sizeof(float4) * std::max<size_t>(layers.size(), 1) *
+                                            (size_t)g.nx * g.ny * g.nz));
         DH_HIP(hipMalloc(&f->d_active, sizeof(int2) * std::max<size_t>(layers.size(), 1) * g.ny * g.nz));
     } catch (...) {
         void dh_influence_free(dh_influence *);
@@ -478,7 +479,8 @@ void dh_influence_free(dh_influence *f)
     if (!f) return;
     (void)hipSetDevice(f->dev);
     void *ps[] = {f->d_layers, f->d_luts, f->d_spots, f->d_block_spots, f->d_blocks, f->d_thr,
-                  f->d_peak, f->d_counts, f->d_offsets, f->d_queue, f->d_active, f->d_stats};
+                  f->d_peak, f->d_counts, f->d_offsets, f->d_queue, f->d_active, f->d_stats,
+                  f->d_factors};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     delete f;
@@ -547,13 +549,19 @@ static void launch(dh_influence *f, const std::vector<dh_infl_block> &blocks, co
     a.luts = f->d_luts; a.spots = f->d_spots; a.thr = f->d_thr; a.wet = d_wet;
     a.peak_bits = f->d_peak; a.counts = f->d_counts; a.offsets = f->d_offsets;
     a.active = f->d_active;
+    a.factors = f->d_factors;
     a.stats = f->d_stats + 4 * MODE;
     hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, (unsigned *)a.stats, (size_t)8);
     a.col = d_col; a.val = d_val; a.queue = f->d_queue;
     if (a.n_items == 0) return;
-    int cus = 256;
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, f->dev) == hipSuccess) cus = prop.multiProcessorCount;
+    if (!f->cus) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, f->dev) == hipSuccess && v > 0)
+            f->cus = v;
+        else
+            f->cus = 256;
+    }
+    const int cus = f->cus;
     const unsigned grid = (unsigned)std::min(a.n_items, cus * 32);
     const char *name = MODE == INF_PEAK ? "influence_peak" : MODE == INF_COUNT ? "influence_count" : "influence_fill";
     dh_timed t(name, s);
@@ -586,7 +594,7 @@ static void wet_bounds(dh_influence *f, const float *d_wet, hipStream_t s)
     {
         dh_timed t("influence_bounds", s);
         hipLaunchKernelGGL(k_layer_bounds, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, s, f->g,
-                           (int)nl, f->d_layers, f->d_luts, d_wet, d_b);
+                           (int)nl, f->d_layers, f->d_luts, d_wet, d_b, f->d_factors);
         DH_HIP(hipGetLastError());
     }
     std::vector<unsigned> b(4 * nl);
@@ -594,6 +602,19 @@ static void wet_bounds(dh_influence *f, const float *d_wet, hipStream_t s)
     DH_HIP(hipStreamSynchronize(s));
     f->bnd.assign(4 * nl, 0.0f);
     for (size_t q = 0; q < 4 * nl; q++) std::memcpy(&f->bnd[q], &b[q], 4);
+}
+
+// Factor planes for FILL (the bounds themselves are not needed again).
+static void factors_only(dh_influence *f, const float *d_wet, hipStream_t s)
+{
+    const size_t nl = f->layers.size();
+    if (!nl) return;
+    unsigned *d_b = (unsigned *)dh_scratch(f->dev, DH_SLOT_TMP1, sizeof(unsigned) * 4 * nl);
+    const size_t nvox = (size_t)f->g.nx * f->g.ny * f->g.nz;
+    dh_timed t("influence_bounds", s);
+    hipLaunchKernelGGL(k_layer_bounds, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, s, f->g,
+                       (int)nl, f->d_layers, f->d_luts, d_wet, d_b, f->d_factors);
+    DH_HIP(hipGetLastError());
 }
 
 int64_t dh_influence_count(dh_influence *f, const float *d_wet, int64_t base, hipStream_t s)
@@ -662,6 +683,7 @@ void dh_influence_fill(dh_influence *f, const float *d_wet, uint32_t *d_col, flo
     // the same boxes and active rows as the COUNT pass, so every entry lands
     // in its slot
     active_rows(f, d_wet, s);
+    factors_only(f, d_wet, s);
     launch<INF_FILL>(f, make_blocks(f, f->radius), d_wet, d_col, d_val, s);
 }
 
