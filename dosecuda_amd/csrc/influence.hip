@@ -251,9 +251,10 @@ k_active_rows(int nx, int ny, int nz, int n_layers, const dh_layer_k *__restrict
 __global__ void __launch_bounds__(256)
 k_layer_bounds(dh_geom g, int n_layers, const dh_layer_k *__restrict__ layers,
                const float4 *__restrict__ luts, const float *__restrict__ wet,
-               unsigned *__restrict__ out, float4 *__restrict__ factors)
+               float *__restrict__ partial, float4 *__restrict__ factors)
 {
     __shared__ float4 s_lut[DH_LUT_LENGTH];
+    __shared__ float s_red[4][4];
     const uint32_t nvox = (uint32_t)g.nx * g.ny * g.nz;
     const uint32_t v = blockIdx.x * 256 + threadIdx.x;
     const bool valid = v < nvox;
@@ -282,13 +283,34 @@ k_layer_bounds(dh_geom g, int n_layers, const dh_layer_k *__restrict__ layers,
             fo = make_float4(f.ap, log2f(fmaxf(f.P, 0.0f)), f.ah, log2f(fmaxf(f.H, 0.0f)));
         }
         if (valid) factors[(size_t)l * nvox + v] = fo;
+        // workgroup maxima -> one partial record per workgroup and layer
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             float x2 = m[q];
             for (int off = 32; off > 0; off >>= 1) x2 = fmaxf(x2, __shfl_xor(x2, off, 64));
-            if (__lane_id() == 0 && x2 > 0.0f) atomicMax(&out[4 * l + q], __float_as_uint(x2));
+            if (__lane_id() == 0) s_red[threadIdx.x >> 6][q] = x2;
+        }
+        __syncthreads();
+        if (threadIdx.x < 4) {
+            const int q = threadIdx.x;
+            const float x2 = fmaxf(fmaxf(s_red[0][q], s_red[1][q]), fmaxf(s_red[2][q], s_red[3][q]));
+            partial[((size_t)l * gridDim.x + blockIdx.x) * 4 + q] = x2;
         }
     }
+}
+
+// Reduces the partial maxima: one workgroup per (layer, quantity).
+__global__ void __launch_bounds__(256)
+k_reduce_bounds(const float *__restrict__ partial, int nblocks, float *__restrict__ out)
+{
+    const int l = blockIdx.x >> 2, q = blockIdx.x & 3;
+    __shared__ float s_m[4];
+    float m = 0.0f;
+    for (int b = threadIdx.x; b < nblocks; b += 256) m = fmaxf(m, partial[((size_t)l * nblocks + b) * 4 + q]);
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+    if (__lane_id() == 0) s_m[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) out[4 * l + q] = fmaxf(fmaxf(s_m[0], s_m[1]), fmaxf(s_m[2], s_m[3]));
 }
 
 __global__ void k_zero_words(unsigned *p, size_t n)
@@ -588,33 +610,21 @@ static void wet_bounds(dh_influence *f, const float *d_wet, hipStream_t s)
 {
     const size_t nl = f->layers.size();
     if (!nl) return;
-    unsigned *d_b = (unsigned *)dh_scratch(f->dev, DH_SLOT_TMP1, sizeof(unsigned) * 4 * nl);
-    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(256), 0, s, d_b, 4 * nl);
     const size_t nvox = (size_t)f->g.nx * f->g.ny * f->g.nz;
+    const unsigned nblk = (unsigned)((nvox + 255) / 256);
+    float *d_part = (float *)dh_scratch(f->dev, DH_SLOT_TMP0, sizeof(float) * 4 * nl * nblk);
+    float *d_b = (float *)dh_scratch(f->dev, DH_SLOT_TMP1, sizeof(float) * 4 * nl);
     {
         dh_timed t("influence_bounds", s);
-        hipLaunchKernelGGL(k_layer_bounds, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, s, f->g,
-                           (int)nl, f->d_layers, f->d_luts, d_wet, d_b, f->d_factors);
+        hipLaunchKernelGGL(k_layer_bounds, dim3(nblk), dim3(256), 0, s, f->g, (int)nl, f->d_layers,
+                           f->d_luts, d_wet, d_part, f->d_factors);
+        DH_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_reduce_bounds, dim3((unsigned)(4 * nl)), dim3(256), 0, s, d_part, (int)nblk, d_b);
         DH_HIP(hipGetLastError());
     }
-    std::vector<unsigned> b(4 * nl);
-    DH_HIP(hipMemcpyAsync(b.data(), d_b, sizeof(unsigned) * 4 * nl, hipMemcpyDeviceToHost, s));
-    DH_HIP(hipStreamSynchronize(s));
     f->bnd.assign(4 * nl, 0.0f);
-    for (size_t q = 0; q < 4 * nl; q++) std::memcpy(&f->bnd[q], &b[q], 4);
-}
-
-// Factor planes for FILL (the bounds themselves are not needed again).
-static void factors_only(dh_influence *f, const float *d_wet, hipStream_t s)
-{
-    const size_t nl = f->layers.size();
-    if (!nl) return;
-    unsigned *d_b = (unsigned *)dh_scratch(f->dev, DH_SLOT_TMP1, sizeof(unsigned) * 4 * nl);
-    const size_t nvox = (size_t)f->g.nx * f->g.ny * f->g.nz;
-    dh_timed t("influence_bounds", s);
-    hipLaunchKernelGGL(k_layer_bounds, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, s, f->g,
-                       (int)nl, f->d_layers, f->d_luts, d_wet, d_b, f->d_factors);
-    DH_HIP(hipGetLastError());
+    DH_HIP(hipMemcpyAsync(f->bnd.data(), d_b, sizeof(float) * 4 * nl, hipMemcpyDeviceToHost, s));
+    DH_HIP(hipStreamSynchronize(s));
 }
 
 int64_t dh_influence_count(dh_influence *f, const float *d_wet, int64_t base, hipStream_t s)
@@ -680,10 +690,9 @@ void dh_influence_fill(dh_influence *f, const float *d_wet, uint32_t *d_col, flo
     DH_REQUIRE(f->nnz >= 0, "dh_influence_count must run before dh_influence_fill");
     const size_t n = f->offsets.size();
     if (n) DH_HIP(hipMemcpyAsync(f->d_offsets, f->offsets.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, s));
-    // the same boxes and active rows as the COUNT pass, so every entry lands
-    // in its slot
-    active_rows(f, d_wet, s);
-    factors_only(f, d_wet, s);
+    // the same boxes as the COUNT pass, so every entry lands in its slot;
+    // the active rows and factor planes of the COUNT pass are reused: the
+    // caller passes the same WET to count and fill
     launch<INF_FILL>(f, make_blocks(f, f->radius), d_wet, d_col, d_val, s);
 }
 

@@ -147,7 +147,7 @@ int dh_photon_dose_dev(int gpu_id, const dh_grid *grid, const dh_photon_cp *cp,
  * spots_per_block (8 or 16) and row_chunks (1..256) are tuning knobs.
  * Usage: create -> count (synchronises; gives nnz and row_ptr, whose entries
  * start at `base` so fields can share one CSR) -> caller allocates col/val ->
- * fill (stream-ordered) -> destroy. */
+ * fill (stream-ordered, with the same WET as count) -> destroy. */
 typedef struct dh_influence dh_influence;
 int dh_influence_create(int gpu_id, const dh_grid *grid, const dh_proton_beam *beam,
                         const dh_proton_lut *lut, const float *spot_rows, int32_t n_spots,
