@@ -97,6 +97,15 @@ SIGNATURES = {
     "dh_influence_info": [_P, C.POINTER(C.c_int32), _P, _P],
     "dh_influence_destroy": [_P],
     "dh_influence_stats": [_P, _P, _P],
+    "dh_stage_ccc_list_dev": [_I, C.POINTER(Grid), C.POINTER(PhotonCP), C.POINTER(PhotonModel), _P,
+                              _P, _P, _P, C.c_int32, _P, _P],
+    "dh_nn_conv3d_dev": [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P, _P,
+                         C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_float, _P, _P, _P],
+    "dh_nn_conv3d_wgrad_dev": [_P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                               C.c_int32, C.c_int32, _P, _P],
+    "dh_nn_prep_weights_dev": [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P, _P],
+    "dh_gamma_dev": [_P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_float,
+                     C.c_float, C.c_int32, _P, _P],
     "dh_prof_enable": [_I],
     "dh_prof_reset": [],
     "dh_prof_get": [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)],

@@ -535,3 +535,22 @@ extern "C" int dh_stage_ccc_dev(int gpu_id, const dh_grid *grid, const dh_photon
         dh_launch_ccc(ca, d_terma, d_density, d_electron, trp, act, cnt, d_dose_out, (hipStream_t)stream);
     });
 }
+
+extern "C" int dh_stage_ccc_list_dev(int gpu_id, const dh_grid *grid, const dh_photon_cp *cp,
+                                     const dh_photon_model *model, const float *d_terma,
+                                     const float *d_density, const float *d_electron,
+                                     const int32_t *d_list, int32_t n, float *d_dose_out, void *stream)
+{
+    return guarded([&] {
+        dh_device_enter(gpu_id);
+        size_t nv = nvox_of(grid);
+        DH_REQUIRE(cp && d_terma && d_density && d_electron && d_dose_out, "null argument");
+        DH_REQUIRE(n >= 0 && (size_t)n <= nv && (n == 0 || d_list), "bad voxel list");
+        dh_ccc_args ca = make_ccc_args(grid, cp, model);
+        dh_pad p = ca.p;
+        float2 *trp = (float2 *)dh_scratch(gpu_id, DH_SLOT_TR_PAD, sizeof(float2) * (size_t)p.pxy * p.pz);
+        int *cnt = (int *)dh_scratch(gpu_id, DH_SLOT_COUNTER, 256);
+        dh_launch_ccc_list(ca, d_terma, d_density, d_electron, trp, d_list, n, cnt, d_dose_out,
+                           (hipStream_t)stream);
+    });
+}

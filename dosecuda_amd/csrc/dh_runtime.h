@@ -128,6 +128,12 @@ void dh_launch_ccc(const dh_ccc_args &a, const float *terma, const float *densit
                    const float *electron, float2 *tr_pad, int *active, int *counter,
                    float *dose, hipStream_t s);
 
+// Cone superposition on an explicit list of n voxel indices (each < nx*ny*nz);
+// only the listed voxels of `dose` are written.
+void dh_launch_ccc_list(const dh_ccc_args &a, const float *terma, const float *density,
+                        const float *electron, float2 *tr_pad, const int *list, int n,
+                        int *counter, float *dose, hipStream_t s);
+
 // A proton field's spots sorted by energy id (stable), its layers
 // (importLayers) and one 400-entry LUT row per layer.  spots holds 8 floats
 // per sorted spot: x, y, unit axis direction x/y components, MU; order[q] is

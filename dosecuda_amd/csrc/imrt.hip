@@ -109,6 +109,11 @@ __global__ void k_zero_int(int *p, int n)
     if (t < n) p[t] = 0;
 }
 
+__global__ void k_set_int(int *p, int v)
+{
+    if (threadIdx.x == 0) *p = v;
+}
+
 // TERMA of the voxel centre sampled like the reference's texture fetch
 // (IMRTClasses.cu:267-273).
 __device__ static inline float center_terma(const dh_geom &g, const dh_pad &p,
@@ -323,4 +328,23 @@ void dh_launch_ccc(const dh_ccc_args &a, const float *terma, const float *densit
                            electron, (const int *)active, (const int *)counter, dose);
         DH_HIP(hipGetLastError());
     }
+}
+
+void dh_launch_ccc_list(const dh_ccc_args &a, const float *terma, const float *density,
+                        const float *electron, float2 *tr_pad, const int *list, int n,
+                        int *counter, float *dose, hipStream_t s)
+{
+    const dh_geom &g = a.g;
+    size_t np = (size_t)a.p.pxy * a.p.pz;
+    hipLaunchKernelGGL(k_pack_tr, dim3((unsigned)std::min<size_t>(grid_for(np), 4096)),
+                       dim3(kBlock), 0, s, terma, density, g.nx, g.ny, g.nz, a.p, tr_pad);
+    DH_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s, counter, n);
+    DH_HIP(hipGetLastError());
+    if (n <= 0) return;
+    dh_timed t("ccc_list", s);
+    const unsigned grid = (unsigned)std::min<size_t>(((size_t)n * 4 + kBlock - 1) / kBlock, 2048);
+    hipLaunchKernelGGL(k_ccc, dim3(std::max(grid, 1u)), dim3(kBlock), 0, s, a, (const float2 *)tr_pad,
+                       electron, list, (const int *)counter, dose);
+    DH_HIP(hipGetLastError());
 }

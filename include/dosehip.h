@@ -184,6 +184,37 @@ int dh_stage_ccc_dev(int gpu_id, const dh_grid *grid, const dh_photon_cp *cp,
                      const float *d_density, const float *d_electron, float *d_dose_out,
                      void *stream);
 
+/* Cone superposition for an explicit list of voxel indices (each in
+ * [0, nx*ny*nz)); only listed voxels of d_dose_out are written.  Used as the
+ * per-voxel plain fallback of the surrogate. */
+int dh_stage_ccc_list_dev(int gpu_id, const dh_grid *grid, const dh_photon_cp *cp,
+                          const dh_photon_model *model, const float *d_terma,
+                          const float *d_density, const float *d_electron,
+                          const int32_t *d_list, int32_t n, float *d_dose_out, void *stream);
+
+/* ---- learned CCCS surrogate: BF16 MFMA building blocks ---------------- */
+/* No reference counterpart.  Tensors are channels-last bf16 [n][d][h][w][c].
+ * Convolution (k = 1 or 3, stride 1, zero "same" padding) with weights in
+ * the [taps][cout_pad][cin] bf16 layout, FP32 accumulation, optional bias and
+ * LeakyReLU (act = 1); writes bf16 y and/or fp32 y32.  cin % 16 == 0,
+ * cout_pad % 32 == 0. */
+int dh_nn_conv3d_dev(const void *x, int32_t n, int32_t d, int32_t h, int32_t w, int32_t cin,
+                     const void *wt, const float *bias, int32_t cout, int32_t cout_pad,
+                     int32_t ksize, int32_t act, float slope, void *y, float *y32, void *stream);
+/* dw[taps][cin][cout] += sum_v x[v + off][ci] * dy[v][co] (FP32 atomics). */
+int dh_nn_conv3d_wgrad_dev(const void *x, const void *dy, int32_t n, int32_t d, int32_t h,
+                           int32_t w, int32_t cin, int32_t cout, int32_t ksize, float *dw,
+                           void *stream);
+/* fp32 [taps][cin][cout] -> bf16 [taps][rows_pad][cin] (mode 0, forward) or
+ * the tap-reversed [taps][rows_pad][cout] (mode 1, input gradient). */
+int dh_nn_prep_weights_dev(const float *w, int32_t taps, int32_t cin, int32_t cout, int32_t mode,
+                           int32_t rows_pad, void *out, void *stream);
+/* Global gamma index on the voxel grid (Low et al. 1998): out = gamma for
+ * voxels with ref >= cut, -1 elsewhere; search radius in voxels. */
+int dh_gamma_dev(const float *eval, const float *ref, int32_t nz, int32_t ny, int32_t nx,
+                 float spacing, float dta_mm, float dd_abs, float cut, int32_t radius_vox,
+                 float *out, void *stream);
+
 /* ---- kernel timing (HIP events on the launch stream) ------------------ */
 /* When enabled, every kernel launch of the library is bracketed by HIP
  * events; dh_prof_get sums the elapsed times of the launches of one kernel
