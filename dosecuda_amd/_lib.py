@@ -99,8 +99,16 @@ SIGNATURES = {
     "dh_influence_stats": [_P, _P, _P],
     "dh_stage_ccc_list_dev": [_I, C.POINTER(Grid), C.POINTER(PhotonCP), C.POINTER(PhotonModel), _P,
                               _P, _P, _P, C.c_int32, _P, _P],
-    "dh_nn_conv3d_dev": [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P, _P,
-                         C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_float, _P, _P, _P],
+    "dh_nn_conv3d_dev": [_P, C.c_int32, C.c_int32, _P, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                         C.c_int32, _P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_float,
+                         _P, _P, C.c_int32, C.c_int32, _P],
+    "dh_nn_pool2_dev": [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P, _P],
+    "dh_sur_pack_dev": [C.c_int32, C.c_int32, C.c_int32, _P, _P, _P, _P, _P, _P],
+    "dh_sur_finish_dev": [C.c_int32, C.c_int32, C.c_int32, _P, _P, C.c_float, _P, _P, C.c_float,
+                          _P, _P, _P],
+    "dh_sur_check_work_size": [C.POINTER(Grid), C.c_int32, C.POINTER(C.c_int64)],
+    "dh_sur_check_dev": [_I, C.POINTER(Grid), C.POINTER(PhotonCP), C.POINTER(PhotonModel), _P, _P,
+                         _P, _P, _P, C.c_int32, C.c_float, _P, C.c_int64, _P, _P],
     "dh_nn_conv3d_wgrad_dev": [_P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                C.c_int32, C.c_int32, _P, _P],
     "dh_nn_prep_weights_dev": [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P, _P],

@@ -554,3 +554,34 @@ extern "C" int dh_stage_ccc_list_dev(int gpu_id, const dh_grid *grid, const dh_p
                            (hipStream_t)stream);
     });
 }
+
+extern "C" int dh_sur_check_work_size(const dh_grid *grid, int32_t stride, int64_t *n_ints)
+{
+    return guarded([&] {
+        DH_REQUIRE(grid && n_ints && stride >= 1, "bad argument");
+        nvox_of(grid);
+        *n_ints = dh_sur_check_work_len(grid->nz, grid->ny, grid->nx, stride);
+    });
+}
+
+extern "C" int dh_sur_check_dev(int gpu_id, const dh_grid *grid, const dh_photon_cp *cp,
+                                const dh_photon_model *model, const float *d_terma,
+                                const float *d_density, const float *d_electron, float *d_dose,
+                                const float *d_dmax, int32_t stride, float tol, int32_t *d_work,
+                                int64_t work_len, float *d_plain, void *stream)
+{
+    return guarded([&] {
+        dh_device_enter(gpu_id);
+        nvox_of(grid);
+        DH_REQUIRE(cp && d_terma && d_density && d_electron && d_dose && d_dmax && d_work && d_plain,
+                   "null argument");
+        DH_REQUIRE(stride >= 1 && stride <= 64, "lattice stride must be 1..64");
+        DH_REQUIRE(work_len >= dh_sur_check_work_len(grid->nz, grid->ny, grid->nx, stride),
+                   "work buffer too small (dh_sur_check_work_size)");
+        dh_ccc_args ca = make_ccc_args(grid, cp, model);
+        dh_pad p = ca.p;
+        float2 *trp = (float2 *)dh_scratch(gpu_id, DH_SLOT_TR_PAD, sizeof(float2) * (size_t)p.pxy * p.pz);
+        dh_launch_sur_check(ca, d_terma, d_density, d_electron, d_dose, d_dmax, stride, tol, d_work,
+                            d_plain, trp, (hipStream_t)stream);
+    });
+}

@@ -134,6 +134,25 @@ void dh_launch_ccc_list(const dh_ccc_args &a, const float *terma, const float *d
                         const float *electron, float2 *tr_pad, const int *list, int n,
                         int *counter, float *dose, hipStream_t s);
 
+// Padded {TERMA, density} pairs for the cone superposition.
+void dh_launch_pack_tr(const dh_ccc_args &a, const float *terma, const float *density,
+                       float2 *tr_pad, hipStream_t s);
+
+// Cone superposition on a voxel list whose length is read on the device from
+// *counter (at most max_n); tr_pad already packed.
+void dh_launch_ccc_counted(const dh_ccc_args &a, const float2 *tr_pad, const float *electron,
+                           const int *list, const int *counter, int64_t max_n, float *dose,
+                           hipStream_t s);
+
+// Checked surrogate: plain CCCS on the lattice, per-cell comparison with the
+// surrogate dose, plain CCCS on the voxels of failed and uncovered cells.
+// work: int32 [4 + 3 * cells + voxels] (counters, lattice flags, failure
+// flags, lattice list, fallback list); plain: fp32 voxels.
+int64_t dh_sur_check_work_len(int nz, int ny, int nx, int stride);
+void dh_launch_sur_check(const dh_ccc_args &a, const float *terma, const float *density,
+                         const float *electron, float *dose, const float *dmax, int stride,
+                         float tol, int *work, float *plain, float2 *tr_pad, hipStream_t s);
+
 // A proton field's spots sorted by energy id (stable), its layers
 // (importLayers) and one 400-entry LUT row per layer.  spots holds 8 floats
 // per sorted spot: x, y, unit axis direction x/y components, MU; order[q] is

@@ -348,3 +348,26 @@ void dh_launch_ccc_list(const dh_ccc_args &a, const float *terma, const float *d
                        electron, list, (const int *)counter, dose);
     DH_HIP(hipGetLastError());
 }
+
+void dh_launch_pack_tr(const dh_ccc_args &a, const float *terma, const float *density,
+                       float2 *tr_pad, hipStream_t s)
+{
+    const dh_geom &g = a.g;
+    size_t np = (size_t)a.p.pxy * a.p.pz;
+    dh_timed t("pack_tr", s);
+    hipLaunchKernelGGL(k_pack_tr, dim3((unsigned)std::min<size_t>(grid_for(np), 4096)),
+                       dim3(kBlock), 0, s, terma, density, g.nx, g.ny, g.nz, a.p, tr_pad);
+    DH_HIP(hipGetLastError());
+}
+
+void dh_launch_ccc_counted(const dh_ccc_args &a, const float2 *tr_pad, const float *electron,
+                           const int *list, const int *counter, int64_t max_n, float *dose,
+                           hipStream_t s)
+{
+    if (max_n <= 0) return;
+    dh_timed t("ccc_list", s);
+    const unsigned grid = (unsigned)std::min<int64_t>((max_n * 4 + kBlock - 1) / kBlock, 2048);
+    hipLaunchKernelGGL(k_ccc, dim3(std::max(grid, 1u)), dim3(kBlock), 0, s, a, tr_pad, electron, list,
+                       counter, dose);
+    DH_HIP(hipGetLastError());
+}

@@ -29,8 +29,10 @@ __device__ static inline uint16_t f2bf(float f)
 }
 
 struct conv_args {
-    const uint16_t *x;       // [N][D][H][W][cin] bf16
-    const uint16_t *w;       // [taps][cout_pad][cin] bf16
+    const uint16_t *xa;      // source A: [N][D>>up][H>>up][W>>up][ca] bf16
+    const uint16_t *xb;      // source B: [N][D][H][W][cb] bf16 or null (cb = 0)
+    int ca, cb, up;          // input channel c < ca reads A (nearest-upsampled if up), else B
+    const uint16_t *w;       // [taps][cout_pad][cin] bf16, cin = ca + cb
     const float *bias;       // [cout] or null
     uint16_t *y;             // [N][D][H][W][cout] bf16 or null
     float *y32;              // same, fp32, or null
@@ -38,20 +40,24 @@ struct conv_args {
     float slope;
 };
 
-// Forward convolution, k = 1 or 3, stride 1, "same" zero padding.
-__global__ void __launch_bounds__(256)
+// Forward convolution, k = 1 or 3, stride 1, "same" zero padding, with the
+// concatenation [upsample(A), B] of a U-Net decoder read in place.
+// A workgroup of KS waves owns 32 output voxels x (32 * NCO) output channels;
+// its waves split the K loop (taps x 16-channel slices) and add their FP32
+// partial tiles through LDS.  Each wave keeps NCO accumulator tiles so one A
+// fragment feeds NCO MFMAs, and fetches the next K step's fragments before
+// issuing the current one's MFMAs.
+template <int NCO, int KS>
+__global__ void __launch_bounds__(64 * KS)
 k_conv3d(conv_args a)
 {
-    const int lane = __lane_id();
-    const int wave = (blockIdx.x * 256 + threadIdx.x) >> 6;
+    extern __shared__ float red[];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int nblk_co = a.cout_pad / (32 * NCO);
+    const int cob = (int)(blockIdx.x % (unsigned)nblk_co);
+    const int64_t vb = blockIdx.x / (unsigned)nblk_co;
     const int64_t nvox = (int64_t)a.N * a.D * a.H * a.W;
-    const int nblk_co = a.cout_pad / 32;
-    const int64_t nblk_v = (nvox + 31) / 32;
-    if (wave >= nblk_v * nblk_co) return;
-    const int cob = wave % nblk_co;
-    const int64_t vb = wave / nblk_co;
     const int row = lane & 31, kh = lane >> 5;
-    // this lane's A row: voxel vb*32 + row
     const int64_t v = vb * 32 + row;
     const bool vin = v < nvox;
     const int64_t vv = vin ? v : nvox - 1;
@@ -60,43 +66,115 @@ k_conv3d(conv_args a)
     const int y = (int)(t % a.H); t /= a.H;
     const int z = (int)(t % a.D);
     const int n = (int)(t / a.D);
-    const int taps = a.ksize * a.ksize * a.ksize;
-    const int rad = a.ksize / 2;
-    dh_f32x16 acc;
+    const int Da = a.D >> a.up, Ha = a.H >> a.up, Wa = a.W >> a.up;
+    const int nsl = a.cin >> 4;
+    const int kk = a.ksize * a.ksize;
+    const int rad = a.ksize >> 1;
+    const int KI = kk * a.ksize * nsl;
+    const int k0 = (int)((int64_t)KI * wid / KS), k1 = (int)((int64_t)KI * (wid + 1) / KS);
+    const int co0 = cob * 32 * NCO + row;
+
+    dh_f32x16 acc[NCO];
 #pragma unroll
-    for (int r = 0; r < 16; r++) acc[r] = 0.0f;
-    const int co = cob * 32 + row;       // this lane's B column
-    for (int tp = 0; tp < taps; tp++) {
-        const int dz = tp / (a.ksize * a.ksize) - rad;
-        const int dy = (tp / a.ksize) % a.ksize - rad;
-        const int dx = tp % a.ksize - rad;
+    for (int j = 0; j < NCO; j++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) acc[j][r] = 0.0f;
+
+    // fragments of K step ki: A (this lane's voxel, 8 channels) and NCO B
+    auto fetch = [&](int ki, uint4 &av, uint4 *bv) {
+        const int tp = ki / nsl;
+        const int c0 = (ki - tp * nsl) * 16;
+        const int dz = tp / kk - rad, dy = (tp / a.ksize) % a.ksize - rad, dx = tp % a.ksize - rad;
         const int zz = z + dz, yy = y + dy, xx = x + dx;
         const bool ok = vin && zz >= 0 && zz < a.D && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
-        const int64_t src = ok ? ((((int64_t)n * a.D + zz) * a.H + yy) * a.W + xx) : 0;
-        const uint16_t *xp = a.x + src * a.cin;
-        const uint16_t *wp = a.w + ((int64_t)tp * a.cout_pad + co) * a.cin;
-        for (int c0 = 0; c0 < a.cin; c0 += 16) {
-            const int cc = c0 + kh * 8;
-            uint4 av = ok ? *(const uint4 *)(xp + cc) : make_uint4(0, 0, 0, 0);
-            uint4 bv = *(const uint4 *)(wp + cc);
-            dh_bf16x8 A, B;
+        av = make_uint4(0, 0, 0, 0);
+        if (c0 < a.ca) {
+            const int64_t src = (((int64_t)n * Da + (zz >> a.up)) * Ha + (yy >> a.up)) * Wa + (xx >> a.up);
+            if (ok) av = *(const uint4 *)(a.xa + src * a.ca + c0 + kh * 8);
+        } else {
+            const int64_t src = (((int64_t)n * a.D + zz) * a.H + yy) * a.W + xx;
+            if (ok) av = *(const uint4 *)(a.xb + src * a.cb + (c0 - a.ca) + kh * 8);
+        }
+        const uint16_t *wp = a.w + ((int64_t)tp * a.cout_pad + co0) * a.cin + c0 + kh * 8;
+#pragma unroll
+        for (int j = 0; j < NCO; j++) bv[j] = *(const uint4 *)(wp + (int64_t)j * 32 * a.cin);
+    };
+
+    if (k0 < k1) {
+        uint4 av, bv[NCO];
+        fetch(k0, av, bv);
+        for (int ki = k0; ki < k1; ki++) {
+            uint4 an = av, bn[NCO];
+#pragma unroll
+            for (int j = 0; j < NCO; j++) bn[j] = bv[j];
+            if (ki + 1 < k1) fetch(ki + 1, an, bn);
+            dh_bf16x8 A;
             __builtin_memcpy(&A, &av, 16);
-            __builtin_memcpy(&B, &bv, 16);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, B, acc, 0, 0, 0);
+#pragma unroll
+            for (int j = 0; j < NCO; j++) {
+                dh_bf16x8 B;
+                __builtin_memcpy(&B, &bv[j], 16);
+                acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, B, acc[j], 0, 0, 0);
+            }
+            av = an;
+#pragma unroll
+            for (int j = 0; j < NCO; j++) bv[j] = bn[j];
         }
     }
+
     // D layout: lane l, register r -> row 8*(r/4) + 4*(l/32) + (r%4), column l%32
-    const int col = cob * 32 + row;
-    const float b = (a.bias && col < a.cout) ? a.bias[col] : 0.0f;
-#pragma unroll
-    for (int r = 0; r < 16; r++) {
-        const int m = 8 * (r >> 2) + 4 * kh + (r & 3);
+    auto emit = [&](int j, int r, int l, float s) {
+        const int m = 8 * (r >> 2) + 4 * (l >> 5) + (r & 3);
+        const int col = cob * 32 * NCO + j * 32 + (l & 31);
         const int64_t ov = vb * 32 + m;
-        if (ov >= nvox || col >= a.cout) continue;
-        float o = acc[r] + b;
+        if (ov >= nvox || col >= a.cout) return;
+        float o = s + (a.bias ? a.bias[col] : 0.0f);
         if (a.act) o = o > 0.0f ? o : o * a.slope;
         if (a.y) a.y[ov * a.cout + col] = f2bf(o);
         if (a.y32) a.y32[ov * a.cout + col] = o;
+    };
+    if (KS == 1) {
+#pragma unroll
+        for (int j = 0; j < NCO; j++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) emit(j, r, lane, acc[j][r]);
+        return;
+    }
+    constexpr int TILE = NCO * 16 * 64;
+#pragma unroll
+    for (int j = 0; j < NCO; j++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) red[wid * TILE + (j * 16 + r) * 64 + lane] = acc[j][r];
+    __syncthreads();
+    for (int e = threadIdx.x; e < TILE; e += 64 * KS) {
+        float s = 0.0f;
+#pragma unroll
+        for (int w = 0; w < KS; w++) s += red[w * TILE + e];
+        emit(e >> 10, (e >> 6) & 15, e & 63, s);
+    }
+}
+
+// avg-pool 2x2x2 on channels-last bf16 [N][D][H][W][C] -> [N][D/2][H/2][W/2][C]
+__global__ void k_pool2(const uint16_t *__restrict__ x, int N, int D, int H, int W, int C,
+                        uint16_t *__restrict__ y)
+{
+    const int d = D >> 1, h = H >> 1, w = W >> 1;
+    const int64_t n = (int64_t)N * d * h * w * C;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int c = (int)(i % C);
+        int64_t r = i / C;
+        const int xx = (int)(r % w); r /= w;
+        const int yy = (int)(r % h); r /= h;
+        const int zz = (int)(r % d);
+        const int nn = (int)(r / d);
+        float s = 0.0f;
+#pragma unroll
+        for (int o = 0; o < 8; o++) {
+            const int64_t src = ((((int64_t)nn * D + 2 * zz + (o >> 2)) * H + 2 * yy + ((o >> 1) & 1)) * W +
+                                 2 * xx + (o & 1)) * C + c;
+            s += bf2f(x[src]);
+        }
+        y[i] = f2bf(s * 0.125f);
     }
 }
 
@@ -232,34 +310,341 @@ k_gamma(const float *__restrict__ eval, const float *__restrict__ ref, int nx, i
     out[v] = sqrtf(best);
 }
 
+// ---- surrogate input / output and the checked fallback ----------------------
+
+// max of a non-negative volume via the ordered bits of non-negative floats
+__global__ void k_max_nonneg(const float *__restrict__ v, int64_t n, float *__restrict__ out)
+{
+    float m = 0.0f;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        m = fmaxf(m, v[i]);
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    if (__lane_id() == 0 && m > 0.0f) atomicMax((unsigned int *)out, __float_as_uint(m));
+}
+
+__global__ void k_fill_f32(float *p, int n, float v)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
+__global__ void k_fill_i32(int *p, int n, int v)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
+// Input cells on the half-resolution grid of the 16-padded volume: channels
+// 0-7 TERMA / max, 8-15 density of the 2x2x2 sub-voxels (sub = 4 dz + 2 dy +
+// dx), 16-18 the beam direction, 19-31 zero.  One thread writes 8 channels.
+__global__ void k_sur_pack(const float *__restrict__ terma, const float *__restrict__ dens, int nz, int ny,
+                           int nx, int cz, int cy, int cx, const float *__restrict__ tmax, float d0,
+                           float d1, float d2, uint16_t *__restrict__ out)
+{
+    const int64_t n = (int64_t)cz * cy * cx * 4;
+    const float scale = fmaxf(*tmax, 1e-12f);
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int grp = (int)(i & 3);
+        const int64_t c = i >> 2;
+        const int ci = (int)(c % cx);
+        const int64_t r = c / cx;
+        const int cj = (int)(r % cy), ck = (int)(r / cy);
+        uint16_t h[8];
+        if (grp < 2) {
+#pragma unroll
+            for (int sub = 0; sub < 8; sub++) {
+                const int k = 2 * ck + (sub >> 2), j = 2 * cj + ((sub >> 1) & 1), ii = 2 * ci + (sub & 1);
+                const bool in = k < nz && j < ny && ii < nx;
+                const int64_t src = ((int64_t)k * ny + j) * nx + ii;
+                const float val = grp == 0 ? (in ? terma[src] / scale : 0.0f) : (in ? dens[src] : DH_AIR_DENSITY);
+                h[sub] = f2bf(val);
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; e++) h[e] = 0;
+            if (grp == 2) { h[0] = f2bf(d0); h[1] = f2bf(d1); h[2] = f2bf(d2); }
+        }
+        uint4 o;
+        __builtin_memcpy(&o, h, 16);
+        *(uint4 *)(out + c * 32 + grp * 8) = o;
+    }
+}
+
+// Surrogate dose: the network's sub-voxel outputs (fp32 [cells][16],
+// channels 0-7) back on the voxel grid, scaled by max TERMA and the
+// network's output scale, plus the electron term, times MU calibration;
+// zero where TERMA <= 0.01 or the value is not positive.  Also the dose max.
+__global__ void k_sur_finish(const float *__restrict__ y32, int cy, int cx, const float *__restrict__ tmax,
+                             float out_scale, const float *__restrict__ terma, const float *__restrict__ electron,
+                             float mu_scale, int nz, int ny, int nx, float *__restrict__ dose,
+                             float *__restrict__ dmax)
+{
+    const int64_t n = (int64_t)nz * ny * nx;
+    const float s = fmaxf(*tmax, 1e-12f) * out_scale;
+    float m = 0.0f;
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(v % nx);
+        const int64_t r = v / nx;
+        const int j = (int)(r % ny), k = (int)(r / ny);
+        const int64_t c = ((int64_t)(k >> 1) * cy + (j >> 1)) * cx + (i >> 1);
+        const int sub = 4 * (k & 1) + 2 * (j & 1) + (i & 1);
+        const float sup = y32[c * 16 + sub] * s;
+        const float d = (sup + electron[v]) * mu_scale;
+        const float o = (terma[v] > 0.01f && d > 0.0f) ? d : 0.0f;
+        dose[v] = o;
+        m = fmaxf(m, o);
+    }
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    if (__lane_id() == 0 && m > 0.0f) atomicMax((unsigned int *)dmax, __float_as_uint(m));
+}
+
+// block-aggregated append of the flagged items of this block to list
+__device__ static inline void block_append(bool flag, int item, int *__restrict__ list, int *__restrict__ counter)
+{
+    __shared__ int s_cnt[4];
+    __shared__ int s_base;
+    const uint64_t mk = __builtin_amdgcn_ballot_w64(flag);
+    const int lane = __lane_id(), wave = threadIdx.x >> 6;
+    if (lane == 0) s_cnt[wave] = __popcll(mk);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int w = 0; w < 4; w++) { const int c = s_cnt[w]; s_cnt[w] = tot; tot += c; }
+        s_base = tot ? atomicAdd(counter, tot) : 0;
+    }
+    __syncthreads();
+    if (flag) list[s_base + s_cnt[wave] + __popcll(mk & ((1ull << lane) - 1ull))] = item;
+}
+
+// lattice voxels (every stride-th voxel along each axis) with TERMA > 0.01
+__global__ void __launch_bounds__(256)
+k_sur_lattice(const float *__restrict__ terma, int nz, int ny, int nx, int stride, int lz, int ly, int lx,
+              int *__restrict__ lat_ok, int *__restrict__ list, int *__restrict__ counter)
+{
+    const int64_t nc = (int64_t)lz * ly * lx;
+    const int64_t c = blockIdx.x * (int64_t)256 + threadIdx.x;
+    bool act = false;
+    int v = 0;
+    if (c < nc) {
+        const int ci = (int)(c % lx);
+        const int64_t r = c / lx;
+        const int cj = (int)(r % ly), ck = (int)(r / ly);
+        v = (int)(((int64_t)ck * stride * ny + (int64_t)cj * stride) * nx + (int64_t)ci * stride);
+        act = !(terma[v] <= 0.01f);
+        lat_ok[c] = act ? 1 : 0;
+    }
+    block_append(act, v, list, counter);
+}
+
+// per lattice cell: the surrogate missed the plain value by more than tol * max
+__global__ void k_sur_flag(const int *__restrict__ lat_ok, const int *__restrict__ list,
+                           const int *__restrict__ counter, const float *__restrict__ dose,
+                           const float *__restrict__ plain, const float *__restrict__ dmax, float tol,
+                           int nz, int ny, int nx, int stride, int lz, int ly, int lx, int *__restrict__ bad)
+{
+    const int64_t nc = (int64_t)lz * ly * lx;
+    const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int ci = (int)(c % lx);
+    const int64_t r = c / lx;
+    const int cj = (int)(r % ly), ck = (int)(r / ly);
+    const int64_t v = ((int64_t)ck * stride * ny + (int64_t)cj * stride) * nx + (int64_t)ci * stride;
+    const float thr = tol * fmaxf(*dmax, 1e-12f);
+    bad[c] = (lat_ok[c] && fabsf(dose[v] - plain[v]) > thr) ? 1 : 0;
+}
+
+// voxels with TERMA > 0.01 whose cell has no checked lattice voxel, or whose
+// cell or a neighbouring cell failed the check
+__global__ void __launch_bounds__(256)
+k_sur_fallback(const float *__restrict__ terma, int nz, int ny, int nx, int stride, int lz, int ly, int lx,
+               const int *__restrict__ lat_ok, const int *__restrict__ bad, int *__restrict__ list,
+               int *__restrict__ counter)
+{
+    const int64_t n = (int64_t)nz * ny * nx;
+    const int64_t v = blockIdx.x * (int64_t)256 + threadIdx.x;
+    bool fb = false;
+    if (v < n && !(terma[v] <= 0.01f)) {
+        const int i = (int)(v % nx);
+        const int64_t r = v / nx;
+        const int j = (int)(r % ny), k = (int)(r / ny);
+        const int ci = i / stride, cj = j / stride, ck = k / stride;
+        fb = !lat_ok[((int64_t)ck * ly + cj) * lx + ci];
+        for (int dk = -1; dk <= 1 && !fb; dk++)
+            for (int dj = -1; dj <= 1 && !fb; dj++)
+                for (int di = -1; di <= 1 && !fb; di++) {
+                    const int a = ck + dk, b = cj + dj, e = ci + di;
+                    if (a < 0 || a >= lz || b < 0 || b >= ly || e < 0 || e >= lx) continue;
+                    fb = bad[((int64_t)a * ly + b) * lx + e] != 0;
+                }
+    }
+    block_append(fb, (int)v, list, counter);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
-extern "C" int dh_nn_conv3d_dev(const void *x, int32_t n, int32_t d, int32_t h, int32_t w,
-                                int32_t cin, const void *wt, const float *bias, int32_t cout,
-                                int32_t cout_pad, int32_t ksize, int32_t act, float slope, void *y,
-                                float *y32, void *stream)
+namespace {
+
+template <int NCO, int KS>
+void launch_conv(const conv_args &a, int64_t items, hipStream_t s)
+{
+    const size_t lds = KS > 1 ? (size_t)KS * NCO * 16 * 64 * sizeof(float) : 0;
+    hipLaunchKernelGGL((k_conv3d<NCO, KS>), dim3((unsigned)items), dim3(64 * KS), lds, s, a);
+}
+
+// (NCO, KS) for a layer: the widest output tile that still gives a few
+// thousand waves, then split K until the grid fills the chip (at least
+// eight K steps per wave; LDS of the partial tiles <= 64 KiB).
+void conv_shape(int64_t nvox, int cout_pad, int ki, int &nco, int &ks)
+{
+    const int64_t nvb = (nvox + 31) / 32;
+    const int ntile = cout_pad / 32;
+    nco = 1;
+    for (int c : {4, 2}) {
+        if (ntile % c == 0 && nvb * (ntile / c) >= 2048) { nco = c; break; }
+    }
+    ks = 1;
+    while (ks < 16 && nvb * (ntile / nco) * ks < 8192 && ki / (2 * ks) >= 8 && 2 * ks * nco <= 16) ks *= 2;
+}
+
+}  // namespace
+
+extern "C" int dh_nn_conv3d_dev(const void *xa, int32_t ca, int32_t up_a, const void *xb, int32_t cb,
+                                int32_t n, int32_t d, int32_t h, int32_t w, const void *wt,
+                                const float *bias, int32_t cout, int32_t cout_pad, int32_t ksize,
+                                int32_t act, float slope, void *y, float *y32, int32_t nco,
+                                int32_t ksplit, void *stream)
 {
     return dh_guarded([&] {
-        DH_REQUIRE(x && wt && (y || y32), "null argument");
+        DH_REQUIRE(xa && wt && (y || y32), "null argument");
         DH_REQUIRE(ksize == 1 || ksize == 3, "kernel size must be 1 or 3");
-        DH_REQUIRE(cin > 0 && cin % 16 == 0, "input channels must be a positive multiple of 16");
+        DH_REQUIRE(ca > 0 && ca % 16 == 0 && cb >= 0 && cb % 16 == 0 && (cb == 0 || xb),
+                   "input channels must be positive multiples of 16");
+        DH_REQUIRE(up_a == 0 || up_a == 1, "up_a must be 0 or 1");
+        DH_REQUIRE(!up_a || (d % 2 == 0 && h % 2 == 0 && w % 2 == 0), "upsampled input needs even sizes");
         DH_REQUIRE(cout > 0 && cout_pad >= cout && cout_pad % 32 == 0, "bad output channel padding");
         DH_REQUIRE(n > 0 && d > 0 && h > 0 && w > 0, "bad shape");
         conv_args a;
-        a.x = (const uint16_t *)x; a.w = (const uint16_t *)wt; a.bias = bias;
+        a.xa = (const uint16_t *)xa; a.xb = (const uint16_t *)xb; a.ca = ca; a.cb = cb; a.up = up_a;
+        a.w = (const uint16_t *)wt; a.bias = bias;
         a.y = (uint16_t *)y; a.y32 = y32;
-        a.N = n; a.D = d; a.H = h; a.W = w; a.cin = cin; a.cout = cout; a.cout_pad = cout_pad;
+        a.N = n; a.D = d; a.H = h; a.W = w; a.cin = ca + cb; a.cout = cout; a.cout_pad = cout_pad;
         a.ksize = ksize; a.act = act; a.slope = slope;
         const int64_t nvox = (int64_t)n * d * h * w;
-        const int64_t waves = ((nvox + 31) / 32) * (cout_pad / 32);
-        const unsigned grid = (unsigned)((waves + 3) / 4);
-        dh_timed t("nn_conv3d", (hipStream_t)stream);
-        hipLaunchKernelGGL(k_conv3d, dim3(grid), dim3(256), 0, (hipStream_t)stream, a);
+        const int ki = ksize * ksize * ksize * (a.cin / 16);
+        int c = nco, k = ksplit;
+        if (c <= 0 || k <= 0) conv_shape(nvox, cout_pad, ki, c, k);
+        DH_REQUIRE((cout_pad / 32) % c == 0, "output tile does not divide the padded channels");
+        const int64_t items = ((nvox + 31) / 32) * (cout_pad / (32 * c));
+        hipStream_t s = (hipStream_t)stream;
+        dh_timed t("nn_conv3d", s);
+        const int key = c * 100 + k;
+        switch (key) {
+        case 101: launch_conv<1, 1>(a, items, s); break;
+        case 102: launch_conv<1, 2>(a, items, s); break;
+        case 104: launch_conv<1, 4>(a, items, s); break;
+        case 108: launch_conv<1, 8>(a, items, s); break;
+        case 116: launch_conv<1, 16>(a, items, s); break;
+        case 201: launch_conv<2, 1>(a, items, s); break;
+        case 202: launch_conv<2, 2>(a, items, s); break;
+        case 204: launch_conv<2, 4>(a, items, s); break;
+        case 208: launch_conv<2, 8>(a, items, s); break;
+        case 401: launch_conv<4, 1>(a, items, s); break;
+        case 402: launch_conv<4, 2>(a, items, s); break;
+        case 404: launch_conv<4, 4>(a, items, s); break;
+        default: DH_REQUIRE(false, "unsupported (output tiles, K split) pair");
+        }
         DH_HIP(hipGetLastError());
     });
+}
+
+extern "C" int dh_nn_pool2_dev(const void *x, int32_t n, int32_t d, int32_t h, int32_t w, int32_t c,
+                               void *y, void *stream)
+{
+    return dh_guarded([&] {
+        DH_REQUIRE(x && y && n > 0 && c > 0, "bad argument");
+        DH_REQUIRE(d > 1 && h > 1 && w > 1 && d % 2 == 0 && h % 2 == 0 && w % 2 == 0, "sizes must be even");
+        const int64_t total = (int64_t)n * (d / 2) * (h / 2) * (w / 2) * c;
+        const unsigned grid = (unsigned)std::min<int64_t>((total + 255) / 256, 8192);
+        hipLaunchKernelGGL(k_pool2, dim3(grid), dim3(256), 0, (hipStream_t)stream, (const uint16_t *)x, n, d,
+                           h, w, c, (uint16_t *)y);
+        DH_HIP(hipGetLastError());
+    });
+}
+
+extern "C" int dh_sur_pack_dev(int32_t nz, int32_t ny, int32_t nx, const float *terma, const float *density,
+                               const float dir[3], void *x_out, float *tmax_out, void *stream)
+{
+    return dh_guarded([&] {
+        DH_REQUIRE(terma && density && dir && x_out && tmax_out, "null argument");
+        DH_REQUIRE(nz > 0 && ny > 0 && nx > 0, "bad shape");
+        hipStream_t s = (hipStream_t)stream;
+        const int64_t n = (int64_t)nz * ny * nx;
+        const int cz = ((nz + 15) / 16) * 8, cy = ((ny + 15) / 16) * 8, cx = ((nx + 15) / 16) * 8;
+        hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, s, tmax_out, 1, 0.0f);
+        hipLaunchKernelGGL(k_max_nonneg, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 2048)), dim3(256), 0, s,
+                           terma, n, tmax_out);
+        const int64_t items = (int64_t)cz * cy * cx * 4;
+        hipLaunchKernelGGL(k_sur_pack, dim3((unsigned)std::min<int64_t>((items + 255) / 256, 8192)), dim3(256), 0, s,
+                           terma, density, nz, ny, nx, cz, cy, cx, (const float *)tmax_out, dir[0], dir[1], dir[2],
+                           (uint16_t *)x_out);
+        DH_HIP(hipGetLastError());
+    });
+}
+
+extern "C" int dh_sur_finish_dev(int32_t nz, int32_t ny, int32_t nx, const float *y32, const float *tmax,
+                                 float out_scale, const float *terma, const float *electron, float mu_scale,
+                                 float *dose_out, float *dmax_out, void *stream)
+{
+    return dh_guarded([&] {
+        DH_REQUIRE(y32 && tmax && terma && electron && dose_out && dmax_out, "null argument");
+        DH_REQUIRE(nz > 0 && ny > 0 && nx > 0, "bad shape");
+        hipStream_t s = (hipStream_t)stream;
+        const int64_t n = (int64_t)nz * ny * nx;
+        const int cy = ((ny + 15) / 16) * 8, cx = ((nx + 15) / 16) * 8;
+        hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, s, dmax_out, 1, 0.0f);
+        hipLaunchKernelGGL(k_sur_finish, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 8192)), dim3(256), 0, s,
+                           y32, cy, cx, tmax, out_scale, terma, electron, mu_scale, nz, ny, nx, dose_out, dmax_out);
+        DH_HIP(hipGetLastError());
+    });
+}
+
+int64_t dh_sur_check_work_len(int nz, int ny, int nx, int stride)
+{
+    const int64_t nc = (int64_t)((nz + stride - 1) / stride) * ((ny + stride - 1) / stride) *
+                       ((nx + stride - 1) / stride);
+    return 4 + 3 * nc + (int64_t)nz * ny * nx;
+}
+
+void dh_launch_sur_check(const dh_ccc_args &ca, const float *terma, const float *density,
+                         const float *electron, float *dose, const float *dmax, int stride,
+                         float tol, int *work, float *plain, float2 *tr_pad, hipStream_t s)
+{
+    const dh_geom &g = ca.g;
+    const int nz = g.nz, ny = g.ny, nx = g.nx;
+    const int lz = (nz + stride - 1) / stride, ly = (ny + stride - 1) / stride, lx = (nx + stride - 1) / stride;
+    const int64_t nc = (int64_t)lz * ly * lx, nv = (int64_t)nz * ny * nx;
+    int *counter = work, *lat_ok = work + 4, *bad = lat_ok + nc, *lat_list = bad + nc, *fb_list = lat_list + nc;
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, counter, 4, 0);
+    dh_launch_pack_tr(ca, terma, density, tr_pad, s);
+    {
+        dh_timed t("sur_lattice", s);
+        hipLaunchKernelGGL(k_sur_lattice, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, terma, nz, ny, nx,
+                           stride, lz, ly, lx, lat_ok, lat_list, counter);
+    }
+    dh_launch_ccc_counted(ca, tr_pad, electron, lat_list, counter, nc, plain, s);
+    hipLaunchKernelGGL(k_sur_flag, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, (const int *)lat_ok,
+                       (const int *)lat_list, (const int *)counter, (const float *)dose, (const float *)plain,
+                       dmax, tol, nz, ny, nx, stride, lz, ly, lx, bad);
+    {
+        dh_timed t("sur_fallback", s);
+        hipLaunchKernelGGL(k_sur_fallback, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, s, terma, nz, ny, nx,
+                           stride, lz, ly, lx, (const int *)lat_ok, (const int *)bad, fb_list, counter + 1);
+    }
+    dh_launch_ccc_counted(ca, tr_pad, electron, fb_list, counter + 1, nv, dose, s);
+    DH_HIP(hipGetLastError());
 }
 
 extern "C" int dh_nn_conv3d_wgrad_dev(const void *x, const void *dy, int32_t n, int32_t d,

@@ -1,13 +1,18 @@
 """Learned CCCS surrogate with plain-CCCS fallback.
 
 A small 3-D convolutional encoder-decoder (U-Net, widths 16/32/64/128, about
-1.5 M parameters) predicts the collapsed-cone dose of one photon beam from
-its TERMA, density and electron-contamination volumes.  The network runs on
-a 2x average-pooled grid (a 128^3 volume becomes a 64^3 patch) and its output
-is trilinearly upsampled.  Every convolution (forward, input gradient and
-weight gradient) is a hand-written BF16 MFMA kernel of libdosehip.so with
-FP32 accumulation; torch supplies memory, pooling/upsampling/concatenation
-glue and the Adam update of the FP32 master weights.
+1.5 M parameters) predicts the kernel-superposition part of the collapsed-
+cone dose of one photon beam from its TERMA and density volumes; the
+electron-contamination term and the MU calibration are added exactly, as the
+plain CCCS adds them (IMRTClasses.cu:338-346).  The network works on a
+half-resolution grid without losing input detail: each 2x2x2 block of
+(TERMA, density) becomes 16 channels of one cell (space-to-depth), three more
+channels carry the beam direction, and 8 output channels are the 2x2x2 dose
+sub-voxels of the cell (depth-to-space).  A 128^3 volume is one 64^3 patch.
+Every convolution (forward, input gradient and weight gradient) is a
+hand-written BF16 MFMA kernel of libdosehip.so with FP32 accumulation; torch
+supplies memory, pooling/upsampling/concatenation glue and the Adam update
+of the FP32 master weights.
 
 Trust, on the timed path: the plain CCCS (`k_ccc`) is evaluated on a sparse
 lattice of voxels (every `stride`-th voxel along each axis among voxels with
@@ -22,7 +27,9 @@ No reference counterpart: DoseCUDA computes only the plain CCCS
 from __future__ import annotations
 
 import ctypes as C
+import json
 import math
+import os
 import time
 
 import numpy as np
@@ -32,7 +39,9 @@ import torch.nn.functional as F
 from . import _lib, engine
 
 SLOPE = 0.1          # LeakyReLU negative slope
-IN_CH = 16           # input channels (T, rho, E + zero padding to a multiple of 16)
+IN_CH = 32           # 8 TERMA + 8 density sub-voxels, 3 beam direction, 13 zero
+N_SUB = 8            # output sub-voxels per cell
+AIR = 0.0012         # density of the padding outside the volume (IMRTClasses.cuh:6)
 
 
 def _st(stream=None):
@@ -58,15 +67,32 @@ def prep_weights(w32: torch.Tensor, mode: int) -> torch.Tensor:
     return out
 
 
-def conv3d(x: torch.Tensor, wprep: torch.Tensor, bias, cout: int, ksize: int, act: bool,
-           want_bf16=True, want_f32=False):
-    n, d, h, w, cin = x.shape
-    y = torch.empty((n, d, h, w, cout), dtype=torch.bfloat16, device=x.device) if want_bf16 else None
-    y32 = torch.empty((n, d, h, w, cout), dtype=torch.float32, device=x.device) if want_f32 else None
-    _lib.check(_lib.load().dh_nn_conv3d_dev(_lib.ptr(x), n, d, h, w, cin, _lib.ptr(wprep),
-                                            _lib.ptr(bias), cout, wprep.shape[1], ksize,
-                                            1 if act else 0, SLOPE, _lib.ptr(y), _lib.ptr(y32), _st()))
+def conv3d(xa: torch.Tensor, wprep: torch.Tensor, bias, cout: int, ksize: int, act: bool,
+           want_bf16=True, want_f32=False, xb=None, up=False, nco=0, ksplit=0):
+    """Convolution of the channel concatenation [upsample(xa) if up else xa,
+    xb] (xb optional) on channels-last bf16 tensors."""
+    ca = xa.shape[-1]
+    if xb is not None:
+        n, d, h, w, cb = xb.shape
+    else:
+        n, d, h, w = xa.shape[:4]
+        if up:
+            d, h, w = 2 * d, 2 * h, 2 * w
+        cb = 0
+    y = torch.empty((n, d, h, w, cout), dtype=torch.bfloat16, device=xa.device) if want_bf16 else None
+    y32 = torch.empty((n, d, h, w, cout), dtype=torch.float32, device=xa.device) if want_f32 else None
+    _lib.check(_lib.load().dh_nn_conv3d_dev(_lib.ptr(xa), ca, 1 if up else 0, _lib.ptr(xb), cb, n, d, h, w,
+                                            _lib.ptr(wprep), _lib.ptr(bias), cout, wprep.shape[1], ksize,
+                                            1 if act else 0, SLOPE, _lib.ptr(y), _lib.ptr(y32), int(nco),
+                                            int(ksplit), _st()))
     return y, y32
+
+
+def pool2(x: torch.Tensor) -> torch.Tensor:
+    n, d, h, w, c = x.shape
+    y = torch.empty((n, d // 2, h // 2, w // 2, c), dtype=torch.bfloat16, device=x.device)
+    _lib.check(_lib.load().dh_nn_pool2_dev(_lib.ptr(x), n, d, h, w, c, _lib.ptr(y), _st()))
+    return y
 
 
 def wgrad(x: torch.Tensor, dy: torch.Tensor, ksize: int) -> torch.Tensor:
@@ -118,24 +144,32 @@ class Conv(torch.nn.Module):
         return _Conv.apply(x, self.w, self.b, self.ksize, self.act, self.out32)
 
 
-def _pool(x):
-    y = F.avg_pool3d(x.permute(0, 4, 1, 2, 3), 2)
-    return y.permute(0, 2, 3, 4, 1).contiguous()
+class _Pool(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x):
+        return pool2(x.contiguous())
+
+    @staticmethod
+    def backward(ctx, gy):
+        g = gy.float() * 0.125
+        g = g.repeat_interleave(2, 1).repeat_interleave(2, 2).repeat_interleave(2, 3)
+        return g.to(torch.bfloat16)
 
 
 def _up(x):
-    y = F.interpolate(x.permute(0, 4, 1, 2, 3), scale_factor=2, mode="nearest")
-    return y.permute(0, 2, 3, 4, 1).contiguous()
+    return x.repeat_interleave(2, 1).repeat_interleave(2, 2).repeat_interleave(2, 3)
 
 
 class SurrogateNet(torch.nn.Module):
-    """U-Net on channels-last bf16 tensors [N, D, H, W, 16] (D, H, W
-    multiples of 8); output fp32 [N, D, H, W] (normalised dose)."""
+    """U-Net on channels-last bf16 tensors [N, D, H, W, IN_CH] (D, H, W
+    multiples of 8); output fp32 [N, D, H, W, 16], of which channels 0-7 are
+    the 2x2x2 sub-voxel values (normalised superposition dose)."""
 
     def __init__(self, widths=(16, 32, 64, 128), seed=1004):
         super().__init__()
         g = torch.Generator().manual_seed(seed)
         a, b, c, d = widths
+        self.widths = tuple(widths)
         self.e0 = torch.nn.ModuleList([Conv(IN_CH, a, gen=g), Conv(a, a, gen=g)])
         self.e1 = torch.nn.ModuleList([Conv(a, b, gen=g), Conv(b, b, gen=g)])
         self.e2 = torch.nn.ModuleList([Conv(b, c, gen=g), Conv(c, c, gen=g)])
@@ -143,9 +177,11 @@ class SurrogateNet(torch.nn.Module):
         self.d2 = torch.nn.ModuleList([Conv(d + c, c, gen=g), Conv(c, c, gen=g)])
         self.d1 = torch.nn.ModuleList([Conv(c + b, b, gen=g), Conv(b, b, gen=g)])
         self.d0 = torch.nn.ModuleList([Conv(b + a, a, gen=g), Conv(a, a, gen=g)])
-        # 1x1 head; 16 output channels of which channel 0 is the dose, so the
-        # input-gradient convolution keeps a multiple-of-16 channel count
+        # 1x1 head with 16 outputs (a multiple of 16 keeps the input-gradient
+        # convolution on whole MFMA K slices); channels 0-7 are used
         self.head = Conv(a, 16, ksize=1, act=False, out32=True, gen=g)
+        # target normalisation: superposition dose / (max TERMA * out_scale)
+        self.register_buffer("out_scale", torch.ones(()))
 
     @staticmethod
     def _seq(mods, x):
@@ -154,48 +190,139 @@ class SurrogateNet(torch.nn.Module):
         return x
 
     def forward(self, x):
+        """Training forward (autograd; decoder concatenations materialised)."""
+        self._wf = None
         s0 = self._seq(self.e0, x)
-        s1 = self._seq(self.e1, _pool(s0))
-        s2 = self._seq(self.e2, _pool(s1))
-        t = self._seq(self.bt, _pool(s2))
+        s1 = self._seq(self.e1, _Pool.apply(s0))
+        s2 = self._seq(self.e2, _Pool.apply(s1))
+        t = self._seq(self.bt, _Pool.apply(s2))
         t = self._seq(self.d2, torch.cat([_up(t), s2], dim=-1))
         t = self._seq(self.d1, torch.cat([_up(t), s1], dim=-1))
         t = self._seq(self.d0, torch.cat([_up(t), s0], dim=-1))
-        return self.head(t)[..., 0]
+        return self.head(t)
+
+    def prepare(self):
+        """Cache the bf16 forward-layout weights for infer()."""
+        self._wf = {id(m): prep_weights(m.w.detach(), 0) for m in self.modules() if isinstance(m, Conv)}
+        return self
+
+    @torch.no_grad()
+    def infer(self, x):
+        """Inference forward with the cached weights: the decoder's
+        upsample + concatenation is read in place by the convolution kernel.
+        Same kernels and K split as forward(), so the same result."""
+        if getattr(self, "_wf", None) is None:
+            self.prepare()
+
+        def c(m, xa, xb=None, up=False):
+            y, y32 = conv3d(xa, self._wf[id(m)], m.b, m.w.shape[2], m.ksize, m.act, want_bf16=not m.out32,
+                            want_f32=m.out32, xb=xb, up=up)
+            return y32 if m.out32 else y
+
+        s0 = c(self.e0[1], c(self.e0[0], x))
+        s1 = c(self.e1[1], c(self.e1[0], pool2(s0)))
+        s2 = c(self.e2[1], c(self.e2[0], pool2(s1)))
+        t = c(self.bt[1], c(self.bt[0], pool2(s2)))
+        t = c(self.d2[1], c(self.d2[0], t, s2, up=True))
+        t = c(self.d1[1], c(self.d1[0], t, s1, up=True))
+        t = c(self.d0[1], c(self.d0[0], t, s0, up=True))
+        return c(self.head, t)
 
     def n_params(self):
-        return sum(p.numel() for p in self.parameters())
+        return sum(p.numel() for n, p in self.named_parameters())
 
-    def flops(self, vox):
-        """Multiply-add FLOPs of one forward pass over `vox` voxels of the
-        network's working grid (2 * taps * cin * cout per output voxel)."""
+    def flops(self, cells):
+        """FLOPs (2 per multiply-add) of one forward pass over `cells` cells
+        of the half-resolution grid: 2 * taps * cin * cout per output cell."""
         tot = 0.0
-        for name, mods, scale in (("e0", self.e0, 1), ("e1", self.e1, 8), ("e2", self.e2, 64),
-                                  ("bt", self.bt, 512), ("d2", self.d2, 64), ("d1", self.d1, 8),
-                                  ("d0", self.d0, 1)):
+        for mods, scale in ((self.e0, 1), (self.e1, 8), (self.e2, 64), (self.bt, 512),
+                            (self.d2, 64), (self.d1, 8), (self.d0, 1)):
             for m in mods:
                 t, ci, co = m.w.shape
-                tot += 2.0 * t * ci * co * vox / scale
+                tot += 2.0 * t * ci * co * cells / scale
         t, ci, co = self.head.w.shape
-        return tot + 2.0 * t * ci * co * vox
+        return tot + 2.0 * t * ci * co * cells
+
+
+def save(net: SurrogateNet, path: str):
+    from safetensors.torch import save_file
+    sd = {k: v.detach().float().cpu().contiguous() for k, v in net.state_dict().items()}
+    save_file(sd, path, metadata={"widths": ",".join(str(w) for w in net.widths),
+                                  "in_ch": str(IN_CH), "n_sub": str(N_SUB)})
+
+
+def load(path: str, device="cuda") -> SurrogateNet:
+    from safetensors import safe_open
+    from safetensors.torch import load_file
+    with safe_open(path, framework="pt") as f:
+        meta = f.metadata() or {}
+    if int(meta.get("in_ch", IN_CH)) != IN_CH or int(meta.get("n_sub", N_SUB)) != N_SUB:
+        raise ValueError(f"{path}: surrogate weights for another input/output layout ({meta})")
+    widths = tuple(int(w) for w in meta.get("widths", "16,32,64,128").split(","))
+    net = SurrogateNet(widths)
+    net.load_state_dict(load_file(path))
+    return net.to(device)
 
 
 # ---------------------------------------------------------------------------
 # inputs / targets
 # ---------------------------------------------------------------------------
-def make_input(terma: torch.Tensor, density: torch.Tensor, electron: torch.Tensor):
-    """Full-resolution fp32 (D, H, W) volumes -> channels-last bf16
-    [1, D/2, H/2, W/2, 16] on the 2x pooled grid and the TERMA scale."""
-    scale = torch.clamp(terma.max(), min=1e-6)
-    v = torch.stack([terma / scale, density, electron / scale], 0)[None]      # [1, 3, D, H, W]
-    v = F.avg_pool3d(v, 2)
-    x = torch.zeros((1,) + tuple(v.shape[2:]) + (IN_CH,), dtype=torch.bfloat16, device=v.device)
-    x[..., :3] = v[0].permute(1, 2, 3, 0).to(torch.bfloat16)
+def beam_direction(gantry, couch):
+    """Unit vector source -> isocentre in volume (x, y, z) axes, with the
+    engine's gantry and couch conventions (CudaClasses.cu:27-49)."""
+    ga = math.radians(math.fmod(gantry + 180.0, 360.0))
+    ta = math.radians(couch)
+    xg, yg = -math.sin(ga), math.cos(ga)
+    src = np.array([xg * math.cos(ta), yg, -xg * math.sin(ta)])
+    return (-src / np.linalg.norm(src)).astype(np.float32)
+
+
+def padded_shape(shape):
+    return tuple(_ceil(int(s), 16) for s in shape)
+
+
+def _pad(v, shape, value):
+    D, H, W = v.shape
+    if (D, H, W) == tuple(shape):
+        return v
+    return F.pad(v, (0, shape[2] - W, 0, shape[1] - H, 0, shape[0] - D), value=value)
+
+
+def _s2d(v):
+    """(D, H, W) -> (D/2, H/2, W/2, 8), sub-voxel (dz, dy, dx) -> 4 dz + 2 dy + dx."""
+    D, H, W = v.shape
+    return v.view(D // 2, 2, H // 2, 2, W // 2, 2).permute(0, 2, 4, 1, 3, 5).reshape(D // 2, H // 2, W // 2, 8)
+
+
+def _d2s(v):
+    d, h, w, _ = v.shape
+    return v.reshape(d, h, w, 2, 2, 2).permute(0, 3, 1, 4, 2, 5).reshape(2 * d, 2 * h, 2 * w)
+
+
+def make_input(terma: torch.Tensor, density: torch.Tensor, direction, out=None):
+    """fp32 (D, H, W) TERMA and density -> channels-last bf16 input
+    [D'/2, H'/2, W'/2, IN_CH] (D', H', W' = sizes padded to multiples of 16)
+    and the TERMA scale (its maximum)."""
+    shp = padded_shape(terma.shape)
+    scale = torch.clamp(terma.amax(), min=1e-12)
+    x = out if out is not None else torch.empty(tuple(s // 2 for s in shp) + (IN_CH,), dtype=torch.bfloat16,
+                                                device=terma.device)
+    x[..., 0:8] = _s2d(_pad(terma / scale, shp, 0.0))
+    x[..., 8:16] = _s2d(_pad(density, shp, AIR))
+    x[..., 16:19] = torch.as_tensor(direction, dtype=torch.float32, device=terma.device)
+    x[..., 19:] = 0
     return x, scale
 
 
 def dose_scale(model, mu):
     return float(np.float32(model.mu_calibration)) * float(mu)
+
+
+def superposition_target(dose, electron, terma, model, mu):
+    """Invert the plain CCCS's output formula: the kernel-superposition sum
+    per voxel (zero where TERMA <= 0.01 or where the plain dose was clamped)."""
+    sup = dose / dose_scale(model, mu) - electron
+    return torch.where((terma > 0.01) & (dose > 0), sup, torch.zeros_like(sup))
 
 
 def random_phantom(n, spacing, rng: np.random.Generator, device="cuda"):
@@ -224,15 +351,17 @@ def random_phantom(n, spacing, rng: np.random.Generator, device="cuda"):
 
 
 def generate(model, n_samples, n=128, spacing=2.5, seed=1004, log=None):
-    """Training data from the plain CCCS: random phantoms, gantry angles and
-    square fields of 3-20 cm; returns inputs [S, n/2, n/2, n/2, 16] bf16 and
-    normalised targets [S, n/2, n/2, n/2] bf16 on the pooled grid."""
+    """Training samples from the plain CCCS: random phantoms, gantry angles,
+    isocentre shifts of up to 2 cm and square fields of 3-20 cm.  Returns
+    inputs X [S, n/2, n/2, n/2, 16] bf16 (TERMA/max and density sub-voxels),
+    beam directions [S, 3] and targets Y [S, n/2, n/2, n/2, 8] fp32
+    (superposition dose / max TERMA, sub-voxels)."""
     from .beam_models import square_field_mlc
     rng = np.random.default_rng(seed)
-    half = n // 2
-    X = torch.empty((n_samples, half, half, half, IN_CH), dtype=torch.bfloat16, device="cuda")
-    Y = torch.empty((n_samples, half, half, half), dtype=torch.bfloat16, device="cuda")
-    iso = np.full(3, (n - 1) / 2 * spacing, np.float32)
+    h = n // 2
+    X = torch.empty((n_samples, h, h, h, 16), dtype=torch.bfloat16, device="cuda")
+    Y = torch.empty((n_samples, h, h, h, N_SUB), dtype=torch.float32, device="cuda")
+    dirs = torch.empty((n_samples, 3), dtype=torch.float32, device="cuda")
     terma = torch.empty((n, n, n), device="cuda")
     el = torch.empty_like(terma)
     dose = torch.empty_like(terma)
@@ -241,105 +370,174 @@ def generate(model, n_samples, n=128, spacing=2.5, seed=1004, log=None):
         rho = random_phantom(n, spacing, rng)
         ga = float(rng.uniform(0.0, 360.0))
         fs = float(rng.uniform(3.0, 20.0))
+        iso = (np.full(3, (n - 1) / 2 * spacing) + rng.uniform(-20.0, 20.0, 3)).astype(np.float32)
         mlc = square_field_mlc(model, fs, fs)
         engine.photon_dose(rho, spacing, np.zeros(3), iso, 100.0, ga, 0.0, 0.0, model, mlc,
                            out=dose, terma_out=terma, electron_out=el)
-        x, scale = make_input(terma, rho, el)
-        X[s] = x[0]
-        tgt = F.avg_pool3d((dose / (dose_scale(model, 100.0) * scale))[None, None], 2)[0, 0]
-        Y[s] = tgt.to(torch.bfloat16)
-        if log and (s + 1) % max(1, n_samples // 10) == 0:
+        scale = torch.clamp(terma.amax(), min=1e-12)
+        X[s, ..., 0:8] = _s2d(terma / scale)
+        X[s, ..., 8:16] = _s2d(rho)
+        dirs[s] = torch.from_numpy(beam_direction(ga, 0.0))
+        Y[s] = _s2d(superposition_target(dose, el, terma, model, 100.0) / scale)
+        if log and (s + 1) % max(1, n_samples // 4) == 0:
             log(f"generated {s + 1}/{n_samples} samples ({time.time() - t0:.1f} s)")
-    return X, Y
+    return X, dirs, Y
 
 
-def train(net: SurrogateNet, X, Y, steps=2000, batch=8, crop=32, lr=2e-3, seed=1004, log=None):
-    """Adam on the FP32 master weights; random crop^3 patches of the pooled
-    training volumes; loss = mean squared error weighted towards dose."""
+def assemble(X, dirs):
+    """[B, d, h, w, 16] + [B, 3] -> network input [B, d, h, w, IN_CH]."""
+    B = X.shape[0]
+    x = torch.zeros(X.shape[:4] + (IN_CH,), dtype=torch.bfloat16, device=X.device)
+    x[..., 0:16] = X
+    x[..., 16:19] = dirs.view(B, 1, 1, 1, 3).to(torch.bfloat16)
+    return x
+
+
+def train(net: SurrogateNet, model, n_samples=20000, chunk=1000, passes=2, batch=4, lr=1e-3,
+          n=128, spacing=2.5, seed=1004, log=None):
+    """Online training: the plain CCCS generates `chunk` samples at a time,
+    each chunk is used for `passes` epochs, `n_samples` in total.  Adam on the
+    FP32 master weights, one-cycle learning rate; loss = mean squared error
+    over voxels of the field (target > 0) plus a small weight outside."""
+    n_chunks = max(1, n_samples // chunk)
+    steps_per_chunk = max(1, chunk * passes // batch)
+    total = n_chunks * steps_per_chunk
     opt = torch.optim.Adam(net.parameters(), lr=lr)
-    sched = torch.optim.lr_scheduler.OneCycleLR(opt, max_lr=lr, total_steps=steps, pct_start=0.1)
+    sched = torch.optim.lr_scheduler.OneCycleLR(opt, max_lr=lr, total_steps=total, pct_start=0.05)
     g = torch.Generator(device="cpu").manual_seed(seed)
-    S, D = X.shape[0], X.shape[1]
     hist = []
     t0 = time.time()
-    for it in range(steps):
-        idx = torch.randint(0, S, (batch,), generator=g)
-        o = torch.randint(0, D - crop + 1, (batch, 3), generator=g)
-        xb = torch.stack([X[i, a:a + crop, b:b + crop, c:c + crop] for i, (a, b, c) in zip(idx.tolist(), o.tolist())])
-        yb = torch.stack([Y[i, a:a + crop, b:b + crop, c:c + crop] for i, (a, b, c) in zip(idx.tolist(), o.tolist())]).float()
-        pred = net(xb.contiguous())
-        wgt = 1.0 + 4.0 * (yb > 0.1 * yb.amax(dim=(1, 2, 3), keepdim=True))
-        loss = (wgt * (pred - yb) ** 2).mean()
-        opt.zero_grad(set_to_none=True)
-        loss.backward()
-        opt.step()
-        sched.step()
-        if (it + 1) % max(1, steps // 20) == 0:
-            hist.append(float(loss.item()))
-            if log:
-                log(f"step {it + 1}/{steps} loss {hist[-1]:.3e} ({time.time() - t0:.1f} s)")
+    step = 0
+    tgen = 0.0
+    run, skipped = None, 0
+    for ci in range(n_chunks):
+        tg = time.time()
+        X, dirs, Y = generate(model, chunk, n=n, spacing=spacing, seed=seed * 1000 + ci)
+        torch.cuda.synchronize()
+        tgen += time.time() - tg
+        peak = Y.amax(dim=(1, 2, 3, 4))
+        if ci == 0:
+            net.out_scale.fill_(float(peak[peak > 0].mean()) if bool((peak > 0).any()) else 1.0)
+        if log:
+            log(f"chunk {ci}: target peak mean {float(peak.mean()):.3g} max {float(peak.max()):.3g}, "
+                f"min TERMA-normalised input max {float(X[..., 0:8].float().amax(dim=(1, 2, 3, 4)).min()):.3g}")
+        inv = 1.0 / float(net.out_scale)
+        for _ in range(steps_per_chunk):
+            idx = torch.randint(0, chunk, (batch,), generator=g).cuda()
+            xb = assemble(X[idx], dirs[idx])
+            yb = Y[idx] * inv
+            pred = net(xb)[..., :N_SUB]
+            wgt = torch.where(yb > 0, 1.0, 0.05)
+            loss = (wgt * (pred - yb) ** 2).mean()
+            opt.zero_grad(set_to_none=True)
+            loss.backward()
+            lv = float(loss.item())
+            # a non-finite loss or a spike far above the running mean skips
+            # the update; gradients are clipped to unit norm
+            if math.isfinite(lv) and (run is None or lv < 20.0 * run):
+                torch.nn.utils.clip_grad_norm_(net.parameters(), 1.0)
+                opt.step()
+                run = lv if run is None else 0.98 * run + 0.02 * lv
+            else:
+                skipped += 1
+            sched.step()
+            step += 1
+            if step % max(1, total // 40) == 0 or step == total:
+                hist.append(run if run is not None else lv)
+                if log:
+                    log(f"step {step}/{total} loss {hist[-1]:.3e} lr {sched.get_last_lr()[0]:.2e} "
+                        f"skipped {skipped} ({time.time() - t0:.1f} s, generation {tgen:.1f} s)")
+        del X, dirs, Y
     return hist
 
 
 # ---------------------------------------------------------------------------
 # inference with the plain fallback
 # ---------------------------------------------------------------------------
+WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "cccs_surrogate.safetensors")
+
+
 class SurrogateCCCS:
-    def __init__(self, net: SurrogateNet, model):
-        self.net = net
+    """Surrogate photon dose of one beam from the plain path's TERMA,
+    electron-contamination and density volumes, checked against the plain
+    CCCS on a lattice of voxels with the plain CCCS as per-voxel fallback.
+    Everything runs on the device without host synchronisation, so a beam
+    (or a plan of beams) can be captured into a HIP graph."""
+
+    def __init__(self, net: SurrogateNet, model, stride=4, tol=0.01):
+        self.net = net.eval().prepare()
+        self.out_scale = float(net.out_scale)
         self.model = model
+        self.stride, self.tol = int(stride), float(tol)
+        self._buf = {}
+
+    def _buffers(self, shape, device, stride):
+        dev = torch.device(device)
+        idx = dev.index if dev.index is not None else torch.cuda.current_device()
+        key = (tuple(int(v) for v in shape), idx, stride)
+        b = self._buf.get(key)
+        if b is None:
+            pz, py, px = padded_shape(shape)
+            nw = C.c_int64()
+            _lib.check(_lib.load().dh_sur_check_work_size(_lib.grid(shape, 1.0, np.zeros(3)), stride, C.byref(nw)))
+            b = dict(x=torch.empty((1, pz // 2, py // 2, px // 2, IN_CH), dtype=torch.bfloat16, device=device),
+                     tmax=torch.zeros(1, device=device), dmax=torch.zeros(1, device=device),
+                     work=torch.zeros(int(nw.value), dtype=torch.int32, device=device),
+                     plain=torch.empty(tuple(shape), dtype=torch.float32, device=device))
+            self._buf[key] = b
+        return b
 
     @torch.no_grad()
-    def predict(self, terma, density, electron, mu):
-        """Surrogate dose (fp32, full resolution) of one beam."""
-        D, H, W = terma.shape
-        x, scale = make_input(terma, density, electron)
-        y = self.net(x)                                        # [1, D/2, H/2, W/2]
-        y = F.interpolate(y[None], size=(D, H, W), mode="trilinear", align_corners=False)[0, 0]
-        return torch.clamp(y, min=0.0) * (dose_scale(self.model, mu) * scale)
+    def predict(self, terma, density, electron, gantry, couch, mu, out=None, stride=None):
+        """Surrogate dose, finished like the plain CCCS's output: MU
+        calibration, electron term, zero outside TERMA > 0.01, no negatives."""
+        nz, ny, nx = terma.shape
+        b = self._buffers(terma.shape, terma.device, self.stride if stride is None else int(stride))
+        out = torch.empty_like(terma) if out is None else out
+        d = beam_direction(gantry, couch)
+        lib = _lib.load()
+        _lib.check(lib.dh_sur_pack_dev(nz, ny, nx, _lib.ptr(terma), _lib.ptr(density), _lib.ptr(d),
+                                       _lib.ptr(b["x"]), _lib.ptr(b["tmax"]), _st()))
+        y = self.net.infer(b["x"])
+        _lib.check(lib.dh_sur_finish_dev(nz, ny, nx, _lib.ptr(y), _lib.ptr(b["tmax"]), self.out_scale,
+                                         _lib.ptr(terma), _lib.ptr(electron), dose_scale(self.model, mu),
+                                         _lib.ptr(out), _lib.ptr(b["dmax"]), _st()))
+        return out
 
-    def dose(self, terma, density, electron, spacing, origin, iso, mu, ga, ca, ta, mlc,
-             stride=4, tol=0.01, timings=None):
-        """Surrogate dose checked on a lattice against the plain CCCS, with the
-        plain CCCS recomputed voxel by voxel wherever the check fails."""
-        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)] if timings is not None else None
+    @torch.no_grad()
+    def dose(self, terma, density, electron, spacing, origin, iso, mu, ga, ca, ta, mlc=None,
+             stride=None, tol=None, out=None, timings=None):
+        """Checked surrogate dose of one control point (see class doc).
+        timings (a dict) receives the split of the time and the counts."""
+        stride = self.stride if stride is None else int(stride)
+        tol = self.tol if tol is None else float(tol)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)] if timings is not None else None
         if ev:
             ev[0].record()
-        d = self.predict(terma, density, electron, mu)
+        d = self.predict(terma, density, electron, ga, ta, mu, out=out, stride=stride)
         if ev:
             ev[1].record()
-        active = ~(terma <= 0.01)
-        lat = torch.zeros_like(active)
-        lat[::stride, ::stride, ::stride] = True
-        lat_idx = torch.nonzero((active & lat).view(-1)).view(-1).to(torch.int32)
-        plain = torch.zeros_like(d)
-        _ccc_list(terma, density, electron, spacing, origin, iso, mu, ga, ca, ta, self.model,
-                  lat_idx, plain)
-        dmax = torch.clamp(d.max(), min=1e-12)
-        bad = torch.zeros_like(active)
-        bad.view(-1)[lat_idx.long()] = (d.view(-1)[lat_idx.long()] - plain.view(-1)[lat_idx.long()]).abs() > tol * dmax
-        # Lattice cells (stride^3 voxels) whose checked voxel failed, dilated by
-        # one cell, and cells with active voxels but no active checked voxel
-        # (field edges), go to the plain CCCS.
-        pool = lambda v: F.max_pool3d(v.float()[None, None], kernel_size=stride, stride=stride,
-                                      ceil_mode=True)
-        cells = F.max_pool3d(pool(bad), kernel_size=3, stride=1, padding=1)
-        cells = torch.maximum(cells, pool(active) * (1.0 - pool(active & lat)))
-        fb = F.interpolate(cells, scale_factor=stride, mode="nearest")[0, 0, :d.shape[0], :d.shape[1], :d.shape[2]] > 0
-        fb = fb & active
+        b = self._buffers(terma.shape, terma.device, stride)
+        keep: list = []
+        m = engine.photon_args(self.model, keep)
+        _lib.check(_lib.load().dh_sur_check_dev(
+            terma.device.index or 0, _lib.grid(terma.shape, spacing, origin), _lib.photon_cp(iso, mu, ga, ca, ta),
+            m, _lib.ptr(terma), _lib.ptr(density), _lib.ptr(electron), _lib.ptr(d), _lib.ptr(b["dmax"]), stride,
+            tol, _lib.ptr(b["work"]), b["work"].numel(), _lib.ptr(b["plain"]), _st()))
         if ev:
             ev[2].record()
-        fb_idx = torch.nonzero(fb.view(-1)).view(-1).to(torch.int32)
-        _ccc_list(terma, density, electron, spacing, origin, iso, mu, ga, ca, ta, self.model, fb_idx, d)
-        if ev:
-            ev[3].record()
             torch.cuda.synchronize()
+            cnt = b["work"][:2].tolist()
             timings.update(net_ms=ev[0].elapsed_time(ev[1]), check_ms=ev[1].elapsed_time(ev[2]),
-                           fallback_ms=ev[2].elapsed_time(ev[3]), total_ms=ev[0].elapsed_time(ev[3]),
-                           fallback_voxels=int(fb_idx.numel()), lattice_voxels=int(lat_idx.numel()),
-                           active_voxels=int(active.sum().item()))
-        d[~active] = 0.0
+                           total_ms=ev[0].elapsed_time(ev[2]), lattice_voxels=int(cnt[0]),
+                           fallback_voxels=int(cnt[1]), active_voxels=int((~(terma <= 0.01)).sum().item()))
         return d
+
+    def counts(self, shape, device="cuda", stride=None):
+        """(lattice voxels, fallback voxels) of the last checked dose."""
+        b = self._buffers(shape, device, self.stride if stride is None else int(stride))
+        c = b["work"][:2].tolist()
+        return int(c[0]), int(c[1])
 
 
 def _ccc_list(terma, density, electron, spacing, origin, iso, mu, ga, ca, ta, model, idx, out):
@@ -368,3 +566,154 @@ def gamma(eval_dose: torch.Tensor, ref_dose: torch.Tensor, spacing, dta_mm=2.0, 
     n = int(m.sum().item())
     rate = float(((out <= 1.0) & m).sum().item()) / max(n, 1)
     return rate, out
+
+
+# ---------------------------------------------------------------------------
+# evaluation on the 7-beam lung/bone phantom and the command line
+# ---------------------------------------------------------------------------
+def _time(fn, reps):
+    fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def evaluate(net: SurrogateNet, model, n_beams=7, stride=4, tol=0.01, reps=5, log=None):
+    """Per-beam and plan gamma (2%/2 mm, 10% cut) of the surrogate alone and
+    of the checked surrogate against the plain CCCS on the 128^3 lung/bone
+    phantom with 10x10 cm fields, and the timings of both paths (eager, and
+    the checked surrogate of the whole plan replayed as one HIP graph)."""
+    from .beam_models import square_field_mlc
+    from .phantoms import imrt_beams, imrt_lung_bone_case, to_density
+    case = imrt_lung_bone_case(seed=1002, n=128, spacing=2.5)
+    rho = torch.from_numpy(to_density(case)).cuda()
+    mlc = square_field_mlc(model, 10.0, 10.0)
+    sur = SurrogateCCCS(net, model, stride=stride, tol=tol)
+    sp, org, iso = case.spacing, case.origin, case.iso
+    angles = imrt_beams(n_beams)
+    res = dict(beams=[], stride=stride, tol=tol)
+    sums = {k: torch.zeros_like(rho) for k in ("plain", "net", "checked")}
+    termas, els, plains, checked = [], [], [], []
+    for ga in angles:
+        terma, el = torch.empty_like(rho), torch.empty_like(rho)
+        plain = engine.photon_dose(rho, sp, org, iso, 100.0, ga, 0.0, 0.0, model, mlc,
+                                   terma_out=terma, electron_out=el)
+        t_plain = _time(lambda: engine.photon_dose(rho, sp, org, iso, 100.0, ga, 0.0, 0.0, model, mlc,
+                                                   terma_out=terma, electron_out=el), reps)
+        engine.prof_reset()
+        engine.prof_enable(True)
+        for _ in range(reps):
+            engine.photon_dose(rho, sp, org, iso, 100.0, ga, 0.0, 0.0, model, mlc,
+                               terma_out=terma, electron_out=el)
+        torch.cuda.synchronize()
+        engine.prof_enable(False)
+        stage = {k: engine.prof_get(k)[0] / reps for k in ("raytrace", "terma", "pack_tr", "ccc_compact", "ccc")}
+        t_net = _time(lambda: sur.predict(terma, rho, el, ga, 0.0, 100.0), reps)
+        dn = sur.predict(terma, rho, el, ga, 0.0, 100.0)
+        tm = {}
+        dc = sur.dose(terma, rho, el, sp, org, iso, 100.0, ga, 0.0, 0.0, timings=tm)
+        t_chk = _time(lambda: sur.dose(terma, rho, el, sp, org, iso, 100.0, ga, 0.0, 0.0), reps)
+        gn, _ = gamma(dn, plain, sp)
+        gc, _ = gamma(dc, plain, sp)
+        pm = float(plain.max())
+        b = dict(gantry=ga, gamma_net=gn, gamma_checked=gc,
+                 max_err_net=float((dn - plain).abs().max()) / pm,
+                 max_err_checked=float((dc - plain).abs().max()) / pm,
+                 rms_err_checked=float((dc - plain).pow(2).mean().sqrt()) / pm,
+                 plain_beam_ms=t_plain, plain_stage_ms=stage, surrogate_net_ms=t_net,
+                 surrogate_checked_ms=t_chk, checked_split=tm)
+        res["beams"].append(b)
+        if log:
+            log(json.dumps(b))
+        sums["plain"] += plain
+        sums["net"] += dn
+        sums["checked"] += dc
+        termas.append(terma)
+        els.append(el)
+        plains.append(plain)
+        checked.append(dc)
+    # the checked surrogate of the whole plan as one HIP graph
+    outs = [torch.empty_like(rho) for _ in angles]
+
+    def plan():
+        for i, ga in enumerate(angles):
+            sur.dose(termas[i], rho, els[i], sp, org, iso, 100.0, ga, 0.0, 0.0, out=outs[i])
+
+    plan()
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        plan()
+    for _ in range(3):
+        g.replay()
+        torch.cuda.synchronize()
+        res["graph_replay_equal"] = all(bool(torch.equal(o, c)) for o, c in zip(outs, checked))
+        if not res["graph_replay_equal"]:
+            break
+    res["graph_checked_ms_per_beam"] = _time(lambda: g.replay(), reps) / len(angles)
+    res["plan_gamma_net"] = gamma(sums["net"], sums["plain"], sp)[0]
+    res["plan_gamma_checked"] = gamma(sums["checked"], sums["plain"], sp)[0]
+    B = res["beams"]
+    mean = lambda k: float(np.mean([b[k] for b in B]))
+    sup_stage = float(np.mean([b["plain_stage_ms"]["ccc"] + b["plain_stage_ms"]["ccc_compact"] +
+                               b["plain_stage_ms"]["pack_tr"] for b in B]))
+    res.update(plain_beam_ms=mean("plain_beam_ms"), plain_superposition_ms=sup_stage,
+               surrogate_net_ms=mean("surrogate_net_ms"), surrogate_checked_ms=mean("surrogate_checked_ms"),
+               min_gamma_net=min(b["gamma_net"] for b in B), min_gamma_checked=min(b["gamma_checked"] for b in B),
+               mean_fallback_fraction=float(np.mean([b["checked_split"]["fallback_voxels"] /
+                                                     max(1, b["checked_split"]["active_voxels"]) for b in B])),
+               speedup_superposition=sup_stage / res["graph_checked_ms_per_beam"],
+               net_gflop=net.flops(64 ** 3) / 1e9, params=net.n_params())
+    return res
+
+
+def main(argv=None):
+    import argparse
+    from .beam_models import PhotonBeamModel
+    ap = argparse.ArgumentParser(prog="python -m dosecuda_amd.surrogate")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("train", help="train on plain-CCCS samples and save the weights")
+    t.add_argument("--samples", type=int, default=20000)
+    t.add_argument("--chunk", type=int, default=1000)
+    t.add_argument("--passes", type=int, default=2)
+    t.add_argument("--batch", type=int, default=4)
+    t.add_argument("--lr", type=float, default=1e-3)
+    t.add_argument("--seed", type=int, default=1004)
+    t.add_argument("--out", default=WEIGHTS)
+    t.add_argument("--eval", action="store_true", help="evaluate after training")
+    e = sub.add_parser("eval", help="gamma and timing on the 7-beam lung/bone phantom")
+    e.add_argument("--weights", default=WEIGHTS)
+    for q in (t, e):
+        q.add_argument("--json", default=None, help="write the evaluation as JSON here")
+        q.add_argument("--stride", type=int, default=4)
+        q.add_argument("--tol", type=float, default=0.01)
+    a = ap.parse_args(argv)
+    log = lambda m: print(m, flush=True)
+    model = PhotonBeamModel()
+    if a.cmd == "train":
+        net = SurrogateNet().cuda()
+        log(f"surrogate: {net.n_params()} parameters, {net.flops(64 ** 3) / 1e9:.1f} GFLOP per 64^3 patch")
+        train(net, model, n_samples=a.samples, chunk=a.chunk, passes=a.passes, batch=a.batch, lr=a.lr,
+              seed=a.seed, log=log)
+        os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
+        save(net, a.out)
+        log(f"saved {a.out}")
+        if not a.eval:
+            return
+    else:
+        net = load(a.weights)
+    r = evaluate(net, model, stride=a.stride, tol=a.tol, log=log)
+    summary = {k: v for k, v in r.items() if k != "beams"}
+    log(json.dumps(summary))
+    if a.json:
+        with open(a.json, "w") as f:
+            json.dump(r, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()

@@ -194,13 +194,23 @@ int dh_stage_ccc_list_dev(int gpu_id, const dh_grid *grid, const dh_photon_cp *c
 
 /* ---- learned CCCS surrogate: BF16 MFMA building blocks ---------------- */
 /* No reference counterpart.  Tensors are channels-last bf16 [n][d][h][w][c].
- * Convolution (k = 1 or 3, stride 1, zero "same" padding) with weights in
- * the [taps][cout_pad][cin] bf16 layout, FP32 accumulation, optional bias and
- * LeakyReLU (act = 1); writes bf16 y and/or fp32 y32.  cin % 16 == 0,
- * cout_pad % 32 == 0. */
-int dh_nn_conv3d_dev(const void *x, int32_t n, int32_t d, int32_t h, int32_t w, int32_t cin,
-                     const void *wt, const float *bias, int32_t cout, int32_t cout_pad,
-                     int32_t ksize, int32_t act, float slope, void *y, float *y32, void *stream);
+ * Convolution (k = 1 or 3, stride 1, zero "same" padding) of the channel
+ * concatenation [A, B]: A has ca channels and, when up_a = 1, half the
+ * spatial size (read nearest-upsampled); B (skip input, cb channels, may be
+ * NULL with cb = 0) has the output's size.  Weights in the
+ * [taps][cout_pad][ca + cb] bf16 layout, FP32 accumulation, optional bias
+ * and LeakyReLU (act = 1); writes bf16 y and/or fp32 y32.  ca, cb multiples
+ * of 16, cout_pad of 32.  nco (output tiles of 32 channels per wave: 1, 2, 4)
+ * and ksplit (waves splitting K per workgroup: 1..16) pick the kernel
+ * variant; 0, 0 lets the library choose. */
+int dh_nn_conv3d_dev(const void *xa, int32_t ca, int32_t up_a, const void *xb, int32_t cb,
+                     int32_t n, int32_t d, int32_t h, int32_t w, const void *wt,
+                     const float *bias, int32_t cout, int32_t cout_pad, int32_t ksize,
+                     int32_t act, float slope, void *y, float *y32, int32_t nco,
+                     int32_t ksplit, void *stream);
+/* 2x2x2 average pooling of channels-last bf16 (even d, h, w). */
+int dh_nn_pool2_dev(const void *x, int32_t n, int32_t d, int32_t h, int32_t w, int32_t c,
+                    void *y, void *stream);
 /* dw[taps][cin][cout] += sum_v x[v + off][ci] * dy[v][co] (FP32 atomics). */
 int dh_nn_conv3d_wgrad_dev(const void *x, const void *dy, int32_t n, int32_t d, int32_t h,
                            int32_t w, int32_t cin, int32_t cout, int32_t ksize, float *dw,
@@ -209,6 +219,35 @@ int dh_nn_conv3d_wgrad_dev(const void *x, const void *dy, int32_t n, int32_t d, 
  * the tap-reversed [taps][rows_pad][cout] (mode 1, input gradient). */
 int dh_nn_prep_weights_dev(const float *w, int32_t taps, int32_t cin, int32_t cout, int32_t mode,
                            int32_t rows_pad, void *out, void *stream);
+/* Surrogate input [nz'/2][ny'/2][nx'/2][32] bf16 (n' = n rounded up to a
+ * multiple of 16): channels 0-7 TERMA / max TERMA and 8-15 density of the
+ * 2x2x2 sub-voxels (sub = 4 dz + 2 dy + dx; TERMA 0 and air density beyond
+ * the volume), 16-18 the beam direction dir, 19-31 zero.  *tmax_out (device)
+ * receives max TERMA. */
+int dh_sur_pack_dev(int32_t nz, int32_t ny, int32_t nx, const float *d_terma,
+                    const float *d_density, const float dir[3], void *d_x_out, float *d_tmax_out,
+                    void *stream);
+/* Surrogate dose from the network output y32 [cells][16] (channels 0-7 the
+ * sub-voxels): dose = (y * tmax * out_scale + electron) * mu_scale where
+ * TERMA > 0.01 and that value is positive, else 0; *dmax_out (device)
+ * receives the dose maximum. */
+int dh_sur_finish_dev(int32_t nz, int32_t ny, int32_t nx, const float *d_y32, const float *d_tmax,
+                      float out_scale, const float *d_terma, const float *d_electron,
+                      float mu_scale, float *d_dose_out, float *d_dmax_out, void *stream);
+/* Plain-CCCS check and fallback of a surrogate dose, all on the device: the
+ * plain CCCS on every stride-th voxel (per axis) with TERMA > 0.01; every
+ * lattice cell whose surrogate value misses the plain one by more than
+ * tol * dmax fails; voxels with TERMA > 0.01 in a failed cell or next to one
+ * (26-neighbourhood of cells), or in a cell whose lattice voxel has
+ * TERMA <= 0.01, get the plain CCCS value in d_dose.  d_work holds int32
+ * [work_len] (dh_sur_check_work_size); afterwards d_work[0] is the number of
+ * lattice voxels and d_work[1] the number of fallback voxels.  d_plain:
+ * fp32 voxels of scratch. */
+int dh_sur_check_work_size(const dh_grid *grid, int32_t stride, int64_t *n_ints);
+int dh_sur_check_dev(int gpu_id, const dh_grid *grid, const dh_photon_cp *cp,
+                     const dh_photon_model *model, const float *d_terma, const float *d_density,
+                     const float *d_electron, float *d_dose, const float *d_dmax, int32_t stride,
+                     float tol, int32_t *d_work, int64_t work_len, float *d_plain, void *stream);
 /* Global gamma index on the voxel grid (Low et al. 1998): out = gamma for
  * voxels with ref >= cut, -1 elsewhere; search radius in voxels. */
 int dh_gamma_dev(const float *eval, const float *ref, int32_t nz, int32_t ny, int32_t nx,
