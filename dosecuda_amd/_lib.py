@@ -114,6 +114,24 @@ SIGNATURES = {
     "dh_nn_prep_weights_dev": [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P, _P],
     "dh_gamma_dev": [_P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_float,
                      C.c_float, C.c_int32, _P, _P],
+    "dh_sur_net_create": [_I, _P, C.c_int32, _P, _P, C.c_float, C.c_float, _P],
+    "dh_sur_net_destroy": [_P],
+    "dh_sur_counts": [_I, C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
+    "dh_sur_net_forward_dev": [_P, _P, C.c_int32, C.c_int32, C.c_int32, _P, _P],
+    "dh_sur_dose_dev": [_P, C.POINTER(Grid), C.POINTER(PhotonCP), C.POINTER(PhotonModel), _P, _P, _P,
+                        C.c_int32, C.c_float, _P, _P],
+    "dh_photon_dose_surrogate_dev": [_I, _P, C.POINTER(Grid), C.POINTER(PhotonCP), C.POINTER(PhotonModel), _P,
+                                     C.c_int32, _P, _P, _P, _P, C.c_int32, C.c_float, _P],
+    "dh_sched_create": [_I, C.c_int32, _P],
+    "dh_sched_destroy": [_P],
+    "dh_sched_add_case": [_P, C.POINTER(Grid), _P, _P, _P],
+    "dh_sched_add_photon": [_P, C.c_int32, C.POINTER(PhotonCP), C.POINTER(PhotonModel), _P, C.c_int32, _P,
+                            C.c_int32, C.c_float],
+    "dh_sched_add_proton_wet": [_P, C.c_int32, C.POINTER(ProtonBeam), _P],
+    "dh_sched_add_proton_dose": [_P, C.c_int32, _P, _P],
+    "dh_sched_add_influence_fill": [_P, C.c_int32, _P, _P, _P, _P],
+    "dh_sched_run": [_P, C.c_int32, _P],
+    "dh_sched_info": [_P, _P, _P, _P, _P, _P],
     "dh_prof_enable": [_I],
     "dh_prof_reset": [],
     "dh_prof_get": [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)],

@@ -445,10 +445,24 @@ static dh_ccc_args make_ccc_args(const dh_grid *grid, const dh_photon_cp *cp, co
     return a;
 }
 
+// Unit vector source -> isocentre of a control point in volume axes (the
+// beam direction channels of the surrogate input).
+static void sur_direction(const dh_photon_cp *cp, float dir[3])
+{
+    const double ga = std::fmod(cp->ga + 180.0, 360.0) * (M_PI / 180.0), ta = cp->ta * (M_PI / 180.0);
+    const double xg = -std::sin(ga), yg = std::cos(ga);
+    const double sx = xg * std::cos(ta), sy = yg, sz = -xg * std::sin(ta);
+    const double nrm = std::sqrt(sx * sx + sy * sy + sz * sz);
+    dir[0] = (float)(-sx / nrm);
+    dir[1] = (float)(-sy / nrm);
+    dir[2] = (float)(-sz / nrm);
+}
+
 static void photon_device(int dev, const dh_grid *grid, const dh_photon_cp *cp,
                           const dh_photon_model *m, const float *mlc_rows, int32_t n_pairs,
                           const float *d_density, float *d_dose, float *d_terma,
-                          float *d_electron, float *d_wet, hipStream_t s)
+                          float *d_electron, float *d_wet, hipStream_t s,
+                          const dh_sur_net *net = nullptr, int stride = 4, float tol = 0.01f)
 {
     size_t n = nvox_of(grid);
     DH_REQUIRE(cp && d_density && d_dose, "null argument");
@@ -465,7 +479,13 @@ static void photon_device(int dev, const dh_grid *grid, const dh_photon_cp *cp,
     dh_launch_pad(d_density, vpad, ca.g.nx, ca.g.ny, ca.g.nz, DH_AIR_DENSITY, s);  // IMRTClasses.cu:394
     dh_launch_raytrace(ca.g, vpad, wet, s);
     dh_launch_terma(ta, wet, terma, el, s);
-    dh_launch_ccc(ca, terma, d_density, el, trp, act, cnt, d_dose, s);
+    if (net) {
+        float dir[3];
+        sur_direction(cp, dir);
+        dh_launch_sur_dose(net, ca, terma, d_density, el, dir, stride, tol, d_dose, s);
+    } else {
+        dh_launch_ccc(ca, terma, d_density, el, trp, act, cnt, d_dose, s);
+    }
 }
 
 extern "C" int dh_photon_dose_dev(int gpu_id, const dh_grid *grid, const dh_photon_cp *cp,
@@ -585,3 +605,58 @@ extern "C" int dh_sur_check_dev(int gpu_id, const dh_grid *grid, const dh_photon
                             d_plain, trp, (hipStream_t)stream);
     });
 }
+
+extern "C" int dh_photon_dose_surrogate_dev(int gpu_id, const dh_sur_net *net, const dh_grid *grid,
+                                            const dh_photon_cp *cp, const dh_photon_model *model,
+                                            const float *mlc_rows, int32_t n_pairs, const float *d_density,
+                                            float *d_dose_out, float *d_terma_out, float *d_electron_out,
+                                            int32_t stride, float tol, void *stream)
+{
+    return guarded([&] {
+        DH_REQUIRE(net, "null surrogate network");
+        DH_REQUIRE(stride >= 1 && stride <= 64, "lattice stride must be 1..64");
+        dh_device_enter(gpu_id);
+        photon_device(gpu_id, grid, cp, model, mlc_rows, n_pairs, d_density, d_dose_out, d_terma_out,
+                      d_electron_out, nullptr, (hipStream_t)stream, net, stride, tol);
+    });
+}
+
+extern "C" int dh_sur_dose_dev(const dh_sur_net *net, const dh_grid *grid, const dh_photon_cp *cp,
+                               const dh_photon_model *model, const float *d_terma, const float *d_density,
+                               const float *d_electron, int32_t stride, float tol, float *d_dose_out,
+                               void *stream)
+{
+    return guarded([&] {
+        DH_REQUIRE(net && cp && d_terma && d_density && d_electron && d_dose_out, "null argument");
+        DH_REQUIRE(stride >= 1 && stride <= 64, "lattice stride must be 1..64");
+        dh_device_enter(dh_sur_net_gpu(net));
+        nvox_of(grid);
+        dh_ccc_args ca = make_ccc_args(grid, cp, model);
+        float dir[3];
+        sur_direction(cp, dir);
+        dh_launch_sur_dose(net, ca, d_terma, d_density, d_electron, dir, stride, tol, d_dose_out,
+                           (hipStream_t)stream);
+    });
+}
+
+// ---- enqueue-only entry points for the scheduler ---------------------------
+void dh_photon_enqueue(int dev, const dh_grid *grid, const dh_photon_cp *cp, const dh_photon_model *m,
+                       const float *mlc_rows, int n_pairs, const float *d_density, float *d_dose,
+                       const dh_sur_net *net, int stride, float tol, hipStream_t s)
+{
+    photon_device(dev, grid, cp, m, mlc_rows, n_pairs, d_density, d_dose, nullptr, nullptr, nullptr, s, net,
+                  stride, tol);
+}
+
+void dh_proton_wet_enqueue(int dev, const dh_grid *grid, const dh_proton_beam *beam, const float *d_rlsp,
+                           float *d_wet, hipStream_t s)
+{
+    proton_wet_device(dev, grid, beam, d_rlsp, d_wet, nullptr, s);
+}
+
+void dh_proton_plan_enqueue(dh_proton_plan *p, const float *d_wet, float *d_dose, hipStream_t s)
+{
+    plan_dose(p, d_wet, d_dose, s);
+}
+
+int dh_proton_plan_device(const dh_proton_plan *p) { return p->dev; }

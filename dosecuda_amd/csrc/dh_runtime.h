@@ -50,10 +50,15 @@ static inline int dh_guarded(F &&f)
 void dh_device_enter(int dev);
 // Library stream of a device (non-blocking, default priority).
 hipStream_t dh_stream(int dev);
-// Grow-only named scratch buffer on a device.  Growing synchronises the
-// device before releasing the old allocation.  Work that uses the same slot
-// from different streams at the same time races.
+// Grow-only named scratch buffer on a device, in the calling thread's
+// current workspace.  Growing synchronises the device before releasing the
+// old allocation (so it must not happen during stream capture).  Work that
+// uses the same workspace from different streams at the same time races;
+// the scheduler gives each of its streams a workspace of its own.
+constexpr int DH_MAX_WORKSPACES = 33;   // 0 = library default, 1.. scheduler streams
 void *dh_scratch(int dev, int slot, size_t bytes);
+void dh_set_workspace(int ws);
+int dh_get_workspace();
 int dh_current_device();
 
 enum dh_slot {
@@ -69,6 +74,12 @@ enum dh_slot {
     DH_SLOT_OUT2,
     DH_SLOT_TMP0,
     DH_SLOT_TMP1,
+    DH_SLOT_NN_X,          // surrogate input cells
+    DH_SLOT_NN_ACT,        // surrogate activations
+    DH_SLOT_NN_Y,          // surrogate fp32 output cells
+    DH_SLOT_NN_WORK,       // surrogate check: counters, flags, lists
+    DH_SLOT_NN_PLAIN,      // surrogate check: plain values on the lattice
+    DH_SLOT_NN_SCAL,       // surrogate scalars (max TERMA, max dose)
     DH_SLOT_COUNT_
 };
 
@@ -146,9 +157,41 @@ void dh_launch_ccc_counted(const dh_ccc_args &a, const float2 *tr_pad, const flo
 
 // Checked surrogate: plain CCCS on the lattice, per-cell comparison with the
 // surrogate dose, plain CCCS on the voxels of failed and uncovered cells.
-// work: int32 [4 + 3 * cells + voxels] (counters, lattice flags, failure
-// flags, lattice list, fallback list); plain: fp32 voxels.
+// work: int32 [4 + 4 * cells + voxels] (counters, lattice flags, failure
+// flags, recompute flags, lattice list, fallback list); plain: fp32 voxels.
 int64_t dh_sur_check_work_len(int nz, int ny, int nx, int stride);
+
+// Enqueue-only entry points for the scheduler (no host synchronisation, no
+// host copies, no allocation once the calling thread's workspace has grown
+// to the job's size): photon dose of one control point (plain CCCS, or the
+// checked surrogate when net is given), proton WET, proton dense dose of a
+// plan, and the fill pass of a prepared influence matrix.
+struct dh_sur_net;
+struct dh_proton_plan;
+struct dh_influence;
+void dh_photon_enqueue(int dev, const dh_grid *grid, const dh_photon_cp *cp, const dh_photon_model *m,
+                       const float *mlc_rows, int n_pairs, const float *d_density, float *d_dose,
+                       const dh_sur_net *net, int stride, float tol, hipStream_t s);
+void dh_proton_wet_enqueue(int dev, const dh_grid *grid, const dh_proton_beam *beam, const float *d_rlsp,
+                           float *d_wet, hipStream_t s);
+void dh_proton_plan_enqueue(dh_proton_plan *p, const float *d_wet, float *d_dose, hipStream_t s);
+int dh_proton_plan_device(const dh_proton_plan *p);
+// Uploads the fill pass's offsets and blocks once (synchronous); later
+// peak/count passes on the same handle invalidate them.
+void dh_influence_fill_prepare(dh_influence *f, hipStream_t s);
+void dh_influence_fill_enqueue(dh_influence *f, const float *d_wet, uint32_t *d_col, float *d_val, hipStream_t s);
+int dh_influence_device(const dh_influence *f);
+int64_t dh_influence_nnz(const dh_influence *f);
+
+// Native surrogate network (layer table and device weight pointers).
+struct dh_sur_net;
+int dh_sur_net_gpu(const dh_sur_net *net);
+// Checked surrogate dose of one control point from its TERMA and electron
+// volumes (pack, network, finish, lattice check, plain fallback), with the
+// workspace of the calling thread; a = the control point's cone arguments.
+void dh_launch_sur_dose(const dh_sur_net *net, const dh_ccc_args &a, const float *terma,
+                        const float *density, const float *electron, const float dir[3],
+                        int stride, float tol, float *dose, hipStream_t s);
 void dh_launch_sur_check(const dh_ccc_args &a, const float *terma, const float *density,
                          const float *electron, float *dose, const float *dmax, int stride,
                          float tol, int *work, float *plain, float2 *tr_pad, hipStream_t s);

@@ -371,6 +371,7 @@ struct dh_influence {
     std::vector<int> order;          // caller's spot row of each matrix row
     std::vector<int64_t> offsets;    // host copy of the exclusive scan
     int64_t nnz = -1;
+    int fill_blocks = -1;            // blocks of the prepared fill pass in d_blocks (-1: none)
 };
 
 // Conservative per-layer bounds on the total sigma and on the dose factor
@@ -556,17 +557,17 @@ static std::vector<dh_infl_block> make_blocks(const dh_influence *f, const std::
     return out;
 }
 
+// Kernel launches of one pass over the blocks already in f->d_blocks (no
+// host copies: safe to enqueue under stream capture).
 template <int MODE>
-static void launch(dh_influence *f, const std::vector<dh_infl_block> &blocks, const float *d_wet,
-                   uint32_t *d_col, float *d_val, hipStream_t s)
+static void launch_kernels(dh_influence *f, int n_blocks, const float *d_wet, uint32_t *d_col, float *d_val,
+                           hipStream_t s)
 {
-    DH_HIP(hipMemcpyAsync(f->d_blocks, blocks.data(), sizeof(dh_infl_block) * blocks.size(),
-                          hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, (unsigned *)f->d_queue, (size_t)1);
     dh_infl_args a;
     std::memset(&a, 0, sizeof(a));
     a.g = f->g; a.vsadx = f->vsadx; a.vsady = f->vsady;
-    a.n_items = (int)blocks.size() * f->C; a.C = f->C;
+    a.n_items = n_blocks * f->C; a.C = f->C;
     a.blocks = f->d_blocks; a.block_spots = f->d_block_spots; a.layers = f->d_layers;
     a.luts = f->d_luts; a.spots = f->d_spots; a.thr = f->d_thr; a.wet = d_wet;
     a.peak_bits = f->d_peak; a.counts = f->d_counts; a.offsets = f->d_offsets;
@@ -590,6 +591,16 @@ static void launch(dh_influence *f, const std::vector<dh_infl_block> &blocks, co
     if (f->S == 8) hipLaunchKernelGGL((k_influence<8, MODE>), dim3(grid), dim3(64), 0, s, a);
     else hipLaunchKernelGGL((k_influence<16, MODE>), dim3(grid), dim3(64), 0, s, a);
     DH_HIP(hipGetLastError());
+}
+
+template <int MODE>
+static void launch(dh_influence *f, const std::vector<dh_infl_block> &blocks, const float *d_wet,
+                   uint32_t *d_col, float *d_val, hipStream_t s)
+{
+    DH_HIP(hipMemcpyAsync(f->d_blocks, blocks.data(), sizeof(dh_infl_block) * blocks.size(),
+                          hipMemcpyHostToDevice, s));
+    f->fill_blocks = -1;      // d_blocks no longer holds the prepared fill blocks
+    launch_kernels<MODE>(f, (int)blocks.size(), d_wet, d_col, d_val, s);
 }
 
 // PEAK + COUNT; returns nnz and fills the host offsets (exclusive scan over
@@ -695,6 +706,28 @@ void dh_influence_fill(dh_influence *f, const float *d_wet, uint32_t *d_col, flo
     // caller passes the same WET to count and fill
     launch<INF_FILL>(f, make_blocks(f, f->radius), d_wet, d_col, d_val, s);
 }
+
+void dh_influence_fill_prepare(dh_influence *f, hipStream_t s)
+{
+    DH_REQUIRE(f->nnz >= 0, "dh_influence_count must run before the fill is prepared");
+    const size_t n = f->offsets.size();
+    if (n) DH_HIP(hipMemcpyAsync(f->d_offsets, f->offsets.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, s));
+    std::vector<dh_infl_block> blocks = make_blocks(f, f->radius);
+    DH_HIP(hipMemcpyAsync(f->d_blocks, blocks.data(), sizeof(dh_infl_block) * blocks.size(),
+                          hipMemcpyHostToDevice, s));
+    DH_HIP(hipStreamSynchronize(s));
+    f->fill_blocks = (int)blocks.size();
+}
+
+void dh_influence_fill_enqueue(dh_influence *f, const float *d_wet, uint32_t *d_col, float *d_val, hipStream_t s)
+{
+    DH_REQUIRE(f->fill_blocks >= 0, "influence fill not prepared (or another pass ran since)");
+    launch_kernels<INF_FILL>(f, f->fill_blocks, d_wet, d_col, d_val, s);
+}
+
+int dh_influence_device(const dh_influence *f) { return f->dev; }
+
+int64_t dh_influence_nnz(const dh_influence *f) { return f->nnz; }
 
 // ---------------------------------------------------------------------------
 // C ABI

@@ -10,16 +10,21 @@
 
 namespace {
 
+struct ws_slots {
+    void *ptr[DH_SLOT_COUNT_] = {};
+    size_t bytes[DH_SLOT_COUNT_] = {};
+};
+
 struct dev_state {
     bool ready = false;
     hipStream_t stream = nullptr;
-    void *slot_ptr[DH_SLOT_COUNT_] = {};
-    size_t slot_bytes[DH_SLOT_COUNT_] = {};
+    ws_slots ws[DH_MAX_WORKSPACES];
 };
 
 constexpr int kMaxDevices = 64;
 dev_state g_dev[kMaxDevices];
 std::recursive_mutex g_mu;
+thread_local int t_ws = 0;
 
 struct prof_rec {
     std::string name;
@@ -72,25 +77,33 @@ hipStream_t dh_stream(int dev)
     return g_dev[dev].stream;
 }
 
+void dh_set_workspace(int ws)
+{
+    if (ws < 0 || ws >= DH_MAX_WORKSPACES) throw dh_error(DH_E_INVALID, "workspace index out of range");
+    t_ws = ws;
+}
+
+int dh_get_workspace() { return t_ws; }
+
 void *dh_scratch(int dev, int slot, size_t bytes)
 {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
-    dev_state &st = g_dev[dev];
+    ws_slots &st = g_dev[dev].ws[t_ws];
     if (bytes == 0) bytes = 256;
-    if (st.slot_bytes[slot] < bytes) {
-        if (st.slot_ptr[slot]) {
+    if (st.bytes[slot] < bytes) {
+        if (st.ptr[slot]) {
             DH_HIP(hipDeviceSynchronize());
-            DH_HIP(hipFree(st.slot_ptr[slot]));
-            st.slot_ptr[slot] = nullptr;
-            st.slot_bytes[slot] = 0;
+            DH_HIP(hipFree(st.ptr[slot]));
+            st.ptr[slot] = nullptr;
+            st.bytes[slot] = 0;
         }
         size_t b = (bytes + 4095) & ~(size_t)4095;
         void *p = nullptr;
         DH_HIP(hipMalloc(&p, b));
-        st.slot_ptr[slot] = p;
-        st.slot_bytes[slot] = b;
+        st.ptr[slot] = p;
+        st.bytes[slot] = b;
     }
-    return st.slot_ptr[slot];
+    return st.ptr[slot];
 }
 
 void dh_device_release(int dev)
@@ -100,11 +113,12 @@ void dh_device_release(int dev)
     if (!st.ready) return;
     DH_HIP(hipSetDevice(dev));
     DH_HIP(hipDeviceSynchronize());
-    for (int s = 0; s < DH_SLOT_COUNT_; s++) {
-        if (st.slot_ptr[s]) DH_HIP(hipFree(st.slot_ptr[s]));
-        st.slot_ptr[s] = nullptr;
-        st.slot_bytes[s] = 0;
-    }
+    for (int w = 0; w < DH_MAX_WORKSPACES; w++)
+        for (int s = 0; s < DH_SLOT_COUNT_; s++) {
+            if (st.ws[w].ptr[s]) DH_HIP(hipFree(st.ws[w].ptr[s]));
+            st.ws[w].ptr[s] = nullptr;
+            st.ws[w].bytes[s] = 0;
+        }
 }
 
 // ---------------------------------------------------------------------------
@@ -113,6 +127,8 @@ void dh_device_release(int dev)
 dh_timed::dh_timed(const char *n, hipStream_t st) : name(n), s(st), ev0(nullptr)
 {
     if (!g_prof_on) return;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
     std::lock_guard<std::recursive_mutex> lk(g_mu);
     hipEvent_t e = take_event();
     DH_HIP(hipEventRecord(e, s));

@@ -154,6 +154,115 @@ k_conv3d(conv_args a)
     }
 }
 
+// 3x3x3 convolution for large volumes with few channels: a workgroup (4
+// waves) owns a 2 (z) x 4 (y) x 32 (x) brick of output voxels and 32 * NCO
+// output channels.  Per 16-channel input slice it stages the input brick
+// with its one-voxel halo (4 x 6 x 34 voxels) and the slice's weights of all
+// 27 taps in LDS, then each wave runs 27 taps x 2 rows x NCO MFMAs from LDS.
+// LDS layouts put the 8-channel halves of a fragment in separate planes so
+// the 32 lanes of a half read 16-byte chunks at consecutive addresses.
+template <int NCO>
+__global__ void __launch_bounds__(256)
+k_conv3d_lds(conv_args a)
+{
+    constexpr int TZ = 2, TY = 4, BY = TY + 2, BX = 34, NB = (TZ + 2) * BY * BX, MR = 2;
+    constexpr int NCOL = 32 * NCO;
+    __shared__ __attribute__((aligned(16))) uint16_t sa[2 * NB * 8];          // [half][voxel][8]
+    __shared__ __attribute__((aligned(16))) uint16_t sb[27 * 2 * NCOL * 8];   // [tap][half][col][8]
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int row = lane & 31, kh = lane >> 5;
+    const int ntx = (a.W + 31) / 32, nty = (a.H + TY - 1) / TY, ntz = (a.D + TZ - 1) / TZ;
+    const int ncb = a.cout_pad / NCOL;
+    int64_t bi = blockIdx.x;
+    const int cob = (int)(bi % ncb); bi /= ncb;
+    const int tx = (int)(bi % ntx); bi /= ntx;
+    const int ty = (int)(bi % nty); bi /= nty;
+    const int tz = (int)(bi % ntz);
+    const int n = (int)(bi / ntz);
+    const int x0 = tx * 32, y0 = ty * TY, z0 = tz * TZ;
+    const int Da = a.D >> a.up, Ha = a.H >> a.up, Wa = a.W >> a.up;
+    const int nsl = a.cin >> 4;
+
+    dh_f32x16 acc[MR][NCO];
+#pragma unroll
+    for (int m = 0; m < MR; m++)
+#pragma unroll
+        for (int j = 0; j < NCO; j++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) acc[m][j][r] = 0.0f;
+
+    for (int sl = 0; sl < nsl; sl++) {
+        const int c0 = sl * 16;
+        for (int q = tid; q < 2 * NB; q += 256) {
+            const int vb = q >> 1, hf = q & 1;
+            const int bx = vb % BX, r = vb / BX, by = r % BY, bz = r / BY;
+            const int gz = z0 - 1 + bz, gy = y0 - 1 + by, gx = x0 - 1 + bx;
+            uint4 val = make_uint4(0, 0, 0, 0);
+            if (gz >= 0 && gz < a.D && gy >= 0 && gy < a.H && gx >= 0 && gx < a.W) {
+                if (c0 < a.ca) {
+                    const int64_t src = (((int64_t)n * Da + (gz >> a.up)) * Ha + (gy >> a.up)) * Wa + (gx >> a.up);
+                    val = *(const uint4 *)(a.xa + src * a.ca + c0 + hf * 8);
+                } else {
+                    const int64_t src = (((int64_t)n * a.D + gz) * a.H + gy) * a.W + gx;
+                    val = *(const uint4 *)(a.xb + src * a.cb + (c0 - a.ca) + hf * 8);
+                }
+            }
+            *(uint4 *)(sa + (hf * NB + vb) * 8) = val;
+        }
+        for (int q = tid; q < 27 * NCOL * 2; q += 256) {
+            const int hf = q & 1, r = q >> 1, col = r % NCOL, tp = r / NCOL;
+            const uint4 val = *(const uint4 *)(a.w + ((int64_t)tp * a.cout_pad + cob * NCOL + col) * a.cin + c0 + hf * 8);
+            *(uint4 *)(sb + ((tp * 2 + hf) * NCOL + col) * 8) = val;
+        }
+        __syncthreads();
+#pragma unroll 3
+        for (int tp = 0; tp < 27; tp++) {
+            const int dz = tp / 9, dy = (tp / 3) % 3, dx = tp % 3;
+            dh_bf16x8 B[NCO];
+#pragma unroll
+            for (int j = 0; j < NCO; j++) {
+                const uint4 bv = *(const uint4 *)(sb + ((tp * 2 + kh) * NCOL + j * 32 + row) * 8);
+                __builtin_memcpy(&B[j], &bv, 16);
+            }
+#pragma unroll
+            for (int m = 0; m < MR; m++) {
+                const int rr = wid * MR + m, rz = rr / TY, ry = rr % TY;
+                const int vb = ((rz + dz) * BY + (ry + dy)) * BX + row + dx;
+                const uint4 av = *(const uint4 *)(sa + (kh * NB + vb) * 8);
+                dh_bf16x8 A;
+                __builtin_memcpy(&A, &av, 16);
+#pragma unroll
+                for (int j = 0; j < NCO; j++)
+                    acc[m][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, B[j], acc[m][j], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    // D layout: register r of lane l -> row (x offset) 8*(r/4) + 4*(l/32) + (r%4), column l%32
+#pragma unroll
+    for (int m = 0; m < MR; m++) {
+        const int rr = wid * MR + m;
+        const int gz = z0 + rr / TY, gy = y0 + rr % TY;
+        if (gz >= a.D || gy >= a.H) continue;
+#pragma unroll
+        for (int j = 0; j < NCO; j++) {
+            const int col = cob * NCOL + j * 32 + row;
+            if (col >= a.cout) continue;
+            const float bsv = a.bias ? a.bias[col] : 0.0f;
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int gx = x0 + 8 * (r >> 2) + 4 * kh + (r & 3);
+                if (gx >= a.W) continue;
+                const int64_t ov = (((int64_t)n * a.D + gz) * a.H + gy) * a.W + gx;
+                float o = acc[m][j][r] + bsv;
+                if (a.act) o = o > 0.0f ? o : o * a.slope;
+                if (a.y) a.y[ov * a.cout + col] = f2bf(o);
+                if (a.y32) a.y32[ov * a.cout + col] = o;
+            }
+        }
+    }
+}
+
 // avg-pool 2x2x2 on channels-last bf16 [N][D][H][W][C] -> [N][D/2][H/2][W/2][C]
 __global__ void k_pool2(const uint16_t *__restrict__ x, int N, int D, int H, int W, int C,
                         uint16_t *__restrict__ y)
@@ -312,14 +421,28 @@ k_gamma(const float *__restrict__ eval, const float *__restrict__ ref, int nx, i
 
 // ---- surrogate input / output and the checked fallback ----------------------
 
-// max of a non-negative volume via the ordered bits of non-negative floats
-__global__ void k_max_nonneg(const float *__restrict__ v, int64_t n, float *__restrict__ out)
+// One atomic max per workgroup (256 threads) of non-negative floats, via
+// their ordered bit patterns.
+__device__ static inline void block_max_to(float m, float *out)
+{
+    __shared__ float s_m[4];
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    if (__lane_id() == 0) s_m[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const float b = fmaxf(fmaxf(s_m[0], s_m[1]), fmaxf(s_m[2], s_m[3]));
+        if (b > 0.0f) atomicMax((unsigned int *)out, __float_as_uint(b));
+    }
+}
+
+// max of a non-negative volume
+__global__ void __launch_bounds__(256)
+k_max_nonneg(const float *__restrict__ v, int64_t n, float *__restrict__ out)
 {
     float m = 0.0f;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         m = fmaxf(m, v[i]);
-    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
-    if (__lane_id() == 0 && m > 0.0f) atomicMax((unsigned int *)out, __float_as_uint(m));
+    block_max_to(m, out);
 }
 
 __global__ void k_fill_f32(float *p, int n, float v)
@@ -374,7 +497,8 @@ __global__ void k_sur_pack(const float *__restrict__ terma, const float *__restr
 // channels 0-7) back on the voxel grid, scaled by max TERMA and the
 // network's output scale, plus the electron term, times MU calibration;
 // zero where TERMA <= 0.01 or the value is not positive.  Also the dose max.
-__global__ void k_sur_finish(const float *__restrict__ y32, int cy, int cx, const float *__restrict__ tmax,
+__global__ void __launch_bounds__(256)
+k_sur_finish(const float *__restrict__ y32, int cy, int cx, const float *__restrict__ tmax,
                              float out_scale, const float *__restrict__ terma, const float *__restrict__ electron,
                              float mu_scale, int nz, int ny, int nx, float *__restrict__ dose,
                              float *__restrict__ dmax)
@@ -394,8 +518,7 @@ __global__ void k_sur_finish(const float *__restrict__ y32, int cy, int cx, cons
         dose[v] = o;
         m = fmaxf(m, o);
     }
-    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
-    if (__lane_id() == 0 && m > 0.0f) atomicMax((unsigned int *)dmax, __float_as_uint(m));
+    block_max_to(m, dmax);
 }
 
 // block-aggregated append of the flagged items of this block to list
@@ -453,12 +576,33 @@ __global__ void k_sur_flag(const int *__restrict__ lat_ok, const int *__restrict
     bad[c] = (lat_ok[c] && fabsf(dose[v] - plain[v]) > thr) ? 1 : 0;
 }
 
-// voxels with TERMA > 0.01 whose cell has no checked lattice voxel, or whose
-// cell or a neighbouring cell failed the check
+// per cell: recompute its voxels when its lattice voxel has TERMA <= 0.01
+// (cell not covered by the check) or the cell or a neighbour (26-
+// neighbourhood) failed; written over lat_ok's companion array `redo`
+__global__ void k_sur_dilate(const int *__restrict__ lat_ok, const int *__restrict__ bad, int lz, int ly, int lx,
+                             int *__restrict__ redo)
+{
+    const int64_t nc = (int64_t)lz * ly * lx;
+    const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int ci = (int)(c % lx);
+    const int64_t r = c / lx;
+    const int cj = (int)(r % ly), ck = (int)(r / ly);
+    bool fb = !lat_ok[c];
+    for (int dk = -1; dk <= 1 && !fb; dk++)
+        for (int dj = -1; dj <= 1 && !fb; dj++)
+            for (int di = -1; di <= 1 && !fb; di++) {
+                const int a = ck + dk, b = cj + dj, e = ci + di;
+                if (a < 0 || a >= lz || b < 0 || b >= ly || e < 0 || e >= lx) continue;
+                fb = bad[((int64_t)a * ly + b) * lx + e] != 0;
+            }
+    redo[c] = fb ? 1 : 0;
+}
+
+// voxels with TERMA > 0.01 in cells marked for recomputation
 __global__ void __launch_bounds__(256)
 k_sur_fallback(const float *__restrict__ terma, int nz, int ny, int nx, int stride, int lz, int ly, int lx,
-               const int *__restrict__ lat_ok, const int *__restrict__ bad, int *__restrict__ list,
-               int *__restrict__ counter)
+               const int *__restrict__ redo, int *__restrict__ list, int *__restrict__ counter)
 {
     const int64_t n = (int64_t)nz * ny * nx;
     const int64_t v = blockIdx.x * (int64_t)256 + threadIdx.x;
@@ -467,15 +611,7 @@ k_sur_fallback(const float *__restrict__ terma, int nz, int ny, int nx, int stri
         const int i = (int)(v % nx);
         const int64_t r = v / nx;
         const int j = (int)(r % ny), k = (int)(r / ny);
-        const int ci = i / stride, cj = j / stride, ck = k / stride;
-        fb = !lat_ok[((int64_t)ck * ly + cj) * lx + ci];
-        for (int dk = -1; dk <= 1 && !fb; dk++)
-            for (int dj = -1; dj <= 1 && !fb; dj++)
-                for (int di = -1; di <= 1 && !fb; di++) {
-                    const int a = ck + dk, b = cj + dj, e = ci + di;
-                    if (a < 0 || a >= lz || b < 0 || b >= ly || e < 0 || e >= lx) continue;
-                    fb = bad[((int64_t)a * ly + b) * lx + e] != 0;
-                }
+        fb = redo[((int64_t)(k / stride) * ly + j / stride) * lx + i / stride] != 0;
     }
     block_append(fb, (int)v, list, counter);
 }
@@ -511,6 +647,79 @@ void conv_shape(int64_t nvox, int cout_pad, int ki, int &nco, int &ks)
 
 }  // namespace
 
+namespace {
+
+void launch_pool(const uint16_t *x, int n, int d, int h, int w, int c, uint16_t *y, hipStream_t s);
+
+// Picks the kernel variant (explicit nco / ksplit, or the library's choice)
+// and launches it; throws dh_error on bad arguments.
+void launch_conv3d(const conv_args &a, int nco, int ksplit, hipStream_t s)
+{
+    const int64_t nvox = (int64_t)a.N * a.D * a.H * a.W;
+    const int ki = a.ksize * a.ksize * a.ksize * (a.cin / 16);
+    int c = nco, k = ksplit;
+    // brick kernel: explicit (ksplit = -1), or by default for 3x3x3 layers
+    // over large volumes with at most 64 output channels
+    const bool brick = ksplit < 0 || (nco <= 0 && ksplit == 0 && a.ksize == 3 && nvox >= 32768 &&
+                                      a.cout_pad <= 64);
+    dh_timed t("nn_conv3d", s);
+    if (brick) {
+        DH_REQUIRE(a.ksize == 3, "the brick kernel is 3x3x3 only");
+        if (c <= 0) c = a.cout_pad / 32;
+        DH_REQUIRE((c == 1 || c == 2) && a.cout_pad % (32 * c) == 0, "brick kernel: nco must be 1 or 2");
+        const int64_t items = (int64_t)a.N * ((a.D + 1) / 2) * ((a.H + 3) / 4) * ((a.W + 31) / 32) *
+                              (a.cout_pad / (32 * c));
+        if (c == 1)
+            hipLaunchKernelGGL(k_conv3d_lds<1>, dim3((unsigned)items), dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL(k_conv3d_lds<2>, dim3((unsigned)items), dim3(256), 0, s, a);
+        DH_HIP(hipGetLastError());
+        return;
+    }
+    if (c <= 0 || k <= 0) conv_shape(nvox, a.cout_pad, ki, c, k);
+    DH_REQUIRE((a.cout_pad / 32) % c == 0, "output tile does not divide the padded channels");
+    const int64_t items = ((nvox + 31) / 32) * (a.cout_pad / (32 * c));
+    switch (c * 100 + k) {
+    case 101: launch_conv<1, 1>(a, items, s); break;
+    case 102: launch_conv<1, 2>(a, items, s); break;
+    case 104: launch_conv<1, 4>(a, items, s); break;
+    case 108: launch_conv<1, 8>(a, items, s); break;
+    case 116: launch_conv<1, 16>(a, items, s); break;
+    case 201: launch_conv<2, 1>(a, items, s); break;
+    case 202: launch_conv<2, 2>(a, items, s); break;
+    case 204: launch_conv<2, 4>(a, items, s); break;
+    case 208: launch_conv<2, 8>(a, items, s); break;
+    case 401: launch_conv<4, 1>(a, items, s); break;
+    case 402: launch_conv<4, 2>(a, items, s); break;
+    case 404: launch_conv<4, 4>(a, items, s); break;
+    default: DH_REQUIRE(false, "unsupported (output tiles, K split) pair");
+    }
+    DH_HIP(hipGetLastError());
+}
+
+conv_args make_conv(const void *xa, int ca, int up, const void *xb, int cb, int n, int d, int h, int w,
+                    const void *wt, const float *bias, int cout, int cout_pad, int ksize, int act,
+                    float slope, void *y, float *y32)
+{
+    DH_REQUIRE(xa && wt && (y || y32), "null argument");
+    DH_REQUIRE(ksize == 1 || ksize == 3, "kernel size must be 1 or 3");
+    DH_REQUIRE(ca > 0 && ca % 16 == 0 && cb >= 0 && cb % 16 == 0 && (cb == 0 || xb),
+               "input channels must be positive multiples of 16");
+    DH_REQUIRE(up == 0 || up == 1, "up_a must be 0 or 1");
+    DH_REQUIRE(!up || (d % 2 == 0 && h % 2 == 0 && w % 2 == 0), "upsampled input needs even sizes");
+    DH_REQUIRE(cout > 0 && cout_pad >= cout && cout_pad % 32 == 0, "bad output channel padding");
+    DH_REQUIRE(n > 0 && d > 0 && h > 0 && w > 0, "bad shape");
+    conv_args a;
+    a.xa = (const uint16_t *)xa; a.xb = (const uint16_t *)xb; a.ca = ca; a.cb = cb; a.up = up;
+    a.w = (const uint16_t *)wt; a.bias = bias;
+    a.y = (uint16_t *)y; a.y32 = y32;
+    a.N = n; a.D = d; a.H = h; a.W = w; a.cin = ca + cb; a.cout = cout; a.cout_pad = cout_pad;
+    a.ksize = ksize; a.act = act; a.slope = slope;
+    return a;
+}
+
+}  // namespace
+
 extern "C" int dh_nn_conv3d_dev(const void *xa, int32_t ca, int32_t up_a, const void *xb, int32_t cb,
                                 int32_t n, int32_t d, int32_t h, int32_t w, const void *wt,
                                 const float *bias, int32_t cout, int32_t cout_pad, int32_t ksize,
@@ -518,45 +727,9 @@ extern "C" int dh_nn_conv3d_dev(const void *xa, int32_t ca, int32_t up_a, const 
                                 int32_t ksplit, void *stream)
 {
     return dh_guarded([&] {
-        DH_REQUIRE(xa && wt && (y || y32), "null argument");
-        DH_REQUIRE(ksize == 1 || ksize == 3, "kernel size must be 1 or 3");
-        DH_REQUIRE(ca > 0 && ca % 16 == 0 && cb >= 0 && cb % 16 == 0 && (cb == 0 || xb),
-                   "input channels must be positive multiples of 16");
-        DH_REQUIRE(up_a == 0 || up_a == 1, "up_a must be 0 or 1");
-        DH_REQUIRE(!up_a || (d % 2 == 0 && h % 2 == 0 && w % 2 == 0), "upsampled input needs even sizes");
-        DH_REQUIRE(cout > 0 && cout_pad >= cout && cout_pad % 32 == 0, "bad output channel padding");
-        DH_REQUIRE(n > 0 && d > 0 && h > 0 && w > 0, "bad shape");
-        conv_args a;
-        a.xa = (const uint16_t *)xa; a.xb = (const uint16_t *)xb; a.ca = ca; a.cb = cb; a.up = up_a;
-        a.w = (const uint16_t *)wt; a.bias = bias;
-        a.y = (uint16_t *)y; a.y32 = y32;
-        a.N = n; a.D = d; a.H = h; a.W = w; a.cin = ca + cb; a.cout = cout; a.cout_pad = cout_pad;
-        a.ksize = ksize; a.act = act; a.slope = slope;
-        const int64_t nvox = (int64_t)n * d * h * w;
-        const int ki = ksize * ksize * ksize * (a.cin / 16);
-        int c = nco, k = ksplit;
-        if (c <= 0 || k <= 0) conv_shape(nvox, cout_pad, ki, c, k);
-        DH_REQUIRE((cout_pad / 32) % c == 0, "output tile does not divide the padded channels");
-        const int64_t items = ((nvox + 31) / 32) * (cout_pad / (32 * c));
-        hipStream_t s = (hipStream_t)stream;
-        dh_timed t("nn_conv3d", s);
-        const int key = c * 100 + k;
-        switch (key) {
-        case 101: launch_conv<1, 1>(a, items, s); break;
-        case 102: launch_conv<1, 2>(a, items, s); break;
-        case 104: launch_conv<1, 4>(a, items, s); break;
-        case 108: launch_conv<1, 8>(a, items, s); break;
-        case 116: launch_conv<1, 16>(a, items, s); break;
-        case 201: launch_conv<2, 1>(a, items, s); break;
-        case 202: launch_conv<2, 2>(a, items, s); break;
-        case 204: launch_conv<2, 4>(a, items, s); break;
-        case 208: launch_conv<2, 8>(a, items, s); break;
-        case 401: launch_conv<4, 1>(a, items, s); break;
-        case 402: launch_conv<4, 2>(a, items, s); break;
-        case 404: launch_conv<4, 4>(a, items, s); break;
-        default: DH_REQUIRE(false, "unsupported (output tiles, K split) pair");
-        }
-        DH_HIP(hipGetLastError());
+        conv_args a = make_conv(xa, ca, up_a, xb, cb, n, d, h, w, wt, bias, cout, cout_pad, ksize, act, slope,
+                                y, y32);
+        launch_conv3d(a, nco, ksplit, (hipStream_t)stream);
     });
 }
 
@@ -566,13 +739,49 @@ extern "C" int dh_nn_pool2_dev(const void *x, int32_t n, int32_t d, int32_t h, i
     return dh_guarded([&] {
         DH_REQUIRE(x && y && n > 0 && c > 0, "bad argument");
         DH_REQUIRE(d > 1 && h > 1 && w > 1 && d % 2 == 0 && h % 2 == 0 && w % 2 == 0, "sizes must be even");
-        const int64_t total = (int64_t)n * (d / 2) * (h / 2) * (w / 2) * c;
-        const unsigned grid = (unsigned)std::min<int64_t>((total + 255) / 256, 8192);
-        hipLaunchKernelGGL(k_pool2, dim3(grid), dim3(256), 0, (hipStream_t)stream, (const uint16_t *)x, n, d,
-                           h, w, c, (uint16_t *)y);
-        DH_HIP(hipGetLastError());
+        launch_pool((const uint16_t *)x, n, d, h, w, c, (uint16_t *)y, (hipStream_t)stream);
     });
 }
+
+namespace {
+
+void launch_pack(int nz, int ny, int nx, const float *terma, const float *density, const float dir[3],
+                 uint16_t *x, float *tmax, hipStream_t s)
+{
+    const int64_t n = (int64_t)nz * ny * nx;
+    const int cz = ((nz + 15) / 16) * 8, cy = ((ny + 15) / 16) * 8, cx = ((nx + 15) / 16) * 8;
+    hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, s, tmax, 1, 0.0f);
+    hipLaunchKernelGGL(k_max_nonneg, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 1024)), dim3(256), 0, s,
+                       terma, n, tmax);
+    const int64_t items = (int64_t)cz * cy * cx * 4;
+    dh_timed t("sur_pack", s);
+    hipLaunchKernelGGL(k_sur_pack, dim3((unsigned)std::min<int64_t>((items + 255) / 256, 8192)), dim3(256), 0, s,
+                       terma, density, nz, ny, nx, cz, cy, cx, (const float *)tmax, dir[0], dir[1], dir[2], x);
+    DH_HIP(hipGetLastError());
+}
+
+void launch_finish(int nz, int ny, int nx, const float *y32, const float *tmax, float out_scale,
+                   const float *terma, const float *electron, float mu_scale, float *dose, float *dmax,
+                   hipStream_t s)
+{
+    const int64_t n = (int64_t)nz * ny * nx;
+    const int cy = ((ny + 15) / 16) * 8, cx = ((nx + 15) / 16) * 8;
+    hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, s, dmax, 1, 0.0f);
+    dh_timed t("sur_finish", s);
+    hipLaunchKernelGGL(k_sur_finish, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 1024)), dim3(256), 0, s,
+                       y32, cy, cx, tmax, out_scale, terma, electron, mu_scale, nz, ny, nx, dose, dmax);
+    DH_HIP(hipGetLastError());
+}
+
+void launch_pool(const uint16_t *x, int n, int d, int h, int w, int c, uint16_t *y, hipStream_t s)
+{
+    const int64_t total = (int64_t)n * (d / 2) * (h / 2) * (w / 2) * c;
+    const unsigned grid = (unsigned)std::min<int64_t>((total + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_pool2, dim3(grid), dim3(256), 0, s, x, n, d, h, w, c, y);
+    DH_HIP(hipGetLastError());
+}
+
+}  // namespace
 
 extern "C" int dh_sur_pack_dev(int32_t nz, int32_t ny, int32_t nx, const float *terma, const float *density,
                                const float dir[3], void *x_out, float *tmax_out, void *stream)
@@ -580,17 +789,7 @@ extern "C" int dh_sur_pack_dev(int32_t nz, int32_t ny, int32_t nx, const float *
     return dh_guarded([&] {
         DH_REQUIRE(terma && density && dir && x_out && tmax_out, "null argument");
         DH_REQUIRE(nz > 0 && ny > 0 && nx > 0, "bad shape");
-        hipStream_t s = (hipStream_t)stream;
-        const int64_t n = (int64_t)nz * ny * nx;
-        const int cz = ((nz + 15) / 16) * 8, cy = ((ny + 15) / 16) * 8, cx = ((nx + 15) / 16) * 8;
-        hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, s, tmax_out, 1, 0.0f);
-        hipLaunchKernelGGL(k_max_nonneg, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 2048)), dim3(256), 0, s,
-                           terma, n, tmax_out);
-        const int64_t items = (int64_t)cz * cy * cx * 4;
-        hipLaunchKernelGGL(k_sur_pack, dim3((unsigned)std::min<int64_t>((items + 255) / 256, 8192)), dim3(256), 0, s,
-                           terma, density, nz, ny, nx, cz, cy, cx, (const float *)tmax_out, dir[0], dir[1], dir[2],
-                           (uint16_t *)x_out);
-        DH_HIP(hipGetLastError());
+        launch_pack(nz, ny, nx, terma, density, dir, (uint16_t *)x_out, tmax_out, (hipStream_t)stream);
     });
 }
 
@@ -601,13 +800,182 @@ extern "C" int dh_sur_finish_dev(int32_t nz, int32_t ny, int32_t nx, const float
     return dh_guarded([&] {
         DH_REQUIRE(y32 && tmax && terma && electron && dose_out && dmax_out, "null argument");
         DH_REQUIRE(nz > 0 && ny > 0 && nx > 0, "bad shape");
-        hipStream_t s = (hipStream_t)stream;
-        const int64_t n = (int64_t)nz * ny * nx;
-        const int cy = ((ny + 15) / 16) * 8, cx = ((nx + 15) / 16) * 8;
-        hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, s, dmax_out, 1, 0.0f);
-        hipLaunchKernelGGL(k_sur_finish, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 8192)), dim3(256), 0, s,
-                           y32, cy, cx, tmax, out_scale, terma, electron, mu_scale, nz, ny, nx, dose_out, dmax_out);
-        DH_HIP(hipGetLastError());
+        launch_finish(nz, ny, nx, y32, tmax, out_scale, terma, electron, mu_scale, dose_out, dmax_out,
+                      (hipStream_t)stream);
+    });
+}
+
+// ---------------------------------------------------------------------------
+// native surrogate network
+// ---------------------------------------------------------------------------
+struct dh_sur_layer {
+    const void *w;           // bf16 [taps][cout_pad][ca + cb]
+    const float *b;          // fp32 [cout]
+    int ca, cb, up, cout, cout_pad, ksize, act, out32;
+};
+
+struct dh_sur_net {
+    int gpu;
+    int in_ch;
+    int widths[4];
+    float slope, out_scale;
+    dh_sur_layer L[15];
+};
+
+namespace {
+
+constexpr int kSurLayers = 15;
+constexpr int kSurOut = 16;      // head output channels (0-7 used)
+
+// activation elements (bf16) of one forward over cz x cy x cx cells
+int64_t sur_act_elems(const dh_sur_net *net, int cz, int cy, int cx)
+{
+    const int64_t L0 = (int64_t)cz * cy * cx, L1 = L0 / 8, L2 = L0 / 64, L3 = L0 / 512;
+    const int a = net->widths[0], b = net->widths[1], c = net->widths[2], d = net->widths[3];
+    return L0 * 3 * a + L1 * (a + 3 * b) + L2 * (b + 3 * c) + L3 * (c + 2 * d);
+}
+
+void sur_conv(const dh_sur_net *net, int li, const uint16_t *xa, const uint16_t *xb, int d, int h, int w,
+              void *y, float *y32, hipStream_t s)
+{
+    const dh_sur_layer &L = net->L[li];
+    conv_args a = make_conv(xa, L.ca, L.up, xb, L.cb, 1, d, h, w, L.w, L.b, L.cout, L.cout_pad, L.ksize, L.act,
+                            net->slope, y, y32);
+    launch_conv3d(a, 0, 0, s);
+}
+
+// U-Net forward: x [cz][cy][cx][in_ch] bf16 -> y32 [cz][cy][cx][16] fp32
+void sur_forward(const dh_sur_net *net, const uint16_t *x, int cz, int cy, int cx, float *y32, uint16_t *ws,
+                 hipStream_t s)
+{
+    const int64_t L0 = (int64_t)cz * cy * cx, L1 = L0 / 8, L2 = L0 / 64, L3 = L0 / 512;
+    const int a = net->widths[0], b = net->widths[1], c = net->widths[2], d = net->widths[3];
+    uint16_t *p = ws;
+    auto take = [&](int64_t n) { uint16_t *q = p; p += n; return q; };
+    uint16_t *A0 = take(L0 * a), *S0 = take(L0 * a), *B0 = take(L0 * a);
+    uint16_t *P1 = take(L1 * a), *A1 = take(L1 * b), *S1 = take(L1 * b), *B1 = take(L1 * b);
+    uint16_t *P2 = take(L2 * b), *A2 = take(L2 * c), *S2 = take(L2 * c), *B2 = take(L2 * c);
+    uint16_t *P3 = take(L3 * c), *A3 = take(L3 * d), *S3 = take(L3 * d);
+    const int z1 = cz / 2, y1 = cy / 2, x1 = cx / 2, z2 = cz / 4, y2 = cy / 4, x2 = cx / 4;
+    const int z3 = cz / 8, y3 = cy / 8, x3 = cx / 8;
+    sur_conv(net, 0, x, nullptr, cz, cy, cx, A0, nullptr, s);
+    sur_conv(net, 1, A0, nullptr, cz, cy, cx, S0, nullptr, s);
+    launch_pool(S0, 1, cz, cy, cx, a, P1, s);
+    sur_conv(net, 2, P1, nullptr, z1, y1, x1, A1, nullptr, s);
+    sur_conv(net, 3, A1, nullptr, z1, y1, x1, S1, nullptr, s);
+    launch_pool(S1, 1, z1, y1, x1, b, P2, s);
+    sur_conv(net, 4, P2, nullptr, z2, y2, x2, A2, nullptr, s);
+    sur_conv(net, 5, A2, nullptr, z2, y2, x2, S2, nullptr, s);
+    launch_pool(S2, 1, z2, y2, x2, c, P3, s);
+    sur_conv(net, 6, P3, nullptr, z3, y3, x3, A3, nullptr, s);
+    sur_conv(net, 7, A3, nullptr, z3, y3, x3, S3, nullptr, s);
+    sur_conv(net, 8, S3, S2, z2, y2, x2, B2, nullptr, s);
+    sur_conv(net, 9, B2, nullptr, z2, y2, x2, A2, nullptr, s);
+    sur_conv(net, 10, A2, S1, z1, y1, x1, B1, nullptr, s);
+    sur_conv(net, 11, B1, nullptr, z1, y1, x1, A1, nullptr, s);
+    sur_conv(net, 12, A1, S0, cz, cy, cx, B0, nullptr, s);
+    sur_conv(net, 13, B0, nullptr, cz, cy, cx, A0, nullptr, s);
+    sur_conv(net, 14, A0, nullptr, cz, cy, cx, nullptr, y32, s);
+}
+
+}  // namespace
+
+extern "C" int dh_sur_net_create(int gpu_id, const int32_t widths[4], int32_t in_ch,
+                                 const void *const *d_weights, const float *const *d_biases, float slope,
+                                 float out_scale, dh_sur_net **out)
+{
+    return dh_guarded([&] {
+        DH_REQUIRE(widths && d_weights && d_biases && out, "null argument");
+        DH_REQUIRE(in_ch > 0 && in_ch % 16 == 0, "input channels must be a positive multiple of 16");
+        for (int i = 0; i < 4; i++) DH_REQUIRE(widths[i] > 0 && widths[i] % 16 == 0, "widths must be multiples of 16");
+        dh_device_enter(gpu_id);
+        dh_sur_net *n = new dh_sur_net();
+        n->gpu = gpu_id;
+        n->in_ch = in_ch;
+        for (int i = 0; i < 4; i++) n->widths[i] = widths[i];
+        n->slope = slope;
+        n->out_scale = out_scale;
+        const int a = widths[0], b = widths[1], c = widths[2], d = widths[3];
+        // (ca, cb, up, cout, ksize) of e0.0 e0.1 e1.0 e1.1 e2.0 e2.1 bt.0 bt.1
+        // d2.0 d2.1 d1.0 d1.1 d0.0 d0.1 head
+        const int tab[kSurLayers][5] = {{in_ch, 0, 0, a, 3}, {a, 0, 0, a, 3}, {a, 0, 0, b, 3}, {b, 0, 0, b, 3},
+                                        {b, 0, 0, c, 3}, {c, 0, 0, c, 3}, {c, 0, 0, d, 3}, {d, 0, 0, d, 3},
+                                        {d, c, 1, c, 3}, {c, 0, 0, c, 3}, {c, b, 1, b, 3}, {b, 0, 0, b, 3},
+                                        {b, a, 1, a, 3}, {a, 0, 0, a, 3}, {a, 0, 0, kSurOut, 1}};
+        for (int i = 0; i < kSurLayers; i++) {
+            if (!d_weights[i] || !d_biases[i]) {
+                delete n;
+                DH_REQUIRE(false, "null layer weights");
+            }
+            dh_sur_layer &L = n->L[i];
+            L.w = d_weights[i];
+            L.b = d_biases[i];
+            L.ca = tab[i][0]; L.cb = tab[i][1]; L.up = tab[i][2]; L.cout = tab[i][3]; L.ksize = tab[i][4];
+            L.cout_pad = (L.cout + 31) / 32 * 32;
+            L.act = i < kSurLayers - 1;
+            L.out32 = i == kSurLayers - 1;
+        }
+        *out = n;
+    });
+}
+
+int dh_sur_net_gpu(const dh_sur_net *net) { return net->gpu; }
+
+extern "C" int dh_sur_net_destroy(dh_sur_net *net)
+{
+    delete net;
+    return DH_OK;
+}
+
+extern "C" int dh_sur_net_forward_dev(const dh_sur_net *net, const void *d_x, int32_t cz, int32_t cy, int32_t cx,
+                                      float *d_y32, void *stream)
+{
+    return dh_guarded([&] {
+        DH_REQUIRE(net && d_x && d_y32, "null argument");
+        DH_REQUIRE(cz > 0 && cy > 0 && cx > 0 && cz % 8 == 0 && cy % 8 == 0 && cx % 8 == 0,
+                   "cell counts must be positive multiples of 8");
+        dh_device_enter(net->gpu);
+        uint16_t *ws = (uint16_t *)dh_scratch(net->gpu, DH_SLOT_NN_ACT,
+                                              sizeof(uint16_t) * (size_t)sur_act_elems(net, cz, cy, cx));
+        sur_forward(net, (const uint16_t *)d_x, cz, cy, cx, d_y32, ws, (hipStream_t)stream);
+    });
+}
+
+void dh_launch_sur_dose(const dh_sur_net *net, const dh_ccc_args &ca, const float *terma,
+                        const float *density, const float *electron, const float dir[3],
+                        int stride, float tol, float *dose, hipStream_t s)
+{
+    const dh_geom &g = ca.g;
+    const int nz = g.nz, ny = g.ny, nx = g.nx;
+    const int cz = ((nz + 15) / 16) * 8, cy = ((ny + 15) / 16) * 8, cx = ((nx + 15) / 16) * 8;
+    const int64_t cells = (int64_t)cz * cy * cx;
+    const int dev = net->gpu;
+    uint16_t *x = (uint16_t *)dh_scratch(dev, DH_SLOT_NN_X, sizeof(uint16_t) * (size_t)(cells * net->in_ch));
+    uint16_t *ws = (uint16_t *)dh_scratch(dev, DH_SLOT_NN_ACT, sizeof(uint16_t) * (size_t)sur_act_elems(net, cz, cy, cx));
+    float *y32 = (float *)dh_scratch(dev, DH_SLOT_NN_Y, sizeof(float) * (size_t)(cells * kSurOut));
+    float *scal = (float *)dh_scratch(dev, DH_SLOT_NN_SCAL, 256);
+    const int64_t wl = dh_sur_check_work_len(nz, ny, nx, stride);
+    int *work = (int *)dh_scratch(dev, DH_SLOT_NN_WORK, sizeof(int) * (size_t)wl);
+    float *plain = (float *)dh_scratch(dev, DH_SLOT_NN_PLAIN, sizeof(float) * (size_t)nz * ny * nx);
+    float2 *trp = (float2 *)dh_scratch(dev, DH_SLOT_TR_PAD, sizeof(float2) * (size_t)ca.p.pxy * ca.p.pz);
+    DH_REQUIRE(net->in_ch == 32, "the surrogate input layout has 32 channels");
+    launch_pack(nz, ny, nx, terma, density, dir, x, scal, s);
+    sur_forward(net, x, cz, cy, cx, y32, ws, s);
+    launch_finish(nz, ny, nx, y32, scal, net->out_scale, terma, electron, ca.mu_cal * ca.mu, dose, scal + 1, s);
+    dh_launch_sur_check(ca, terma, density, electron, dose, scal + 1, stride, tol, work, plain, trp, s);
+}
+
+extern "C" int dh_sur_counts(int gpu_id, int32_t *lattice, int32_t *fallback)
+{
+    return dh_guarded([&] {
+        DH_REQUIRE(lattice && fallback, "null argument");
+        dh_device_enter(gpu_id);
+        int *work = (int *)dh_scratch(gpu_id, DH_SLOT_NN_WORK, 16);
+        int h[2] = {0, 0};
+        DH_HIP(hipDeviceSynchronize());
+        DH_HIP(hipMemcpy(h, work, sizeof(h), hipMemcpyDeviceToHost));
+        *lattice = h[0];
+        *fallback = h[1];
     });
 }
 
@@ -615,7 +983,7 @@ int64_t dh_sur_check_work_len(int nz, int ny, int nx, int stride)
 {
     const int64_t nc = (int64_t)((nz + stride - 1) / stride) * ((ny + stride - 1) / stride) *
                        ((nx + stride - 1) / stride);
-    return 4 + 3 * nc + (int64_t)nz * ny * nx;
+    return 4 + 4 * nc + (int64_t)nz * ny * nx;
 }
 
 void dh_launch_sur_check(const dh_ccc_args &ca, const float *terma, const float *density,
@@ -626,7 +994,8 @@ void dh_launch_sur_check(const dh_ccc_args &ca, const float *terma, const float 
     const int nz = g.nz, ny = g.ny, nx = g.nx;
     const int lz = (nz + stride - 1) / stride, ly = (ny + stride - 1) / stride, lx = (nx + stride - 1) / stride;
     const int64_t nc = (int64_t)lz * ly * lx, nv = (int64_t)nz * ny * nx;
-    int *counter = work, *lat_ok = work + 4, *bad = lat_ok + nc, *lat_list = bad + nc, *fb_list = lat_list + nc;
+    int *counter = work, *lat_ok = work + 4, *bad = lat_ok + nc, *redo = bad + nc, *lat_list = redo + nc;
+    int *fb_list = lat_list + nc;
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, counter, 4, 0);
     dh_launch_pack_tr(ca, terma, density, tr_pad, s);
     {
@@ -640,8 +1009,10 @@ void dh_launch_sur_check(const dh_ccc_args &ca, const float *terma, const float 
                        dmax, tol, nz, ny, nx, stride, lz, ly, lx, bad);
     {
         dh_timed t("sur_fallback", s);
+        hipLaunchKernelGGL(k_sur_dilate, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, (const int *)lat_ok,
+                           (const int *)bad, lz, ly, lx, redo);
         hipLaunchKernelGGL(k_sur_fallback, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, s, terma, nz, ny, nx,
-                           stride, lz, ly, lx, (const int *)lat_ok, (const int *)bad, fb_list, counter + 1);
+                           stride, lz, ly, lx, (const int *)redo, fb_list, counter + 1);
     }
     dh_launch_ccc_counted(ca, tr_pad, electron, fb_list, counter + 1, nv, dose, s);
     DH_HIP(hipGetLastError());

@@ -457,6 +457,36 @@ def train(net: SurrogateNet, model, n_samples=20000, chunk=1000, passes=2, batch
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "cccs_surrogate.safetensors")
 
 
+class NativeNet:
+    """Handle of the library's native forward pass over a network's cached
+    bf16 weights (dh_sur_net_create); keeps those buffers alive."""
+
+    def __init__(self, net: SurrogateNet, device=None):
+        net.prepare()
+        convs = [*net.e0, *net.e1, *net.e2, *net.bt, *net.d2, *net.d1, *net.d0, net.head]
+        self.weights = [net._wf[id(m)] for m in convs]
+        self.biases = [m.b.detach().contiguous() for m in convs]
+        dev = self.weights[0].device.index or 0 if device is None else int(device)
+        W = (C.c_void_p * len(convs))(*[w.data_ptr() for w in self.weights])
+        B = (C.c_void_p * len(convs))(*[b.data_ptr() for b in self.biases])
+        widths = (C.c_int32 * 4)(*net.widths)
+        h = C.c_void_p()
+        _lib.check(_lib.load().dh_sur_net_create(dev, widths, IN_CH, W, B, SLOPE, float(net.out_scale), C.byref(h)))
+        self.handle = h
+        self.device = dev
+
+    def close(self):
+        if self.handle:
+            _lib.load().dh_sur_net_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class SurrogateCCCS:
     """Surrogate photon dose of one beam from the plain path's TERMA,
     electron-contamination and density volumes, checked against the plain
@@ -470,6 +500,7 @@ class SurrogateCCCS:
         self.model = model
         self.stride, self.tol = int(stride), float(tol)
         self._buf = {}
+        self.native = NativeNet(self.net)
 
     def _buffers(self, shape, device, stride):
         dev = torch.device(device)
@@ -507,8 +538,34 @@ class SurrogateCCCS:
     @torch.no_grad()
     def dose(self, terma, density, electron, spacing, origin, iso, mu, ga, ca, ta, mlc=None,
              stride=None, tol=None, out=None, timings=None):
-        """Checked surrogate dose of one control point (see class doc).
-        timings (a dict) receives the split of the time and the counts."""
+        """Checked surrogate dose of one control point (see class doc) in one
+        call of the library (dh_sur_dose_dev).  timings (a dict) receives the
+        total time and the lattice and fallback voxel counts."""
+        stride = self.stride if stride is None else int(stride)
+        tol = self.tol if tol is None else float(tol)
+        out = torch.empty_like(terma) if out is None else out
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)] if timings is not None else None
+        if ev:
+            ev[0].record()
+        keep: list = []
+        m = engine.photon_args(self.model, keep)
+        _lib.check(_lib.load().dh_sur_dose_dev(
+            self.native.handle, _lib.grid(terma.shape, spacing, origin), _lib.photon_cp(iso, mu, ga, ca, ta), m,
+            _lib.ptr(terma), _lib.ptr(density), _lib.ptr(electron), stride, tol, _lib.ptr(out), _st()))
+        if ev:
+            ev[1].record()
+            torch.cuda.synchronize()
+            lat, fb = C.c_int32(), C.c_int32()
+            _lib.check(_lib.load().dh_sur_counts(terma.device.index or 0, C.byref(lat), C.byref(fb)))
+            timings.update(total_ms=ev[0].elapsed_time(ev[1]), lattice_voxels=int(lat.value),
+                           fallback_voxels=int(fb.value), active_voxels=int((~(terma <= 0.01)).sum().item()))
+        return out
+
+    @torch.no_grad()
+    def dose_torch(self, terma, density, electron, spacing, origin, iso, mu, ga, ca, ta, mlc=None,
+                   stride=None, tol=None, out=None, timings=None):
+        """The same checked dose with the network driven from Python
+        (SurrogateNet.infer) and the check through dh_sur_check_dev."""
         stride = self.stride if stride is None else int(stride)
         tol = self.tol if tol is None else float(tol)
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)] if timings is not None else None
@@ -618,6 +675,15 @@ def evaluate(net: SurrogateNet, model, n_beams=7, stride=4, tol=0.01, reps=5, lo
         tm = {}
         dc = sur.dose(terma, rho, el, sp, org, iso, 100.0, ga, 0.0, 0.0, timings=tm)
         t_chk = _time(lambda: sur.dose(terma, rho, el, sp, org, iso, 100.0, ga, 0.0, 0.0), reps)
+        t_chk_torch = _time(lambda: sur.dose_torch(terma, rho, el, sp, org, iso, 100.0, ga, 0.0, 0.0), reps)
+        engine.prof_reset()
+        engine.prof_enable(True)
+        for _ in range(reps):
+            sur.dose(terma, rho, el, sp, org, iso, 100.0, ga, 0.0, 0.0)
+        torch.cuda.synchronize()
+        engine.prof_enable(False)
+        sstage = {k: engine.prof_get(k)[0] / reps for k in ("sur_pack", "nn_conv3d", "sur_finish", "pack_tr",
+                                                           "sur_lattice", "ccc_list", "sur_fallback")}
         gn, _ = gamma(dn, plain, sp)
         gc, _ = gamma(dc, plain, sp)
         pm = float(plain.max())
@@ -626,7 +692,8 @@ def evaluate(net: SurrogateNet, model, n_beams=7, stride=4, tol=0.01, reps=5, lo
                  max_err_checked=float((dc - plain).abs().max()) / pm,
                  rms_err_checked=float((dc - plain).pow(2).mean().sqrt()) / pm,
                  plain_beam_ms=t_plain, plain_stage_ms=stage, surrogate_net_ms=t_net,
-                 surrogate_checked_ms=t_chk, checked_split=tm)
+                 surrogate_checked_ms=t_chk, surrogate_checked_torch_ms=t_chk_torch,
+                 surrogate_stage_ms=sstage, checked_split=tm)
         res["beams"].append(b)
         if log:
             log(json.dumps(b))
@@ -688,6 +755,7 @@ def main(argv=None):
     t.add_argument("--eval", action="store_true", help="evaluate after training")
     e = sub.add_parser("eval", help="gamma and timing on the 7-beam lung/bone phantom")
     e.add_argument("--weights", default=WEIGHTS)
+    e.add_argument("--sweep", default=None, help="stride:tol pairs, e.g. 4:0.01,4:0.02")
     for q in (t, e):
         q.add_argument("--json", default=None, help="write the evaluation as JSON here")
         q.add_argument("--stride", type=int, default=4)
@@ -707,6 +775,18 @@ def main(argv=None):
             return
     else:
         net = load(a.weights)
+    if a.cmd == "eval" and a.sweep:
+        out = []
+        for pair in a.sweep.split(","):
+            st, tl = pair.split(":")
+            r = evaluate(net, model, stride=int(st), tol=float(tl))
+            summary = {k: v for k, v in r.items() if k != "beams"}
+            log(json.dumps(summary))
+            out.append(r)
+        if a.json:
+            with open(a.json, "w") as f:
+                json.dump(out, f, indent=1)
+        return
     r = evaluate(net, model, stride=a.stride, tol=a.tol, log=log)
     summary = {k: v for k, v in r.items() if k != "beams"}
     log(json.dumps(summary))

@@ -248,11 +248,86 @@ int dh_sur_check_dev(int gpu_id, const dh_grid *grid, const dh_photon_cp *cp,
                      const dh_photon_model *model, const float *d_terma, const float *d_density,
                      const float *d_electron, float *d_dose, const float *d_dmax, int32_t stride,
                      float tol, int32_t *d_work, int64_t work_len, float *d_plain, void *stream);
+/* Native surrogate network: the U-Net's layer weights on the device in the
+ * forward bf16 layout ([taps][cout_pad][cin], dh_nn_prep_weights_dev mode 0)
+ * and fp32 biases, in the order e0.0 e0.1 e1.0 e1.1 e2.0 e2.1 bt.0 bt.1
+ * d2.0 d2.1 d1.0 d1.1 d0.0 d0.1 head (widths w0..w3, in_ch input channels,
+ * head 1x1 with 16 outputs).  The buffers stay owned by the caller and must
+ * outlive the handle. */
+typedef struct dh_sur_net dh_sur_net;
+int dh_sur_net_create(int gpu_id, const int32_t widths[4], int32_t in_ch,
+                      const void *const *d_weights, const float *const *d_biases, float slope,
+                      float out_scale, dh_sur_net **out);
+int dh_sur_net_destroy(dh_sur_net *net);
+/* Forward pass on cz x cy x cx cells (multiples of 8): x [cells][in_ch] bf16
+ * -> y32 [cells][16] fp32; activations in the library's scratch. */
+int dh_sur_net_forward_dev(const dh_sur_net *net, const void *d_x, int32_t cz, int32_t cy,
+                           int32_t cx, float *d_y32, void *stream);
+/* Checked surrogate dose of one control point from the plain path's TERMA
+ * and electron volumes: pack, network, finish, lattice check and plain
+ * fallback (dh_sur_check_dev semantics), all on the device. */
+int dh_sur_dose_dev(const dh_sur_net *net, const dh_grid *grid, const dh_photon_cp *cp,
+                    const dh_photon_model *model, const float *d_terma, const float *d_density,
+                    const float *d_electron, int32_t stride, float tol, float *d_dose_out,
+                    void *stream);
+/* Lattice and fallback voxel counts of the last checked surrogate dose run
+ * by dh_sur_dose_dev / dh_photon_dose_surrogate_dev from this thread on
+ * gpu_id (synchronises the device). */
+int dh_sur_counts(int gpu_id, int32_t *lattice, int32_t *fallback);
+/* dh_photon_dose_dev with the cone superposition replaced by the checked
+ * surrogate (ray trace and TERMA as in the plain path). */
+int dh_photon_dose_surrogate_dev(int gpu_id, const dh_sur_net *net, const dh_grid *grid,
+                                 const dh_photon_cp *cp, const dh_photon_model *model,
+                                 const float *mlc_rows, int32_t n_pairs, const float *d_density,
+                                 float *d_dose_out, float *d_terma_out, float *d_electron_out,
+                                 int32_t stride, float tol, void *stream);
 /* Global gamma index on the voxel grid (Low et al. 1998): out = gamma for
  * voxels with ref >= cut, -1 elsewhere; search radius in voxels. */
 int dh_gamma_dev(const float *eval, const float *ref, int32_t nz, int32_t ny, int32_t nx,
                  float spacing, float dta_mm, float dd_abs, float cut, int32_t radius_vox,
                  float *out, void *stream);
+
+/* ---- batching scheduler ----------------------------------------------- */
+/* No reference counterpart (DoseCUDA runs one synchronous call per beam,
+ * dosemodule.cu:95-538).  A case is one phantom: grid, input volume on the
+ * device (density for photon jobs, RLSP for proton jobs) and an optional
+ * dose output that every run zeroes and then accumulates the case's photon
+ * and proton-dose jobs into, in the order they were added.  The jobs of a
+ * case run in order on one of the scheduler's n_streams streams (each with
+ * its own scratch workspace); cases go longest first to the least-loaded
+ * stream, by per-job GPU times measured in the last eager run (estimated
+ * from voxel counts before).  Host arrays (MLC, beam model) are copied when
+ * a job is added; device buffers, plans, influence handles and surrogate
+ * networks are borrowed and must outlive the scheduler's runs. */
+typedef struct dh_sched dh_sched;
+int dh_sched_create(int gpu_id, int32_t n_streams, dh_sched **out);
+int dh_sched_destroy(dh_sched *sched);
+int dh_sched_add_case(dh_sched *sched, const dh_grid *grid, const float *d_volume, float *d_dose,
+                      int32_t *case_id);
+/* One photon control point: plain CCCS (net = NULL) or the checked
+ * surrogate (dh_photon_dose_surrogate_dev). */
+int dh_sched_add_photon(dh_sched *sched, int32_t case_id, const dh_photon_cp *cp,
+                        const dh_photon_model *model, const float *mlc_rows, int32_t n_pairs,
+                        const dh_sur_net *net, int32_t stride, float tol);
+/* Smoothed WET of the case's RLSP for one proton field into d_wet_out. */
+int dh_sched_add_proton_wet(dh_sched *sched, int32_t case_id, const dh_proton_beam *beam,
+                            float *d_wet_out);
+/* Dense dose of a proton plan from a WET, accumulated into the case dose. */
+int dh_sched_add_proton_dose(dh_sched *sched, int32_t case_id, dh_proton_plan *plan,
+                             const float *d_wet);
+/* Fill pass of a counted influence matrix (uploads its fill blocks now;
+ * no peak/count pass may run on the handle afterwards). */
+int dh_sched_add_influence_fill(dh_sched *sched, int32_t case_id, dh_influence *f,
+                                const float *d_wet, uint32_t *d_col, float *d_val);
+/* mode 0: eager run with per-job timing (synchronises); mode 1: one HIP
+ * graph of the whole batch, captured on first use after an eager warm-up and
+ * replayed afterwards (asynchronous).  The batch starts after the work
+ * queued on `stream`, whose later work waits for it. */
+int dh_sched_run(dh_sched *sched, int32_t mode, void *stream);
+/* Case and job counts, the last eager batch time, the stream of each case
+ * and each job's last measured time (arrays may be NULL). */
+int dh_sched_info(const dh_sched *sched, int32_t *n_cases, int32_t *n_jobs, double *last_ms,
+                  int32_t *case_stream, double *job_ms);
 
 /* ---- kernel timing (HIP events on the launch stream) ------------------ */
 /* When enabled, every kernel launch of the library is bracketed by HIP

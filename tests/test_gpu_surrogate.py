@@ -35,6 +35,8 @@ def _ref_conv(x_bf, w32, b, ksize, act, slope):
     ((2, 6, 7, 9), 32, 48, 3, True),
     ((1, 8, 8, 8), 16, 32, 3, False),
     ((1, 5, 6, 11), 48, 16, 1, False),
+    ((1, 20, 30, 60), 32, 16, 3, True),      # large: LDS brick kernel, partial x and y tiles
+    ((2, 6, 12, 64), 48, 48, 3, False),      # large, batch of 2, 64 padded output channels
 ])
 def test_conv3d_forward(shape, cin, cout, ksize, act):
     t = _torch()
@@ -80,7 +82,7 @@ def test_conv3d_backward():
 
 
 @pytest.mark.parametrize("nco, ks", [(1, 1), (1, 2), (1, 4), (1, 8), (1, 16), (2, 1), (2, 2), (2, 4),
-                                     (2, 8), (4, 1), (4, 2), (4, 4)])
+                                     (2, 8), (4, 1), (4, 2), (4, 4), (1, -1), (2, -1)])
 def test_conv3d_variants(nco, ks):
     """Every (output tiles, K split) instantiation against the fp32 reference."""
     t = _torch()
@@ -113,6 +115,18 @@ def test_conv3d_fused_upsample_concat():
     y0, _ = S.conv3d(cat, wf, b, 32, 3, True)
     y1, _ = S.conv3d(xa, wf, b, 32, 3, True, xb=xb, up=True)
     assert t.equal(y0, y1)
+    # large enough for the LDS brick kernel
+    xa = t.randn((1, 10, 15, 30, 32), generator=g).to(t.bfloat16).cuda()
+    xb = t.randn((1, 20, 30, 60, 16), generator=g).to(t.bfloat16).cuda()
+    w2 = (t.randn((27, 48, 16), generator=g) * 0.1).cuda()
+    wf2 = S.prep_weights(w2, 0)
+    cat = t.cat([S._up(xa), xb], dim=-1).contiguous()
+    y4, _ = S.conv3d(cat, wf2, b[:16], 16, 3, True)
+    y5, _ = S.conv3d(xa, wf2, b[:16], 16, 3, True, xb=xb, up=True)
+    assert t.equal(y4, y5)
+    _, y6 = S.conv3d(xa, wf2, b[:16], 16, 3, True, want_bf16=False, want_f32=True, xb=xb, up=True)
+    ref = _ref_conv(cat, w2, b[:16], 3, True, S.SLOPE)
+    assert float((y6.cpu() - ref).abs().max() / ref.abs().max()) < 1e-5
     # upsampled source alone
     y2, _ = S.conv3d(S._up(xa).contiguous(), S.prep_weights(w[:, :64], 0), b, 32, 3, False)
     y3, _ = S.conv3d(xa, S.prep_weights(w[:, :64], 0), b, 32, 3, False, up=True)
@@ -343,3 +357,45 @@ def test_device_check_matches_torch(photon_model, stride):
     assert (lat, fb) == (n_lat, n_fb)
     assert 0 < fb < int((~(terma <= 0.01)).sum())
     assert t.equal(got, want)
+
+
+def test_native_dose_equals_torch_path(photon_model):
+    """One library call (dh_sur_dose_dev: native forward) equals the
+    Python-driven network + device check bit for bit, and the photon path
+    with the surrogate equals plain TERMA + dh_sur_dose_dev."""
+    t = _torch()
+    from dosecuda_amd import _lib, engine, surrogate as S
+    from dosecuda_amd.beam_models import square_field_mlc
+    pm = photon_model
+    shape, s = (40, 50, 36), 2.5
+    rho = t.full(shape, 1.00653, device="cuda")
+    rho[10:25, 5:30, 8:20] = 0.3
+    iso = np.array([(shape[2] - 1) / 2 * s, (shape[1] - 1) / 2 * s, (shape[0] - 1) / 2 * s], np.float32)
+    mlc = square_field_mlc(pm, 6.0, 6.0)
+    terma, el = t.empty_like(rho), t.empty_like(rho)
+    engine.photon_dose(rho, s, np.zeros(3), iso, 100.0, 120.0, 0.0, 0.0, pm, mlc, terma_out=terma,
+                       electron_out=el)
+    g = t.Generator().manual_seed(2)
+    net = S.SurrogateNet().cuda()
+    with t.no_grad():
+        for p in net.parameters():
+            p.mul_(0.5).add_(0.01 * t.randn(p.shape, generator=g).cuda())
+    net.out_scale.fill_(1.7)
+    sur = S.SurrogateCCCS(net, pm, stride=4, tol=0.02)
+    a = sur.dose(terma, rho, el, s, np.zeros(3), iso, 100.0, 120.0, 0.0, 0.0)
+    b = sur.dose_torch(terma, rho, el, s, np.zeros(3), iso, 100.0, 120.0, 0.0, 0.0)
+    assert t.equal(a, b)
+    # native forward alone
+    x = t.randn((1, 16, 24, 16, S.IN_CH), generator=g).to(t.bfloat16).cuda()
+    y = t.empty((1, 16, 24, 16, 16), device="cuda")
+    _lib.check(_lib.load().dh_sur_net_forward_dev(sur.native.handle, _lib.ptr(x), 16, 24, 16, _lib.ptr(y), S._st()))
+    assert t.equal(y, net.infer(x))
+    # whole photon path with the surrogate
+    d = t.empty_like(rho)
+    keep: list = []
+    m = engine.photon_args(pm, keep)
+    rows = np.ascontiguousarray(np.asarray(mlc, dtype=np.float32)[:, :4])
+    _lib.check(_lib.load().dh_photon_dose_surrogate_dev(
+        0, sur.native.handle, _lib.grid(shape, s, np.zeros(3)), _lib.photon_cp(iso, 100.0, 120.0, 0.0, 0.0), m,
+        _lib.ptr(rows), rows.shape[0], _lib.ptr(rho), _lib.ptr(d), None, None, 4, 0.02, S._st()))
+    assert t.equal(d, a)
