@@ -132,6 +132,17 @@ SIGNATURES = {
     "dh_sched_add_influence_fill": [_P, C.c_int32, _P, _P, _P, _P],
     "dh_sched_run": [_P, C.c_int32, _P],
     "dh_sched_info": [_P, _P, _P, _P, _P, _P],
+    "dh_tune_set": [C.c_char_p, C.c_char_p, C.c_int32, C.c_double],
+    "dh_tune_get": [C.c_char_p, C.c_char_p, C.POINTER(C.c_int32)],
+    "dh_tune_count": [C.POINTER(C.c_int32)],
+    "dh_tune_entry": [C.c_int32, C.c_char_p, C.c_int32, C.c_char_p, C.c_int32, C.POINTER(C.c_int32),
+                      C.POINTER(C.c_double)],
+    "dh_tune_clear": [],
+    "dh_autotune_proton_wet": [_I, C.POINTER(Grid), C.POINTER(ProtonBeam), _P, C.c_int32, C.POINTER(C.c_int32),
+                               C.POINTER(C.c_int32), _P, _P, _P],
+    "dh_autotune_pencil": [_P, _P, C.c_int32, C.POINTER(C.c_int32), _P, _P],
+    "dh_autotune_conv3d": [_I, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                           C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_int32), _P, _P, C.POINTER(C.c_int32), _P],
     "dh_prof_enable": [_I],
     "dh_prof_reset": [],
     "dh_prof_get": [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)],

@@ -71,15 +71,23 @@ static float proton_source_distance(const dh_proton_beam *b)
 // ---------------------------------------------------------------------------
 // proton: WET
 // ---------------------------------------------------------------------------
+dh_geom dh_proton_geom(const dh_grid *grid, const dh_proton_beam *beam)
+{
+    nvox_of(grid);
+    DH_REQUIRE(beam, "null beam");
+    dh_geom g = dh_make_geom(*grid, beam->iso, beam->gantry_angle, beam->couch_angle, 0.0,
+                             proton_source_distance(beam));
+    g.sinca = 0.0f; g.cosca = 1.0f;   // protons have no collimator rotation
+    return g;
+}
+
 static void proton_wet_device(int dev, const dh_grid *grid, const dh_proton_beam *beam,
                               const float *d_rlsp, float *d_wet_out, float *d_wet_raw,
                               hipStream_t s)
 {
     size_t n = nvox_of(grid);
     DH_REQUIRE(beam && d_rlsp && d_wet_out, "null argument");
-    dh_geom g = dh_make_geom(*grid, beam->iso, beam->gantry_angle, beam->couch_angle, 0.0,
-                             proton_source_distance(beam));
-    g.sinca = 0.0f; g.cosca = 1.0f;   // protons have no collimator rotation
+    dh_geom g = dh_proton_geom(grid, beam);
     dh_pad p = dh_make_pad(g.nx, g.ny, g.nz);
     float *vpad = (float *)dh_scratch(dev, DH_SLOT_VOL_PAD, sizeof(float) * (size_t)p.pxy * p.pz);
     float *raw = d_wet_raw ? d_wet_raw : (float *)dh_scratch(dev, DH_SLOT_WET_RAW, sizeof(float) * n);
@@ -660,3 +668,31 @@ void dh_proton_plan_enqueue(dh_proton_plan *p, const float *d_wet, float *d_dose
 }
 
 int dh_proton_plan_device(const dh_proton_plan *p) { return p->dev; }
+
+extern "C" int dh_autotune_pencil(dh_proton_plan *p, const float *d_wet, int32_t reps, int32_t *best,
+                                  float *ms_out, void *stream)
+{
+    return guarded([&] {
+        DH_REQUIRE(p && d_wet, "null argument");
+        dh_device_enter(p->dev);
+        hipStream_t s = (hipStream_t)stream;
+        const size_t n = (size_t)p->g.nx * p->g.ny * p->g.nz;
+        float *ref = (float *)dh_scratch(p->dev, DH_SLOT_TMP0, sizeof(float) * n);
+        float *out = (float *)dh_scratch(p->dev, DH_SLOT_TMP1, sizeof(float) * n);
+        int b = 0;
+        float bms = 0.0f;
+        for (int c = 0; c < 2 * DH_TILE_CHOICES; c++) {
+            float *o = c == 0 ? ref : out;
+            const float ms = dh_time_median([&] {
+                dh_launch_pencil(p->g, (float)p->beam.vsadx, (float)p->beam.vsady, p->n_layers, p->d_layers,
+                                 p->d_luts, p->d_spots, p->fold, d_wet, o, s, c);
+            }, reps, s);
+            if (ms_out) ms_out[c] = ms;
+            DH_REQUIRE(c == 0 || dh_count_diff(ref, out, (int64_t)n, s) == 0,
+                       "pencil variants disagree (tile or LUT staging changed the result)");
+            if (c == 0 || ms < bms) { b = c; bms = ms; }
+        }
+        dh_tune_store("pencil", dh_tune_key_geom(p->g), b, bms);
+        if (best) *best = b;
+    });
+}

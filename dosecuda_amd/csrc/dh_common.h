@@ -145,6 +145,31 @@ __device__ static inline int dh_xcd_block(int b, int nb)
     return x * base + min(x, rem) + q;
 }
 
+// Voxels of a 256-thread workgroup: linear runs along x (tx == 0, the
+// voxel index is b * 256 + thread) or a tx x ty x tz brick (product 256),
+// bricks numbered x fastest.  Returns whether the voxel is in the volume.
+struct dh_tile {
+    int tx, ty, tz, ntx, nty;
+};
+
+__device__ static inline bool dh_tile_voxel(const dh_tile &t, int b, int tid, int nx, int ny, int nz, int &i,
+                                            int &j, int &k)
+{
+    if (t.tx == 0) {
+        const uint32_t v = (uint32_t)b * 256u + (uint32_t)tid;
+        i = (int)(v % (uint32_t)nx);
+        const uint32_t r = v / (uint32_t)nx;
+        j = (int)(r % (uint32_t)ny);
+        k = (int)(r / (uint32_t)ny);
+        return v < (uint32_t)nx * ny * nz;
+    }
+    const int bx = b % t.ntx, r = b / t.ntx, by = r % t.nty, bz = r / t.nty;
+    i = bx * t.tx + tid % t.tx;
+    j = by * t.ty + (tid / t.tx) % t.ty;
+    k = bz * t.tz + tid / (t.tx * t.ty);
+    return i < nx && j < ny && k < nz;
+}
+
 // Trilinear sample at a texel-space point u = t - 0.5 known to lie in
 // [-0.5, n - 0.5) on every axis (no clamping needed).
 __device__ static inline float dh_trilinear_u(const float *__restrict__ vol, const dh_pad &p,

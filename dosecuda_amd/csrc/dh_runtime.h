@@ -6,6 +6,7 @@
 #include "dh_common.h"
 #include "../../include/dosehip.h"
 
+#include <functional>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -83,6 +84,27 @@ enum dh_slot {
     DH_SLOT_COUNT_
 };
 
+// ---------------------------------------------------------------------------
+// Tuning table: the variant a launcher uses for a (kernel, shape key), set by
+// the autotuner or the caller; launchers fall back to their default choice.
+// ---------------------------------------------------------------------------
+int dh_tune_lookup(const char *kernel, const std::string &key, int dflt);
+void dh_tune_store(const char *kernel, const std::string &key, int choice, double ms);
+// "nz,ny,nx,axis" with the dominant axis of the beam direction
+std::string dh_tune_key_geom(const dh_geom &g);
+
+// Autotuner helpers: count of differing 32-bit words of two device arrays
+// (synchronises s) and the median time of reps runs of f on s (ms).
+int64_t dh_count_diff(const void *a, const void *b, int64_t n_words, hipStream_t s);
+float dh_time_median(const std::function<void()> &f, int reps, hipStream_t s);
+// proton field geometry (no collimator; source at (VSADX + VSADY) / 2)
+dh_geom dh_proton_geom(const dh_grid *grid, const dh_proton_beam *beam);
+
+// Voxel tiles of the voxel-parallel kernels: choice 0 = linear runs along x,
+// 1.. = bricks (see dh_tile); grid receives the workgroup count.
+constexpr int DH_TILE_CHOICES = 8;
+dh_tile dh_make_tile(int choice, int nx, int ny, int nz, unsigned *grid);
+
 // Launch timing: when enabled, launches are bracketed by HIP events on the
 // launch stream under a kernel name.
 struct dh_timed {
@@ -128,11 +150,13 @@ struct dh_ccc_args {
 
 void dh_launch_pad(const float *src, float *dst, int nx, int ny, int nz, float border,
                    hipStream_t s);
-void dh_launch_raytrace(const dh_geom &g, const float *vol_pad, float *wet, hipStream_t s);
-void dh_launch_smooth(const dh_geom &g, const float *wet_raw, float *wet_out, hipStream_t s);
+// choice < 0: the tuning table's choice for the shape (default 0); the
+// pencil's choice is tile + DH_TILE_CHOICES * (LUT in LDS ? 0 : 1)
+void dh_launch_raytrace(const dh_geom &g, const float *vol_pad, float *wet, hipStream_t s, int choice = -1);
+void dh_launch_smooth(const dh_geom &g, const float *wet_raw, float *wet_out, hipStream_t s, int choice = -1);
 void dh_launch_pencil(const dh_geom &g, float vsadx, float vsady, int n_layers,
                       const dh_layer_k *d_layers, const float4 *d_luts, const float *d_spots,
-                      bool fold, const float *wet, float *dose, hipStream_t s);
+                      bool fold, const float *wet, float *dose, hipStream_t s, int choice = -1);
 void dh_launch_terma(const dh_terma_args &a, const float *wet, float *terma, float *electron,
                      hipStream_t s);
 void dh_launch_ccc(const dh_ccc_args &a, const float *terma, const float *density,

@@ -60,15 +60,11 @@ __device__ static inline int safe_samples(const dh_geom &g, float x, float y, fl
 }
 
 __global__ void __launch_bounds__(kBlock)
-k_raytrace(dh_geom g, dh_pad p, const float *__restrict__ vol, float *__restrict__ wet)
+k_raytrace(dh_geom g, dh_pad p, dh_tile tile, const float *__restrict__ vol, float *__restrict__ wet)
 {
-    const uint32_t nvox = (uint32_t)g.nx * g.ny * g.nz;
-    const uint32_t v = (uint32_t)dh_xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    if (v >= nvox) return;
-    const int i = (int)(v % (uint32_t)g.nx);
-    const uint32_t r = v / (uint32_t)g.nx;
-    const int j = (int)(r % (uint32_t)g.ny);
-    const int k = (int)(r / (uint32_t)g.ny);
+    int i, j, k;
+    if (!dh_tile_voxel(tile, dh_xcd_block(blockIdx.x, gridDim.x), threadIdx.x, g.nx, g.ny, g.nz, i, j, k)) return;
+    const uint32_t v = (uint32_t)i + (uint32_t)g.nx * ((uint32_t)j + (uint32_t)g.ny * (uint32_t)k);
     float x, y, z, ux, uy, uz;
     dh_vox_xyz(g, i, j, k, x, y, z);
     dh_uvec(g, x, y, z, ux, uy, uz);
@@ -133,15 +129,11 @@ struct smooth_dirs {
 // only radii below 10 * WET(cm).  Neighbour indices come from the reference's
 // FP32 head->image rotation and roundf().
 __global__ void __launch_bounds__(kBlock)
-k_smooth(dh_geom g, smooth_dirs dirs, const float *__restrict__ wet_in, float *__restrict__ wet_out)
+k_smooth(dh_geom g, smooth_dirs dirs, dh_tile tile, const float *__restrict__ wet_in, float *__restrict__ wet_out)
 {
-    const uint32_t nvox = (uint32_t)g.nx * g.ny * g.nz;
-    const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
-    if (v >= nvox) return;
-    const int i = (int)(v % (uint32_t)g.nx);
-    const uint32_t r = v / (uint32_t)g.nx;
-    const int j = (int)(r % (uint32_t)g.ny);
-    const int k = (int)(r / (uint32_t)g.ny);
+    int i, j, k;
+    if (!dh_tile_voxel(tile, dh_xcd_block(blockIdx.x, gridDim.x), threadIdx.x, g.nx, g.ny, g.nz, i, j, k)) return;
+    const uint32_t v = (uint32_t)i + (uint32_t)g.nx * ((uint32_t)j + (uint32_t)g.ny * (uint32_t)k);
     float x, y, z, hx, hy, hz;
     dh_vox_xyz(g, i, j, k, x, y, z);
     dh_img_to_head(g, x, y, z, hx, hy, hz);
@@ -174,21 +166,19 @@ k_smooth(dh_geom g, smooth_dirs dirs, const float *__restrict__ wet_in, float *_
 // With `Fold`, the normalisation is folded into the exponent,
 // P*exp(a d^2) = 2^(a' d^2 + log2 P), which requires P, H >= 0 (the host checks
 // that every IDD sample of the plan is non-negative).
-template <bool Fold>
+template <bool Fold, bool Lds>
 __global__ void __launch_bounds__(kBlock)
-k_pencil(dh_geom g, float vsadx, float vsady, int n_layers, const dh_layer_k *__restrict__ layers,
+k_pencil(dh_geom g, float vsadx, float vsady, int n_layers, dh_tile tile, const dh_layer_k *__restrict__ layers,
          const float4 *__restrict__ luts, const float *__restrict__ spots,
          const float *__restrict__ wet, float *__restrict__ dose)
 {
-    __shared__ float4 s_lut[DH_LUT_LENGTH];
-    const uint32_t nvox = (uint32_t)g.nx * g.ny * g.nz;
-    const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
-    const bool valid = v < nvox;
-    const uint32_t vv = valid ? v : nvox - 1;
-    const int i = (int)(vv % (uint32_t)g.nx);
-    const uint32_t r = vv / (uint32_t)g.nx;
-    const int j = (int)(r % (uint32_t)g.ny);
-    const int k = (int)(r / (uint32_t)g.ny);
+    __shared__ float4 s_lut[Lds ? DH_LUT_LENGTH : 1];
+    int i, j, k;
+    const bool valid = dh_tile_voxel(tile, dh_xcd_block(blockIdx.x, gridDim.x), threadIdx.x, g.nx, g.ny, g.nz,
+                                     i, j, k);
+    if (!valid) { i = g.nx - 1; j = g.ny - 1; k = g.nz - 1; }
+    const uint32_t vv = (uint32_t)i + (uint32_t)g.nx * ((uint32_t)j + (uint32_t)g.ny * (uint32_t)k);
+    const uint32_t v = vv;
     float x, y, z, hx, hy, hz;
     dh_vox_xyz(g, i, j, k, x, y, z);
     dh_img_to_head(g, x, y, z, hx, hy, hz);
@@ -205,15 +195,19 @@ k_pencil(dh_geom g, float vsadx, float vsady, int n_layers, const dh_layer_k *__
 
     for (int l = 0; l < n_layers; l++) {
         const dh_layer_k L = layers[l];
-        __syncthreads();
-        for (int q = threadIdx.x; q < DH_LUT_LENGTH; q += kBlock) s_lut[q] = luts[(size_t)l * DH_LUT_LENGTH + q];
-        __syncthreads();
+        const float4 *lut = luts + (size_t)l * DH_LUT_LENGTH;
+        if (Lds) {
+            __syncthreads();
+            for (int q = threadIdx.x; q < DH_LUT_LENGTH; q += kBlock) s_lut[q] = lut[q];
+            __syncthreads();
+            lut = s_lut;
+        }
         const bool act = valid && !(w > L.cut);
         if (!dh_wave_any(act)) continue;
         if (!act) continue;
 
         // LUT, sigma_air, halo and normalisations (IMPTClasses.cu:69-121,222-264)
-        const dh_pb_factors fc = dh_pencil_factors(s_lut, w, dist, L.r80, L.A, L.B, L.C);
+        const dh_pb_factors fc = dh_pencil_factors(lut, w, dist, L.r80, L.A, L.B, L.C);
         const float P = fc.P, H = fc.H, ap = fc.ap, ah = fc.ah;
         const float lp = Fold ? log2f(P) : P;
         const float lh = Fold ? log2f(H) : H;
@@ -257,16 +251,18 @@ void dh_launch_pad(const float *src, float *dst, int nx, int ny, int nz, float b
     DH_HIP(hipGetLastError());
 }
 
-void dh_launch_raytrace(const dh_geom &g, const float *vol_pad, float *wet, hipStream_t s)
+void dh_launch_raytrace(const dh_geom &g, const float *vol_pad, float *wet, hipStream_t s, int choice)
 {
-    size_t n = (size_t)g.nx * g.ny * g.nz;
     dh_pad p = dh_make_pad(g.nx, g.ny, g.nz);
+    if (choice < 0) choice = dh_tune_lookup("raytrace", dh_tune_key_geom(g), 0);
+    unsigned grid = 0;
+    const dh_tile tile = dh_make_tile(choice, g.nx, g.ny, g.nz, &grid);
     dh_timed t("raytrace", s);
-    hipLaunchKernelGGL(k_raytrace, dim3(grid_for(n)), dim3(kBlock), 0, s, g, p, vol_pad, wet);
+    hipLaunchKernelGGL(k_raytrace, dim3(grid), dim3(kBlock), 0, s, g, p, tile, vol_pad, wet);
     DH_HIP(hipGetLastError());
 }
 
-void dh_launch_smooth(const dh_geom &g, const float *wet_raw, float *wet_out, hipStream_t s)
+void dh_launch_smooth(const dh_geom &g, const float *wet_raw, float *wet_out, hipStream_t s, int choice)
 {
     smooth_dirs d;
     for (int a = 0; a < 6; a++) {           // IMPTClasses.cu:170
@@ -274,23 +270,35 @@ void dh_launch_smooth(const dh_geom &g, const float *wet_raw, float *wet_out, hi
         d.s[a] = sinf(ang);
         d.c[a] = cosf(ang);
     }
-    size_t n = (size_t)g.nx * g.ny * g.nz;
+    if (choice < 0) choice = dh_tune_lookup("smooth", dh_tune_key_geom(g), 0);
+    unsigned grid = 0;
+    const dh_tile tile = dh_make_tile(choice, g.nx, g.ny, g.nz, &grid);
     dh_timed t("smooth", s);
-    hipLaunchKernelGGL(k_smooth, dim3(grid_for(n)), dim3(kBlock), 0, s, g, d, wet_raw, wet_out);
+    hipLaunchKernelGGL(k_smooth, dim3(grid), dim3(kBlock), 0, s, g, d, tile, wet_raw, wet_out);
     DH_HIP(hipGetLastError());
 }
 
 void dh_launch_pencil(const dh_geom &g, float vsadx, float vsady, int n_layers, const dh_layer_k *d_layers,
                       const float4 *d_luts, const float *d_spots, bool fold, const float *wet,
-                      float *dose, hipStream_t s)
+                      float *dose, hipStream_t s, int choice)
 {
-    size_t n = (size_t)g.nx * g.ny * g.nz;
+    if (choice < 0) choice = dh_tune_lookup("pencil", dh_tune_key_geom(g), 0);
+    DH_REQUIRE(choice >= 0 && choice < 2 * DH_TILE_CHOICES, "pencil choice out of range");
+    const bool lds = choice < DH_TILE_CHOICES;
+    unsigned grid = 0;
+    const dh_tile tile = dh_make_tile(choice % DH_TILE_CHOICES, g.nx, g.ny, g.nz, &grid);
     dh_timed t("pencil", s);
-    if (fold)
-        hipLaunchKernelGGL(k_pencil<true>, dim3(grid_for(n)), dim3(kBlock), 0, s, g, vsadx, vsady, n_layers,
+    if (fold && lds)
+        hipLaunchKernelGGL((k_pencil<true, true>), dim3(grid), dim3(kBlock), 0, s, g, vsadx, vsady, n_layers, tile,
+                           d_layers, d_luts, d_spots, wet, dose);
+    else if (fold)
+        hipLaunchKernelGGL((k_pencil<true, false>), dim3(grid), dim3(kBlock), 0, s, g, vsadx, vsady, n_layers, tile,
+                           d_layers, d_luts, d_spots, wet, dose);
+    else if (lds)
+        hipLaunchKernelGGL((k_pencil<false, true>), dim3(grid), dim3(kBlock), 0, s, g, vsadx, vsady, n_layers, tile,
                            d_layers, d_luts, d_spots, wet, dose);
     else
-        hipLaunchKernelGGL(k_pencil<false>, dim3(grid_for(n)), dim3(kBlock), 0, s, g, vsadx, vsady, n_layers,
-                           d_layers, d_luts, d_spots, wet, dose);
+        hipLaunchKernelGGL((k_pencil<false, false>), dim3(grid), dim3(kBlock), 0, s, g, vsadx, vsady, n_layers,
+                           tile, d_layers, d_luts, d_spots, wet, dose);
     DH_HIP(hipGetLastError());
 }

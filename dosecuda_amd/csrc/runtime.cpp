@@ -122,6 +122,115 @@ void dh_device_release(int dev)
 }
 
 // ---------------------------------------------------------------------------
+// tuning table
+// ---------------------------------------------------------------------------
+namespace {
+struct tune_entry {
+    std::string kernel, key;
+    int choice;
+    double ms;
+};
+std::vector<tune_entry> g_tune;
+std::mutex g_tune_mu;
+}  // namespace
+
+int dh_tune_lookup(const char *kernel, const std::string &key, int dflt)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (const auto &e : g_tune)
+        if (e.kernel == kernel && e.key == key) return e.choice;
+    return dflt;
+}
+
+void dh_tune_store(const char *kernel, const std::string &key, int choice, double ms)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (auto &e : g_tune)
+        if (e.kernel == kernel && e.key == key) {
+            e.choice = choice;
+            e.ms = ms;
+            return;
+        }
+    g_tune.push_back(tune_entry{kernel, key, choice, ms});
+}
+
+std::string dh_tune_key_geom(const dh_geom &g)
+{
+    const float a0 = std::fabs(g.src[0]), a1 = std::fabs(g.src[1]), a2 = std::fabs(g.src[2]);
+    const int axis = (a0 >= a1 && a0 >= a2) ? 0 : (a1 >= a2 ? 1 : 2);
+    return std::to_string(g.nz) + "," + std::to_string(g.ny) + "," + std::to_string(g.nx) + "," +
+           std::to_string(axis);
+}
+
+dh_tile dh_make_tile(int choice, int nx, int ny, int nz, unsigned *grid)
+{
+    static const int shapes[DH_TILE_CHOICES][3] = {{0, 0, 0},  {64, 4, 1}, {32, 8, 1}, {16, 16, 1},
+                                                   {32, 4, 2}, {16, 4, 4}, {8, 8, 4},  {64, 2, 2}};
+    DH_REQUIRE(choice >= 0 && choice < DH_TILE_CHOICES, "tile choice out of range");
+    dh_tile t;
+    t.tx = shapes[choice][0];
+    t.ty = shapes[choice][1];
+    t.tz = shapes[choice][2];
+    if (t.tx == 0) {
+        t.ntx = t.nty = 0;
+        *grid = (unsigned)(((size_t)nx * ny * nz + 255) / 256);
+    } else {
+        t.ntx = (nx + t.tx - 1) / t.tx;
+        t.nty = (ny + t.ty - 1) / t.ty;
+        *grid = (unsigned)((size_t)t.ntx * t.nty * ((nz + t.tz - 1) / t.tz));
+    }
+    return t;
+}
+
+extern "C" int dh_tune_set(const char *kernel, const char *key, int32_t choice, double ms)
+{
+    return dh_guarded([&] {
+        DH_REQUIRE(kernel && key, "null argument");
+        dh_tune_store(kernel, key, choice, ms);
+    });
+}
+
+extern "C" int dh_tune_get(const char *kernel, const char *key, int32_t *choice)
+{
+    return dh_guarded([&] {
+        DH_REQUIRE(kernel && key && choice, "null argument");
+        *choice = dh_tune_lookup(kernel, key, -1);
+    });
+}
+
+extern "C" int dh_tune_count(int32_t *n)
+{
+    return dh_guarded([&] {
+        DH_REQUIRE(n, "null argument");
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        *n = (int32_t)g_tune.size();
+    });
+}
+
+extern "C" int dh_tune_entry(int32_t index, char *kernel, int32_t kernel_cap, char *key, int32_t key_cap,
+                             int32_t *choice, double *ms)
+{
+    return dh_guarded([&] {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        DH_REQUIRE(index >= 0 && index < (int32_t)g_tune.size(), "entry index out of range");
+        const tune_entry &e = g_tune[index];
+        DH_REQUIRE(kernel && key && kernel_cap > (int32_t)e.kernel.size() && key_cap > (int32_t)e.key.size(),
+                   "name buffers too small");
+        std::memcpy(kernel, e.kernel.c_str(), e.kernel.size() + 1);
+        std::memcpy(key, e.key.c_str(), e.key.size() + 1);
+        if (choice) *choice = e.choice;
+        if (ms) *ms = e.ms;
+    });
+}
+
+extern "C" int dh_tune_clear(void)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune.clear();
+    return DH_OK;
+}
+
+// ---------------------------------------------------------------------------
 // launch timing
 // ---------------------------------------------------------------------------
 dh_timed::dh_timed(const char *n, hipStream_t st) : name(n), s(st), ev0(nullptr)

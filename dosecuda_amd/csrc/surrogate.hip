@@ -653,10 +653,23 @@ void launch_pool(const uint16_t *x, int n, int d, int h, int w, int c, uint16_t 
 
 // Picks the kernel variant (explicit nco / ksplit, or the library's choice)
 // and launches it; throws dh_error on bad arguments.
+std::string conv_key(const conv_args &a)
+{
+    char b[160];
+    snprintf(b, sizeof(b), "%d,%d,%d,%d,%d,%d,%d,%d,%d", a.N, a.D, a.H, a.W, a.ca, a.cb, a.up, a.cout, a.ksize);
+    return b;
+}
+
 void launch_conv3d(const conv_args &a, int nco, int ksplit, hipStream_t s)
 {
     const int64_t nvox = (int64_t)a.N * a.D * a.H * a.W;
     const int ki = a.ksize * a.ksize * a.ksize * (a.cin / 16);
+    if (nco == 0 && ksplit == 0) {
+        // tuned variant: 1000 + nco for the brick kernel, else nco * 100 + ksplit
+        const int code = dh_tune_lookup("conv3d", conv_key(a), -1);
+        if (code >= 1000) { nco = code - 1000; ksplit = -1; }
+        else if (code > 0) { nco = code / 100; ksplit = code % 100; }
+    }
     int c = nco, k = ksplit;
     // brick kernel: explicit (ksplit = -1), or by default for 3x3x3 layers
     // over large volumes with at most 64 output channels
@@ -1065,5 +1078,89 @@ extern "C" int dh_gamma_dev(const float *eval, const float *ref, int32_t nz, int
         hipLaunchKernelGGL(k_gamma, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
                            eval, ref, nx, ny, nz, spacing, dta_mm, dd_abs, cut, radius_vox, out);
         DH_HIP(hipGetLastError());
+    });
+}
+
+namespace {
+
+__global__ void k_fill_bf16_hash(uint16_t *p, int64_t n, uint32_t seed, float scale)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t h = (uint32_t)i * 2654435761u ^ seed;
+        h ^= h >> 15; h *= 2246822519u; h ^= h >> 13;
+        p[i] = f2bf(((float)(h & 0xffffu) / 65536.0f - 0.5f) * scale);
+    }
+}
+
+}  // namespace
+
+// Tunes the convolution variant of one layer shape on synthetic data: every
+// (output tiles, K split) pair and the brick kernel; a variant is admitted
+// when its fp32 output matches the first variant's to 1e-5 of the largest
+// magnitude (the variants differ only in summation order); the fastest is
+// stored for the shape.  codes[] / ms[] receive up to 16 tried variants.
+extern "C" int dh_autotune_conv3d(int gpu_id, int32_t n, int32_t d, int32_t h, int32_t w, int32_t ca, int32_t cb,
+                                  int32_t up, int32_t cout, int32_t ksize, int32_t reps, int32_t *best_code,
+                                  int32_t *codes, float *ms, int32_t *n_tried, void *stream)
+{
+    return dh_guarded([&] {
+        dh_device_enter(gpu_id);
+        hipStream_t s = (hipStream_t)stream;
+        const int cout_pad = (cout + 31) / 32 * 32;
+        const int cin = ca + cb;
+        const int64_t nvox = (int64_t)n * d * h * w;
+        const int64_t na = (int64_t)n * (d >> up) * (h >> up) * (w >> up) * ca, nb = nvox * cb;
+        const int64_t nw = (int64_t)ksize * ksize * ksize * cout_pad * cin;
+        uint16_t *xa = (uint16_t *)dh_scratch(gpu_id, DH_SLOT_IN0, sizeof(uint16_t) * (size_t)na);
+        uint16_t *xb = cb ? (uint16_t *)dh_scratch(gpu_id, DH_SLOT_IN1, sizeof(uint16_t) * (size_t)nb) : nullptr;
+        uint16_t *wt = (uint16_t *)dh_scratch(gpu_id, DH_SLOT_TMP0, sizeof(uint16_t) * (size_t)nw);
+        float *ref = (float *)dh_scratch(gpu_id, DH_SLOT_OUT0, sizeof(float) * (size_t)(nvox * cout));
+        float *out = (float *)dh_scratch(gpu_id, DH_SLOT_OUT1, sizeof(float) * (size_t)(nvox * cout));
+        hipLaunchKernelGGL(k_fill_bf16_hash, dim3(1024), dim3(256), 0, s, xa, na, 17u, 2.0f);
+        if (cb) hipLaunchKernelGGL(k_fill_bf16_hash, dim3(1024), dim3(256), 0, s, xb, nb, 29u, 2.0f);
+        hipLaunchKernelGGL(k_fill_bf16_hash, dim3(1024), dim3(256), 0, s, wt, nw, 41u, 0.2f);
+        std::vector<int> cand;
+        if (ksize == 3)
+            for (int c : {1, 2})
+                if (cout_pad % (32 * c) == 0) cand.push_back(1000 + c);
+        for (int c : {1, 2, 4})
+            for (int k : {1, 2, 4, 8, 16}) {
+                if ((cout_pad / 32) % c || c * k > 16 || (c == 2 && k == 16) || (c == 4 && k > 4)) continue;
+                cand.push_back(c * 100 + k);
+            }
+        std::vector<float> host_ref, host_out;
+        int best = -1, tried = 0;
+        float best_ms = 0.0f;
+        for (size_t q = 0; q < cand.size() && tried < 16; q++) {
+            const int code = cand[q];
+            const int c = code >= 1000 ? code - 1000 : code / 100, k = code >= 1000 ? -1 : code % 100;
+            conv_args a = make_conv(xa, ca, up, xb, cb, n, d, h, w, wt, nullptr, cout, cout_pad, ksize, 0, 0.1f,
+                                    nullptr, tried == 0 ? ref : out);
+            const float t = dh_time_median([&] { launch_conv3d(a, c, k, s); }, reps, s);
+            bool ok = true;
+            if (tried > 0) {
+                host_ref.resize((size_t)(nvox * cout));
+                host_out.resize((size_t)(nvox * cout));
+                DH_HIP(hipMemcpyAsync(host_ref.data(), ref, sizeof(float) * host_ref.size(), hipMemcpyDeviceToHost, s));
+                DH_HIP(hipMemcpyAsync(host_out.data(), out, sizeof(float) * host_out.size(), hipMemcpyDeviceToHost, s));
+                DH_HIP(hipStreamSynchronize(s));
+                float mx = 0.0f, md = 0.0f;
+                for (size_t i = 0; i < host_ref.size(); i++) {
+                    mx = std::max(mx, std::fabs(host_ref[i]));
+                    md = std::max(md, std::fabs(host_ref[i] - host_out[i]));
+                }
+                ok = md <= 1e-5f * mx;
+            }
+            if (codes) codes[tried] = ok ? code : -code;
+            if (ms) ms[tried] = t;
+            tried++;
+            if (ok && (best < 0 || t < best_ms)) { best = code; best_ms = t; }
+        }
+        DH_REQUIRE(best > 0, "no convolution variant ran");
+        conv_args a = make_conv(xa, ca, up, xb, cb, n, d, h, w, wt, nullptr, cout, cout_pad, ksize, 0, 0.1f, nullptr,
+                                out);
+        dh_tune_store("conv3d", conv_key(a), best, best_ms);
+        if (best_code) *best_code = best;
+        if (n_tried) *n_tried = tried;
     });
 }

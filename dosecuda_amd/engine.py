@@ -102,8 +102,16 @@ class InfluenceMatrix:
     (`base`) and the shared col/val tensors."""
 
     def __init__(self, device: int, shape, spacing, origin, iso, gantry, couch, model, spot_rows,
-                 spots_per_block=16, row_chunks=16, rel=1e-4):
+                 spots_per_block=None, row_chunks=None, rel=1e-4):
         self._keep: list = []
+        if spots_per_block is None or row_chunks is None:
+            # the autotuner's choice for this shape and spot count, else 16 / 16
+            ch = C.c_int32()
+            key = f"{int(shape[0])},{int(shape[1])},{int(shape[2])},{len(spot_rows)}"
+            _lib.check(_lib.load().dh_tune_get(b"influence", key.encode(), C.byref(ch)))
+            tS, tC = (ch.value // 1000, ch.value % 1000) if ch.value > 0 else (16, 16)
+            spots_per_block = tS if spots_per_block is None else spots_per_block
+            row_chunks = tC if row_chunks is None else row_chunks
         lut = _lib.proton_lut(model.divergence_params, model.lut_depths, model.lut_sigmas,
                               model.lut_idds, self._keep)
         rows = np.ascontiguousarray(np.asarray(spot_rows, dtype=np.float32).reshape(-1, 4))

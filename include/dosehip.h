@@ -329,6 +329,36 @@ int dh_sched_run(dh_sched *sched, int32_t mode, void *stream);
 int dh_sched_info(const dh_sched *sched, int32_t *n_cases, int32_t *n_jobs, double *last_ms,
                   int32_t *case_stream, double *job_ms);
 
+/* ---- autotuner --------------------------------------------------------- */
+/* No reference counterpart (the reference's launch shapes are compile-time
+ * constants, CudaClasses.cuh:14).  The tuning table maps (kernel, shape key)
+ * to the variant its launcher uses; the tuners time every variant on the
+ * caller's data (median of reps), require the outputs to agree (bit for bit
+ * for the voxel-tile and LUT-staging variants, to 1e-5 of the largest value
+ * for the convolution's summation orders) and store the fastest.  Keys:
+ * "raytrace" / "smooth" / "pencil": "nz,ny,nx,axis" (dominant axis of the
+ * beam direction); "conv3d": "n,d,h,w,ca,cb,up,cout,ksize". */
+int dh_tune_set(const char *kernel, const char *key, int32_t choice, double ms);
+int dh_tune_get(const char *kernel, const char *key, int32_t *choice);   /* -1: none */
+int dh_tune_count(int32_t *n);
+int dh_tune_entry(int32_t index, char *kernel, int32_t kernel_cap, char *key, int32_t key_cap,
+                  int32_t *choice, double *ms);
+int dh_tune_clear(void);
+/* Ray-trace and smoothing voxel tiles (8 choices each) for a proton field;
+ * ms arrays hold 8 entries. */
+int dh_autotune_proton_wet(int gpu_id, const dh_grid *grid, const dh_proton_beam *beam,
+                           const float *d_rlsp, int32_t reps, int32_t *best_raytrace,
+                           int32_t *best_smooth, float *ms_raytrace, float *ms_smooth, void *stream);
+/* Pencil-beam voxel tile x LUT staging (LDS or global): 16 choices. */
+int dh_autotune_pencil(dh_proton_plan *plan, const float *d_wet, int32_t reps, int32_t *best,
+                       float *ms, void *stream);
+/* Convolution variant of one layer shape (codes: 1000 + nco for the LDS
+ * brick kernel, nco * 100 + ksplit otherwise; a rejected variant's code is
+ * reported negated); up to 16 variants. */
+int dh_autotune_conv3d(int gpu_id, int32_t n, int32_t d, int32_t h, int32_t w, int32_t ca,
+                       int32_t cb, int32_t up, int32_t cout, int32_t ksize, int32_t reps,
+                       int32_t *best_code, int32_t *codes, float *ms, int32_t *n_tried, void *stream);
+
 /* ---- kernel timing (HIP events on the launch stream) ------------------ */
 /* When enabled, every kernel launch of the library is bracketed by HIP
  * events; dh_prof_get sums the elapsed times of the launches of one kernel
