@@ -9,8 +9,12 @@ model: per beam ray trace + TERMA + 72-cone superposition; density resident
 in HBM.  `value` is ms per beam.  The same line carries the plain IMPT field
 (100^3 at 2.5 mm, water with a seeded 2 cm bone slab, 20x20 spots x 10
 layers of 100-160 MeV, unit MU) in Gvoxel-spot/s, the roofline of the
-dominant kernel from HIP events around its launches, and the FP64 oracle
-timed on a bounded sample on the host cores.
+dominant kernel from HIP events around its launches, the FP64 oracle timed
+on a bounded sample on the host cores, the batched influence matrix
+(160x256x256 at 2 mm, 3 fields, ~15k spots), the learned surrogate CCCS on
+the same 7-beam plan (gamma against the plain CCCS, speed-up, MFMA use) and
+the mixed workload of 64 seeded cases run as one HIP graph.  The launch
+shapes are autotuned first (--no-tune skips it).
 """
 from __future__ import annotations
 
@@ -36,8 +40,13 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     p.add_argument("--cpu-seconds", type=float, default=15.0,
                    help="approximate budget of the CPU baseline sample")
-    p.add_argument("--workload", default="all", choices=["all", "cccs", "impt", "influence"])
+    p.add_argument("--workload", default="all",
+                   choices=["all", "cccs", "impt", "influence", "surrogate", "mixed"])
     p.add_argument("--influence-steps", type=int, default=3)
+    p.add_argument("--no-tune", action="store_true",
+                   help="skip the autotuners (launch shapes from the cache or the defaults)")
+    p.add_argument("--mixed-cases", type=int, default=64)
+    p.add_argument("--mixed-steps", type=int, default=1)
     p.add_argument("--copy-calib-mb", type=int, default=0,
                    help="after the run, copy a buffer of this many MiB once (a counter "
                         "calibration kernel of known traffic for rocprofv3 --pmc runs)")
@@ -240,6 +249,15 @@ def main():
     if a.workload in ("all", "impt"):
         log("setting up IMPT 100^3 field")
         st = setup_impt(torch)
+        if not a.no_tune:
+            from dosecuda_amd import autotune
+            c, m = st["case"], st["m"]
+            tw = autotune.tune_proton_wet(st["d_rlsp"], c.spacing, c.origin, c.iso, 0.0, 0.0, m, reps=5)
+            run_impt(torch, st)
+            tp = autotune.tune_pencil(st["plan"], st["wet"], reps=5)
+            res["impt_tuning"] = dict(raytrace=tw["raytrace"], smooth=tw["smooth"], pencil_tile=tp["tile"],
+                                      pencil_lut_in_lds=tp["lut_in_lds"])
+            log(f"IMPT tuning: {res['impt_tuning']}")
         ms = timed(torch, run_impt, st, a.steps, a.warmup)
         wet = st["wet"].cpu().numpy()
         pairs = st["plan"].pairs(wet)
@@ -260,6 +278,18 @@ def main():
     if a.workload in ("all", "cccs"):
         log("setting up CCCS 128^3 x 7 beams")
         st = setup_cccs(torch)
+        if not a.no_tune:
+            from dosecuda_amd import autotune
+            c = st["case"]
+            # the choices are keyed by shape and dominant beam axis: one
+            # gantry angle per axis the 7 beams use
+            for g in (0.0, 51.42857142857143):
+                tw = autotune.tune_photon(st["d_dens"], c.spacing, c.origin, c.iso, 100.0, g, 0.0, 0.0, st["pm"],
+                                          st["mlc"], reps=5)
+                res.setdefault("cccs_tuning", {})[f"gantry {g:.1f}"] = dict(
+                    raytrace=tw["raytrace"], ccc_layout=tw["ccc_layout"],
+                    ccc_ms={"pairs": tw["ccc_ms"][0], "quads": tw["ccc_ms"][1]})
+            log(f"CCCS tuning: {res['cccs_tuning']}")
         ms = timed(torch, run_cccs, st, a.steps, a.warmup)
         nb = len(st["angles"])
         kt = {k: engine.prof_get(k) for k in ("pad", "raytrace", "terma", "pack_tr", "ccc_compact", "ccc")}
@@ -321,6 +351,77 @@ def main():
             f.close()
         del st
 
+    if a.workload in ("all", "surrogate"):
+        from dosecuda_amd import autotune, surrogate as S
+        from dosecuda_amd.beam_models import PhotonBeamModel
+        if not os.path.exists(S.WEIGHTS):
+            log(f"surrogate: no trained weights at {S.WEIGHTS}; skipped")
+        else:
+            net = S.load(S.WEIGHTS)
+            if not a.no_tune:
+                t0 = time.time()
+                autotune.tune_surrogate(net, (64, 64, 64), reps=5)
+                log(f"surrogate: convolution variants tuned ({time.time() - t0:.1f} s)")
+            log("surrogate: 7-beam plan, checked surrogate vs plain CCCS")
+            r = S.evaluate(net, PhotonBeamModel(), stride=4, tol=0.01, reps=5)
+            conv_ms = float(np.mean([b["surrogate_stage_ms"]["nn_conv3d"] for b in r["beams"]]))
+            tf = r["net_gflop"] / (conv_ms * 1e-3) / 1e3
+            res["surrogate"] = dict(
+                weights=os.path.relpath(S.WEIGHTS, ROOT), params=r["params"], stride=r["stride"], tol=r["tol"],
+                plan_gamma_checked=r["plan_gamma_checked"], min_beam_gamma_checked=r["min_gamma_checked"],
+                plan_gamma_net_only=r["plan_gamma_net"], min_beam_gamma_net_only=r["min_gamma_net"],
+                max_err_checked=max(b["max_err_checked"] for b in r["beams"]),
+                rms_err_checked=max(b["rms_err_checked"] for b in r["beams"]),
+                mean_fallback_fraction=r["mean_fallback_fraction"],
+                checked_ms_per_beam=r["graph_checked_ms_per_beam"],
+                plain_superposition_ms_per_beam=r["plain_superposition_ms"],
+                speedup_vs_plain_superposition=r["speedup_superposition"],
+                plain_beam_ms=r["plain_beam_ms"],
+                surrogate_beam_ms=r["plain_beam_ms"] - r["plain_superposition_ms"] + r["graph_checked_ms_per_beam"],
+                mfma=dict(net_gflop=r["net_gflop"], conv_ms_per_beam=conv_ms, achieved_tflops=tf, peak_tflops=2500.0,
+                          frac=tf / 2500.0),
+                graph_replay_equal=r["graph_replay_equal"])
+            log(f"surrogate: {r['graph_checked_ms_per_beam']:.3f} ms/beam checked vs {r['plain_superposition_ms']:.3f} "
+                f"plain superposition ({r['speedup_superposition']:.2f}x), plan gamma {r['plan_gamma_checked']:.5f}, "
+                f"min beam {r['min_gamma_checked']:.5f}, fallback {r['mean_fallback_fraction']:.3f}, "
+                f"MFMA {tf:.0f} TFLOP/s")
+
+    if a.workload in ("all", "mixed"):
+        from dosecuda_amd import mixed, phantoms, surrogate as S
+        from dosecuda_amd.beam_models import PhotonBeamModel, ProtonBeamModel
+        pm, xm = ProtonBeamModel(), PhotonBeamModel()
+        cases = phantoms.mixed_cases(pm, xm, seed=1005, n_cases=a.mixed_cases)
+        nat = S.NativeNet(S.load(S.WEIGHTS)) if os.path.exists(S.WEIGHTS) else None
+        log(f"mixed: {len(cases)} cases, surrogate {'on' if nat else 'off (no weights)'}")
+        W = mixed.MixedWorkload(cases, pm, xm, surrogate=nat, streams=4, tol=0.01, log=log)
+        cnt = W.counts()
+        log(f"mixed: set up in {W.setup_s:.1f} s: {cnt}")
+        t0 = time.time()
+        W.run(graph=True)          # eager warm-up (sizes the workspaces, times the jobs), capture, launch
+        torch.cuda.synchronize()
+        log(f"mixed: warm-up + capture + first replay {time.time() - t0:.1f} s")
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(a.mixed_steps):
+            W.run(graph=True)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / a.mixed_steps
+        errs = W.errors()
+        imrt = [e for e in errs if e["kind"] == "imrt"]
+        impt = [e for e in errs if e["kind"] == "impt"]
+        res["mixed"] = dict(
+            counts=cnt, ms_per_batch=ms, cases_per_s=cnt["cases"] / (ms * 1e-3),
+            nnz_per_s=cnt["nnz"] / (ms * 1e-3), voxel_cps_per_s=cnt["voxel_cps"] / (ms * 1e-3),
+            surrogate=nat is not None,
+            imrt_min_gamma=min((e["gamma"] for e in imrt), default=None),
+            imrt_max_err=max((e["max_err"] for e in imrt), default=None),
+            impt_max_err_csr_vs_dense=max((e["max_err"] for e in impt), default=None),
+            per_case=errs)
+        log(f"mixed: {ms / 1e3:.2f} s per batch, {res['mixed']['cases_per_s']:.1f} cases/s, min IMRT gamma "
+            f"{res['mixed']['imrt_min_gamma']}, max IMPT CSR error {res['mixed']['impt_max_err_csr_vs_dense']}")
+        W.close()
+
     cc = res.get("cccs")
     line = {
         "metric": "IMPT Gvoxel-spot/s + CCCS ms/beam; surrogate vs plain at 2%/2mm gamma>=99%",
@@ -343,9 +444,13 @@ def main():
         "cpu_baseline": ({"value": cpu["cccs"]["value"], "unit": "ms/beam", "cores": threads,
                           "kind": "port", "sample": cpu["cccs"]["sample"]} if "cccs" in cpu else None),
         "impt": res.get("impt"),
+        "tuning": {k: res[k] for k in ("impt_tuning", "cccs_tuning") if k in res},
         "impt_cpu_baseline": ({**cpu["impt"], "cores": threads, "kind": "port"} if "impt" in cpu else None),
         "cccs_detail": cc,
         "influence": res.get("influence"),
+        "surrogate": res.get("surrogate"),
+        "mixed": res.get("mixed"),
+        "autotuned": not a.no_tune,
     }
     print(json.dumps(line))
     if a.copy_calib_mb:

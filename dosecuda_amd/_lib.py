@@ -103,7 +103,7 @@ SIGNATURES = {
                          C.c_int32, _P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_float,
                          _P, _P, C.c_int32, C.c_int32, _P],
     "dh_nn_pool2_dev": [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P, _P],
-    "dh_sur_pack_dev": [C.c_int32, C.c_int32, C.c_int32, _P, _P, _P, _P, _P, _P],
+    "dh_sur_pack_dev": [C.c_int32, C.c_int32, C.c_int32, _P, _P, _P, C.c_float, _P, _P, _P],
     "dh_sur_finish_dev": [C.c_int32, C.c_int32, C.c_int32, _P, _P, C.c_float, _P, _P, C.c_float,
                           _P, _P, _P],
     "dh_sur_check_work_size": [C.POINTER(Grid), C.c_int32, C.POINTER(C.c_int64)],
@@ -141,6 +141,8 @@ SIGNATURES = {
     "dh_autotune_proton_wet": [_I, C.POINTER(Grid), C.POINTER(ProtonBeam), _P, C.c_int32, C.POINTER(C.c_int32),
                                C.POINTER(C.c_int32), _P, _P, _P],
     "dh_autotune_pencil": [_P, _P, C.c_int32, C.POINTER(C.c_int32), _P, _P],
+    "dh_autotune_photon": [_I, C.POINTER(Grid), C.POINTER(PhotonCP), C.POINTER(PhotonModel), _P, C.c_int32, _P,
+                           C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _P, _P, _P],
     "dh_autotune_conv3d": [_I, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                            C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_int32), _P, _P, C.POINTER(C.c_int32), _P],
     "dh_prof_enable": [_I],

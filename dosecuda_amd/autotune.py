@@ -98,6 +98,23 @@ def tune_proton_wet(rlsp: torch.Tensor, spacing, origin, iso, gantry, couch, mod
     return dict(raytrace=TILES[br.value], raytrace_ms=mr.tolist(), smooth=TILES[bs.value], smooth_ms=ms.tolist())
 
 
+def tune_photon(density: torch.Tensor, spacing, origin, iso, mu, ga, ca, ta, model, mlc, reps=5, stream=None):
+    """Ray-trace tile and cone-superposition layout of one photon control
+    point (the choices apply to every control point of that shape and beam
+    axis)."""
+    keep: list = []
+    m = engine.photon_args(model, keep)
+    rows = np.ascontiguousarray(np.asarray(mlc, dtype=np.float32)[:, :4])
+    br, bl = C.c_int32(), C.c_int32()
+    mr, mc = np.zeros(len(TILES), np.float32), np.zeros(2, np.float32)
+    _lib.check(_lib.load().dh_autotune_photon(
+        engine._dev(density), _lib.grid(density.shape, spacing, origin), _lib.photon_cp(iso, mu, ga, ca, ta), m,
+        _lib.ptr(rows), rows.shape[0], _lib.ptr(density), int(reps), C.byref(br), C.byref(bl), _lib.ptr(mr),
+        _lib.ptr(mc), engine._stream(stream)))
+    return dict(raytrace=TILES[br.value], raytrace_ms=mr.tolist(), ccc_layout=["pairs", "quads"][bl.value],
+                ccc_ms=mc.tolist())
+
+
 def tune_pencil(plan: "engine.ProtonPlan", wet: torch.Tensor, reps=5, stream=None):
     b = C.c_int32()
     ms = np.zeros(2 * len(TILES), np.float32)

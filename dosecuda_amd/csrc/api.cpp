@@ -89,9 +89,9 @@ static void proton_wet_device(int dev, const dh_grid *grid, const dh_proton_beam
     DH_REQUIRE(beam && d_rlsp && d_wet_out, "null argument");
     dh_geom g = dh_proton_geom(grid, beam);
     dh_pad p = dh_make_pad(g.nx, g.ny, g.nz);
-    float *vpad = (float *)dh_scratch(dev, DH_SLOT_VOL_PAD, sizeof(float) * (size_t)p.pxy * p.pz);
+    float4 *vpad = (float4 *)dh_scratch(dev, DH_SLOT_VOL_PAD, sizeof(float4) * (size_t)p.pxy * p.pz);
     float *raw = d_wet_raw ? d_wet_raw : (float *)dh_scratch(dev, DH_SLOT_WET_RAW, sizeof(float) * n);
-    dh_launch_pad(d_rlsp, vpad, g.nx, g.ny, g.nz, 0.0f, s);      // IMPTClasses.cu:292, border 0
+    dh_launch_pad_quad(d_rlsp, vpad, g.nx, g.ny, g.nz, 0.0f, s);      // IMPTClasses.cu:292, border 0
     dh_launch_raytrace(g, vpad, raw, s);
     dh_launch_smooth(g, raw, d_wet_out, s);
 }
@@ -147,8 +147,8 @@ extern "C" int dh_stage_raytrace_dev(int gpu_id, const dh_grid *grid, const floa
         DH_REQUIRE(iso && d_vol && d_wet_out, "null argument");
         dh_geom g = dh_make_geom(*grid, iso, gantry, couch, 0.0, src_dist);
         dh_pad p = dh_make_pad(g.nx, g.ny, g.nz);
-        float *vpad = (float *)dh_scratch(gpu_id, DH_SLOT_VOL_PAD, sizeof(float) * (size_t)p.pxy * p.pz);
-        dh_launch_pad(d_vol, vpad, g.nx, g.ny, g.nz, border, (hipStream_t)stream);
+        float4 *vpad = (float4 *)dh_scratch(gpu_id, DH_SLOT_VOL_PAD, sizeof(float4) * (size_t)p.pxy * p.pz);
+        dh_launch_pad_quad(d_vol, vpad, g.nx, g.ny, g.nz, border, (hipStream_t)stream);
         dh_launch_raytrace(g, vpad, d_wet_out, (hipStream_t)stream);
     });
 }
@@ -477,14 +477,14 @@ static void photon_device(int dev, const dh_grid *grid, const dh_photon_cp *cp,
     dh_terma_args ta = make_terma_args(grid, cp, m, mlc_rows, n_pairs);
     dh_ccc_args ca = make_ccc_args(grid, cp, m);
     dh_pad p = ca.p;
-    float *vpad = (float *)dh_scratch(dev, DH_SLOT_VOL_PAD, sizeof(float) * (size_t)p.pxy * p.pz);
+    float4 *vpad = (float4 *)dh_scratch(dev, DH_SLOT_VOL_PAD, sizeof(float4) * (size_t)p.pxy * p.pz);
     float *wet = d_wet ? d_wet : (float *)dh_scratch(dev, DH_SLOT_WET_RAW, sizeof(float) * n);
     float *terma = d_terma ? d_terma : (float *)dh_scratch(dev, DH_SLOT_TMP0, sizeof(float) * n);
     float *el = d_electron ? d_electron : (float *)dh_scratch(dev, DH_SLOT_TMP1, sizeof(float) * n);
-    float2 *trp = (float2 *)dh_scratch(dev, DH_SLOT_TR_PAD, sizeof(float2) * (size_t)p.pxy * p.pz);
+    void *trp = dh_scratch(dev, DH_SLOT_TR_PAD, dh_tr_pad_bytes(ca));
     int *act = (int *)dh_scratch(dev, DH_SLOT_ACTIVE, sizeof(int) * n);
     int *cnt = (int *)dh_scratch(dev, DH_SLOT_COUNTER, 256);
-    dh_launch_pad(d_density, vpad, ca.g.nx, ca.g.ny, ca.g.nz, DH_AIR_DENSITY, s);  // IMRTClasses.cu:394
+    dh_launch_pad_quad(d_density, vpad, ca.g.nx, ca.g.ny, ca.g.nz, DH_AIR_DENSITY, s);  // IMRTClasses.cu:394
     dh_launch_raytrace(ca.g, vpad, wet, s);
     dh_launch_terma(ta, wet, terma, el, s);
     if (net) {
@@ -557,7 +557,7 @@ extern "C" int dh_stage_ccc_dev(int gpu_id, const dh_grid *grid, const dh_photon
         DH_REQUIRE(cp && d_terma && d_density && d_electron && d_dose_out, "null argument");
         dh_ccc_args ca = make_ccc_args(grid, cp, model);
         dh_pad p = ca.p;
-        float2 *trp = (float2 *)dh_scratch(gpu_id, DH_SLOT_TR_PAD, sizeof(float2) * (size_t)p.pxy * p.pz);
+        void *trp = dh_scratch(gpu_id, DH_SLOT_TR_PAD, dh_tr_pad_bytes(ca));
         int *act = (int *)dh_scratch(gpu_id, DH_SLOT_ACTIVE, sizeof(int) * n);
         int *cnt = (int *)dh_scratch(gpu_id, DH_SLOT_COUNTER, 256);
         dh_launch_ccc(ca, d_terma, d_density, d_electron, trp, act, cnt, d_dose_out, (hipStream_t)stream);
@@ -576,7 +576,7 @@ extern "C" int dh_stage_ccc_list_dev(int gpu_id, const dh_grid *grid, const dh_p
         DH_REQUIRE(n >= 0 && (size_t)n <= nv && (n == 0 || d_list), "bad voxel list");
         dh_ccc_args ca = make_ccc_args(grid, cp, model);
         dh_pad p = ca.p;
-        float2 *trp = (float2 *)dh_scratch(gpu_id, DH_SLOT_TR_PAD, sizeof(float2) * (size_t)p.pxy * p.pz);
+        void *trp = dh_scratch(gpu_id, DH_SLOT_TR_PAD, dh_tr_pad_bytes(ca));
         int *cnt = (int *)dh_scratch(gpu_id, DH_SLOT_COUNTER, 256);
         dh_launch_ccc_list(ca, d_terma, d_density, d_electron, trp, d_list, n, cnt, d_dose_out,
                            (hipStream_t)stream);
@@ -608,7 +608,7 @@ extern "C" int dh_sur_check_dev(int gpu_id, const dh_grid *grid, const dh_photon
                    "work buffer too small (dh_sur_check_work_size)");
         dh_ccc_args ca = make_ccc_args(grid, cp, model);
         dh_pad p = ca.p;
-        float2 *trp = (float2 *)dh_scratch(gpu_id, DH_SLOT_TR_PAD, sizeof(float2) * (size_t)p.pxy * p.pz);
+        void *trp = dh_scratch(gpu_id, DH_SLOT_TR_PAD, dh_tr_pad_bytes(ca));
         dh_launch_sur_check(ca, d_terma, d_density, d_electron, d_dose, d_dmax, stride, tol, d_work,
                             d_plain, trp, (hipStream_t)stream);
     });
@@ -694,5 +694,60 @@ extern "C" int dh_autotune_pencil(dh_proton_plan *p, const float *d_wet, int32_t
         }
         dh_tune_store("pencil", dh_tune_key_geom(p->g), b, bms);
         if (best) *best = b;
+    });
+}
+
+// Photon control point: ray-trace voxel tile (8 choices) and the cone
+// superposition's sampling layout (pairs / quads), both required to give the
+// same dose bit for bit.
+extern "C" int dh_autotune_photon(int gpu_id, const dh_grid *grid, const dh_photon_cp *cp,
+                                  const dh_photon_model *model, const float *mlc_rows, int32_t n_pairs,
+                                  const float *d_density, int32_t reps, int32_t *best_raytrace,
+                                  int32_t *best_layout, float *ms_raytrace, float *ms_ccc, void *stream)
+{
+    return guarded([&] {
+        dh_device_enter(gpu_id);
+        hipStream_t s = (hipStream_t)stream;
+        const size_t n = nvox_of(grid);
+        DH_REQUIRE(cp && d_density, "null argument");
+        dh_terma_args ta = make_terma_args(grid, cp, model, mlc_rows, n_pairs);
+        dh_ccc_args ca = make_ccc_args(grid, cp, model);
+        const dh_pad p = ca.p;
+        float4 *vpad = (float4 *)dh_scratch(gpu_id, DH_SLOT_VOL_PAD, sizeof(float4) * (size_t)p.pxy * p.pz);
+        float *wref = (float *)dh_scratch(gpu_id, DH_SLOT_WET_RAW, sizeof(float) * n);
+        float *out = (float *)dh_scratch(gpu_id, DH_SLOT_OUT0, sizeof(float) * n);
+        float *dref = (float *)dh_scratch(gpu_id, DH_SLOT_OUT1, sizeof(float) * n);
+        float *terma = (float *)dh_scratch(gpu_id, DH_SLOT_TMP0, sizeof(float) * n);
+        float *el = (float *)dh_scratch(gpu_id, DH_SLOT_TMP1, sizeof(float) * n);
+        void *trp = dh_scratch(gpu_id, DH_SLOT_TR_PAD, dh_tr_pad_bytes(ca));
+        int *act = (int *)dh_scratch(gpu_id, DH_SLOT_ACTIVE, sizeof(int) * n);
+        int *cnt = (int *)dh_scratch(gpu_id, DH_SLOT_COUNTER, 256);
+        dh_launch_pad_quad(d_density, vpad, ca.g.nx, ca.g.ny, ca.g.nz, DH_AIR_DENSITY, s);
+        const std::string key = dh_tune_key_geom(ca.g);
+        int best = 0;
+        float bms = 0.0f;
+        for (int c = 0; c < DH_TILE_CHOICES; c++) {
+            float *o = c == 0 ? wref : out;
+            const float ms = dh_time_median([&] { dh_launch_raytrace(ca.g, vpad, o, s, c); }, reps, s);
+            if (ms_raytrace) ms_raytrace[c] = ms;
+            DH_REQUIRE(c == 0 || dh_count_diff(wref, out, (int64_t)n, s) == 0,
+                       "ray-trace variants disagree (tile choice changed the result)");
+            if (c == 0 || ms < bms) { best = c; bms = ms; }
+        }
+        dh_tune_store("raytrace", key, best, bms);
+        if (best_raytrace) *best_raytrace = best;
+        dh_launch_terma(ta, wref, terma, el, s);
+        best = 0;
+        for (int c = 0; c < 2; c++) {
+            float *o = c == 0 ? dref : out;
+            const float ms = dh_time_median([&] { dh_launch_ccc(ca, terma, d_density, el, trp, act, cnt, o, s, c); },
+                                            reps, s);
+            if (ms_ccc) ms_ccc[c] = ms;
+            DH_REQUIRE(c == 0 || dh_count_diff(dref, out, (int64_t)n, s) == 0,
+                       "cone superposition layouts disagree");
+            if (c == 0 || ms < bms) { best = c; bms = ms; }
+        }
+        dh_tune_store("ccc", key, best, bms);
+        if (best_layout) *best_layout = best;
     });
 }

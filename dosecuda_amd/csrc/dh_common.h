@@ -190,6 +190,39 @@ __device__ static inline float dh_trilinear_u(const float *__restrict__ vol, con
     return fmaf(az, b1 - b0, b0);
 }
 
+// The same two samplers over the quad-packed padded volume: element e holds
+// the padded values (e, e + 1, e + px, e + px + 1), so the 8 corners come in
+// two 16-byte loads (planes k0 and k0 + 1) instead of eight; the corner
+// values and the blend are the same, so are the results.
+__device__ static inline float dh_blend8(float4 lo, float4 hi, float ax, float ay, float az)
+{
+    float a00 = fmaf(ax, lo.y - lo.x, lo.x);
+    float a10 = fmaf(ax, lo.w - lo.z, lo.z);
+    float a01 = fmaf(ax, hi.y - hi.x, hi.x);
+    float a11 = fmaf(ax, hi.w - hi.z, hi.z);
+    float b0 = fmaf(ay, a10 - a00, a00);
+    float b1 = fmaf(ay, a11 - a01, a01);
+    return fmaf(az, b1 - b0, b0);
+}
+
+__device__ static inline float dh_trilinear_uq(const float4 *__restrict__ q, const dh_pad &p,
+                                               float ux, float uy, float uz)
+{
+    float fx0 = floorf(ux), fy0 = floorf(uy), fz0 = floorf(uz);
+    float ax = ux - fx0, ay = uy - fy0, az = uz - fz0;
+    const float4 *b = q + dh_pad_index(p, (int)fx0, (int)fy0, (int)fz0);
+    return dh_blend8(b[0], b[p.pxy], ax, ay, az);
+}
+
+__device__ static inline float dh_trilinear_q(const float4 *__restrict__ q, const dh_pad &p,
+                                              int nx, int ny, int nz, float tx, float ty, float tz)
+{
+    float ux = __builtin_amdgcn_fmed3f(tx - 0.5f, -1.0f, (float)nx);
+    float uy = __builtin_amdgcn_fmed3f(ty - 0.5f, -1.0f, (float)ny);
+    float uz = __builtin_amdgcn_fmed3f(tz - 0.5f, -1.0f, (float)nz);
+    return dh_trilinear_uq(q, p, ux, uy, uz);
+}
+
 // Per (voxel, layer) factors of the double-Gaussian pencil beam
 // (IMPTClasses.cu:69-121,222-264): LUT lookup at WET w (mm) in a 400-entry
 // {depth, sigma, idd, -} table, sigma_air at source distance dist, nuclear

@@ -63,7 +63,7 @@ int dh_get_workspace();
 int dh_current_device();
 
 enum dh_slot {
-    DH_SLOT_VOL_PAD = 0,   // padded scalar volume for the ray-trace
+    DH_SLOT_VOL_PAD = 0,   // quad-packed padded volume for the ray trace
     DH_SLOT_WET_RAW,       // unsmoothed WET
     DH_SLOT_TR_PAD,        // padded {TERMA, density} pairs for the CCCS
     DH_SLOT_ACTIVE,        // compacted list of CCCS voxels
@@ -148,36 +148,42 @@ struct dh_ccc_args {
     float dir[72 * 3];              // cone directions in image space
 };
 
-void dh_launch_pad(const float *src, float *dst, int nx, int ny, int nz, float border,
-                   hipStream_t s);
+// quad-packed padded volume of the ray trace (float4 per padded voxel)
+void dh_launch_pad_quad(const float *src, float4 *dst, int nx, int ny, int nz, float border,
+                        hipStream_t s);
 // choice < 0: the tuning table's choice for the shape (default 0); the
 // pencil's choice is tile + DH_TILE_CHOICES * (LUT in LDS ? 0 : 1)
-void dh_launch_raytrace(const dh_geom &g, const float *vol_pad, float *wet, hipStream_t s, int choice = -1);
+void dh_launch_raytrace(const dh_geom &g, const float4 *vol_pad, float *wet, hipStream_t s, int choice = -1);
 void dh_launch_smooth(const dh_geom &g, const float *wet_raw, float *wet_out, hipStream_t s, int choice = -1);
 void dh_launch_pencil(const dh_geom &g, float vsadx, float vsady, int n_layers,
                       const dh_layer_k *d_layers, const float4 *d_luts, const float *d_spots,
                       bool fold, const float *wet, float *dose, hipStream_t s, int choice = -1);
 void dh_launch_terma(const dh_terma_args &a, const float *wet, float *terma, float *electron,
                      hipStream_t s);
+// Sampling layout of the cone superposition's {TERMA, density} volume:
+// 0 = float2 pairs, 1 = float4 quads (see k_pack_tr); choice < 0 takes the
+// tuning table's choice ("ccc", default 0).  The scratch for either layout
+// is dh_tr_pad_bytes.
+int dh_ccc_layout(const dh_ccc_args &a, int choice);
+size_t dh_tr_pad_bytes(const dh_ccc_args &a);
 void dh_launch_ccc(const dh_ccc_args &a, const float *terma, const float *density,
-                   const float *electron, float2 *tr_pad, int *active, int *counter,
-                   float *dose, hipStream_t s);
+                   const float *electron, void *tr_pad, int *active, int *counter,
+                   float *dose, hipStream_t s, int choice = -1);
 
 // Cone superposition on an explicit list of n voxel indices (each < nx*ny*nz);
 // only the listed voxels of `dose` are written.
 void dh_launch_ccc_list(const dh_ccc_args &a, const float *terma, const float *density,
-                        const float *electron, float2 *tr_pad, const int *list, int n,
+                        const float *electron, void *tr_pad, const int *list, int n,
                         int *counter, float *dose, hipStream_t s);
 
-// Padded {TERMA, density} pairs for the cone superposition.
-void dh_launch_pack_tr(const dh_ccc_args &a, const float *terma, const float *density,
-                       float2 *tr_pad, hipStream_t s);
+// The padded {TERMA, density} volume in a layout.
+void dh_launch_pack_tr(const dh_ccc_args &a, const float *terma, const float *density, void *tr_pad, int layout,
+                       hipStream_t s);
 
 // Cone superposition on a voxel list whose length is read on the device from
-// *counter (at most max_n); tr_pad already packed.
-void dh_launch_ccc_counted(const dh_ccc_args &a, const float2 *tr_pad, const float *electron,
-                           const int *list, const int *counter, int64_t max_n, float *dose,
-                           hipStream_t s);
+// *counter (at most max_n); tr_pad already packed in `layout`.
+void dh_launch_ccc_counted(const dh_ccc_args &a, const void *tr_pad, int layout, const float *electron,
+                           const int *list, const int *counter, int64_t max_n, float *dose, hipStream_t s);
 
 // Checked surrogate: plain CCCS on the lattice, per-cell comparison with the
 // surrogate dose, plain CCCS on the voxels of failed and uncovered cells.
@@ -218,7 +224,7 @@ void dh_launch_sur_dose(const dh_sur_net *net, const dh_ccc_args &a, const float
                         int stride, float tol, float *dose, hipStream_t s);
 void dh_launch_sur_check(const dh_ccc_args &a, const float *terma, const float *density,
                          const float *electron, float *dose, const float *dmax, int stride,
-                         float tol, int *work, float *plain, float2 *tr_pad, hipStream_t s);
+                         float tol, int *work, float *plain, void *tr_pad, hipStream_t s);
 
 // A proton field's spots sorted by energy id (stable), its layers
 // (importLayers) and one 400-entry LUT row per layer.  spots holds 8 floats

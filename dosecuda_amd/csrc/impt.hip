@@ -12,20 +12,30 @@ namespace {
 
 constexpr int kBlock = 256;
 
+// Quad-packed padded volume (1 voxel of border below, 2 above on each axis):
+// element (x, y, z) of the padded grid holds the padded values at (x, y),
+// (x + 1, y), (x, y + 1), (x + 1, y + 1) of plane z; outside the volume the
+// value is the border.  Elements of the last padded row / column are never
+// read by the samplers and hold the border.
 __global__ void __launch_bounds__(kBlock)
-k_pad(const float *__restrict__ src, float *__restrict__ dst, int nx, int ny, int nz,
-      dh_pad p, float border)
+k_pad_quad(const float *__restrict__ src, float4 *__restrict__ dst, int nx, int ny, int nz,
+           dh_pad p, float border)
 {
     const uint32_t n = (uint32_t)(p.pxy * p.pz);
     for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < n; t += gridDim.x * kBlock) {
-        int i = (int)(t % (uint32_t)p.px) - 1;
-        uint32_t r = t / (uint32_t)p.px;
-        int j = (int)(r % (uint32_t)p.py) - 1;
-        int k = (int)(r / (uint32_t)p.py) - 1;
-        float v = border;
-        if (i >= 0 && i < nx && j >= 0 && j < ny && k >= 0 && k < nz)
-            v = src[(uint32_t)i + (uint32_t)nx * ((uint32_t)j + (uint32_t)ny * (uint32_t)k)];
-        dst[t] = v;
+        const int i = (int)(t % (uint32_t)p.px) - 1;
+        const uint32_t r = t / (uint32_t)p.px;
+        const int j = (int)(r % (uint32_t)p.py) - 1;
+        const int k = (int)(r / (uint32_t)p.py) - 1;
+        float v[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int ii = i + (c & 1), jj = j + (c >> 1);
+            v[c] = (ii >= 0 && ii < nx && jj >= 0 && jj < ny && k >= 0 && k < nz)
+                       ? src[(uint32_t)ii + (uint32_t)nx * ((uint32_t)jj + (uint32_t)ny * (uint32_t)k)]
+                       : border;
+        }
+        dst[t] = make_float4(v[0], v[1], v[2], v[3]);
     }
 }
 
@@ -60,7 +70,7 @@ __device__ static inline int safe_samples(const dh_geom &g, float x, float y, fl
 }
 
 __global__ void __launch_bounds__(kBlock)
-k_raytrace(dh_geom g, dh_pad p, dh_tile tile, const float *__restrict__ vol, float *__restrict__ wet)
+k_raytrace(dh_geom g, dh_pad p, dh_tile tile, const float4 *__restrict__ vol, float *__restrict__ wet)
 {
     int i, j, k;
     if (!dh_tile_voxel(tile, dh_xcd_block(blockIdx.x, gridDim.x), threadIdx.x, g.nx, g.ny, g.nz, i, j, k)) return;
@@ -83,8 +93,8 @@ k_raytrace(dh_geom g, dh_pad p, dh_tile tile, const float *__restrict__ vol, flo
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const float L = (float)(m + q);
-            part += fmaxf(dh_trilinear_u(vol, p, fmaf(dux, L, u0x), fmaf(duy, L, u0y),
-                                         fmaf(duz, L, u0z)), 0.0f);
+            part += fmaxf(dh_trilinear_uq(vol, p, fmaf(dux, L, u0x), fmaf(duy, L, u0y),
+                                          fmaf(duz, L, u0z)), 0.0f);
         }
         acc += (double)part;
     }
@@ -104,7 +114,7 @@ k_raytrace(dh_geom g, dh_pad p, dh_tile tile, const float *__restrict__ vol, flo
             const float ty = fmaf(g.inv_s, qy + g.iso[1], 0.5f);
             const float tz = fmaf(g.inv_s, qz + g.iso[2], 0.5f);
             in[q] = tx >= 0.0f && tx < fnx && ty >= 0.0f && ty < fny && tz >= 0.0f && tz < fnz;
-            d[q] = dh_trilinear(vol, p, g.nx, g.ny, g.nz, tx, ty, tz);
+            d[q] = dh_trilinear_q(vol, p, g.nx, g.ny, g.nz, tx, ty, tz);
         }
         float part = 0.0f;
         bool all = true;
@@ -241,17 +251,17 @@ inline unsigned grid_for(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock
 
 }  // namespace
 
-void dh_launch_pad(const float *src, float *dst, int nx, int ny, int nz, float border, hipStream_t s)
+void dh_launch_pad_quad(const float *src, float4 *dst, int nx, int ny, int nz, float border, hipStream_t s)
 {
     dh_pad p = dh_make_pad(nx, ny, nz);
     size_t n = (size_t)p.pxy * p.pz;
     unsigned grid = (unsigned)std::min<size_t>(grid_for(n), 4096);
     dh_timed t("pad", s);
-    hipLaunchKernelGGL(k_pad, dim3(grid), dim3(kBlock), 0, s, src, dst, nx, ny, nz, p, border);
+    hipLaunchKernelGGL(k_pad_quad, dim3(grid), dim3(kBlock), 0, s, src, dst, nx, ny, nz, p, border);
     DH_HIP(hipGetLastError());
 }
 
-void dh_launch_raytrace(const dh_geom &g, const float *vol_pad, float *wet, hipStream_t s, int choice)
+void dh_launch_raytrace(const dh_geom &g, const float4 *vol_pad, float *wet, hipStream_t s, int choice)
 {
     dh_pad p = dh_make_pad(g.nx, g.ny, g.nz);
     if (choice < 0) choice = dh_tune_lookup("raytrace", dh_tune_key_geom(g), 0);

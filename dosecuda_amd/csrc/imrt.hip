@@ -84,22 +84,64 @@ k_terma(dh_terma_args a, const float *__restrict__ wet, float *__restrict__ term
 }
 
 // Packs {TERMA, density} into the padded float2 layout with their borders.
+// {TERMA, density} of the padded grid (border TERMA 0, density air), as
+// float2 pairs, or (Quad) as two float4 per padded element holding the
+// element's TERMA and density at (x, y), (x + 1, y), (x, y + 1), (x + 1, y + 1),
+// so a trilinear sample takes four 16-byte loads instead of eight 8-byte
+// ones.  Elements of the last padded row / column are never read.
+__device__ static inline float2 tr_at(const float *__restrict__ terma, const float *__restrict__ dens, int nx,
+                                      int ny, int nz, int i, int j, int k)
+{
+    if (i >= 0 && i < nx && j >= 0 && j < ny && k >= 0 && k < nz) {
+        const uint32_t s = (uint32_t)i + (uint32_t)nx * ((uint32_t)j + (uint32_t)ny * (uint32_t)k);
+        return make_float2(terma[s], dens[s]);
+    }
+    return make_float2(0.0f, DH_AIR_DENSITY);
+}
+
+template <bool Quad>
 __global__ void __launch_bounds__(kBlock)
 k_pack_tr(const float *__restrict__ terma, const float *__restrict__ dens, int nx, int ny, int nz,
-          dh_pad p, float2 *__restrict__ out)
+          dh_pad p, void *__restrict__ outv)
 {
     const uint32_t n = (uint32_t)(p.pxy * p.pz);
     for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < n; t += gridDim.x * kBlock) {
-        int i = (int)(t % (uint32_t)p.px) - 1;
-        uint32_t r = t / (uint32_t)p.px;
-        int j = (int)(r % (uint32_t)p.py) - 1;
-        int k = (int)(r / (uint32_t)p.py) - 1;
-        float2 v = make_float2(0.0f, DH_AIR_DENSITY);
-        if (i >= 0 && i < nx && j >= 0 && j < ny && k >= 0 && k < nz) {
-            uint32_t s = (uint32_t)i + (uint32_t)nx * ((uint32_t)j + (uint32_t)ny * (uint32_t)k);
-            v = make_float2(terma[s], dens[s]);
+        const int i = (int)(t % (uint32_t)p.px) - 1;
+        const uint32_t r = t / (uint32_t)p.px;
+        const int j = (int)(r % (uint32_t)p.py) - 1;
+        const int k = (int)(r / (uint32_t)p.py) - 1;
+        if (!Quad) {
+            ((float2 *)outv)[t] = tr_at(terma, dens, nx, ny, nz, i, j, k);
+        } else {
+            const float2 a = tr_at(terma, dens, nx, ny, nz, i, j, k), b = tr_at(terma, dens, nx, ny, nz, i + 1, j, k);
+            const float2 c = tr_at(terma, dens, nx, ny, nz, i, j + 1, k);
+            const float2 d = tr_at(terma, dens, nx, ny, nz, i + 1, j + 1, k);
+            float4 *o = (float4 *)outv + 2 * (size_t)t;
+            o[0] = make_float4(a.x, b.x, c.x, d.x);
+            o[1] = make_float4(a.y, b.y, c.y, d.y);
         }
-        out[t] = v;
+    }
+}
+
+// The 8 trilinear corners {TERMA, density} at padded element e (planes z and
+// z + 1) from either layout.
+template <bool Quad>
+__device__ static inline void tr_corners(const void *__restrict__ trv, const dh_pad &p, int64_t e, float2 &v000,
+                                         float2 &v100, float2 &v010, float2 &v110, float2 &v001, float2 &v101,
+                                         float2 &v011, float2 &v111)
+{
+    if (!Quad) {
+        const float2 *b = (const float2 *)trv + e;
+        const float2 *c = b + p.pxy;
+        v000 = b[0]; v100 = b[1]; v010 = b[p.px]; v110 = b[p.px + 1];
+        v001 = c[0]; v101 = c[1]; v011 = c[p.px]; v111 = c[p.px + 1];
+    } else {
+        const float4 *q = (const float4 *)trv;
+        const float4 tl = q[2 * e], rl = q[2 * e + 1], th = q[2 * (e + p.pxy)], rh = q[2 * (e + p.pxy) + 1];
+        v000 = make_float2(tl.x, rl.x); v100 = make_float2(tl.y, rl.y);
+        v010 = make_float2(tl.z, rl.z); v110 = make_float2(tl.w, rl.w);
+        v001 = make_float2(th.x, rh.x); v101 = make_float2(th.y, rh.y);
+        v011 = make_float2(th.z, rh.z); v111 = make_float2(th.w, rh.w);
     }
 }
 
@@ -116,8 +158,9 @@ __global__ void k_set_int(int *p, int v)
 
 // TERMA of the voxel centre sampled like the reference's texture fetch
 // (IMRTClasses.cu:267-273).
+template <bool Quad>
 __device__ static inline float center_terma(const dh_geom &g, const dh_pad &p,
-                                            const float2 *__restrict__ tr, int i, int j, int k)
+                                            const void *__restrict__ tr, int i, int j, int k)
 {
     float x, y, z;
     dh_vox_xyz(g, i, j, k, x, y, z);
@@ -129,12 +172,13 @@ __device__ static inline float center_terma(const dh_geom &g, const dh_pad &p,
     const float uz = __builtin_amdgcn_fmed3f(tz, -1.0f, (float)g.nz);
     const float fx = floorf(ux), fy = floorf(uy), fz = floorf(uz);
     const float ax = ux - fx, ay = uy - fy, az = uz - fz;
-    const float2 *b = tr + dh_pad_index(p, (int)fx, (int)fy, (int)fz);
-    const float2 *c = b + p.pxy;
-    const float a00 = fmaf(ax, b[1].x - b[0].x, b[0].x);
-    const float a10 = fmaf(ax, b[p.px + 1].x - b[p.px].x, b[p.px].x);
-    const float a01 = fmaf(ax, c[1].x - c[0].x, c[0].x);
-    const float a11 = fmaf(ax, c[p.px + 1].x - c[p.px].x, c[p.px].x);
+    float2 v000, v100, v010, v110, v001, v101, v011, v111;
+    tr_corners<Quad>(tr, p, dh_pad_index(p, (int)fx, (int)fy, (int)fz), v000, v100, v010, v110, v001, v101,
+                     v011, v111);
+    const float a00 = fmaf(ax, v100.x - v000.x, v000.x);
+    const float a10 = fmaf(ax, v110.x - v010.x, v010.x);
+    const float a01 = fmaf(ax, v101.x - v001.x, v001.x);
+    const float a11 = fmaf(ax, v111.x - v011.x, v011.x);
     const float b0 = fmaf(ay, a10 - a00, a00), b1 = fmaf(ay, a11 - a01, a01);
     return fmaf(az, b1 - b0, b0);
 }
@@ -142,8 +186,9 @@ __device__ static inline float center_terma(const dh_geom &g, const dh_pad &p,
 // Builds the list of voxels with TERMA > 0.01 and zeroes the dose of the rest.
 // One atomic per workgroup: waves ballot, the workgroup prefix-sums the four
 // wave counts in LDS and reserves its slice of the list at once.
+template <bool Quad>
 __global__ void __launch_bounds__(kBlock)
-k_ccc_compact(dh_geom g, dh_pad p, const float2 *__restrict__ tr, int *__restrict__ list,
+k_ccc_compact(dh_geom g, dh_pad p, const void *__restrict__ tr, int *__restrict__ list,
               int *__restrict__ counter, float *__restrict__ dose)
 {
     __shared__ int s_cnt[kBlock / 64];
@@ -156,7 +201,7 @@ k_ccc_compact(dh_geom g, dh_pad p, const float2 *__restrict__ tr, int *__restric
         const uint32_t r = v / (uint32_t)g.nx;
         const int j = (int)(r % (uint32_t)g.ny);
         const int k = (int)(r / (uint32_t)g.ny);
-        const float T0 = center_terma(g, p, tr, i, j, k);
+        const float T0 = center_terma<Quad>(g, p, tr, i, j, k);
         act = !(T0 <= 0.01f);
         if (!act) dose[v] = 0.0f;
     }
@@ -191,8 +236,9 @@ k_ccc_compact(dh_geom g, dh_pad p, const float2 *__restrict__ tr, int *__restric
 // combined with a fixed xor-shuffle tree, so the result is deterministic.
 constexpr int kVoxPerWave = 16;
 
+template <bool Quad>
 __global__ void __launch_bounds__(kBlock)
-k_ccc(dh_ccc_args a, const float2 *__restrict__ tr, const float *__restrict__ electron,
+k_ccc(dh_ccc_args a, const void *__restrict__ tr, const float *__restrict__ electron,
       const int *__restrict__ list, const int *__restrict__ counter, float *__restrict__ dose)
 {
     const dh_geom &g = a.g;
@@ -254,10 +300,9 @@ k_ccc(dh_ccc_args a, const float2 *__restrict__ tr, const float *__restrict__ el
                     float qz = __builtin_amdgcn_fmed3f(fmaf(tdz, Lm, ucz), -1.0f, fnz);
                     const float fx = floorf(qx), fy = floorf(qy), fz = floorf(qz);
                     const float wx = qx - fx, wy = qy - fy, wz = qz - fz;
-                    const float2 *b = tr + dh_pad_index(a.p, (int)fx, (int)fy, (int)fz);
-                    const float2 *cc = b + a.p.pxy;
-                    const float2 v000 = b[0], v100 = b[1], v010 = b[a.p.px], v110 = b[a.p.px + 1];
-                    const float2 v001 = cc[0], v101 = cc[1], v011 = cc[a.p.px], v111 = cc[a.p.px + 1];
+                    float2 v000, v100, v010, v110, v001, v101, v011, v111;
+                    tr_corners<Quad>(tr, a.p, dh_pad_index(a.p, (int)fx, (int)fy, (int)fz), v000, v100, v010,
+                                     v110, v001, v101, v011, v111);
                     const float t00 = fmaf(wx, v100.x - v000.x, v000.x);
                     const float t10 = fmaf(wx, v110.x - v010.x, v010.x);
                     const float t01 = fmaf(wx, v101.x - v001.x, v001.x);
@@ -300,74 +345,85 @@ void dh_launch_terma(const dh_terma_args &a, const float *wet, float *terma, flo
     DH_HIP(hipGetLastError());
 }
 
-void dh_launch_ccc(const dh_ccc_args &a, const float *terma, const float *density,
-                   const float *electron, float2 *tr_pad, int *active, int *counter,
-                   float *dose, hipStream_t s)
+int dh_ccc_layout(const dh_ccc_args &a, int choice)
+{
+    if (choice < 0) choice = dh_tune_lookup("ccc", dh_tune_key_geom(a.g), 0);
+    DH_REQUIRE(choice == 0 || choice == 1, "CCCS layout choice must be 0 (pairs) or 1 (quads)");
+    return choice;
+}
+
+size_t dh_tr_pad_bytes(const dh_ccc_args &a) { return (size_t)32 * a.p.pxy * a.p.pz; }
+
+void dh_launch_pack_tr(const dh_ccc_args &a, const float *terma, const float *density, void *tr_pad, int layout,
+                       hipStream_t s)
 {
     const dh_geom &g = a.g;
-    size_t n = (size_t)g.nx * g.ny * g.nz;
     size_t np = (size_t)a.p.pxy * a.p.pz;
-    {
-        dh_timed t("pack_tr", s);
-        hipLaunchKernelGGL(k_pack_tr, dim3((unsigned)std::min<size_t>(grid_for(np), 4096)),
-                           dim3(kBlock), 0, s, terma, density, g.nx, g.ny, g.nz, a.p, tr_pad);
-        DH_HIP(hipGetLastError());
-    }
+    const unsigned grid = (unsigned)std::min<size_t>(grid_for(np), 4096);
+    dh_timed t("pack_tr", s);
+    if (layout)
+        hipLaunchKernelGGL(k_pack_tr<true>, dim3(grid), dim3(kBlock), 0, s, terma, density, g.nx, g.ny, g.nz, a.p,
+                           tr_pad);
+    else
+        hipLaunchKernelGGL(k_pack_tr<false>, dim3(grid), dim3(kBlock), 0, s, terma, density, g.nx, g.ny, g.nz, a.p,
+                           tr_pad);
+    DH_HIP(hipGetLastError());
+}
+
+void dh_launch_ccc_counted(const dh_ccc_args &a, const void *tr_pad, int layout, const float *electron,
+                           const int *list, const int *counter, int64_t max_n, float *dose, hipStream_t s)
+{
+    if (max_n <= 0) return;
+    dh_timed t("ccc_list", s);
+    const unsigned grid = std::max(1u, (unsigned)std::min<int64_t>((max_n * 4 + kBlock - 1) / kBlock, 2048));
+    if (layout)
+        hipLaunchKernelGGL(k_ccc<true>, dim3(grid), dim3(kBlock), 0, s, a, tr_pad, electron, list, counter, dose);
+    else
+        hipLaunchKernelGGL(k_ccc<false>, dim3(grid), dim3(kBlock), 0, s, a, tr_pad, electron, list, counter, dose);
+    DH_HIP(hipGetLastError());
+}
+
+void dh_launch_ccc(const dh_ccc_args &a, const float *terma, const float *density,
+                   const float *electron, void *tr_pad, int *active, int *counter,
+                   float *dose, hipStream_t s, int choice)
+{
+    const dh_geom &g = a.g;
+    const int layout = dh_ccc_layout(a, choice);
+    size_t n = (size_t)g.nx * g.ny * g.nz;
+    dh_launch_pack_tr(a, terma, density, tr_pad, layout, s);
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, counter, 1);
     DH_HIP(hipGetLastError());
     {
         dh_timed t("ccc_compact", s);
-        hipLaunchKernelGGL(k_ccc_compact, dim3(grid_for(n)), dim3(kBlock), 0, s, g, a.p,
-                           (const float2 *)tr_pad, active, counter, dose);
+        if (layout)
+            hipLaunchKernelGGL(k_ccc_compact<true>, dim3(grid_for(n)), dim3(kBlock), 0, s, g, a.p,
+                               (const void *)tr_pad, active, counter, dose);
+        else
+            hipLaunchKernelGGL(k_ccc_compact<false>, dim3(grid_for(n)), dim3(kBlock), 0, s, g, a.p,
+                               (const void *)tr_pad, active, counter, dose);
         DH_HIP(hipGetLastError());
     }
     {
         dh_timed t("ccc", s);
-        unsigned grid = (unsigned)std::min<size_t>(grid_for(n), 2048);
-        hipLaunchKernelGGL(k_ccc, dim3(grid), dim3(kBlock), 0, s, a, (const float2 *)tr_pad,
-                           electron, (const int *)active, (const int *)counter, dose);
+        const unsigned grid = (unsigned)std::min<size_t>(grid_for(n), 2048);
+        if (layout)
+            hipLaunchKernelGGL(k_ccc<true>, dim3(grid), dim3(kBlock), 0, s, a, (const void *)tr_pad, electron,
+                               (const int *)active, (const int *)counter, dose);
+        else
+            hipLaunchKernelGGL(k_ccc<false>, dim3(grid), dim3(kBlock), 0, s, a, (const void *)tr_pad, electron,
+                               (const int *)active, (const int *)counter, dose);
         DH_HIP(hipGetLastError());
     }
 }
 
 void dh_launch_ccc_list(const dh_ccc_args &a, const float *terma, const float *density,
-                        const float *electron, float2 *tr_pad, const int *list, int n,
+                        const float *electron, void *tr_pad, const int *list, int n,
                         int *counter, float *dose, hipStream_t s)
 {
-    const dh_geom &g = a.g;
-    size_t np = (size_t)a.p.pxy * a.p.pz;
-    hipLaunchKernelGGL(k_pack_tr, dim3((unsigned)std::min<size_t>(grid_for(np), 4096)),
-                       dim3(kBlock), 0, s, terma, density, g.nx, g.ny, g.nz, a.p, tr_pad);
-    DH_HIP(hipGetLastError());
+    const int layout = dh_ccc_layout(a, -1);
+    dh_launch_pack_tr(a, terma, density, tr_pad, layout, s);
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s, counter, n);
     DH_HIP(hipGetLastError());
     if (n <= 0) return;
-    dh_timed t("ccc_list", s);
-    const unsigned grid = (unsigned)std::min<size_t>(((size_t)n * 4 + kBlock - 1) / kBlock, 2048);
-    hipLaunchKernelGGL(k_ccc, dim3(std::max(grid, 1u)), dim3(kBlock), 0, s, a, (const float2 *)tr_pad,
-                       electron, list, (const int *)counter, dose);
-    DH_HIP(hipGetLastError());
-}
-
-void dh_launch_pack_tr(const dh_ccc_args &a, const float *terma, const float *density,
-                       float2 *tr_pad, hipStream_t s)
-{
-    const dh_geom &g = a.g;
-    size_t np = (size_t)a.p.pxy * a.p.pz;
-    dh_timed t("pack_tr", s);
-    hipLaunchKernelGGL(k_pack_tr, dim3((unsigned)std::min<size_t>(grid_for(np), 4096)),
-                       dim3(kBlock), 0, s, terma, density, g.nx, g.ny, g.nz, a.p, tr_pad);
-    DH_HIP(hipGetLastError());
-}
-
-void dh_launch_ccc_counted(const dh_ccc_args &a, const float2 *tr_pad, const float *electron,
-                           const int *list, const int *counter, int64_t max_n, float *dose,
-                           hipStream_t s)
-{
-    if (max_n <= 0) return;
-    dh_timed t("ccc_list", s);
-    const unsigned grid = (unsigned)std::min<int64_t>((max_n * 4 + kBlock - 1) / kBlock, 2048);
-    hipLaunchKernelGGL(k_ccc, dim3(std::max(grid, 1u)), dim3(kBlock), 0, s, a, tr_pad, electron, list,
-                       counter, dose);
-    DH_HIP(hipGetLastError());
+    dh_launch_ccc_counted(a, tr_pad, layout, electron, list, counter, n, dose, s);
 }

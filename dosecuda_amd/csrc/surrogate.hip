@@ -459,10 +459,11 @@ __global__ void k_fill_i32(int *p, int n, int v)
 
 // Input cells on the half-resolution grid of the 16-padded volume: channels
 // 0-7 TERMA / max, 8-15 density of the 2x2x2 sub-voxels (sub = 4 dz + 2 dy +
-// dx), 16-18 the beam direction, 19-31 zero.  One thread writes 8 channels.
+// dx), 16-18 the beam direction, 19 the spacing channel, 20-31 zero.  One
+// thread writes 8 channels.
 __global__ void k_sur_pack(const float *__restrict__ terma, const float *__restrict__ dens, int nz, int ny,
                            int nx, int cz, int cy, int cx, const float *__restrict__ tmax, float d0,
-                           float d1, float d2, uint16_t *__restrict__ out)
+                           float d1, float d2, float spc, uint16_t *__restrict__ out)
 {
     const int64_t n = (int64_t)cz * cy * cx * 4;
     const float scale = fmaxf(*tmax, 1e-12f);
@@ -485,7 +486,7 @@ __global__ void k_sur_pack(const float *__restrict__ terma, const float *__restr
         } else {
 #pragma unroll
             for (int e = 0; e < 8; e++) h[e] = 0;
-            if (grp == 2) { h[0] = f2bf(d0); h[1] = f2bf(d1); h[2] = f2bf(d2); }
+            if (grp == 2) { h[0] = f2bf(d0); h[1] = f2bf(d1); h[2] = f2bf(d2); h[3] = f2bf(spc); }
         }
         uint4 o;
         __builtin_memcpy(&o, h, 16);
@@ -758,8 +759,11 @@ extern "C" int dh_nn_pool2_dev(const void *x, int32_t n, int32_t d, int32_t h, i
 
 namespace {
 
+// spacing channel: spacing / 2.5 mm
+constexpr float kSpRef = 2.5f;
+
 void launch_pack(int nz, int ny, int nx, const float *terma, const float *density, const float dir[3],
-                 uint16_t *x, float *tmax, hipStream_t s)
+                 float spacing, uint16_t *x, float *tmax, hipStream_t s)
 {
     const int64_t n = (int64_t)nz * ny * nx;
     const int cz = ((nz + 15) / 16) * 8, cy = ((ny + 15) / 16) * 8, cx = ((nx + 15) / 16) * 8;
@@ -769,7 +773,8 @@ void launch_pack(int nz, int ny, int nx, const float *terma, const float *densit
     const int64_t items = (int64_t)cz * cy * cx * 4;
     dh_timed t("sur_pack", s);
     hipLaunchKernelGGL(k_sur_pack, dim3((unsigned)std::min<int64_t>((items + 255) / 256, 8192)), dim3(256), 0, s,
-                       terma, density, nz, ny, nx, cz, cy, cx, (const float *)tmax, dir[0], dir[1], dir[2], x);
+                       terma, density, nz, ny, nx, cz, cy, cx, (const float *)tmax, dir[0], dir[1], dir[2],
+                       spacing / kSpRef, x);
     DH_HIP(hipGetLastError());
 }
 
@@ -797,12 +802,12 @@ void launch_pool(const uint16_t *x, int n, int d, int h, int w, int c, uint16_t 
 }  // namespace
 
 extern "C" int dh_sur_pack_dev(int32_t nz, int32_t ny, int32_t nx, const float *terma, const float *density,
-                               const float dir[3], void *x_out, float *tmax_out, void *stream)
+                               const float dir[3], float spacing, void *x_out, float *tmax_out, void *stream)
 {
     return dh_guarded([&] {
         DH_REQUIRE(terma && density && dir && x_out && tmax_out, "null argument");
-        DH_REQUIRE(nz > 0 && ny > 0 && nx > 0, "bad shape");
-        launch_pack(nz, ny, nx, terma, density, dir, (uint16_t *)x_out, tmax_out, (hipStream_t)stream);
+        DH_REQUIRE(nz > 0 && ny > 0 && nx > 0 && spacing > 0.0f, "bad shape");
+        launch_pack(nz, ny, nx, terma, density, dir, spacing, (uint16_t *)x_out, tmax_out, (hipStream_t)stream);
     });
 }
 
@@ -970,9 +975,9 @@ void dh_launch_sur_dose(const dh_sur_net *net, const dh_ccc_args &ca, const floa
     const int64_t wl = dh_sur_check_work_len(nz, ny, nx, stride);
     int *work = (int *)dh_scratch(dev, DH_SLOT_NN_WORK, sizeof(int) * (size_t)wl);
     float *plain = (float *)dh_scratch(dev, DH_SLOT_NN_PLAIN, sizeof(float) * (size_t)nz * ny * nx);
-    float2 *trp = (float2 *)dh_scratch(dev, DH_SLOT_TR_PAD, sizeof(float2) * (size_t)ca.p.pxy * ca.p.pz);
+    void *trp = dh_scratch(dev, DH_SLOT_TR_PAD, dh_tr_pad_bytes(ca));
     DH_REQUIRE(net->in_ch == 32, "the surrogate input layout has 32 channels");
-    launch_pack(nz, ny, nx, terma, density, dir, x, scal, s);
+    launch_pack(nz, ny, nx, terma, density, dir, g.s, x, scal, s);
     sur_forward(net, x, cz, cy, cx, y32, ws, s);
     launch_finish(nz, ny, nx, y32, scal, net->out_scale, terma, electron, ca.mu_cal * ca.mu, dose, scal + 1, s);
     dh_launch_sur_check(ca, terma, density, electron, dose, scal + 1, stride, tol, work, plain, trp, s);
@@ -1001,7 +1006,7 @@ int64_t dh_sur_check_work_len(int nz, int ny, int nx, int stride)
 
 void dh_launch_sur_check(const dh_ccc_args &ca, const float *terma, const float *density,
                          const float *electron, float *dose, const float *dmax, int stride,
-                         float tol, int *work, float *plain, float2 *tr_pad, hipStream_t s)
+                         float tol, int *work, float *plain, void *tr_pad, hipStream_t s)
 {
     const dh_geom &g = ca.g;
     const int nz = g.nz, ny = g.ny, nx = g.nx;
@@ -1010,13 +1015,14 @@ void dh_launch_sur_check(const dh_ccc_args &ca, const float *terma, const float 
     int *counter = work, *lat_ok = work + 4, *bad = lat_ok + nc, *redo = bad + nc, *lat_list = redo + nc;
     int *fb_list = lat_list + nc;
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, counter, 4, 0);
-    dh_launch_pack_tr(ca, terma, density, tr_pad, s);
+    const int layout = dh_ccc_layout(ca, -1);
+    dh_launch_pack_tr(ca, terma, density, tr_pad, layout, s);
     {
         dh_timed t("sur_lattice", s);
         hipLaunchKernelGGL(k_sur_lattice, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, terma, nz, ny, nx,
                            stride, lz, ly, lx, lat_ok, lat_list, counter);
     }
-    dh_launch_ccc_counted(ca, tr_pad, electron, lat_list, counter, nc, plain, s);
+    dh_launch_ccc_counted(ca, tr_pad, layout, electron, lat_list, counter, nc, plain, s);
     hipLaunchKernelGGL(k_sur_flag, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, (const int *)lat_ok,
                        (const int *)lat_list, (const int *)counter, (const float *)dose, (const float *)plain,
                        dmax, tol, nz, ny, nx, stride, lz, ly, lx, bad);
@@ -1027,7 +1033,7 @@ void dh_launch_sur_check(const dh_ccc_args &ca, const float *terma, const float 
         hipLaunchKernelGGL(k_sur_fallback, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, s, terma, nz, ny, nx,
                            stride, lz, ly, lx, (const int *)redo, fb_list, counter + 1);
     }
-    dh_launch_ccc_counted(ca, tr_pad, electron, fb_list, counter + 1, nv, dose, s);
+    dh_launch_ccc_counted(ca, tr_pad, layout, electron, fb_list, counter + 1, nv, dose, s);
     DH_HIP(hipGetLastError());
 }
 

@@ -71,11 +71,11 @@ extern "C" int dh_autotune_proton_wet(int gpu_id, const dh_grid *grid, const dh_
         const dh_geom g = dh_proton_geom(grid, beam);
         const size_t n = (size_t)g.nx * g.ny * g.nz;
         const dh_pad p = dh_make_pad(g.nx, g.ny, g.nz);
-        float *vpad = (float *)dh_scratch(gpu_id, DH_SLOT_VOL_PAD, sizeof(float) * (size_t)p.pxy * p.pz);
+        float4 *vpad = (float4 *)dh_scratch(gpu_id, DH_SLOT_VOL_PAD, sizeof(float4) * (size_t)p.pxy * p.pz);
         float *ref = (float *)dh_scratch(gpu_id, DH_SLOT_TMP0, sizeof(float) * n);
         float *out = (float *)dh_scratch(gpu_id, DH_SLOT_TMP1, sizeof(float) * n);
         float *sm_ref = (float *)dh_scratch(gpu_id, DH_SLOT_OUT0, sizeof(float) * n);
-        dh_launch_pad(d_rlsp, vpad, g.nx, g.ny, g.nz, 0.0f, s);
+        dh_launch_pad_quad(d_rlsp, vpad, g.nx, g.ny, g.nz, 0.0f, s);
         const std::string key = dh_tune_key_geom(g);
         int best = 0;
         float best_ms = 0.0f;

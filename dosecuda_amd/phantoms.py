@@ -145,3 +145,93 @@ def influence_fields(model: ProtonBeamModel, seed=1003, gantries=(0.0, 120.0, 24
         pos += nl
         out.append((float(g), np.array(np.vstack(rows), dtype=np.single)))
     return out
+
+
+@dataclass
+class MixedCase:
+    """One case of the mixed workload: a phantom (water with spheres) and
+    either proton fields (spot rows per gantry) or photon beams (control
+    points: gantry, MU, MLC rows)."""
+    kind: str
+    shape: tuple
+    spacing: float
+    spheres: list
+    fields: list = field(default_factory=list)      # proton: [(gantry, spot_rows)]
+    cps: list = field(default_factory=list)         # photon: [(gantry, mu, mlc_rows)]
+
+
+def mixed_cases(pmodel: ProtonBeamModel, xmodel: PhotonBeamModel, seed=1005, n_cases=64,
+                sides=(96, 256), spacing=(2.0, 3.0), spots=(2000, 8000), fields=(1, 3), beams=(5, 9),
+                segments=(5, 20)):
+    """The mixed batched workload: n_cases phantoms with per-axis sides
+    uniform in `sides` voxels and spacing uniform in `spacing` mm, water with
+    2-5 spheres (radius 10-40 mm, HU uniform in [-800, 1200]); even cases
+    are IMPT (1-3 fields at random gantry angles, 2k-8k unit-MU spots each
+    over 6-15 energy layers whose R80 fits in 70% of the smallest extent,
+    positions uniform in +-45 mm BEV), odd cases IMRT (5-9 equally spaced
+    beams from a random start angle, 5-20 MLC segments each: the leaf pairs
+    within a random y window open on a random x interval, MU 5-40)."""
+    rng = np.random.default_rng(seed)
+    r80 = np.asarray(pmodel.divergence_params)[:, 1]
+    offs = np.asarray(xmodel.mlc_offsets, np.float64)
+    widths = np.asarray(xmodel.mlc_widths, np.float64)
+    out = []
+    for c in range(n_cases):
+        shape = tuple(int(v) for v in rng.integers(sides[0], sides[1] + 1, 3))
+        s = float(rng.uniform(*spacing))
+        ext = np.array([shape[2], shape[1], shape[0]], np.float64) * s
+        sph = []
+        for _ in range(int(rng.integers(2, 6))):
+            r = float(rng.uniform(10.0, min(40.0, 0.25 * ext.min())))
+            sph.append((rng.uniform(r, ext - r).tolist(), r, float(rng.uniform(-800.0, 1200.0))))
+        mc = MixedCase("impt" if c % 2 == 0 else "imrt", shape, s, sph)
+        if mc.kind == "impt":
+            ok = np.nonzero((r80 >= 30.0) & (r80 <= 0.7 * ext.min()))[0]
+            if ok.size == 0:
+                ok = np.array([int(np.argmin(r80))])
+            for _ in range(int(rng.integers(fields[0], fields[1] + 1))):
+                g = float(rng.uniform(0.0, 360.0))
+                n = int(rng.integers(spots[0], spots[1] + 1))
+                nl = int(min(ok.size, rng.integers(6, 16)))
+                ids = np.sort(rng.choice(ok, size=nl, replace=False))
+                per = rng.multinomial(n - nl, np.full(nl, 1.0 / nl)) + 1
+                rows = [np.column_stack([rng.uniform(-45.0, 45.0, (k, 2)), np.ones(k), np.full(k, e)])
+                        for e, k in zip(ids, per)]
+                mc.fields.append((g, np.array(np.vstack(rows), dtype=np.single)))
+        else:
+            nb = int(rng.integers(beams[0], beams[1] + 1))
+            g0 = float(rng.uniform(0.0, 360.0))
+            for b in range(nb):
+                g = float((g0 + 360.0 * b / nb) % 360.0)
+                for _ in range(int(rng.integers(segments[0], segments[1] + 1))):
+                    ylo = float(rng.uniform(-60.0, 20.0))
+                    yhi = ylo + float(rng.uniform(10.0, 60.0))
+                    mlc = np.zeros((offs.size, 4), np.float32)
+                    mlc[:, 2] = offs
+                    mlc[:, 3] = widths
+                    for i in np.nonzero((offs >= ylo) & (offs <= yhi))[0]:
+                        x1 = float(rng.uniform(-60.0, 20.0))
+                        mlc[i, 0] = x1
+                        mlc[i, 1] = min(x1 + float(rng.uniform(5.0, 50.0)), 70.0)
+                    mc.cps.append((g, float(rng.uniform(5.0, 40.0)), mlc))
+        out.append(mc)
+    return out
+
+
+def voxelize_mixed(case: MixedCase, device="cpu"):
+    """HU volume of a mixed case (torch float32 on `device`)."""
+    import torch
+    nz, ny, nx = case.shape
+    s = case.spacing
+    z = torch.arange(nz, dtype=torch.float64, device=device)[:, None, None] * s
+    y = torch.arange(ny, dtype=torch.float64, device=device)[None, :, None] * s
+    x = torch.arange(nx, dtype=torch.float64, device=device)[None, None, :] * s
+    hu = torch.zeros(case.shape, dtype=torch.float32, device=device)
+    for c, r, h in case.spheres:
+        m = (x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2 <= r * r
+        hu[m] = h
+    return hu
+
+
+def mixed_iso(case: MixedCase):
+    return _iso(case.shape, case.spacing)

@@ -39,7 +39,8 @@ import torch.nn.functional as F
 from . import _lib, engine
 
 SLOPE = 0.1          # LeakyReLU negative slope
-IN_CH = 32           # 8 TERMA + 8 density sub-voxels, 3 beam direction, 13 zero
+IN_CH = 32           # 8 TERMA + 8 density sub-voxels, 3 beam direction, spacing / 2.5 mm, 12 zero
+SP_REF = 2.5         # spacing channel = spacing / SP_REF
 N_SUB = 8            # output sub-voxels per cell
 AIR = 0.0012         # density of the padding outside the volume (IMRTClasses.cuh:6)
 
@@ -299,7 +300,7 @@ def _d2s(v):
     return v.reshape(d, h, w, 2, 2, 2).permute(0, 3, 1, 4, 2, 5).reshape(2 * d, 2 * h, 2 * w)
 
 
-def make_input(terma: torch.Tensor, density: torch.Tensor, direction, out=None):
+def make_input(terma: torch.Tensor, density: torch.Tensor, direction, spacing, out=None):
     """fp32 (D, H, W) TERMA and density -> channels-last bf16 input
     [D'/2, H'/2, W'/2, IN_CH] (D', H', W' = sizes padded to multiples of 16)
     and the TERMA scale (its maximum)."""
@@ -310,7 +311,8 @@ def make_input(terma: torch.Tensor, density: torch.Tensor, direction, out=None):
     x[..., 0:8] = _s2d(_pad(terma / scale, shp, 0.0))
     x[..., 8:16] = _s2d(_pad(density, shp, AIR))
     x[..., 16:19] = torch.as_tensor(direction, dtype=torch.float32, device=terma.device)
-    x[..., 19:] = 0
+    x[..., 19] = float(np.float32(spacing) / np.float32(SP_REF))
+    x[..., 20:] = 0
     return x, scale
 
 
@@ -350,51 +352,70 @@ def random_phantom(n, spacing, rng: np.random.Generator, device="cuda"):
     return rho.contiguous()
 
 
-def generate(model, n_samples, n=128, spacing=2.5, seed=1004, log=None):
-    """Training samples from the plain CCCS: random phantoms, gantry angles,
-    isocentre shifts of up to 2 cm and square fields of 3-20 cm.  Returns
-    inputs X [S, n/2, n/2, n/2, 16] bf16 (TERMA/max and density sub-voxels),
-    beam directions [S, 3] and targets Y [S, n/2, n/2, n/2, 8] fp32
-    (superposition dose / max TERMA, sub-voxels)."""
+def generate(model, n_samples, n=128, spacing=(2.0, 3.0), seed=1004, log=None):
+    """Training samples from the plain CCCS: random phantoms, voxel spacings
+    (uniform in the `spacing` range, or fixed), gantry angles, isocentre
+    shifts of up to 2 cm and square fields of 3-20 cm.  Returns inputs
+    X [S, n/2, n/2, n/2, 16] bf16 (TERMA/max and density sub-voxels), the
+    per-sample channels [S, 4] (beam direction, spacing / 2.5 mm) and targets
+    Y [S, n/2, n/2, n/2, 8] fp32 (superposition dose / max TERMA)."""
     from .beam_models import square_field_mlc
     rng = np.random.default_rng(seed)
     h = n // 2
     X = torch.empty((n_samples, h, h, h, 16), dtype=torch.bfloat16, device="cuda")
     Y = torch.empty((n_samples, h, h, h, N_SUB), dtype=torch.float32, device="cuda")
-    dirs = torch.empty((n_samples, 3), dtype=torch.float32, device="cuda")
+    dirs = torch.empty((n_samples, 4), dtype=torch.float32, device="cuda")
     terma = torch.empty((n, n, n), device="cuda")
     el = torch.empty_like(terma)
     dose = torch.empty_like(terma)
     t0 = time.time()
     for s in range(n_samples):
-        rho = random_phantom(n, spacing, rng)
+        sp = float(rng.uniform(*spacing)) if isinstance(spacing, (tuple, list)) else float(spacing)
+        rho = random_phantom(n, sp, rng)
         ga = float(rng.uniform(0.0, 360.0))
         fs = float(rng.uniform(3.0, 20.0))
-        iso = (np.full(3, (n - 1) / 2 * spacing) + rng.uniform(-20.0, 20.0, 3)).astype(np.float32)
+        iso = (np.full(3, (n - 1) / 2 * sp) + rng.uniform(-20.0, 20.0, 3)).astype(np.float32)
         mlc = square_field_mlc(model, fs, fs)
-        engine.photon_dose(rho, spacing, np.zeros(3), iso, 100.0, ga, 0.0, 0.0, model, mlc,
+        engine.photon_dose(rho, sp, np.zeros(3), iso, 100.0, ga, 0.0, 0.0, model, mlc,
                            out=dose, terma_out=terma, electron_out=el)
         scale = torch.clamp(terma.amax(), min=1e-12)
         X[s, ..., 0:8] = _s2d(terma / scale)
         X[s, ..., 8:16] = _s2d(rho)
-        dirs[s] = torch.from_numpy(beam_direction(ga, 0.0))
+        dirs[s, :3] = torch.from_numpy(beam_direction(ga, 0.0))
+        dirs[s, 3] = float(np.float32(sp) / np.float32(SP_REF))
         Y[s] = _s2d(superposition_target(dose, el, terma, model, 100.0) / scale)
         if log and (s + 1) % max(1, n_samples // 4) == 0:
             log(f"generated {s + 1}/{n_samples} samples ({time.time() - t0:.1f} s)")
     return X, dirs, Y
 
 
-def assemble(X, dirs):
-    """[B, d, h, w, 16] + [B, 3] -> network input [B, d, h, w, IN_CH]."""
+def assemble(X, chans):
+    """[B, d, h, w, 16] + [B, 4] (direction, spacing channel) -> network
+    input [B, d, h, w, IN_CH]."""
     B = X.shape[0]
     x = torch.zeros(X.shape[:4] + (IN_CH,), dtype=torch.bfloat16, device=X.device)
     x[..., 0:16] = X
-    x[..., 16:19] = dirs.view(B, 1, 1, 1, 3).to(torch.bfloat16)
+    x[..., 16:20] = chans.view(B, 1, 1, 1, 4).to(torch.bfloat16)
     return x
 
 
+@torch.no_grad()
+def eval_loss(net: SurrogateNet, X, chans, Y, batch=4):
+    """The training loss (weighted mean squared error, in target units) of
+    the network on samples X / chans / Y from generate()."""
+    tot, n = 0.0, 0
+    for i in range(0, X.shape[0], batch):
+        xb = assemble(X[i:i + batch], chans[i:i + batch])
+        yb = Y[i:i + batch]
+        pred = net(xb)[..., :N_SUB] * net.out_scale
+        wgt = torch.where(yb > 0, 1.0, 0.05)
+        tot += float((wgt * (pred - yb) ** 2).sum())
+        n += yb.numel()
+    return tot / max(n, 1)
+
+
 def train(net: SurrogateNet, model, n_samples=20000, chunk=1000, passes=2, batch=4, lr=1e-3,
-          n=128, spacing=2.5, seed=1004, log=None):
+          n=128, spacing=(2.0, 3.0), seed=1004, log=None):
     """Online training: the plain CCCS generates `chunk` samples at a time,
     each chunk is used for `passes` epochs, `n_samples` in total.  Adam on the
     FP32 master weights, one-cycle learning rate; loss = mean squared error
@@ -432,14 +453,17 @@ def train(net: SurrogateNet, model, n_samples=20000, chunk=1000, passes=2, batch
             opt.zero_grad(set_to_none=True)
             loss.backward()
             lv = float(loss.item())
-            # a non-finite loss or a spike far above the running mean skips
-            # the update; gradients are clipped to unit norm
+            # A non-finite loss or a spike above 20x the running level skips
+            # the update (gradients are clipped to unit norm otherwise).  The
+            # level follows every finite loss, capped at 20x itself, so it
+            # adapts when the loss level itself moves.
             if math.isfinite(lv) and (run is None or lv < 20.0 * run):
                 torch.nn.utils.clip_grad_norm_(net.parameters(), 1.0)
                 opt.step()
-                run = lv if run is None else 0.98 * run + 0.02 * lv
             else:
                 skipped += 1
+            if math.isfinite(lv):
+                run = lv if run is None else 0.98 * run + 0.02 * min(lv, 20.0 * run)
             sched.step()
             step += 1
             if step % max(1, total // 40) == 0 or step == total:
@@ -519,7 +543,7 @@ class SurrogateCCCS:
         return b
 
     @torch.no_grad()
-    def predict(self, terma, density, electron, gantry, couch, mu, out=None, stride=None):
+    def predict(self, terma, density, electron, gantry, couch, mu, spacing, out=None, stride=None):
         """Surrogate dose, finished like the plain CCCS's output: MU
         calibration, electron term, zero outside TERMA > 0.01, no negatives."""
         nz, ny, nx = terma.shape
@@ -527,7 +551,7 @@ class SurrogateCCCS:
         out = torch.empty_like(terma) if out is None else out
         d = beam_direction(gantry, couch)
         lib = _lib.load()
-        _lib.check(lib.dh_sur_pack_dev(nz, ny, nx, _lib.ptr(terma), _lib.ptr(density), _lib.ptr(d),
+        _lib.check(lib.dh_sur_pack_dev(nz, ny, nx, _lib.ptr(terma), _lib.ptr(density), _lib.ptr(d), float(spacing),
                                        _lib.ptr(b["x"]), _lib.ptr(b["tmax"]), _st()))
         y = self.net.infer(b["x"])
         _lib.check(lib.dh_sur_finish_dev(nz, ny, nx, _lib.ptr(y), _lib.ptr(b["tmax"]), self.out_scale,
@@ -571,7 +595,7 @@ class SurrogateCCCS:
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)] if timings is not None else None
         if ev:
             ev[0].record()
-        d = self.predict(terma, density, electron, ga, ta, mu, out=out, stride=stride)
+        d = self.predict(terma, density, electron, ga, ta, mu, spacing, out=out, stride=stride)
         if ev:
             ev[1].record()
         b = self._buffers(terma.shape, terma.device, stride)
@@ -670,8 +694,8 @@ def evaluate(net: SurrogateNet, model, n_beams=7, stride=4, tol=0.01, reps=5, lo
         torch.cuda.synchronize()
         engine.prof_enable(False)
         stage = {k: engine.prof_get(k)[0] / reps for k in ("raytrace", "terma", "pack_tr", "ccc_compact", "ccc")}
-        t_net = _time(lambda: sur.predict(terma, rho, el, ga, 0.0, 100.0), reps)
-        dn = sur.predict(terma, rho, el, ga, 0.0, 100.0)
+        t_net = _time(lambda: sur.predict(terma, rho, el, ga, 0.0, 100.0, sp), reps)
+        dn = sur.predict(terma, rho, el, ga, 0.0, 100.0, sp)
         tm = {}
         dc = sur.dose(terma, rho, el, sp, org, iso, 100.0, ga, 0.0, 0.0, timings=tm)
         t_chk = _time(lambda: sur.dose(terma, rho, el, sp, org, iso, 100.0, ga, 0.0, 0.0), reps)

@@ -222,11 +222,11 @@ int dh_nn_prep_weights_dev(const float *w, int32_t taps, int32_t cin, int32_t co
 /* Surrogate input [nz'/2][ny'/2][nx'/2][32] bf16 (n' = n rounded up to a
  * multiple of 16): channels 0-7 TERMA / max TERMA and 8-15 density of the
  * 2x2x2 sub-voxels (sub = 4 dz + 2 dy + dx; TERMA 0 and air density beyond
- * the volume), 16-18 the beam direction dir, 19-31 zero.  *tmax_out (device)
- * receives max TERMA. */
+ * the volume), 16-18 the beam direction dir, 19 spacing / 2.5 mm, 20-31
+ * zero.  *tmax_out (device) receives max TERMA. */
 int dh_sur_pack_dev(int32_t nz, int32_t ny, int32_t nx, const float *d_terma,
-                    const float *d_density, const float dir[3], void *d_x_out, float *d_tmax_out,
-                    void *stream);
+                    const float *d_density, const float dir[3], float spacing, void *d_x_out,
+                    float *d_tmax_out, void *stream);
 /* Surrogate dose from the network output y32 [cells][16] (channels 0-7 the
  * sub-voxels): dose = (y * tmax * out_scale + electron) * mu_scale where
  * TERMA > 0.01 and that value is positive, else 0; *dmax_out (device)
@@ -336,8 +336,8 @@ int dh_sched_info(const dh_sched *sched, int32_t *n_cases, int32_t *n_jobs, doub
  * caller's data (median of reps), require the outputs to agree (bit for bit
  * for the voxel-tile and LUT-staging variants, to 1e-5 of the largest value
  * for the convolution's summation orders) and store the fastest.  Keys:
- * "raytrace" / "smooth" / "pencil": "nz,ny,nx,axis" (dominant axis of the
- * beam direction); "conv3d": "n,d,h,w,ca,cb,up,cout,ksize". */
+ * "raytrace" / "smooth" / "pencil" / "ccc": "nz,ny,nx,axis" (dominant axis
+ * of the beam direction); "conv3d": "n,d,h,w,ca,cb,up,cout,ksize". */
 int dh_tune_set(const char *kernel, const char *key, int32_t choice, double ms);
 int dh_tune_get(const char *kernel, const char *key, int32_t *choice);   /* -1: none */
 int dh_tune_count(int32_t *n);
@@ -349,6 +349,13 @@ int dh_tune_clear(void);
 int dh_autotune_proton_wet(int gpu_id, const dh_grid *grid, const dh_proton_beam *beam,
                            const float *d_rlsp, int32_t reps, int32_t *best_raytrace,
                            int32_t *best_smooth, float *ms_raytrace, float *ms_smooth, void *stream);
+/* Photon control point: ray-trace voxel tile (8 choices) and the cone
+ * superposition's {TERMA, density} sampling layout (0 float2 pairs, 1 float4
+ * quads); ms_raytrace holds 8 entries, ms_ccc 2. */
+int dh_autotune_photon(int gpu_id, const dh_grid *grid, const dh_photon_cp *cp,
+                       const dh_photon_model *model, const float *mlc_rows, int32_t n_pairs,
+                       const float *d_density, int32_t reps, int32_t *best_raytrace,
+                       int32_t *best_layout, float *ms_raytrace, float *ms_ccc, void *stream);
 /* Pencil-beam voxel tile x LUT staging (LDS or global): 16 choices. */
 int dh_autotune_pencil(dh_proton_plan *plan, const float *d_wet, int32_t reps, int32_t *best,
                        float *ms, void *stream);

@@ -76,3 +76,28 @@ def test_tune_influence(proton_model):
     f = engine.InfluenceMatrix(0, shape, s, np.zeros(3), iso, 0.0, 0.0, m, spots)
     assert f.count(wet) > 0
     autotune.clear()
+
+
+def test_tune_photon(photon_model):
+    t = _torch()
+    from dosecuda_amd import autotune, engine
+    from dosecuda_amd.beam_models import square_field_mlc
+    pm = photon_model
+    shape, s = (36, 44, 40), 2.5
+    rho = t.full(shape, 1.00653, device="cuda")
+    rho[10:20, 12:30, 8:22] = 0.3
+    iso = np.array([(shape[2] - 1) / 2 * s, (shape[1] - 1) / 2 * s, (shape[0] - 1) / 2 * s], np.float32)
+    mlc = square_field_mlc(pm, 7.0, 6.0)
+    autotune.clear()
+    ref = engine.photon_dose(rho, s, np.zeros(3), iso, 100.0, 20.0, 0.0, 0.0, pm, mlc)
+    print("[case] tune photon", flush=True)
+    r = autotune.tune_photon(rho, s, np.zeros(3), iso, 100.0, 20.0, 0.0, 0.0, pm, mlc, reps=3)
+    print(f"[tune] {r}", flush=True)
+    got = engine.photon_dose(rho, s, np.zeros(3), iso, 100.0, 20.0, 0.0, 0.0, pm, mlc)
+    assert t.equal(got, ref)
+    # the quad layout explicitly, also for the voxel-list path
+    key = ",".join(map(str, shape)) + ",1"
+    for layout in (0, 1):
+        autotune.set_choice("ccc", key, layout)
+        assert t.equal(engine.photon_dose(rho, s, np.zeros(3), iso, 100.0, 20.0, 0.0, 0.0, pm, mlc), ref)
+    autotune.clear()

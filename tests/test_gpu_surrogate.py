@@ -231,9 +231,13 @@ def test_surrogate_trains_and_falls_back(photon_model):
     pm = photon_model
     print("[case] surrogate small train", flush=True)
     net = S.SurrogateNet().cuda()
-    hist = S.train(net, pm, n_samples=8, chunk=8, passes=10, batch=2, n=64, lr=2e-3)
-    print(f"[parity] surrogate loss {hist[0]:.3e} -> {hist[-1]:.3e}", flush=True)
-    assert hist[-1] < hist[0]
+    # the chunk train() generates first (seed * 1000 + 0)
+    X, ch, Y = S.generate(pm, 8, n=64, spacing=(2.0, 3.0), seed=5 * 1000)
+    l0 = S.eval_loss(net, X, ch, Y)
+    S.train(net, pm, n_samples=8, chunk=8, passes=10, batch=2, n=64, lr=2e-3, spacing=(2.0, 3.0), seed=5)
+    l1 = S.eval_loss(net, X, ch, Y)
+    print(f"[parity] surrogate training-set loss {l0:.3e} -> {l1:.3e}", flush=True)
+    assert l1 < 0.5 * l0
     n, s = 64, 2.5
     rng = np.random.default_rng(9)
     rho = S.random_phantom(n, s, rng)
@@ -290,9 +294,9 @@ def test_pack_and_finish_match_torch(photon_model):
     net = S.SurrogateNet().cuda()
     net.out_scale.fill_(2.5)
     sur = S.SurrogateCCCS(net, pm)
-    d = sur.predict(terma, rho, el, 70.0, 0.0, 100.0)
+    d = sur.predict(terma, rho, el, 70.0, 0.0, 100.0, s)
     b = sur._buffers(shape, terma.device, sur.stride)
-    x_ref, scale = S.make_input(terma, rho, S.beam_direction(70.0, 0.0))
+    x_ref, scale = S.make_input(terma, rho, S.beam_direction(70.0, 0.0), s)
     assert t.equal(b["x"][0], x_ref)
     assert t.equal(b["tmax"][0], scale)
     y = net(x_ref[None])[0, ..., :S.N_SUB]
