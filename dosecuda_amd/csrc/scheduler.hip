@@ -372,19 +372,28 @@ extern "C" int dh_sched_run(dh_sched *s, int32_t mode, void *stream)
         hipStream_t caller = (hipStream_t)stream;
         DH_HIP(hipEventRecord(s->done, caller));
         DH_HIP(hipStreamWaitEvent(s->origin, s->done, 0));
-        if (mode == 1 && !s->exec && !s->timed) {
-            // warm-up: sizes every workspace and measures the jobs
-            if (!s->planned) plan_streams(s);
-            DH_HIP(hipEventRecord(s->fork, s->origin));
-            enqueue_batch(s, true);
-            DH_HIP(hipStreamSynchronize(s->origin));
-            for (auto &J : s->jobs) {
-                float ms = 0.f;
-                DH_HIP(hipEventElapsedTime(&ms, J->ev[0], J->ev[1]));
-                J->ms = ms;
+        if (mode == 1 && !s->exec) {
+            if (!s->timed) {
+                // timed warm-up on the estimated plan
+                if (!s->planned) plan_streams(s);
+                DH_HIP(hipEventRecord(s->fork, s->origin));
+                enqueue_batch(s, true);
+                DH_HIP(hipStreamSynchronize(s->origin));
+                for (auto &J : s->jobs) {
+                    float ms = 0.f;
+                    DH_HIP(hipEventElapsedTime(&ms, J->ev[0], J->ev[1]));
+                    J->ms = ms;
+                }
+                s->timed = true;
+                s->planned = false;
             }
-            s->timed = true;
-            s->planned = false;
+            // the graph uses the plan on measured times; one eager run on that
+            // plan grows every workspace to its cases' sizes, so nothing is
+            // allocated (or synchronised) under capture
+            plan_streams(s);
+            DH_HIP(hipEventRecord(s->fork, s->origin));
+            enqueue_batch(s, false);
+            DH_HIP(hipStreamSynchronize(s->origin));
         }
         if (!s->planned) plan_streams(s);
         if (mode == 0) {

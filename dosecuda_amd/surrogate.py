@@ -245,9 +245,15 @@ class SurrogateNet(torch.nn.Module):
         return tot + 2.0 * t * ci * co * cells
 
 
-def save(net: SurrogateNet, path: str):
+def save(net: SurrogateNet, path: str, compact: bool = False):
+    """safetensors; compact = convolution weights as bf16 (what inference
+    uses: the bf16 rounding of the stored values is exact, so a compact
+    file gives the same inference results) and the rest fp32."""
     from safetensors.torch import save_file
-    sd = {k: v.detach().float().cpu().contiguous() for k, v in net.state_dict().items()}
+    sd = {}
+    for k, v in net.state_dict().items():
+        v = v.detach().cpu().contiguous()
+        sd[k] = v.to(torch.bfloat16) if (compact and k.endswith(".w")) else v.float()
     save_file(sd, path, metadata={"widths": ",".join(str(w) for w in net.widths),
                                   "in_ch": str(IN_CH), "n_sub": str(N_SUB)})
 
@@ -261,7 +267,7 @@ def load(path: str, device="cuda") -> SurrogateNet:
         raise ValueError(f"{path}: surrogate weights for another input/output layout ({meta})")
     widths = tuple(int(w) for w in meta.get("widths", "16,32,64,128").split(","))
     net = SurrogateNet(widths)
-    net.load_state_dict(load_file(path))
+    net.load_state_dict({k: v.float() for k, v in load_file(path).items()})
     return net.to(device)
 
 
@@ -414,7 +420,7 @@ def eval_loss(net: SurrogateNet, X, chans, Y, batch=4):
     return tot / max(n, 1)
 
 
-def train(net: SurrogateNet, model, n_samples=20000, chunk=1000, passes=2, batch=4, lr=1e-3,
+def train(net: SurrogateNet, model, n_samples=20000, chunk=1000, passes=3, batch=4, lr=5e-4,
           n=128, spacing=(2.0, 3.0), seed=1004, log=None):
     """Online training: the plain CCCS generates `chunk` samples at a time,
     each chunk is used for `passes` epochs, `n_samples` in total.  Adam on the
@@ -423,7 +429,9 @@ def train(net: SurrogateNet, model, n_samples=20000, chunk=1000, passes=2, batch
     n_chunks = max(1, n_samples // chunk)
     steps_per_chunk = max(1, chunk * passes // batch)
     total = n_chunks * steps_per_chunk
-    opt = torch.optim.Adam(net.parameters(), lr=lr)
+    # beta2 0.99: the second moment follows changes of the gradient scale
+    # (phantoms of different spacing) faster than the default 0.999
+    opt = torch.optim.Adam(net.parameters(), lr=lr, betas=(0.9, 0.99))
     sched = torch.optim.lr_scheduler.OneCycleLR(opt, max_lr=lr, total_steps=total, pct_start=0.05)
     g = torch.Generator(device="cpu").manual_seed(seed)
     hist = []
@@ -431,6 +439,7 @@ def train(net: SurrogateNet, model, n_samples=20000, chunk=1000, passes=2, batch
     step = 0
     tgen = 0.0
     run, skipped = None, 0
+    best, good, lr_scale, restores = None, None, 1.0, 0
     for ci in range(n_chunks):
         tg = time.time()
         X, dirs, Y = generate(model, chunk, n=n, spacing=spacing, seed=seed * 1000 + ci)
@@ -464,13 +473,31 @@ def train(net: SurrogateNet, model, n_samples=20000, chunk=1000, passes=2, batch
                 skipped += 1
             if math.isfinite(lv):
                 run = lv if run is None else 0.98 * run + 0.02 * min(lv, 20.0 * run)
+            # divergence guard: keep a snapshot while the running level is
+            # near its best; when it climbs 10x above the best, restore the
+            # snapshot, reset Adam's moments and halve the learning rate
+            if run is not None and step % 250 == 249:
+                if best is None or run < 2.0 * best:
+                    best = run if best is None else min(best, run)
+                    good = {k: v.detach().clone() for k, v in net.state_dict().items()}
+                elif run > 10.0 * best and good is not None:
+                    net.load_state_dict(good)
+                    opt.state.clear()
+                    lr_scale *= 0.5
+                    restores += 1
+                    run = best
+                    if log:
+                        log(f"step {step + 1}: divergence, weights restored, learning rate x{lr_scale}")
             sched.step()
+            if lr_scale != 1.0:
+                for gp in opt.param_groups:
+                    gp["lr"] *= lr_scale
             step += 1
             if step % max(1, total // 40) == 0 or step == total:
                 hist.append(run if run is not None else lv)
                 if log:
-                    log(f"step {step}/{total} loss {hist[-1]:.3e} lr {sched.get_last_lr()[0]:.2e} "
-                        f"skipped {skipped} ({time.time() - t0:.1f} s, generation {tgen:.1f} s)")
+                    log(f"step {step}/{total} loss {hist[-1]:.3e} lr {opt.param_groups[0]['lr']:.2e} "
+                        f"skipped {skipped} restored {restores} ({time.time() - t0:.1f} s, generation {tgen:.1f} s)")
         del X, dirs, Y
     return hist
 
@@ -771,9 +798,9 @@ def main(argv=None):
     t = sub.add_parser("train", help="train on plain-CCCS samples and save the weights")
     t.add_argument("--samples", type=int, default=20000)
     t.add_argument("--chunk", type=int, default=1000)
-    t.add_argument("--passes", type=int, default=2)
+    t.add_argument("--passes", type=int, default=3)
     t.add_argument("--batch", type=int, default=4)
-    t.add_argument("--lr", type=float, default=1e-3)
+    t.add_argument("--lr", type=float, default=5e-4)
     t.add_argument("--seed", type=int, default=1004)
     t.add_argument("--out", default=WEIGHTS)
     t.add_argument("--eval", action="store_true", help="evaluate after training")
@@ -793,7 +820,7 @@ def main(argv=None):
         train(net, model, n_samples=a.samples, chunk=a.chunk, passes=a.passes, batch=a.batch, lr=a.lr,
               seed=a.seed, log=log)
         os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
-        save(net, a.out)
+        save(net, a.out, compact=True)
         log(f"saved {a.out}")
         if not a.eval:
             return
