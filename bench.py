@@ -263,8 +263,14 @@ def main():
         pairs = st["plan"].pairs(wet)
         kt = {k: engine.prof_get(k) for k in ("pad", "raytrace", "smooth", "pencil")}
         pen_ms = kt["pencil"][0] / max(kt["pencil"][1], 1)
+        # k_pencil is VALU-bound: 11 full-rate VALU ops + 2 quarter-rate
+        # v_exp_f32 = 19 lane-op slots per pair; 256 CUs x 64 lane-ops / clock
+        valu_peak = 256 * 64 * 2.4e9 / 19 / 1e9
         res["impt"] = dict(ms_per_field=ms, pairs=pairs, gvoxel_spot_per_s=pairs / (ms * 1e-3) / 1e9,
                            pencil_ms=pen_ms, pencil_gvoxel_spot_per_s=pairs / (pen_ms * 1e-3) / 1e9,
+                           pencil_valu_roofline=dict(bound="VALU", achieved=pairs / (pen_ms * 1e-3) / 1e9,
+                                                     peak=valu_peak, unit="Gpair/s",
+                                                     frac=pairs / (pen_ms * 1e-3) / 1e9 / valu_peak),
                            kernels_ms={k: v[0] / max(v[1], 1) for k, v in kt.items()})
         log(f"IMPT: {ms:.3f} ms/field, {res['impt']['gvoxel_spot_per_s']:.1f} Gvoxel-spot/s "
             f"(pencil {pen_ms:.3f} ms)")
@@ -309,9 +315,17 @@ def main():
         bytes_launch = 8 * npad + 12 * n_act
         achieved = bytes_launch / (ccc_ms * 1e-3) / 1e9
         samples = n_act * 12 * sum(cone_steps(L0, c.spacing) for L0 in st["pm"].kernel[:, 5])
+        # k_ccc is bound by its trilinear gathers: 8 corners x {TERMA,
+        # density} = 64 bytes per cone sample through the vector L1, whose
+        # return path moves 64 B per clock per CU
+        cus, clk = 256, 2.4e9
+        l1_peak = cus * 64 * clk / 1e12
+        l1_ach = samples * 64 / (ccc_ms * 1e-3) / 1e12
         res["cccs"] = dict(ms_per_step=ms, ms_per_beam=ms / nb, ccc_ms=ccc_ms, n_active=n_act,
                            cone_samples_per_beam=samples,
                            gsamples_per_s=samples / (ccc_ms * 1e-3) / 1e9,
+                           gather_roofline=dict(bound="L1 gather", bytes_per_sample=64, achieved=l1_ach,
+                                                peak=l1_peak, unit="TB/s", frac=l1_ach / l1_peak),
                            kernels_ms={k: v[0] / max(v[1], 1) for k, v in kt.items()},
                            roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS,
                                          unit="GB/s", frac=achieved / HBM_PEAK_GBS, traffic=None))
@@ -363,11 +377,12 @@ def main():
                 autotune.tune_surrogate(net, (64, 64, 64), reps=5)
                 log(f"surrogate: convolution variants tuned ({time.time() - t0:.1f} s)")
             log("surrogate: 7-beam plan, checked surrogate vs plain CCCS")
-            r = S.evaluate(net, PhotonBeamModel(), stride=4, tol=0.01, reps=5)
+            r = S.evaluate(net, PhotonBeamModel(), stride=4, tol=0.01, dilate=1, reps=5)
             conv_ms = float(np.mean([b["surrogate_stage_ms"]["nn_conv3d"] for b in r["beams"]]))
             tf = r["net_gflop"] / (conv_ms * 1e-3) / 1e3
             res["surrogate"] = dict(
                 weights=os.path.relpath(S.WEIGHTS, ROOT), params=r["params"], stride=r["stride"], tol=r["tol"],
+                dilate=r["dilate"],
                 plan_gamma_checked=r["plan_gamma_checked"], min_beam_gamma_checked=r["min_gamma_checked"],
                 plan_gamma_net_only=r["plan_gamma_net"], min_beam_gamma_net_only=r["min_gamma_net"],
                 max_err_checked=max(b["max_err_checked"] for b in r["beams"]),
@@ -393,7 +408,7 @@ def main():
         cases = phantoms.mixed_cases(pm, xm, seed=1005, n_cases=a.mixed_cases)
         nat = S.NativeNet(S.load(S.WEIGHTS)) if os.path.exists(S.WEIGHTS) else None
         log(f"mixed: {len(cases)} cases, surrogate {'on' if nat else 'off (no weights)'}")
-        W = mixed.MixedWorkload(cases, pm, xm, surrogate=nat, streams=4, tol=0.01, log=log)
+        W = mixed.MixedWorkload(cases, pm, xm, surrogate=nat, streams=4, stride=4, tol=0.01, dilate=1, log=log)
         cnt = W.counts()
         log(f"mixed: set up in {W.setup_s:.1f} s: {cnt}")
         t0 = time.time()

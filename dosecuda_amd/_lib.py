@@ -108,7 +108,7 @@ SIGNATURES = {
                           _P, _P, _P],
     "dh_sur_check_work_size": [C.POINTER(Grid), C.c_int32, C.POINTER(C.c_int64)],
     "dh_sur_check_dev": [_I, C.POINTER(Grid), C.POINTER(PhotonCP), C.POINTER(PhotonModel), _P, _P,
-                         _P, _P, _P, C.c_int32, C.c_float, _P, C.c_int64, _P, _P],
+                         _P, _P, _P, C.c_int32, C.c_float, C.c_int32, _P, C.c_int64, _P, _P],
     "dh_nn_conv3d_wgrad_dev": [_P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                C.c_int32, C.c_int32, _P, _P],
     "dh_nn_prep_weights_dev": [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P, _P],
@@ -119,14 +119,14 @@ SIGNATURES = {
     "dh_sur_counts": [_I, C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
     "dh_sur_net_forward_dev": [_P, _P, C.c_int32, C.c_int32, C.c_int32, _P, _P],
     "dh_sur_dose_dev": [_P, C.POINTER(Grid), C.POINTER(PhotonCP), C.POINTER(PhotonModel), _P, _P, _P,
-                        C.c_int32, C.c_float, _P, _P],
+                        C.c_int32, C.c_float, C.c_int32, _P, _P],
     "dh_photon_dose_surrogate_dev": [_I, _P, C.POINTER(Grid), C.POINTER(PhotonCP), C.POINTER(PhotonModel), _P,
-                                     C.c_int32, _P, _P, _P, _P, C.c_int32, C.c_float, _P],
+                                     C.c_int32, _P, _P, _P, _P, C.c_int32, C.c_float, C.c_int32, _P],
     "dh_sched_create": [_I, C.c_int32, _P],
     "dh_sched_destroy": [_P],
     "dh_sched_add_case": [_P, C.POINTER(Grid), _P, _P, _P],
     "dh_sched_add_photon": [_P, C.c_int32, C.POINTER(PhotonCP), C.POINTER(PhotonModel), _P, C.c_int32, _P,
-                            C.c_int32, C.c_float],
+                            C.c_int32, C.c_float, C.c_int32],
     "dh_sched_add_proton_wet": [_P, C.c_int32, C.POINTER(ProtonBeam), _P],
     "dh_sched_add_proton_dose": [_P, C.c_int32, _P, _P],
     "dh_sched_add_influence_fill": [_P, C.c_int32, _P, _P, _P, _P],

@@ -470,7 +470,7 @@ static void photon_device(int dev, const dh_grid *grid, const dh_photon_cp *cp,
                           const dh_photon_model *m, const float *mlc_rows, int32_t n_pairs,
                           const float *d_density, float *d_dose, float *d_terma,
                           float *d_electron, float *d_wet, hipStream_t s,
-                          const dh_sur_net *net = nullptr, int stride = 4, float tol = 0.01f)
+                          const dh_sur_net *net = nullptr, int stride = 4, float tol = 0.01f, int dilate = 2)
 {
     size_t n = nvox_of(grid);
     DH_REQUIRE(cp && d_density && d_dose, "null argument");
@@ -490,7 +490,7 @@ static void photon_device(int dev, const dh_grid *grid, const dh_photon_cp *cp,
     if (net) {
         float dir[3];
         sur_direction(cp, dir);
-        dh_launch_sur_dose(net, ca, terma, d_density, el, dir, stride, tol, d_dose, s);
+        dh_launch_sur_dose(net, ca, terma, d_density, el, dir, stride, tol, dilate, d_dose, s);
     } else {
         dh_launch_ccc(ca, terma, d_density, el, trp, act, cnt, d_dose, s);
     }
@@ -595,10 +595,11 @@ extern "C" int dh_sur_check_work_size(const dh_grid *grid, int32_t stride, int64
 extern "C" int dh_sur_check_dev(int gpu_id, const dh_grid *grid, const dh_photon_cp *cp,
                                 const dh_photon_model *model, const float *d_terma,
                                 const float *d_density, const float *d_electron, float *d_dose,
-                                const float *d_dmax, int32_t stride, float tol, int32_t *d_work,
-                                int64_t work_len, float *d_plain, void *stream)
+                                const float *d_dmax, int32_t stride, float tol, int32_t dilate,
+                                int32_t *d_work, int64_t work_len, float *d_plain, void *stream)
 {
     return guarded([&] {
+        DH_REQUIRE(dilate >= 0 && dilate <= 2, "dilate must be 0, 1 or 2");
         dh_device_enter(gpu_id);
         nvox_of(grid);
         DH_REQUIRE(cp && d_terma && d_density && d_electron && d_dose && d_dmax && d_work && d_plain,
@@ -609,7 +610,7 @@ extern "C" int dh_sur_check_dev(int gpu_id, const dh_grid *grid, const dh_photon
         dh_ccc_args ca = make_ccc_args(grid, cp, model);
         dh_pad p = ca.p;
         void *trp = dh_scratch(gpu_id, DH_SLOT_TR_PAD, dh_tr_pad_bytes(ca));
-        dh_launch_sur_check(ca, d_terma, d_density, d_electron, d_dose, d_dmax, stride, tol, d_work,
+        dh_launch_sur_check(ca, d_terma, d_density, d_electron, d_dose, d_dmax, stride, tol, dilate, d_work,
                             d_plain, trp, (hipStream_t)stream);
     });
 }
@@ -618,31 +619,33 @@ extern "C" int dh_photon_dose_surrogate_dev(int gpu_id, const dh_sur_net *net, c
                                             const dh_photon_cp *cp, const dh_photon_model *model,
                                             const float *mlc_rows, int32_t n_pairs, const float *d_density,
                                             float *d_dose_out, float *d_terma_out, float *d_electron_out,
-                                            int32_t stride, float tol, void *stream)
+                                            int32_t stride, float tol, int32_t dilate, void *stream)
 {
     return guarded([&] {
         DH_REQUIRE(net, "null surrogate network");
         DH_REQUIRE(stride >= 1 && stride <= 64, "lattice stride must be 1..64");
+        DH_REQUIRE(dilate >= 0 && dilate <= 2, "dilate must be 0, 1 or 2");
         dh_device_enter(gpu_id);
         photon_device(gpu_id, grid, cp, model, mlc_rows, n_pairs, d_density, d_dose_out, d_terma_out,
-                      d_electron_out, nullptr, (hipStream_t)stream, net, stride, tol);
+                      d_electron_out, nullptr, (hipStream_t)stream, net, stride, tol, dilate);
     });
 }
 
 extern "C" int dh_sur_dose_dev(const dh_sur_net *net, const dh_grid *grid, const dh_photon_cp *cp,
                                const dh_photon_model *model, const float *d_terma, const float *d_density,
-                               const float *d_electron, int32_t stride, float tol, float *d_dose_out,
-                               void *stream)
+                               const float *d_electron, int32_t stride, float tol, int32_t dilate,
+                               float *d_dose_out, void *stream)
 {
     return guarded([&] {
         DH_REQUIRE(net && cp && d_terma && d_density && d_electron && d_dose_out, "null argument");
         DH_REQUIRE(stride >= 1 && stride <= 64, "lattice stride must be 1..64");
+        DH_REQUIRE(dilate >= 0 && dilate <= 2, "dilate must be 0, 1 or 2");
         dh_device_enter(dh_sur_net_gpu(net));
         nvox_of(grid);
         dh_ccc_args ca = make_ccc_args(grid, cp, model);
         float dir[3];
         sur_direction(cp, dir);
-        dh_launch_sur_dose(net, ca, d_terma, d_density, d_electron, dir, stride, tol, d_dose_out,
+        dh_launch_sur_dose(net, ca, d_terma, d_density, d_electron, dir, stride, tol, dilate, d_dose_out,
                            (hipStream_t)stream);
     });
 }
@@ -650,10 +653,10 @@ extern "C" int dh_sur_dose_dev(const dh_sur_net *net, const dh_grid *grid, const
 // ---- enqueue-only entry points for the scheduler ---------------------------
 void dh_photon_enqueue(int dev, const dh_grid *grid, const dh_photon_cp *cp, const dh_photon_model *m,
                        const float *mlc_rows, int n_pairs, const float *d_density, float *d_dose,
-                       const dh_sur_net *net, int stride, float tol, hipStream_t s)
+                       const dh_sur_net *net, int stride, float tol, int dilate, hipStream_t s)
 {
     photon_device(dev, grid, cp, m, mlc_rows, n_pairs, d_density, d_dose, nullptr, nullptr, nullptr, s, net,
-                  stride, tol);
+                  stride, tol, dilate);
 }
 
 void dh_proton_wet_enqueue(int dev, const dh_grid *grid, const dh_proton_beam *beam, const float *d_rlsp,

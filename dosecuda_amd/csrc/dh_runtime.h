@@ -201,7 +201,7 @@ struct dh_proton_plan;
 struct dh_influence;
 void dh_photon_enqueue(int dev, const dh_grid *grid, const dh_photon_cp *cp, const dh_photon_model *m,
                        const float *mlc_rows, int n_pairs, const float *d_density, float *d_dose,
-                       const dh_sur_net *net, int stride, float tol, hipStream_t s);
+                       const dh_sur_net *net, int stride, float tol, int dilate, hipStream_t s);
 void dh_proton_wet_enqueue(int dev, const dh_grid *grid, const dh_proton_beam *beam, const float *d_rlsp,
                            float *d_wet, hipStream_t s);
 void dh_proton_plan_enqueue(dh_proton_plan *p, const float *d_wet, float *d_dose, hipStream_t s);
@@ -221,10 +221,10 @@ int dh_sur_net_gpu(const dh_sur_net *net);
 // workspace of the calling thread; a = the control point's cone arguments.
 void dh_launch_sur_dose(const dh_sur_net *net, const dh_ccc_args &a, const float *terma,
                         const float *density, const float *electron, const float dir[3],
-                        int stride, float tol, float *dose, hipStream_t s);
+                        int stride, float tol, int dilate, float *dose, hipStream_t s);
 void dh_launch_sur_check(const dh_ccc_args &a, const float *terma, const float *density,
                          const float *electron, float *dose, const float *dmax, int stride,
-                         float tol, int *work, float *plain, void *tr_pad, hipStream_t s);
+                         float tol, int dilate, int *work, float *plain, void *tr_pad, hipStream_t s);
 
 // A proton field's spots sorted by energy id (stable), its layers
 // (importLayers) and one 400-entry LUT row per layer.  spots holds 8 floats

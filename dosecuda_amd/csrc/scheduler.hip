@@ -75,6 +75,7 @@ struct job {
     const dh_sur_net *net = nullptr;
     int stride = 4;
     float tol = 0.01f;
+    int dilate = 2;
     // proton
     dh_proton_beam beam{};
     float *wet_out = nullptr;
@@ -157,7 +158,7 @@ void enqueue_job(dh_sched *s, job &J, scase &C, hipStream_t st, bool timing)
     case J_PHOTON: {
         float *tmp = (float *)dh_scratch(s->gpu, DH_SLOT_OUT1, sizeof(float) * (size_t)n);
         dh_photon_enqueue(s->gpu, &C.grid, &J.cp, &J.model.m, J.mlc.data(), (int)(J.mlc.size() / 4), C.vol, tmp,
-                          J.net, J.stride, J.tol, st);
+                          J.net, J.stride, J.tol, J.dilate, st);
         hipLaunchKernelGGL(k_sched_accum, dim3(grid_of(n)), dim3(256), 0, st, C.dose, (const float *)tmp, n);
         break;
     }
@@ -294,10 +295,11 @@ job &new_job(dh_sched *s, int32_t case_id, int kind, double est)
 
 extern "C" int dh_sched_add_photon(dh_sched *s, int32_t case_id, const dh_photon_cp *cp,
                                    const dh_photon_model *model, const float *mlc_rows, int32_t n_pairs,
-                                   const dh_sur_net *net, int32_t stride, float tol)
+                                   const dh_sur_net *net, int32_t stride, float tol, int32_t dilate)
 {
     return dh_guarded([&] {
         DH_REQUIRE(cp && model, "null argument");
+        DH_REQUIRE(dilate >= 0 && dilate <= 2, "dilate must be 0, 1 or 2");
         DH_REQUIRE(n_pairs >= 0 && n_pairs <= DH_MAX_PAIRS && (n_pairs == 0 || mlc_rows), "bad MLC");
         DH_REQUIRE(!net || (stride >= 1 && stride <= 64), "lattice stride must be 1..64");
         DH_REQUIRE(s && case_id >= 0 && case_id < (int32_t)s->cases.size(), "unknown case");
@@ -310,6 +312,7 @@ extern "C" int dh_sched_add_photon(dh_sched *s, int32_t case_id, const dh_photon
         J.net = net;
         J.stride = stride;
         J.tol = tol;
+        J.dilate = dilate;
     });
 }
 

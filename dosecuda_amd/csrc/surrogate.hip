@@ -578,10 +578,10 @@ __global__ void k_sur_flag(const int *__restrict__ lat_ok, const int *__restrict
 }
 
 // per cell: recompute its voxels when its lattice voxel has TERMA <= 0.01
-// (cell not covered by the check) or the cell or a neighbour (26-
-// neighbourhood) failed; written over lat_ok's companion array `redo`
+// (cell not covered by the check) or the cell failed, or (dilate 1) one of
+// its 6 face neighbours or (dilate 2) one of its 26 neighbours failed
 __global__ void k_sur_dilate(const int *__restrict__ lat_ok, const int *__restrict__ bad, int lz, int ly, int lx,
-                             int *__restrict__ redo)
+                             int dilate, int *__restrict__ redo)
 {
     const int64_t nc = (int64_t)lz * ly * lx;
     const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -589,10 +589,11 @@ __global__ void k_sur_dilate(const int *__restrict__ lat_ok, const int *__restri
     const int ci = (int)(c % lx);
     const int64_t r = c / lx;
     const int cj = (int)(r % ly), ck = (int)(r / ly);
-    bool fb = !lat_ok[c];
-    for (int dk = -1; dk <= 1 && !fb; dk++)
+    bool fb = !lat_ok[c] || bad[c] != 0;
+    for (int dk = -1; dk <= 1 && !fb && dilate > 0; dk++)
         for (int dj = -1; dj <= 1 && !fb; dj++)
             for (int di = -1; di <= 1 && !fb; di++) {
+                if (dilate == 1 && (dk != 0) + (dj != 0) + (di != 0) != 1) continue;
                 const int a = ck + dk, b = cj + dj, e = ci + di;
                 if (a < 0 || a >= lz || b < 0 || b >= ly || e < 0 || e >= lx) continue;
                 fb = bad[((int64_t)a * ly + b) * lx + e] != 0;
@@ -961,7 +962,7 @@ extern "C" int dh_sur_net_forward_dev(const dh_sur_net *net, const void *d_x, in
 
 void dh_launch_sur_dose(const dh_sur_net *net, const dh_ccc_args &ca, const float *terma,
                         const float *density, const float *electron, const float dir[3],
-                        int stride, float tol, float *dose, hipStream_t s)
+                        int stride, float tol, int dilate, float *dose, hipStream_t s)
 {
     const dh_geom &g = ca.g;
     const int nz = g.nz, ny = g.ny, nx = g.nx;
@@ -980,7 +981,7 @@ void dh_launch_sur_dose(const dh_sur_net *net, const dh_ccc_args &ca, const floa
     launch_pack(nz, ny, nx, terma, density, dir, g.s, x, scal, s);
     sur_forward(net, x, cz, cy, cx, y32, ws, s);
     launch_finish(nz, ny, nx, y32, scal, net->out_scale, terma, electron, ca.mu_cal * ca.mu, dose, scal + 1, s);
-    dh_launch_sur_check(ca, terma, density, electron, dose, scal + 1, stride, tol, work, plain, trp, s);
+    dh_launch_sur_check(ca, terma, density, electron, dose, scal + 1, stride, tol, dilate, work, plain, trp, s);
 }
 
 extern "C" int dh_sur_counts(int gpu_id, int32_t *lattice, int32_t *fallback)
@@ -1006,7 +1007,7 @@ int64_t dh_sur_check_work_len(int nz, int ny, int nx, int stride)
 
 void dh_launch_sur_check(const dh_ccc_args &ca, const float *terma, const float *density,
                          const float *electron, float *dose, const float *dmax, int stride,
-                         float tol, int *work, float *plain, void *tr_pad, hipStream_t s)
+                         float tol, int dilate, int *work, float *plain, void *tr_pad, hipStream_t s)
 {
     const dh_geom &g = ca.g;
     const int nz = g.nz, ny = g.ny, nx = g.nx;
@@ -1029,7 +1030,7 @@ void dh_launch_sur_check(const dh_ccc_args &ca, const float *terma, const float 
     {
         dh_timed t("sur_fallback", s);
         hipLaunchKernelGGL(k_sur_dilate, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, (const int *)lat_ok,
-                           (const int *)bad, lz, ly, lx, redo);
+                           (const int *)bad, lz, ly, lx, dilate, redo);
         hipLaunchKernelGGL(k_sur_fallback, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, s, terma, nz, ny, nx,
                            stride, lz, ly, lx, (const int *)redo, fb_list, counter + 1);
     }

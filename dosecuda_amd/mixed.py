@@ -37,7 +37,8 @@ def _material(hu: torch.Tensor, conv) -> torch.Tensor:
 
 
 class MixedWorkload:
-    def __init__(self, cases: list, pmodel, xmodel, surrogate=None, streams=4, stride=4, tol=0.01, log=None):
+    def __init__(self, cases: list, pmodel, xmodel, surrogate=None, streams=4, stride=4, tol=0.01, dilate=1,
+                 log=None):
         self.cases, self.pmodel, self.xmodel = cases, pmodel, xmodel
         self.surrogate = surrogate
         self.sched = Scheduler(0, streams=streams)
@@ -73,7 +74,7 @@ class MixedWorkload:
                 cid = self.sched.add_case(rho, c.spacing, org, dose)
                 for g, mu, mlc in c.cps:
                     self.sched.add_photon(cid, iso, mu, g, 0.0, 0.0, xmodel, mlc, surrogate=surrogate,
-                                          stride=stride, tol=tol)
+                                          stride=stride, tol=tol, dilate=dilate)
                     self.n_cps += 1
                     self.voxel_cps += int(np.prod(c.shape))
                 self.items.append(dict(kind="imrt", case=c, rho=rho, iso=iso, dose=dose))

@@ -52,7 +52,8 @@ class Scheduler:
         self._keep += [volume, dose]
         return int(cid.value)
 
-    def add_photon(self, case: int, iso, mu, ga, ca, ta, model, mlc, surrogate=None, stride=4, tol=0.01):
+    def add_photon(self, case: int, iso, mu, ga, ca, ta, model, mlc, surrogate=None, stride=4, tol=0.01,
+                   dilate=1):
         """One photon control point; surrogate = a surrogate.NativeNet (the
         checked surrogate replaces the cone superposition) or None (plain)."""
         keep: list = []
@@ -61,7 +62,7 @@ class Scheduler:
         _lib.check(_lib.load().dh_sched_add_photon(self._h, int(case), _lib.photon_cp(iso, mu, ga, ca, ta), m,
                                                    _lib.ptr(rows), rows.shape[0],
                                                    surrogate.handle if surrogate is not None else None,
-                                                   int(stride), float(tol)))
+                                                   int(stride), float(tol), int(dilate)))
         if surrogate is not None:
             self._keep.append(surrogate)
 

@@ -545,11 +545,11 @@ class SurrogateCCCS:
     Everything runs on the device without host synchronisation, so a beam
     (or a plan of beams) can be captured into a HIP graph."""
 
-    def __init__(self, net: SurrogateNet, model, stride=4, tol=0.01):
+    def __init__(self, net: SurrogateNet, model, stride=4, tol=0.01, dilate=1):
         self.net = net.eval().prepare()
         self.out_scale = float(net.out_scale)
         self.model = model
-        self.stride, self.tol = int(stride), float(tol)
+        self.stride, self.tol, self.dilate = int(stride), float(tol), int(dilate)
         self._buf = {}
         self.native = NativeNet(self.net)
 
@@ -602,7 +602,7 @@ class SurrogateCCCS:
         m = engine.photon_args(self.model, keep)
         _lib.check(_lib.load().dh_sur_dose_dev(
             self.native.handle, _lib.grid(terma.shape, spacing, origin), _lib.photon_cp(iso, mu, ga, ca, ta), m,
-            _lib.ptr(terma), _lib.ptr(density), _lib.ptr(electron), stride, tol, _lib.ptr(out), _st()))
+            _lib.ptr(terma), _lib.ptr(density), _lib.ptr(electron), stride, tol, self.dilate, _lib.ptr(out), _st()))
         if ev:
             ev[1].record()
             torch.cuda.synchronize()
@@ -631,7 +631,7 @@ class SurrogateCCCS:
         _lib.check(_lib.load().dh_sur_check_dev(
             terma.device.index or 0, _lib.grid(terma.shape, spacing, origin), _lib.photon_cp(iso, mu, ga, ca, ta),
             m, _lib.ptr(terma), _lib.ptr(density), _lib.ptr(electron), _lib.ptr(d), _lib.ptr(b["dmax"]), stride,
-            tol, _lib.ptr(b["work"]), b["work"].numel(), _lib.ptr(b["plain"]), _st()))
+            tol, self.dilate, _lib.ptr(b["work"]), b["work"].numel(), _lib.ptr(b["plain"]), _st()))
         if ev:
             ev[2].record()
             torch.cuda.synchronize()
@@ -691,7 +691,7 @@ def _time(fn, reps):
     return e0.elapsed_time(e1) / reps
 
 
-def evaluate(net: SurrogateNet, model, n_beams=7, stride=4, tol=0.01, reps=5, log=None):
+def evaluate(net: SurrogateNet, model, n_beams=7, stride=4, tol=0.01, reps=5, log=None, dilate=1):
     """Per-beam and plan gamma (2%/2 mm, 10% cut) of the surrogate alone and
     of the checked surrogate against the plain CCCS on the 128^3 lung/bone
     phantom with 10x10 cm fields, and the timings of both paths (eager, and
@@ -701,10 +701,10 @@ def evaluate(net: SurrogateNet, model, n_beams=7, stride=4, tol=0.01, reps=5, lo
     case = imrt_lung_bone_case(seed=1002, n=128, spacing=2.5)
     rho = torch.from_numpy(to_density(case)).cuda()
     mlc = square_field_mlc(model, 10.0, 10.0)
-    sur = SurrogateCCCS(net, model, stride=stride, tol=tol)
+    sur = SurrogateCCCS(net, model, stride=stride, tol=tol, dilate=dilate)
     sp, org, iso = case.spacing, case.origin, case.iso
     angles = imrt_beams(n_beams)
-    res = dict(beams=[], stride=stride, tol=tol)
+    res = dict(beams=[], stride=stride, tol=tol, dilate=dilate)
     sums = {k: torch.zeros_like(rho) for k in ("plain", "net", "checked")}
     termas, els, plains, checked = [], [], [], []
     for ga in angles:
@@ -806,11 +806,12 @@ def main(argv=None):
     t.add_argument("--eval", action="store_true", help="evaluate after training")
     e = sub.add_parser("eval", help="gamma and timing on the 7-beam lung/bone phantom")
     e.add_argument("--weights", default=WEIGHTS)
-    e.add_argument("--sweep", default=None, help="stride:tol pairs, e.g. 4:0.01,4:0.02")
+    e.add_argument("--sweep", default=None, help="stride:tol[:dilate] settings, e.g. 4:0.01,4:0.02:1")
     for q in (t, e):
         q.add_argument("--json", default=None, help="write the evaluation as JSON here")
         q.add_argument("--stride", type=int, default=4)
         q.add_argument("--tol", type=float, default=0.01)
+        q.add_argument("--dilate", type=int, default=1, help="0 none, 1 face neighbours, 2 all 26")
     a = ap.parse_args(argv)
     log = lambda m: print(m, flush=True)
     model = PhotonBeamModel()
@@ -829,8 +830,8 @@ def main(argv=None):
     if a.cmd == "eval" and a.sweep:
         out = []
         for pair in a.sweep.split(","):
-            st, tl = pair.split(":")
-            r = evaluate(net, model, stride=int(st), tol=float(tl))
+            f = pair.split(":")
+            r = evaluate(net, model, stride=int(f[0]), tol=float(f[1]), dilate=int(f[2]) if len(f) > 2 else 1)
             summary = {k: v for k, v in r.items() if k != "beams"}
             log(json.dumps(summary))
             out.append(r)
@@ -838,7 +839,7 @@ def main(argv=None):
             with open(a.json, "w") as f:
                 json.dump(out, f, indent=1)
         return
-    r = evaluate(net, model, stride=a.stride, tol=a.tol, log=log)
+    r = evaluate(net, model, stride=a.stride, tol=a.tol, dilate=a.dilate, log=log)
     summary = {k: v for k, v in r.items() if k != "beams"}
     log(json.dumps(summary))
     if a.json:

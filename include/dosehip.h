@@ -247,7 +247,8 @@ int dh_sur_check_work_size(const dh_grid *grid, int32_t stride, int64_t *n_ints)
 int dh_sur_check_dev(int gpu_id, const dh_grid *grid, const dh_photon_cp *cp,
                      const dh_photon_model *model, const float *d_terma, const float *d_density,
                      const float *d_electron, float *d_dose, const float *d_dmax, int32_t stride,
-                     float tol, int32_t *d_work, int64_t work_len, float *d_plain, void *stream);
+                     float tol, int32_t dilate, int32_t *d_work, int64_t work_len, float *d_plain,
+                     void *stream);
 /* Native surrogate network: the U-Net's layer weights on the device in the
  * forward bf16 layout ([taps][cout_pad][cin], dh_nn_prep_weights_dev mode 0)
  * and fp32 biases, in the order e0.0 e0.1 e1.0 e1.1 e2.0 e2.1 bt.0 bt.1
@@ -268,8 +269,8 @@ int dh_sur_net_forward_dev(const dh_sur_net *net, const void *d_x, int32_t cz, i
  * fallback (dh_sur_check_dev semantics), all on the device. */
 int dh_sur_dose_dev(const dh_sur_net *net, const dh_grid *grid, const dh_photon_cp *cp,
                     const dh_photon_model *model, const float *d_terma, const float *d_density,
-                    const float *d_electron, int32_t stride, float tol, float *d_dose_out,
-                    void *stream);
+                    const float *d_electron, int32_t stride, float tol, int32_t dilate,
+                    float *d_dose_out, void *stream);
 /* Lattice and fallback voxel counts of the last checked surrogate dose run
  * by dh_sur_dose_dev / dh_photon_dose_surrogate_dev from this thread on
  * gpu_id (synchronises the device). */
@@ -280,7 +281,7 @@ int dh_photon_dose_surrogate_dev(int gpu_id, const dh_sur_net *net, const dh_gri
                                  const dh_photon_cp *cp, const dh_photon_model *model,
                                  const float *mlc_rows, int32_t n_pairs, const float *d_density,
                                  float *d_dose_out, float *d_terma_out, float *d_electron_out,
-                                 int32_t stride, float tol, void *stream);
+                                 int32_t stride, float tol, int32_t dilate, void *stream);
 /* Global gamma index on the voxel grid (Low et al. 1998): out = gamma for
  * voxels with ref >= cut, -1 elsewhere; search radius in voxels. */
 int dh_gamma_dev(const float *eval, const float *ref, int32_t nz, int32_t ny, int32_t nx,
@@ -308,7 +309,7 @@ int dh_sched_add_case(dh_sched *sched, const dh_grid *grid, const float *d_volum
  * surrogate (dh_photon_dose_surrogate_dev). */
 int dh_sched_add_photon(dh_sched *sched, int32_t case_id, const dh_photon_cp *cp,
                         const dh_photon_model *model, const float *mlc_rows, int32_t n_pairs,
-                        const dh_sur_net *net, int32_t stride, float tol);
+                        const dh_sur_net *net, int32_t stride, float tol, int32_t dilate);
 /* Smoothed WET of the case's RLSP for one proton field into d_wet_out. */
 int dh_sched_add_proton_wet(dh_sched *sched, int32_t case_id, const dh_proton_beam *beam,
                             float *d_wet_out);

@@ -85,7 +85,9 @@ def test_mixed_with_trained_surrogate(proton_model, photon_model):
     net = S.load(S.WEIGHTS)
     cases = [c for c in _small_cases(proton_model, photon_model, seed=33, n=6) if c.kind == "imrt"]
     print(f"[case] mixed surrogate: {[c.shape for c in cases]}", flush=True)
-    W = mixed.MixedWorkload(cases, proton_model, photon_model, surrogate=S.NativeNet(net), streams=2, tol=0.01)
+    # the strict check setting (26-neighbour dilation); the bench reports the default one
+    W = mixed.MixedWorkload(cases, proton_model, photon_model, surrogate=S.NativeNet(net), streams=2, tol=0.01,
+                            dilate=2)
     W.run(graph=True)
     t.cuda.synchronize()
     for e in W.errors():

@@ -307,7 +307,7 @@ def test_pack_and_finish_match_torch(photon_model):
     assert float(b["dmax"][0]) == float(ref.max())
 
 
-def _torch_checked(pred, plain, terma, stride, tol):
+def _torch_checked(pred, plain, terma, stride, tol, dilate=2):
     """The lattice check and fallback restated with torch operations."""
     import torch
     import torch.nn.functional as F
@@ -319,15 +319,25 @@ def _torch_checked(pred, plain, terma, stride, tol):
     li = torch.nonzero((active & lat).view(-1)).view(-1)
     bad.view(-1)[li] = (pred.view(-1)[li] - plain.view(-1)[li]).abs() > tol * dmax
     pool = lambda v: F.max_pool3d(v.float()[None, None], kernel_size=stride, stride=stride, ceil_mode=True)
-    cells = F.max_pool3d(pool(bad), kernel_size=3, stride=1, padding=1)
+    pb = pool(bad)
+    if dilate == 2:
+        cells = F.max_pool3d(pb, kernel_size=3, stride=1, padding=1)
+    elif dilate == 1:
+        k = torch.zeros((1, 1, 3, 3, 3), device=pb.device)
+        k[0, 0, 1, 1, :] = 1
+        k[0, 0, 1, :, 1] = 1
+        k[0, 0, :, 1, 1] = 1
+        cells = (F.conv3d(pb, k, padding=1) > 0).float()
+    else:
+        cells = pb
     cells = torch.maximum(cells, pool(active) * (1.0 - pool(active & lat)))
     up = cells.repeat_interleave(stride, 2).repeat_interleave(stride, 3).repeat_interleave(stride, 4)
     fb = (up[0, 0, :pred.shape[0], :pred.shape[1], :pred.shape[2]] > 0) & active
     return torch.where(fb, plain, pred), int(li.numel()), int(fb.sum())
 
 
-@pytest.mark.parametrize("stride", [3, 4])
-def test_device_check_matches_torch(photon_model, stride):
+@pytest.mark.parametrize("stride, dilate", [(3, 2), (4, 2), (4, 1), (4, 0)])
+def test_device_check_matches_torch(photon_model, stride, dilate):
     t = _torch()
     from dosecuda_amd import engine, surrogate as S
     from dosecuda_amd.beam_models import square_field_mlc
@@ -343,8 +353,8 @@ def test_device_check_matches_torch(photon_model, stride):
     pred[10:18, 10:20, 12:22] *= 1.05          # fails a 1% check where the dose is high
     pred[25:35, 25:35, 25:35] *= 1.002         # passes
     pred[30:32, 2:6, 2:6] = 0.0                # isolated failures
-    want, n_lat, n_fb = _torch_checked(pred, plain, terma, stride, 0.01)
-    sur = S.SurrogateCCCS(S.SurrogateNet().cuda(), pm, stride=stride)
+    want, n_lat, n_fb = _torch_checked(pred, plain, terma, stride, 0.01, dilate)
+    sur = S.SurrogateCCCS(S.SurrogateNet().cuda(), pm, stride=stride, dilate=dilate)
     b = sur._buffers(shape, terma.device, stride)
     b["dmax"].fill_(float(pred.max()))
     got = pred.clone()
@@ -354,10 +364,11 @@ def test_device_check_matches_torch(photon_model, stride):
     _lib.check(_lib.load().dh_sur_check_dev(0, _lib.grid(shape, s, np.zeros(3)),
                                             _lib.photon_cp(iso, 100.0, 200.0, 0.0, 0.0), m, _lib.ptr(terma),
                                             _lib.ptr(rho), _lib.ptr(el), _lib.ptr(got), _lib.ptr(b["dmax"]),
-                                            stride, 0.01, _lib.ptr(b["work"]), b["work"].numel(),
+                                            stride, 0.01, dilate, _lib.ptr(b["work"]), b["work"].numel(),
                                             _lib.ptr(b["plain"]), S._st()))
     lat, fb = sur.counts(shape, stride=stride)
-    print(f"[parity] device check stride {stride}: lattice {lat} fallback {fb} (torch {n_lat}, {n_fb})", flush=True)
+    print(f"[parity] device check stride {stride} dilate {dilate}: lattice {lat} fallback {fb} "
+          f"(torch {n_lat}, {n_fb})", flush=True)
     assert (lat, fb) == (n_lat, n_fb)
     assert 0 < fb < int((~(terma <= 0.01)).sum())
     assert t.equal(got, want)
@@ -401,5 +412,5 @@ def test_native_dose_equals_torch_path(photon_model):
     rows = np.ascontiguousarray(np.asarray(mlc, dtype=np.float32)[:, :4])
     _lib.check(_lib.load().dh_photon_dose_surrogate_dev(
         0, sur.native.handle, _lib.grid(shape, s, np.zeros(3)), _lib.photon_cp(iso, 100.0, 120.0, 0.0, 0.0), m,
-        _lib.ptr(rows), rows.shape[0], _lib.ptr(rho), _lib.ptr(d), None, None, 4, 0.02, S._st()))
+        _lib.ptr(rows), rows.shape[0], _lib.ptr(rho), _lib.ptr(d), None, None, 4, 0.02, 2, S._st()))
     assert t.equal(d, a)
