@@ -170,6 +170,44 @@ __device__ static inline bool dh_tile_voxel(const dh_tile &t, int b, int tid, in
     return i < nx && j < ny && k < nz;
 }
 
+// Ordered compaction by a 256-thread workgroup over K rounds of 256
+// consecutive items starting at `base`: flag(v) is evaluated once per item
+// (coalesced), the flagged items are appended to `list` in item order with
+// one atomic per workgroup (K times fewer than one per round).
+template <int K, class Flag>
+__device__ static inline void dh_block_compact(uint32_t base, uint32_t n, Flag flag, int *__restrict__ list,
+                                               int *__restrict__ counter)
+{
+    __shared__ int s_cnt[K][4];
+    __shared__ int s_base;
+    const int lane = __lane_id(), wave = threadIdx.x >> 6;
+    uint64_t m[K];
+#pragma unroll
+    for (int q = 0; q < K; q++) {
+        const uint32_t v = base + (uint32_t)q * 256u + threadIdx.x;
+        const bool f = v < n && flag(v);
+        m[q] = __builtin_amdgcn_ballot_w64(f);
+        if (lane == 0) s_cnt[q][wave] = __popcll(m[q]);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int q = 0; q < K; q++)
+            for (int w = 0; w < 4; w++) {
+                const int c = s_cnt[q][w];
+                s_cnt[q][w] = tot;
+                tot += c;
+            }
+        s_base = tot ? atomicAdd(counter, tot) : 0;
+    }
+    __syncthreads();
+    const uint64_t below = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int q = 0; q < K; q++)
+        if ((m[q] >> lane) & 1ull)
+            list[s_base + s_cnt[q][wave] + __popcll(m[q] & below)] = (int)(base + (uint32_t)q * 256u + threadIdx.x);
+}
+
 // Trilinear sample at a texel-space point u = t - 0.5 known to lie in
 // [-0.5, n - 0.5) on every axis (no clamping needed).
 __device__ static inline float dh_trilinear_u(const float *__restrict__ vol, const dh_pad &p,

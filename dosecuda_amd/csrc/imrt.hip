@@ -12,8 +12,10 @@
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kCompactRounds = 8;     // voxels per thread of the compaction
 
 inline unsigned grid_for(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+inline unsigned grid_compact(size_t n) { return (unsigned)((n + kBlock * kCompactRounds - 1) / (kBlock * kCompactRounds)); }
 
 // Source-occlusion head transmission (IMRTClasses.cu:120-150) over the open
 // leaf pairs; the scaled leaf edges {xL, xR, yB, yT} are precomputed per pair.
@@ -191,35 +193,17 @@ __global__ void __launch_bounds__(kBlock)
 k_ccc_compact(dh_geom g, dh_pad p, const void *__restrict__ tr, int *__restrict__ list,
               int *__restrict__ counter, float *__restrict__ dose)
 {
-    __shared__ int s_cnt[kBlock / 64];
-    __shared__ int s_base;
     const uint32_t nvox = (uint32_t)g.nx * g.ny * g.nz;
-    const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
-    bool act = false;
-    if (v < nvox) {
+    dh_block_compact<kCompactRounds>(blockIdx.x * (uint32_t)(kBlock * kCompactRounds), nvox, [&](uint32_t v) {
         const int i = (int)(v % (uint32_t)g.nx);
         const uint32_t r = v / (uint32_t)g.nx;
         const int j = (int)(r % (uint32_t)g.ny);
         const int k = (int)(r / (uint32_t)g.ny);
         const float T0 = center_terma<Quad>(g, p, tr, i, j, k);
-        act = !(T0 <= 0.01f);
+        const bool act = !(T0 <= 0.01f);
         if (!act) dose[v] = 0.0f;
-    }
-    const uint64_t m = __builtin_amdgcn_ballot_w64(act);
-    const int lane = __lane_id();
-    const int wave = threadIdx.x >> 6;
-    if (lane == 0) s_cnt[wave] = __popcll(m);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int tot = 0;
-        for (int w = 0; w < kBlock / 64; w++) { int c = s_cnt[w]; s_cnt[w] = tot; tot += c; }
-        s_base = tot ? atomicAdd(counter, tot) : 0;
-    }
-    __syncthreads();
-    if (act) {
-        const uint64_t below = m & ((1ull << lane) - 1ull);
-        list[s_base + s_cnt[wave] + __popcll(below)] = (int)v;
-    }
+        return act;
+    }, list, counter);
 }
 
 // Collapsed-cone superposition (IMRTClasses.cu:252-350) for the listed voxels:
@@ -396,10 +380,10 @@ void dh_launch_ccc(const dh_ccc_args &a, const float *terma, const float *densit
     {
         dh_timed t("ccc_compact", s);
         if (layout)
-            hipLaunchKernelGGL(k_ccc_compact<true>, dim3(grid_for(n)), dim3(kBlock), 0, s, g, a.p,
+            hipLaunchKernelGGL(k_ccc_compact<true>, dim3(grid_compact(n)), dim3(kBlock), 0, s, g, a.p,
                                (const void *)tr_pad, active, counter, dose);
         else
-            hipLaunchKernelGGL(k_ccc_compact<false>, dim3(grid_for(n)), dim3(kBlock), 0, s, g, a.p,
+            hipLaunchKernelGGL(k_ccc_compact<false>, dim3(grid_compact(n)), dim3(kBlock), 0, s, g, a.p,
                                (const void *)tr_pad, active, counter, dose);
         DH_HIP(hipGetLastError());
     }

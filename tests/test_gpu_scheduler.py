@@ -151,14 +151,14 @@ def test_scheduler_surrogate_photon(photon_model):
         ref = t.zeros_like(rho)
         for ga, fs in beams:
             mlc = square_field_mlc(pm, fs, fs)
-            sched.add_photon(cid, iso, 100.0, ga, 0.0, 0.0, pm, mlc, surrogate=native, stride=4, tol=0.02)
+            sched.add_photon(cid, iso, 100.0, ga, 0.0, 0.0, pm, mlc, surrogate=native, stride=4, tol=0.02, dilate=1)
             one = t.empty_like(rho)
             keep: list = []
             m = engine.photon_args(pm, keep)
             rows = np.ascontiguousarray(np.asarray(mlc, dtype=np.float32)[:, :4])
             _lib.check(_lib.load().dh_photon_dose_surrogate_dev(
                 0, native.handle, _lib.grid(rho.shape, s, np.zeros(3)), _lib.photon_cp(iso, 100.0, ga, 0.0, 0.0),
-                m, _lib.ptr(rows), rows.shape[0], _lib.ptr(rho), _lib.ptr(one), None, None, 4, 0.02, 2, S._st()))
+                m, _lib.ptr(rows), rows.shape[0], _lib.ptr(rho), _lib.ptr(one), None, None, 4, 0.02, 1, S._st()))
             ref += one
         doses.append(d)
         refs.append(ref)

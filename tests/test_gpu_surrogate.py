@@ -412,5 +412,5 @@ def test_native_dose_equals_torch_path(photon_model):
     rows = np.ascontiguousarray(np.asarray(mlc, dtype=np.float32)[:, :4])
     _lib.check(_lib.load().dh_photon_dose_surrogate_dev(
         0, sur.native.handle, _lib.grid(shape, s, np.zeros(3)), _lib.photon_cp(iso, 100.0, 120.0, 0.0, 0.0), m,
-        _lib.ptr(rows), rows.shape[0], _lib.ptr(rho), _lib.ptr(d), None, None, 4, 0.02, 2, S._st()))
+        _lib.ptr(rows), rows.shape[0], _lib.ptr(rho), _lib.ptr(d), None, None, 4, 0.02, sur.dilate, S._st()))
     assert t.equal(d, a)
