@@ -30,6 +30,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per k_ccc launch on the 128^3 plan (mean over the 7 beams) from
+# `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE` runs of this bench,
+# scaled by a 256 MiB copy calibration (factors 2.0 and 1.0); the commands and
+# per-kernel figures are in profiles/hbm_traffic_cccs128_7beams.md
+CCC_TRAFFIC_BYTES = (2.0 * 99099 + 4554) * 1024
 
 
 def parse():
@@ -328,7 +333,8 @@ def main():
                                                 peak=l1_peak, unit="TB/s", frac=l1_ach / l1_peak),
                            kernels_ms={k: v[0] / max(v[1], 1) for k, v in kt.items()},
                            roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS,
-                                         unit="GB/s", frac=achieved / HBM_PEAK_GBS, traffic=None))
+                                         unit="GB/s", frac=achieved / HBM_PEAK_GBS,
+                                         traffic=CCC_TRAFFIC_BYTES))
         log(f"CCCS: {ms / nb:.3f} ms/beam (ccc kernel {ccc_ms:.3f} ms, {n_act} active voxels)")
         if not a.no_cpu:
             v, sample = cpu_baseline_cccs(st, a.cpu_seconds)

@@ -17,8 +17,20 @@ constexpr int kCompactRounds = 8;     // voxels per thread of the compaction
 inline unsigned grid_for(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 inline unsigned grid_compact(size_t n) { return (unsigned)((n + kBlock * kCompactRounds - 1) / (kBlock * kCompactRounds)); }
 
+// erff(x) is exactly +1 for x >= 4.5 and -1 for x <= -4.5 in FP32
+// (erfc(4.5) = 2e-10 is far below half an ulp of 1)
+constexpr float kErfSaturated = 4.5f;
+
+__device__ static inline bool erf_pair_zero(float a, float b)
+{
+    return (a >= kErfSaturated && b >= kErfSaturated) || (a <= -kErfSaturated && b <= -kErfSaturated);
+}
+
 // Source-occlusion head transmission (IMRTClasses.cu:120-150) over the open
 // leaf pairs; the scaled leaf edges {xL, xR, yB, yT} are precomputed per pair.
+// A pair whose two y (or two x) edges lie on the same saturated side of erf
+// contributes exactly 0 and is skipped: the sum is the same bit for bit, and
+// a wave whose voxels all skip a pair skips its four erff evaluations.
 __device__ static inline float head_transmission(const dh_terma_args &a, float hx, float hy,
                                                  float hz, float Z, float inv)
 {
@@ -26,12 +38,16 @@ __device__ static inline float head_transmission(const dh_terma_args &a, float h
     float tr = 0.0f;
     for (int q = 0; q < a.n_pairs; q++) {
         const float4 m = a.mlc[q];
-        const float t1 = ((m.x - hx) * dv) + hx;
-        const float t2 = ((m.y - hx) * dv) + hx;
         const float e1 = ((m.z - hy) * dv) + hy;
         const float e2 = ((m.w - hy) * dv) + hy;
-        const float ex = 0.5f * (erff(t2 * inv) - erff(t1 * inv));
-        const float ey = 0.5f * (erff(e2 * inv) - erff(e1 * inv));
+        const float y1 = e1 * inv, y2 = e2 * inv;
+        if (erf_pair_zero(y1, y2)) continue;
+        const float t1 = ((m.x - hx) * dv) + hx;
+        const float t2 = ((m.y - hx) * dv) + hx;
+        const float x1 = t1 * inv, x2 = t2 * inv;
+        if (erf_pair_zero(x1, x2)) continue;
+        const float ex = 0.5f * (erff(x2) - erff(x1));
+        const float ey = 0.5f * (erff(y2) - erff(y1));
         tr += ex * ey;
     }
     return tr;
