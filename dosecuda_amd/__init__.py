@@ -5,6 +5,9 @@ the call surface of DoseCUDA's ``dose_kernels`` extension.
   proton_spot_cuda, photon_dose_cuda) over libdosehip.so's C ABI;
 * ``dosecuda_amd.engine``        device-pointer API (torch tensors, streams);
 * ``dosecuda_amd.beam_models``   loaders for the unchanged lookup tables;
+* ``dosecuda_amd.plan``          dose grids, plans and beams with the reference's
+  class names, computing plan sums on the device; ``dosecuda_amd.nrrd``
+  reads and writes the NRRD exports;
 * ``dosecuda_amd.phantoms``      seeded synthetic phantoms and fields.
 """
 __version__ = "0.1.0"

@@ -189,13 +189,29 @@ def photon_args(model, keep):
     return _lib.photon_model(vals, arrs, model.kernel, keep)
 
 
+class PhotonParams:
+    """The C-ABI struct of one photon beam model, built once and reused for
+    every control point of a beam.  With ``validate`` the attributes are read
+    with the drop-in's checks (dosemodule.cu:386-437), so a malformed model
+    raises the same error as ``dose_kernels.photon_dose_cuda``."""
+
+    def __init__(self, model, validate=True):
+        self._keep: list = []
+        if validate:
+            from .dose_kernels import read_photon_model
+            self.struct = read_photon_model(model, self._keep)
+        else:
+            self.struct = photon_args(model, self._keep)
+
+
 def photon_dose(density: torch.Tensor, spacing, origin, iso, mu, ga, ca, ta, model, mlc,
                 out=None, terma_out=None, electron_out=None, wet_out=None, stream=None):
-    """Ray trace + TERMA + CCCS for one control point."""
+    """Ray trace + TERMA + CCCS for one control point.  ``model`` is a beam
+    model object or a PhotonParams built from one."""
     dev = _dev(density)
     out = torch.empty_like(density) if out is None else out
     keep: list = []
-    m = photon_args(model, keep)
+    m = model.struct if isinstance(model, PhotonParams) else photon_args(model, keep)
     rows = np.ascontiguousarray(np.asarray(mlc, dtype=np.float32)[:, :4])
     _lib.check(_lib.load().dh_photon_dose_dev(
         dev, _g(density.shape, spacing, origin), _lib.photon_cp(iso, mu, ga, ca, ta), m,
