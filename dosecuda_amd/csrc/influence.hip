@@ -78,8 +78,10 @@ __global__ void __launch_bounds__(64)
 k_influence(dh_infl_args a)
 {
     __shared__ int s_item;
+    __shared__ int s_rlo[64], s_rex[64], s_rid[64];   // packed rows of a batch
     const dh_geom &g = a.g;
     const int lane = __lane_id();
+    const uint64_t below = (1ull << lane) - 1ull;
     for (;;) {
         if (lane == 0) s_item = atomicAdd(a.queue, 1);
         __syncthreads();
@@ -103,11 +105,13 @@ k_influence(dh_infl_args a)
         }
         float pk[S];
         int cnt[S];
-        int64_t cur[S];
+        int64_t cur64[S];
+        float lth[S];
 #pragma unroll
         for (int s = 0; s < S; s++) {
             pk[s] = 0.0f; cnt[s] = 0;
-            cur[s] = (MODE == INF_FILL && sid[s] >= 0) ? a.offsets[(size_t)sid[s] * a.C + chunk] : 0;
+            cur64[s] = (MODE == INF_FILL && sid[s] >= 0) ? a.offsets[(size_t)sid[s] * a.C + chunk] : 0;
+            lth[s] = log2f(th[s]) - 1.01f;
         }
         // slot end of each (spot, chunk): no write can leave its slot even if a
         // fill decision disagreed with the count pass
@@ -147,18 +151,47 @@ k_influence(dh_infl_args a)
                     my_lo = ilo; my_hi = ihi;
                 }
             }
-            uint64_t rows_left = __builtin_amdgcn_ballot_w64(my_lo <= my_hi);
-            while (rows_left) {
-                const int r = __ffsll((unsigned long long)rows_left) - 1;
-                rows_left &= rows_left - 1;
-                const int ilo = __shfl(my_lo, r, 64), ihi = __shfl(my_hi, r, 64);
-                const int64_t rsel = rb + r;
-                const int k = B.k0 + (int)(rsel / nj), j = B.j0 + (int)(rsel % nj);
-                st_rows++;
-            for (int i0 = ilo; i0 <= ihi; i0 += 64) {
-                const int i = i0 + lane;
-                const bool inb = i <= ihi;
-                const uint32_t v = (uint32_t)min(i, ihi) + (uint32_t)g.nx * ((uint32_t)j + (uint32_t)g.ny * (uint32_t)k);
+            // Pack the batch's non-empty row intervals end to end so that every
+            // 64-voxel chunk fills all lanes: compacted row q starts at flat
+            // index s_rex[q]; the order (rows ascending, i ascending) keeps each
+            // spot's columns ascending.
+            const bool ne = my_lo <= my_hi;
+            const uint64_t nem = __builtin_amdgcn_ballot_w64(ne);
+            if (!nem) continue;
+            const int len = ne ? my_hi - my_lo + 1 : 0;
+            int incl = len;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const int t = __shfl_up(incl, off, 64);
+                if (lane >= off) incl += t;
+            }
+            const int total = __shfl(incl, 63, 64);
+            const int nrow = __popcll(nem);
+            if (ne) {
+                const int q = __popcll(nem & below);
+                s_rlo[q] = my_lo;
+                s_rex[q] = incl - len;
+                s_rid[q] = lane;
+            }
+            __syncthreads();
+            st_rows += nrow;
+            int cur = 0;                 // compacted row of the chunk's first voxel
+            for (int f0 = 0; f0 < total; f0 += 64) {
+                const int f = min(f0 + lane, total - 1);
+                const bool inb = f0 + lane < total;
+                // rows starting inside this chunk (a uniform walk; usually 1-2)
+                int q = cur, my_q = cur;
+                for (int nq = cur + 1; nq < nrow; nq++) {
+                    const int e = s_rex[nq];
+                    if (e > f0 + 63) break;
+                    my_q = f >= e ? nq : my_q;
+                    q = nq;
+                }
+                cur = q;
+                const int i = s_rlo[my_q] + (f - s_rex[my_q]);
+                const int rglob = (int)rb + s_rid[my_q];
+                const int k = B.k0 + rglob / nj, j = B.j0 + rglob % nj;
+                const uint32_t v = (uint32_t)i + (uint32_t)g.nx * ((uint32_t)j + (uint32_t)g.ny * (uint32_t)k);
                 const float w = a.wet[v] * 10.0f;
                 const bool act = inb && !(w > L.cut);
                 st_chunk++;
@@ -166,7 +199,7 @@ k_influence(dh_infl_args a)
                 st_act += __popcll(__builtin_amdgcn_ballot_w64(act));
                 if (!dh_wave_any(act)) continue;
                 float px, py, pz, hx, hy, hz;
-                dh_vox_xyz(g, min(i, ihi), j, k, px, py, pz);
+                dh_vox_xyz(g, i, j, k, px, py, pz);
                 dh_img_to_head(g, px, py, pz, hx, hy, hz);
                 // factors of this (voxel, layer), written by the bounds pass
                 const float4 fc = a.factors[(size_t)B.layer * g.nx * g.ny * g.nz + v];
@@ -177,27 +210,33 @@ k_influence(dh_infl_args a)
                     const float r2 = fmaf(ux, ux, uy * uy);
                     const float dt = fmaf(tx[s], ux, ty[s] * uy);
                     const float d2 = fmaf(-dt, dt, r2);
-                    const float val = __builtin_amdgcn_exp2f(fmaf(fc.x, d2, fc.y)) +
-                                      __builtin_amdgcn_exp2f(fmaf(fc.z, d2, fc.w));
+                    const float ep = fmaf(fc.x, d2, fc.y), eh = fmaf(fc.z, d2, fc.w);
                     if (MODE == INF_PEAK) {
+                        const float val = __builtin_amdgcn_exp2f(ep) + __builtin_amdgcn_exp2f(eh);
                         pk[s] = act ? fmaxf(pk[s], val) : pk[s];
                     } else {
+                        // where both exponents are below log2(th) - 1.01 both
+                        // terms are below th / 2 (v_exp_f32 errs by ~1 ulp),
+                        // so the value is below th: a chunk with no other lane
+                        // needs no exponentials and contributes nothing
+                        if (!dh_wave_any(act && fmaxf(ep, eh) >= lth[s])) continue;
+                        const float val = __builtin_amdgcn_exp2f(ep) + __builtin_amdgcn_exp2f(eh);
                         const bool keep = act && val >= th[s];
                         const uint64_t m = __builtin_amdgcn_ballot_w64(keep);
                         if (MODE == INF_COUNT) {
                             cnt[s] += __popcll(m);
                         } else {
-                            const int64_t pos = cur[s] + __popcll(m & ((1ull << lane) - 1ull));
+                            const int64_t pos = cur64[s] + __popcll(m & below);
                             if (keep && pos < lim[s]) {
                                 a.col[pos] = v;
                                 a.val[pos] = val;
                             }
-                            cur[s] += __popcll(m);
+                            cur64[s] += __popcll(m);
                         }
                     }
                 }
             }
-            }
+            __syncthreads();
         }
         if (a.stats && lane == 0) {
             atomicAdd(&a.stats[0], st_visit);
@@ -217,6 +256,52 @@ k_influence(dh_infl_args a)
             }
         }
     }
+}
+
+// Lower bound of each spot's peak: one wave per spot walks the spot axis
+// through the volume in steps of half a voxel and evaluates, with the sweep's
+// own formula, the voxel nearest to each axis point (active voxels only).
+__global__ void __launch_bounds__(64)
+k_axis_peak(dh_geom g, float vsadx, float vsady, int n_spots, const int *__restrict__ spot_layer,
+            const dh_layer_k *__restrict__ layers, const float *__restrict__ spots,
+            const float *__restrict__ wet, const float4 *__restrict__ factors, float hz0, float hz1,
+            unsigned *__restrict__ peak_bits)
+{
+    const int q = blockIdx.x;
+    const int lane = __lane_id();
+    if (q >= n_spots) return;
+    const int l = spot_layer[q];
+    if (l < 0) return;
+    const float sx = spots[8 * q], sy = spots[8 * q + 1], tx = spots[8 * q + 2], ty = spots[8 * q + 3];
+    const float cut = layers[l].cut;
+    const size_t nvox = (size_t)g.nx * g.ny * g.nz;
+    const float step = 0.5f * g.s;
+    const int nsteps = (int)ceilf((hz1 - hz0) / step) + 1;
+    float m = 0.0f;
+    for (int t = lane; t < nsteps; t += 64) {
+        const float hza = hz0 + (float)t * step;
+        float ax, ay, az;
+        dh_head_to_img(g, sx * (1.0f - hza / vsadx), sy * (1.0f - hza / vsady), hza, ax, ay, az);
+        const int i = (int)rintf((ax + g.iso[0]) * g.inv_s);
+        const int j = (int)rintf((ay + g.iso[1]) * g.inv_s);
+        const int k = (int)rintf((az + g.iso[2]) * g.inv_s);
+        if (i < 0 || j < 0 || k < 0 || i >= g.nx || j >= g.ny || k >= g.nz) continue;
+        const uint32_t v = (uint32_t)i + (uint32_t)g.nx * ((uint32_t)j + (uint32_t)g.ny * (uint32_t)k);
+        if (wet[v] * 10.0f > cut) continue;
+        float px, py, pz, hx, hy, hz;
+        dh_vox_xyz(g, i, j, k, px, py, pz);
+        dh_img_to_head(g, px, py, pz, hx, hy, hz);
+        const float4 fc = factors[(size_t)l * nvox + v];
+        const float cx = 1.0f - hz / vsadx, cy = 1.0f - hz / vsady;
+        const float ux = fmaf(-sx, cx, hx), uy = fmaf(-sy, cy, hy);
+        const float r2 = fmaf(ux, ux, uy * uy);
+        const float dt = fmaf(tx, ux, ty * uy);
+        const float d2 = fmaf(-dt, dt, r2);
+        const float ep = fmaf(fc.x, d2, fc.y), eh = fmaf(fc.z, d2, fc.w);
+        m = fmaxf(m, __builtin_amdgcn_exp2f(ep) + __builtin_amdgcn_exp2f(eh));
+    }
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+    if (lane == 0 && m > 0.0f) atomicMax(&peak_bits[q], __float_as_uint(m));
 }
 
 // Per layer and image row (k, j): first and last i with 10 * WET <= cut.
@@ -363,6 +448,7 @@ struct dh_influence {
     int64_t *d_offsets = nullptr;
     int *d_queue = nullptr;
     int2 *d_active = nullptr;
+    int *d_spot_layer = nullptr;
     unsigned long long *d_stats = nullptr;   // 4 counters per pass
     float4 *d_factors = nullptr;             // [layer][voxel]
     std::vector<float> peak;         // host copy after the peak pass
@@ -489,6 +575,9 @@ dh_influence *dh_influence_make(int dev, const dh_geom &g, float vsadx, float vs
         DH_HIP(hipMalloc(&f->d_factors, sizeof(float4) * std::max<size_t>(layers.size(), 1) *
                                             (size_t)g.nx * g.ny * g.nz));
         DH_HIP(hipMalloc(&f->d_active, sizeof(int2) * std::max<size_t>(layers.size(), 1) * g.ny * g.nz));
+        DH_HIP(hipMalloc(&f->d_spot_layer, sizeof(int) * std::max(n_spots, 1)));
+        if (n_spots)
+            DH_HIP(hipMemcpy(f->d_spot_layer, f->spot_layer.data(), sizeof(int) * n_spots, hipMemcpyHostToDevice));
     } catch (...) {
         void dh_influence_free(dh_influence *);
         dh_influence_free(f);
@@ -503,7 +592,7 @@ void dh_influence_free(dh_influence *f)
     (void)hipSetDevice(f->dev);
     void *ps[] = {f->d_layers, f->d_luts, f->d_spots, f->d_block_spots, f->d_blocks, f->d_thr,
                   f->d_peak, f->d_counts, f->d_offsets, f->d_queue, f->d_active, f->d_stats,
-                  f->d_factors};
+                  f->d_factors, f->d_spot_layer};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     delete f;
@@ -644,19 +733,42 @@ int64_t dh_influence_count(dh_influence *f, const float *d_wet, int64_t base, hi
     active_rows(f, d_wet, s);
     wet_bounds(f, d_wet, s);
     hipLaunchKernelGGL(k_zero_words, dim3(64), dim3(256), 0, s, f->d_peak, (size_t)std::max(n, 1));
-    // Peak pass radius.  At lateral distance d every voxel's dose is at most
-    // Pmax e^(-d^2 / 2 st_max^2) + Hmax e^(-d^2 / 2 sht_max^2); a voxel within
-    // one voxel diagonal of the axis at the depth of the layer's largest P + H
-    // bounds the peak from below by (P + H) e^(-dg^2 / 2 st^2), so the peak
-    // is found within the radius where the upper bound drops below half of
-    // Pmax (3 sht_max is a generous version of that).
+    // Peak in two steps.  k_axis_peak evaluates the voxels nearest to each
+    // spot axis; their maximum pk_a is a value of the row, so a lower bound of
+    // the peak.  At lateral distance d every voxel's dose is
+    // at most Pmax e^(-d^2 / 2 st_max^2) + Hmax e^(-d^2 / 2 sht_max^2) (the
+    // layer's maxima), so beyond the radius where both terms are below pk_a / 2
+    // no voxel can exceed pk_a: the sweep covers that radius only, and the
+    // maximum of both is the exact peak.  A spot whose axis met no active
+    // voxel keeps the generous radius 3 sht_max + s.
     std::vector<float> r(n, 0.0f);
+    if (n) {
+        float hz0, hz1;
+        head_depth_range(f->g, hz0, hz1);
+        dh_timed t("influence_axis", s);
+        hipLaunchKernelGGL(k_axis_peak, dim3((unsigned)n), dim3(64), 0, s, f->g, f->vsadx, f->vsady, n,
+                           f->d_spot_layer, f->d_layers, f->d_spots, d_wet, f->d_factors, hz0, hz1, f->d_peak);
+        DH_HIP(hipGetLastError());
+    }
+    std::vector<unsigned> pk(std::max(n, 1));
+    DH_HIP(hipMemcpyAsync(pk.data(), f->d_peak, sizeof(unsigned) * n, hipMemcpyDeviceToHost, s));
+    DH_HIP(hipStreamSynchronize(s));
     for (int q = 0; q < n; q++) {
         const int l = f->spot_layer[q];
-        r[q] = l >= 0 ? 3.0f * f->bnd[4 * l + 1] + f->g.s : 0.0f;
+        if (l < 0) { r[q] = 0.0f; continue; }
+        float pa;
+        std::memcpy(&pa, &pk[q], 4);
+        const float st = f->bnd[4 * l], sht = f->bnd[4 * l + 1];
+        const float Pm = f->bnd[4 * l + 2], Hm = f->bnd[4 * l + 3];
+        if (!(pa > 0.0f)) {
+            r[q] = 3.0f * sht + f->g.s;
+            continue;
+        }
+        const float rp = Pm > 0.5f * pa ? st * std::sqrt(2.0f * std::log(2.0f * Pm / pa)) : 0.0f;
+        const float rh = Hm > 0.5f * pa ? sht * std::sqrt(2.0f * std::log(2.0f * Hm / pa)) : 0.0f;
+        r[q] = 1.001f * std::max(rp, rh) + f->g.s;
     }
     launch<INF_PEAK>(f, make_blocks(f, r), d_wet, nullptr, nullptr, s);
-    std::vector<unsigned> pk(std::max(n, 1));
     DH_HIP(hipMemcpyAsync(pk.data(), f->d_peak, sizeof(unsigned) * n, hipMemcpyDeviceToHost, s));
     DH_HIP(hipStreamSynchronize(s));
     f->peak.assign(n, 0.0f);
