@@ -107,18 +107,24 @@ def run_impt(torch, st):
     st["plan"].dose(st["wet"], out=st["dose"])
 
 
-def setup_influence(torch):
-    from dosecuda_amd import engine, phantoms
+def setup_influence(torch, tune=True):
+    from dosecuda_amd import autotune, engine, phantoms
     from dosecuda_amd.beam_models import ProtonBeamModel
     m = ProtonBeamModel()
     case = phantoms.spheres_case(seed=1003)
     rlsp = phantoms.to_rlsp(case)
     d = torch.from_numpy(rlsp).cuda()
     fields = phantoms.influence_fields(m, seed=1003)
-    st = dict(m=m, case=case, d_rlsp=d, fields=fields, wets=[], infl=[])
+    st = dict(m=m, case=case, d_rlsp=d, fields=fields, wets=[], infl=[], tuning=[])
     for g, spots in fields:
         w = engine.proton_wet(d, case.spacing, case.origin, case.iso, g, 0.0, m.VSADX, m.VSADY)
         st["wets"].append(w)
+        if tune:
+            # spots per workgroup and row chunks of this field (identical CSR
+            # required of every candidate); the matrix below reads the choice
+            t = autotune.tune_influence(w, rlsp.shape, case.spacing, case.origin, case.iso, g, 0.0,
+                                        m, spots, reps=2)
+            st["tuning"].append(t["best"])
         st["infl"].append(engine.InfluenceMatrix(0, rlsp.shape, case.spacing, case.origin, case.iso,
                                                  g, 0.0, m, spots))
     # size the shared CSR once
@@ -343,10 +349,13 @@ def main():
 
     if a.workload in ("all", "influence"):
         log("setting up the batched influence matrix: (160,256,256) @ 2 mm, 3 fields, ~15k spots")
-        st = setup_influence(torch)
+        st = setup_influence(torch, tune=not a.no_tune)
+        if st["tuning"]:
+            log(f"influence tuning (spots per workgroup, row chunks): {st['tuning']}")
         ms = timed(torch, run_influence, st, a.influence_steps, 1)
-        kt = {k: engine.prof_get(k) for k in ("raytrace", "smooth", "influence_rows", "influence_bounds", "influence_peak",
-                                              "influence_count", "influence_fill")}
+        kt = {k: engine.prof_get(k) for k in ("raytrace", "smooth", "influence_rows", "influence_bounds",
+                                              "influence_axis", "influence_peak", "influence_count",
+                                              "influence_fill")}
         fill_ms = kt["influence_fill"][0] / max(kt["influence_fill"][1], 1)
         nvox = st["d_rlsp"].numel()
         n_rows = sum(len(sp) for _, sp in st["fields"])
@@ -354,7 +363,7 @@ def main():
         # field's entries (u32 column + f32 value) and one read of its WET
         bytes_launch = (8 * st["nnz"] + 4 * nvox * len(st["fields"])) / len(st["fields"])
         ach = bytes_launch / (fill_ms * 1e-3) / 1e9
-        res["influence"] = dict(ms_per_build=ms, nnz=st["nnz"], rows=n_rows,
+        res["influence"] = dict(ms_per_build=ms, nnz=st["nnz"], rows=n_rows, tuning=st["tuning"],
                                 fill_ms=fill_ms, csr_bytes=8 * st["nnz"] + 8 * (n_rows + 1),
                                 kernels_ms={k: v[0] / max(v[1], 1) for k, v in kt.items()},
                                 roofline=dict(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS,

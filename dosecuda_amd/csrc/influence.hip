@@ -503,7 +503,7 @@ dh_influence *dh_influence_make(int dev, const dh_geom &g, float vsadx, float vs
                                 const std::vector<dh_layer_k> &layers, const std::vector<float4> &luts,
                                 const std::vector<float> &spots, int n_spots, int S, int C, float rel)
 {
-    DH_REQUIRE(S == 8 || S == 16, "spots per block must be 8 or 16");
+    DH_REQUIRE(S == 4 || S == 8 || S == 16, "spots per block must be 4, 8 or 16");
     DH_REQUIRE(C >= 1 && C <= 256, "row chunks per block must be 1..256");
     dh_influence *f = new dh_influence();
     f->dev = dev; f->g = g; f->vsadx = vsadx; f->vsady = vsady; f->S = S; f->C = C;
@@ -677,7 +677,8 @@ static void launch_kernels(dh_influence *f, int n_blocks, const float *d_wet, ui
     const unsigned grid = (unsigned)std::min(a.n_items, cus * 32);
     const char *name = MODE == INF_PEAK ? "influence_peak" : MODE == INF_COUNT ? "influence_count" : "influence_fill";
     dh_timed t(name, s);
-    if (f->S == 8) hipLaunchKernelGGL((k_influence<8, MODE>), dim3(grid), dim3(64), 0, s, a);
+    if (f->S == 4) hipLaunchKernelGGL((k_influence<4, MODE>), dim3(grid), dim3(64), 0, s, a);
+    else if (f->S == 8) hipLaunchKernelGGL((k_influence<8, MODE>), dim3(grid), dim3(64), 0, s, a);
     else hipLaunchKernelGGL((k_influence<16, MODE>), dim3(grid), dim3(64), 0, s, a);
     DH_HIP(hipGetLastError());
 }

@@ -144,7 +144,7 @@ int dh_photon_dose_dev(int gpu_id, const dh_grid *grid, const dh_photon_cp *cp,
  * order, stable; dh_influence_info gives the caller's spot row) with MU = 1;
  * columns are voxel indices i + nx*(j + ny*k) in ascending order; values are
  * the pencil-beam dose of that spot, kept where >= rel_threshold * row peak.
- * spots_per_block (8 or 16) and row_chunks (1..256) are tuning knobs.
+ * spots_per_block (4, 8 or 16) and row_chunks (1..256) are tuning knobs.
  * Usage: create -> count (synchronises; gives nnz and row_ptr, whose entries
  * start at `base` so fields can share one CSR) -> caller allocates col/val ->
  * fill (stream-ordered, with the same WET as count) -> destroy. */

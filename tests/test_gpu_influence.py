@@ -37,7 +37,7 @@ def _case(shape=(40, 48, 44), s=2.5, seed=41):
     return rho, iso, s
 
 
-@pytest.mark.parametrize("gantry, S, C", [(0.0, 8, 4), (120.0, 16, 3), (250.0, 8, 1)])
+@pytest.mark.parametrize("gantry, S, C", [(0.0, 8, 4), (120.0, 16, 3), (250.0, 8, 1), (60.0, 4, 2)])
 def test_influence_rows_match_oracle(proton_model, gantry, S, C):
     t = _torch()
     from dosecuda_amd import engine
