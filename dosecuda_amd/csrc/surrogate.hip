@@ -15,6 +15,7 @@
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 dh_bf16x8;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float dh_f32x16;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float dh_f32x4;
 
 namespace {
 
@@ -261,6 +262,163 @@ k_conv3d_lds(conv_args a)
             }
         }
     }
+}
+
+// 3x3x3 convolution with at most 16 output channels (the full-resolution
+// layers), on v_mfma_f32_16x16x32_bf16.  Operand layout: lane l holds A row
+// (l mod 16) and B column (l mod 16), each with the 8 consecutive K elements
+// 8*(l/16) .. 8*(l/16)+7; the result lane l holds column (l mod 16), rows
+// 4*(l/16) .. 4*(l/16)+3.  A K step is two taps x 16 channels (K element k:
+// tap 2p + k/16, channel k mod 16), so a 16-channel slice takes 14 MFMAs per
+// tile of 16 voxels (the 14th pairs tap 26 with a zero weight), with no
+// padded output columns.  A workgroup (4 waves) owns a 2 (z) x TY (y) x 32 (x)
+// brick; per slice it stages the input brick with its one-voxel halo in LDS
+// (two 8-channel planes, so 16 lanes read consecutive 16-byte chunks), each
+// lane loads its 14 weight fragments into registers, and each wave runs its
+// TY/2 rows x 2 tiles x 14 MFMAs.
+template <int TY>
+__global__ void __launch_bounds__(256)
+k_conv3d_c16(conv_args a)
+{
+    constexpr int TZ = 2, BY = TY + 2, BX = 34, NB = (TZ + 2) * BY * BX;
+    constexpr int RPW = TZ * TY / 4;                 // rows of 32 voxels per wave
+    __shared__ __attribute__((aligned(16))) uint16_t sa[2 * NB * 8];   // [half][voxel][8]
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int col = lane & 15, kq = lane >> 4;
+    const int half = kq & 1, tsub = kq >> 1;
+    const int ntx = (a.W + 31) / 32, nty = (a.H + TY - 1) / TY, ntz = (a.D + TZ - 1) / TZ;
+    int64_t bi = blockIdx.x;
+    const int tx = (int)(bi % ntx); bi /= ntx;
+    const int ty = (int)(bi % nty); bi /= nty;
+    const int tz = (int)(bi % ntz);
+    const int n = (int)(bi / ntz);
+    const int x0 = tx * 32, y0 = ty * TY, z0 = tz * TZ;
+    const int Da = a.D >> a.up, Ha = a.H >> a.up, Wa = a.W >> a.up;
+    const int nsl = a.cin >> 4;
+
+    dh_f32x4 acc[RPW][2];
+#pragma unroll
+    for (int m = 0; m < RPW; m++)
+#pragma unroll
+        for (int h2 = 0; h2 < 2; h2++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) acc[m][h2][r] = 0.0f;
+
+    for (int sl = 0; sl < nsl; sl++) {
+        const int c0 = sl * 16;
+        for (int q = tid; q < 2 * NB; q += 256) {
+            const int vb = q >> 1, hf = q & 1;
+            const int bx = vb % BX, r = vb / BX, by = r % BY, bz = r / BY;
+            const int gz = z0 - 1 + bz, gy = y0 - 1 + by, gx = x0 - 1 + bx;
+            uint4 val = make_uint4(0, 0, 0, 0);
+            if (gz >= 0 && gz < a.D && gy >= 0 && gy < a.H && gx >= 0 && gx < a.W) {
+                if (c0 < a.ca) {
+                    const int64_t src = (((int64_t)n * Da + (gz >> a.up)) * Ha + (gy >> a.up)) * Wa + (gx >> a.up);
+                    val = *(const uint4 *)(a.xa + src * a.ca + c0 + hf * 8);
+                } else {
+                    const int64_t src = (((int64_t)n * a.D + gz) * a.H + gy) * a.W + gx;
+                    val = *(const uint4 *)(a.xb + src * a.cb + (c0 - a.ca) + hf * 8);
+                }
+            }
+            *(uint4 *)(sa + (hf * NB + vb) * 8) = val;
+        }
+        dh_bf16x8 B[14];
+#pragma unroll
+        for (int p = 0; p < 14; p++) {
+            const int tp = 2 * p + tsub;
+            uint4 bv = make_uint4(0, 0, 0, 0);
+            if (tp < 27 && col < a.cout)
+                bv = *(const uint4 *)(a.w + ((int64_t)tp * a.cout_pad + col) * a.cin + c0 + half * 8);
+            __builtin_memcpy(&B[p], &bv, 16);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int p = 0; p < 14; p++) {
+            const int tp = min(2 * p + tsub, 26);
+            const int dz = tp / 9, dy = (tp / 3) % 3, dx = tp % 3;
+#pragma unroll
+            for (int m = 0; m < RPW; m++) {
+                const int rr = wid * RPW + m, rz = rr / TY, ry = rr % TY;
+#pragma unroll
+                for (int h2 = 0; h2 < 2; h2++) {
+                    const int vb = ((rz + dz) * BY + (ry + dy)) * BX + h2 * 16 + col + dx;
+                    const uint4 av = *(const uint4 *)(sa + (half * NB + vb) * 8);
+                    dh_bf16x8 A;
+                    __builtin_memcpy(&A, &av, 16);
+                    acc[m][h2] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[p], acc[m][h2], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (col >= a.cout) return;
+    const float bsv = a.bias ? a.bias[col] : 0.0f;
+#pragma unroll
+    for (int m = 0; m < RPW; m++) {
+        const int rr = wid * RPW + m;
+        const int gz = z0 + rr / TY, gy = y0 + rr % TY;
+        if (gz >= a.D || gy >= a.H) continue;
+#pragma unroll
+        for (int h2 = 0; h2 < 2; h2++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int gx = x0 + 16 * h2 + 4 * kq + r;
+                if (gx >= a.W) continue;
+                const int64_t ov = (((int64_t)n * a.D + gz) * a.H + gy) * a.W + gx;
+                float o = acc[m][h2][r] + bsv;
+                if (a.act) o = o > 0.0f ? o : o * a.slope;
+                if (a.y) a.y[ov * a.cout + col] = f2bf(o);
+                if (a.y32) a.y32[ov * a.cout + col] = o;
+            }
+    }
+}
+
+// 1x1x1 convolution with at most 16 output channels and one source (the
+// output layer), on v_mfma_f32_16x16x32_bf16 with the operand layout above:
+// a wave owns 64 consecutive voxels (4 tiles of 16), the K loop runs over
+// 32-channel steps (channels past cin are zero), every fragment is one
+// 16-byte load.
+__global__ void __launch_bounds__(256)
+k_conv1_c16(conv_args a)
+{
+    const int lane = threadIdx.x & 63, col = lane & 15, kq = lane >> 4;
+    const int64_t nvox = (int64_t)a.N * a.D * a.H * a.W;
+    const int64_t v0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
+    if (v0 >= nvox) return;
+    dh_f32x4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) acc[t][r] = 0.0f;
+    for (int c0 = 0; c0 < a.cin; c0 += 32) {
+        const int ci = c0 + 8 * kq;
+        uint4 bv = make_uint4(0, 0, 0, 0);
+        if (ci < a.cin && col < a.cout) bv = *(const uint4 *)(a.w + (int64_t)col * a.cin + ci);
+        dh_bf16x8 B;
+        __builtin_memcpy(&B, &bv, 16);
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int64_t v = v0 + 16 * t + col;
+            uint4 av = make_uint4(0, 0, 0, 0);
+            if (ci < a.cin && v < nvox) av = *(const uint4 *)(a.xa + v * a.cin + ci);
+            dh_bf16x8 A;
+            __builtin_memcpy(&A, &av, 16);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[t], 0, 0, 0);
+        }
+    }
+    if (col >= a.cout) return;
+    const float bsv = a.bias ? a.bias[col] : 0.0f;
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int64_t v = v0 + 16 * t + 4 * kq + r;
+            if (v >= nvox) continue;
+            float o = acc[t][r] + bsv;
+            if (a.act) o = o > 0.0f ? o : o * a.slope;
+            if (a.y) a.y[v * a.cout + col] = f2bf(o);
+            if (a.y32) a.y32[v * a.cout + col] = o;
+        }
 }
 
 // avg-pool 2x2x2 on channels-last bf16 [N][D][H][W][C] -> [N][D/2][H/2][W/2][C]
@@ -669,15 +827,34 @@ void launch_conv3d(const conv_args &a, int nco, int ksplit, hipStream_t s)
     if (nco == 0 && ksplit == 0) {
         // tuned variant: 1000 + nco for the brick kernel, else nco * 100 + ksplit
         const int code = dh_tune_lookup("conv3d", conv_key(a), -1);
-        if (code >= 1000) { nco = code - 1000; ksplit = -1; }
+        if (code >= 3000) { nco = code - 3000; ksplit = -3; }
+        else if (code >= 1000) { nco = code - 1000; ksplit = -1; }
         else if (code > 0) { nco = code / 100; ksplit = code % 100; }
     }
     int c = nco, k = ksplit;
+    dh_timed t("nn_conv3d", s);
+    // 16-channel brick kernel (ksplit = -3, nco = brick rows 4 or 8)
+    if (ksplit == -3 && c == 1) {
+        DH_REQUIRE(a.ksize == 1 && a.cout <= 16 && a.cb == 0 && a.up == 0,
+                   "the 16-channel 1x1 kernel needs k = 1, at most 16 outputs and one source");
+        const int64_t nvox = (int64_t)a.N * a.D * a.H * a.W;
+        hipLaunchKernelGGL(k_conv1_c16, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, s, a);
+        DH_HIP(hipGetLastError());
+        return;
+    }
+    if (ksplit == -3) {
+        DH_REQUIRE(a.ksize == 3 && a.cout <= 16, "the 16-channel brick kernel is 3x3x3 with at most 16 outputs");
+        DH_REQUIRE(c == 4 || c == 8, "16-channel brick kernel: rows must be 4 or 8");
+        const int64_t items = (int64_t)a.N * ((a.D + 1) / 2) * ((a.H + c - 1) / c) * ((a.W + 31) / 32);
+        if (c == 4) hipLaunchKernelGGL(k_conv3d_c16<4>, dim3((unsigned)items), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL(k_conv3d_c16<8>, dim3((unsigned)items), dim3(256), 0, s, a);
+        DH_HIP(hipGetLastError());
+        return;
+    }
     // brick kernel: explicit (ksplit = -1), or by default for 3x3x3 layers
     // over large volumes with at most 64 output channels
     const bool brick = ksplit < 0 || (nco <= 0 && ksplit == 0 && a.ksize == 3 && nvox >= 32768 &&
                                       a.cout_pad <= 64);
-    dh_timed t("nn_conv3d", s);
     if (brick) {
         DH_REQUIRE(a.ksize == 3, "the brick kernel is 3x3x3 only");
         if (c <= 0) c = a.cout_pad / 32;
@@ -1131,6 +1308,11 @@ extern "C" int dh_autotune_conv3d(int gpu_id, int32_t n, int32_t d, int32_t h, i
         if (ksize == 3)
             for (int c : {1, 2})
                 if (cout_pad % (32 * c) == 0) cand.push_back(1000 + c);
+        if (ksize == 3 && cout <= 16) {
+            cand.push_back(3004);
+            cand.push_back(3008);
+        }
+        if (ksize == 1 && cout <= 16 && cb == 0 && up == 0) cand.push_back(3001);
         for (int c : {1, 2, 4})
             for (int k : {1, 2, 4, 8, 16}) {
                 if ((cout_pad / 32) % c || c * k > 16 || (c == 2 && k == 16) || (c == 4 && k > 4)) continue;
@@ -1141,7 +1323,8 @@ extern "C" int dh_autotune_conv3d(int gpu_id, int32_t n, int32_t d, int32_t h, i
         float best_ms = 0.0f;
         for (size_t q = 0; q < cand.size() && tried < 16; q++) {
             const int code = cand[q];
-            const int c = code >= 1000 ? code - 1000 : code / 100, k = code >= 1000 ? -1 : code % 100;
+            const int c = code >= 3000 ? code - 3000 : code >= 1000 ? code - 1000 : code / 100;
+            const int k = code >= 3000 ? -3 : code >= 1000 ? -1 : code % 100;
             conv_args a = make_conv(xa, ca, up, xb, cb, n, d, h, w, wt, nullptr, cout, cout_pad, ksize, 0, 0.1f,
                                     nullptr, tried == 0 ? ref : out);
             const float t = dh_time_median([&] { launch_conv3d(a, c, k, s); }, reps, s);

@@ -100,6 +100,56 @@ def test_conv3d_variants(nco, ks):
     assert err < 1e-5, err
 
 
+@pytest.mark.parametrize("shape, cin, cout, rows", [
+    ((1, 9, 13, 40), 16, 16, 8),      # partial z, y and x bricks
+    ((2, 6, 8, 33), 48, 16, 4),       # batch of 2, three 16-channel slices
+    ((1, 7, 9, 20), 32, 9, 8),        # fewer than 16 outputs
+])
+def test_conv3d_c16(shape, cin, cout, rows):
+    """The 16-output-channel brick kernel (16x16x32 MFMAs, two taps per K
+    step) against the fp32 reference."""
+    t = _torch()
+    from dosecuda_amd import surrogate as S
+    g = t.Generator().manual_seed(31)
+    x = t.randn(shape + (cin,), generator=g).to(t.bfloat16).cuda()
+    w = (t.randn((27, cin, cout), generator=g) * 0.1).cuda()
+    b = (t.randn(cout, generator=g) * 0.1).cuda()
+    print(f"[case] conv3d c16 {shape} {cin}->{cout} rows={rows}", flush=True)
+    _, y32 = S.conv3d(x, S.prep_weights(w, 0), b, cout, 3, True, want_bf16=False, want_f32=True,
+                      nco=rows, ksplit=-3)
+    ref = _ref_conv(x, w, b, 3, True, S.SLOPE)
+    err = float((y32.cpu() - ref).abs().max() / ref.abs().max())
+    print(f"[parity] conv3d c16 rel err {err:.2e}", flush=True)
+    assert err < 1e-5, err
+    # fused [upsample(A), B] read in place
+    xa = t.randn((1, 4, 5, 10, 32), generator=g).to(t.bfloat16).cuda()
+    xb = t.randn((1, 8, 10, 20, 16), generator=g).to(t.bfloat16).cuda()
+    w2 = (t.randn((27, 48, 16), generator=g) * 0.1).cuda()
+    cat = t.cat([S._up(xa), xb], dim=-1).contiguous()
+    _, y2 = S.conv3d(xa, S.prep_weights(w2, 0), b[:16] if cout == 16 else None, 16, 3, False,
+                     want_bf16=False, want_f32=True, xb=xb, up=True, nco=rows, ksplit=-3)
+    ref2 = _ref_conv(cat, w2, b[:16] if cout == 16 else None, 3, False, S.SLOPE)
+    assert float((y2.cpu() - ref2).abs().max() / ref2.abs().max()) < 1e-5
+
+
+@pytest.mark.parametrize("cin, cout", [(16, 16), (48, 11)])
+def test_conv1_c16(cin, cout):
+    """The 16-output-channel 1x1 kernel against the fp32 reference."""
+    t = _torch()
+    from dosecuda_amd import surrogate as S
+    g = t.Generator().manual_seed(37)
+    shape = (1, 5, 7, 9)                 # 315 voxels: a partial last wave
+    x = t.randn(shape + (cin,), generator=g).to(t.bfloat16).cuda()
+    w = (t.randn((1, cin, cout), generator=g) * 0.1).cuda()
+    b = (t.randn(cout, generator=g) * 0.1).cuda()
+    print(f"[case] conv1 c16 {cin}->{cout}", flush=True)
+    _, y32 = S.conv3d(x, S.prep_weights(w, 0), b, cout, 1, False, want_bf16=False, want_f32=True,
+                      nco=1, ksplit=-3)
+    ref = _ref_conv(x, w, b, 1, False, S.SLOPE)
+    err = float((y32.cpu() - ref).abs().max() / ref.abs().max())
+    assert err < 1e-5, err
+
+
 def test_conv3d_fused_upsample_concat():
     """Reading [upsample(A), B] in place equals the materialised concatenation
     bit for bit (same kernel, same K order)."""
