@@ -126,8 +126,8 @@ def tune_pencil(plan: "engine.ProtonPlan", wet: torch.Tensor, reps=5, stream=Non
 
 def tune_conv3d(n, d, h, w, ca, cb, up, cout, ksize, reps=5, device=0, stream=None):
     best, nt = C.c_int32(), C.c_int32()
-    codes = np.zeros(16, np.int32)
-    ms = np.zeros(16, np.float32)
+    codes = np.zeros(24, np.int32)
+    ms = np.zeros(24, np.float32)
     _lib.check(_lib.load().dh_autotune_conv3d(device, n, d, h, w, ca, cb, 1 if up else 0, cout, ksize, int(reps),
                                               C.byref(best), _lib.ptr(codes), _lib.ptr(ms), C.byref(nt),
                                               engine._stream(stream)))

@@ -156,17 +156,17 @@ k_conv3d(conv_args a)
 }
 
 // 3x3x3 convolution for large volumes with few channels: a workgroup (4
-// waves) owns a 2 (z) x 4 (y) x 32 (x) brick of output voxels and 32 * NCO
-// output channels.  Per 16-channel input slice it stages the input brick
+// waves) owns a 2 (z) x TY (y) x 32 (x) brick of output voxels (TY = 4, or 2
+// to double the grid of mid-size volumes) and 32 * NCO output channels.  Per 16-channel input slice it stages the input brick
 // with its one-voxel halo (4 x 6 x 34 voxels) and the slice's weights of all
 // 27 taps in LDS, then each wave runs 27 taps x 2 rows x NCO MFMAs from LDS.
 // LDS layouts put the 8-channel halves of a fragment in separate planes so
 // the 32 lanes of a half read 16-byte chunks at consecutive addresses.
-template <int NCO>
+template <int NCO, int TY>
 __global__ void __launch_bounds__(256)
 k_conv3d_lds(conv_args a)
 {
-    constexpr int TZ = 2, TY = 4, BY = TY + 2, BX = 34, NB = (TZ + 2) * BY * BX, MR = 2;
+    constexpr int TZ = 2, BY = TY + 2, BX = 34, NB = (TZ + 2) * BY * BX, MR = TZ * TY / 4;
     constexpr int NCOL = 32 * NCO;
     __shared__ __attribute__((aligned(16))) uint16_t sa[2 * NB * 8];          // [half][voxel][8]
     __shared__ __attribute__((aligned(16))) uint16_t sb[27 * 2 * NCOL * 8];   // [tap][half][col][8]
@@ -825,9 +825,11 @@ void launch_conv3d(const conv_args &a, int nco, int ksplit, hipStream_t s)
     const int64_t nvox = (int64_t)a.N * a.D * a.H * a.W;
     const int ki = a.ksize * a.ksize * a.ksize * (a.cin / 16);
     if (nco == 0 && ksplit == 0) {
-        // tuned variant: 1000 + nco for the brick kernel, else nco * 100 + ksplit
+        // tuned variant: 1000 + nco for the brick kernel (1100 + nco: two-row
+        // bricks), 3000 + rows for the 16-channel kernels, else nco * 100 + ksplit
         const int code = dh_tune_lookup("conv3d", conv_key(a), -1);
         if (code >= 3000) { nco = code - 3000; ksplit = -3; }
+        else if (code >= 1100) { nco = code - 1100; ksplit = -2; }
         else if (code >= 1000) { nco = code - 1000; ksplit = -1; }
         else if (code > 0) { nco = code / 100; ksplit = code % 100; }
     }
@@ -853,18 +855,23 @@ void launch_conv3d(const conv_args &a, int nco, int ksplit, hipStream_t s)
     }
     // brick kernel: explicit (ksplit = -1), or by default for 3x3x3 layers
     // over large volumes with at most 64 output channels
-    const bool brick = ksplit < 0 || (nco <= 0 && ksplit == 0 && a.ksize == 3 && nvox >= 32768 &&
-                                      a.cout_pad <= 64);
+    const bool brick = ksplit == -1 || ksplit == -2 ||
+                       (nco <= 0 && ksplit == 0 && a.ksize == 3 && nvox >= 32768 && a.cout_pad <= 64);
     if (brick) {
         DH_REQUIRE(a.ksize == 3, "the brick kernel is 3x3x3 only");
         if (c <= 0) c = a.cout_pad / 32;
         DH_REQUIRE((c == 1 || c == 2) && a.cout_pad % (32 * c) == 0, "brick kernel: nco must be 1 or 2");
-        const int64_t items = (int64_t)a.N * ((a.D + 1) / 2) * ((a.H + 3) / 4) * ((a.W + 31) / 32) *
+        const int ty = ksplit == -2 ? 2 : 4;
+        const int64_t items = (int64_t)a.N * ((a.D + 1) / 2) * ((a.H + ty - 1) / ty) * ((a.W + 31) / 32) *
                               (a.cout_pad / (32 * c));
-        if (c == 1)
-            hipLaunchKernelGGL(k_conv3d_lds<1>, dim3((unsigned)items), dim3(256), 0, s, a);
+        if (c == 1 && ty == 4)
+            hipLaunchKernelGGL((k_conv3d_lds<1, 4>), dim3((unsigned)items), dim3(256), 0, s, a);
+        else if (c == 2 && ty == 4)
+            hipLaunchKernelGGL((k_conv3d_lds<2, 4>), dim3((unsigned)items), dim3(256), 0, s, a);
+        else if (c == 1)
+            hipLaunchKernelGGL((k_conv3d_lds<1, 2>), dim3((unsigned)items), dim3(256), 0, s, a);
         else
-            hipLaunchKernelGGL(k_conv3d_lds<2>, dim3((unsigned)items), dim3(256), 0, s, a);
+            hipLaunchKernelGGL((k_conv3d_lds<2, 2>), dim3((unsigned)items), dim3(256), 0, s, a);
         DH_HIP(hipGetLastError());
         return;
     }
@@ -1283,7 +1290,7 @@ __global__ void k_fill_bf16_hash(uint16_t *p, int64_t n, uint32_t seed, float sc
 // (output tiles, K split) pair and the brick kernel; a variant is admitted
 // when its fp32 output matches the first variant's to 1e-5 of the largest
 // magnitude (the variants differ only in summation order); the fastest is
-// stored for the shape.  codes[] / ms[] receive up to 16 tried variants.
+// stored for the shape.  codes[] / ms[] receive up to 24 tried variants.
 extern "C" int dh_autotune_conv3d(int gpu_id, int32_t n, int32_t d, int32_t h, int32_t w, int32_t ca, int32_t cb,
                                   int32_t up, int32_t cout, int32_t ksize, int32_t reps, int32_t *best_code,
                                   int32_t *codes, float *ms, int32_t *n_tried, void *stream)
@@ -1307,7 +1314,10 @@ extern "C" int dh_autotune_conv3d(int gpu_id, int32_t n, int32_t d, int32_t h, i
         std::vector<int> cand;
         if (ksize == 3)
             for (int c : {1, 2})
-                if (cout_pad % (32 * c) == 0) cand.push_back(1000 + c);
+                if (cout_pad % (32 * c) == 0) {
+                    cand.push_back(1000 + c);
+                    cand.push_back(1100 + c);
+                }
         if (ksize == 3 && cout <= 16) {
             cand.push_back(3004);
             cand.push_back(3008);
@@ -1321,10 +1331,10 @@ extern "C" int dh_autotune_conv3d(int gpu_id, int32_t n, int32_t d, int32_t h, i
         std::vector<float> host_ref, host_out;
         int best = -1, tried = 0;
         float best_ms = 0.0f;
-        for (size_t q = 0; q < cand.size() && tried < 16; q++) {
+        for (size_t q = 0; q < cand.size() && tried < 24; q++) {
             const int code = cand[q];
-            const int c = code >= 3000 ? code - 3000 : code >= 1000 ? code - 1000 : code / 100;
-            const int k = code >= 3000 ? -3 : code >= 1000 ? -1 : code % 100;
+            const int c = code >= 3000 ? code - 3000 : code >= 1100 ? code - 1100 : code >= 1000 ? code - 1000 : code / 100;
+            const int k = code >= 3000 ? -3 : code >= 1100 ? -2 : code >= 1000 ? -1 : code % 100;
             conv_args a = make_conv(xa, ca, up, xb, cb, n, d, h, w, wt, nullptr, cout, cout_pad, ksize, 0, 0.1f,
                                     nullptr, tried == 0 ? ref : out);
             const float t = dh_time_median([&] { launch_conv3d(a, c, k, s); }, reps, s);

@@ -202,7 +202,9 @@ int dh_stage_ccc_list_dev(int gpu_id, const dh_grid *grid, const dh_photon_cp *c
  * and LeakyReLU (act = 1); writes bf16 y and/or fp32 y32.  ca, cb multiples
  * of 16, cout_pad of 32.  nco (output tiles of 32 channels per wave: 1, 2, 4)
  * and ksplit (waves splitting K per workgroup: 1..16) pick the kernel
- * variant; 0, 0 lets the library choose. */
+ * variant; ksplit -1 / -2 select the LDS brick kernel (4- / 2-row bricks,
+ * nco 1 or 2), ksplit -3 the 16-output-channel kernels (nco = 4 or 8 brick
+ * rows for k = 3, nco = 1 for k = 1); 0, 0 lets the library choose. */
 int dh_nn_conv3d_dev(const void *xa, int32_t ca, int32_t up_a, const void *xb, int32_t cb,
                      int32_t n, int32_t d, int32_t h, int32_t w, const void *wt,
                      const float *bias, int32_t cout, int32_t cout_pad, int32_t ksize,
@@ -361,8 +363,11 @@ int dh_autotune_photon(int gpu_id, const dh_grid *grid, const dh_photon_cp *cp,
 int dh_autotune_pencil(dh_proton_plan *plan, const float *d_wet, int32_t reps, int32_t *best,
                        float *ms, void *stream);
 /* Convolution variant of one layer shape (codes: 1000 + nco for the LDS
- * brick kernel, nco * 100 + ksplit otherwise; a rejected variant's code is
- * reported negated); up to 16 variants. */
+ * brick kernel, 1100 + nco for its two-row bricks, 3004 / 3008 for the
+ * 16-output-channel brick kernel (4 / 8 rows), 3001 for the 16-channel 1x1
+ * kernel, nco * 100 + ksplit otherwise; a rejected variant's code is reported
+ * negated); up to 24 variants.  dh_nn_conv3d_dev selects the same kernels
+ * with (nco, ksplit) = (nco, -1), (nco, -2), (rows, -3) or (1, -3). */
 int dh_autotune_conv3d(int gpu_id, int32_t n, int32_t d, int32_t h, int32_t w, int32_t ca,
                        int32_t cb, int32_t up, int32_t cout, int32_t ksize, int32_t reps,
                        int32_t *best_code, int32_t *codes, float *ms, int32_t *n_tried, void *stream);

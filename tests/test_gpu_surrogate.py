@@ -82,7 +82,7 @@ def test_conv3d_backward():
 
 
 @pytest.mark.parametrize("nco, ks", [(1, 1), (1, 2), (1, 4), (1, 8), (1, 16), (2, 1), (2, 2), (2, 4),
-                                     (2, 8), (4, 1), (4, 2), (4, 4), (1, -1), (2, -1)])
+                                     (2, 8), (4, 1), (4, 2), (4, 4), (1, -1), (2, -1), (1, -2), (2, -2)])
 def test_conv3d_variants(nco, ks):
     """Every (output tiles, K split) instantiation against the fp32 reference."""
     t = _torch()
