@@ -386,6 +386,9 @@ static dh_terma_args make_terma_args(const dh_grid *grid, const dh_photon_cp *cp
                                  (r[2] + 0.5f * r[3]) * ms);
     }
     a.n_pairs = q;
+    a.sorted = 1;
+    for (int i = 1; i < q; i++)
+        if (!(a.mlc[i].z >= a.mlc[i - 1].z && a.mlc[i].w >= a.mlc[i - 1].w)) a.sorted = 0;
     a.sad = sad;
     a.ssd_s = (float)m->scatter_source_distance;
     a.inv_p = 1.0f / ((float)m->primary_source_size * sqrtf(2.f));     // IMRTClasses.cu:123-124

@@ -131,6 +131,7 @@ struct dh_layer_k {
 struct dh_terma_args {
     dh_geom g;
     int n_pairs, n_prof, n_spec;
+    int sorted;                      // open pairs' y edges non-decreasing in the pair index
     float sad, ssd_s, inv_p, inv_s_src, d_mlc, ssw, el_att, esw, el_norm_a, el_norm_b;
     float4 mlc[DH_MAX_PAIRS];        // scaled leaf edges {xL, xR, yB, yT} of open pairs
     float prof_r[DH_MAX_PROFILE], prof_i[DH_MAX_PROFILE], prof_s[DH_MAX_PROFILE];

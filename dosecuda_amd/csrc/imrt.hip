@@ -26,17 +26,44 @@ __device__ static inline bool erf_pair_zero(float a, float b)
     return (a >= kErfSaturated && b >= kErfSaturated) || (a <= -kErfSaturated && b <= -kErfSaturated);
 }
 
+// Projected, scaled y edge of leaf-pair edge ey for a voxel: the same FP32
+// sequence as the transmission sum, monotone in ey when dv > 0.
+__device__ static inline float pair_y(float ey, float hy, float dv, float inv)
+{
+    return (((ey - hy) * dv) + hy) * inv;
+}
+
 // Source-occlusion head transmission (IMRTClasses.cu:120-150) over the open
 // leaf pairs; the scaled leaf edges {xL, xR, yB, yT} are precomputed per pair.
 // A pair whose two y (or two x) edges lie on the same saturated side of erf
 // contributes exactly 0 and is skipped: the sum is the same bit for bit, and
 // a wave whose voxels all skip a pair skips its four erff evaluations.
+// When the pairs are sorted by y (both edges non-decreasing, as MLC banks
+// are) and dv > 0, the projected edges are monotone in the pair index, so the
+// pairs before the first with y2 > -4.5 and from the first with y1 >= 4.5 on
+// are all saturated on one side: two binary searches with the sum's own
+// arithmetic bound the window, and the loop inside it is unchanged.
 __device__ static inline float head_transmission(const dh_terma_args &a, float hx, float hy,
                                                  float hz, float Z, float inv)
 {
     const float dv = (Z - hz) / (a.d_mlc - hz);
+    int q0 = 0, q1 = a.n_pairs;
+    if (a.sorted && dv > 0.0f) {
+        int lo = 0, hi = a.n_pairs;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (pair_y(a.mlc[mid].w, hy, dv, inv) > -kErfSaturated) hi = mid; else lo = mid + 1;
+        }
+        q0 = lo;
+        hi = a.n_pairs;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (pair_y(a.mlc[mid].z, hy, dv, inv) >= kErfSaturated) hi = mid; else lo = mid + 1;
+        }
+        q1 = lo;
+    }
     float tr = 0.0f;
-    for (int q = 0; q < a.n_pairs; q++) {
+    for (int q = q0; q < q1; q++) {
         const float4 m = a.mlc[q];
         const float e1 = ((m.z - hy) * dv) + hy;
         const float e2 = ((m.w - hy) * dv) + hy;

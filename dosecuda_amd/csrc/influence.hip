@@ -46,6 +46,8 @@ struct dh_infl_args {
     const float *wet;
     unsigned *peak_bits;         // per spot (PEAK)
     int *counts;                 // per spot * C (COUNT)
+    int *spans;                  // per work item: first and last packed chunk with entries
+                                 // (COUNT writes, FILL reads; -1: none)
     const int64_t *offsets;      // per spot * C (FILL)
     const int2 *active;          // [layer][nz][ny] first/last i with w <= cut (lo > hi: none)
     const float4 *factors;       // [layer][voxel] {ap, log2 P, ah, log2 H} (P, H >= 0 folded)
@@ -113,6 +115,16 @@ k_influence(dh_infl_args a)
             cur64[s] = (MODE == INF_FILL && sid[s] >= 0) ? a.offsets[(size_t)sid[s] * a.C + chunk] : 0;
             lth[s] = log2f(th[s]) - 1.01f;
         }
+        // Packed chunks of an item come in the same order in COUNT and FILL
+        // (same blocks, rows and packing), so COUNT records the first and last
+        // one where any spot kept entries, and FILL skips the chunks before the
+        // first and ends the item after the last.
+        int span_first = 0x7fffffff, span_last = -1;
+        if (MODE == INF_FILL) {
+            const int2 sp = a.spans ? ((const int2 *)a.spans)[item] : make_int2(0, 0x7fffffff);
+            span_first = __builtin_amdgcn_readfirstlane(sp.x);
+            span_last = __builtin_amdgcn_readfirstlane(sp.y);
+        }
         // slot end of each (spot, chunk): no write can leave its slot even if a
         // fill decision disagreed with the count pass
         int64_t lim[S];
@@ -133,7 +145,9 @@ k_influence(dh_infl_args a)
         }
         const int2 *act_rows = a.active + (size_t)B.layer * g.nz * g.ny;
         unsigned long long st_visit = 0, st_act = 0, st_chunk = 0, st_rows = 0;
-        for (int64_t rb = r0; rb < r1; rb += 64) {
+        int pc = 0;                      // packed chunk index within the item
+        bool done = MODE == INF_FILL && span_first > span_last;
+        for (int64_t rb = r0; rb < r1 && !done; rb += 64) {
             // lane r prepares row rb + r: box interval intersected with the
             // layer's active interval on that row
             int my_lo = 1, my_hi = 0;
@@ -176,7 +190,11 @@ k_influence(dh_infl_args a)
             __syncthreads();
             st_rows += nrow;
             int cur = 0;                 // compacted row of the chunk's first voxel
-            for (int f0 = 0; f0 < total; f0 += 64) {
+            for (int f0 = 0; f0 < total; f0 += 64, pc++) {
+                if (MODE == INF_FILL && pc > span_last) {
+                    done = true;
+                    break;
+                }
                 const int f = min(f0 + lane, total - 1);
                 const bool inb = f0 + lane < total;
                 // rows starting inside this chunk (a uniform walk; usually 1-2)
@@ -188,6 +206,7 @@ k_influence(dh_infl_args a)
                     q = nq;
                 }
                 cur = q;
+                if (MODE == INF_FILL && pc < span_first) continue;
                 const int i = s_rlo[my_q] + (f - s_rex[my_q]);
                 const int rglob = (int)rb + s_rid[my_q];
                 const int k = B.k0 + rglob / nj, j = B.j0 + rglob % nj;
@@ -225,6 +244,10 @@ k_influence(dh_infl_args a)
                         const uint64_t m = __builtin_amdgcn_ballot_w64(keep);
                         if (MODE == INF_COUNT) {
                             cnt[s] += __popcll(m);
+                            if (m) {
+                                span_first = min(span_first, pc);
+                                span_last = pc;
+                            }
                         } else {
                             const int64_t pos = cur64[s] + __popcll(m & below);
                             if (keep && pos < lim[s]) {
@@ -238,6 +261,8 @@ k_influence(dh_infl_args a)
             }
             __syncthreads();
         }
+        if (MODE == INF_COUNT && lane == 0 && a.spans)
+            ((int2 *)a.spans)[item] = make_int2(span_first, span_last);
         if (a.stats && lane == 0) {
             atomicAdd(&a.stats[0], st_visit);
             atomicAdd(&a.stats[1], st_act);
@@ -445,6 +470,7 @@ struct dh_influence {
     float *d_thr = nullptr;
     unsigned *d_peak = nullptr;
     int *d_counts = nullptr;
+    int *d_spans = nullptr;
     int64_t *d_offsets = nullptr;
     int *d_queue = nullptr;
     int2 *d_active = nullptr;
@@ -569,6 +595,7 @@ dh_influence *dh_influence_make(int dev, const dh_geom &g, float vsadx, float vs
         DH_HIP(hipMalloc(&f->d_thr, sizeof(float) * std::max(n_spots, 1)));
         DH_HIP(hipMalloc(&f->d_peak, sizeof(unsigned) * std::max(n_spots, 1)));
         DH_HIP(hipMalloc(&f->d_counts, sizeof(int) * std::max<size_t>((size_t)n_spots * C, 1)));
+        DH_HIP(hipMalloc(&f->d_spans, sizeof(int2) * std::max<size_t>(nb * (size_t)C, 1)));
         DH_HIP(hipMalloc(&f->d_offsets, sizeof(int64_t) * ((size_t)n_spots * C + 1)));
         DH_HIP(hipMalloc(&f->d_queue, 256));
         DH_HIP(hipMalloc(&f->d_stats, sizeof(unsigned long long) * 12));
@@ -591,7 +618,7 @@ void dh_influence_free(dh_influence *f)
     if (!f) return;
     (void)hipSetDevice(f->dev);
     void *ps[] = {f->d_layers, f->d_luts, f->d_spots, f->d_block_spots, f->d_blocks, f->d_thr,
-                  f->d_peak, f->d_counts, f->d_offsets, f->d_queue, f->d_active, f->d_stats,
+                  f->d_peak, f->d_counts, f->d_spans, f->d_offsets, f->d_queue, f->d_active, f->d_stats,
                   f->d_factors, f->d_spot_layer};
     for (void *p : ps)
         if (p) (void)hipFree(p);
@@ -659,7 +686,7 @@ static void launch_kernels(dh_influence *f, int n_blocks, const float *d_wet, ui
     a.n_items = n_blocks * f->C; a.C = f->C;
     a.blocks = f->d_blocks; a.block_spots = f->d_block_spots; a.layers = f->d_layers;
     a.luts = f->d_luts; a.spots = f->d_spots; a.thr = f->d_thr; a.wet = d_wet;
-    a.peak_bits = f->d_peak; a.counts = f->d_counts; a.offsets = f->d_offsets;
+    a.peak_bits = f->d_peak; a.counts = f->d_counts; a.spans = f->d_spans; a.offsets = f->d_offsets;
     a.active = f->d_active;
     a.factors = f->d_factors;
     a.stats = f->d_stats + 4 * MODE;

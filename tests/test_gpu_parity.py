@@ -146,6 +146,31 @@ def test_terma_stage(photon_model, ga, ca, ta):
     _report("electron", eg.cpu().numpy(), er, 1e-5)
 
 
+@pytest.mark.parametrize("ga, ca", [(0.0, 0.0), (77.0, 25.0)])
+def test_terma_pair_window_is_exact(photon_model, ga, ca):
+    """The y-sorted leaf pairs are summed over a binary-searched window; a
+    far-away open pair placed first breaks the sort and forces the full scan.
+    It contributes exactly 0 everywhere, so both sums must agree bit for bit."""
+    t = _torch()
+    from dosecuda_amd import engine
+    from dosecuda_amd.beam_models import square_field_mlc
+    pm = photon_model
+    rho, iso = _imrt_case(pm)
+    s = 2.5
+    rng = np.random.default_rng(3)
+    mlc = square_field_mlc(pm, 8.0, 6.0).copy()
+    mlc[:, 0] += rng.uniform(-8, 8, mlc.shape[0]).astype(F)        # irregular leaf edges
+    mlc[:, 1] = np.maximum(mlc[:, 1] + rng.uniform(-8, 8, mlc.shape[0]).astype(F), mlc[:, 0])
+    far = np.array([[-5.0, 5.0, 1000.0, 5.0]], F)                   # open, 1 m off axis
+    wet = _hetero(rho.shape, 4, 0.0, 2.0)
+    print(f"[case] terma pair window ga={ga} ca={ca}", flush=True)
+    a, ea = engine.stage_terma(_cuda(t, wet), s, np.zeros(3), iso, 100.0, ga, ca, 0.0, pm, mlc)
+    b, eb = engine.stage_terma(_cuda(t, wet), s, np.zeros(3), iso, 100.0, ga, ca, 0.0, pm,
+                               np.vstack([far, mlc]))
+    assert float(a.max()) > 0.0
+    assert t.equal(a, b) and t.equal(ea, eb)
+
+
 @pytest.mark.parametrize("ga, ca, ta", [(0.0, 0.0, 0.0), (102.86, 30.0, 0.0), (308.57, 0.0, 20.0)])
 def test_ccc_stage(photon_model, ga, ca, ta):
     t = _torch()
