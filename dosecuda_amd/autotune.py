@@ -156,7 +156,7 @@ def tune_surrogate(net, cells, reps=5):
 
 
 def tune_influence(rlsp_wet: torch.Tensor, shape, spacing, origin, iso, gantry, couch, model, spots,
-                   candidates=((4, 16), (4, 32), (8, 8), (8, 16), (8, 32), (16, 16), (16, 32)), reps=3):
+                   candidates=((4, 32), (8, 16), (8, 32), (8, 64), (16, 16), (16, 32), (16, 64)), reps=3):
     """Spots per workgroup and row chunks of an influence matrix: builds the
     matrix with each pair (count + fill), requires identical CSR arrays and
     keeps the fastest fill under the key "nz,ny,nx,n_spots"."""

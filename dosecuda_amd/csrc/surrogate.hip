@@ -833,6 +833,14 @@ void launch_conv3d(const conv_args &a, int nco, int ksplit, hipStream_t s)
         else if (code >= 1000) { nco = code - 1000; ksplit = -1; }
         else if (code > 0) { nco = code / 100; ksplit = code % 100; }
     }
+    if (nco == 0 && ksplit == 0) {
+        // untuned shape: the variants the tuner picks for the surrogate's
+        // layers (16-output-channel kernels for <= 16 outputs; LDS bricks for
+        // large 3x3x3 layers, two-row bricks below 2^17 voxels)
+        if (a.cout <= 16 && a.ksize == 1 && a.cb == 0 && a.up == 0) { nco = 1; ksplit = -3; }
+        else if (a.cout <= 16 && a.ksize == 3 && nvox >= 32768) { nco = 8; ksplit = -3; }
+        else if (a.ksize == 3 && nvox >= 32768 && a.cout_pad <= 64) ksplit = nvox < 131072 ? -2 : -1;
+    }
     int c = nco, k = ksplit;
     dh_timed t("nn_conv3d", s);
     // 16-channel brick kernel (ksplit = -3, nco = brick rows 4 or 8)
@@ -853,10 +861,8 @@ void launch_conv3d(const conv_args &a, int nco, int ksplit, hipStream_t s)
         DH_HIP(hipGetLastError());
         return;
     }
-    // brick kernel: explicit (ksplit = -1), or by default for 3x3x3 layers
-    // over large volumes with at most 64 output channels
-    const bool brick = ksplit == -1 || ksplit == -2 ||
-                       (nco <= 0 && ksplit == 0 && a.ksize == 3 && nvox >= 32768 && a.cout_pad <= 64);
+    // brick kernel (ksplit = -1: four-row bricks, -2: two-row bricks)
+    const bool brick = ksplit == -1 || ksplit == -2;
     if (brick) {
         DH_REQUIRE(a.ksize == 3, "the brick kernel is 3x3x3 only");
         if (c <= 0) c = a.cout_pad / 32;

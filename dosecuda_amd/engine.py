@@ -105,11 +105,12 @@ class InfluenceMatrix:
                  spots_per_block=None, row_chunks=None, rel=1e-4):
         self._keep: list = []
         if spots_per_block is None or row_chunks is None:
-            # the autotuner's choice for this shape and spot count, else 16 / 16
+            # the autotuner's choice for this shape and spot count, else 16 / 32
+            # (what it picks for the benchmark's fields)
             ch = C.c_int32()
             key = f"{int(shape[0])},{int(shape[1])},{int(shape[2])},{len(spot_rows)}"
             _lib.check(_lib.load().dh_tune_get(b"influence", key.encode(), C.byref(ch)))
-            tS, tC = (ch.value // 1000, ch.value % 1000) if ch.value > 0 else (16, 16)
+            tS, tC = (ch.value // 1000, ch.value % 1000) if ch.value > 0 else (16, 32)
             spots_per_block = tS if spots_per_block is None else spots_per_block
             row_chunks = tC if row_chunks is None else row_chunks
         lut = _lib.proton_lut(model.divergence_params, model.lut_depths, model.lut_sigmas,
