@@ -469,11 +469,20 @@ static void sur_direction(const dh_photon_cp *cp, float dir[3])
     dir[2] = (float)(-sz / nrm);
 }
 
+// Which padded volume and radiological depth each (device, workspace) holds
+// from its last photon ray trace, so a batch can reuse them for the next
+// control point of the same case with the same source position.
+struct wet_memo {
+    const void *vpad = nullptr, *wet = nullptr;
+};
+static wet_memo g_wet_memo[64][DH_MAX_WORKSPACES];
+
 static void photon_device(int dev, const dh_grid *grid, const dh_photon_cp *cp,
                           const dh_photon_model *m, const float *mlc_rows, int32_t n_pairs,
                           const float *d_density, float *d_dose, float *d_terma,
                           float *d_electron, float *d_wet, hipStream_t s,
-                          const dh_sur_net *net = nullptr, int stride = 4, float tol = 0.01f, int dilate = 2)
+                          const dh_sur_net *net = nullptr, int stride = 4, float tol = 0.01f, int dilate = 2,
+                          bool reuse_wet = false)
 {
     size_t n = nvox_of(grid);
     DH_REQUIRE(cp && d_density && d_dose, "null argument");
@@ -487,8 +496,17 @@ static void photon_device(int dev, const dh_grid *grid, const dh_photon_cp *cp,
     void *trp = dh_scratch(dev, DH_SLOT_TR_PAD, dh_tr_pad_bytes(ca));
     int *act = (int *)dh_scratch(dev, DH_SLOT_ACTIVE, sizeof(int) * n);
     int *cnt = (int *)dh_scratch(dev, DH_SLOT_COUNTER, 256);
-    dh_launch_pad_quad(d_density, vpad, ca.g.nx, ca.g.ny, ca.g.nz, DH_AIR_DENSITY, s);  // IMRTClasses.cu:394
-    dh_launch_raytrace(ca.g, vpad, wet, s);
+    wet_memo &memo = g_wet_memo[dev & 63][dh_get_workspace()];
+    // The radiological depth depends on the density, the grid and the source
+    // position only (not on the collimator or the MLC): a caller that knows
+    // the previous control point on this workspace had the same ones may ask
+    // to reuse it; it is reused only if this workspace still holds it.
+    if (!(reuse_wet && !d_wet && memo.vpad == vpad && memo.wet == wet)) {
+        dh_launch_pad_quad(d_density, vpad, ca.g.nx, ca.g.ny, ca.g.nz, DH_AIR_DENSITY, s);  // IMRTClasses.cu:394
+        dh_launch_raytrace(ca.g, vpad, wet, s);
+    }
+    memo.vpad = d_wet ? nullptr : vpad;
+    memo.wet = d_wet ? nullptr : wet;
     dh_launch_terma(ta, wet, terma, el, s);
     if (net) {
         float dir[3];
@@ -656,10 +674,10 @@ extern "C" int dh_sur_dose_dev(const dh_sur_net *net, const dh_grid *grid, const
 // ---- enqueue-only entry points for the scheduler ---------------------------
 void dh_photon_enqueue(int dev, const dh_grid *grid, const dh_photon_cp *cp, const dh_photon_model *m,
                        const float *mlc_rows, int n_pairs, const float *d_density, float *d_dose,
-                       const dh_sur_net *net, int stride, float tol, int dilate, hipStream_t s)
+                       const dh_sur_net *net, int stride, float tol, int dilate, hipStream_t s, bool reuse_wet)
 {
     photon_device(dev, grid, cp, m, mlc_rows, n_pairs, d_density, d_dose, nullptr, nullptr, nullptr, s, net,
-                  stride, tol, dilate);
+                  stride, tol, dilate, reuse_wet);
 }
 
 void dh_proton_wet_enqueue(int dev, const dh_grid *grid, const dh_proton_beam *beam, const float *d_rlsp,

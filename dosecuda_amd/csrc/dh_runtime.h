@@ -202,7 +202,8 @@ struct dh_proton_plan;
 struct dh_influence;
 void dh_photon_enqueue(int dev, const dh_grid *grid, const dh_photon_cp *cp, const dh_photon_model *m,
                        const float *mlc_rows, int n_pairs, const float *d_density, float *d_dose,
-                       const dh_sur_net *net, int stride, float tol, int dilate, hipStream_t s);
+                       const dh_sur_net *net, int stride, float tol, int dilate, hipStream_t s,
+                       bool reuse_wet = false);
 void dh_proton_wet_enqueue(int dev, const dh_grid *grid, const dh_proton_beam *beam, const float *d_rlsp,
                            float *d_wet, hipStream_t s);
 void dh_proton_plan_enqueue(dh_proton_plan *p, const float *d_wet, float *d_dose, hipStream_t s);

@@ -150,7 +150,17 @@ void plan_streams(dh_sched *s)
     s->planned = true;
 }
 
-void enqueue_job(dh_sched *s, job &J, scase &C, hipStream_t st, bool timing)
+// Two photon control points with the same isocentre, gantry and couch angle
+// and source distance have the same radiological depth (it does not depend
+// on the collimator, the MLC or the MU).
+bool same_source(const job &a, const job &b)
+{
+    return a.kind == J_PHOTON && b.kind == J_PHOTON && a.cp.iso[0] == b.cp.iso[0] && a.cp.iso[1] == b.cp.iso[1] &&
+           a.cp.iso[2] == b.cp.iso[2] && a.cp.ga == b.cp.ga && a.cp.ta == b.cp.ta &&
+           a.model.m.primary_source_distance == b.model.m.primary_source_distance;
+}
+
+void enqueue_job(dh_sched *s, job &J, scase &C, hipStream_t st, bool timing, bool reuse_wet)
 {
     if (timing) DH_HIP(hipEventRecord(J.ev[0], st));
     const int64_t n = nvox(C.grid);
@@ -158,7 +168,7 @@ void enqueue_job(dh_sched *s, job &J, scase &C, hipStream_t st, bool timing)
     case J_PHOTON: {
         float *tmp = (float *)dh_scratch(s->gpu, DH_SLOT_OUT1, sizeof(float) * (size_t)n);
         dh_photon_enqueue(s->gpu, &C.grid, &J.cp, &J.model.m, J.mlc.data(), (int)(J.mlc.size() / 4), C.vol, tmp,
-                          J.net, J.stride, J.tol, J.dilate, st);
+                          J.net, J.stride, J.tol, J.dilate, st, reuse_wet);
         hipLaunchKernelGGL(k_sched_accum, dim3(grid_of(n)), dim3(256), 0, st, C.dose, (const float *)tmp, n);
         break;
     }
@@ -192,7 +202,14 @@ void enqueue_batch(dh_sched *s, bool timing)
                 const int64_t n = nvox(C.grid);
                 hipLaunchKernelGGL(k_sched_fill, dim3(grid_of(n)), dim3(256), 0, st, C.dose, n);
             }
-            for (int j : C.jobs) enqueue_job(s, *s->jobs[j], C, st, timing);
+            // a case's jobs run back to back on its stream and workspace, so a
+            // control point may reuse the ray trace of the one before it
+            const job *prev = nullptr;
+            for (int j : C.jobs) {
+                job &J = *s->jobs[j];
+                enqueue_job(s, J, C, st, timing, prev && same_source(*prev, J));
+                prev = &J;
+            }
         }
         for (size_t k = 0; k < s->streams.size(); k++) {
             DH_HIP(hipEventRecord(s->join[k], s->streams[k]));
