@@ -194,26 +194,47 @@ k_conv3d_lds(conv_args a)
 
     for (int sl = 0; sl < nsl; sl++) {
         const int c0 = sl * 16;
-        for (int q = tid; q < 2 * NB; q += 256) {
+        // brick loads first, then their LDS stores (latencies overlap)
+        constexpr int NQ = (2 * NB + 255) / 256;
+        uint4 stage[NQ];
+#pragma unroll
+        for (int u = 0; u < NQ; u++) {
+            const int q = tid + 256 * u;
             const int vb = q >> 1, hf = q & 1;
             const int bx = vb % BX, r = vb / BX, by = r % BY, bz = r / BY;
             const int gz = z0 - 1 + bz, gy = y0 - 1 + by, gx = x0 - 1 + bx;
-            uint4 val = make_uint4(0, 0, 0, 0);
-            if (gz >= 0 && gz < a.D && gy >= 0 && gy < a.H && gx >= 0 && gx < a.W) {
+            stage[u] = make_uint4(0, 0, 0, 0);
+            if (q < 2 * NB && gz >= 0 && gz < a.D && gy >= 0 && gy < a.H && gx >= 0 && gx < a.W) {
                 if (c0 < a.ca) {
                     const int64_t src = (((int64_t)n * Da + (gz >> a.up)) * Ha + (gy >> a.up)) * Wa + (gx >> a.up);
-                    val = *(const uint4 *)(a.xa + src * a.ca + c0 + hf * 8);
+                    stage[u] = *(const uint4 *)(a.xa + src * a.ca + c0 + hf * 8);
                 } else {
                     const int64_t src = (((int64_t)n * a.D + gz) * a.H + gy) * a.W + gx;
-                    val = *(const uint4 *)(a.xb + src * a.cb + (c0 - a.ca) + hf * 8);
+                    stage[u] = *(const uint4 *)(a.xb + src * a.cb + (c0 - a.ca) + hf * 8);
                 }
             }
-            *(uint4 *)(sa + (hf * NB + vb) * 8) = val;
         }
-        for (int q = tid; q < 27 * NCOL * 2; q += 256) {
-            const int hf = q & 1, r = q >> 1, col = r % NCOL, tp = r / NCOL;
-            const uint4 val = *(const uint4 *)(a.w + ((int64_t)tp * a.cout_pad + cob * NCOL + col) * a.cin + c0 + hf * 8);
-            *(uint4 *)(sb + ((tp * 2 + hf) * NCOL + col) * 8) = val;
+#pragma unroll
+        for (int u = 0; u < NQ; u++) {
+            const int q = tid + 256 * u;
+            if (q < 2 * NB) *(uint4 *)(sa + ((q & 1) * NB + (q >> 1)) * 8) = stage[u];
+        }
+        // the slice's weights, likewise
+        constexpr int NW = (27 * NCOL * 2 + 255) / 256;
+        uint4 wst[NW];
+#pragma unroll
+        for (int u = 0; u < NW; u++) {
+            const int q = tid + 256 * u;
+            const int r = q >> 1, col = r % NCOL, tp = r / NCOL;
+            wst[u] = q < 27 * NCOL * 2
+                         ? *(const uint4 *)(a.w + ((int64_t)tp * a.cout_pad + cob * NCOL + col) * a.cin + c0 + (q & 1) * 8)
+                         : make_uint4(0, 0, 0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < NW; u++) {
+            const int q = tid + 256 * u;
+            const int r = q >> 1, col = r % NCOL, tp = r / NCOL;
+            if (q < 27 * NCOL * 2) *(uint4 *)(sb + ((tp * 2 + (q & 1)) * NCOL + col) * 8) = wst[u];
         }
         __syncthreads();
 #pragma unroll 3
@@ -306,21 +327,31 @@ k_conv3d_c16(conv_args a)
 
     for (int sl = 0; sl < nsl; sl++) {
         const int c0 = sl * 16;
-        for (int q = tid; q < 2 * NB; q += 256) {
+        // all of a thread's brick loads are issued before any LDS store, so
+        // their latencies overlap instead of adding up
+        constexpr int NQ = (2 * NB + 255) / 256;
+        uint4 stage[NQ];
+#pragma unroll
+        for (int u = 0; u < NQ; u++) {
+            const int q = tid + 256 * u;
             const int vb = q >> 1, hf = q & 1;
             const int bx = vb % BX, r = vb / BX, by = r % BY, bz = r / BY;
             const int gz = z0 - 1 + bz, gy = y0 - 1 + by, gx = x0 - 1 + bx;
-            uint4 val = make_uint4(0, 0, 0, 0);
-            if (gz >= 0 && gz < a.D && gy >= 0 && gy < a.H && gx >= 0 && gx < a.W) {
+            stage[u] = make_uint4(0, 0, 0, 0);
+            if (q < 2 * NB && gz >= 0 && gz < a.D && gy >= 0 && gy < a.H && gx >= 0 && gx < a.W) {
                 if (c0 < a.ca) {
                     const int64_t src = (((int64_t)n * Da + (gz >> a.up)) * Ha + (gy >> a.up)) * Wa + (gx >> a.up);
-                    val = *(const uint4 *)(a.xa + src * a.ca + c0 + hf * 8);
+                    stage[u] = *(const uint4 *)(a.xa + src * a.ca + c0 + hf * 8);
                 } else {
                     const int64_t src = (((int64_t)n * a.D + gz) * a.H + gy) * a.W + gx;
-                    val = *(const uint4 *)(a.xb + src * a.cb + (c0 - a.ca) + hf * 8);
+                    stage[u] = *(const uint4 *)(a.xb + src * a.cb + (c0 - a.ca) + hf * 8);
                 }
             }
-            *(uint4 *)(sa + (hf * NB + vb) * 8) = val;
+        }
+#pragma unroll
+        for (int u = 0; u < NQ; u++) {
+            const int q = tid + 256 * u;
+            if (q < 2 * NB) *(uint4 *)(sa + ((q & 1) * NB + (q >> 1)) * 8) = stage[u];
         }
         dh_bf16x8 B[14];
 #pragma unroll
