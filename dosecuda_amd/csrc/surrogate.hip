@@ -624,13 +624,19 @@ __device__ static inline void block_max_to(float m, float *out)
     }
 }
 
-// max of a non-negative volume
+// max of a non-negative volume (16-byte loads when the base allows)
 __global__ void __launch_bounds__(256)
 k_max_nonneg(const float *__restrict__ v, int64_t n, float *__restrict__ out)
 {
     float m = 0.0f;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        m = fmaxf(m, v[i]);
+    const int64_t n4 = ((uintptr_t)v & 15) == 0 ? n / 4 : 0;
+    const float4 *v4 = (const float4 *)v;
+    const int64_t step = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n4; i += step) {
+        const float4 q = v4[i];
+        m = fmaxf(m, fmaxf(fmaxf(q.x, q.y), fmaxf(q.z, q.w)));
+    }
+    for (int64_t i = 4 * n4 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += step) m = fmaxf(m, v[i]);
     block_max_to(m, out);
 }
 
@@ -649,37 +655,40 @@ __global__ void k_fill_i32(int *p, int n, int v)
 // Input cells on the half-resolution grid of the 16-padded volume: channels
 // 0-7 TERMA / max, 8-15 density of the 2x2x2 sub-voxels (sub = 4 dz + 2 dy +
 // dx), 16-18 the beam direction, 19 the spacing channel, 20-31 zero.  One
-// thread writes 8 channels.
-__global__ void k_sur_pack(const float *__restrict__ terma, const float *__restrict__ dens, int nz, int ny,
-                           int nx, int cz, int cy, int cx, const float *__restrict__ tmax, float d0,
-                           float d1, float d2, float spc, uint16_t *__restrict__ out)
+// thread writes one cell's 32 channels (four 16-byte stores).
+__global__ void __launch_bounds__(256)
+k_sur_pack(const float *__restrict__ terma, const float *__restrict__ dens, int nz, int ny,
+           int nx, int cz, int cy, int cx, const float *__restrict__ tmax, float d0,
+           float d1, float d2, float spc, uint16_t *__restrict__ out)
 {
-    const int64_t n = (int64_t)cz * cy * cx * 4;
+    const int64_t n = (int64_t)cz * cy * cx;
     const float scale = fmaxf(*tmax, 1e-12f);
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const int grp = (int)(i & 3);
-        const int64_t c = i >> 2;
+    uint4 cst;
+    {
+        uint16_t h[8] = {f2bf(d0), f2bf(d1), f2bf(d2), f2bf(spc), 0, 0, 0, 0};
+        __builtin_memcpy(&cst, h, 16);
+    }
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
         const int ci = (int)(c % cx);
         const int64_t r = c / cx;
         const int cj = (int)(r % cy), ck = (int)(r / cy);
-        uint16_t h[8];
-        if (grp < 2) {
+        uint16_t ht[8], hd[8];
 #pragma unroll
-            for (int sub = 0; sub < 8; sub++) {
-                const int k = 2 * ck + (sub >> 2), j = 2 * cj + ((sub >> 1) & 1), ii = 2 * ci + (sub & 1);
-                const bool in = k < nz && j < ny && ii < nx;
-                const int64_t src = ((int64_t)k * ny + j) * nx + ii;
-                const float val = grp == 0 ? (in ? terma[src] / scale : 0.0f) : (in ? dens[src] : DH_AIR_DENSITY);
-                h[sub] = f2bf(val);
-            }
-        } else {
-#pragma unroll
-            for (int e = 0; e < 8; e++) h[e] = 0;
-            if (grp == 2) { h[0] = f2bf(d0); h[1] = f2bf(d1); h[2] = f2bf(d2); h[3] = f2bf(spc); }
+        for (int sub = 0; sub < 8; sub++) {
+            const int k = 2 * ck + (sub >> 2), j = 2 * cj + ((sub >> 1) & 1), ii = 2 * ci + (sub & 1);
+            const bool in = k < nz && j < ny && ii < nx;
+            const int64_t src = ((int64_t)k * ny + j) * nx + ii;
+            ht[sub] = f2bf(in ? terma[src] / scale : 0.0f);
+            hd[sub] = f2bf(in ? dens[src] : DH_AIR_DENSITY);
         }
-        uint4 o;
-        __builtin_memcpy(&o, h, 16);
-        *(uint4 *)(out + c * 32 + grp * 8) = o;
+        uint4 a, b;
+        __builtin_memcpy(&a, ht, 16);
+        __builtin_memcpy(&b, hd, 16);
+        uint4 *o = (uint4 *)(out + c * 32);
+        o[0] = a;
+        o[1] = b;
+        o[2] = cst;
+        o[3] = make_uint4(0, 0, 0, 0);
     }
 }
 
@@ -687,26 +696,35 @@ __global__ void k_sur_pack(const float *__restrict__ terma, const float *__restr
 // channels 0-7) back on the voxel grid, scaled by max TERMA and the
 // network's output scale, plus the electron term, times MU calibration;
 // zero where TERMA <= 0.01 or the value is not positive.  Also the dose max.
+// One thread per cell of the volume (two 16-byte loads of its outputs).
 __global__ void __launch_bounds__(256)
 k_sur_finish(const float *__restrict__ y32, int cy, int cx, const float *__restrict__ tmax,
-                             float out_scale, const float *__restrict__ terma, const float *__restrict__ electron,
-                             float mu_scale, int nz, int ny, int nx, float *__restrict__ dose,
-                             float *__restrict__ dmax)
+             float out_scale, const float *__restrict__ terma, const float *__restrict__ electron,
+             float mu_scale, int nz, int ny, int nx, float *__restrict__ dose,
+             float *__restrict__ dmax)
 {
-    const int64_t n = (int64_t)nz * ny * nx;
+    const int vz = (nz + 1) / 2, vy = (ny + 1) / 2, vx = (nx + 1) / 2;
+    const int64_t n = (int64_t)vz * vy * vx;
     const float s = fmaxf(*tmax, 1e-12f) * out_scale;
     float m = 0.0f;
-    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
-        const int i = (int)(v % nx);
-        const int64_t r = v / nx;
-        const int j = (int)(r % ny), k = (int)(r / ny);
-        const int64_t c = ((int64_t)(k >> 1) * cy + (j >> 1)) * cx + (i >> 1);
-        const int sub = 4 * (k & 1) + 2 * (j & 1) + (i & 1);
-        const float sup = y32[c * 16 + sub] * s;
-        const float d = (sup + electron[v]) * mu_scale;
-        const float o = (terma[v] > 0.01f && d > 0.0f) ? d : 0.0f;
-        dose[v] = o;
-        m = fmaxf(m, o);
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+        const int ci = (int)(c % vx);
+        const int64_t r = c / vx;
+        const int cj = (int)(r % vy), ck = (int)(r / vy);
+        const int64_t cc = ((int64_t)ck * cy + cj) * cx + ci;
+        const float4 y0 = *(const float4 *)(y32 + cc * 16), y1 = *(const float4 *)(y32 + cc * 16 + 4);
+        const float yy[8] = {y0.x, y0.y, y0.z, y0.w, y1.x, y1.y, y1.z, y1.w};
+#pragma unroll
+        for (int sub = 0; sub < 8; sub++) {
+            const int k = 2 * ck + (sub >> 2), j = 2 * cj + ((sub >> 1) & 1), i = 2 * ci + (sub & 1);
+            if (k >= nz || j >= ny || i >= nx) continue;
+            const int64_t v = ((int64_t)k * ny + j) * nx + i;
+            const float sup = yy[sub] * s;
+            const float d = (sup + electron[v]) * mu_scale;
+            const float o = (terma[v] > 0.01f && d > 0.0f) ? d : 0.0f;
+            dose[v] = o;
+            m = fmaxf(m, o);
+        }
     }
     block_max_to(m, dmax);
 }
@@ -990,9 +1008,9 @@ void launch_pack(int nz, int ny, int nx, const float *terma, const float *densit
     const int64_t n = (int64_t)nz * ny * nx;
     const int cz = ((nz + 15) / 16) * 8, cy = ((ny + 15) / 16) * 8, cx = ((nx + 15) / 16) * 8;
     hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, s, tmax, 1, 0.0f);
-    hipLaunchKernelGGL(k_max_nonneg, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 1024)), dim3(256), 0, s,
+    hipLaunchKernelGGL(k_max_nonneg, dim3((unsigned)std::min<int64_t>((n / 4 + 255) / 256 + 1, 2048)), dim3(256), 0, s,
                        terma, n, tmax);
-    const int64_t items = (int64_t)cz * cy * cx * 4;
+    const int64_t items = (int64_t)cz * cy * cx;
     dh_timed t("sur_pack", s);
     hipLaunchKernelGGL(k_sur_pack, dim3((unsigned)std::min<int64_t>((items + 255) / 256, 8192)), dim3(256), 0, s,
                        terma, density, nz, ny, nx, cz, cy, cx, (const float *)tmax, dir[0], dir[1], dir[2],
@@ -1008,7 +1026,8 @@ void launch_finish(int nz, int ny, int nx, const float *y32, const float *tmax, 
     const int cy = ((ny + 15) / 16) * 8, cx = ((nx + 15) / 16) * 8;
     hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, s, dmax, 1, 0.0f);
     dh_timed t("sur_finish", s);
-    hipLaunchKernelGGL(k_sur_finish, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 1024)), dim3(256), 0, s,
+    const int64_t ncell = (int64_t)((nz + 1) / 2) * ((ny + 1) / 2) * ((nx + 1) / 2);
+    hipLaunchKernelGGL(k_sur_finish, dim3((unsigned)std::min<int64_t>((ncell + 255) / 256, 2048)), dim3(256), 0, s,
                        y32, cy, cx, tmax, out_scale, terma, electron, mu_scale, nz, ny, nx, dose, dmax);
     DH_HIP(hipGetLastError());
 }
