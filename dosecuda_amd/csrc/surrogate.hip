@@ -101,25 +101,31 @@ k_conv3d(conv_args a)
         for (int j = 0; j < NCO; j++) bv[j] = *(const uint4 *)(wp + (int64_t)j * 32 * a.cin);
     };
 
+    // A ring of PF K steps in flight: step ki's slot is refilled with step
+    // ki + PF right after its MFMAs, so the loads of the next PF - 1 steps
+    // overlap this one (the K order, hence the sum, is unchanged).
+    constexpr int PF = 4;
     if (k0 < k1) {
-        uint4 av, bv[NCO];
-        fetch(k0, av, bv);
-        for (int ki = k0; ki < k1; ki++) {
-            uint4 an = av, bn[NCO];
+        uint4 av[PF], bv[PF][NCO];
 #pragma unroll
-            for (int j = 0; j < NCO; j++) bn[j] = bv[j];
-            if (ki + 1 < k1) fetch(ki + 1, an, bn);
-            dh_bf16x8 A;
-            __builtin_memcpy(&A, &av, 16);
+        for (int p = 0; p < PF; p++)
+            if (k0 + p < k1) fetch(k0 + p, av[p], bv[p]);
+        for (int kb = k0; kb < k1; kb += PF) {
 #pragma unroll
-            for (int j = 0; j < NCO; j++) {
-                dh_bf16x8 B;
-                __builtin_memcpy(&B, &bv[j], 16);
-                acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, B, acc[j], 0, 0, 0);
+            for (int p = 0; p < PF; p++) {
+                const int ki = kb + p;
+                if (ki < k1) {
+                    dh_bf16x8 A;
+                    __builtin_memcpy(&A, &av[p], 16);
+#pragma unroll
+                    for (int j = 0; j < NCO; j++) {
+                        dh_bf16x8 B;
+                        __builtin_memcpy(&B, &bv[p][j], 16);
+                        acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, B, acc[j], 0, 0, 0);
+                    }
+                    if (ki + PF < k1) fetch(ki + PF, av[p], bv[p]);
+                }
             }
-            av = an;
-#pragma unroll
-            for (int j = 0; j < NCO; j++) bv[j] = bn[j];
         }
     }
 
