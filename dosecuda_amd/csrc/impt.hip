@@ -132,6 +132,7 @@ k_raytrace(dh_geom g, dh_pad p, dh_tile tile, const float4 *__restrict__ vol, fl
 
 struct smooth_dirs {
     float c[6], s[6];
+    float tx[6], ty[6], tz[6];   // image-index step per mm along each direction
 };
 
 // Lateral smoothing (IMPTClasses.cu:139-202): mean of the voxel's WET and the
@@ -151,13 +152,33 @@ k_smooth(dh_geom g, smooth_dirs dirs, dh_tile tile, const float *__restrict__ we
     double sum = (double)c;
     int n = 1;
     const float lim = c * 10.0f;
+    // The neighbour's continuous image index is linear in the radius: from
+    // this voxel's index t0 in steps of dirs.t per mm it is known to ~1e-4
+    // voxel.  Where every coordinate is more than 2e-3 from a rounding tie
+    // that fixes the rounded index; lanes near a tie redo the reference's
+    // FP32 sequence (rotation, division by the spacing, roundf).
+    float q0x, q0y, q0z;
+    dh_head_to_img(g, hx, hy, hz, q0x, q0y, q0z);
+    const float t0x = (q0x + g.iso[0]) * g.inv_s, t0y = (q0y + g.iso[1]) * g.inv_s;
+    const float t0z = (q0z + g.iso[2]) * g.inv_s;
     for (int a = 0; a < 6; a++) {
         for (float dr = 1.0f; dr < lim && dr < 10.0f; dr += 1.0f) {
-            float qx, qy, qz;
-            dh_head_to_img(g, hx + dr * dirs.c[a], hy + dr * dirs.s[a], hz, qx, qy, qz);
-            unsigned ii = (unsigned)(int)roundf((qx + g.iso[0]) / g.s);
-            unsigned jj = (unsigned)(int)roundf((qy + g.iso[1]) / g.s);
-            unsigned kk = (unsigned)(int)roundf((qz + g.iso[2]) / g.s);
+            const float tx = fmaf(dr, dirs.tx[a], t0x), ty = fmaf(dr, dirs.ty[a], t0y);
+            const float tz = fmaf(dr, dirs.tz[a], t0z);
+            const float fx = tx - floorf(tx), fy = ty - floorf(ty), fz = tz - floorf(tz);
+            const bool tie = fabsf(fx - 0.5f) < 2e-3f || fabsf(fy - 0.5f) < 2e-3f || fabsf(fz - 0.5f) < 2e-3f;
+            unsigned ii, jj, kk;
+            if (!tie) {
+                ii = (unsigned)(int)floorf(tx + 0.5f);
+                jj = (unsigned)(int)floorf(ty + 0.5f);
+                kk = (unsigned)(int)floorf(tz + 0.5f);
+            } else {
+                float qx, qy, qz;
+                dh_head_to_img(g, hx + dr * dirs.c[a], hy + dr * dirs.s[a], hz, qx, qy, qz);
+                ii = (unsigned)(int)roundf((qx + g.iso[0]) / g.s);
+                jj = (unsigned)(int)roundf((qy + g.iso[1]) / g.s);
+                kk = (unsigned)(int)roundf((qz + g.iso[2]) / g.s);
+            }
             if (ii < (unsigned)g.nx && jj < (unsigned)g.ny && kk < (unsigned)g.nz) {
                 sum += (double)wet_in[ii + (unsigned)g.nx * (jj + (unsigned)g.ny * kk)];
                 n++;
@@ -279,6 +300,12 @@ void dh_launch_smooth(const dh_geom &g, const float *wet_raw, float *wet_out, hi
         float ang = (float)a * DH_PI_F / 3.0f;
         d.s[a] = sinf(ang);
         d.c[a] = cosf(ang);
+        // the head->image rotation is linear: image-index step per mm
+        float x, y, z;
+        dh_head_to_img(g, d.c[a], d.s[a], 0.0f, x, y, z);
+        d.tx[a] = x * g.inv_s;
+        d.ty[a] = y * g.inv_s;
+        d.tz[a] = z * g.inv_s;
     }
     if (choice < 0) choice = dh_tune_lookup("smooth", dh_tune_key_geom(g), 0);
     unsigned grid = 0;
