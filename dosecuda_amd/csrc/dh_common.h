@@ -229,36 +229,57 @@ __device__ static inline float dh_trilinear_u(const float *__restrict__ vol, con
 }
 
 // The same two samplers over the quad-packed padded volume: element e holds
-// the padded values (e, e + 1, e + px, e + px + 1), so the 8 corners come in
-// two 16-byte loads (planes k0 and k0 + 1) instead of eight; the corner
-// values and the blend are the same, so are the results.
+// {v(e), v(e + 1) - v(e), v(e + px), v(e + px + 1) - v(e + px)}, so the 8
+// corners come in two 16-byte loads (planes k0 and k0 + 1) instead of eight,
+// and each x interpolation fmaf(a, v1 - v0, v0) reads the difference it would
+// have formed itself: the results are the same as from the plain corners.
 __device__ static inline float dh_blend8(float4 lo, float4 hi, float ax, float ay, float az)
 {
-    float a00 = fmaf(ax, lo.y - lo.x, lo.x);
-    float a10 = fmaf(ax, lo.w - lo.z, lo.z);
-    float a01 = fmaf(ax, hi.y - hi.x, hi.x);
-    float a11 = fmaf(ax, hi.w - hi.z, hi.z);
+    float a00 = fmaf(ax, lo.y, lo.x);
+    float a10 = fmaf(ax, lo.w, lo.z);
+    float a01 = fmaf(ax, hi.y, hi.x);
+    float a11 = fmaf(ax, hi.w, hi.z);
     float b0 = fmaf(ay, a10 - a00, a00);
     float b1 = fmaf(ay, a11 - a01, a01);
     return fmaf(az, b1 - b0, b0);
 }
 
-__device__ static inline float dh_trilinear_uq(const float4 *__restrict__ q, const dh_pad &p,
-                                               float ux, float uy, float uz)
+// Element index of padded voxel (i + 1, j + 1, k + 1), i, j, k >= -1, in 32
+// bits (24-bit products): valid where px * py < 2^23 and the volume has fewer
+// than 2^31 elements.
+__device__ static inline uint32_t dh_pad_index32(const dh_pad &p, int i, int j, int k)
+{
+    const int pxy = (int)p.pxy;
+    return (uint32_t)(__mul24(k, pxy) + __mul24(j, p.px) + i + (1 + p.px + pxy));
+}
+
+// Trilinear sample of the quad-packed volume at u = t - 0.5 (already clamped
+// or known inside); q_hi = q + pxy.  Off32 takes 32-bit byte offsets against
+// the two uniform plane bases (volumes under 2 GiB, px * py < 2^23).
+template <bool Off32>
+__device__ static inline float dh_trilinear_uq(const float4 *__restrict__ q, const float4 *__restrict__ q_hi,
+                                               const dh_pad &p, float ux, float uy, float uz)
 {
     float fx0 = floorf(ux), fy0 = floorf(uy), fz0 = floorf(uz);
     float ax = ux - fx0, ay = uy - fy0, az = uz - fz0;
-    const float4 *b = q + dh_pad_index(p, (int)fx0, (int)fy0, (int)fz0);
-    return dh_blend8(b[0], b[p.pxy], ax, ay, az);
+    if constexpr (Off32) {
+        const uint32_t off = dh_pad_index32(p, (int)fx0, (int)fy0, (int)fz0) * 16u;
+        return dh_blend8(*(const float4 *)((const char *)q + off), *(const float4 *)((const char *)q_hi + off),
+                         ax, ay, az);
+    } else {
+        const int64_t e = dh_pad_index(p, (int)fx0, (int)fy0, (int)fz0);
+        return dh_blend8(q[e], q_hi[e], ax, ay, az);
+    }
 }
 
-__device__ static inline float dh_trilinear_q(const float4 *__restrict__ q, const dh_pad &p,
-                                              int nx, int ny, int nz, float tx, float ty, float tz)
+template <bool Off32>
+__device__ static inline float dh_trilinear_q(const float4 *__restrict__ q, const float4 *__restrict__ q_hi,
+                                              const dh_pad &p, int nx, int ny, int nz, float tx, float ty, float tz)
 {
     float ux = __builtin_amdgcn_fmed3f(tx - 0.5f, -1.0f, (float)nx);
     float uy = __builtin_amdgcn_fmed3f(ty - 0.5f, -1.0f, (float)ny);
     float uz = __builtin_amdgcn_fmed3f(tz - 0.5f, -1.0f, (float)nz);
-    return dh_trilinear_uq(q, p, ux, uy, uz);
+    return dh_trilinear_uq<Off32>(q, q_hi, p, ux, uy, uz);
 }
 
 // Per (voxel, layer) factors of the double-Gaussian pencil beam

@@ -162,7 +162,8 @@ void dh_launch_pencil(const dh_geom &g, float vsadx, float vsady, int n_layers,
 void dh_launch_terma(const dh_terma_args &a, const float *wet, float *terma, float *electron,
                      hipStream_t s);
 // Sampling layout of the cone superposition's {TERMA, density} volume:
-// 0 = float2 pairs, 1 = float4 quads (see k_pack_tr); choice < 0 takes the
+// 0 = 8-byte pairs, 1 = 32-byte quads with forward x differences (see
+// k_pack_tr); choice < 0 takes the
 // tuning table's choice ("ccc", default 0).  The scratch for either layout
 // is dh_tr_pad_bytes.
 int dh_ccc_layout(const dh_ccc_args &a, int choice);

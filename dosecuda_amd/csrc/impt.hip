@@ -13,10 +13,10 @@ namespace {
 constexpr int kBlock = 256;
 
 // Quad-packed padded volume (1 voxel of border below, 2 above on each axis):
-// element (x, y, z) of the padded grid holds the padded values at (x, y),
-// (x + 1, y), (x, y + 1), (x + 1, y + 1) of plane z; outside the volume the
-// value is the border.  Elements of the last padded row / column are never
-// read by the samplers and hold the border.
+// element (x, y, z) of the padded grid holds, of plane z, the padded values
+// v(x, y) and v(x, y + 1), each followed by its forward x difference
+// v(x + 1, .) - v(x, .); outside the volume the value is the border.  Elements
+// of the last padded row / column are never read by the samplers.
 __global__ void __launch_bounds__(kBlock)
 k_pad_quad(const float *__restrict__ src, float4 *__restrict__ dst, int nx, int ny, int nz,
            dh_pad p, float border)
@@ -35,7 +35,7 @@ k_pad_quad(const float *__restrict__ src, float4 *__restrict__ dst, int nx, int 
                        ? src[(uint32_t)ii + (uint32_t)nx * ((uint32_t)jj + (uint32_t)ny * (uint32_t)k)]
                        : border;
         }
-        dst[t] = make_float4(v[0], v[1], v[2], v[3]);
+        dst[t] = make_float4(v[0], v[1] - v[0], v[2], v[3] - v[2]);
     }
 }
 
@@ -69,9 +69,11 @@ __device__ static inline int safe_samples(const dh_geom &g, float x, float y, fl
     return (int)fmin(floor(Lmin) + 1.0, 1.0e7);
 }
 
+template <bool Off32>
 __global__ void __launch_bounds__(kBlock)
 k_raytrace(dh_geom g, dh_pad p, dh_tile tile, const float4 *__restrict__ vol, float *__restrict__ wet)
 {
+    const float4 *vol_hi = vol + p.pxy;
     int i, j, k;
     if (!dh_tile_voxel(tile, dh_xcd_block(blockIdx.x, gridDim.x), threadIdx.x, g.nx, g.ny, g.nz, i, j, k)) return;
     const uint32_t v = (uint32_t)i + (uint32_t)g.nx * ((uint32_t)j + (uint32_t)g.ny * (uint32_t)k);
@@ -93,8 +95,8 @@ k_raytrace(dh_geom g, dh_pad p, dh_tile tile, const float4 *__restrict__ vol, fl
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const float L = (float)(m + q);
-            part += fmaxf(dh_trilinear_uq(vol, p, fmaf(dux, L, u0x), fmaf(duy, L, u0y),
-                                          fmaf(duz, L, u0z)), 0.0f);
+            part += fmaxf(dh_trilinear_uq<Off32>(vol, vol_hi, p, fmaf(dux, L, u0x), fmaf(duy, L, u0y),
+                                                 fmaf(duz, L, u0z)), 0.0f);
         }
         acc += (double)part;
     }
@@ -114,7 +116,7 @@ k_raytrace(dh_geom g, dh_pad p, dh_tile tile, const float4 *__restrict__ vol, fl
             const float ty = fmaf(g.inv_s, qy + g.iso[1], 0.5f);
             const float tz = fmaf(g.inv_s, qz + g.iso[2], 0.5f);
             in[q] = tx >= 0.0f && tx < fnx && ty >= 0.0f && ty < fny && tz >= 0.0f && tz < fnz;
-            d[q] = dh_trilinear_q(vol, p, g.nx, g.ny, g.nz, tx, ty, tz);
+            d[q] = dh_trilinear_q<Off32>(vol, vol_hi, p, g.nx, g.ny, g.nz, tx, ty, tz);
         }
         float part = 0.0f;
         bool all = true;
@@ -289,7 +291,11 @@ void dh_launch_raytrace(const dh_geom &g, const float4 *vol_pad, float *wet, hip
     unsigned grid = 0;
     const dh_tile tile = dh_make_tile(choice, g.nx, g.ny, g.nz, &grid);
     dh_timed t("raytrace", s);
-    hipLaunchKernelGGL(k_raytrace, dim3(grid), dim3(kBlock), 0, s, g, p, tile, vol_pad, wet);
+    // 32-bit offsets for quad volumes under 2 GiB whose planes fit 24-bit products
+    if ((size_t)16 * p.pxy * p.pz < ((size_t)1 << 31) && p.pxy < ((int64_t)1 << 23))
+        hipLaunchKernelGGL(k_raytrace<true>, dim3(grid), dim3(kBlock), 0, s, g, p, tile, vol_pad, wet);
+    else
+        hipLaunchKernelGGL(k_raytrace<false>, dim3(grid), dim3(kBlock), 0, s, g, p, tile, vol_pad, wet);
     DH_HIP(hipGetLastError());
 }
 

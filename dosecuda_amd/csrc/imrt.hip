@@ -2,8 +2,8 @@
 // 72-cone collapsed-cone superposition.
 //
 // The CCCS samples TERMA and density at the same point, so both are packed
-// into one padded float2 volume {TERMA (border 0), density (border 0.0012)}:
-// one 8-byte gather per trilinear corner serves both fields.  Voxels with
+// into one padded volume {TERMA (border 0), density (border 0.0012)} read as
+// 16-byte gathers: one per trilinear x-row serves both fields.  Voxels with
 // TERMA <= 0.01 are written as zero by a compaction pass that also builds the
 // list of voxels the cone kernel visits (wave ballots, one atomic per
 // workgroup), so the cone kernel's waves are full of working lanes.
@@ -128,12 +128,17 @@ k_terma(dh_terma_args a, const float *__restrict__ wet, float *__restrict__ term
     electron[v] = a.esw * (0.4f + (0.3f * tr)) * el * Tp;
 }
 
-// Packs {TERMA, density} into the padded float2 layout with their borders.
-// {TERMA, density} of the padded grid (border TERMA 0, density air), as
-// float2 pairs, or (Quad) as two float4 per padded element holding the
-// element's TERMA and density at (x, y), (x + 1, y), (x, y + 1), (x + 1, y + 1),
-// so a trilinear sample takes four 16-byte loads instead of eight 8-byte
-// ones.  Elements of the last padded row / column are never read.
+// {TERMA, density} of the padded grid (border TERMA 0, density air) in one
+// of two layouts:
+//   pairs (8 B per padded element e):   {T(e), rho(e)}; an x-row of a cell is
+//                                       the 16 bytes at e, e + 1
+//   quads (32 B per padded element e):  {T, T(x+1) - T} of rows y and y + 1, then
+//                                       the same for rho
+// The quads' x interpolation fmaf(w, v1 - v0, v0) reads v1 - v0 as stored,
+// the same FP32 difference it would form itself, so both layouts give
+// bit-identical samples.  Elements of the last padded row / column are never
+// read.  (A 16-byte pairs record with stored differences was measured: the
+// doubled footprint costs more in cache misses than the saved subtractions.)
 __device__ static inline float2 tr_at(const float *__restrict__ terma, const float *__restrict__ dens, int nx,
                                       int ny, int nz, int i, int j, int k)
 {
@@ -155,38 +160,80 @@ k_pack_tr(const float *__restrict__ terma, const float *__restrict__ dens, int n
         const uint32_t r = t / (uint32_t)p.px;
         const int j = (int)(r % (uint32_t)p.py) - 1;
         const int k = (int)(r / (uint32_t)p.py) - 1;
+        const float2 a = tr_at(terma, dens, nx, ny, nz, i, j, k), b = tr_at(terma, dens, nx, ny, nz, i + 1, j, k);
         if (!Quad) {
-            ((float2 *)outv)[t] = tr_at(terma, dens, nx, ny, nz, i, j, k);
+            ((float2 *)outv)[t] = a;
         } else {
-            const float2 a = tr_at(terma, dens, nx, ny, nz, i, j, k), b = tr_at(terma, dens, nx, ny, nz, i + 1, j, k);
             const float2 c = tr_at(terma, dens, nx, ny, nz, i, j + 1, k);
             const float2 d = tr_at(terma, dens, nx, ny, nz, i + 1, j + 1, k);
             float4 *o = (float4 *)outv + 2 * (size_t)t;
-            o[0] = make_float4(a.x, b.x, c.x, d.x);
-            o[1] = make_float4(a.y, b.y, c.y, d.y);
+            o[0] = make_float4(a.x, b.x - a.x, c.x, d.x - c.x);
+            o[1] = make_float4(a.y, b.y - a.y, c.y, d.y - c.y);
         }
     }
 }
 
-// The 8 trilinear corners {TERMA, density} at padded element e (planes z and
-// z + 1) from either layout.
+// Bytes per padded element of a layout.
+template <bool Quad> constexpr uint32_t tr_elem_bytes() { return Quad ? 32u : 8u; }
+
+// Base addresses of the four x-rows of a trilinear cell, relative to the
+// cell's element: rows (y, z), (y + 1, z), (y, z + 1), (y + 1, z + 1).  In the
+// quad layout the y + 1 row sits in the same record as row y.
+struct tr_bases {
+    const char *r00, *r10, *r01, *r11;
+};
+
 template <bool Quad>
-__device__ static inline void tr_corners(const void *__restrict__ trv, const dh_pad &p, int64_t e, float2 &v000,
-                                         float2 &v100, float2 &v010, float2 &v110, float2 &v001, float2 &v101,
-                                         float2 &v011, float2 &v111)
+__device__ static inline tr_bases tr_make_bases(const void *tr, const dh_pad &p)
+{
+    const size_t eb = tr_elem_bytes<Quad>();
+    tr_bases B;
+    B.r00 = (const char *)tr;
+    B.r10 = B.r00 + (Quad ? 0 : eb * (size_t)p.px);
+    B.r01 = B.r00 + eb * (size_t)p.pxy;
+    B.r11 = B.r01 + (Quad ? 0 : eb * (size_t)p.px);
+    return B;
+}
+
+// x-interpolated {TERMA, density} of the cell's four x-rows at weight wx;
+// `off` is the byte offset of the cell's element (a 32-bit offset keeps the
+// address arithmetic in 32 bits against the rows' uniform bases).
+template <bool Quad, typename Off>
+__device__ static inline void tr_xrows(const tr_bases &B, Off off, float wx, float &t00, float &t10, float &t01,
+                                       float &t11, float &r00, float &r10, float &r01, float &r11)
 {
     if (!Quad) {
-        const float2 *b = (const float2 *)trv + e;
-        const float2 *c = b + p.pxy;
-        v000 = b[0]; v100 = b[1]; v010 = b[p.px]; v110 = b[p.px + 1];
-        v001 = c[0]; v101 = c[1]; v011 = c[p.px]; v111 = c[p.px + 1];
+        // {T, rho} at e and e + 1 of each row (the compiler merges each pair
+        // into one 16-byte load)
+        const float2 *a = (const float2 *)(B.r00 + off), *b = (const float2 *)(B.r10 + off);
+        const float2 *c = (const float2 *)(B.r01 + off), *d = (const float2 *)(B.r11 + off);
+        const float2 a0 = a[0], a1 = a[1], b0 = b[0], b1 = b[1], c0 = c[0], c1 = c[1], d0 = d[0], d1 = d[1];
+        t00 = fmaf(wx, a1.x - a0.x, a0.x); t10 = fmaf(wx, b1.x - b0.x, b0.x);
+        t01 = fmaf(wx, c1.x - c0.x, c0.x); t11 = fmaf(wx, d1.x - d0.x, d0.x);
+        r00 = fmaf(wx, a1.y - a0.y, a0.y); r10 = fmaf(wx, b1.y - b0.y, b0.y);
+        r01 = fmaf(wx, c1.y - c0.y, c0.y); r11 = fmaf(wx, d1.y - d0.y, d0.y);
     } else {
-        const float4 *q = (const float4 *)trv;
-        const float4 tl = q[2 * e], rl = q[2 * e + 1], th = q[2 * (e + p.pxy)], rh = q[2 * (e + p.pxy) + 1];
-        v000 = make_float2(tl.x, rl.x); v100 = make_float2(tl.y, rl.y);
-        v010 = make_float2(tl.z, rl.z); v110 = make_float2(tl.w, rl.w);
-        v001 = make_float2(th.x, rh.x); v101 = make_float2(th.y, rh.y);
-        v011 = make_float2(th.z, rh.z); v111 = make_float2(th.w, rh.w);
+        const float4 tl = *(const float4 *)(B.r00 + off), rl = *(const float4 *)(B.r00 + off + 16);
+        const float4 th = *(const float4 *)(B.r01 + off), rh = *(const float4 *)(B.r01 + off + 16);
+        t00 = fmaf(wx, tl.y, tl.x); t10 = fmaf(wx, tl.w, tl.z);
+        t01 = fmaf(wx, th.y, th.x); t11 = fmaf(wx, th.w, th.z);
+        r00 = fmaf(wx, rl.y, rl.x); r10 = fmaf(wx, rl.w, rl.z);
+        r01 = fmaf(wx, rh.y, rh.x); r11 = fmaf(wx, rh.w, rh.z);
+    }
+}
+
+// Byte offset of padded element (i + 1, j + 1, k + 1) for i, j, k >= -1:
+// 32-bit (24-bit products; the host admits volumes under 2 GiB with
+// px * py < 2^23) or 64-bit.
+template <bool Quad, bool Off32>
+__device__ static inline auto tr_offset(const dh_pad &p, int i, int j, int k)
+{
+    if constexpr (Off32) {
+        const int pxy = (int)p.pxy;
+        const int e = __mul24(k, pxy) + __mul24(j, p.px) + i + (1 + p.px + pxy);
+        return (uint32_t)e * tr_elem_bytes<Quad>();
+    } else {
+        return (uint64_t)dh_pad_index(p, i, j, k) * tr_elem_bytes<Quad>();
     }
 }
 
@@ -217,13 +264,9 @@ __device__ static inline float center_terma(const dh_geom &g, const dh_pad &p,
     const float uz = __builtin_amdgcn_fmed3f(tz, -1.0f, (float)g.nz);
     const float fx = floorf(ux), fy = floorf(uy), fz = floorf(uz);
     const float ax = ux - fx, ay = uy - fy, az = uz - fz;
-    float2 v000, v100, v010, v110, v001, v101, v011, v111;
-    tr_corners<Quad>(tr, p, dh_pad_index(p, (int)fx, (int)fy, (int)fz), v000, v100, v010, v110, v001, v101,
-                     v011, v111);
-    const float a00 = fmaf(ax, v100.x - v000.x, v000.x);
-    const float a10 = fmaf(ax, v110.x - v010.x, v010.x);
-    const float a01 = fmaf(ax, v101.x - v001.x, v001.x);
-    const float a11 = fmaf(ax, v111.x - v011.x, v011.x);
+    float a00, a10, a01, a11, r00, r10, r01, r11;
+    tr_xrows<Quad>(tr_make_bases<Quad>(tr, p), tr_offset<Quad, false>(p, (int)fx, (int)fy, (int)fz), ax, a00, a10,
+                   a01, a11, r00, r10, r01, r11);
     const float b0 = fmaf(ay, a10 - a00, a00), b1 = fmaf(ay, a11 - a01, a01);
     return fmaf(az, b1 - b0, b0);
 }
@@ -263,12 +306,13 @@ k_ccc_compact(dh_geom g, dh_pad p, const void *__restrict__ tr, int *__restrict_
 // combined with a fixed xor-shuffle tree, so the result is deterministic.
 constexpr int kVoxPerWave = 16;
 
-template <bool Quad>
+template <bool Quad, bool Off32>
 __global__ void __launch_bounds__(kBlock)
 k_ccc(dh_ccc_args a, const void *__restrict__ tr, const float *__restrict__ electron,
       const int *__restrict__ list, const int *__restrict__ counter, float *__restrict__ dose)
 {
     const dh_geom &g = a.g;
+    const tr_bases B = tr_make_bases<Quad>(tr, a.p);
     const int n = *counter;
     const float fnx = (float)g.nx, fny = (float)g.ny, fnz = (float)g.nz;
     const int lane = __lane_id();
@@ -327,17 +371,9 @@ k_ccc(dh_ccc_args a, const void *__restrict__ tr, const float *__restrict__ elec
                     float qz = __builtin_amdgcn_fmed3f(fmaf(tdz, Lm, ucz), -1.0f, fnz);
                     const float fx = floorf(qx), fy = floorf(qy), fz = floorf(qz);
                     const float wx = qx - fx, wy = qy - fy, wz = qz - fz;
-                    float2 v000, v100, v010, v110, v001, v101, v011, v111;
-                    tr_corners<Quad>(tr, a.p, dh_pad_index(a.p, (int)fx, (int)fy, (int)fz), v000, v100, v010,
-                                     v110, v001, v101, v011, v111);
-                    const float t00 = fmaf(wx, v100.x - v000.x, v000.x);
-                    const float t10 = fmaf(wx, v110.x - v010.x, v010.x);
-                    const float t01 = fmaf(wx, v101.x - v001.x, v001.x);
-                    const float t11 = fmaf(wx, v111.x - v011.x, v011.x);
-                    const float r00 = fmaf(wx, v100.y - v000.y, v000.y);
-                    const float r10 = fmaf(wx, v110.y - v010.y, v010.y);
-                    const float r01 = fmaf(wx, v101.y - v001.y, v001.y);
-                    const float r11 = fmaf(wx, v111.y - v011.y, v011.y);
+                    float t00, t10, t01, t11, r00, r10, r01, r11;
+                    tr_xrows<Quad>(B, tr_offset<Quad, Off32>(a.p, (int)fx, (int)fy, (int)fz), wx, t00, t10, t01,
+                                   t11, r00, r10, r01, r11);
                     const float t0 = fmaf(wy, t10 - t00, t00), t1 = fmaf(wy, t11 - t01, t01);
                     const float r0 = fmaf(wy, r10 - r00, r00), r1 = fmaf(wy, r11 - r01, r01);
                     const float T = fmaf(wz, t1 - t0, t0);
@@ -381,6 +417,30 @@ int dh_ccc_layout(const dh_ccc_args &a, int choice)
 
 size_t dh_tr_pad_bytes(const dh_ccc_args &a) { return (size_t)32 * a.p.pxy * a.p.pz; }
 
+namespace {
+// 32-bit byte offsets serve packed volumes under 2 GiB whose padded planes fit
+// the 24-bit products (px * py < 2^23); larger ones take 64-bit offsets.
+bool ccc_off32(const dh_ccc_args &a)
+{
+    return dh_tr_pad_bytes(a) < ((size_t)1 << 31) && a.p.pxy < ((int64_t)1 << 23);
+}
+
+void launch_k_ccc(const dh_ccc_args &a, int layout, unsigned grid, const void *tr, const float *electron,
+                  const int *list, const int *counter, float *dose, hipStream_t s)
+{
+    const bool o32 = ccc_off32(a);
+    if (layout && o32)
+        hipLaunchKernelGGL((k_ccc<true, true>), dim3(grid), dim3(kBlock), 0, s, a, tr, electron, list, counter, dose);
+    else if (layout)
+        hipLaunchKernelGGL((k_ccc<true, false>), dim3(grid), dim3(kBlock), 0, s, a, tr, electron, list, counter, dose);
+    else if (o32)
+        hipLaunchKernelGGL((k_ccc<false, true>), dim3(grid), dim3(kBlock), 0, s, a, tr, electron, list, counter, dose);
+    else
+        hipLaunchKernelGGL((k_ccc<false, false>), dim3(grid), dim3(kBlock), 0, s, a, tr, electron, list, counter, dose);
+    DH_HIP(hipGetLastError());
+}
+}  // namespace
+
 void dh_launch_pack_tr(const dh_ccc_args &a, const float *terma, const float *density, void *tr_pad, int layout,
                        hipStream_t s)
 {
@@ -403,11 +463,7 @@ void dh_launch_ccc_counted(const dh_ccc_args &a, const void *tr_pad, int layout,
     if (max_n <= 0) return;
     dh_timed t("ccc_list", s);
     const unsigned grid = std::max(1u, (unsigned)std::min<int64_t>((max_n * 4 + kBlock - 1) / kBlock, 2048));
-    if (layout)
-        hipLaunchKernelGGL(k_ccc<true>, dim3(grid), dim3(kBlock), 0, s, a, tr_pad, electron, list, counter, dose);
-    else
-        hipLaunchKernelGGL(k_ccc<false>, dim3(grid), dim3(kBlock), 0, s, a, tr_pad, electron, list, counter, dose);
-    DH_HIP(hipGetLastError());
+    launch_k_ccc(a, layout, grid, tr_pad, electron, list, counter, dose, s);
 }
 
 void dh_launch_ccc(const dh_ccc_args &a, const float *terma, const float *density,
@@ -433,13 +489,7 @@ void dh_launch_ccc(const dh_ccc_args &a, const float *terma, const float *densit
     {
         dh_timed t("ccc", s);
         const unsigned grid = (unsigned)std::min<size_t>(grid_for(n), 2048);
-        if (layout)
-            hipLaunchKernelGGL(k_ccc<true>, dim3(grid), dim3(kBlock), 0, s, a, (const void *)tr_pad, electron,
-                               (const int *)active, (const int *)counter, dose);
-        else
-            hipLaunchKernelGGL(k_ccc<false>, dim3(grid), dim3(kBlock), 0, s, a, (const void *)tr_pad, electron,
-                               (const int *)active, (const int *)counter, dose);
-        DH_HIP(hipGetLastError());
+        launch_k_ccc(a, layout, grid, tr_pad, electron, active, counter, dose, s);
     }
 }
 
