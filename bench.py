@@ -309,11 +309,13 @@ def main():
             log(f"CCCS tuning: {res['cccs_tuning']}")
         ms = timed(torch, run_cccs, st, a.steps, a.warmup)
         nb = len(st["angles"])
-        kt = {k: engine.prof_get(k) for k in ("pad", "raytrace", "terma", "pack_tr", "ccc_compact", "ccc")}
+        kt = {k: engine.prof_get(k) for k in ("pad", "raytrace", "terma", "pack_tr", "ccc_compact", "ccc",
+                                              "ccc_finish")}
         ccc_ms = kt["ccc"][0] / max(kt["ccc"][1], 1)
-        # algorithmic bytes of one ccc launch: the padded {TERMA, density}
-        # volume once (8 B per padded voxel) + per active voxel: list index,
-        # electron read and dose write (12 B)
+        # algorithmic bytes of one k_ccc launch: the padded {TERMA, density}
+        # volume once (8 B per padded voxel) + per active voxel: its list
+        # index and its 4 azimuth-group sums (20 B; k_ccc_finish adds them,
+        # the electron term and writes the dose)
         c = st["case"]
         from dosecuda_amd import _lib as L
         n = st["dens"].size
@@ -323,7 +325,7 @@ def main():
         engine.photon_dose(st["d_dens"], c.spacing, c.origin, c.iso, 100.0, st["angles"][-1], 0.0,
                            0.0, st["pm"], st["mlc"], out=st["out"], terma_out=terma)
         n_act = int((~(terma <= 0.01)).sum().item())
-        bytes_launch = 8 * npad + 12 * n_act
+        bytes_launch = 8 * npad + 20 * n_act
         achieved = bytes_launch / (ccc_ms * 1e-3) / 1e9
         samples = n_act * 12 * sum(cone_steps(L0, c.spacing) for L0 in st["pm"].kernel[:, 5])
         # k_ccc is bound by its trilinear gathers: 8 corners x {TERMA,

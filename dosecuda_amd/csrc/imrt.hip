@@ -298,36 +298,39 @@ k_ccc_compact(dh_geom g, dh_pad p, const void *__restrict__ tr, int *__restrict_
 // to 0 in FP32 steps of spacing/10 cm (the reference's own step lattice), with
 // the primary (exponential) and scatter (linear) recursions.  Cone results are
 // summed with Neumaier compensation.
-// Work split: a wave holds 16 listed voxels x 4 azimuth groups (lane / 16);
-// group q marches azimuths 3q..3q+2 of all six zenith rows (equal work), so a
-// wave's 16 lanes of one group sample the same cone direction for 16
-// neighbouring voxels (coalesced gathers) and the tail of the voxel list is
-// spread over 4x more, 4x shorter work items.  The four group sums are
-// combined with a fixed xor-shuffle tree, so the result is deterministic.
-constexpr int kVoxPerWave = 16;
+// Work split: a work item is 64 consecutive listed voxels (one per lane) x
+// one of 4 azimuth groups; group q marches azimuths 3q..3q+2 of all six zenith
+// rows (equal work).  All 64 lanes then sample the same cone direction from
+// neighbouring voxels, so a wave's gathers touch few cache lines (the gathers
+// are bound by the lines moved into L1, not by VALU), and the 4 groups keep
+// work items short for the tail.  Each group's compensated sum goes to its own
+// partial buffer; k_ccc_finish adds them as (g0 + g1) + (g2 + g3), so the
+// result is deterministic.
+constexpr int kVoxPerItem = 64;
+constexpr int kGroups = 4;
 
 template <bool Quad, bool Off32>
 __global__ void __launch_bounds__(kBlock)
-k_ccc(dh_ccc_args a, const void *__restrict__ tr, const float *__restrict__ electron,
-      const int *__restrict__ list, const int *__restrict__ counter, float *__restrict__ dose)
+k_ccc(dh_ccc_args a, const void *__restrict__ tr, const int *__restrict__ list, const int *__restrict__ counter,
+      float *__restrict__ part, uint32_t pstride)
 {
     const dh_geom &g = a.g;
     const tr_bases B = tr_make_bases<Quad>(tr, a.p);
     const int n = *counter;
     const float fnx = (float)g.nx, fny = (float)g.ny, fnz = (float)g.nz;
     const int lane = __lane_id();
-    const int grp = lane >> 4;
-    // Each wave takes one contiguous run of 16-voxel chunks of the list, and
-    // the waves of one XCD take neighbouring runs, so an XCD's gathers stay in
-    // one slab of the volume (its own L2).
+    // Each wave takes one contiguous run of work items (a chunk's 4 groups are
+    // neighbours), and the waves of one XCD take neighbouring runs, so an
+    // XCD's gathers stay in one slab of the volume (its own L2).
     const int wave = dh_xcd_block(blockIdx.x, gridDim.x) * (kBlock / 64) + (threadIdx.x >> 6);
     const int nwaves = gridDim.x * (kBlock / 64);
-    const int nchunk = (n + kVoxPerWave - 1) / kVoxPerWave;
-    const int c0 = (int)((int64_t)nchunk * wave / nwaves);
-    const int c1 = (int)((int64_t)nchunk * (wave + 1) / nwaves);
-    for (int chunk = c0; chunk < c1; chunk++) {
-        const int base = chunk * kVoxPerWave;
-        const int idx = base + (lane & (kVoxPerWave - 1));
+    const int nitem = (n + kVoxPerItem - 1) / kVoxPerItem * kGroups;
+    const int c0 = (int)((int64_t)nitem * wave / nwaves);
+    const int c1 = (int)((int64_t)nitem * (wave + 1) / nwaves);
+    for (int item = c0; item < c1; item++) {
+        const int grp = item % kGroups;
+        const int base = item / kGroups * kVoxPerItem;
+        const int idx = base + lane;
         const bool valid = idx < n;
         const uint32_t v = (uint32_t)list[valid ? idx : base];
         const int i = (int)(v % (uint32_t)g.nx);
@@ -387,13 +390,24 @@ k_ccc(dh_ccc_args a, const void *__restrict__ tr, const float *__restrict__ elec
                 dh_neumaier(ds, dcmp, fmaf(am, Rp, bm * Rs));
             }
         }
-        float part = ds + dcmp;
-        part = part + __shfl_xor(part, 16, 64);
-        part = part + __shfl_xor(part, 32, 64);
-        if (grp == 0 && valid) {
-            float dv = part + electron[v];
-            dose[v] = (!isnan(dv) && dv >= 0.0f) ? a.mu_cal * dv * a.mu : 0.0f;
-        }
+        if (valid) part[(size_t)grp * pstride + idx] = ds + dcmp;
+    }
+}
+
+// Dose of the listed voxels from the 4 group sums, the electron term and the
+// MU calibration (IMRTClasses.cu:338-346).
+__global__ void __launch_bounds__(kBlock)
+k_ccc_finish(float mu_cal, float mu, const float *__restrict__ electron, const int *__restrict__ list,
+             const int *__restrict__ counter, const float *__restrict__ part, uint32_t pstride,
+             float *__restrict__ dose)
+{
+    const int n = *counter;
+    for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < n; idx += gridDim.x * kBlock) {
+        const uint32_t v = (uint32_t)list[idx];
+        const float s01 = part[idx] + part[pstride + idx];
+        const float s23 = part[2 * (size_t)pstride + idx] + part[3 * (size_t)pstride + idx];
+        const float dv = (s01 + s23) + electron[v];
+        dose[v] = (!isnan(dv) && dv >= 0.0f) ? mu_cal * dv * mu : 0.0f;
     }
 }
 
@@ -415,28 +429,52 @@ int dh_ccc_layout(const dh_ccc_args &a, int choice)
     return choice;
 }
 
-size_t dh_tr_pad_bytes(const dh_ccc_args &a) { return (size_t)32 * a.p.pxy * a.p.pz; }
+// The packed volume (up to 32 B per padded element), then the cone kernel's
+// 4 group sums per voxel.
+static size_t tr_vol_bytes(const dh_ccc_args &a) { return ((size_t)32 * a.p.pxy * a.p.pz + 255) & ~(size_t)255; }
+size_t dh_tr_pad_bytes(const dh_ccc_args &a)
+{
+    return tr_vol_bytes(a) + sizeof(float) * kGroups * (size_t)a.g.nx * a.g.ny * a.g.nz;
+}
 
 namespace {
 // 32-bit byte offsets serve packed volumes under 2 GiB whose padded planes fit
 // the 24-bit products (px * py < 2^23); larger ones take 64-bit offsets.
 bool ccc_off32(const dh_ccc_args &a)
 {
-    return dh_tr_pad_bytes(a) < ((size_t)1 << 31) && a.p.pxy < ((int64_t)1 << 23);
+    return tr_vol_bytes(a) < ((size_t)1 << 31) && a.p.pxy < ((int64_t)1 << 23);
 }
 
-void launch_k_ccc(const dh_ccc_args &a, int layout, unsigned grid, const void *tr, const float *electron,
-                  const int *list, const int *counter, float *dose, hipStream_t s)
+// Cone kernel + finish over the first *counter entries of list (at most
+// max_n, which sizes the grids).
+void launch_k_ccc(const char *timer, const dh_ccc_args &a, int layout, int64_t max_n, const void *tr,
+                  const float *electron, const int *list, const int *counter, float *dose, hipStream_t s)
 {
     const bool o32 = ccc_off32(a);
-    if (layout && o32)
-        hipLaunchKernelGGL((k_ccc<true, true>), dim3(grid), dim3(kBlock), 0, s, a, tr, electron, list, counter, dose);
-    else if (layout)
-        hipLaunchKernelGGL((k_ccc<true, false>), dim3(grid), dim3(kBlock), 0, s, a, tr, electron, list, counter, dose);
-    else if (o32)
-        hipLaunchKernelGGL((k_ccc<false, true>), dim3(grid), dim3(kBlock), 0, s, a, tr, electron, list, counter, dose);
-    else
-        hipLaunchKernelGGL((k_ccc<false, false>), dim3(grid), dim3(kBlock), 0, s, a, tr, electron, list, counter, dose);
+    const uint32_t pstride = (uint32_t)((size_t)a.g.nx * a.g.ny * a.g.nz);
+    DH_REQUIRE(max_n >= 0 && max_n <= (int64_t)pstride, "cone list longer than the volume");
+    float *part = (float *)((char *)tr + tr_vol_bytes(a));
+    const int64_t items = (max_n + kVoxPerItem - 1) / kVoxPerItem * kGroups;
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((items + 3) / 4, 2048));
+    {
+        dh_timed t(timer, s);
+        if (layout && o32)
+            hipLaunchKernelGGL((k_ccc<true, true>), dim3(grid), dim3(kBlock), 0, s, a, tr, list, counter, part, pstride);
+        else if (layout)
+            hipLaunchKernelGGL((k_ccc<true, false>), dim3(grid), dim3(kBlock), 0, s, a, tr, list, counter, part,
+                               pstride);
+        else if (o32)
+            hipLaunchKernelGGL((k_ccc<false, true>), dim3(grid), dim3(kBlock), 0, s, a, tr, list, counter, part,
+                               pstride);
+        else
+            hipLaunchKernelGGL((k_ccc<false, false>), dim3(grid), dim3(kBlock), 0, s, a, tr, list, counter, part,
+                               pstride);
+        DH_HIP(hipGetLastError());
+    }
+    const unsigned fgrid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((max_n + kBlock - 1) / kBlock, 4096));
+    dh_timed t("ccc_finish", s);
+    hipLaunchKernelGGL(k_ccc_finish, dim3(fgrid), dim3(kBlock), 0, s, a.mu_cal, a.mu, electron, list, counter,
+                       (const float *)part, pstride, dose);
     DH_HIP(hipGetLastError());
 }
 }  // namespace
@@ -461,9 +499,7 @@ void dh_launch_ccc_counted(const dh_ccc_args &a, const void *tr_pad, int layout,
                            const int *list, const int *counter, int64_t max_n, float *dose, hipStream_t s)
 {
     if (max_n <= 0) return;
-    dh_timed t("ccc_list", s);
-    const unsigned grid = std::max(1u, (unsigned)std::min<int64_t>((max_n * 4 + kBlock - 1) / kBlock, 2048));
-    launch_k_ccc(a, layout, grid, tr_pad, electron, list, counter, dose, s);
+    launch_k_ccc("ccc_list", a, layout, max_n, tr_pad, electron, list, counter, dose, s);
 }
 
 void dh_launch_ccc(const dh_ccc_args &a, const float *terma, const float *density,
@@ -486,11 +522,7 @@ void dh_launch_ccc(const dh_ccc_args &a, const float *terma, const float *densit
                                (const void *)tr_pad, active, counter, dose);
         DH_HIP(hipGetLastError());
     }
-    {
-        dh_timed t("ccc", s);
-        const unsigned grid = (unsigned)std::min<size_t>(grid_for(n), 2048);
-        launch_k_ccc(a, layout, grid, tr_pad, electron, active, counter, dose, s);
-    }
+    launch_k_ccc("ccc", a, layout, (int64_t)n, tr_pad, electron, active, counter, dose, s);
 }
 
 void dh_launch_ccc_list(const dh_ccc_args &a, const float *terma, const float *density,

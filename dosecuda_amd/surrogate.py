@@ -720,7 +720,8 @@ def evaluate(net: SurrogateNet, model, n_beams=7, stride=4, tol=0.01, reps=5, lo
                                terma_out=terma, electron_out=el)
         torch.cuda.synchronize()
         engine.prof_enable(False)
-        stage = {k: engine.prof_get(k)[0] / reps for k in ("raytrace", "terma", "pack_tr", "ccc_compact", "ccc")}
+        stage = {k: engine.prof_get(k)[0] / reps for k in ("raytrace", "terma", "pack_tr", "ccc_compact", "ccc",
+                                                           "ccc_finish")}
         t_net = _time(lambda: sur.predict(terma, rho, el, ga, 0.0, 100.0, sp), reps)
         dn = sur.predict(terma, rho, el, ga, 0.0, 100.0, sp)
         tm = {}
@@ -734,7 +735,8 @@ def evaluate(net: SurrogateNet, model, n_beams=7, stride=4, tol=0.01, reps=5, lo
         torch.cuda.synchronize()
         engine.prof_enable(False)
         sstage = {k: engine.prof_get(k)[0] / reps for k in ("sur_pack", "nn_conv3d", "sur_finish", "pack_tr",
-                                                           "sur_lattice", "ccc_list", "sur_fallback")}
+                                                           "sur_lattice", "ccc_list", "ccc_finish",
+                                                           "sur_fallback")}
         gn, _ = gamma(dn, plain, sp)
         gc, _ = gamma(dc, plain, sp)
         pm = float(plain.max())
@@ -778,8 +780,8 @@ def evaluate(net: SurrogateNet, model, n_beams=7, stride=4, tol=0.01, reps=5, lo
     res["plan_gamma_checked"] = gamma(sums["checked"], sums["plain"], sp)[0]
     B = res["beams"]
     mean = lambda k: float(np.mean([b[k] for b in B]))
-    sup_stage = float(np.mean([b["plain_stage_ms"]["ccc"] + b["plain_stage_ms"]["ccc_compact"] +
-                               b["plain_stage_ms"]["pack_tr"] for b in B]))
+    sup_stage = float(np.mean([b["plain_stage_ms"]["ccc"] + b["plain_stage_ms"]["ccc_finish"] +
+                               b["plain_stage_ms"]["ccc_compact"] + b["plain_stage_ms"]["pack_tr"] for b in B]))
     res.update(plain_beam_ms=mean("plain_beam_ms"), plain_superposition_ms=sup_stage,
                surrogate_net_ms=mean("surrogate_net_ms"), surrogate_checked_ms=mean("surrogate_checked_ms"),
                min_gamma_net=min(b["gamma_net"] for b in B), min_gamma_checked=min(b["gamma_checked"] for b in B),
