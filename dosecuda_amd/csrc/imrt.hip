@@ -301,9 +301,9 @@ k_ccc_compact(dh_geom g, dh_pad p, const void *__restrict__ tr, int *__restrict_
 // Work split: a work item is 64 consecutive listed voxels (one per lane) x
 // one of 4 azimuth groups; group q marches azimuths 3q..3q+2 of all six zenith
 // rows (equal work).  All 64 lanes then sample the same cone direction from
-// neighbouring voxels, so a wave's gathers touch few cache lines (the gathers
-// are bound by the lines moved into L1, not by VALU), and the 4 groups keep
-// work items short for the tail.  Each group's compensated sum goes to its own
+// neighbouring voxels, so a wave's gathers touch few cache lines, and the 4
+// groups keep work items short for the tail.  The loop is co-limited by the
+// texture address/data pipe (16 cycles per 16-byte wave64 load) and VALU.  Each group's compensated sum goes to its own
 // partial buffer; k_ccc_finish adds them as (g0 + g1) + (g2 + g3), so the
 // result is deterministic.
 constexpr int kVoxPerItem = 64;
