@@ -6,8 +6,9 @@ the call surface of DoseCUDA's ``dose_kernels`` extension.
 * ``dosecuda_amd.engine``        device-pointer API (torch tensors, streams);
 * ``dosecuda_amd.beam_models``   loaders for the unchanged lookup tables;
 * ``dosecuda_amd.plan``          dose grids, plans and beams with the reference's
-  class names, computing plan sums on the device; ``dosecuda_amd.nrrd``
-  reads and writes the NRRD exports;
+  class names, computing plan sums on the device; ``dosecuda_amd.nrrd``,
+  ``dosecuda_amd.nifti`` and ``dosecuda_amd.dicom`` read and write the
+  NRRD, NIfTI and DICOM (CT, RT Dose, RT Structure Set, RT (Ion) Plan) files;
 * ``dosecuda_amd.phantoms``      seeded synthetic phantoms and fields.
 """
 __version__ = "0.1.0"

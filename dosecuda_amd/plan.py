@@ -28,19 +28,24 @@ callers that go on computing on the GPU.
 Inputs are read with the drop-in's checks (dose_kernels' attribute rules), so
 a malformed beam or model raises what ``dose_kernels`` raises.
 
-The DICOM and resampling methods need pydicom / SimpleITK, which this build
-does not depend on; they raise NotImplementedError naming the missing
-package.  NRRD files are read and written by ``dosecuda_amd.nrrd``.
+DICOM files (CT series, RT Dose, RT Structure Set, RT Plan, RT Ion Plan) are
+read and written by ``dosecuda_amd.dicom`` (uncompressed little-endian
+transfer syntaxes), NRRD by ``dosecuda_amd.nrrd`` and NIfTI by
+``dosecuda_amd.nifti``; the CT resampling methods interpolate linearly like
+SimpleITK's ResampleImageFilter (half-voxel border, -1000 HU outside).
 """
 from __future__ import annotations
 
 import os
 import sys
+from struct import error as struct_error
 
 import numpy as np
 
 from . import beam_models as _bm
+from . import dicom as _dcm
 from . import dose_kernels
+from . import nifti as _nifti
 from . import nrrd as _nrrd
 from .beam_models import VolumeObject
 
@@ -49,8 +54,58 @@ __all__ = ["VolumeObject", "DoseGrid", "Beam", "Plan", "IMPTBeamModel", "IMPTDos
            "IMRTBeam", "IMRTPlan"]
 
 
-def _needs(pkg: str, what: str):
-    raise NotImplementedError("%s needs %s, which this build does not depend on" % (what, pkg))
+def _axis_taps(n_in, o_in, s_in, n_out, o_out, s_out):
+    """Per output index along one axis: the two input indices, the weight
+    of the second and whether the point is inside the input.  The continuous
+    index c = (o_out + s_out i - o_in) / s_in is inside for -0.5 <= c <
+    n_in - 0.5; within the half-voxel border the neighbours clamp to the edge
+    (ITK's linear interpolator and ResampleImageFilter)."""
+    c = (float(o_out) + float(s_out) * np.arange(int(n_out), dtype=np.float64) - float(o_in)) / float(s_in)
+    inside = (c >= -0.5) & (c < n_in - 0.5)
+    f = np.floor(c)
+    w = c - f
+    i0 = np.clip(f, 0, n_in - 1).astype(np.int64)
+    i1 = np.clip(f + 1, 0, n_in - 1).astype(np.int64)
+    return i0, i1, w, inside
+
+
+def _resample_linear(vol, origin, spacing, new_origin, new_spacing, new_size_xyz, default=-1000.0):
+    """Trilinear resampling of a (z, y, x) volume with identity direction
+    onto the grid new_origin + new_spacing * index, index < new_size (x, y,
+    z order, as SimpleITK takes them); points outside the input take
+    ``default``.  Separable: x, then y, then z, in float64."""
+    v = np.asarray(vol, dtype=np.float64)
+    nz, ny, nx = v.shape
+    tx = _axis_taps(nx, origin[0], spacing[0], new_size_xyz[0], new_origin[0], new_spacing[0])
+    ty = _axis_taps(ny, origin[1], spacing[1], new_size_xyz[1], new_origin[1], new_spacing[1])
+    tz = _axis_taps(nz, origin[2], spacing[2], new_size_xyz[2], new_origin[2], new_spacing[2])
+    a = v[:, :, tx[0]] * (1.0 - tx[2]) + v[:, :, tx[1]] * tx[2]
+    a = a[:, ty[0], :] * (1.0 - ty[2])[None, :, None] + a[:, ty[1], :] * ty[2][None, :, None]
+    a = a[tz[0], :, :] * (1.0 - tz[2])[:, None, None] + a[tz[1], :, :] * tz[2][:, None, None]
+    inside = tz[3][:, None, None] & ty[3][None, :, None] & tx[3][None, None, :]
+    return np.where(inside, a, default).astype(np.single)
+
+
+def _ct_series(ct_dir):
+    """The image slices of the first series in ct_dir (files in name
+    order; the series of the first file that holds pixel data)."""
+    slices = []
+    series = None
+    for f in _dcm.list_files(ct_dir):
+        try:
+            ds = _dcm.read(f, force=True)
+        except (ValueError, NotImplementedError, struct_error):
+            continue
+        if "PixelData" not in ds or "ImagePositionPatient" not in ds:
+            continue
+        uid = ds.get("SeriesInstanceUID")
+        if series is None:
+            series = uid
+        if uid == series:
+            slices.append(ds)
+    if not slices:
+        raise ValueError("no DICOM image series found in %s" % ct_dir)
+    return slices
 
 
 def _torch_device(gpu_id):
@@ -130,16 +185,66 @@ class DoseGrid:
         self.size = np.array(self.HU.shape)
 
     def loadCTDCM(self, ct_path):
-        _needs("pydicom and SimpleITK", "loadCTDCM")
+        """A CT series directory (plan.py:35-49): slices sorted by
+        ImagePositionPatient z, HU = stored value * RescaleSlope +
+        RescaleIntercept clipped below at -1000; origin = the first slice's
+        position, spacing = (column spacing, row spacing, distance between
+        the first two slices)."""
+        slices = _ct_series(ct_path)
+        slices.sort(key=lambda d: float(d.ImagePositionPatient[2]))
+        hu = np.empty((len(slices), int(slices[0].Rows), int(slices[0].Columns)), dtype=np.float64)
+        for k, d in enumerate(slices):
+            px = d.pixel_array
+            if px.shape != hu.shape[1:]:
+                raise ValueError("CT slices of different sizes in %s" % ct_path)
+            hu[k] = px * float(d.get("RescaleSlope", 1.0) or 1.0) + float(d.get("RescaleIntercept", 0.0) or 0.0)
+        ps = slices[0].PixelSpacing
+        if len(slices) > 1:
+            dz = abs(float(slices[1].ImagePositionPatient[2]) - float(slices[0].ImagePositionPatient[2]))
+        else:
+            dz = float(slices[0].get("SliceThickness", 1.0) or 1.0)
+        self.origin = np.array([float(v) for v in slices[0].ImagePositionPatient])
+        self.spacing = np.array([float(ps[1]), float(ps[0]), dz])
+        self.HU = np.clip(np.array(hu, dtype=np.single), -1000.0, None)
+        self.size = np.array(self.HU.shape)
 
     def resampleCT(self, new_spacing, new_size, new_origin):
-        _needs("SimpleITK", "resampleCT")
+        """Linear resampling onto origin/spacing/size (x, y, z), -1000 HU
+        outside the CT (plan.py:51-67)."""
+        self.HU = _resample_linear(self.HU, self.origin, self.spacing, new_origin, new_spacing, new_size)
+        self.origin = new_origin
+        self.spacing = new_spacing
+        self.size = np.array(self.HU.shape)
 
     def resampleCTfromSpacing(self, spacing):
-        _needs("SimpleITK", "resampleCTfromSpacing")
+        """Isotropic resampling from the same origin; each axis keeps
+        int(extent / spacing) voxels (plan.py:69-89)."""
+        sp_new = (spacing, spacing, spacing)
+        sz_new = (int(self.size[2] * self.spacing[0] / sp_new[0]),
+                  int(self.size[1] * self.spacing[1] / sp_new[1]),
+                  int(self.size[0] * self.spacing[2] / sp_new[2]))
+        self.HU = _resample_linear(self.HU, self.origin, self.spacing, self.origin, sp_new, sz_new)
+        self.spacing = sp_new
+        self.size = np.array(self.HU.shape)
 
     def resampleCTfromReferenceDose(self, ref_dose_path):
-        _needs("pydicom and SimpleITK", "resampleCTfromReferenceDose")
+        """Resampling onto an RT Dose grid (plan.py:91-116): spacing
+        (PixelSpacing[0], PixelSpacing[1], frame offset step), origin its
+        ImagePositionPatient, size its pixel array's."""
+        ref_dose = _dcm.read(ref_dose_path, force=True)
+        gfov = ref_dose.GridFrameOffsetVector
+        slice_thickness = float(gfov[1]) - float(gfov[0])
+        ref_spacing = np.array([float(ref_dose.PixelSpacing[0]), float(ref_dose.PixelSpacing[1]),
+                                slice_thickness])
+        ref_origin = np.array([float(v) for v in ref_dose.ImagePositionPatient])
+        shape = ref_dose.pixel_array.shape
+        if len(shape) == 2:
+            shape = (1,) + shape
+        self.HU = _resample_linear(self.HU, self.origin, self.spacing, ref_origin, ref_spacing,
+                                   (shape[2], shape[1], shape[0]))
+        self.size = np.array(self.HU.shape)
+        self.origin = ref_origin
+        self.spacing = ref_spacing
 
     def applyCouchModel(self, couch_wet=8.0):
         """HU override of the bottom 5 cm of rows so that they hold the
@@ -161,7 +266,33 @@ class DoseGrid:
 
     # --- exports -----------------------------------------------------------
     def writeDoseDCM(self, dose_path, ref_dose_path, dose_type="EFFECTIVE", individual_beams=False):
-        _needs("pydicom", "writeDoseDCM")
+        """The plan (or each beam's) dose times the RBE into a copy of an
+        RT Dose file: unsigned 16-bit at 500 per Gy, DoseGridScaling 0.002, a
+        new referenced plan UID (plan.py:125-163)."""
+        if not dose_path.endswith(".dcm"):
+            raise Exception("Dose path must have .dcm extension")
+        if dose_type == "EFFECTIVE":
+            RBE = 1.1
+        elif dose_type == "PHYSICAL":
+            RBE = 1.0
+        else:
+            raise Exception("Unknown dose type: %s" % dose_type)
+
+        def one(dose, summation, path):
+            ref_dose = _dcm.read(ref_dose_path)
+            ref_dose.SeriesDescription = ref_dose.SeriesDescription + "_DoseCUDA"
+            ref_dose.DoseSummationType = summation
+            ref_dose.DoseType = dose_type
+            ref_dose.ReferencedRTPlanSequence[0].ReferencedSOPInstanceUID = _dcm.generate_uid()
+            ref_dose.PixelData = np.array(dose * 500.0 * RBE, dtype=np.uint16)
+            ref_dose.DoseGridScaling = 0.002
+            _dcm.write(ref_dose, path)
+
+        if individual_beams:
+            for i, beam_dose in enumerate(self.beam_doses):
+                one(beam_dose, "BEAM", dose_path.replace(".dcm", "_beam%02i.dcm" % (i + 1)))
+        else:
+            one(self.dose, "PLAN", dose_path)
 
     def writeDoseNRRD(self, dose_path, individual_beams=False, dose_type="EFFECTIVE"):
         """Plan dose (or one file per beam) times the RBE: 1.1 for
@@ -188,7 +319,9 @@ class DoseGrid:
         _nrrd.write(ct_path, np.asarray(self.HU), self.origin, self.spacing)
 
     def writeCTNIFTI(self, ct_path):
-        _needs("SimpleITK", "writeCTNIFTI")
+        if not ct_path.endswith(".nii.gz"):
+            raise Exception("CT path must have .nii.gz extension")
+        _nifti.write(ct_path, np.asarray(self.HU), self.origin, self.spacing)
 
     # --- shared device plumbing --------------------------------------------
     def _check_isotropic(self):
@@ -300,13 +433,259 @@ class IMPTPlan(Plan):
                                             self.folder_rangeshifter_label)]
 
     def load_ct_series(self, ct_dir):
-        _needs("pydicom", "load_ct_series")
+        """All .dcm files of ct_dir, read and sorted by InstanceNumber
+        (plan_impt.py:298-305)."""
+        files = [os.path.join(ct_dir, f) for f in os.listdir(ct_dir) if f.lower().endswith(".dcm")]
+        dsets = [_dcm.read(f) for f in files]
+        dsets.sort(key=lambda ds: ds.InstanceNumber)
+        return dsets
 
     def writePlanDicom(self, ct_dir, output_file, rt_struct_file=None):
-        _needs("pydicom", "writePlanDicom")
+        """An RT Ion Plan for the plan's pencil-beam fields (plan_impt.py:307-603):
+        patient, study and frame of reference from the CT series, one fraction
+        group, and per field a pair of ion control points per energy layer (the
+        layer's spots and weights, then the same spots with zero weight)."""
+        import datetime
+        ct_series = self.load_ct_series(ct_dir)
+        first = ct_series[0]
+        ds = _dcm.Dataset()
+        sop_uid = _dcm.generate_uid()
+        ds.PatientName = first.PatientName
+        ds.PatientID = first.PatientID
+        ds.StudyInstanceUID = first.StudyInstanceUID
+        ds.SeriesInstanceUID = _dcm.generate_uid()
+        ds.FrameOfReferenceUID = first.FrameOfReferenceUID
+        ds.SOPClassUID = _dcm.RT_ION_PLAN_STORAGE
+        ds.SOPInstanceUID = sop_uid
+        ds.Modality = "RTPLAN"
+        ds.RTPlanLabel = "IMPT_PencilBeam"
+        ds.RTPlanDate = datetime.date.today().strftime("%Y%m%d")
+        ds.RTPlanTime = datetime.datetime.now().strftime("%H%M%S.%f")
+        ds.RTPlanGeometry = "PATIENT"
+        ds.ApprovalStatus = "UNAPPROVED"
+        ds.InstanceCreationDate = ds.RTPlanDate
+        ds.InstanceCreationTime = ds.RTPlanTime
+        ds.Manufacturer = "RaySearch Laboratories"
+        ds.PlanIntent = "CURATIVE"
+        ds.TreatmentProtocols = ["PencilBeamScanning"]
+        ds.PatientIdentityRemoved = "YES"
+        ds.DeidentificationMethod = "RayStation Custom Export"
+        ds.NumberOfBeams = len(self.beam_list)
+        ds.NumberOfBrachyApplicationSetups = 0
+
+        rss = _dcm.Dataset()
+        rss.ReferencedSOPClassUID = _dcm.RT_STRUCT_STORAGE
+        rss.ReferencedSOPInstanceUID = _dcm.generate_uid()
+        ds.ReferencedStructureSetSequence = [rss]
+
+        def dose_reference(number, roi, description):
+            dr = _dcm.Dataset()
+            dr.DoseReferenceNumber = number
+            dr.ReferencedROINumber = roi
+            dr.DoseReferenceStructureType = "VOLUME"
+            dr.DoseReferenceDescription = description
+            dr.DoseReferenceType = "TARGET"
+            dr.TargetPrescriptionDose = 60
+            dr.TargetUnderdoseVolumeFraction = 2
+            dr.add_new((0x4001, 0x0010), "LO", "RAYSEARCHLABS 2.0")
+            dr.add_new((0x4001, 0x1011), "UN", "{0}".format(60).encode())
+            return dr
+
+        if rt_struct_file:
+            rt_ds = _dcm.read(rt_struct_file)
+            rss.ReferencedSOPInstanceUID = rt_ds.SOPInstanceUID
+            ptv_roi_num = None
+            for roi in rt_ds.StructureSetROISequence:
+                if roi.ROIName == "PTV":
+                    ptv_roi_num = roi.ROINumber
+                    break
+            if ptv_roi_num is None:
+                raise ValueError("No structure labelled 'PTV' found in the provided RT structure set.")
+            ds.DoseReferenceSequence = [dose_reference(1, ptv_roi_num, "PTV Prescription")]
+        else:
+            ds.DoseReferenceSequence = [dose_reference(i + 1, i + 1, f"Dose Reference {i + 1}")
+                                        for i in range(2)]
+
+        fg = _dcm.Dataset()
+        fg.FractionGroupNumber = 1
+        fg.NumberOfFractionsPlanned = self.n_fractions
+        fg.NumberOfBeams = len(self.beam_list)
+        fg.NumberOfBrachyApplicationSetups = 0
+        referenced_beams = []
+        ps = _dcm.Dataset()
+        ps.PatientPosition = "HFS"
+        ps.PatientSetupNumber = 1
+        ds.PatientSetupSequence = [ps]
+
+        ion_beams = []
+        for i, beam in enumerate(self.beam_list):
+            try:
+                model_index = list(self.dicom_rangeshifter_label.astype(str)).index(beam.dicom_rangeshifter_label)
+            except ValueError:
+                raise ValueError(f"Beam model not found for rangeshifter ID {beam.dicom_rangeshifter_label}")
+            beam_model = self.beam_models[model_index]
+            ion_beams.append(_ion_beam(i + 1, beam, beam_model, referenced_beams))
+        fg.ReferencedBeamSequence = referenced_beams
+        ds.FractionGroupSequence = [fg]
+        ds.IonBeamSequence = ion_beams
+        _dcm.write(ds, output_file)
 
     def readPlanDicom(self, plan_path):
-        _needs("pydicom", "readPlanDicom")
+        """Fields from an RT Ion Plan (plan_impt.py:605-643): range shifter
+        label ('0' without one), gantry / couch / isocentre of the first
+        control point, and the spots of every even-numbered control point,
+        energies matched to the first model's labels."""
+        ds = _dcm.read(plan_path, force=True)
+        n_beams = len(ds.IonBeamSequence)
+        self.n_fractions = float(ds.FractionGroupSequence[0].NumberOfFractionsPlanned)
+        self.beam_list = []
+        self.n_beams = 0
+        for i in range(n_beams):
+            ibs = ds.IonBeamSequence[i]
+            beam = IMPTBeam()
+            if ibs.NumberOfRangeShifters and ibs.RangeShifterSequence[0].RangeShifterID is not None:
+                beam.dicom_rangeshifter_label = ibs.RangeShifterSequence[0].RangeShifterID
+            else:
+                beam.dicom_rangeshifter_label = "0"
+            cp0 = ibs.IonControlPointSequence[0]
+            beam.gantry_angle = float(cp0.GantryAngle)
+            beam.couch_angle = float(cp0.PatientSupportAngle)
+            beam.iso = np.array(cp0.IsocenterPosition, dtype=np.single)
+            for j, cp in enumerate(ibs.IonControlPointSequence):
+                if j % 2 == 0:
+                    energy_id = self.beam_models[0].energyIDFromLabel(float(cp.NominalBeamEnergy))
+                    beam.addSpotData(cp, energy_id)
+            self.addBeam(beam)
+
+
+def _ion_control_point(index, beam, beam_model, energy_id, spots, cumulative, weights):
+    """One ion control point of an energy layer (plan_impt.py:506-581)."""
+    cp = _dcm.Dataset()
+    cp.ControlPointIndex = index
+    cp.NominalBeamEnergyUnit = "MEV"
+    cp.NominalBeamEnergy = float(beam_model.energy_labels[energy_id])
+    cp.GantryAngle = float(beam.gantry_angle)
+    cp.GantryRotationDirection = "NONE"
+    cp.BeamLimitingDeviceAngle = 0.0
+    cp.BeamLimitingDeviceRotationDirection = "NONE"
+    cp.PatientSupportAngle = float(beam.couch_angle)
+    cp.PatientSupportRotationDirection = "NONE"
+    cp.TableTopEccentricAngle = float(beam.couch_angle)
+    cp.TableTopEccentricRotationDirection = "NONE"
+    cp.TableTopPitchAngle = 0.0
+    cp.TableTopPitchRotationDirection = "NONE"
+    cp.TableTopRollAngle = 0.0
+    cp.TableTopRollRotationDirection = "NONE"
+    cp.GantryPitchAngle = 0.0
+    cp.GantryPitchRotationDirection = "NONE"
+    cp.TableTopVerticalPosition = ""
+    cp.TableTopLongitudinalPosition = ""
+    cp.TableTopLateralPosition = ""
+    cp.IsocenterPosition = [float(c) for c in beam.iso]
+    cp.SnoutPosition = 391.0
+    cp.MetersetRate = 480.0
+    cp.CumulativeMetersetWeight = float(cumulative)
+    cp.ScanSpotTuneID = "Standard"
+    cp.NumberOfScanSpotPositions = len(spots)
+    cp.ScanSpotPositionMap = spots[:, 0:2].flatten().tolist()
+    cp.ScanSpotMetersetWeights = list(weights)
+    width = beam_model.reference_sigmas[energy_id] * 2.0 * np.sqrt(2.0 * np.log(2.0)) * 10.0
+    cp.ScanningSpotSize = [width, width]
+    cp.NumberOfPaintings = 1
+    if beam.dicom_rangeshifter_label:
+        cp.RangeShifterSetting = "ON"
+    return cp
+
+
+def _ion_beam(number, beam, beam_model, referenced_beams):
+    """One Ion Beam Sequence item and its fraction-group reference
+    (plan_impt.py:426-601)."""
+    b = _dcm.Dataset()
+    b.BeamNumber = number
+    b.BeamName = f"PBS_Beam{number}"
+    b.BeamType = "STATIC"
+    b.RadiationType = "PROTON"
+    b.ScanMode = "MODULATED_SPEC"
+    b.ModulatedScanModeType = "STATIONARY"
+    b.TreatmentMachineName = "ProBeat_noMRF_"
+    b.Manufacturer = "Generic"
+    b.PrimaryDosimeterUnit = "MU"
+    b.TreatmentDeliveryType = "TREATMENT"
+    b.NumberOfWedges = 0
+    b.NumberOfCompensators = 0
+    b.NumberOfBoli = 0
+    b.NumberOfBlocks = 0
+    b.PatientSupportType = "TABLE"
+    b.PatientSupportID = "TABLE"
+    b.VirtualSourceAxisDistances = [float(beam_model.VSADX), float(beam_model.VSADY)]
+    snout = _dcm.Dataset()
+    snout.SnoutID = "0"
+    b.SnoutSequence = [snout]
+    b.NumberOfRangeShifters = 0
+    if beam.dicom_rangeshifter_label:
+        b.NumberOfRangeShifters = 1
+        rs = _dcm.Dataset()
+        rs.RangeShifterNumber = 1
+        rs.RangeShifterID = beam.dicom_rangeshifter_label
+        rs.RangeShifterType = "BINARY"
+        rs.AccessoryCode = beam.dicom_rangeshifter_label
+        b.RangeShifterSequence = [rs]
+    b.NumberOfLateralSpreadingDevices = 1
+    lsd = _dcm.Dataset()
+    lsd.LateralSpreadingDeviceNumber = 1
+    lsd.LateralSpreadingDeviceID = "0"
+    lsd.LateralSpreadingDeviceType = "SCATTERER"
+    b.LateralSpreadingDeviceSequence = [lsd]
+    b.NumberOfRangeModulators = 1
+    rm = _dcm.Dataset()
+    rm.RangeModulatorNumber = 0
+    rm.RangeModulatorID = "0"
+    rm.RangeModulatorType = "FIXED"
+    rm.RangeModulatorDescription = ""
+    b.RangeModulatorSequence = [rm]
+
+    total_beam_mu = np.sum(beam.spot_list[:, 2])
+    rb = _dcm.Dataset()
+    rb.ReferencedBeamNumber = number
+    rb.BeamMeterset = float(total_beam_mu)
+    referenced_beams.append(rb)
+    b.FinalCumulativeMetersetWeight = float(total_beam_mu)
+    b.add_new((0x4001, 0x0010), "LO", "RAYSEARCHLABS 2.0")
+    b.add_new((0x4001, 0x1002), "UN", b"Constant 1.1")
+    b.add_new((0x4001, 0x1012), "UN", b"ProBeat_noMRF_")
+    b.add_new((0x4001, 0x1033), "UN", b"RBE_NOT_INCLUDED")
+    b.add_new((0x4001, 0x1003), "UN", b"20181030034922.000000 ")
+
+    lsds = _dcm.Dataset()
+    lsds.LateralSpreadingDeviceSetting = "IN"
+    lsds.IsocenterToLateralSpreadingDeviceDistance = 2275.0
+    lsds.LateralSpreadingDeviceWaterEquivalentThickness = 0.0
+    lsds.ReferencedLateralSpreadingDeviceNumber = 1
+    rms = _dcm.Dataset()
+    rms.ReferencedRangeModulatorNumber = 0
+    rdrs = []
+    for j in range(2):
+        rdr = _dcm.Dataset()
+        rdr.ReferencedDoseReferenceNumber = j + 1
+        rdr.CumulativeDoseReferenceCoefficient = None
+        rdrs.append(rdr)
+
+    cps = []
+    cumulative_mu = 0.0
+    for energy_id in np.unique(beam.spot_list[:, 3]).astype(int):
+        spots = beam.spot_list[beam.spot_list[:, 3] == energy_id]
+        layer_mu = np.sum(spots[:, 2])
+        for cumulative, weights in ((cumulative_mu, spots[:, 2].tolist()),
+                                    (cumulative_mu + layer_mu, [0.0] * len(spots))):
+            cp = _ion_control_point(len(cps), beam, beam_model, energy_id, spots, cumulative, weights)
+            cp.LateralSpreadingDeviceSettingsSequence = [lsds]
+            cp.RangeModulatorSettingsSequence = [rms]
+            cp.ReferencedDoseReferenceSequence = rdrs
+            cps.append(cp)
+        cumulative_mu += layer_mu
+    b.IonControlPointSequence = cps
+    b.NumberOfControlPoints = len(cps)
+    return b
 
 
 def _proton_field(model, vol, beam):
@@ -482,7 +861,69 @@ class IMRTPlan(Plan):
         beam_model.validate_parameters()
 
     def readPlanDicom(self, plan_path):
-        _needs("pydicom", "readPlanDicom")
+        """Treatment beams of an RT Plan (plan_imrt.py:273-351).  Each control
+        point after the first becomes a control point with the aperture and
+        angles in force before it and MU = (its cumulative meterset weight -
+        the previous one) x the first referenced beam's meterset; jaws and MLC
+        come from the beam-limiting device items as the reference indexes them
+        (X and Y jaws from item 1, MLCX from item 2)."""
+        ds = _dcm.read(plan_path)
+        total_mu = ds.FractionGroupSequence[0].ReferencedBeamSequence[0].BeamMeterset
+        self.n_fractions = float(ds.FractionGroupSequence[0].NumberOfFractionsPlanned)
+        self.beam_list = []
+        self.n_beams = 0
+        bm = self.beam_models[0]
+
+        def limits(cp):
+            xjaws, yjaws, mlc = None, None, None
+            seq = cp.BeamLimitingDevicePositionSequence
+            for ps in seq:
+                if ps.RTBeamLimitingDeviceType == "X":
+                    xjaws = np.array(seq[1].LeafJawPositions, dtype=np.single)
+                if ps.RTBeamLimitingDeviceType == "Y":
+                    yjaws = np.array(seq[1].LeafJawPositions, dtype=np.single)
+                if ps.RTBeamLimitingDeviceType == "MLCX":
+                    mlc = np.array(seq[2].LeafJawPositions, dtype=np.single)
+            return xjaws, yjaws, mlc
+
+        for beam in ds.BeamSequence:
+            if beam.TreatmentDeliveryType != "TREATMENT":
+                continue
+            imrt_beam = IMRTBeam()
+            for cp in beam.ControlPointSequence:
+                if cp.ControlPointIndex == 0:
+                    imrt_beam.dicom_energy_label = cp.NominalBeamEnergy
+                    imrt_beam.iso = np.array([cp.IsocenterPosition[0], cp.IsocenterPosition[1],
+                                              cp.IsocenterPosition[2]], dtype=np.single)
+                    cumulative_mu = 0.0
+                    ca = cp.BeamLimitingDeviceAngle
+                    ta = cp.PatientSupportAngle
+                    ga = cp.GantryAngle
+                    xjaws, yjaws, mlc = limits(cp)
+                    continue
+                mu = cp.CumulativeMetersetWeight - cumulative_mu
+                if mlc is not None:
+                    mlc = np.reshape(mlc, (2, bm.n_mlc_pairs))
+                else:
+                    mlc = np.zeros((2, bm.n_mlc_pairs), dtype=np.single)
+                    if xjaws is not None:
+                        mlc[0, :] = xjaws[0]
+                        mlc[1, :] = xjaws[1]
+                    if yjaws is not None:
+                        for i in range(bm.n_mlc_pairs):
+                            if bm.mlc_offsets[i] < yjaws[0] or bm.mlc_offsets[i] > yjaws[1]:
+                                mlc[0, i] = 0.0
+                                mlc[1, i] = 0.0
+                mlc = np.transpose(np.array(np.vstack((mlc, bm.mlc_offsets.reshape(1, -1),
+                                                       bm.mlc_widths.reshape(1, -1))), dtype=np.single))
+                imrt_beam.addControlPoint(IMRTControlPoint(imrt_beam.iso, mu * total_mu, mlc, ga, ca, ta,
+                                                           xjaws, yjaws))
+                if "GantryAngle" in cp:
+                    ga = cp.GantryAngle
+                if "BeamLimitingDevicePositionSequence" in cp:
+                    xjaws, yjaws, mlc = limits(cp)
+                cumulative_mu = cp.CumulativeMetersetWeight
+            self.addBeam(imrt_beam)
 
     def addSquareField(self, dicom_energy_label='6', dimx=10, dimy=10, mu=100, gantry_angle=0.0,
                        collimator_angle=0.0, table_angle=0.0):

@@ -259,13 +259,20 @@ def test_photon_energy_validation():
         e.validate_parameters()
 
 
-def test_dicom_and_resampling_raise_clearly():
+def test_dicom_inputs_that_are_missing_raise(tmp_path):
+    """The DICOM readers (tests/test_dicom.py covers them) raise on missing
+    files and on directories without an image series."""
     g = P.DoseGrid()
-    for call in (lambda: g.loadCTDCM("x"), lambda: g.resampleCTfromSpacing(2.0),
-                 lambda: g.writeDoseDCM("a.dcm", "b.dcm"), lambda: P.IMPTPlan().readPlanDicom("x"),
-                 lambda: P.IMRTPlan().readPlanDicom("x"), lambda: g.writeCTNIFTI("x.nii.gz")):
-        with pytest.raises(NotImplementedError, match="needs"):
+    for call in (lambda: g.loadCTDCM(str(tmp_path / "none")),
+                 lambda: g.resampleCTfromReferenceDose(str(tmp_path / "none.dcm")),
+                 lambda: P.IMPTPlan().readPlanDicom(str(tmp_path / "none.dcm")),
+                 lambda: P.IMRTPlan().readPlanDicom(str(tmp_path / "none.dcm"))):
+        with pytest.raises(FileNotFoundError):
             call()
+    (tmp_path / "empty").mkdir()
+    (tmp_path / "empty" / "notes.txt").write_text("no images here")
+    with pytest.raises(ValueError, match="no DICOM image series"):
+        g.loadCTDCM(str(tmp_path / "empty"))
 
 
 # ---------------------------------------------------------------------------
