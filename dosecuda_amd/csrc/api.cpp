@@ -137,6 +137,26 @@ extern "C" int dh_stage_smooth_dev(int gpu_id, const dh_grid *grid, const dh_pro
     });
 }
 
+// HU -> RLSP / density on the device (plan_impt.py:112-122, plan_imrt.py:114-124).
+extern "C" int dh_hu_convert_dev(int gpu_id, const void *d_hu, int32_t hu_is_double, int64_t n,
+                                 const double *xp, const double *fp, int32_t m, float *d_out, void *stream)
+{
+    return guarded([&] {
+        dh_device_enter(gpu_id);
+        DH_REQUIRE(n >= 0, "negative voxel count");
+        DH_REQUIRE(n == 0 || (d_hu && d_out), "null argument");
+        DH_REQUIRE(xp && fp, "null HU table");
+        DH_REQUIRE(m >= 1 && m <= DH_HU_TABLE_MAX, "HU table must have 1..128 nodes");
+        dh_hu_table t;
+        t.m = m;
+        for (int i = 0; i < m; i++) {
+            t.x[i] = xp[i];
+            t.y[i] = fp[i];
+        }
+        dh_launch_hu_table(t, d_hu, hu_is_double != 0, n, d_out, (hipStream_t)stream);
+    });
+}
+
 extern "C" int dh_stage_raytrace_dev(int gpu_id, const dh_grid *grid, const float iso[3],
                                      double gantry, double couch, float src_dist, float border,
                                      const float *d_vol, float *d_wet_out, void *stream)

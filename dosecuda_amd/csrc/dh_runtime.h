@@ -138,6 +138,16 @@ struct dh_terma_args {
     float mu_e[DH_MAX_SPECTRAL], pw[DH_MAX_SPECTRAL], sw[DH_MAX_SPECTRAL];
 };
 
+// Piecewise-linear HU table (ascending nodes) for the device conversion.
+constexpr int DH_HU_TABLE_MAX = 128;
+struct dh_hu_table {
+    int m;
+    double x[DH_HU_TABLE_MAX], y[DH_HU_TABLE_MAX];
+};
+// out[v] = float32(np.interp(hu[v], t.x, t.y)); hu is float32 or float64
+void dh_launch_hu_table(const dh_hu_table &t, const void *hu, bool hu_double, int64_t n, float *out,
+                        hipStream_t s);
+
 struct dh_ccc_args {
     dh_geom g;
     dh_pad p;

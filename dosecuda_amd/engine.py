@@ -29,6 +29,28 @@ def _g(shape, spacing, origin):
     return _lib.grid(shape, spacing, origin)
 
 
+def hu_convert(hu: torch.Tensor, xp, fp, out=None, stream=None) -> torch.Tensor:
+    """HU -> RLSP or density on the device: float32(np.interp(hu, xp, fp))
+    bit for bit (plan_impt.py:112-122, plan_imrt.py:114-124).  ``hu`` is a
+    contiguous float32 or float64 tensor on the GPU; xp / fp the machine's
+    HU table (``beam_models.hu_table``)."""
+    if not (hu.is_cuda and hu.is_contiguous() and hu.dtype in (torch.float32, torch.float64)):
+        raise ValueError("expected a contiguous float32 or float64 tensor on the GPU")
+    dev = hu.device.index if hu.device.index is not None else 0
+    out = torch.empty(hu.shape, dtype=torch.float32, device=hu.device) if out is None else out
+    if out.shape != hu.shape:
+        raise ValueError("out must have the shape of hu")
+    _dev(out)
+    x = np.ascontiguousarray(np.asarray(xp, dtype=np.float64))
+    y = np.ascontiguousarray(np.asarray(fp, dtype=np.float64))
+    if x.ndim != 1 or x.shape != y.shape:
+        raise ValueError("xp and fp must be 1-D arrays of one length")
+    _lib.check(_lib.load().dh_hu_convert_dev(dev, _lib.ptr(hu), int(hu.dtype == torch.float64), hu.numel(),
+                                             x.ctypes.data, y.ctypes.data, x.size, _lib.ptr(out),
+                                             _stream(stream)))
+    return out
+
+
 def proton_wet(rlsp: torch.Tensor, spacing, origin, iso, gantry, couch, vsadx, vsady,
                out: torch.Tensor = None, raw_out: torch.Tensor = None, stream=None):
     """Ray trace + lateral smoothing; returns the smoothed WET (cm)."""

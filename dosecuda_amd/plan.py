@@ -131,20 +131,49 @@ def _volume_object(data, origin, spacing):
 class _Resident:
     """A plan's volume and dose sum in HBM, uploaded at the first kernel
     call, so that the input errors the reference raises before any GPU work
-    are raised here before it too."""
+    are raised here before it too.  With ``table`` = (HU nodes, values) the
+    data is the HU volume: it is uploaded and converted on the device
+    (float32(np.interp) bit for bit), which replaces the host conversion."""
 
-    def __init__(self, data, gpu_id):
-        self.data, self.gpu_id, self.torch = data, gpu_id, None
+    def __init__(self, data, gpu_id, table=None):
+        self.data, self.gpu_id, self.table, self.torch = data, gpu_id, table, None
 
     def up(self):
         if self.torch is None:
             self.torch, self.dev = _torch_device(self.gpu_id)
-            self.vol = self.torch.from_numpy(np.ascontiguousarray(self.data)).to(self.dev)
-            self.sum = self.torch.zeros_like(self.vol)
             # torch's elementwise work on these tensors runs on this stream
             # too, so the engine's launches and the sums stay in order
             self.stream = self.torch.cuda.current_stream(self.dev)
+            if self.table is None:
+                self.vol = self.torch.from_numpy(np.ascontiguousarray(self.data)).to(self.dev)
+            else:
+                from . import engine
+                hu = np.asarray(self.data)
+                if hu.dtype not in (np.float32, np.float64):
+                    hu = hu.astype(np.float64)          # np.interp computes in float64
+                hu_t = self.torch.from_numpy(np.ascontiguousarray(hu)).to(self.dev)
+                self.vol = engine.hu_convert(hu_t, self.table[0], self.table[1], stream=self.stream)
+                del hu_t
+            self.sum = self.torch.zeros_like(self.vol)
         return self
+
+    def zeros(self):
+        return self.up().torch.zeros_like(self.vol)
+
+    def host(self):
+        return self.up().vol.cpu().numpy()
+
+
+def _checks_volume(hu, origin, spacing):
+    """A VolumeObject for the drop-in's argument checks before the converted
+    volume exists: voxel_data is a zero-stride float32 view with the HU
+    volume's shape, which the checks (rank, dtype) treat exactly as the
+    converted array."""
+    v = VolumeObject()
+    v.voxel_data = np.broadcast_to(np.float32(0.0), np.shape(hu))
+    v.origin = np.array(origin, dtype=np.single)
+    v.spacing = np.array(spacing, dtype=np.single)
+    return v
 
     def zeros(self):
         return self.up().torch.zeros_like(self.vol)
@@ -729,10 +758,10 @@ class IMPTDoseGrid(DoseGrid):
         from . import engine
         self.beam_doses = []
         self.dose = np.zeros(self.size, dtype=np.single)
-        self.RLSP = self.RLSPFromHU(plan.machine_name)
+        table = _bm.hu_table("protons", plan.machine_name)
         self._check_isotropic()
-        vol = self._rlsp_object()
-        res = _Resident(vol.voxel_data, gpu_id)
+        vol = _checks_volume(self.HU, self.origin, self.spacing)
+        res = _Resident(self.HU, gpu_id, table)
         beam_ts = []
         for beam in plan.beam_list:
             model = plan.beam_models[_model_index(plan.dicom_rangeshifter_label,
@@ -752,19 +781,27 @@ class IMPTDoseGrid(DoseGrid):
                 pp.close()
             beam_ts.append(beam_t)
             res.sum.add_(beam_t)
+        self.RLSP = res.host() if res.torch is not None else self.RLSPFromHU(plan.machine_name)
         self._finish(res, beam_ts, plan.n_fractions, scale_beams=True)
 
     def computeIMPTWET(self, plan, gpu_id=0):
-        """One smoothed-WET volume per field (plan_impt.py:162-190)."""
-        self.RLSP = self.RLSPFromHU(plan.machine_name)
-        vol = self._rlsp_object()
+        """One smoothed-WET volume per field (plan_impt.py:162-190); the RLSP
+        is converted on the device at the first field."""
+        table = _bm.hu_table("protons", plan.machine_name)
+        res = _Resident(self.HU, gpu_id, table)
+        vol = None
         beam_wets = []
         for beam in plan.beam_list:
             model = plan.beam_models[_model_index(plan.dicom_rangeshifter_label,
                                                   beam.dicom_rangeshifter_label,
                                                   "rangeshifter ID")]
+            if vol is None:
+                self.RLSP = res.host()
+                vol = self._rlsp_object()
             beam_wet = dose_kernels.proton_raytrace_cuda(model, vol, beam, gpu_id)
             beam_wets.append(_volume_object(beam_wet, self.origin, self.spacing))
+        if vol is None:
+            self.RLSP = self.RLSPFromHU(plan.machine_name)
         return beam_wets
 
     def computeIMPTPlanFromWET(self, plan, wet_object, beam, gpu_id=0):
@@ -956,10 +993,10 @@ class IMRTDoseGrid(DoseGrid):
         from . import engine
         self.beam_doses = []
         self.dose = np.zeros(self.size, dtype=np.single)
-        self.Density = self.DensityFromHU(plan.machine_name)
+        table = _bm.hu_table("photons", plan.machine_name)
         self._check_isotropic()
-        vol = _volume_object(self.Density, self.origin, self.spacing)
-        res = _Resident(vol.voxel_data, gpu_id)
+        vol = _checks_volume(self.HU, self.origin, self.spacing)
+        res = _Resident(self.HU, gpu_id, table)
         beam_ts = []
         cp_t = None
         for beam in plan.beam_list:
@@ -999,4 +1036,5 @@ class IMRTDoseGrid(DoseGrid):
             beam_t = res.zeros() if beam_t is None else beam_t
             beam_ts.append(beam_t)
             res.sum.add_(beam_t)
+        self.Density = res.host() if res.torch is not None else self.DensityFromHU(plan.machine_name)
         self._finish(res, beam_ts, plan.n_fractions, scale_beams=False)

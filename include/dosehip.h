@@ -163,6 +163,16 @@ int dh_influence_info(const dh_influence *f, int32_t *n_rows, int32_t *row_spot,
 int dh_influence_stats(const dh_influence *f, uint64_t *stats12, float *row_radius);
 int dh_influence_destroy(dh_influence *f);
 
+/* ---- HU conversion --------------------------------------------------- */
+/* Replaces the host conversions RLSPFromHU (plan_impt.py:112-122) and
+ * DensityFromHU (plan_imrt.py:114-124): d_out[v] = float32(np.interp(
+ * d_hu[v], xp, fp)) over the machine's HU table (m <= 128 ascending nodes;
+ * d_hu float32, or float64 when hu_is_double), evaluated on the device with
+ * np.interp's FP64 arithmetic, so the result equals the host conversion bit
+ * for bit. */
+int dh_hu_convert_dev(int gpu_id, const void *d_hu, int32_t hu_is_double, int64_t n, const double *xp,
+                      const double *fp, int32_t m, float *d_out, void *stream);
+
 /* ---- single stages on device pointers (stage-wise parity) ------------ */
 /* Unsmoothed WET of a volume sampled with the given border value, source
  * at src_dist from iso (CudaClasses.cu:180-228). */
