@@ -19,6 +19,8 @@
 //           voxel order and the output is deterministic.
 #include "dh_runtime.h"
 
+#include <type_traits>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -53,6 +55,7 @@ struct dh_infl_args {
     const float4 *factors;       // [layer][voxel] {ap, log2 P, ah, log2 H} (P, H >= 0 folded)
     uint32_t *col;
     float *val;
+    uint32_t nnz32;              // FILL with 32-bit positions: entries of the matrix
     int *queue;
     unsigned long long *stats;   // optional: lane visits, active visits, chunks, rows
 };
@@ -75,10 +78,15 @@ __device__ static inline void interval(float a, float b, float lo, float hi, int
     ihi = min(ihi, min(h, n - 1));
 }
 
-template <int S, int MODE>
+// P32 (FILL): write positions in 32 bits, bounded by the matrix's entry count
+// instead of each slot's end (COUNT and FILL evaluate the same FP32
+// expressions, so an entry never leaves its slot; the bound keeps every write
+// inside the arrays), which keeps the fill's per-spot state in fewer VGPRs.
+template <int S, int MODE, bool P32 = false>
 __global__ void __launch_bounds__(64)
 k_influence(dh_infl_args a)
 {
+    using PosT = typename std::conditional<P32, uint32_t, int64_t>::type;
     __shared__ int s_item;
     __shared__ int s_rlo[64], s_rex[64], s_rid[64];   // packed rows of a batch
     const dh_geom &g = a.g;
@@ -107,12 +115,12 @@ k_influence(dh_infl_args a)
         }
         float pk[S];
         int cnt[S];
-        int64_t cur64[S];
+        PosT cur64[S];
         float lth[S];
 #pragma unroll
         for (int s = 0; s < S; s++) {
             pk[s] = 0.0f; cnt[s] = 0;
-            cur64[s] = (MODE == INF_FILL && sid[s] >= 0) ? a.offsets[(size_t)sid[s] * a.C + chunk] : 0;
+            cur64[s] = (MODE == INF_FILL && sid[s] >= 0) ? (PosT)a.offsets[(size_t)sid[s] * a.C + chunk] : 0;
             lth[s] = log2f(th[s]) - 1.01f;
         }
         // Packed chunks of an item come in the same order in COUNT and FILL
@@ -130,7 +138,7 @@ k_influence(dh_infl_args a)
         int64_t lim[S];
 #pragma unroll
         for (int s = 0; s < S; s++)
-            lim[s] = (MODE == INF_FILL && sid[s] >= 0) ? a.offsets[(size_t)sid[s] * a.C + chunk + 1] : 0;
+            lim[s] = (MODE == INF_FILL && !P32 && sid[s] >= 0) ? a.offsets[(size_t)sid[s] * a.C + chunk + 1] : 0;
         // rows of this chunk
         const int nj = B.j1 - B.j0 + 1, nk = B.k1 - B.k0 + 1;
         const int64_t nrows = (int64_t)nj * nk;
@@ -249,12 +257,12 @@ k_influence(dh_infl_args a)
                                 span_last = pc;
                             }
                         } else {
-                            const int64_t pos = cur64[s] + __popcll(m & below);
-                            if (keep && pos < lim[s]) {
+                            const PosT pos = cur64[s] + (PosT)__popcll(m & below);
+                            if (keep && (P32 ? (uint32_t)pos < a.nnz32 : (int64_t)pos < lim[s])) {
                                 a.col[pos] = v;
                                 a.val[pos] = val;
                             }
-                            cur64[s] += __popcll(m);
+                            cur64[s] += (PosT)__popcll(m);
                         }
                     }
                 }
@@ -704,6 +712,18 @@ static void launch_kernels(dh_influence *f, int n_blocks, const float *d_wet, ui
     const unsigned grid = (unsigned)std::min(a.n_items, cus * 32);
     const char *name = MODE == INF_PEAK ? "influence_peak" : MODE == INF_COUNT ? "influence_count" : "influence_fill";
     dh_timed t(name, s);
+    // the fill takes 32-bit positions below 2^32 entries
+    const bool p32 = MODE == INF_FILL && f->nnz >= 0 && f->nnz < ((int64_t)1 << 32);
+    a.nnz32 = p32 ? (uint32_t)f->nnz : 0u;
+    if constexpr (MODE == INF_FILL) {
+        if (p32) {
+            if (f->S == 4) hipLaunchKernelGGL((k_influence<4, MODE, true>), dim3(grid), dim3(64), 0, s, a);
+            else if (f->S == 8) hipLaunchKernelGGL((k_influence<8, MODE, true>), dim3(grid), dim3(64), 0, s, a);
+            else hipLaunchKernelGGL((k_influence<16, MODE, true>), dim3(grid), dim3(64), 0, s, a);
+            DH_HIP(hipGetLastError());
+            return;
+        }
+    }
     if (f->S == 4) hipLaunchKernelGGL((k_influence<4, MODE>), dim3(grid), dim3(64), 0, s, a);
     else if (f->S == 8) hipLaunchKernelGGL((k_influence<8, MODE>), dim3(grid), dim3(64), 0, s, a);
     else hipLaunchKernelGGL((k_influence<16, MODE>), dim3(grid), dim3(64), 0, s, a);
