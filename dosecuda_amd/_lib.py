@@ -128,6 +128,8 @@ SIGNATURES = {
     "dh_sched_add_case": [_P, C.POINTER(Grid), _P, _P, _P],
     "dh_sched_add_photon": [_P, C.c_int32, C.POINTER(PhotonCP), C.POINTER(PhotonModel), _P, C.c_int32, _P,
                             C.c_int32, C.c_float, C.c_int32],
+    "dh_sched_add_photon_scaled": [_P, C.c_int32, C.POINTER(PhotonCP), C.POINTER(PhotonModel), _P, C.c_int32,
+                                   _P, C.c_int32, C.c_float, C.c_int32, C.c_float],
     "dh_sched_add_proton_wet": [_P, C.c_int32, C.POINTER(ProtonBeam), _P],
     "dh_sched_add_proton_dose": [_P, C.c_int32, _P, _P],
     "dh_sched_add_influence_fill": [_P, C.c_int32, _P, _P, _P, _P],

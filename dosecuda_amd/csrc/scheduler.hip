@@ -13,6 +13,8 @@
 // (dosemodule.cu:95-538).
 #include "dh_runtime.h"
 
+#include <cmath>
+
 #include <algorithm>
 #include <memory>
 #include <numeric>
@@ -26,10 +28,14 @@ __global__ void k_sched_fill(float *__restrict__ p, int64_t n)
         p[i] = 0.0f;
 }
 
-__global__ void k_sched_accum(float *__restrict__ dst, const float *__restrict__ src, int64_t n)
+// dst += src * scale: the product rounded to float32, then the sum (as a
+// float32 array times a float32 factor added to another); scale 1 is exact.
+__global__ void k_sched_accum(float *__restrict__ dst, const float *__restrict__ src, int64_t n, float scale)
 {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        dst[i] += src[i];
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float p = __fmul_rn(src[i], scale);
+        dst[i] = __fadd_rn(dst[i], p);
+    }
 }
 
 unsigned grid_of(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 4096)); }
@@ -76,6 +82,7 @@ struct job {
     int stride = 4;
     float tol = 0.01f;
     int dilate = 2;
+    float scale = 1.0f;          // factor of the photon dose before it is added
     // proton
     dh_proton_beam beam{};
     float *wet_out = nullptr;
@@ -169,7 +176,8 @@ void enqueue_job(dh_sched *s, job &J, scase &C, hipStream_t st, bool timing, boo
         float *tmp = (float *)dh_scratch(s->gpu, DH_SLOT_OUT1, sizeof(float) * (size_t)n);
         dh_photon_enqueue(s->gpu, &C.grid, &J.cp, &J.model.m, J.mlc.data(), (int)(J.mlc.size() / 4), C.vol, tmp,
                           J.net, J.stride, J.tol, J.dilate, st, reuse_wet);
-        hipLaunchKernelGGL(k_sched_accum, dim3(grid_of(n)), dim3(256), 0, st, C.dose, (const float *)tmp, n);
+        hipLaunchKernelGGL(k_sched_accum, dim3(grid_of(n)), dim3(256), 0, st, C.dose, (const float *)tmp, n,
+                           J.scale);
         break;
     }
     case J_PROTON_WET:
@@ -178,7 +186,7 @@ void enqueue_job(dh_sched *s, job &J, scase &C, hipStream_t st, bool timing, boo
     case J_PROTON_DOSE: {
         float *tmp = (float *)dh_scratch(s->gpu, DH_SLOT_OUT1, sizeof(float) * (size_t)n);
         dh_proton_plan_enqueue(J.plan, J.wet_in, tmp, st);
-        hipLaunchKernelGGL(k_sched_accum, dim3(grid_of(n)), dim3(256), 0, st, C.dose, (const float *)tmp, n);
+        hipLaunchKernelGGL(k_sched_accum, dim3(grid_of(n)), dim3(256), 0, st, C.dose, (const float *)tmp, n, 1.0f);
         break;
     }
     case J_INFL_FILL:
@@ -310,11 +318,13 @@ job &new_job(dh_sched *s, int32_t case_id, int kind, double est)
 
 }  // namespace
 
-extern "C" int dh_sched_add_photon(dh_sched *s, int32_t case_id, const dh_photon_cp *cp,
-                                   const dh_photon_model *model, const float *mlc_rows, int32_t n_pairs,
-                                   const dh_sur_net *net, int32_t stride, float tol, int32_t dilate)
+extern "C" int dh_sched_add_photon_scaled(dh_sched *s, int32_t case_id, const dh_photon_cp *cp,
+                                          const dh_photon_model *model, const float *mlc_rows, int32_t n_pairs,
+                                          const dh_sur_net *net, int32_t stride, float tol, int32_t dilate,
+                                          float scale)
 {
     return dh_guarded([&] {
+        DH_REQUIRE(std::isfinite(scale), "the dose scale must be finite");
         DH_REQUIRE(cp && model, "null argument");
         DH_REQUIRE(dilate >= 0 && dilate <= 2, "dilate must be 0, 1 or 2");
         DH_REQUIRE(n_pairs >= 0 && n_pairs <= DH_MAX_PAIRS && (n_pairs == 0 || mlc_rows), "bad MLC");
@@ -330,7 +340,15 @@ extern "C" int dh_sched_add_photon(dh_sched *s, int32_t case_id, const dh_photon
         J.stride = stride;
         J.tol = tol;
         J.dilate = dilate;
+        J.scale = scale;
     });
+}
+
+extern "C" int dh_sched_add_photon(dh_sched *s, int32_t case_id, const dh_photon_cp *cp,
+                                   const dh_photon_model *model, const float *mlc_rows, int32_t n_pairs,
+                                   const dh_sur_net *net, int32_t stride, float tol, int32_t dilate)
+{
+    return dh_sched_add_photon_scaled(s, case_id, cp, model, mlc_rows, n_pairs, net, stride, tol, dilate, 1.0f);
 }
 
 extern "C" int dh_sched_add_proton_wet(dh_sched *s, int32_t case_id, const dh_proton_beam *beam, float *d_wet_out)

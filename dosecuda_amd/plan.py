@@ -988,22 +988,28 @@ class IMRTDoseGrid(DoseGrid):
 
     def computeIMRTPlan(self, plan, gpu_id=0):
         """Sum over beams of Σ_cp outputFactor(cp) · CCCS dose(cp), times the
-        fraction count (plan_imrt.py:126-158).  The density is uploaded once;
-        each control point's dose is scaled and added on the device."""
+        fraction count (plan_imrt.py:126-158).  Every control point is read
+        and checked first (the reference's reads, in its order); then the
+        density is converted on the device and the beams run as cases of the
+        batching scheduler (two streams when there are several beams), each
+        accumulating its control points' doses times their float32 output
+        factors in order, so the per-beam and plan sums equal the reference
+        loop's bit for bit.  Control points of a beam with the same source
+        position share one ray trace."""
         from . import engine
+        from .scheduler import Scheduler
         self.beam_doses = []
         self.dose = np.zeros(self.size, dtype=np.single)
         table = _bm.hu_table("photons", plan.machine_name)
         self._check_isotropic()
         vol = _checks_volume(self.HU, self.origin, self.spacing)
         res = _Resident(self.HU, gpu_id, table)
-        beam_ts = []
-        cp_t = None
+        beams = []
         for beam in plan.beam_list:
             model = plan.beam_models[_model_index(plan.dicom_energy_label,
                                                   beam.dicom_energy_label, "beam energy")]
             params = None
-            beam_t = None
+            cps = []
             for cp in beam.cp_list:
                 output_factor = model.outputFactor(cp)
                 # photon_dose_cuda's reads in its order (dosemodule.cu:386-460):
@@ -1021,20 +1027,26 @@ class IMRTDoseGrid(DoseGrid):
                 ta = dose_kernels._getfloat(cp, "ta")
                 if mlc.shape[0] and mlc.shape[1] < 4:
                     raise ValueError("'mlc' rows must hold x1, x2, y_offset and y_width")
-                if beam_t is None:
-                    beam_t = res.zeros()
-                if cp_t is None:
-                    cp_t = res.zeros()
-                engine.photon_dose(res.vol, dose_kernels._spacing0(spacing),
-                                   dose_kernels._vec3(origin, "origin"),
-                                   dose_kernels._vec3(iso, "iso"), mu, ga, ca, ta, params, mlc,
-                                   out=cp_t, stream=res.stream)
                 # float32 product then float32 sum, as numpy < 2 evaluates
                 # beam_dose += cp_dose * output_factor
-                cp_t.mul_(float(np.float32(output_factor)))
-                beam_t.add_(cp_t)
-            beam_t = res.zeros() if beam_t is None else beam_t
-            beam_ts.append(beam_t)
+                cps.append((dose_kernels._spacing0(spacing), dose_kernels._vec3(origin, "origin"),
+                            dose_kernels._vec3(iso, "iso"), mu, ga, ca, ta, mlc,
+                            float(np.float32(output_factor))))
+            beams.append((params, cps))
+        beam_ts = [res.zeros() for _ in beams]
+        busy = [i for i, (_, cps) in enumerate(beams) if cps]
+        if busy:
+            sched = Scheduler(res.dev.index, streams=min(2, len(busy)))
+            try:
+                for i in busy:
+                    params, cps = beams[i]
+                    case = sched.add_case(res.vol, cps[0][0], cps[0][1], dose=beam_ts[i])
+                    for sp, org, iso, mu, ga, ca, ta, mlc, of in cps:
+                        sched.add_photon(case, iso, mu, ga, ca, ta, params, mlc, scale=of)
+                sched.run(graph=False, stream=res.stream)
+            finally:
+                sched.close()
+        for beam_t in beam_ts:
             res.sum.add_(beam_t)
         self.Density = res.host() if res.torch is not None else self.DensityFromHU(plan.machine_name)
         self._finish(res, beam_ts, plan.n_fractions, scale_beams=False)

@@ -53,16 +53,20 @@ class Scheduler:
         return int(cid.value)
 
     def add_photon(self, case: int, iso, mu, ga, ca, ta, model, mlc, surrogate=None, stride=4, tol=0.01,
-                   dilate=1):
+                   dilate=1, scale=1.0):
         """One photon control point; surrogate = a surrogate.NativeNet (the
-        checked surrogate replaces the cone superposition) or None (plain)."""
+        checked surrogate replaces the cone superposition) or None (plain).
+        ``model`` is a beam model or an engine.PhotonParams; the dose is
+        multiplied by ``scale`` (rounded to float32) before it is added to
+        the case dose, so a case sums its control points' scaled doses in
+        the order they were added."""
         keep: list = []
-        m = engine.photon_args(model, keep)
+        m = model.struct if isinstance(model, engine.PhotonParams) else engine.photon_args(model, keep)
         rows = np.ascontiguousarray(np.asarray(mlc, dtype=np.float32)[:, :4])
-        _lib.check(_lib.load().dh_sched_add_photon(self._h, int(case), _lib.photon_cp(iso, mu, ga, ca, ta), m,
-                                                   _lib.ptr(rows), rows.shape[0],
-                                                   surrogate.handle if surrogate is not None else None,
-                                                   int(stride), float(tol), int(dilate)))
+        _lib.check(_lib.load().dh_sched_add_photon_scaled(
+            self._h, int(case), _lib.photon_cp(iso, mu, ga, ca, ta), m, _lib.ptr(rows), rows.shape[0],
+            surrogate.handle if surrogate is not None else None, int(stride), float(tol), int(dilate),
+            float(np.float32(scale))))
         if surrogate is not None:
             self._keep.append(surrogate)
 

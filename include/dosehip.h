@@ -322,6 +322,15 @@ int dh_sched_add_case(dh_sched *sched, const dh_grid *grid, const float *d_volum
 int dh_sched_add_photon(dh_sched *sched, int32_t case_id, const dh_photon_cp *cp,
                         const dh_photon_model *model, const float *mlc_rows, int32_t n_pairs,
                         const dh_sur_net *net, int32_t stride, float tol, int32_t dilate);
+/* As dh_sched_add_photon, with the control point's dose multiplied by scale
+ * (rounded to float32) before it is added to the case dose: a case's jobs
+ * then sum Σ dose_cp * scale_cp in job order, as the plan loop does with the
+ * output factors (plan_imrt.py:140-153). No reference counterpart (it batches
+ * the plan loop). */
+int dh_sched_add_photon_scaled(dh_sched *sched, int32_t case_id, const dh_photon_cp *cp,
+                               const dh_photon_model *model, const float *mlc_rows, int32_t n_pairs,
+                               const dh_sur_net *net, int32_t stride, float tol, int32_t dilate,
+                               float scale);
 /* Smoothed WET of the case's RLSP for one proton field into d_wet_out. */
 int dh_sched_add_proton_wet(dh_sched *sched, int32_t case_id, const dh_proton_beam *beam,
                             float *d_wet_out);
