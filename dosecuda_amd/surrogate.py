@@ -421,11 +421,12 @@ def eval_loss(net: SurrogateNet, X, chans, Y, batch=4):
 
 
 def train(net: SurrogateNet, model, n_samples=20000, chunk=1000, passes=3, batch=4, lr=5e-4,
-          n=128, spacing=(2.0, 3.0), seed=1004, log=None):
+          n=128, spacing=(2.0, 3.0), seed=1004, log=None, fit_scale=True):
     """Online training: the plain CCCS generates `chunk` samples at a time,
     each chunk is used for `passes` epochs, `n_samples` in total.  Adam on the
     FP32 master weights, one-cycle learning rate; loss = mean squared error
-    over voxels of the field (target > 0) plus a small weight outside."""
+    over voxels of the field (target > 0) plus a small weight outside.
+    fit_scale=False keeps the network's output scale (continued training)."""
     n_chunks = max(1, n_samples // chunk)
     steps_per_chunk = max(1, chunk * passes // batch)
     total = n_chunks * steps_per_chunk
@@ -446,7 +447,7 @@ def train(net: SurrogateNet, model, n_samples=20000, chunk=1000, passes=3, batch
         torch.cuda.synchronize()
         tgen += time.time() - tg
         peak = Y.amax(dim=(1, 2, 3, 4))
-        if ci == 0:
+        if ci == 0 and fit_scale:
             net.out_scale.fill_(float(peak[peak > 0].mean()) if bool((peak > 0).any()) else 1.0)
         if log:
             log(f"chunk {ci}: target peak mean {float(peak.mean()):.3g} max {float(peak.max()):.3g}, "
@@ -805,6 +806,7 @@ def main(argv=None):
     t.add_argument("--lr", type=float, default=5e-4)
     t.add_argument("--seed", type=int, default=1004)
     t.add_argument("--out", default=WEIGHTS)
+    t.add_argument("--init", default=None, help="continue from these weights (keeps their output scale)")
     t.add_argument("--eval", action="store_true", help="evaluate after training")
     e = sub.add_parser("eval", help="gamma and timing on the 7-beam lung/bone phantom")
     e.add_argument("--weights", default=WEIGHTS)
@@ -818,10 +820,10 @@ def main(argv=None):
     log = lambda m: print(m, flush=True)
     model = PhotonBeamModel()
     if a.cmd == "train":
-        net = SurrogateNet().cuda()
+        net = load(a.init) if a.init else SurrogateNet().cuda()
         log(f"surrogate: {net.n_params()} parameters, {net.flops(64 ** 3) / 1e9:.1f} GFLOP per 64^3 patch")
         train(net, model, n_samples=a.samples, chunk=a.chunk, passes=a.passes, batch=a.batch, lr=a.lr,
-              seed=a.seed, log=log)
+              seed=a.seed, log=log, fit_scale=not a.init)
         os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
         save(net, a.out, compact=True)
         log(f"saved {a.out}")
