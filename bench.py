@@ -394,7 +394,7 @@ def main():
                 autotune.tune_surrogate(net, (64, 64, 64), reps=5)
                 log(f"surrogate: convolution variants tuned ({time.time() - t0:.1f} s)")
             log("surrogate: 7-beam plan, checked surrogate vs plain CCCS")
-            r = S.evaluate(net, PhotonBeamModel(), stride=4, tol=0.01, dilate=1, reps=5)
+            r = S.evaluate(net, PhotonBeamModel(), stride=3, tol=0.01, dilate=1, reps=5)
             conv_ms = float(np.mean([b["surrogate_stage_ms"]["nn_conv3d"] for b in r["beams"]]))
             tf = r["net_gflop"] / (conv_ms * 1e-3) / 1e3
             res["surrogate"] = dict(
@@ -425,7 +425,7 @@ def main():
         cases = phantoms.mixed_cases(pm, xm, seed=1005, n_cases=a.mixed_cases)
         nat = S.NativeNet(S.load(S.WEIGHTS)) if os.path.exists(S.WEIGHTS) else None
         log(f"mixed: {len(cases)} cases, surrogate {'on' if nat else 'off (no weights)'}")
-        W = mixed.MixedWorkload(cases, pm, xm, surrogate=nat, streams=4, stride=4, tol=0.01, dilate=1, log=log)
+        W = mixed.MixedWorkload(cases, pm, xm, surrogate=nat, streams=4, stride=3, tol=0.01, dilate=1, log=log)
         cnt = W.counts()
         log(f"mixed: set up in {W.setup_s:.1f} s: {cnt}")
         t0 = time.time()

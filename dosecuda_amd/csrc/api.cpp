@@ -501,7 +501,7 @@ static void photon_device(int dev, const dh_grid *grid, const dh_photon_cp *cp,
                           const dh_photon_model *m, const float *mlc_rows, int32_t n_pairs,
                           const float *d_density, float *d_dose, float *d_terma,
                           float *d_electron, float *d_wet, hipStream_t s,
-                          const dh_sur_net *net = nullptr, int stride = 4, float tol = 0.01f, int dilate = 2,
+                          const dh_sur_net *net = nullptr, int stride = 3, float tol = 0.01f, int dilate = 2,
                           bool reuse_wet = false)
 {
     size_t n = nvox_of(grid);

@@ -79,7 +79,7 @@ struct job {
     model_copy model;
     std::vector<float> mlc;
     const dh_sur_net *net = nullptr;
-    int stride = 4;
+    int stride = 3;
     float tol = 0.01f;
     int dilate = 2;
     float scale = 1.0f;          // factor of the photon dose before it is added

@@ -37,7 +37,7 @@ def _material(hu: torch.Tensor, conv) -> torch.Tensor:
 
 
 class MixedWorkload:
-    def __init__(self, cases: list, pmodel, xmodel, surrogate=None, streams=4, stride=4, tol=0.01, dilate=1,
+    def __init__(self, cases: list, pmodel, xmodel, surrogate=None, streams=4, stride=3, tol=0.01, dilate=1,
                  log=None):
         self.cases, self.pmodel, self.xmodel = cases, pmodel, xmodel
         self.surrogate = surrogate

@@ -52,7 +52,7 @@ class Scheduler:
         self._keep += [volume, dose]
         return int(cid.value)
 
-    def add_photon(self, case: int, iso, mu, ga, ca, ta, model, mlc, surrogate=None, stride=4, tol=0.01,
+    def add_photon(self, case: int, iso, mu, ga, ca, ta, model, mlc, surrogate=None, stride=3, tol=0.01,
                    dilate=1, scale=1.0):
         """One photon control point; surrogate = a surrogate.NativeNet (the
         checked surrogate replaces the cone superposition) or None (plain).

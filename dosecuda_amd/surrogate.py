@@ -546,7 +546,7 @@ class SurrogateCCCS:
     Everything runs on the device without host synchronisation, so a beam
     (or a plan of beams) can be captured into a HIP graph."""
 
-    def __init__(self, net: SurrogateNet, model, stride=4, tol=0.01, dilate=1):
+    def __init__(self, net: SurrogateNet, model, stride=3, tol=0.01, dilate=1):
         self.net = net.eval().prepare()
         self.out_scale = float(net.out_scale)
         self.model = model
@@ -692,7 +692,7 @@ def _time(fn, reps):
     return e0.elapsed_time(e1) / reps
 
 
-def evaluate(net: SurrogateNet, model, n_beams=7, stride=4, tol=0.01, reps=5, log=None, dilate=1):
+def evaluate(net: SurrogateNet, model, n_beams=7, stride=3, tol=0.01, reps=5, log=None, dilate=1):
     """Per-beam and plan gamma (2%/2 mm, 10% cut) of the surrogate alone and
     of the checked surrogate against the plain CCCS on the 128^3 lung/bone
     phantom with 10x10 cm fields, and the timings of both paths (eager, and
@@ -813,7 +813,7 @@ def main(argv=None):
     e.add_argument("--sweep", default=None, help="stride:tol[:dilate] settings, e.g. 4:0.01,4:0.02:1")
     for q in (t, e):
         q.add_argument("--json", default=None, help="write the evaluation as JSON here")
-        q.add_argument("--stride", type=int, default=4)
+        q.add_argument("--stride", type=int, default=3)
         q.add_argument("--tol", type=float, default=0.01)
         q.add_argument("--dilate", type=int, default=1, help="0 none, 1 face neighbours, 2 all 26")
     a = ap.parse_args(argv)
