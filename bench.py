@@ -243,8 +243,8 @@ def cpu_baseline_impt(st, budget_s):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of "
-                                   "this build (see DESIGN.md for how the path would shard)",
+        print(json.dumps({"error": "this engine runs on one GPU per process; a run over several "
+                                   "GPUs is not implemented (--gpus must be 1)",
                           "n_gpus": a.gpus}))
         sys.exit(2)
     import torch

@@ -1,5 +1,5 @@
 // surrogate.hip -- BF16 MFMA kernels of the learned CCCS surrogate (a small
-// 3-D convolutional encoder-decoder), plus the gamma index used to judge it.
+// 3-D convolutional encoder-decoder), plus the gamma index that scores it.
 //
 // Tensors are channels-last bf16: element (n, z, y, x, c) at
 // ((((n * D + z) * H + y) * W + x) * C + c).  Convolutions are implicit GEMMs

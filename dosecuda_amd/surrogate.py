@@ -19,7 +19,7 @@ lattice of voxels (every `stride`-th voxel along each axis among voxels with
 TERMA > 0.01); every lattice cell whose surrogate value differs from the
 plain value by more than `tol` of the maximum, together with its neighbour
 cells, is recomputed with the plain CCCS voxel by voxel.  The full plain
-CCCS and the 2%/2 mm gamma index are the judge, off the timed path.
+CCCS and the 2%/2 mm gamma index score it, off the timed path.
 
 No reference counterpart: DoseCUDA computes only the plain CCCS
 (IMRTClasses.cu:252-350).
