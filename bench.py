@@ -314,7 +314,7 @@ def main():
         ccc_ms = kt["ccc"][0] / max(kt["ccc"][1], 1)
         # algorithmic bytes of one k_ccc launch: the padded {TERMA, density}
         # volume once (8 B per padded voxel) + per active voxel: its list
-        # index and its 4 azimuth-group sums (20 B; k_ccc_finish adds them,
+        # index and its 12 azimuth sums (52 B; k_ccc_finish adds them,
         # the electron term and writes the dose)
         c = st["case"]
         from dosecuda_amd import _lib as L
@@ -325,7 +325,7 @@ def main():
         engine.photon_dose(st["d_dens"], c.spacing, c.origin, c.iso, 100.0, st["angles"][-1], 0.0,
                            0.0, st["pm"], st["mlc"], out=st["out"], terma_out=terma)
         n_act = int((~(terma <= 0.01)).sum().item())
-        bytes_launch = 8 * npad + 20 * n_act
+        bytes_launch = 8 * npad + 52 * n_act
         achieved = bytes_launch / (ccc_ms * 1e-3) / 1e9
         samples = n_act * 12 * sum(cone_steps(L0, c.spacing) for L0 in st["pm"].kernel[:, 5])
         # k_ccc is bound by its trilinear gathers: 8 corners x {TERMA,

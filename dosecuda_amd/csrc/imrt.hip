@@ -299,15 +299,17 @@ k_ccc_compact(dh_geom g, dh_pad p, const void *__restrict__ tr, int *__restrict_
 // the primary (exponential) and scatter (linear) recursions.  Cone results are
 // summed with Neumaier compensation.
 // Work split: a work item is 64 consecutive listed voxels (one per lane) x
-// one of 4 azimuth groups; group q marches azimuths 3q..3q+2 of all six zenith
-// rows (equal work).  All 64 lanes then sample the same cone direction from
-// neighbouring voxels, so a wave's gathers touch few cache lines, and the 4
-// groups keep work items short for the tail.  The loop is co-limited by the
-// texture address/data pipe (16 cycles per 16-byte wave64 load) and VALU.  Each group's compensated sum goes to its own
-// partial buffer; k_ccc_finish adds them as (g0 + g1) + (g2 + g3), so the
-// result is deterministic.
+// one of the 12 azimuths, marched through all six zenith rows (equal work).
+// All 64 lanes then sample the same cone direction from neighbouring voxels,
+// so a wave's gathers touch few cache lines.  The items are short so that the
+// last round of waves is nearly full: every wave takes an equal run of items
+// and all of them are resident at once, so the kernel lasts ceil(items /
+// waves) items.  The loop is co-limited by the texture address/data pipe (16
+// cycles per 16-byte wave64 load) and VALU.  Each azimuth's compensated sum
+// goes to its own partial buffer and k_ccc_finish adds the 12 in a fixed
+// order, so the result is deterministic.
 constexpr int kVoxPerItem = 64;
-constexpr int kGroups = 4;
+constexpr int kGroups = 12;
 
 template <bool Quad, bool Off32>
 __global__ void __launch_bounds__(kBlock)
@@ -355,8 +357,8 @@ k_ccc(dh_ccc_args a, const void *__restrict__ tr, const int *__restrict__ list, 
             const float c1 = a.c1[row], c2p = a.c2p[row], bp = a.bp[row], a2 = a.a2[row];
             const float am = a.a[row], bm = a.b[row], L0 = a.L0[row];
             const int ns = a.nsteps[row];
-            for (int az3 = 0; az3 < 3; az3++) {
-                const int cone = row * 12 + grp * 3 + az3;
+            {
+                const int cone = row * 12 + grp;
                 const float dx0 = a.dir[3 * cone + 0];
                 const float dy0 = a.dir[3 * cone + 1];
                 const float dz0 = a.dir[3 * cone + 2];
@@ -394,7 +396,7 @@ k_ccc(dh_ccc_args a, const void *__restrict__ tr, const int *__restrict__ list, 
     }
 }
 
-// Dose of the listed voxels from the 4 group sums, the electron term and the
+// Dose of the listed voxels from the 12 azimuth sums, the electron term and the
 // MU calibration (IMRTClasses.cu:338-346).
 __global__ void __launch_bounds__(kBlock)
 k_ccc_finish(float mu_cal, float mu, const float *__restrict__ electron, const int *__restrict__ list,
@@ -404,9 +406,13 @@ k_ccc_finish(float mu_cal, float mu, const float *__restrict__ electron, const i
     const int n = *counter;
     for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < n; idx += gridDim.x * kBlock) {
         const uint32_t v = (uint32_t)list[idx];
-        const float s01 = part[idx] + part[pstride + idx];
-        const float s23 = part[2 * (size_t)pstride + idx] + part[3 * (size_t)pstride + idx];
-        const float dv = (s01 + s23) + electron[v];
+        float p[kGroups];
+#pragma unroll
+        for (int q = 0; q < kGroups; q++) p[q] = part[(size_t)q * pstride + idx];
+        const float s0 = (p[0] + p[1]) + (p[2] + p[3]);
+        const float s1 = (p[4] + p[5]) + (p[6] + p[7]);
+        const float s2 = (p[8] + p[9]) + (p[10] + p[11]);
+        const float dv = ((s0 + s1) + s2) + electron[v];
         dose[v] = (!isnan(dv) && dv >= 0.0f) ? mu_cal * dv * mu : 0.0f;
     }
 }
@@ -430,7 +436,7 @@ int dh_ccc_layout(const dh_ccc_args &a, int choice)
 }
 
 // The packed volume (up to 32 B per padded element), then the cone kernel's
-// 4 group sums per voxel.
+// 12 azimuth sums per voxel.
 static size_t tr_vol_bytes(const dh_ccc_args &a) { return ((size_t)32 * a.p.pxy * a.p.pz + 255) & ~(size_t)255; }
 size_t dh_tr_pad_bytes(const dh_ccc_args &a)
 {
