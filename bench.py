@@ -34,7 +34,7 @@ HBM_PEAK_GBS = 8000.0
 # `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE` runs of this bench,
 # scaled by a 256 MiB copy calibration (factors 2.0 and 1.0); the commands and
 # per-kernel figures are in profiles/hbm_traffic_cccs128_7beams.md
-CCC_TRAFFIC_BYTES = (2.0 * 92930 + 10944) * 1024
+CCC_TRAFFIC_BYTES = (2.0 * 111437 + 32833) * 1024
 
 
 def parse():
