@@ -63,18 +63,44 @@ __device__ static inline float head_transmission(const dh_terma_args &a, float h
         q1 = lo;
     }
     float tr = 0.0f;
+    // Adjacent leaf pairs usually share a y edge: when this pair's lower edge
+    // is the same float as the previous evaluated pair's upper edge, its
+    // scaled coordinate and erf are the same values, so the erf is reused.
+    // Likewise a pair whose leaf tips equal the previous evaluated pair's has
+    // the same x factor (rectangular apertures: every open pair).  Every term
+    // and the order of the sum are unchanged.
+    float prev_w = NAN, prev_erf = 0.0f;
+    float prev_x1 = NAN, prev_x2 = NAN, prev_ex = 0.0f;
     for (int q = q0; q < q1; q++) {
         const float4 m = a.mlc[q];
         const float e1 = ((m.z - hy) * dv) + hy;
         const float e2 = ((m.w - hy) * dv) + hy;
         const float y1 = e1 * inv, y2 = e2 * inv;
-        if (erf_pair_zero(y1, y2)) continue;
+        if (erf_pair_zero(y1, y2)) {
+            prev_w = NAN;
+            continue;
+        }
         const float t1 = ((m.x - hx) * dv) + hx;
         const float t2 = ((m.y - hx) * dv) + hx;
         const float x1 = t1 * inv, x2 = t2 * inv;
-        if (erf_pair_zero(x1, x2)) continue;
-        const float ex = 0.5f * (erff(x2) - erff(x1));
-        const float ey = 0.5f * (erff(y2) - erff(y1));
+        if (erf_pair_zero(x1, x2)) {
+            prev_w = NAN;
+            continue;
+        }
+        float ex;
+        if (m.x == prev_x1 && m.y == prev_x2) {
+            ex = prev_ex;
+        } else {
+            ex = 0.5f * (erff(x2) - erff(x1));
+            prev_x1 = m.x;
+            prev_x2 = m.y;
+            prev_ex = ex;
+        }
+        const float f1 = m.z == prev_w ? prev_erf : erff(y1);
+        const float f2 = erff(y2);
+        prev_w = m.w;
+        prev_erf = f2;
+        const float ey = 0.5f * (f2 - f1);
         tr += ex * ey;
     }
     return tr;
