@@ -67,8 +67,9 @@ __device__ static inline float head_transmission(const dh_terma_args &a, float h
     // is the same float as the previous evaluated pair's upper edge, its
     // scaled coordinate and erf are the same values, so the erf is reused.
     // Likewise a pair whose leaf tips equal the previous evaluated pair's has
-    // the same x factor (rectangular apertures: every open pair).  Every term
-    // and the order of the sum are unchanged.
+    // the same x factor (rectangular apertures: every open pair), and a pair
+    // whose x factor is exactly 0 adds nothing.  The sum is unchanged bit for
+    // bit.
     float prev_w = NAN, prev_erf = 0.0f;
     float prev_x1 = NAN, prev_x2 = NAN, prev_ex = 0.0f;
     for (int q = q0; q < q1; q++) {
@@ -95,6 +96,12 @@ __device__ static inline float head_transmission(const dh_terma_args &a, float h
             prev_x1 = m.x;
             prev_x2 = m.y;
             prev_ex = ex;
+        }
+        // a closed pair (equal tips) has ex = 0 exactly: its term adds +0 to
+        // the non-negative sum, so its y factor is not needed
+        if (ex == 0.0f) {
+            prev_w = NAN;
+            continue;
         }
         const float f1 = m.z == prev_w ? prev_erf : erff(y1);
         const float f2 = erff(y2);
