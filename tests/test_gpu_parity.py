@@ -171,6 +171,49 @@ def test_terma_pair_window_is_exact(photon_model, ga, ca):
     assert t.equal(a, b) and t.equal(ea, eb)
 
 
+def _modulated_mlc(pm, seed):
+    """Irregular aperture: random leaf tips, closed pairs at random positions
+    inside the field (equal tips away from the axis), and a run of pairs with
+    equal tips, so TERMA's shared y-edge erf, shared x factor and closed-pair
+    shortcuts all take both branches."""
+    from dosecuda_amd.beam_models import square_field_mlc
+    rng = np.random.default_rng(seed)
+    mlc = square_field_mlc(pm, 12.0, 12.0).copy()
+    n = mlc.shape[0]
+    x1 = rng.uniform(-50.0, 10.0, n).astype(F)
+    mlc[:, 0] = x1
+    mlc[:, 1] = x1 + rng.uniform(0.0, 60.0, n).astype(F)
+    closed = rng.choice(n, n // 5, replace=False)
+    mlc[closed, 0] = rng.uniform(-40.0, 40.0, closed.size).astype(F)
+    mlc[closed, 1] = mlc[closed, 0]
+    mlc[n // 2 - 5:n // 2 + 5, 0] = -30.0
+    mlc[n // 2 - 5:n // 2 + 5, 1] = 25.0
+    return mlc
+
+
+@pytest.mark.parametrize("ga, ca, shuffle", [(0.0, 0.0, False), (51.43, 20.0, False), (200.0, 0.0, True)])
+def test_terma_modulated_mlc(photon_model, ga, ca, shuffle):
+    """TERMA of an irregular MLC aperture against the FP64 oracle; shuffled
+    pair order takes the full scan instead of the y-sorted window."""
+    t = _torch()
+    from dosecuda_amd import engine
+    pm = photon_model
+    rho, iso = _imrt_case(pm)
+    shape, s = rho.shape, 2.5
+    mlc = _modulated_mlc(pm, 41)
+    if shuffle:
+        mlc = mlc[np.random.default_rng(42).permutation(mlc.shape[0])]
+    g = O.geom(shape, s, np.zeros(3), iso, ga, 0.0, ca, F(pm.primary_source_distance))
+    raw, _ = O.raytrace(g, rho, float(F(0.0012)))
+    wet = raw.astype(F)
+    print(f"[case] terma modulated mlc ga={ga} ca={ca} shuffle={shuffle}", flush=True)
+    tg, eg = engine.stage_terma(_cuda(t, wet), s, np.zeros(3), iso, 100.0, ga, ca, 0.0, pm, mlc)
+    tr, er = O.terma(g, pm, mlc, wet)
+    assert float(tr.max()) > 0.0
+    _report("terma modulated", tg.cpu().numpy(), tr, 1e-5)
+    _report("electron modulated", eg.cpu().numpy(), er, 1e-5)
+
+
 @pytest.mark.parametrize("ga, ca, ta", [(0.0, 0.0, 0.0), (102.86, 30.0, 0.0), (308.57, 0.0, 20.0)])
 def test_ccc_stage(photon_model, ga, ca, ta):
     t = _torch()
@@ -227,6 +270,25 @@ def test_dropin_photon_end_to_end(photon_model):
     assert dose.dtype == np.float32 and dose.shape == rho.shape
     ref = O.photon_dose(rho, s, np.zeros(3), iso, 100.0, 30.0, 15.0, 0.0, pm, cp.mlc)
     _report("dropin photon dose", dose, ref, TOL_DOSE)
+
+
+def test_dropin_photon_modulated_mlc(photon_model):
+    """An irregular aperture (closed pairs inside the field, equal-tip runs)
+    through the drop-in, with collimator and couch rotation, against the
+    FP64 oracle's whole beam."""
+    _torch()
+    from dosecuda_amd import dose_kernels
+    from dosecuda_amd.beam_models import PhotonControlPoint, VolumeObject
+    pm = photon_model
+    rho, iso = _imrt_case(pm, shape=(26, 30, 28), seed=43)
+    s = 2.5
+    vol = VolumeObject(rho, origin=(0, 0, 0), spacing=(s, s, s))
+    cp = PhotonControlPoint(iso, 100.0, _modulated_mlc(pm, 44), 320.0, 35.0, 10.0)
+    print("[case] drop-in photon modulated mlc", flush=True)
+    dose = dose_kernels.photon_dose_cuda(pm, vol, cp, 0)
+    ref = O.photon_dose(rho, s, np.zeros(3), iso, 100.0, 320.0, 35.0, 10.0, pm, cp.mlc)
+    assert float(ref.max()) > 0.0
+    _report("dropin photon modulated", dose, ref, TOL_DOSE)
 
 
 def test_invalid_gpu_id_raises(proton_model):
