@@ -3,7 +3,8 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the shipped engine (dosecuda_amd/)
  * links, loads or calls this file; only tests/, the smoke check and the
- * cpu_baseline leg of bench.py use it, as the checker.
+ * cpu_baseline leg of bench.py use it, to check and to time the engine's
+ * results, never as a result of the engine.
  *
  * It restates, per voxel, the formulae of the reference kernels:
  *   rayTraceKernel      DoseCUDA/dose_kernels/CudaClasses.cu:180-228

@@ -2,7 +2,8 @@
 
 TEST INFRASTRUCTURE ONLY: the engine under dosecuda_amd/ never imports this
 module.  Only tests/, ``__graft_entry__.smoke()`` and the ``cpu_baseline``
-leg of bench.py use it, as the checker.
+leg of bench.py use it, to check (and time beside) the engine, never in
+place of it.
 
 The pipeline helpers chain the stages through float32 exactly where the
 reference hands float32 buffers from one kernel to the next

@@ -1,7 +1,7 @@
 """GPU parity: every stage of the HIP engine against the FP64 oracle on the
 same inputs, then the drop-in entry points end to end.
 
-Tolerance (north star): max |GPU - oracle| <= 1e-4 * Dmax of the oracle for
+Tolerance: max |GPU - oracle| <= 1e-4 * Dmax of the oracle for
 doses; the WET stages are held to 1e-5 of their maximum.  Each test prints
 its max and RMS error relative to the oracle's maximum.
 Stage tests feed both sides the same float32 input, so a float32 decision
