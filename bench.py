@@ -395,7 +395,8 @@ def main():
                 log(f"surrogate: convolution variants tuned ({time.time() - t0:.1f} s)")
             log("surrogate: 7-beam plan, checked surrogate vs plain CCCS")
             r = S.evaluate(net, PhotonBeamModel(), stride=3, tol=0.01, dilate=1, reps=5)
-            conv_ms = float(np.mean([b["surrogate_stage_ms"]["nn_conv3d"] for b in r["beams"]]))
+            conv_ms_single = float(np.mean([b["surrogate_stage_ms"]["nn_conv3d"] for b in r["beams"]]))
+            conv_ms = r["batch_conv_ms_per_beam"]
             tf = r["net_gflop"] / (conv_ms * 1e-3) / 1e3
             res["surrogate"] = dict(
                 weights=os.path.relpath(S.WEIGHTS, ROOT), params=r["params"], stride=r["stride"], tol=r["tol"],
@@ -406,12 +407,14 @@ def main():
                 rms_err_checked=max(b["rms_err_checked"] for b in r["beams"]),
                 mean_fallback_fraction=r["mean_fallback_fraction"],
                 checked_ms_per_beam=r["graph_checked_ms_per_beam"],
+                checked_ms_per_beam_unbatched=r["graph_single_ms_per_beam"],
+                batch_equal_single=r["batch_equal_single"],
                 plain_superposition_ms_per_beam=r["plain_superposition_ms"],
                 speedup_vs_plain_superposition=r["speedup_superposition"],
                 plain_beam_ms=r["plain_beam_ms"],
                 surrogate_beam_ms=r["plain_beam_ms"] - r["plain_superposition_ms"] + r["graph_checked_ms_per_beam"],
                 mfma=dict(net_gflop=r["net_gflop"], conv_ms_per_beam=conv_ms, achieved_tflops=tf, peak_tflops=2500.0,
-                          frac=tf / 2500.0),
+                          frac=tf / 2500.0, conv_ms_per_beam_unbatched=conv_ms_single),
                 graph_replay_equal=r["graph_replay_equal"])
             log(f"surrogate: {r['graph_checked_ms_per_beam']:.3f} ms/beam checked vs {r['plain_superposition_ms']:.3f} "
                 f"plain superposition ({r['speedup_superposition']:.2f}x), plan gamma {r['plan_gamma_checked']:.5f}, "

@@ -121,6 +121,8 @@ SIGNATURES = {
     "dh_sur_net_forward_dev": [_P, _P, C.c_int32, C.c_int32, C.c_int32, _P, _P],
     "dh_sur_dose_dev": [_P, C.POINTER(Grid), C.POINTER(PhotonCP), C.POINTER(PhotonModel), _P, _P, _P,
                         C.c_int32, C.c_float, C.c_int32, _P, _P],
+    "dh_sur_dose_batch_dev": [_P, C.POINTER(Grid), C.c_int32, C.POINTER(PhotonCP), C.POINTER(PhotonModel), _P, _P,
+                              _P, C.c_int32, C.c_float, C.c_int32, _P, _P],
     "dh_photon_dose_surrogate_dev": [_I, _P, C.POINTER(Grid), C.POINTER(PhotonCP), C.POINTER(PhotonModel), _P,
                                      C.c_int32, _P, _P, _P, _P, C.c_int32, C.c_float, C.c_int32, _P],
     "dh_sched_create": [_I, C.c_int32, _P],

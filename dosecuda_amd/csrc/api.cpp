@@ -691,6 +691,34 @@ extern "C" int dh_sur_dose_dev(const dh_sur_net *net, const dh_grid *grid, const
     });
 }
 
+extern "C" int dh_sur_dose_batch_dev(const dh_sur_net *net, const dh_grid *grid, int32_t n_cp,
+                                     const dh_photon_cp *cps, const dh_photon_model *model,
+                                     const float *const *d_terma, const float *d_density,
+                                     const float *const *d_electron, int32_t stride, float tol, int32_t dilate,
+                                     float *const *d_dose_out, void *stream)
+{
+    return guarded([&] {
+        DH_REQUIRE(net && cps && d_terma && d_density && d_electron && d_dose_out, "null argument");
+        DH_REQUIRE(n_cp >= 1 && n_cp <= 4096, "control point count must be 1..4096");
+        DH_REQUIRE(stride >= 1 && stride <= 64, "lattice stride must be 1..64");
+        DH_REQUIRE(dilate >= 0 && dilate <= 2, "dilate must be 0, 1 or 2");
+        for (int i = 0; i < n_cp; i++)
+            DH_REQUIRE(d_terma[i] && d_electron[i] && d_dose_out[i], "null control point volume");
+        dh_device_enter(dh_sur_net_gpu(net));
+        nvox_of(grid);
+        std::vector<dh_ccc_args> cas;
+        std::vector<float> dirs(3 * (size_t)n_cp);
+        cas.reserve(n_cp);
+        for (int i = 0; i < n_cp; i++) {
+            cas.push_back(make_ccc_args(grid, &cps[i], model));
+            sur_direction(&cps[i], &dirs[3 * (size_t)i]);
+        }
+        dh_launch_sur_dose_batch(net, cas.data(), n_cp, d_terma, d_density, d_electron,
+                                 reinterpret_cast<const float (*)[3]>(dirs.data()), stride, tol, dilate, d_dose_out,
+                                 (hipStream_t)stream);
+    });
+}
+
 // ---- enqueue-only entry points for the scheduler ---------------------------
 void dh_photon_enqueue(int dev, const dh_grid *grid, const dh_photon_cp *cp, const dh_photon_model *m,
                        const float *mlc_rows, int n_pairs, const float *d_density, float *d_dose,

@@ -235,6 +235,12 @@ int dh_sur_net_gpu(const dh_sur_net *net);
 void dh_launch_sur_dose(const dh_sur_net *net, const dh_ccc_args &a, const float *terma,
                         const float *density, const float *electron, const float dir[3],
                         int stride, float tol, int dilate, float *dose, hipStream_t s);
+// The same for ncp control points on one grid and density, the network run
+// once per batch of points; each dose equals the single-point one bit for bit.
+void dh_launch_sur_dose_batch(const dh_sur_net *net, const dh_ccc_args *cas, int ncp,
+                              const float *const *terma, const float *density, const float *const *electron,
+                              const float (*dirs)[3], int stride, float tol, int dilate, float *const *dose,
+                              hipStream_t s);
 void dh_launch_sur_check(const dh_ccc_args &a, const float *terma, const float *density,
                          const float *electron, float *dose, const float *dmax, int stride,
                          float tol, int dilate, int *work, float *plain, void *tr_pad, hipStream_t s);

@@ -875,6 +875,31 @@ std::string conv_key(const conv_args &a)
     return b;
 }
 
+// The variant (nco, ksplit) the library picks for a shape: the tuned code,
+// else the default rule, with the generic kernel's tile and K split resolved.
+// Under a fixed variant each output element's K sum runs in the same order
+// whichever voxels share its tile, and every input index carries its sample,
+// so a batch launched with the pick of its N = 1 shape gives each sample the
+// N = 1 result bit for bit.
+void conv_pick(const conv_args &a, int &nco, int &ksplit)
+{
+    const int64_t nvox = (int64_t)a.N * a.D * a.H * a.W;
+    const int ki = a.ksize * a.ksize * a.ksize * (a.cin / 16);
+    nco = 0;
+    ksplit = 0;
+    const int code = dh_tune_lookup("conv3d", conv_key(a), -1);
+    if (code >= 3000) { nco = code - 3000; ksplit = -3; }
+    else if (code >= 1100) { nco = code - 1100; ksplit = -2; }
+    else if (code >= 1000) { nco = code - 1000; ksplit = -1; }
+    else if (code > 0) { nco = code / 100; ksplit = code % 100; }
+    if (nco == 0 && ksplit == 0) {
+        if (a.cout <= 16 && a.ksize == 1 && a.cb == 0 && a.up == 0) { nco = 1; ksplit = -3; }
+        else if (a.cout <= 16 && a.ksize == 3 && nvox >= 32768) { nco = 8; ksplit = -3; }
+        else if (a.ksize == 3 && nvox >= 32768 && a.cout_pad <= 64) ksplit = nvox < 131072 ? -2 : -1;
+    }
+    if (ksplit >= 0 && (nco <= 0 || ksplit <= 0)) conv_shape(nvox, a.cout_pad, ki, nco, ksplit);
+}
+
 void launch_conv3d(const conv_args &a, int nco, int ksplit, hipStream_t s)
 {
     const int64_t nvox = (int64_t)a.N * a.D * a.H * a.W;
@@ -1091,29 +1116,35 @@ namespace {
 
 constexpr int kSurLayers = 15;
 constexpr int kSurOut = 16;      // head output channels (0-7 used)
+constexpr int kSurBatch = 8;     // control points per network batch
 
-// activation elements (bf16) of one forward over cz x cy x cx cells
-int64_t sur_act_elems(const dh_sur_net *net, int cz, int cy, int cx)
+// activation elements (bf16) of one forward over n samples of cz x cy x cx cells
+int64_t sur_act_elems(const dh_sur_net *net, int n, int cz, int cy, int cx)
 {
-    const int64_t L0 = (int64_t)cz * cy * cx, L1 = L0 / 8, L2 = L0 / 64, L3 = L0 / 512;
+    const int64_t L0 = (int64_t)n * cz * cy * cx, L1 = L0 / 8, L2 = L0 / 64, L3 = L0 / 512;
     const int a = net->widths[0], b = net->widths[1], c = net->widths[2], d = net->widths[3];
     return L0 * 3 * a + L1 * (a + 3 * b) + L2 * (b + 3 * c) + L3 * (c + 2 * d);
 }
 
-void sur_conv(const dh_sur_net *net, int li, const uint16_t *xa, const uint16_t *xb, int d, int h, int w,
+// One layer over n samples with the variant picked for one sample, so every
+// sample's output equals that of a forward over it alone.
+void sur_conv(const dh_sur_net *net, int li, const uint16_t *xa, const uint16_t *xb, int n, int d, int h, int w,
               void *y, float *y32, hipStream_t s)
 {
     const dh_sur_layer &L = net->L[li];
     conv_args a = make_conv(xa, L.ca, L.up, xb, L.cb, 1, d, h, w, L.w, L.b, L.cout, L.cout_pad, L.ksize, L.act,
                             net->slope, y, y32);
-    launch_conv3d(a, 0, 0, s);
+    int nco, ksplit;
+    conv_pick(a, nco, ksplit);
+    a.N = n;
+    launch_conv3d(a, nco, ksplit, s);
 }
 
-// U-Net forward: x [cz][cy][cx][in_ch] bf16 -> y32 [cz][cy][cx][16] fp32
-void sur_forward(const dh_sur_net *net, const uint16_t *x, int cz, int cy, int cx, float *y32, uint16_t *ws,
-                 hipStream_t s)
+// U-Net forward: x [n][cz][cy][cx][in_ch] bf16 -> y32 [n][cz][cy][cx][16] fp32
+void sur_forward(const dh_sur_net *net, const uint16_t *x, int n, int cz, int cy, int cx, float *y32,
+                 uint16_t *ws, hipStream_t s)
 {
-    const int64_t L0 = (int64_t)cz * cy * cx, L1 = L0 / 8, L2 = L0 / 64, L3 = L0 / 512;
+    const int64_t L0 = (int64_t)n * cz * cy * cx, L1 = L0 / 8, L2 = L0 / 64, L3 = L0 / 512;
     const int a = net->widths[0], b = net->widths[1], c = net->widths[2], d = net->widths[3];
     uint16_t *p = ws;
     auto take = [&](int64_t n) { uint16_t *q = p; p += n; return q; };
@@ -1123,24 +1154,24 @@ void sur_forward(const dh_sur_net *net, const uint16_t *x, int cz, int cy, int c
     uint16_t *P3 = take(L3 * c), *A3 = take(L3 * d), *S3 = take(L3 * d);
     const int z1 = cz / 2, y1 = cy / 2, x1 = cx / 2, z2 = cz / 4, y2 = cy / 4, x2 = cx / 4;
     const int z3 = cz / 8, y3 = cy / 8, x3 = cx / 8;
-    sur_conv(net, 0, x, nullptr, cz, cy, cx, A0, nullptr, s);
-    sur_conv(net, 1, A0, nullptr, cz, cy, cx, S0, nullptr, s);
-    launch_pool(S0, 1, cz, cy, cx, a, P1, s);
-    sur_conv(net, 2, P1, nullptr, z1, y1, x1, A1, nullptr, s);
-    sur_conv(net, 3, A1, nullptr, z1, y1, x1, S1, nullptr, s);
-    launch_pool(S1, 1, z1, y1, x1, b, P2, s);
-    sur_conv(net, 4, P2, nullptr, z2, y2, x2, A2, nullptr, s);
-    sur_conv(net, 5, A2, nullptr, z2, y2, x2, S2, nullptr, s);
-    launch_pool(S2, 1, z2, y2, x2, c, P3, s);
-    sur_conv(net, 6, P3, nullptr, z3, y3, x3, A3, nullptr, s);
-    sur_conv(net, 7, A3, nullptr, z3, y3, x3, S3, nullptr, s);
-    sur_conv(net, 8, S3, S2, z2, y2, x2, B2, nullptr, s);
-    sur_conv(net, 9, B2, nullptr, z2, y2, x2, A2, nullptr, s);
-    sur_conv(net, 10, A2, S1, z1, y1, x1, B1, nullptr, s);
-    sur_conv(net, 11, B1, nullptr, z1, y1, x1, A1, nullptr, s);
-    sur_conv(net, 12, A1, S0, cz, cy, cx, B0, nullptr, s);
-    sur_conv(net, 13, B0, nullptr, cz, cy, cx, A0, nullptr, s);
-    sur_conv(net, 14, A0, nullptr, cz, cy, cx, nullptr, y32, s);
+    sur_conv(net, 0, x, nullptr, n, cz, cy, cx, A0, nullptr, s);
+    sur_conv(net, 1, A0, nullptr, n, cz, cy, cx, S0, nullptr, s);
+    launch_pool(S0, n, cz, cy, cx, a, P1, s);
+    sur_conv(net, 2, P1, nullptr, n, z1, y1, x1, A1, nullptr, s);
+    sur_conv(net, 3, A1, nullptr, n, z1, y1, x1, S1, nullptr, s);
+    launch_pool(S1, n, z1, y1, x1, b, P2, s);
+    sur_conv(net, 4, P2, nullptr, n, z2, y2, x2, A2, nullptr, s);
+    sur_conv(net, 5, A2, nullptr, n, z2, y2, x2, S2, nullptr, s);
+    launch_pool(S2, n, z2, y2, x2, c, P3, s);
+    sur_conv(net, 6, P3, nullptr, n, z3, y3, x3, A3, nullptr, s);
+    sur_conv(net, 7, A3, nullptr, n, z3, y3, x3, S3, nullptr, s);
+    sur_conv(net, 8, S3, S2, n, z2, y2, x2, B2, nullptr, s);
+    sur_conv(net, 9, B2, nullptr, n, z2, y2, x2, A2, nullptr, s);
+    sur_conv(net, 10, A2, S1, n, z1, y1, x1, B1, nullptr, s);
+    sur_conv(net, 11, B1, nullptr, n, z1, y1, x1, A1, nullptr, s);
+    sur_conv(net, 12, A1, S0, n, cz, cy, cx, B0, nullptr, s);
+    sur_conv(net, 13, B0, nullptr, n, cz, cy, cx, A0, nullptr, s);
+    sur_conv(net, 14, A0, nullptr, n, cz, cy, cx, nullptr, y32, s);
 }
 
 }  // namespace
@@ -1201,33 +1232,61 @@ extern "C" int dh_sur_net_forward_dev(const dh_sur_net *net, const void *d_x, in
                    "cell counts must be positive multiples of 8");
         dh_device_enter(net->gpu);
         uint16_t *ws = (uint16_t *)dh_scratch(net->gpu, DH_SLOT_NN_ACT,
-                                              sizeof(uint16_t) * (size_t)sur_act_elems(net, cz, cy, cx));
-        sur_forward(net, (const uint16_t *)d_x, cz, cy, cx, d_y32, ws, (hipStream_t)stream);
+                                              sizeof(uint16_t) * (size_t)sur_act_elems(net, 1, cz, cy, cx));
+        sur_forward(net, (const uint16_t *)d_x, 1, cz, cy, cx, d_y32, ws, (hipStream_t)stream);
     });
+}
+
+// Checked surrogate doses of ncp control points on one grid (one density,
+// per-point TERMA and electron volumes): each point is packed into its slot
+// of one batch, the network runs once per batch of up to kSurBatch points,
+// and each point is finished and checked in turn on the stream.  Every
+// layer uses the variant of its one-sample shape, so each dose equals the
+// single-point result bit for bit.
+void dh_launch_sur_dose_batch(const dh_sur_net *net, const dh_ccc_args *cas, int ncp,
+                              const float *const *terma, const float *density, const float *const *electron,
+                              const float (*dirs)[3], int stride, float tol, int dilate, float *const *dose,
+                              hipStream_t s)
+{
+    DH_REQUIRE(net->in_ch == 32, "the surrogate input layout has 32 channels");
+    DH_REQUIRE(ncp >= 1, "no control points");
+    const dh_geom &g = cas[0].g;
+    const int nz = g.nz, ny = g.ny, nx = g.nx;
+    const int cz = ((nz + 15) / 16) * 8, cy = ((ny + 15) / 16) * 8, cx = ((nx + 15) / 16) * 8;
+    const int64_t cells = (int64_t)cz * cy * cx;
+    const int nb = std::min(ncp, kSurBatch);
+    const int dev = net->gpu;
+    uint16_t *x = (uint16_t *)dh_scratch(dev, DH_SLOT_NN_X, sizeof(uint16_t) * (size_t)(nb * cells * net->in_ch));
+    uint16_t *ws = (uint16_t *)dh_scratch(dev, DH_SLOT_NN_ACT,
+                                          sizeof(uint16_t) * (size_t)sur_act_elems(net, nb, cz, cy, cx));
+    float *y32 = (float *)dh_scratch(dev, DH_SLOT_NN_Y, sizeof(float) * (size_t)(nb * cells * kSurOut));
+    float *scal = (float *)dh_scratch(dev, DH_SLOT_NN_SCAL, 256);     // {max TERMA, max dose} per slot
+    const int64_t wl = dh_sur_check_work_len(nz, ny, nx, stride);
+    int *work = (int *)dh_scratch(dev, DH_SLOT_NN_WORK, sizeof(int) * (size_t)wl);
+    float *plain = (float *)dh_scratch(dev, DH_SLOT_NN_PLAIN, sizeof(float) * (size_t)nz * ny * nx);
+    void *trp = dh_scratch(dev, DH_SLOT_TR_PAD, dh_tr_pad_bytes(cas[0]));
+    for (int c0 = 0; c0 < ncp; c0 += nb) {
+        const int n = std::min(nb, ncp - c0);
+        for (int i = 0; i < n; i++)
+            launch_pack(nz, ny, nx, terma[c0 + i], density, dirs[c0 + i], g.s, x + (size_t)i * cells * net->in_ch,
+                        scal + 2 * i, s);
+        sur_forward(net, x, n, cz, cy, cx, y32, ws, s);
+        for (int i = 0; i < n; i++) {
+            const dh_ccc_args &ca = cas[c0 + i];
+            launch_finish(nz, ny, nx, y32 + (size_t)i * cells * kSurOut, scal + 2 * i, net->out_scale,
+                          terma[c0 + i], electron[c0 + i], ca.mu_cal * ca.mu, dose[c0 + i], scal + 2 * i + 1, s);
+            dh_launch_sur_check(ca, terma[c0 + i], density, electron[c0 + i], dose[c0 + i], scal + 2 * i + 1,
+                                stride, tol, dilate, work, plain, trp, s);
+        }
+    }
 }
 
 void dh_launch_sur_dose(const dh_sur_net *net, const dh_ccc_args &ca, const float *terma,
                         const float *density, const float *electron, const float dir[3],
                         int stride, float tol, int dilate, float *dose, hipStream_t s)
 {
-    const dh_geom &g = ca.g;
-    const int nz = g.nz, ny = g.ny, nx = g.nx;
-    const int cz = ((nz + 15) / 16) * 8, cy = ((ny + 15) / 16) * 8, cx = ((nx + 15) / 16) * 8;
-    const int64_t cells = (int64_t)cz * cy * cx;
-    const int dev = net->gpu;
-    uint16_t *x = (uint16_t *)dh_scratch(dev, DH_SLOT_NN_X, sizeof(uint16_t) * (size_t)(cells * net->in_ch));
-    uint16_t *ws = (uint16_t *)dh_scratch(dev, DH_SLOT_NN_ACT, sizeof(uint16_t) * (size_t)sur_act_elems(net, cz, cy, cx));
-    float *y32 = (float *)dh_scratch(dev, DH_SLOT_NN_Y, sizeof(float) * (size_t)(cells * kSurOut));
-    float *scal = (float *)dh_scratch(dev, DH_SLOT_NN_SCAL, 256);
-    const int64_t wl = dh_sur_check_work_len(nz, ny, nx, stride);
-    int *work = (int *)dh_scratch(dev, DH_SLOT_NN_WORK, sizeof(int) * (size_t)wl);
-    float *plain = (float *)dh_scratch(dev, DH_SLOT_NN_PLAIN, sizeof(float) * (size_t)nz * ny * nx);
-    void *trp = dh_scratch(dev, DH_SLOT_TR_PAD, dh_tr_pad_bytes(ca));
-    DH_REQUIRE(net->in_ch == 32, "the surrogate input layout has 32 channels");
-    launch_pack(nz, ny, nx, terma, density, dir, g.s, x, scal, s);
-    sur_forward(net, x, cz, cy, cx, y32, ws, s);
-    launch_finish(nz, ny, nx, y32, scal, net->out_scale, terma, electron, ca.mu_cal * ca.mu, dose, scal + 1, s);
-    dh_launch_sur_check(ca, terma, density, electron, dose, scal + 1, stride, tol, dilate, work, plain, trp, s);
+    const float d[1][3] = {{dir[0], dir[1], dir[2]}};
+    dh_launch_sur_dose_batch(net, &ca, 1, &terma, density, &electron, d, stride, tol, dilate, &dose, s);
 }
 
 extern "C" int dh_sur_counts(int gpu_id, int32_t *lattice, int32_t *fallback)

@@ -614,6 +614,33 @@ class SurrogateCCCS:
         return out
 
     @torch.no_grad()
+    def dose_batch(self, termas, density, electrons, spacing, origin, cps, stride=None, tol=None, outs=None):
+        """Checked surrogate doses of several control points of one grid and
+        density in one call of the library (dh_sur_dose_batch_dev): cps is a
+        list of (iso, mu, ga, ca, ta), termas / electrons the matching
+        volumes.  The points share network launches (up to 8 per batch);
+        each dose equals dose() of that point alone, bit for bit."""
+        stride = self.stride if stride is None else int(stride)
+        tol = self.tol if tol is None else float(tol)
+        n = len(cps)
+        if n == 0 or len(termas) != n or len(electrons) != n:
+            raise ValueError("need one TERMA and one electron volume per control point")
+        outs = [torch.empty_like(t) for t in termas] if outs is None else list(outs)
+        if len(outs) != n:
+            raise ValueError("need one output volume per control point")
+        for v in (*termas, *electrons, *outs):
+            if v.shape != density.shape or v.dtype != torch.float32 or not v.is_contiguous():
+                raise ValueError("volumes must be contiguous float32 of the density's shape")
+        arr = (_lib.PhotonCP * n)(*[_lib.photon_cp(*cp) for cp in cps])
+        P = lambda vs: (C.c_void_p * n)(*[v.data_ptr() for v in vs])
+        keep: list = []
+        m = engine.photon_args(self.model, keep)
+        _lib.check(_lib.load().dh_sur_dose_batch_dev(
+            self.native.handle, _lib.grid(density.shape, spacing, origin), n, arr, m, P(termas),
+            _lib.ptr(density), P(electrons), stride, tol, self.dilate, P(outs), _st()))
+        return outs
+
+    @torch.no_grad()
     def dose_torch(self, terma, density, electron, spacing, origin, iso, mu, ga, ca, ta, mlc=None,
                    stride=None, tol=None, out=None, timings=None):
         """The same checked dose with the network driven from Python
@@ -776,7 +803,35 @@ def evaluate(net: SurrogateNet, model, n_beams=7, stride=3, tol=0.01, reps=5, lo
         res["graph_replay_equal"] = all(bool(torch.equal(o, c)) for o, c in zip(outs, checked))
         if not res["graph_replay_equal"]:
             break
-    res["graph_checked_ms_per_beam"] = _time(lambda: g.replay(), reps) / len(angles)
+    res["graph_single_ms_per_beam"] = _time(lambda: g.replay(), reps) / len(angles)
+    # the same plan with the beams batched through the network (one launch
+    # per layer for all beams); must equal the per-beam doses bit for bit
+    cps = [(iso, 100.0, ga, 0.0, 0.0) for ga in angles]
+    outs_b = [torch.empty_like(rho) for _ in angles]
+
+    def plan_batch():
+        sur.dose_batch(termas, rho, els, sp, org, cps, outs=outs_b)
+
+    plan_batch()
+    torch.cuda.synchronize()
+    res["batch_equal_single"] = all(bool(torch.equal(o, c)) for o, c in zip(outs_b, checked))
+    eager_b = [o.clone() for o in outs_b]
+    gb = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(gb):
+        plan_batch()
+    for _ in range(3):
+        gb.replay()
+        torch.cuda.synchronize()
+        res["graph_replay_equal"] = res["graph_replay_equal"] and all(
+            bool(torch.equal(o, c)) for o, c in zip(outs_b, eager_b))
+    res["graph_checked_ms_per_beam"] = _time(lambda: gb.replay(), reps) / len(angles)
+    engine.prof_reset()
+    engine.prof_enable(True)
+    for _ in range(reps):
+        plan_batch()
+    torch.cuda.synchronize()
+    engine.prof_enable(False)
+    res["batch_conv_ms_per_beam"] = engine.prof_get("nn_conv3d")[0] / reps / len(angles)
     res["plan_gamma_net"] = gamma(sums["net"], sums["plain"], sp)[0]
     res["plan_gamma_checked"] = gamma(sums["checked"], sums["plain"], sp)[0]
     B = res["beams"]

@@ -283,6 +283,16 @@ int dh_sur_dose_dev(const dh_sur_net *net, const dh_grid *grid, const dh_photon_
                     const dh_photon_model *model, const float *d_terma, const float *d_density,
                     const float *d_electron, int32_t stride, float tol, int32_t dilate,
                     float *d_dose_out, void *stream);
+/* dh_sur_dose_dev for n_cp control points of one grid and density (per-point
+ * TERMA, electron and output volumes, arrays of device pointers): the points
+ * share network launches, up to 8 per batch, and each dose equals the
+ * single-point call's bit for bit.  No reference counterpart (the reference
+ * runs one control point per photon_dose_cuda call, plan_imrt.py:150-153). */
+int dh_sur_dose_batch_dev(const dh_sur_net *net, const dh_grid *grid, int32_t n_cp,
+                          const dh_photon_cp *cps, const dh_photon_model *model,
+                          const float *const *d_terma, const float *d_density,
+                          const float *const *d_electron, int32_t stride, float tol, int32_t dilate,
+                          float *const *d_dose_out, void *stream);
 /* Lattice and fallback voxel counts of the last checked surrogate dose run
  * by dh_sur_dose_dev / dh_photon_dose_surrogate_dev from this thread on
  * gpu_id (synchronises the device). */

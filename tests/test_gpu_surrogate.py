@@ -464,3 +464,39 @@ def test_native_dose_equals_torch_path(photon_model):
         0, sur.native.handle, _lib.grid(shape, s, np.zeros(3)), _lib.photon_cp(iso, 100.0, 120.0, 0.0, 0.0), m,
         _lib.ptr(rows), rows.shape[0], _lib.ptr(rho), _lib.ptr(d), None, None, 4, 0.02, sur.dilate, S._st()))
     assert t.equal(d, a)
+
+
+@pytest.mark.parametrize("shape, n_cp", [((40, 50, 36), 3), ((96, 96, 96), 9)])
+def test_dose_batch_equals_single(photon_model, shape, n_cp):
+    """dh_sur_dose_batch_dev (control points sharing network launches, up
+    to 8 per batch) gives each point the dose of dh_sur_dose_dev on that
+    point alone, bit for bit; the larger grid takes the LDS-brick and
+    16-channel convolution kernels at full resolution, and 9 points take
+    two batches."""
+    t = _torch()
+    from dosecuda_amd import engine, surrogate as S
+    from dosecuda_amd.beam_models import square_field_mlc
+    pm = photon_model
+    s = 2.5
+    rho = t.full(shape, 1.00653, device="cuda")
+    rho[shape[0] // 4:shape[0] // 2, shape[1] // 8:shape[1] // 2, shape[2] // 4:shape[2] // 2] = 0.3
+    iso = np.array([(shape[2] - 1) / 2 * s, (shape[1] - 1) / 2 * s, (shape[0] - 1) / 2 * s], np.float32)
+    mlc = square_field_mlc(pm, 6.0, 6.0)
+    sur = S.SurrogateCCCS(S.load(S.WEIGHTS), pm, stride=3, tol=0.01)
+    cps, termas, els = [], [], []
+    for i in range(n_cp):
+        ga, ca = 360.0 * i / n_cp, 10.0 * (i % 3)
+        terma, el = t.empty_like(rho), t.empty_like(rho)
+        engine.photon_dose(rho, s, np.zeros(3), iso, 100.0, ga, ca, 0.0, pm, mlc, terma_out=terma,
+                           electron_out=el)
+        cps.append((iso, 100.0 + i, ga, ca, 0.0))
+        termas.append(terma)
+        els.append(el)
+    print(f"[case] surrogate dose batch shape={shape} points={n_cp}", flush=True)
+    got = sur.dose_batch(termas, rho, els, s, np.zeros(3), cps)
+    for i, cp in enumerate(cps):
+        want = sur.dose(termas[i], rho, els[i], s, np.zeros(3), *cp)
+        assert float(want.max()) > 0.0
+        assert t.equal(got[i], want), f"control point {i}"
+    with pytest.raises(ValueError):
+        sur.dose_batch(termas[:1], rho, els, s, np.zeros(3), cps)
