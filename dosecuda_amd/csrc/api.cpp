@@ -704,6 +704,12 @@ extern "C" int dh_sur_dose_batch_dev(const dh_sur_net *net, const dh_grid *grid,
         DH_REQUIRE(dilate >= 0 && dilate <= 2, "dilate must be 0, 1 or 2");
         for (int i = 0; i < n_cp; i++)
             DH_REQUIRE(d_terma[i] && d_electron[i] && d_dose_out[i], "null control point volume");
+        {
+            std::vector<const float *> o(d_dose_out, d_dose_out + n_cp);
+            std::sort(o.begin(), o.end());
+            DH_REQUIRE(std::adjacent_find(o.begin(), o.end()) == o.end(),
+                       "control points need distinct output volumes");
+        }
         dh_device_enter(dh_sur_net_gpu(net));
         nvox_of(grid);
         std::vector<dh_ccc_args> cas;

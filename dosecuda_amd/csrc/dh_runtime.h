@@ -81,6 +81,9 @@ enum dh_slot {
     DH_SLOT_NN_WORK,       // surrogate check: counters, flags, lists
     DH_SLOT_NN_PLAIN,      // surrogate check: plain values on the lattice
     DH_SLOT_NN_SCAL,       // surrogate scalars (max TERMA, max dose)
+    DH_SLOT_NN_WORK2,      // second check set (batched points on the side stream)
+    DH_SLOT_NN_PLAIN2,
+    DH_SLOT_TR_PAD2,
     DH_SLOT_COUNT_
 };
 

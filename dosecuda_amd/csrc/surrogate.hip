@@ -11,6 +11,8 @@
 // fragment is one 16-byte load.  Padding is "same" (zero outside).
 #include "dh_runtime.h"
 
+#include <mutex>
+
 #include <hip/hip_bf16.h>
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 dh_bf16x8;
@@ -1039,7 +1041,9 @@ void launch_pack(int nz, int ny, int nx, const float *terma, const float *densit
     const int64_t n = (int64_t)nz * ny * nx;
     const int cz = ((nz + 15) / 16) * 8, cy = ((ny + 15) / 16) * 8, cx = ((nx + 15) / 16) * 8;
     hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, s, tmax, 1, 0.0f);
-    hipLaunchKernelGGL(k_max_nonneg, dim3((unsigned)std::min<int64_t>((n / 4 + 255) / 256 + 1, 2048)), dim3(256), 0, s,
+    // a grid-stride loop over at most 256 workgroups: each ends in one atomic
+    // on the same word, and thousands of them serialise for longer than the read
+    hipLaunchKernelGGL(k_max_nonneg, dim3((unsigned)std::min<int64_t>((n / 4 + 255) / 256 + 1, 256)), dim3(256), 0, s,
                        terma, n, tmax);
     const int64_t items = (int64_t)cz * cy * cx;
     dh_timed t("sur_pack", s);
@@ -1058,7 +1062,7 @@ void launch_finish(int nz, int ny, int nx, const float *y32, const float *tmax, 
     hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, s, dmax, 1, 0.0f);
     dh_timed t("sur_finish", s);
     const int64_t ncell = (int64_t)((nz + 1) / 2) * ((ny + 1) / 2) * ((nx + 1) / 2);
-    hipLaunchKernelGGL(k_sur_finish, dim3((unsigned)std::min<int64_t>((ncell + 255) / 256, 2048)), dim3(256), 0, s,
+    hipLaunchKernelGGL(k_sur_finish, dim3((unsigned)std::min<int64_t>((ncell + 255) / 256, 512)), dim3(256), 0, s,
                        y32, cy, cx, tmax, out_scale, terma, electron, mu_scale, nz, ny, nx, dose, dmax);
     DH_HIP(hipGetLastError());
 }
@@ -1237,12 +1241,38 @@ extern "C" int dh_sur_net_forward_dev(const dh_sur_net *net, const void *d_x, in
     });
 }
 
+// Side stream (default priority) and its fork / join events, per device:
+// batched points alternate between the caller's stream and this one.
+struct sur_side {
+    hipStream_t st = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+};
+
+static sur_side &sur_side_of(int dev)
+{
+    static std::mutex mu;
+    static sur_side side[64];
+    DH_REQUIRE(dev >= 0 && dev < 64, "device index out of range");
+    std::lock_guard<std::mutex> lk(mu);
+    sur_side &sd = side[dev];
+    if (!sd.st) {
+        DH_HIP(hipStreamCreateWithFlags(&sd.st, hipStreamNonBlocking));
+        DH_HIP(hipEventCreateWithFlags(&sd.fork, hipEventDisableTiming));
+        DH_HIP(hipEventCreateWithFlags(&sd.join, hipEventDisableTiming));
+    }
+    return sd;
+}
+
 // Checked surrogate doses of ncp control points on one grid (one density,
 // per-point TERMA and electron volumes): each point is packed into its slot
-// of one batch, the network runs once per batch of up to kSurBatch points,
-// and each point is finished and checked in turn on the stream.  Every
-// layer uses the variant of its one-sample shape, so each dose equals the
-// single-point result bit for bit.
+// of one batch and the network runs once per batch of up to kSurBatch
+// points.  The points' finish and lattice check then alternate between the
+// caller's stream and a side stream with a second set of check buffers, so
+// one point's lattice pass (a short, latency-bound cone launch) overlaps
+// another's fallback pass; the side stream joins the caller's before the
+// next batch reuses the network buffers.  Every layer uses the variant of
+// its one-sample shape and every point the same kernels on its own buffers,
+// so each dose equals the single-point result bit for bit.
 void dh_launch_sur_dose_batch(const dh_sur_net *net, const dh_ccc_args *cas, int ncp,
                               const float *const *terma, const float *density, const float *const *electron,
                               const float (*dirs)[3], int stride, float tol, int dilate, float *const *dose,
@@ -1262,21 +1292,40 @@ void dh_launch_sur_dose_batch(const dh_sur_net *net, const dh_ccc_args *cas, int
     float *y32 = (float *)dh_scratch(dev, DH_SLOT_NN_Y, sizeof(float) * (size_t)(nb * cells * kSurOut));
     float *scal = (float *)dh_scratch(dev, DH_SLOT_NN_SCAL, 256);     // {max TERMA, max dose} per slot
     const int64_t wl = dh_sur_check_work_len(nz, ny, nx, stride);
-    int *work = (int *)dh_scratch(dev, DH_SLOT_NN_WORK, sizeof(int) * (size_t)wl);
-    float *plain = (float *)dh_scratch(dev, DH_SLOT_NN_PLAIN, sizeof(float) * (size_t)nz * ny * nx);
-    void *trp = dh_scratch(dev, DH_SLOT_TR_PAD, dh_tr_pad_bytes(cas[0]));
+    const size_t nvb = sizeof(float) * (size_t)nz * ny * nx;
+    const bool two = ncp > 1;
+    int *work[2] = {(int *)dh_scratch(dev, DH_SLOT_NN_WORK, sizeof(int) * (size_t)wl), nullptr};
+    float *plain[2] = {(float *)dh_scratch(dev, DH_SLOT_NN_PLAIN, nvb), nullptr};
+    void *trp[2] = {dh_scratch(dev, DH_SLOT_TR_PAD, dh_tr_pad_bytes(cas[0])), nullptr};
+    sur_side *side = nullptr;
+    if (two) {
+        work[1] = (int *)dh_scratch(dev, DH_SLOT_NN_WORK2, sizeof(int) * (size_t)wl);
+        plain[1] = (float *)dh_scratch(dev, DH_SLOT_NN_PLAIN2, nvb);
+        trp[1] = dh_scratch(dev, DH_SLOT_TR_PAD2, dh_tr_pad_bytes(cas[0]));
+        side = &sur_side_of(dev);
+    }
     for (int c0 = 0; c0 < ncp; c0 += nb) {
         const int n = std::min(nb, ncp - c0);
         for (int i = 0; i < n; i++)
             launch_pack(nz, ny, nx, terma[c0 + i], density, dirs[c0 + i], g.s, x + (size_t)i * cells * net->in_ch,
                         scal + 2 * i, s);
         sur_forward(net, x, n, cz, cy, cx, y32, ws, s);
+        if (two && n > 1) {
+            DH_HIP(hipEventRecord(side->fork, s));
+            DH_HIP(hipStreamWaitEvent(side->st, side->fork, 0));
+        }
         for (int i = 0; i < n; i++) {
+            const int k = (two && n > 1) ? (i & 1) : 0;
+            hipStream_t st = k ? side->st : s;
             const dh_ccc_args &ca = cas[c0 + i];
             launch_finish(nz, ny, nx, y32 + (size_t)i * cells * kSurOut, scal + 2 * i, net->out_scale,
-                          terma[c0 + i], electron[c0 + i], ca.mu_cal * ca.mu, dose[c0 + i], scal + 2 * i + 1, s);
+                          terma[c0 + i], electron[c0 + i], ca.mu_cal * ca.mu, dose[c0 + i], scal + 2 * i + 1, st);
             dh_launch_sur_check(ca, terma[c0 + i], density, electron[c0 + i], dose[c0 + i], scal + 2 * i + 1,
-                                stride, tol, dilate, work, plain, trp, s);
+                                stride, tol, dilate, work[k], plain[k], trp[k], st);
+        }
+        if (two && n > 1) {
+            DH_HIP(hipEventRecord(side->join, side->st));
+            DH_HIP(hipStreamWaitEvent(s, side->join, 0));
         }
     }
 }

@@ -500,3 +500,21 @@ def test_dose_batch_equals_single(photon_model, shape, n_cp):
         assert t.equal(got[i], want), f"control point {i}"
     with pytest.raises(ValueError):
         sur.dose_batch(termas[:1], rho, els, s, np.zeros(3), cps)
+    with pytest.raises(ValueError):
+        sur.dose_batch(termas[:2], rho, els[:2], s, np.zeros(3), cps[:2], outs=[got[0], got[0]])
+    # the batch (with its side stream) captured as one HIP graph; replays
+    # interleaved with eager work must reproduce the eager doses
+    outs = [t.empty_like(rho) for _ in cps]
+    sur.dose_batch(termas, rho, els, s, np.zeros(3), cps, outs=outs)
+    t.cuda.synchronize()
+    gr = t.cuda.CUDAGraph()
+    with t.cuda.graph(gr):
+        sur.dose_batch(termas, rho, els, s, np.zeros(3), cps, outs=outs)
+    for r in range(3):
+        for o in outs:
+            o.fill_(-1.0)
+        gr.replay()
+        t.cuda.synchronize()
+        for i in range(n_cp):
+            assert t.equal(outs[i], got[i]), f"replay {r}, control point {i}"
+        sur.dose(termas[0], rho, els[0], s, np.zeros(3), *cps[0])
