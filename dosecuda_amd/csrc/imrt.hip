@@ -63,21 +63,36 @@ __device__ static inline float head_transmission(const dh_terma_args &a, float h
         q1 = lo;
     }
     float tr = 0.0f;
-    // Adjacent leaf pairs usually share a y edge: when this pair's lower edge
-    // is the same float as the previous evaluated pair's upper edge, its
-    // scaled coordinate and erf are the same values, so the erf is reused.
-    // Likewise a pair whose leaf tips equal the previous evaluated pair's has
-    // the same x factor (rectangular apertures: every open pair), and a pair
-    // whose x factor is exactly 0 adds nothing.  The sum is unchanged bit for
-    // bit.
-    float prev_w = NAN, prev_erf = 0.0f;
+    // Consecutive pairs with equal leaf tips (the same x factor) whose y
+    // ranges meet edge to edge form a run, and the run's terms telescope:
+    // sum_q ex * (erf(y2_q) - erf(y1_q)) / 2 = ex * (erf(y2_last) - erf(y1_first)) / 2,
+    // so a run costs two erf evaluations (one when the previous run ends on
+    // its lower edge) instead of one per pair; a rectangular aperture is one
+    // run.  Pairs whose edges lie on one saturated side of erf, or whose x
+    // factor is exactly 0 (closed), add nothing and end the run.
+    bool run = false;
+    float run_ex = 0.0f, run_f1 = 0.0f, run_y2 = 0.0f, run_w = NAN, run_x1 = NAN, run_x2 = NAN;
+    float prev_w = NAN, prev_erf = 0.0f;            // upper edge of the last run and its erf
     float prev_x1 = NAN, prev_x2 = NAN, prev_ex = 0.0f;
     for (int q = q0; q < q1; q++) {
         const float4 m = a.mlc[q];
         const float e1 = ((m.z - hy) * dv) + hy;
         const float e2 = ((m.w - hy) * dv) + hy;
         const float y1 = e1 * inv, y2 = e2 * inv;
-        if (erf_pair_zero(y1, y2)) {
+        const bool y_zero = erf_pair_zero(y1, y2);
+        if (!y_zero && run && m.x == run_x1 && m.y == run_x2 && m.z == run_w) {
+            run_y2 = y2;
+            run_w = m.w;
+            continue;
+        }
+        if (run) {
+            const float f2 = erff(run_y2);
+            tr += run_ex * (0.5f * (f2 - run_f1));
+            prev_w = run_w;
+            prev_erf = f2;
+            run = false;
+        }
+        if (y_zero) {
             prev_w = NAN;
             continue;
         }
@@ -97,19 +112,19 @@ __device__ static inline float head_transmission(const dh_terma_args &a, float h
             prev_x2 = m.y;
             prev_ex = ex;
         }
-        // a closed pair (equal tips) has ex = 0 exactly: its term adds +0 to
-        // the non-negative sum, so its y factor is not needed
         if (ex == 0.0f) {
             prev_w = NAN;
             continue;
         }
-        const float f1 = m.z == prev_w ? prev_erf : erff(y1);
-        const float f2 = erff(y2);
-        prev_w = m.w;
-        prev_erf = f2;
-        const float ey = 0.5f * (f2 - f1);
-        tr += ex * ey;
+        run = true;
+        run_ex = ex;
+        run_x1 = m.x;
+        run_x2 = m.y;
+        run_w = m.w;
+        run_y2 = y2;
+        run_f1 = m.z == prev_w ? prev_erf : erff(y1);
     }
+    if (run) tr += run_ex * (0.5f * (erff(run_y2) - run_f1));
     return tr;
 }
 
