@@ -405,6 +405,19 @@ static dh_terma_args make_terma_args(const dh_grid *grid, const dh_photon_cp *cp
         a.mlc[q++] = make_float4(r[0] * ms, r[1] * ms, (r[2] - 0.5f * r[3]) * ms,
                                  (r[2] + 0.5f * r[3]) * ms);
     }
+    // Consecutive open pairs with equal leaf tips whose y ranges meet edge to
+    // edge become one rectangle: the head transmission sums such a run as
+    // ex * (erf(y_top) - erf(y_bottom)) / 2 (the per-pair terms telescope),
+    // which needs only the run's outer edges.  A rectangular field is one.
+    int nr = 0;
+    for (int i = 0; i < q; i++) {
+        if (nr > 0 && a.mlc[i].x == a.mlc[nr - 1].x && a.mlc[i].y == a.mlc[nr - 1].y &&
+            a.mlc[i].z == a.mlc[nr - 1].w)
+            a.mlc[nr - 1].w = a.mlc[i].w;
+        else
+            a.mlc[nr++] = a.mlc[i];
+    }
+    q = nr;
     a.n_pairs = q;
     a.sorted = 1;
     for (int i = 1; i < q; i++)
