@@ -498,6 +498,51 @@ static void kernel_tilt_d(const orc_geom *g, const double p[3], double d[3])
     d[0] = rx + kx * sc; d[1] = ry + ky * sc; d[2] = rz + kz * sc;
 }
 
+/* Exported single-vector views of the geometry above, so the tests can pin
+ * the rotation chain and the tilt by their invariants.  fp32 != 0 selects
+ * the FP32 decision-arithmetic variants (CudaClasses.cu:65-126 with the
+ * collimator step of IMRTClasses.cu:41-118); otherwise the FP64 ones. */
+void orc_img_to_head(const orc_geom *g, int fp32, const double in[3], double out[3])
+{
+    if (fp32) {
+        float a[3] = { (float)in[0], (float)in[1], (float)in[2] }, b[3];
+        img_to_head_f(g, a, b);
+        for (int c = 0; c < 3; c++) out[c] = b[c];
+    } else {
+        img_to_head_d(g, in, out);
+    }
+}
+
+void orc_head_to_img(const orc_geom *g, int fp32, const double in[3], double out[3])
+{
+    if (fp32) {
+        float a[3] = { (float)in[0], (float)in[1], (float)in[2] }, b[3];
+        head_to_img_f(g, a, b);
+        for (int c = 0; c < 3; c++) out[c] = b[c];
+    } else {
+        head_to_img_d(g, in, out);
+    }
+}
+
+/* kernelTilt of an image-frame vector d for the voxel at image point p. */
+void orc_kernel_tilt(const orc_geom *g, const double p[3], double d[3])
+{
+    kernel_tilt_d(g, p, d);
+}
+
+/* Image-frame direction of cone (row r, azimuth az) at image point p, as
+ * orc_ccc forms it (IMRTClasses.cu:282-306). */
+void orc_cone_dir(const orc_geom *g, const float *kernel36, int r, int az, const double p[3],
+                  double d[3])
+{
+    float thf = kernel36[6 * r + 0] * ORC_PI_F / 180.0f;
+    double sth = sin((double)thf), cth = cos((double)thf);
+    double phi = (double)((float)az * ORC_PI_F / 6.0f);
+    double dh[3] = { sth * cos(phi), sth * sin(phi), cth };
+    head_to_img_d(g, dh, d);
+    kernel_tilt_d(g, p, d);
+}
+
 /* cccKernel, IMRTClasses.cu:252-350.  `terma`, `rho`, `electron` are the
  * FP32 volumes the kernel samples.  Returns the number of cone samples. */
 int64_t orc_ccc(const orc_geom *g, const orc_photon_model *m, double mu,
@@ -526,12 +571,10 @@ int64_t orc_ccc(const orc_geom *g, const orc_photon_model *m, double mu,
                     double A = m->kernel[6 * r + 1], a = m->kernel[6 * r + 2];
                     double B = m->kernel[6 * r + 3], b = m->kernel[6 * r + 4];
                     float L0 = m->kernel[6 * r + 5];
-                    double sth = sin((double)thf), cth = cos((double)thf);
+                    double sth = sin((double)thf);
                     for (int az = 0; az < 12; az++) {
-                        double phi = (double)((float)az * ORC_PI_F / 6.0f);
-                        double dh[3] = { sth * cos(phi), sth * sin(phi), cth }, d[3];
-                        head_to_img_d(g, dh, d);
-                        kernel_tilt_d(g, p, d);
+                        double d[3];
+                        orc_cone_dir(g, m->kernel, r, az, p, d);
                         double Rp = 0.0, Rs = 0.0;
                         for (float L = L0; L >= 0.0f; L = L - sp) {
                             double Lmm = (double)(L * 10.0f);

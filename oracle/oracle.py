@@ -63,31 +63,60 @@ def build(force: bool = False) -> str:
 _lib = None
 
 
+def load(path: str):
+    """Open an oracle library at `path` and declare its signatures."""
+    L = C.CDLL(path)
+    P = C.c_void_p
+    L.orc_geom_init.argtypes = [C.POINTER(Geom), C.c_int, C.c_int, C.c_int, C.c_float,
+                                P, P, C.c_double, C.c_double, C.c_double, C.c_float]
+    L.orc_sample.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                             C.c_double, C.c_double]
+    L.orc_sample.restype = C.c_double
+    L.orc_raytrace.argtypes = [C.POINTER(Geom), P, C.c_double, P, P]
+    L.orc_smooth.argtypes = [C.POINTER(Geom), P, P]
+    L.orc_make_layers.argtypes = [P, C.c_int, C.c_int, P, P, P]
+    L.orc_make_layers.restype = C.c_int
+    L.orc_pencil.argtypes = [C.POINTER(Geom), C.c_float, C.c_float, P, P, P, C.c_int,
+                             P, P, P, P, C.c_int, P]
+    L.orc_pencil.restype = C.c_int64
+    L.orc_terma.argtypes = [C.POINTER(Geom), C.POINTER(PhotonModel), P, C.c_int, P, P, P]
+    L.orc_ccc.argtypes = [C.POINTER(Geom), C.POINTER(PhotonModel), C.c_double, P, P, P, P, P]
+    L.orc_ccc.restype = C.c_int64
+    L.orc_cone_steps.argtypes = [C.c_float, C.c_float]
+    L.orc_cone_steps.restype = C.c_int
+    L.orc_img_to_head.argtypes = [C.POINTER(Geom), C.c_int, P, P]
+    L.orc_head_to_img.argtypes = [C.POINTER(Geom), C.c_int, P, P]
+    L.orc_kernel_tilt.argtypes = [C.POINTER(Geom), P, P]
+    L.orc_cone_dir.argtypes = [C.POINTER(Geom), P, C.c_int, C.c_int, P, P]
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
         build()
-        L = C.CDLL(_LIB_PATH)
-        P = C.c_void_p
-        L.orc_geom_init.argtypes = [C.POINTER(Geom), C.c_int, C.c_int, C.c_int, C.c_float,
-                                    P, P, C.c_double, C.c_double, C.c_double, C.c_float]
-        L.orc_sample.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
-                                 C.c_double, C.c_double]
-        L.orc_sample.restype = C.c_double
-        L.orc_raytrace.argtypes = [C.POINTER(Geom), P, C.c_double, P, P]
-        L.orc_smooth.argtypes = [C.POINTER(Geom), P, P]
-        L.orc_make_layers.argtypes = [P, C.c_int, C.c_int, P, P, P]
-        L.orc_make_layers.restype = C.c_int
-        L.orc_pencil.argtypes = [C.POINTER(Geom), C.c_float, C.c_float, P, P, P, C.c_int,
-                                 P, P, P, P, C.c_int, P]
-        L.orc_pencil.restype = C.c_int64
-        L.orc_terma.argtypes = [C.POINTER(Geom), C.POINTER(PhotonModel), P, C.c_int, P, P, P]
-        L.orc_ccc.argtypes = [C.POINTER(Geom), C.POINTER(PhotonModel), C.c_double, P, P, P, P, P]
-        L.orc_ccc.restype = C.c_int64
-        L.orc_cone_steps.argtypes = [C.c_float, C.c_float]
-        L.orc_cone_steps.restype = C.c_int
-        _lib = L
+        _lib = load(_LIB_PATH)
     return _lib
+
+
+class using:
+    """Context manager: route every helper of this module through the oracle
+    library at `path` (a variant built from a modified dose_oracle.c, for the
+    tests that check that the pins catch planted mistakes)."""
+
+    def __init__(self, path: str):
+        self.path = path
+
+    def __enter__(self):
+        global _lib
+        self.saved = lib()
+        _lib = load(self.path)
+        return _lib
+
+    def __exit__(self, *exc):
+        global _lib
+        _lib = self.saved
+        return False
 
 
 def _p(a):
@@ -108,6 +137,38 @@ def geom(shape, spacing, origin, iso, gantry, couch, collimator, src_dist) -> Ge
                         float(gantry), float(couch), float(collimator),
                         C.c_float(float(src_dist)))
     return g
+
+
+def img_to_head(g: Geom, p, fp32=False):
+    """Image -> head (beam's-eye) coordinates of one point."""
+    a = np.ascontiguousarray(p, np.float64).reshape(3)
+    out = np.empty(3, np.float64)
+    lib().orc_img_to_head(C.byref(g), int(bool(fp32)), _p(a), _p(out))
+    return out
+
+
+def head_to_img(g: Geom, h, fp32=False):
+    a = np.ascontiguousarray(h, np.float64).reshape(3)
+    out = np.empty(3, np.float64)
+    lib().orc_head_to_img(C.byref(g), int(bool(fp32)), _p(a), _p(out))
+    return out
+
+
+def kernel_tilt(g: Geom, p, d):
+    """kernelTilt (IMRTClasses.cu:173-203) of image-frame vector d at point p."""
+    a = np.ascontiguousarray(p, np.float64).reshape(3)
+    out = np.ascontiguousarray(d, np.float64).reshape(3).copy()
+    lib().orc_kernel_tilt(C.byref(g), _p(a), _p(out))
+    return out
+
+
+def cone_dir(g: Geom, kernel, r, az, p):
+    """Image-frame direction of cone (zenith row r, azimuth az) at point p."""
+    k = np.ascontiguousarray(np.asarray(kernel, np.float32)[:6, :6]).reshape(36)
+    a = np.ascontiguousarray(p, np.float64).reshape(3)
+    out = np.empty(3, np.float64)
+    lib().orc_cone_dir(C.byref(g), _p(k), int(r), int(az), _p(a), _p(out))
+    return out
 
 
 def sample(vol, t, border=0.0):
