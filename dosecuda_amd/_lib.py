@@ -76,6 +76,7 @@ SIGNATURES = {
     "dh_photon_dose": [_I, C.POINTER(Grid), C.POINTER(PhotonCP), C.POINTER(PhotonModel), _P,
                        C.c_int32, _P, _P, _P, _P],
     "dh_proton_raytrace_dev": [_I, C.POINTER(Grid), C.POINTER(ProtonBeam), _P, _P, _P, _P],
+    "dh_proton_raytrace_dev_noalloc": [_I, C.POINTER(Grid), C.POINTER(ProtonBeam), _P, _P, _P],
     "dh_proton_plan_create": [_I, C.POINTER(Grid), C.POINTER(ProtonBeam), C.POINTER(ProtonLut),
                               _P, C.c_int32, C.POINTER(C.c_void_p)],
     "dh_proton_plan_layers": [_P, C.POINTER(C.c_int32), _P, _P],

@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libdosehip.so")
 
-SOURCES = ["impt.hip", "imrt.hip", "hu.hip", "influence.hip", "surrogate.hip", "scheduler.hip", "tuner.hip", "runtime.cpp", "api.cpp"]
+SOURCES = ["impt.hip", "imrt.hip", "hu.hip", "influence.hip", "surrogate.hip", "scheduler.hip", "tuner.hip", "noalloc.hip", "runtime.cpp", "api.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

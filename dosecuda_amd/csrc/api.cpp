@@ -41,9 +41,13 @@ extern "C" int dh_device_count(int *count)
     });
 }
 
+void dh_noalloc_setup(int dev);
 extern "C" int dh_init(int gpu_id)
 {
-    return guarded([&] { dh_device_enter(gpu_id); });
+    return guarded([&] {
+        dh_device_enter(gpu_id);
+        dh_noalloc_setup(gpu_id);   // device address of the fixed scratch array
+    });
 }
 
 void dh_device_release(int dev);

@@ -119,6 +119,18 @@ int dh_photon_dose(int gpu_id, const dh_grid *grid, const dh_photon_cp *cp,
 int dh_proton_raytrace_dev(int gpu_id, const dh_grid *grid, const dh_proton_beam *beam,
                            const float *d_rlsp, float *d_wet_out, float *d_wet_raw_out,
                            void *stream);
+/* dh_proton_raytrace_dev (smoothed WET into d_wet_out) that allocates no
+ * device memory on any call.  No reference counterpart: it is for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The padded volume and the unsmoothed
+ * WET go to one fixed 64 MiB device array per GPU (held from the library's
+ * first use on that GPU until the process ends), so the volume must satisfy
+ * (nx+3)(ny+3)(nz+3)*16 + nx*ny*nz*4 <= 64 MiB (each term rounded up to 256
+ * bytes; about 145^3 voxels), else DH_E_INVALID.  All calls on one GPU share
+ * the array: calls that overlap on one GPU race (order them on one stream).
+ * Same kernels, so the WET equals dh_proton_raytrace_dev's bit for bit. */
+int dh_proton_raytrace_dev_noalloc(int gpu_id, const dh_grid *grid, const dh_proton_beam *beam,
+                                   const float *d_rlsp, float *d_wet_out, void *stream);
 /* A proton plan holds one field's sorted spots, layers and per-layer LUT rows
  * on the device (make_spot_array + importLayers, dosemodule.cu:211-225 and
  * IMPTClasses.cu:29-67), so repeated dose evaluations upload nothing. */
