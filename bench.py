@@ -240,6 +240,117 @@ def cpu_baseline_impt(st, budget_s):
                                          f"{total_pairs} voxel-spot pairs, scaled by pairs"
 
 
+def _errs(got, ref):
+    """(max, RMS) of |got - ref| relative to the oracle's Dmax."""
+    got = np.asarray(got, np.float64)
+    ref = np.asarray(ref, np.float64)
+    dm = float(np.abs(ref).max()) or 1.0
+    d = np.abs(got - ref)
+    return float(d.max()) / dm, float(np.sqrt(np.mean(d * d))) / dm
+
+
+def oracle_errors_impt(st):
+    """The field through the FP64 oracle: the raw WET end to end from the
+    same RLSP; smoothing and pencil beam from the engine's float32 raw WET
+    (the smoothing's neighbour set is a decision on the raw WET), against
+    the engine's smoothed WET and dose."""
+    import torch
+    from dosecuda_amd import engine
+    from oracle import oracle as O
+    m, c = st["m"], st["case"]
+    raw = torch.empty_like(st["d_rlsp"])
+    sm_e = engine.proton_wet(st["d_rlsp"], c.spacing, c.origin, c.iso, 0.0, 0.0, m.VSADX, m.VSADY,
+                             raw_out=raw)
+    raw = raw.cpu().numpy()
+    t0 = time.perf_counter()
+    ref_raw, _, _ = O.proton_wet(st["rlsp"], c.spacing, c.origin, c.iso, 0.0, 0.0, m.VSADX, m.VSADY)
+    g = O.geom(raw.shape, c.spacing, c.origin, c.iso, 0.0, 0.0, 0.0,
+               (np.float32(m.VSADX) + np.float32(m.VSADY)) / np.float32(2.0))
+    sm = O.smooth(g, raw)
+    ref, _ = O.pencil(g, m.VSADX, m.VSADY, sm_e.cpu().numpy(), st["spots"], m.divergence_params, m.lut_depths,
+                      m.lut_sigmas, m.lut_idds)
+    mx, rms = _errs(st["dose"].cpu().numpy(), ref)
+    wmx, wrms = _errs(raw, ref_raw)
+    smx, srms = _errs(sm_e.cpu().numpy(), sm)
+    return dict(max_rel=mx, rms_rel=rms, dmax=float(ref.max()), raw_wet_max_rel=wmx, raw_wet_rms_rel=wrms,
+                smoothed_wet_max_rel=smx, smoothed_wet_rms_rel=srms, oracle_s=time.perf_counter() - t0,
+                what="dose relative to the oracle's Dmax (pencil beam on the engine's smoothed WET); raw WET "
+                     "end to end from the RLSP and smoothed WET from the engine's raw WET, relative to "
+                     "their maxima")
+
+
+def oracle_errors_cccs(torch, st):
+    """Every beam of the plan through the FP64 oracle (ray trace, TERMA,
+    72-cone superposition) from the same density, against the engine's
+    per-beam dose, and the plan sum."""
+    from dosecuda_amd import engine
+    from oracle import oracle as O
+    c, pm = st["case"], st["pm"]
+    t0 = time.perf_counter()
+    beams, psum, rsum = [], 0.0, 0.0
+    terma = torch.empty_like(st["d_dens"])
+    keep = np.ones(st["dens"].shape, bool)
+    for ga in st["angles"]:
+        got = engine.photon_dose(st["d_dens"], c.spacing, c.origin, c.iso, 100.0, ga, 0.0, 0.0, pm,
+                                 st["mlc"], terma_out=terma).cpu().numpy()
+        ref, stg = O.photon_dose(st["dens"], c.spacing, c.origin, c.iso, 100.0, ga, 0.0, 0.0, pm, st["mlc"],
+                                 return_stages=True)
+        # voxels whose TERMA sits on the 0.01 early-out to within rounding
+        # (engine FP32 vs oracle FP64) are all-or-nothing: counted apart
+        flip = (terma.cpu().numpy() <= np.float32(0.01)) != (stg["terma32"] <= np.float32(0.01))
+        mx, rms = _errs(np.where(flip, 0.0, got), np.where(flip, 0.0, ref))
+        beams.append(dict(gantry=float(ga), max_rel=mx, rms_rel=rms, early_out_flips=int(flip.sum()),
+                          flip_terma_max_dev=float(np.abs(stg["terma"][flip] - 0.01).max()) if flip.any() else 0.0))
+        keep &= ~flip
+        psum = psum + got.astype(np.float64)
+        rsum = rsum + ref
+    mx, rms = _errs(np.where(keep, psum, 0.0), np.where(keep, rsum, 0.0))
+    return dict(max_rel=max(b["max_rel"] for b in beams), rms_rel=max(b["rms_rel"] for b in beams),
+                plan_max_rel=mx, plan_rms_rel=rms, early_out_flips=sum(b["early_out_flips"] for b in beams),
+                per_beam=beams, oracle_s=time.perf_counter() - t0,
+                what="each of the 7 beams end to end from the density, relative to that beam's oracle Dmax; "
+                     "voxels whose TERMA lies on the 0.01 early-out to within rounding are counted in "
+                     "early_out_flips and left out")
+
+
+def oracle_errors_influence(torch, st, rows_per_field=8):
+    """A sample of CSR rows against the oracle's single-spot (MU 1) dose on
+    the same float32 WET: kept entries relative to the row peak, and the
+    largest oracle value the truncation dropped."""
+    from oracle import oracle as O
+    c, m = st["case"], st["m"]
+    col = st["col"].cpu().numpy().view(np.uint32)
+    val = st["val"].cpu().numpy()
+    n = int(np.prod(c.shape))
+    t0 = time.perf_counter()
+    worst, rms_w, dropped, nrows = 0.0, 0.0, 0.0, 0
+    rng = np.random.default_rng(1003)
+    for (g, spots), f, w in zip(st["fields"], st["infl"], st["wets"]):
+        wet = w.cpu().numpy()
+        row_spot, peak = f.info()
+        rp = f.row_ptr
+        for q in np.sort(rng.choice(len(row_spot), size=min(rows_per_field, len(row_spot)), replace=False)):
+            one = np.array(spots[row_spot[q]], np.float32).reshape(1, 4)
+            one[0, 2] = 1.0
+            ref, _ = O.proton_dose(wet, c.spacing, c.origin, c.iso, g, 0.0, m.VSADX, m.VSADY, one,
+                                   m.divergence_params, m.lut_depths, m.lut_sigmas, m.lut_idds)
+            ref = ref.reshape(-1)
+            pk = float(ref.max()) or 1.0
+            got = np.zeros(n)
+            a, b = int(rp[q]), int(rp[q + 1])
+            got[col[a:b]] = val[a:b]
+            kept = got != 0
+            d = np.abs(got[kept] - ref[kept]) / pk
+            worst = max(worst, float(d.max()) if d.size else 0.0)
+            rms_w = max(rms_w, float(np.sqrt(np.mean(d * d))) if d.size else 0.0)
+            dropped = max(dropped, float(ref[~kept].max()) / pk if (~kept).any() else 0.0)
+            nrows += 1
+    return dict(rows=nrows, max_rel=worst, rms_rel=rms_w, max_dropped_rel=dropped,
+                oracle_s=time.perf_counter() - t0,
+                what="sampled rows on the engine's float32 WET, relative to each row's oracle peak; "
+                     "max_dropped_rel is the largest oracle value left out by the 1e-4 truncation")
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -289,6 +400,8 @@ def main():
             v, sample = cpu_baseline_impt(st, a.cpu_seconds / 2)
             cpu["impt"] = dict(value=v, unit="Gvoxel-spot/s", sample=sample)
             log(f"IMPT CPU oracle: {v:.4f} Gvoxel-spot/s")
+            res["impt"]["oracle_error"] = oracle_errors_impt(st)
+            log(f"IMPT vs FP64 oracle: {res['impt']['oracle_error']}")
         st["plan"].close()
         del st
 
@@ -348,6 +461,9 @@ def main():
             v, sample = cpu_baseline_cccs(st, a.cpu_seconds)
             cpu["cccs"] = dict(value=v, unit="ms/beam", sample=sample)
             log(f"CCCS CPU oracle: {v:.1f} ms/beam")
+            res["cccs"]["oracle_error"] = oracle_errors_cccs(torch, st)
+            log(f"CCCS vs FP64 oracle: max {res['cccs']['oracle_error']['max_rel']:.2e} "
+                f"rms {res['cccs']['oracle_error']['rms_rel']:.2e}")
 
     if a.workload in ("all", "influence"):
         log("setting up the batched influence matrix: (160,256,256) @ 2 mm, 3 fields, ~15k spots")
@@ -375,6 +491,9 @@ def main():
             c, rad = f.stats()
             sweep.append(dict(c, mean_radius_mm=float(rad.mean())))
         res["influence"]["sweep"] = sweep
+        if not a.no_cpu:
+            res["influence"]["oracle_error"] = oracle_errors_influence(torch, st)
+            log(f"influence rows vs FP64 oracle: {res['influence']['oracle_error']}")
         log(f"influence sweep counters: {sweep}")
         log(f"influence: {ms:.2f} ms/build, nnz {st['nnz']}, fill {fill_ms:.3f} ms/field "
             f"({ach:.0f} GB/s = {100 * ach / HBM_PEAK_GBS:.1f}% of HBM)")
@@ -478,6 +597,9 @@ def main():
         "roofline": cc["roofline"] if cc else None,
         "cpu_baseline": ({"value": cpu["cccs"]["value"], "unit": "ms/beam", "cores": threads,
                           "kind": "port", "sample": cpu["cccs"]["sample"]} if "cccs" in cpu else None),
+        "oracle_error": {k: ({kk: vv for kk, vv in res[k]["oracle_error"].items() if kk != "per_beam"}
+                             if res.get(k) and "oracle_error" in res[k] else None)
+                         for k in ("impt", "cccs", "influence")},
         "impt": res.get("impt"),
         "tuning": {k: res[k] for k in ("impt_tuning", "cccs_tuning") if k in res},
         "impt_cpu_baseline": ({**cpu["impt"], "cores": threads, "kind": "port"} if "impt" in cpu else None),
