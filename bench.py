@@ -321,9 +321,9 @@ def oracle_errors_influence(torch, st, rows_per_field=8):
     c, m = st["case"], st["m"]
     col = st["col"].cpu().numpy().view(np.uint32)
     val = st["val"].cpu().numpy()
-    n = int(np.prod(c.shape))
+    n = st["d_rlsp"].numel()
     t0 = time.perf_counter()
-    worst, rms_w, dropped, nrows = 0.0, 0.0, 0.0, 0
+    worst, rms_w, dropped, nrows, worst_row = 0.0, 0.0, 0.0, 0, None
     rng = np.random.default_rng(1003)
     for (g, spots), f, w in zip(st["fields"], st["infl"], st["wets"]):
         wet = w.cpu().numpy()
@@ -343,9 +343,14 @@ def oracle_errors_influence(torch, st, rows_per_field=8):
             d = np.abs(got[kept] - ref[kept]) / pk
             worst = max(worst, float(d.max()) if d.size else 0.0)
             rms_w = max(rms_w, float(np.sqrt(np.mean(d * d))) if d.size else 0.0)
-            dropped = max(dropped, float(ref[~kept].max()) / pk if (~kept).any() else 0.0)
+            dr = float(ref[~kept].max()) / pk if (~kept).any() else 0.0
+            if dr > dropped:
+                dropped = dr
+                worst_row = dict(gantry=float(g), row=int(q), spot=[float(v) for v in one[0]], entries=b - a,
+                                 engine_peak=float(peak[q]), oracle_peak=pk,
+                                 oracle_peak_voxel=int(np.argmax(ref)))
             nrows += 1
-    return dict(rows=nrows, max_rel=worst, rms_rel=rms_w, max_dropped_rel=dropped,
+    return dict(rows=nrows, max_rel=worst, rms_rel=rms_w, max_dropped_rel=dropped, worst_dropped_row=worst_row,
                 oracle_s=time.perf_counter() - t0,
                 what="sampled rows on the engine's float32 WET, relative to each row's oracle peak; "
                      "max_dropped_rel is the largest oracle value left out by the 1e-4 truncation")

@@ -55,7 +55,7 @@ struct dh_infl_args {
     const float4 *factors;       // [layer][voxel] {ap, log2 P, ah, log2 H} (P, H >= 0 folded)
     uint32_t *col;
     float *val;
-    uint32_t nnz32;              // FILL with 32-bit positions: entries of the matrix
+    uint32_t nnz32;              // FILL with 32-bit positions: end of the field's slots
     int *queue;
     unsigned long long *stats;   // optional: lane visits, active visits, chunks, rows
 };
@@ -712,9 +712,12 @@ static void launch_kernels(dh_influence *f, int n_blocks, const float *d_wet, ui
     const unsigned grid = (unsigned)std::min(a.n_items, cus * 32);
     const char *name = MODE == INF_PEAK ? "influence_peak" : MODE == INF_COUNT ? "influence_count" : "influence_fill";
     dh_timed t(name, s);
-    // the fill takes 32-bit positions below 2^32 entries
-    const bool p32 = MODE == INF_FILL && f->nnz >= 0 && f->nnz < ((int64_t)1 << 32);
-    a.nnz32 = p32 ? (uint32_t)f->nnz : 0u;
+    // the fill takes 32-bit positions when the matrix's last slot (absolute,
+    // counting the `base` of a CSR shared between fields) lies below 2^32;
+    // stores are bounded by that end
+    const int64_t end = f->offsets.empty() ? 0 : f->offsets.back();
+    const bool p32 = MODE == INF_FILL && f->nnz >= 0 && end < ((int64_t)1 << 32);
+    a.nnz32 = p32 ? (uint32_t)end : 0u;
     if constexpr (MODE == INF_FILL) {
         if (p32) {
             if (f->S == 4) hipLaunchKernelGGL((k_influence<4, MODE, true>), dim3(grid), dim3(64), 0, s, a);

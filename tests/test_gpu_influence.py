@@ -82,6 +82,49 @@ def test_influence_rows_match_oracle(proton_model, gantry, S, C):
     infl.close()
 
 
+def test_influence_shared_csr_equals_per_field(proton_model):
+    """Two fields counted into one CSR (the second starting at the first's
+    entry count, `base`) and filled: every slot written, and each field's
+    rows equal, bit for bit, the CSR that field builds alone."""
+    t = _torch()
+    from dosecuda_amd import engine
+    m = proton_model
+    rho, iso, s = _case(seed=43)
+    d = t.from_numpy(rho).cuda()
+    rng = np.random.default_rng(9)
+    fields = []
+    for gantry in (30.0, 200.0):
+        n = 50
+        spots = np.column_stack([rng.uniform(-25, 25, n), rng.uniform(-20, 20, n), np.ones(n),
+                                 rng.choice([10, 18], n)]).astype(F)
+        wet = engine.proton_wet(d, s, np.zeros(3), iso, gantry, 0.0, m.VSADX, m.VSADY)
+        fields.append((gantry, spots, wet))
+    print("[case] influence shared CSR of 2 fields", flush=True)
+    alone = []
+    for gantry, spots, wet in fields:
+        f = engine.InfluenceMatrix(0, rho.shape, s, np.zeros(3), iso, gantry, 0.0, m, spots)
+        rp, col, val = f.build(wet)
+        alone.append((rp.cpu().numpy(), col.clone(), val.clone()))
+        f.close()
+    hs = [engine.InfluenceMatrix(0, rho.shape, s, np.zeros(3), iso, g, 0.0, m, sp) for g, sp, _ in fields]
+    base = 0
+    for f, (_, _, wet) in zip(hs, fields):
+        base += f.count(wet, base)
+    col = t.full((base,), -1, dtype=t.int32, device="cuda")
+    val = t.zeros((base,), dtype=t.float32, device="cuda")
+    for f, (_, _, wet) in zip(hs, fields):
+        f.fill(wet, col, val)
+    t.cuda.synchronize()
+    assert int((col == -1).sum().item()) == 0, "unwritten slots"
+    for f, (rp1, c1, v1) in zip(hs, alone):
+        rp = np.asarray(f.row_ptr)
+        assert np.array_equal(rp - rp[0], rp1)
+        a, b = int(rp[0]), int(rp[-1])
+        assert t.equal(col[a:b], c1) and t.equal(val[a:b], v1)
+        f.close()
+    print(f"[parity] shared CSR: {base} entries, both fields equal their own CSR", flush=True)
+
+
 def test_influence_full_size_properties(proton_model):
     """Batched IMPT influence matrix at full size: (160, 256, 256) @ 2 mm with
     seeded spheres, 3 fields at gantry 0/120/240, ~15k spots over 25 layers."""
