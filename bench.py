@@ -118,15 +118,15 @@ def setup_influence(torch, tune=True):
     st = dict(m=m, case=case, d_rlsp=d, fields=fields, wets=[], infl=[], tuning=[])
     for g, spots in fields:
         w = engine.proton_wet(d, case.spacing, case.origin, case.iso, g, 0.0, m.VSADX, m.VSADY)
-        st["wets"].append(w)
         if tune:
             # spots per workgroup and row chunks of this field (identical CSR
             # required of every candidate); the matrix below reads the choice
             t = autotune.tune_influence(w, rlsp.shape, case.spacing, case.origin, case.iso, g, 0.0,
                                         m, spots, reps=2)
             st["tuning"].append(t["best"])
-        st["infl"].append(engine.InfluenceMatrix(0, rlsp.shape, case.spacing, case.origin, case.iso,
-                                                 g, 0.0, m, spots))
+        f = engine.InfluenceMatrix(0, rlsp.shape, case.spacing, case.origin, case.iso, g, 0.0, m, spots)
+        st["infl"].append(f)
+        st["wets"].append(f.wet(d, out=w))
     # size the shared CSR once
     base = 0
     for f, w in zip(st["infl"], st["wets"]):
@@ -138,13 +138,12 @@ def setup_influence(torch, tune=True):
 
 
 def run_influence(torch, st):
-    """One build of the 3-field CSR: per field ray trace + smoothing, peak and
-    count passes (one host sync each), fill into the shared CSR."""
-    from dosecuda_amd import engine
-    c, m = st["case"], st["m"]
+    """One build of the 3-field CSR: per field the WET on the matrix's
+    footprint (ray trace + smoothing), peak and count passes (one host sync
+    each), fill into the shared CSR."""
     base = 0
-    for (g, _), f, w in zip(st["fields"], st["infl"], st["wets"]):
-        engine.proton_wet(st["d_rlsp"], c.spacing, c.origin, c.iso, g, 0.0, m.VSADX, m.VSADY, out=w)
+    for f, w in zip(st["infl"], st["wets"]):
+        f.wet(st["d_rlsp"], out=w)
         base += f.count(w, base)
     for f, w in zip(st["infl"], st["wets"]):
         f.fill(w, st["col"], st["val"])
@@ -238,6 +237,37 @@ def cpu_baseline_impt(st, budget_s):
     t_field = t_wet + t_p * total_pairs / max(done_pairs, 1)
     return total_pairs / t_field / 1e9, f"WET on the full grid, pencil beam on {done_pairs} of " \
                                          f"{total_pairs} voxel-spot pairs, scaled by pairs"
+
+
+def cpu_baseline_influence(st, budget_s):
+    """FP64 oracle for one influence build: the WET of each field (ray trace
+    + smoothing) on the full grid timed for field 0 and scaled by fields, and
+    the single-spot pencil beam (every voxel of the grid, as a row needs) for
+    a sample of spots, scaled by the spot count."""
+    from oracle import oracle as O
+    c, m = st["case"], st["m"]
+    rlsp = st["d_rlsp"].cpu().numpy()
+    g0 = st["fields"][0][0]
+    t0 = time.perf_counter()
+    _, sm, _ = O.proton_wet(rlsp, c.spacing, c.origin, c.iso, g0, 0.0, m.VSADX, m.VSADY)
+    t_wet = time.perf_counter() - t0
+    wet32 = sm.astype(np.float32)
+    spots = st["fields"][0][1]
+    n_all = sum(len(sp) for _, sp in st["fields"])
+    done, t_sp = 0, 0.0
+    for q in range(0, len(spots), max(1, len(spots) // 64)):
+        one = np.array(spots[q], np.float32).reshape(1, 4)
+        one[0, 2] = 1.0
+        t1 = time.perf_counter()
+        O.proton_dose(wet32, c.spacing, c.origin, c.iso, g0, 0.0, m.VSADX, m.VSADY, one, m.divergence_params,
+                      m.lut_depths, m.lut_sigmas, m.lut_idds)
+        t_sp += time.perf_counter() - t1
+        done += 1
+        if t_sp > budget_s:
+            break
+    ms = (t_wet * len(st["fields"]) + t_sp / max(done, 1) * n_all) * 1000.0
+    return ms, f"WET of field 0 on the full grid (x {len(st['fields'])} fields), single-spot dose of " \
+               f"{done} of {n_all} spots on the full grid, scaled by spots"
 
 
 def _errs(got, ref):
@@ -460,7 +490,10 @@ def main():
                            kernels_ms={k: v[0] / max(v[1], 1) for k, v in kt.items()},
                            roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS,
                                          unit="GB/s", frac=achieved / HBM_PEAK_GBS,
-                                         traffic=CCC_TRAFFIC_BYTES))
+                                         traffic=CCC_TRAFFIC_BYTES,
+                                         traffic_source="profiled constant: rocprofv3 --pmc FETCH_SIZE / "
+                                                        "WRITE_SIZE of k_ccc in this bench, calibrated "
+                                                        "(profiles/hbm_traffic_cccs128_7beams.md)"))
         log(f"CCCS: {ms / nb:.3f} ms/beam (ccc kernel {ccc_ms:.3f} ms, {n_act} active voxels)")
         if not a.no_cpu:
             v, sample = cpu_baseline_cccs(st, a.cpu_seconds)
@@ -486,7 +519,14 @@ def main():
         # field's entries (u32 column + f32 value) and one read of its WET
         bytes_launch = (8 * st["nnz"] + 4 * nvox * len(st["fields"])) / len(st["fields"])
         ach = bytes_launch / (fill_ms * 1e-3) / 1e9
+        # algorithmic bytes of a whole build: the CSR (u32 column + f32 value
+        # per entry, i64 row pointers) and per field the RLSP read plus the
+        # WET written and read once
+        build_bytes = 8 * st["nnz"] + 8 * (n_rows + 1) + len(st["fields"]) * 3 * 4 * nvox
+        build_ach = build_bytes / (ms * 1e-3) / 1e9
         res["influence"] = dict(ms_per_build=ms, nnz=st["nnz"], rows=n_rows, tuning=st["tuning"],
+                                build_roofline=dict(bound="hbm", bytes=build_bytes, achieved=build_ach,
+                                                    peak=HBM_PEAK_GBS, unit="GB/s", frac=build_ach / HBM_PEAK_GBS),
                                 fill_ms=fill_ms, csr_bytes=8 * st["nnz"] + 8 * (n_rows + 1),
                                 kernels_ms={k: v[0] / max(v[1], 1) for k, v in kt.items()},
                                 roofline=dict(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS,
@@ -496,9 +536,19 @@ def main():
             c, rad = f.stats()
             sweep.append(dict(c, mean_radius_mm=float(rad.mean())))
         res["influence"]["sweep"] = sweep
+        fps = []
+        for f, w in zip(st["infl"], st["wets"]):
+            box, on, fb = f.footprint()
+            fps.append(dict(box_mm=[float(v) for v in box], in_use=on, fallbacks=fb,
+                            voxel_fraction=float((w < 1e29).float().mean())))
+        res["influence"]["footprint"] = fps
+        log(f"influence footprints: {fps}")
         if not a.no_cpu:
             res["influence"]["oracle_error"] = oracle_errors_influence(torch, st)
             log(f"influence rows vs FP64 oracle: {res['influence']['oracle_error']}")
+            v, sample = cpu_baseline_influence(st, a.cpu_seconds / 2)
+            cpu["influence"] = dict(value=v, unit="ms/build", sample=sample)
+            log(f"influence CPU oracle: {v:.0f} ms/build")
         log(f"influence sweep counters: {sweep}")
         log(f"influence: {ms:.2f} ms/build, nnz {st['nnz']}, fill {fill_ms:.3f} ms/field "
             f"({ach:.0f} GB/s = {100 * ach / HBM_PEAK_GBS:.1f}% of HBM)")
@@ -522,7 +572,11 @@ def main():
             conv_ms_single = float(np.mean([b["surrogate_stage_ms"]["nn_conv3d"] for b in r["beams"]]))
             conv_ms = r["batch_conv_ms_per_beam"]
             tf = r["net_gflop"] / (conv_ms * 1e-3) / 1e3
+            sur_beam = r["plain_beam_ms"] - r["plain_superposition_ms"] + r["graph_checked_ms_per_beam"]
             res["surrogate"] = dict(
+                speedup_vs_plain_beam=r["plain_beam_ms"] / sur_beam,
+                surrogate_beam_ms=sur_beam, plain_beam_ms=r["plain_beam_ms"],
+                speedup_target=10.0,
                 weights=os.path.relpath(S.WEIGHTS, ROOT), params=r["params"], stride=r["stride"], tol=r["tol"],
                 dilate=r["dilate"],
                 plan_gamma_checked=r["plan_gamma_checked"], min_beam_gamma_checked=r["min_gamma_checked"],
@@ -535,12 +589,12 @@ def main():
                 batch_equal_single=r["batch_equal_single"],
                 plain_superposition_ms_per_beam=r["plain_superposition_ms"],
                 speedup_vs_plain_superposition=r["speedup_superposition"],
-                plain_beam_ms=r["plain_beam_ms"],
-                surrogate_beam_ms=r["plain_beam_ms"] - r["plain_superposition_ms"] + r["graph_checked_ms_per_beam"],
                 mfma=dict(net_gflop=r["net_gflop"], conv_ms_per_beam=conv_ms, achieved_tflops=tf, peak_tflops=2500.0,
                           frac=tf / 2500.0, conv_ms_per_beam_unbatched=conv_ms_single),
                 graph_replay_equal=r["graph_replay_equal"])
-            log(f"surrogate: {r['graph_checked_ms_per_beam']:.3f} ms/beam checked vs {r['plain_superposition_ms']:.3f} "
+            log(f"surrogate: whole beam {sur_beam:.3f} ms vs plain {r['plain_beam_ms']:.3f} ms "
+                f"({r['plain_beam_ms'] / sur_beam:.2f}x); checked superposition "
+                f"{r['graph_checked_ms_per_beam']:.3f} ms/beam vs {r['plain_superposition_ms']:.3f} "
                 f"plain superposition ({r['speedup_superposition']:.2f}x), plan gamma {r['plan_gamma_checked']:.5f}, "
                 f"min beam {r['min_gamma_checked']:.5f}, fallback {r['mean_fallback_fraction']:.3f}, "
                 f"MFMA {tf:.0f} TFLOP/s")
@@ -569,15 +623,21 @@ def main():
         errs = W.errors()
         imrt = [e for e in errs if e["kind"] == "imrt"]
         impt = [e for e in errs if e["kind"] == "impt"]
+        # the IMPT fields' peak/count passes run once at set-up (they size the
+        # CSR arrays the graph fills); both rates are reported
+        count_ms = W.count_s * 1e3
         res["mixed"] = dict(
             counts=cnt, ms_per_batch=ms, cases_per_s=cnt["cases"] / (ms * 1e-3),
+            count_passes_ms=count_ms, ms_per_batch_with_counts=ms + count_ms,
+            cases_per_s_with_counts=cnt["cases"] / ((ms + count_ms) * 1e-3),
             nnz_per_s=cnt["nnz"] / (ms * 1e-3), voxel_cps_per_s=cnt["voxel_cps"] / (ms * 1e-3),
             surrogate=nat is not None,
             imrt_min_gamma=min((e["gamma"] for e in imrt), default=None),
             imrt_max_err=max((e["max_err"] for e in imrt), default=None),
             impt_max_err_csr_vs_dense=max((e["max_err"] for e in impt), default=None),
             per_case=errs)
-        log(f"mixed: {ms / 1e3:.2f} s per batch, {res['mixed']['cases_per_s']:.1f} cases/s, min IMRT gamma "
+        log(f"mixed: {ms / 1e3:.2f} s per batch ({(ms + count_ms) / 1e3:.2f} s with the IMPT count passes, "
+            f"{res['mixed']['cases_per_s_with_counts']:.1f} cases/s), {res['mixed']['cases_per_s']:.1f} cases/s, min IMRT gamma "
             f"{res['mixed']['imrt_min_gamma']}, max IMPT CSR error {res['mixed']['impt_max_err_csr_vs_dense']}")
         W.close()
 
@@ -608,6 +668,8 @@ def main():
         "impt": res.get("impt"),
         "tuning": {k: res[k] for k in ("impt_tuning", "cccs_tuning") if k in res},
         "impt_cpu_baseline": ({**cpu["impt"], "cores": threads, "kind": "port"} if "impt" in cpu else None),
+        "influence_cpu_baseline": ({**cpu["influence"], "cores": threads, "kind": "port"}
+                                   if "influence" in cpu else None),
         "cccs_detail": cc,
         "influence": res.get("influence"),
         "surrogate": res.get("surrogate"),

@@ -162,13 +162,24 @@ struct dh_ccc_args {
     float dir[72 * 3];              // cone directions in image space
 };
 
+// Lateral box in the head (beam's-eye-view) frame: voxels whose head x, y lie
+// outside [x0, x1] x [y0, y1] are skipped by the ray trace (nothing written)
+// and get DH_WET_OUTSIDE from the smoothing.  on = 0: every voxel.
+struct dh_hbox {
+    float x0, x1, y0, y1;
+    int on;
+};
+constexpr float DH_WET_OUTSIDE = 1e30f;   // cm: beyond every layer's range cut
+
 // quad-packed padded volume of the ray trace (float4 per padded voxel)
 void dh_launch_pad_quad(const float *src, float4 *dst, int nx, int ny, int nz, float border,
                         hipStream_t s);
 // choice < 0: the tuning table's choice for the shape (default 0); the
 // pencil's choice is tile + DH_TILE_CHOICES * (LUT in LDS ? 0 : 1)
-void dh_launch_raytrace(const dh_geom &g, const float4 *vol_pad, float *wet, hipStream_t s, int choice = -1);
-void dh_launch_smooth(const dh_geom &g, const float *wet_raw, float *wet_out, hipStream_t s, int choice = -1);
+void dh_launch_raytrace(const dh_geom &g, const float4 *vol_pad, float *wet, hipStream_t s, int choice = -1,
+                        const dh_hbox *box = nullptr);
+void dh_launch_smooth(const dh_geom &g, const float *wet_raw, float *wet_out, hipStream_t s, int choice = -1,
+                      const dh_hbox *box = nullptr);
 void dh_launch_pencil(const dh_geom &g, float vsadx, float vsady, int n_layers,
                       const dh_layer_k *d_layers, const float4 *d_luts, const float *d_spots,
                       bool fold, const float *wet, float *dose, hipStream_t s, int choice = -1);

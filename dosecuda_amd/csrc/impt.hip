@@ -69,9 +69,18 @@ __device__ static inline int safe_samples(const dh_geom &g, float x, float y, fl
     return (int)fmin(floor(Lmin) + 1.0, 1.0e7);
 }
 
+// True when the voxel at image point (x, y, z) lies outside the head-frame box.
+__device__ static inline bool dh_outside(const dh_geom &g, const dh_hbox &box, float x, float y, float z)
+{
+    if (!box.on) return false;
+    float hx, hy, hz;
+    dh_img_to_head(g, x, y, z, hx, hy, hz);
+    return !(hx >= box.x0 && hx <= box.x1 && hy >= box.y0 && hy <= box.y1);
+}
+
 template <bool Off32>
 __global__ void __launch_bounds__(kBlock)
-k_raytrace(dh_geom g, dh_pad p, dh_tile tile, const float4 *__restrict__ vol, float *__restrict__ wet)
+k_raytrace(dh_geom g, dh_pad p, dh_tile tile, dh_hbox box, const float4 *__restrict__ vol, float *__restrict__ wet)
 {
     const float4 *vol_hi = vol + p.pxy;
     int i, j, k;
@@ -79,6 +88,7 @@ k_raytrace(dh_geom g, dh_pad p, dh_tile tile, const float4 *__restrict__ vol, fl
     const uint32_t v = (uint32_t)i + (uint32_t)g.nx * ((uint32_t)j + (uint32_t)g.ny * (uint32_t)k);
     float x, y, z, ux, uy, uz;
     dh_vox_xyz(g, i, j, k, x, y, z);
+    if (dh_outside(g, box, x, y, z)) return;
     dh_uvec(g, x, y, z, ux, uy, uz);
     const float fnx = (float)g.nx, fny = (float)g.ny, fnz = (float)g.nz;
     double acc = 0.0;
@@ -142,7 +152,8 @@ struct smooth_dirs {
 // only radii below 10 * WET(cm).  Neighbour indices come from the reference's
 // FP32 head->image rotation and roundf().
 __global__ void __launch_bounds__(kBlock)
-k_smooth(dh_geom g, smooth_dirs dirs, dh_tile tile, const float *__restrict__ wet_in, float *__restrict__ wet_out)
+k_smooth(dh_geom g, smooth_dirs dirs, dh_tile tile, dh_hbox box, const float *__restrict__ wet_in,
+         float *__restrict__ wet_out)
 {
     int i, j, k;
     if (!dh_tile_voxel(tile, dh_xcd_block(blockIdx.x, gridDim.x), threadIdx.x, g.nx, g.ny, g.nz, i, j, k)) return;
@@ -150,6 +161,10 @@ k_smooth(dh_geom g, smooth_dirs dirs, dh_tile tile, const float *__restrict__ we
     float x, y, z, hx, hy, hz;
     dh_vox_xyz(g, i, j, k, x, y, z);
     dh_img_to_head(g, x, y, z, hx, hy, hz);
+    if (box.on && !(hx >= box.x0 && hx <= box.x1 && hy >= box.y0 && hy <= box.y1)) {
+        wet_out[v] = DH_WET_OUTSIDE;
+        return;
+    }
     const float c = wet_in[v];
     double sum = (double)c;
     int n = 1;
@@ -284,8 +299,10 @@ void dh_launch_pad_quad(const float *src, float4 *dst, int nx, int ny, int nz, f
     DH_HIP(hipGetLastError());
 }
 
-void dh_launch_raytrace(const dh_geom &g, const float4 *vol_pad, float *wet, hipStream_t s, int choice)
+void dh_launch_raytrace(const dh_geom &g, const float4 *vol_pad, float *wet, hipStream_t s, int choice,
+                        const dh_hbox *box)
 {
+    const dh_hbox b = box ? *box : dh_hbox{0.0f, 0.0f, 0.0f, 0.0f, 0};
     dh_pad p = dh_make_pad(g.nx, g.ny, g.nz);
     if (choice < 0) choice = dh_tune_lookup("raytrace", dh_tune_key_geom(g), 0);
     unsigned grid = 0;
@@ -293,14 +310,16 @@ void dh_launch_raytrace(const dh_geom &g, const float4 *vol_pad, float *wet, hip
     dh_timed t("raytrace", s);
     // 32-bit offsets for quad volumes under 2 GiB whose planes fit 24-bit products
     if ((size_t)16 * p.pxy * p.pz < ((size_t)1 << 31) && p.pxy < ((int64_t)1 << 23))
-        hipLaunchKernelGGL(k_raytrace<true>, dim3(grid), dim3(kBlock), 0, s, g, p, tile, vol_pad, wet);
+        hipLaunchKernelGGL(k_raytrace<true>, dim3(grid), dim3(kBlock), 0, s, g, p, tile, b, vol_pad, wet);
     else
-        hipLaunchKernelGGL(k_raytrace<false>, dim3(grid), dim3(kBlock), 0, s, g, p, tile, vol_pad, wet);
+        hipLaunchKernelGGL(k_raytrace<false>, dim3(grid), dim3(kBlock), 0, s, g, p, tile, b, vol_pad, wet);
     DH_HIP(hipGetLastError());
 }
 
-void dh_launch_smooth(const dh_geom &g, const float *wet_raw, float *wet_out, hipStream_t s, int choice)
+void dh_launch_smooth(const dh_geom &g, const float *wet_raw, float *wet_out, hipStream_t s, int choice,
+                      const dh_hbox *box)
 {
+    const dh_hbox b = box ? *box : dh_hbox{0.0f, 0.0f, 0.0f, 0.0f, 0};
     smooth_dirs d;
     for (int a = 0; a < 6; a++) {           // IMPTClasses.cu:170
         float ang = (float)a * DH_PI_F / 3.0f;
@@ -317,7 +336,7 @@ void dh_launch_smooth(const dh_geom &g, const float *wet_raw, float *wet_out, hi
     unsigned grid = 0;
     const dh_tile tile = dh_make_tile(choice, g.nx, g.ny, g.nz, &grid);
     dh_timed t("smooth", s);
-    hipLaunchKernelGGL(k_smooth, dim3(grid), dim3(kBlock), 0, s, g, d, tile, wet_raw, wet_out);
+    hipLaunchKernelGGL(k_smooth, dim3(grid), dim3(kBlock), 0, s, g, d, tile, b, wet_raw, wet_out);
     DH_HIP(hipGetLastError());
 }
 

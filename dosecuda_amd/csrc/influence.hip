@@ -468,6 +468,8 @@ struct dh_influence {
     std::vector<float> spots;        // 8 floats per sorted spot
     std::vector<int> spot_layer;
     std::vector<float> sig_max, dose_max;   // per-layer bounds
+    std::vector<std::vector<float4>> cells;  // per layer: depth-resolved table bounds
+    std::vector<float> lthr;         // per layer: smallest threshold of the last count (inf: none yet)
     std::vector<int> block_layer, block_first, block_n;
     std::vector<int> block_spots;    // S per block
     dh_layer_k *d_layers = nullptr;
@@ -492,6 +494,13 @@ struct dh_influence {
     std::vector<int64_t> offsets;    // host copy of the exclusive scan
     int64_t nnz = -1;
     int fill_blocks = -1;            // blocks of the prepared fill pass in d_blocks (-1: none)
+    // footprint-limited WET (dh_influence_wet): the inner head-frame box
+    // where the WET is exact, the RLSP it came from and the buffer it went to
+    bool fp_on = false;
+    const float *fp_rlsp = nullptr;
+    const float *fp_wet = nullptr;
+    dh_hbox fp_inner{0.0f, 0.0f, 0.0f, 0.0f, 0};
+    int64_t fp_fallbacks = 0;
 };
 
 // Conservative per-layer bounds on the total sigma and on the dose factor
@@ -533,6 +542,79 @@ static void layer_bounds(const dh_layer_k &L, const float4 *lut, float dist_min,
     dose_max = 1.05f * idd_max / (2.0f * DH_PI_F * st_min * st_min);
 }
 
+// Depth-resolved table bounds of one layer: for each 1 mm cell [d_q, d_q+1]
+// of the depth table within the range cut, bounds over every WET in the
+// cell and every source distance in [dist_min, dist_max] of
+// {sigma_t max, P max, sigma_ht max, H max} (the formulas of
+// dh_pencil_factors: IDD and sigma_ms interpolate linearly, so their extremes
+// are at the cell ends; sigma_air is a quadratic in its argument; the halo
+// forms are sampled at the cell ends and middle and widened by 2%).
+static std::vector<float4> depth_cells(const dh_layer_k &L, const float4 *lut, float dist_min, float dist_max)
+{
+    std::vector<float4> out;
+    const float r80 = L.r80;
+    auto halo = [&](float w, float &sh, float &hw) {
+        const float wc = std::min(std::max(w, 0.1f), r80 - 0.1f);
+        const float t5 = wc - r80 - 5.0f, r3 = r80 + 3.0f;
+        sh = 2.85f + 0.0014f * r80 * logf(wc + 3.0f) + 0.06f * wc - 7.4e-5f * wc * wc - 0.22f * r80 / (t5 * t5);
+        hw = 0.052f * logf(1.13f + wc / (11.2f - 0.023f * r80)) +
+             0.35f * (0.0017f * r80 * r80 - r80) / (r3 * r3 - wc * wc) - 1.61e-9f * wc * r3 * r3;
+        sh = std::max(sh, 0.0f);
+        hw = std::min(std::max(hw, 0.0f), 0.9f);
+    };
+    for (int q = 0; q + 1 < DH_LUT_LENGTH; q++) {
+        // the first cell also covers the small negative WET of voxels next
+        // to the upstream face (-0.05 cm + one sample), which read the first
+        // table entry
+        const float w0 = q == 0 ? std::min(lut[0].x, 0.0f) - 1.0f : lut[q].x;
+        const float w1 = std::min(lut[q + 1].x, L.cut);
+        if (q > 0 && w0 > L.cut) break;
+        if (w1 < w0) continue;
+        const float idd = std::max(std::max(lut[q].z, lut[q + 1].z), 0.0f);
+        const float sm0 = std::min(lut[q].y, lut[q + 1].y), sm1 = std::max(lut[q].y, lut[q + 1].y);
+        const float a0 = dist_min - w1 + 0.7f * r80, a1 = dist_max - w0 + 0.7f * r80;
+        auto sa = [&](float d) { return (L.A * d + L.B) * d + L.C; };
+        float sa0 = std::min(sa(a0), sa(a1)), sa1 = std::max(sa(a0), sa(a1));
+        if (L.A != 0.0f) {
+            const float dv = -L.B / (2.0f * L.A);
+            if (dv > a0 && dv < a1) { sa0 = std::min(sa0, sa(dv)); sa1 = std::max(sa1, sa(dv)); }
+        }
+        const float st_lo = sa0 + sm0, st_hi = sa1 + sm1;
+        // |sigma_t| range (P uses sigma_t^2); a range through 0 has no bound
+        const float at_min = (st_lo <= 0.0f && st_hi >= 0.0f) ? 0.0f : std::min(std::fabs(st_lo), std::fabs(st_hi));
+        const float at_max = 1.01f * std::max(std::fabs(st_lo), std::fabs(st_hi));
+        float sh_lo = 1e30f, sh_hi = 0.0f, hw_lo = 1.0f, hw_hi = 0.0f;
+        for (int e = 0; e < 3; e++) {
+            float sh, hw;
+            halo(e == 0 ? w0 : e == 1 ? w1 : 0.5f * (w0 + w1), sh, hw);
+            sh_lo = std::min(sh_lo, sh); sh_hi = std::max(sh_hi, sh);
+            hw_lo = std::min(hw_lo, hw); hw_hi = std::max(hw_hi, hw);
+        }
+        sh_lo *= 0.98f; sh_hi *= 1.02f;
+        hw_lo = std::max(0.0f, hw_lo - 0.005f); hw_hi = std::min(0.9f, hw_hi + 0.005f);
+        const float sht_min = std::sqrt(at_min * at_min + sh_lo * sh_lo);
+        const float sht_max = 1.01f * std::sqrt(at_max * at_max + sh_hi * sh_hi);
+        const float P = at_min > 0.0f ? 1.02f * (1.0f - hw_lo) * idd / (2.0f * DH_PI_F * at_min * at_min) : INFINITY;
+        const float H = sht_min > 0.0f ? 1.02f * hw_hi * idd / (2.0f * DH_PI_F * sht_min * sht_min) : INFINITY;
+        out.push_back(make_float4(at_max, P, sht_max, H));
+    }
+    return out;
+}
+
+// Radius beyond which every voxel of the layer, whatever its WET, has a dose
+// below thr: both Gaussian terms below thr / 2 in every depth cell.
+static float depth_radius(const std::vector<float4> &cells, float thr)
+{
+    if (!(thr > 0.0f)) return INFINITY;
+    float R = 0.0f;
+    for (const float4 &c : cells) {
+        if (!std::isfinite(c.y) || !std::isfinite(c.w)) return INFINITY;
+        if (2.0f * c.y > thr) R = std::max(R, c.x * std::sqrt(2.0f * std::log(2.0f * c.y / thr)));
+        if (2.0f * c.w > thr) R = std::max(R, c.z * std::sqrt(2.0f * std::log(2.0f * c.w / thr)));
+    }
+    return 1.002f * R;
+}
+
 dh_influence *dh_influence_make(int dev, const dh_geom &g, float vsadx, float vsady,
                                 const std::vector<dh_layer_k> &layers, const std::vector<float4> &luts,
                                 const std::vector<float> &spots, int n_spots, int S, int C, float rel)
@@ -561,6 +643,8 @@ dh_influence *dh_influence_make(int dev, const dh_geom &g, float vsadx, float vs
         layer_bounds(layers[l], &luts[l * DH_LUT_LENGTH], dmin, dmax, sm, dm);
         f->sig_max.push_back(sm);
         f->dose_max.push_back(dm);
+        f->cells.push_back(depth_cells(layers[l], &luts[l * DH_LUT_LENGTH], dmin, dmax));
+        f->lthr.push_back(INFINITY);
         // blocks of S spatially adjacent spots: sort the layer's spots by a
         // Morton key of their position on a 4 mm grid
         const int a0 = layers[l].start, n = layers[l].n;
@@ -743,6 +827,114 @@ static void launch(dh_influence *f, const std::vector<dh_infl_block> &blocks, co
     launch_kernels<MODE>(f, (int)blocks.size(), d_wet, d_col, d_val, s);
 }
 
+// Radius beyond which every voxel of layer l, whatever its WET, has a dose
+// below thr: dose <= (P + H) e^(-d^2 / 2 sigma^2) with the layer's table
+// bounds P + H <= dose_max and sigma_t, sigma_ht <= sig_max.
+static float table_radius(const dh_influence *f, int l, float thr)
+{
+    return depth_radius(f->cells[l], thr);
+}
+
+// Divergence factors 1 - hz / VSAD over the volume's head depths.
+static void divergence_range(const dh_influence *f, float &cx0, float &cx1, float &cy0, float &cy1)
+{
+    float hz_min, hz_max;
+    head_depth_range(f->g, hz_min, hz_max);
+    cx0 = std::min(1.0f - hz_max / f->vsadx, 1.0f - hz_min / f->vsadx);
+    cx1 = std::max(1.0f - hz_max / f->vsadx, 1.0f - hz_min / f->vsadx);
+    cy0 = std::min(1.0f - hz_max / f->vsady, 1.0f - hz_min / f->vsady);
+    cy1 = std::max(1.0f - hz_max / f->vsady, 1.0f - hz_min / f->vsady);
+}
+
+// Smoothed WET of the field on the device: ray trace + lateral smoothing
+// (CudaClasses.cu:180-228, IMPTClasses.cu:139-202) limited to a head-frame
+// box, or on every voxel when box is null.  The ray trace covers the box
+// widened by the smoothing's reach (9 mm radius, rounded to the nearest
+// voxel), so every value inside the box equals the full-grid one bit for bit.
+static void field_wet(dh_influence *f, const float *d_rlsp, float *d_wet, const dh_hbox *box, hipStream_t s)
+{
+    const dh_geom &g = f->g;
+    const dh_pad p = dh_make_pad(g.nx, g.ny, g.nz);
+    float4 *vpad = (float4 *)dh_scratch(f->dev, DH_SLOT_VOL_PAD, sizeof(float4) * (size_t)p.pxy * p.pz);
+    float *raw = (float *)dh_scratch(f->dev, DH_SLOT_WET_RAW, sizeof(float) * (size_t)g.nx * g.ny * g.nz);
+    dh_launch_pad_quad(d_rlsp, vpad, g.nx, g.ny, g.nz, 0.0f, s);      // IMPTClasses.cu:292, border 0
+    if (box) {
+        const float reach = 10.0f + g.s;
+        const dh_hbox outer{box->x0 - reach, box->x1 + reach, box->y0 - reach, box->y1 + reach, 1};
+        dh_launch_raytrace(g, vpad, raw, s, -1, &outer);
+    } else {
+        dh_launch_raytrace(g, vpad, raw, s);
+    }
+    dh_launch_smooth(g, raw, d_wet, s, -1, box);
+}
+
+// Footprint of the field: the spot axes widened per layer by the table
+// radius at an estimated threshold, rel * 0.05 * (a lower bound of the
+// primary dose at the entrance, IDD(0) / (2 pi sig_max^2) * (1 - 0.9)),
+// scaled by margin_scale.  dh_influence_count checks every spot's table
+// radius at its actual threshold against this box and recomputes the full
+// WET when one does not fit, so the matrix never depends on the estimate.
+static void footprint_wet(dh_influence *f, const float *d_rlsp, float margin_scale, float *d_wet, hipStream_t s)
+{
+    const dh_geom &g = f->g;
+    float cx0, cx1, cy0, cy1;
+    divergence_range(f, cx0, cx1, cy0, cy1);
+    float x0 = 1e30f, x1 = -1e30f, y0 = 1e30f, y1 = -1e30f;
+    for (size_t l = 0; l < f->layers.size(); l++) {
+        // threshold: the layer's smallest one of the last count, else an
+        // estimate from the entrance dose
+        float thr = f->lthr[l];
+        if (!std::isfinite(thr)) {
+            const float idd0 = f->luts[l * DH_LUT_LENGTH].z;
+            const float sm = f->sig_max[l];
+            thr = f->rel * 0.05f * 0.1f * std::max(idd0, 0.0f) / (2.0f * DH_PI_F * sm * sm);
+        }
+        const float R = table_radius(f, (int)l, thr);
+        const float M = margin_scale * (std::isfinite(R) ? R : 1e4f) + 0.87f * g.s;
+        const dh_layer_k &L = f->layers[l];
+        for (int q = L.start; q < L.start + L.n; q++) {
+            const float sx = f->spots[8 * q], sy = f->spots[8 * q + 1];
+            x0 = std::min(x0, std::min(sx * cx0, sx * cx1) - M);
+            x1 = std::max(x1, std::max(sx * cx0, sx * cx1) + M);
+            y0 = std::min(y0, std::min(sy * cy0, sy * cy1) - M);
+            y1 = std::max(y1, std::max(sy * cy0, sy * cy1) + M);
+        }
+    }
+    if (x0 > x1) { x0 = 1.0f; x1 = -1.0f; y0 = 1.0f; y1 = -1.0f; }   // no spots: nothing inside
+    const dh_hbox inner{x0, x1, y0, y1, 1};
+    field_wet(f, d_rlsp, d_wet, &inner, s);
+    f->fp_on = true;
+    f->fp_rlsp = d_rlsp;
+    f->fp_wet = d_wet;
+    f->fp_inner = inner;
+}
+
+// True when every spot's table radius at its threshold fits the footprint:
+// then every voxel outside it is below every threshold (and below every
+// peak), so the matrix equals the one of the full-grid WET.
+static bool footprint_holds(const dh_influence *f, const std::vector<float> &thr)
+{
+    float cx0, cx1, cy0, cy1;
+    divergence_range(f, cx0, cx1, cy0, cy1);
+    const dh_hbox &b = f->fp_inner;
+    // one radius per layer, at the layer's smallest threshold (the largest radius)
+    std::vector<float> lmin(f->layers.size(), INFINITY), Rl(f->layers.size(), 0.0f);
+    for (int q = 0; q < f->n_spots; q++)
+        if (f->spot_layer[q] >= 0) lmin[f->spot_layer[q]] = std::min(lmin[f->spot_layer[q]], thr[q]);
+    for (size_t l = 0; l < lmin.size(); l++)
+        Rl[l] = std::isfinite(lmin[l]) ? table_radius(f, (int)l, lmin[l]) + 0.87f * f->g.s : 0.0f;
+    for (int q = 0; q < f->n_spots; q++) {
+        const int l = f->spot_layer[q];
+        if (l < 0 || !std::isfinite(thr[q])) continue;
+        const float R = Rl[l];
+        const float sx = f->spots[8 * q], sy = f->spots[8 * q + 1];
+        if (!(std::min(sx * cx0, sx * cx1) - R >= b.x0 && std::max(sx * cx0, sx * cx1) + R <= b.x1 &&
+              std::min(sy * cy0, sy * cy1) - R >= b.y0 && std::max(sy * cy0, sy * cy1) + R <= b.y1))
+            return false;
+    }
+    return true;
+}
+
 // PEAK + COUNT; returns nnz and fills the host offsets (exclusive scan over
 // (spot, chunk) in row-major order, starting at base).
 static void active_rows(dh_influence *f, const float *d_wet, hipStream_t s)
@@ -839,6 +1031,16 @@ int64_t dh_influence_count(dh_influence *f, const float *d_wet, int64_t base, hi
         const float rh = Hm > 0.5f * thr[q] ? sht * std::sqrt(2.0f * std::log(2.0f * Hm / thr[q])) : 0.0f;
         r[q] = std::max(rp, rh) + f->g.s;
     }
+    for (size_t l = 0; l < f->lthr.size(); l++) f->lthr[l] = INFINITY;
+    for (int q = 0; q < n; q++)
+        if (f->spot_layer[q] >= 0) f->lthr[f->spot_layer[q]] = std::min(f->lthr[f->spot_layer[q]], thr[q]);
+    if (f->fp_on && d_wet == f->fp_wet && !footprint_holds(f, thr)) {
+        // a spot reaches past the footprint: recompute the WET on every voxel
+        f->fp_fallbacks++;
+        f->fp_on = false;
+        field_wet(f, f->fp_rlsp, const_cast<float *>(d_wet), nullptr, s);
+        return dh_influence_count(f, d_wet, base, s);
+    }
     DH_HIP(hipMemcpyAsync(f->d_thr, thr.data(), sizeof(float) * n, hipMemcpyHostToDevice, s));
     launch<INF_COUNT>(f, make_blocks(f, r), d_wet, nullptr, nullptr, s);
     std::vector<int> cnt((size_t)n * f->C);
@@ -915,6 +1117,30 @@ extern "C" int dh_influence_create(int gpu_id, const dh_grid *grid, const dh_pro
                                             rel_threshold);
         f->order = t.order;
         *out = f;
+    });
+}
+
+extern "C" int dh_influence_wet(dh_influence *f, const float *d_rlsp, float margin_scale, float *d_wet_out,
+                                void *stream)
+{
+    return dh_guarded([&] {
+        DH_REQUIRE(f && d_rlsp && d_wet_out, "null argument");
+        DH_REQUIRE(margin_scale >= 0.0f, "margin_scale must be >= 0");
+        dh_device_enter(f->dev);
+        footprint_wet(f, d_rlsp, margin_scale, d_wet_out, (hipStream_t)stream);
+    });
+}
+
+extern "C" int dh_influence_footprint(const dh_influence *f, float box[4], int32_t *active, int64_t *fallbacks)
+{
+    return dh_guarded([&] {
+        DH_REQUIRE(f, "null handle");
+        if (box) {
+            box[0] = f->fp_inner.x0; box[1] = f->fp_inner.x1;
+            box[2] = f->fp_inner.y0; box[3] = f->fp_inner.y1;
+        }
+        if (active) *active = f->fp_on ? 1 : 0;
+        if (fallbacks) *fallbacks = f->fp_fallbacks;
     });
 }
 

@@ -149,6 +149,23 @@ class InfluenceMatrix:
         self.nnz = None
         self.row_ptr = None
 
+    def wet(self, rlsp: torch.Tensor, out: torch.Tensor = None, margin_scale: float = 1.0, stream=None):
+        """The field's smoothed WET where the matrix can have entries (1e30 cm
+        elsewhere); count() checks the footprint and falls back to the full
+        grid when a spot reaches past it, so the matrix equals the one of
+        proton_wet's WET."""
+        out = torch.empty_like(rlsp) if out is None else out
+        _lib.check(_lib.load().dh_influence_wet(self._h, _lib.ptr(rlsp), float(margin_scale), _lib.ptr(out),
+                                                _stream(stream)))
+        return out
+
+    def footprint(self):
+        """(box [x0, x1, y0, y1] mm in the head frame, in use, fallbacks so far)."""
+        box = np.zeros(4, np.float32)
+        act, fb = C.c_int32(), C.c_int64()
+        _lib.check(_lib.load().dh_influence_footprint(self._h, _lib.ptr(box), C.byref(act), C.byref(fb)))
+        return box, bool(act.value), int(fb.value)
+
     def count(self, wet: torch.Tensor, base: int = 0, stream=None) -> int:
         nnz = C.c_int64()
         rp = np.zeros(self.n_rows + 1, np.int64)

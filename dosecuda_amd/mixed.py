@@ -1,5 +1,5 @@
 """The mixed batched workload on one GPU: IMPT cases as influence matrices
-(WET per field, counted once, filled in the batch) and IMRT cases as the
+(WET per field, counted once at set-up (timed: count_s), filled in the batch) and IMRT cases as the
 checked surrogate CCCS per MLC segment (or the plain CCCS), all through one
 scheduler whose batch is replayed as a single HIP graph.
 
@@ -47,6 +47,7 @@ class MixedWorkload:
         self.n_spots = 0
         self.n_cps = 0
         self.voxel_cps = 0
+        self.count_s = 0.0                 # wall time of the IMPT fields' peak/count passes
         t0 = time.time()
         for ci, c in enumerate(cases):
             hu = voxelize_mixed(c, "cuda")
@@ -59,7 +60,10 @@ class MixedWorkload:
                 for g, spots in c.fields:
                     wet = engine.proton_wet(rlsp, c.spacing, org, iso, g, 0.0, pmodel.VSADX, pmodel.VSADY)
                     f = engine.InfluenceMatrix(0, c.shape, c.spacing, org, iso, g, 0.0, pmodel, spots)
-                    nnz = f.count(wet)
+                    torch.cuda.synchronize()
+                    tc = time.perf_counter()
+                    nnz = f.count(wet)           # bounds, peak and count passes (host-synchronised)
+                    self.count_s += time.perf_counter() - tc
                     col = torch.empty(max(nnz, 1), dtype=torch.int32, device="cuda")
                     val = torch.empty(max(nnz, 1), dtype=torch.float32, device="cuda")
                     self.sched.add_proton_wet(cid, iso, g, 0.0, pmodel, wet)

@@ -165,6 +165,22 @@ int dh_influence_create(int gpu_id, const dh_grid *grid, const dh_proton_beam *b
                         const dh_proton_lut *lut, const float *spot_rows, int32_t n_spots,
                         int32_t spots_per_block, int32_t row_chunks, float rel_threshold,
                         dh_influence **out);
+/* The field's smoothed WET (as dh_proton_raytrace_dev) computed only where
+ * the matrix can have entries: inside a beam's-eye-view box around the spot
+ * axes, widened per layer by the radius beyond which the layer's table
+ * bounds put every voxel below an estimated threshold (times margin_scale,
+ * 1 by default); voxels outside get 1e30 cm (beyond every range cut).
+ * dh_influence_count with this buffer checks every spot's radius at its
+ * actual threshold against the box and, when one does not fit, recomputes
+ * the WET on every voxel into the same buffer first, so the matrix always
+ * equals the one built from the full-grid WET.  No reference counterpart
+ * (the reference traces every voxel, IMPTClasses.cu:274-299). */
+int dh_influence_wet(dh_influence *f, const float *d_rlsp, float margin_scale, float *d_wet_out,
+                     void *stream);
+/* Footprint box {x0, x1, y0, y1} (head frame, mm) of the last
+ * dh_influence_wet, whether it is still in use (0 after a fallback) and the
+ * number of fallbacks to the full-grid WET so far. */
+int dh_influence_footprint(const dh_influence *f, float box[4], int32_t *active, int64_t *fallbacks);
 int dh_influence_count(dh_influence *f, const float *d_wet, int64_t base, int64_t *nnz,
                        int64_t *row_ptr, void *stream);
 int dh_influence_fill(dh_influence *f, const float *d_wet, uint32_t *d_col, float *d_val,

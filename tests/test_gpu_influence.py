@@ -125,6 +125,56 @@ def test_influence_shared_csr_equals_per_field(proton_model):
     print(f"[parity] shared CSR: {base} entries, both fields equal their own CSR", flush=True)
 
 
+def _csr_equal(t, a, b):
+    return np.array_equal(a[0], b[0]) and t.equal(a[1], b[1]) and t.equal(a[2], b[2])
+
+
+@pytest.mark.parametrize("full_size", [False, True])
+def test_influence_footprint_wet_gives_the_same_matrix(proton_model, full_size):
+    """The WET computed only on the matrix's footprint (dh_influence_wet)
+    equals the full-grid WET bit for bit inside the footprint, and the CSR
+    built from it equals the CSR of the full-grid WET bit for bit.  With the
+    margin forced to zero some spot reaches past the footprint: count()
+    falls back to the full grid and the CSR is still the same."""
+    t = _torch()
+    from dosecuda_amd import engine, phantoms
+    m = proton_model
+    if full_size:
+        case = phantoms.spheres_case(seed=1003)
+        rho, s, iso, org = phantoms.to_rlsp(case), case.spacing, case.iso, case.origin
+        gantry, spots = phantoms.influence_fields(m, seed=1003)[1]
+    else:
+        rho, iso, s = _case(seed=45)
+        org = np.zeros(3)
+        rng = np.random.default_rng(11)
+        n = 60
+        spots = np.column_stack([rng.uniform(-15, 15, n), rng.uniform(-12, 12, n), np.ones(n),
+                                 rng.choice([8, 14, 22], n)]).astype(F)
+        gantry = 75.0
+    print(f"[case] influence footprint full_size={full_size} gantry={gantry} spots={len(spots)}", flush=True)
+    d = t.from_numpy(rho).cuda()
+    full = engine.proton_wet(d, s, org, iso, gantry, 0.0, m.VSADX, m.VSADY)
+    f_ref = engine.InfluenceMatrix(0, rho.shape, s, org, iso, gantry, 0.0, m, spots)
+    rp, col, val = f_ref.build(full)
+    ref = (rp.cpu().numpy(), col.clone(), val.clone())
+    f_ref.close()
+    for scale, expect_fallback in ((1.0, False), (0.0, True)):
+        f = engine.InfluenceMatrix(0, rho.shape, s, org, iso, gantry, 0.0, m, spots)
+        w = f.wet(d, margin_scale=scale)
+        inside = w < 1e29
+        frac = float(inside.float().mean())
+        assert t.equal(w[inside], full[inside])
+        got = f.build(w)
+        box, active, fb = f.footprint()
+        print(f"[parity] footprint scale {scale}: box {np.round(box, 1).tolist()}, {frac:.3f} of voxels, "
+              f"fallbacks {fb}", flush=True)
+        assert (fb > 0) == expect_fallback and active == (not expect_fallback)
+        assert _csr_equal(t, (got[0].cpu().numpy(), got[1], got[2]), ref)
+        if expect_fallback:
+            assert t.equal(w, full)          # the fallback rewrote the whole WET
+        f.close()
+
+
 def test_influence_full_size_properties(proton_model):
     """Batched IMPT influence matrix at full size: (160, 256, 256) @ 2 mm with
     seeded spheres, 3 fields at gantry 0/120/240, ~15k spots over 25 layers."""
