@@ -33,6 +33,8 @@ struct dh_infl_block {
     int layer, first, n, pad_;   // first: index of the block's spot list
     float bx0, bx1, by0, by1;    // BEV (head) lateral box
     int k0, k1, j0, j1;          // image rows swept (inclusive)
+    float ex0, ex1, ey0, ey1;    // extent of the spot axes (the box without the radius)
+    float R, pad1_, pad2_, pad3_;  // largest spot radius of the block (box = extent +- R)
 };
 
 struct dh_infl_args {
@@ -53,6 +55,7 @@ struct dh_infl_args {
     const int64_t *offsets;      // per spot * C (FILL)
     const int2 *active;          // [layer][nz][ny] first/last i with w <= cut (lo > hi: none)
     const float4 *factors;       // [layer][voxel] {ap, log2 P, ah, log2 H} (P, H >= 0 folded)
+    const float4 *row_bounds;    // [layer][nz * ny] maxima of {sigma_t, sigma_ht, P, H} per image row
     uint32_t *col;
     float *val;
     uint32_t nnz32;              // FILL with 32-bit positions: end of the field's slots
@@ -104,6 +107,9 @@ k_influence(dh_infl_args a)
         // spot records of the block (wave-uniform)
         float sx[S], sy[S], tx[S], ty[S], th[S];
         int sid[S];
+        // smallest radius threshold of the block: its entry threshold (COUNT,
+        // FILL) or the lower bound of its peak (PEAK)
+        float tmin = INFINITY;
 #pragma unroll
         for (int s = 0; s < S; s++) {
             const int id = a.block_spots[B.first * S + s];
@@ -111,7 +117,9 @@ k_influence(dh_infl_args a)
             const int c = id >= 0 ? id : 0;
             sx[s] = a.spots[8 * c]; sy[s] = a.spots[8 * c + 1];
             tx[s] = a.spots[8 * c + 2]; ty[s] = a.spots[8 * c + 3];
-            th[s] = (MODE == INF_PEAK || id < 0) ? INFINITY : a.thr[c];
+            const float tq = id < 0 ? INFINITY : a.thr[c];
+            th[s] = MODE == INF_PEAK ? INFINITY : tq;
+            tmin = fminf(tmin, tq);
         }
         float pk[S];
         int cnt[S];
@@ -167,9 +175,17 @@ k_influence(dh_infl_args a)
                     float x, y, z, hx0, hy0, hz0;
                     dh_vox_xyz(g, 0, j, k, x, y, z);
                     dh_img_to_head(g, x, y, z, hx0, hy0, hz0);
+                    // this row's radius: beyond it both Gaussian terms of every
+                    // reached voxel of the row are below tmin / 2 (the row's
+                    // maxima of sigma_t, sigma_ht, P and H bound each voxel's)
+                    const float4 rb = a.row_bounds[(size_t)B.layer * g.ny * g.nz + (size_t)k * g.ny + j];
+                    float R2 = 0.0f;
+                    if (2.0f * rb.z > tmin) R2 = 2.0f * rb.x * rb.x * logf(2.0f * rb.z / tmin);
+                    if (2.0f * rb.w > tmin) R2 = fmaxf(R2, 2.0f * rb.y * rb.y * logf(2.0f * rb.w / tmin));
+                    const float R = fminf(sqrtf(R2) * 1.002f + 0.5f * g.s, B.R);
                     int ilo = ar.x, ihi = ar.y;
-                    interval(hx0, dhx, B.bx0, B.bx1, g.nx, ilo, ihi);
-                    interval(hy0, dhy, B.by0, B.by1, g.nx, ilo, ihi);
+                    interval(hx0, dhx, B.ex0 - R, B.ex1 + R, g.nx, ilo, ihi);
+                    interval(hy0, dhy, B.ey0 - R, B.ey1 + R, g.nx, ilo, ihi);
                     my_lo = ilo; my_hi = ihi;
                 }
             }
@@ -365,21 +381,27 @@ k_active_rows(int nx, int ny, int nz, int n_layers, const dh_layer_k *__restrict
 
 // Per layer, over the voxels the layer reaches (10 * WET <= 1.1 R80): the
 // maxima of sigma_t, sigma_halo_total, P and H (float bits; all >= 0), which
-// bound every spot's lateral dose profile in that layer.
+// bound every spot's lateral dose profile in that layer, per workgroup (for
+// the layer maxima) and per image row (row_bits [layer][nz * ny][4], zeroed
+// by the caller: the per-row radius of the sweeps).  The folded factors are
+// written for reached voxels only: the sweeps read a voxel's factors only
+// where it is reached.  Workgroups with no reached voxel skip the layer.
 __global__ void __launch_bounds__(256)
 k_layer_bounds(dh_geom g, int n_layers, const dh_layer_k *__restrict__ layers,
                const float4 *__restrict__ luts, const float *__restrict__ wet,
-               float *__restrict__ partial, float4 *__restrict__ factors)
+               float *__restrict__ partial, float4 *__restrict__ factors, unsigned *__restrict__ row_bits)
 {
     __shared__ float4 s_lut[DH_LUT_LENGTH];
     __shared__ float s_red[4][4];
     const uint32_t nvox = (uint32_t)g.nx * g.ny * g.nz;
+    const uint32_t nrows = (uint32_t)g.ny * g.nz;
     const uint32_t v = blockIdx.x * 256 + threadIdx.x;
     const bool valid = v < nvox;
     const uint32_t vv = valid ? v : nvox - 1;
     const int i = (int)(vv % (uint32_t)g.nx);
     const uint32_t r = vv / (uint32_t)g.nx;
     const int j = (int)(r % (uint32_t)g.ny), k = (int)(r / (uint32_t)g.ny);
+    const uint32_t r_first = __shfl(r, 0, 64), r_last = __shfl(r, 63, 64);
     float x, y, z;
     dh_vox_xyz(g, i, j, k, x, y, z);
     const float dxs = g.src[0] - x, dys = g.src[1] - y, dzs = g.src[2] - z;
@@ -387,20 +409,37 @@ k_layer_bounds(dh_geom g, int n_layers, const dh_layer_k *__restrict__ layers,
     const float w = wet[vv] * 10.0f;
     for (int l = 0; l < n_layers; l++) {
         const dh_layer_k L = layers[l];
-        __syncthreads();
+        const bool act = valid && !(w > L.cut);
+        // also the barrier before s_lut and s_red are rewritten
+        if (!__syncthreads_or(act)) {
+            if (threadIdx.x < 4) partial[((size_t)l * gridDim.x + blockIdx.x) * 4 + threadIdx.x] = 0.0f;
+            continue;
+        }
         for (int q = threadIdx.x; q < DH_LUT_LENGTH; q += 256) s_lut[q] = luts[(size_t)l * DH_LUT_LENGTH + q];
         __syncthreads();
-        const bool act = valid && !(w > L.cut);
         float m[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-        // inactive voxels get a zero dose factor: exp2(-inf) = 0
-        float4 fo = make_float4(0.0f, -INFINITY, 0.0f, -INFINITY);
         if (act) {
             const dh_pb_factors f = dh_pencil_factors(s_lut, w, dist, L.r80, L.A, L.B, L.C);
             m[0] = fabsf(f.st); m[1] = fabsf(f.sht);
             m[2] = fmaxf(f.P, 0.0f); m[3] = fmaxf(f.H, 0.0f);
-            fo = make_float4(f.ap, log2f(fmaxf(f.P, 0.0f)), f.ah, log2f(fmaxf(f.H, 0.0f)));
+            factors[(size_t)l * nvox + v] =
+                make_float4(f.ap, log2f(fmaxf(f.P, 0.0f)), f.ah, log2f(fmaxf(f.H, 0.0f)));
         }
-        if (valid) factors[(size_t)l * nvox + v] = fo;
+        // per-row maxima: one atomic per quantity and wave when the wave lies
+        // in one row, else per reached lane
+        unsigned *rb = row_bits + (size_t)l * nrows * 4;
+        if (r_first == r_last) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                float x2 = m[q];
+                for (int off = 32; off > 0; off >>= 1) x2 = fmaxf(x2, __shfl_xor(x2, off, 64));
+                if (__lane_id() == 0 && x2 > 0.0f) atomicMax(&rb[(size_t)r_first * 4 + q], __float_as_uint(x2));
+            }
+        } else if (act) {
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                if (m[q] > 0.0f) atomicMax(&rb[(size_t)r * 4 + q], __float_as_uint(m[q]));
+        }
         // workgroup maxima -> one partial record per workgroup and layer
 #pragma unroll
         for (int q = 0; q < 4; q++) {
@@ -487,6 +526,7 @@ struct dh_influence {
     int *d_spot_layer = nullptr;
     unsigned long long *d_stats = nullptr;   // 4 counters per pass
     float4 *d_factors = nullptr;             // [layer][voxel]
+    unsigned *d_row_bits = nullptr;          // [layer][row][4] per-row maxima (float bits)
     std::vector<float> peak;         // host copy after the peak pass
     std::vector<float> radius;       // truncation radius per spot (COUNT = FILL)
     std::vector<float> bnd;          // per layer: sigma_t, sigma_ht, P, H maxima
@@ -694,6 +734,7 @@ dh_influence *dh_influence_make(int dev, const dh_geom &g, float vsadx, float vs
         DH_HIP(hipMalloc(&f->d_factors, sizeof(float4) * std::max<size_t>(layers.size(), 1) *
                                             (size_t)g.nx * g.ny * g.nz));
         DH_HIP(hipMalloc(&f->d_active, sizeof(int2) * std::max<size_t>(layers.size(), 1) * g.ny * g.nz));
+        DH_HIP(hipMalloc(&f->d_row_bits, sizeof(unsigned) * 4 * std::max<size_t>(layers.size(), 1) * g.ny * g.nz));
         DH_HIP(hipMalloc(&f->d_spot_layer, sizeof(int) * std::max(n_spots, 1)));
         if (n_spots)
             DH_HIP(hipMemcpy(f->d_spot_layer, f->spot_layer.data(), sizeof(int) * n_spots, hipMemcpyHostToDevice));
@@ -711,7 +752,7 @@ void dh_influence_free(dh_influence *f)
     (void)hipSetDevice(f->dev);
     void *ps[] = {f->d_layers, f->d_luts, f->d_spots, f->d_block_spots, f->d_blocks, f->d_thr,
                   f->d_peak, f->d_counts, f->d_spans, f->d_offsets, f->d_queue, f->d_active, f->d_stats,
-                  f->d_factors, f->d_spot_layer};
+                  f->d_factors, f->d_spot_layer, f->d_row_bits};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     delete f;
@@ -732,6 +773,7 @@ static std::vector<dh_infl_block> make_blocks(const dh_influence *f, const std::
         dh_infl_block &B = out[b];
         B.layer = f->block_layer[b]; B.first = (int)b; B.n = f->block_n[b]; B.pad_ = 0;
         float x0 = 1e30f, x1 = -1e30f, y0 = 1e30f, y1 = -1e30f;
+        float e0 = 1e30f, e1 = -1e30f, f0 = 1e30f, f1 = -1e30f, Rb = 0.0f;
         for (int s = 0; s < f->S; s++) {
             const int id = f->block_spots[b * f->S + s];
             if (id < 0) continue;
@@ -741,8 +783,15 @@ static std::vector<dh_infl_block> make_blocks(const dh_influence *f, const std::
             x1 = std::max(x1, std::max(sx * cmin_x, sx * cmax_x) + R);
             y0 = std::min(y0, std::min(sy * cmin_y, sy * cmax_y) - R);
             y1 = std::max(y1, std::max(sy * cmin_y, sy * cmax_y) + R);
+            e0 = std::min(e0, std::min(sx * cmin_x, sx * cmax_x));
+            e1 = std::max(e1, std::max(sx * cmin_x, sx * cmax_x));
+            f0 = std::min(f0, std::min(sy * cmin_y, sy * cmax_y));
+            f1 = std::max(f1, std::max(sy * cmin_y, sy * cmax_y));
+            Rb = std::max(Rb, R);
         }
         B.bx0 = x0; B.bx1 = x1; B.by0 = y0; B.by1 = y1;
+        B.ex0 = e0; B.ex1 = e1; B.ey0 = f0; B.ey1 = f1; B.R = Rb;
+        B.pad1_ = B.pad2_ = B.pad3_ = 0.0f;
         // image-space bounding rows of the prism box x [hz_min, hz_max]
         float kmin = 1e30f, kmax = -1e30f, jmin = 1e30f, jmax = -1e30f;
         for (int c = 0; c < 8; c++) {
@@ -781,6 +830,7 @@ static void launch_kernels(dh_influence *f, int n_blocks, const float *d_wet, ui
     a.peak_bits = f->d_peak; a.counts = f->d_counts; a.spans = f->d_spans; a.offsets = f->d_offsets;
     a.active = f->d_active;
     a.factors = f->d_factors;
+    a.row_bounds = (const float4 *)f->d_row_bits;
     a.stats = f->d_stats + 4 * MODE;
     hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, (unsigned *)a.stats, (size_t)8);
     a.col = d_col; a.val = d_val; a.queue = f->d_queue;
@@ -859,13 +909,18 @@ static void field_wet(dh_influence *f, const float *d_rlsp, float *d_wet, const 
     float *raw = (float *)dh_scratch(f->dev, DH_SLOT_WET_RAW, sizeof(float) * (size_t)g.nx * g.ny * g.nz);
     dh_launch_pad_quad(d_rlsp, vpad, g.nx, g.ny, g.nz, 0.0f, s);      // IMPTClasses.cu:292, border 0
     if (box) {
+        // bricks of 16 x 4 x 4 voxels: a wave covers a 16 x 4 patch, so few
+        // waves straddle the box edge (the best measured tile for the
+        // (160, 256, 256) fields; "raytrace_box" / "smooth_box" override it)
+        const std::string key = dh_tune_key_geom(g);
         const float reach = 10.0f + g.s;
         const dh_hbox outer{box->x0 - reach, box->x1 + reach, box->y0 - reach, box->y1 + reach, 1};
-        dh_launch_raytrace(g, vpad, raw, s, -1, &outer);
+        dh_launch_raytrace(g, vpad, raw, s, dh_tune_lookup("raytrace_box", key, 5), &outer);
+        dh_launch_smooth(g, raw, d_wet, s, dh_tune_lookup("smooth_box", key, 5), box);
     } else {
         dh_launch_raytrace(g, vpad, raw, s);
+        dh_launch_smooth(g, raw, d_wet, s);
     }
-    dh_launch_smooth(g, raw, d_wet, s, -1, box);
 }
 
 // Footprint of the field: the spot axes widened per layer by the table
@@ -959,8 +1014,11 @@ static void wet_bounds(dh_influence *f, const float *d_wet, hipStream_t s)
     float *d_b = (float *)dh_scratch(f->dev, DH_SLOT_TMP1, sizeof(float) * 4 * nl);
     {
         dh_timed t("influence_bounds", s);
+        const size_t nrb = 4 * nl * (size_t)f->g.ny * f->g.nz;
+        hipLaunchKernelGGL(k_zero_words, dim3((unsigned)std::min<size_t>((nrb + 255) / 256, 4096)), dim3(256), 0, s,
+                           f->d_row_bits, nrb);
         hipLaunchKernelGGL(k_layer_bounds, dim3(nblk), dim3(256), 0, s, f->g, (int)nl, f->d_layers,
-                           f->d_luts, d_wet, d_part, f->d_factors);
+                           f->d_luts, d_wet, d_part, f->d_factors, f->d_row_bits);
         DH_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_reduce_bounds, dim3((unsigned)(4 * nl)), dim3(256), 0, s, d_part, (int)nblk, d_b);
         DH_HIP(hipGetLastError());
@@ -1010,6 +1068,13 @@ int64_t dh_influence_count(dh_influence *f, const float *d_wet, int64_t base, hi
         const float rp = Pm > 0.5f * pa ? st * std::sqrt(2.0f * std::log(2.0f * Pm / pa)) : 0.0f;
         const float rh = Hm > 0.5f * pa ? sht * std::sqrt(2.0f * std::log(2.0f * Hm / pa)) : 0.0f;
         r[q] = 1.001f * std::max(rp, rh) + f->g.s;
+    }
+    // the PEAK sweep's per-row radius threshold: each spot's peak lower bound
+    {
+        std::vector<float> pa(std::max(n, 1), 0.0f);
+        for (int q = 0; q < n; q++) std::memcpy(&pa[q], &pk[q], 4);
+        DH_HIP(hipMemcpyAsync(f->d_thr, pa.data(), sizeof(float) * n, hipMemcpyHostToDevice, s));
+        DH_HIP(hipStreamSynchronize(s));
     }
     launch<INF_PEAK>(f, make_blocks(f, r), d_wet, nullptr, nullptr, s);
     DH_HIP(hipMemcpyAsync(pk.data(), f->d_peak, sizeof(unsigned) * n, hipMemcpyDeviceToHost, s));
