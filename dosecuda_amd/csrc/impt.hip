@@ -44,29 +44,31 @@ k_pad_quad(const float *__restrict__ src, float4 *__restrict__ dst, int nx, int 
 // leaves the volume; WET = -0.05 + 0.1 * sum(max(rho, 0)) cm.  The sample
 // lattice and exit test are the reference's FP32 arithmetic; the running sum
 // is FP64 so long heterogeneous rays do not lose low-order bits.
-// Number of leading samples L = 0, 1, ... that are provably inside the volume:
-// the exact ray/box exit distance in FP64 from the same FP32 inputs, minus a
-// margin of 1e-3 voxel per axis (the FP32 texel coordinate is within a few ulp
-// of the exact one, far below that margin).
+// Number of leading samples L = 0, 1, ... that are provably inside the volume,
+// from the ray/box exit distance evaluated in FP32, minus a margin per axis
+// of (2e-3 + 8e-6 n) voxel / |du|.  That margin bounds every rounding here
+// and in the sample's own texel coordinate: the texel coordinates are below
+// n + 1 (FP32 error < 2 ulp of n, i.e. < 2.4e-7 n voxel), and the quotient
+// (lim - t0) / du carries a relative error below 4 ulp, i.e. below
+// 5e-7 * La < 5e-7 * n / |du| voxel-steps -- both far below the margin.
+// A short bulk only moves samples to the exactly tested tail.
 __device__ static inline int safe_samples(const dh_geom &g, float x, float y, float z,
                                           float ux, float uy, float uz)
 {
-    const double inv = g.inv_s;
     const float p[3] = {x, y, z}, u[3] = {ux, uy, uz};
     const int n[3] = {g.nx, g.ny, g.nz};
-    double Lmin = 1e30;
+    float Lmin = 1e30f;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        const double t0 = inv * ((double)p[a] + g.iso[a]) + 0.5;
-        const double du = inv * (double)u[a];
-        if (du == 0.0) continue;
-        const double lim = du > 0.0 ? (double)n[a] : 0.0;
-        // margin: 1e-3 voxel plus the FP32 rounding of a coordinate of size n
-        const double La = (lim - t0) / du - (1e-3 + 2e-6 * n[a]) / fabs(du);
-        Lmin = fmin(Lmin, La);
+        const float t0 = fmaf(g.inv_s, p[a] + g.iso[a], 0.5f);
+        const float du = g.inv_s * u[a];
+        if (du == 0.0f) continue;
+        const float lim = du > 0.0f ? (float)n[a] : 0.0f;
+        const float La = (lim - t0) / du - (2e-3f + 8e-6f * (float)n[a]) / fabsf(du);
+        Lmin = fminf(Lmin, La);
     }
-    if (!(Lmin >= 0.0)) return 0;
-    return (int)fmin(floor(Lmin) + 1.0, 1.0e7);
+    if (!(Lmin >= 0.0f)) return 0;
+    return (int)fminf(floorf(Lmin) + 1.0f, 1.0e7f);
 }
 
 // True when the voxel at image point (x, y, z) lies outside the head-frame box.
@@ -166,7 +168,9 @@ k_smooth(dh_geom g, smooth_dirs dirs, dh_tile tile, dh_hbox box, const float *__
         return;
     }
     const float c = wet_in[v];
-    double sum = (double)c;
+    // Neumaier-compensated FP32 sum of at most 55 values: the rounded sum is
+    // within one ulp of the exact one before the single division
+    float sum = c, comp = 0.0f;
     int n = 1;
     const float lim = c * 10.0f;
     // The neighbour's continuous image index is linear in the radius: from
@@ -197,12 +201,12 @@ k_smooth(dh_geom g, smooth_dirs dirs, dh_tile tile, dh_hbox box, const float *__
                 kk = (unsigned)(int)roundf((qz + g.iso[2]) / g.s);
             }
             if (ii < (unsigned)g.nx && jj < (unsigned)g.ny && kk < (unsigned)g.nz) {
-                sum += (double)wet_in[ii + (unsigned)g.nx * (jj + (unsigned)g.ny * kk)];
+                dh_neumaier(sum, comp, wet_in[ii + (unsigned)g.nx * (jj + (unsigned)g.ny * kk)]);
                 n++;
             }
         }
     }
-    wet_out[v] = (float)(sum / (double)n);
+    wet_out[v] = (sum + comp) / (float)n;
 }
 
 // Pencil-beam sum (IMPTClasses.cu:204-271).  One lane owns one voxel and walks
