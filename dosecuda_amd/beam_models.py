@@ -143,35 +143,56 @@ class PhotonBeamModel:
                 setattr(self, key, float(parts[1]))
 
     def outputFactor(self, cp):
-        """Equivalent-square output factor (plan_imrt.py:61-92)."""
-        min_y, max_y, max_x_diff, area = 10000.0, -10000.0, 0.0, 0.0
-        for i in range(cp.mlc.shape[0]):
-            x1, x2, yo, yw = (float(v) for v in cp.mlc[i, :4])
-            area += (x2 - x1) * yw
-            if (x2 - x1) > 3.0 and (x2 - x1) > max_x_diff:
-                max_x_diff = x2 - x1
-            if (x2 - x1) > 3.0:
-                min_y = min(min_y, yo - yw / 2.0)
-                max_y = max(max_y, yo + yw / 2.0)
-        perimeter = 2 * (max_y - min_y) + 2 * max_x_diff
-        eq = 4 * area / perimeter
-        return np.interp(eq, self.output_factor_equivalent_squares, self.output_factor_values)
+        """Equivalent-square output factor of a control point's aperture
+        (plan_imrt.py:61-92): 4 * area / perimeter, where the area sums every
+        pair's opening times its width and the perimeter spans the y extent
+        and the widest opening of the pairs open by more than 3 mm, looked up
+        in the model's output-factor table.
+
+        The arithmetic is the reference's under numpy < 2 (its pinned
+        environment), written out: a pair's opening and area are float32
+        (float32 scalar operations), the running area, the pair edges
+        (offset -/+ width / 2.0) and the perimeter are float64, and the areas
+        are added in pair order."""
+        rows = np.asarray(cp.mlc)
+        x1, x2 = rows[:, 0].astype(np.single), rows[:, 1].astype(np.single)
+        off, wid = rows[:, 2].astype(np.single), rows[:, 3].astype(np.single)
+        opening = x2 - x1                                   # float32
+        area = 0.0
+        for a in (opening * wid).tolist():                  # float32 products, float64 sum
+            area += a
+        big = opening > 3.0
+        lo, hi, widest = 10000.0, -10000.0, 0.0
+        if big.any():
+            half = wid[big].astype(np.float64) / 2.0
+            lo = min(lo, float((off[big] - half).min()))
+            hi = max(hi, float((off[big] + half).max()))
+            widest = float(opening[big].max())
+        perimeter = 2.0 * (hi - lo) + 2.0 * widest
+        return np.interp(4.0 * area / perimeter, self.output_factor_equivalent_squares,
+                         self.output_factor_values)
+
+
+def aperture_rows(model, x_lo, x_hi, open_pairs):
+    """[n_pairs, 4] float32 rows (x1, x2, y offset, width), column-major as
+    the reference's transposed stack: pairs in open_pairs span [x_lo, x_hi]
+    (scalars or per-pair arrays, in mm), the others are closed at 0."""
+    n = model.n_mlc_pairs
+    rows = np.zeros((n, 4), dtype=np.single, order="F")
+    rows[:, 0] = np.where(open_pairs, x_lo, 0.0)
+    rows[:, 1] = np.where(open_pairs, x_hi, 0.0)
+    rows[:, 2] = model.mlc_offsets
+    rows[:, 3] = model.mlc_widths
+    return rows
 
 
 def square_field_mlc(model: PhotonBeamModel, dimx_cm=10.0, dimy_cm=10.0):
-    """MLC of addSquareField (plan_imrt.py:360-374): [n_pairs, 4] float32
-    rows x1, x2, y_offset, y_width, F-ordered like np.transpose's result."""
-    dimx, dimy = dimx_cm * 10.0, dimy_cm * 10.0
-    mlc = np.zeros((2, model.n_mlc_pairs), dtype=np.single)
-    mlc[0, :] = -dimx / 2.0
-    mlc[1, :] = dimx / 2.0
-    for i in range(model.n_mlc_pairs):
-        if model.mlc_offsets[i] < -(dimy / 2.0) or model.mlc_offsets[i] > (dimy / 2.0):
-            mlc[0, i] = 0.0
-            mlc[1, i] = 0.0
-    mlc = np.array(np.vstack((mlc, model.mlc_offsets.reshape(1, -1),
-                              model.mlc_widths.reshape(1, -1))), dtype=np.single)
-    return np.transpose(mlc)
+    """Aperture of addSquareField (plan_imrt.py:360-374): a dimx x dimy cm
+    field centred on the axis; a pair opens from -X/2 to +X/2 (X in mm)
+    when its centre offset lies within +-Y/2, otherwise it stays closed."""
+    half_x, half_y = dimx_cm * 10.0 / 2.0, dimy_cm * 10.0 / 2.0
+    off = np.asarray(model.mlc_offsets)
+    return aperture_rows(model, -half_x, half_x, (off >= -half_y) & (off <= half_y))
 
 
 class VolumeObject:

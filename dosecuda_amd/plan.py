@@ -47,7 +47,7 @@ from . import dicom as _dcm
 from . import dose_kernels
 from . import nifti as _nifti
 from . import nrrd as _nrrd
-from .beam_models import VolumeObject
+from .beam_models import VolumeObject, aperture_rows
 
 __all__ = ["VolumeObject", "DoseGrid", "Beam", "Plan", "IMPTBeamModel", "IMPTDoseGrid",
            "IMPTBeam", "IMPTPlan", "IMRTPhotonEnergy", "IMRTControlPoint", "IMRTDoseGrid",
@@ -174,9 +174,6 @@ def _checks_volume(hu, origin, spacing):
     v.origin = np.array(origin, dtype=np.single)
     v.spacing = np.array(spacing, dtype=np.single)
     return v
-
-    def zeros(self):
-        return self.up().torch.zeros_like(self.vol)
 
 
 def _model_index(labels, label, what):
@@ -901,66 +898,16 @@ class IMRTPlan(Plan):
         """Treatment beams of an RT Plan (plan_imrt.py:273-351).  Each control
         point after the first becomes a control point with the aperture and
         angles in force before it and MU = (its cumulative meterset weight -
-        the previous one) x the first referenced beam's meterset; jaws and MLC
-        come from the beam-limiting device items as the reference indexes them
-        (X and Y jaws from item 1, MLCX from item 2)."""
+        the previous one) x the first referenced beam's meterset."""
         ds = _dcm.read(plan_path)
-        total_mu = ds.FractionGroupSequence[0].ReferencedBeamSequence[0].BeamMeterset
-        self.n_fractions = float(ds.FractionGroupSequence[0].NumberOfFractionsPlanned)
+        group = ds.FractionGroupSequence[0]
+        meterset = group.ReferencedBeamSequence[0].BeamMeterset
+        self.n_fractions = float(group.NumberOfFractionsPlanned)
         self.beam_list = []
         self.n_beams = 0
-        bm = self.beam_models[0]
-
-        def limits(cp):
-            xjaws, yjaws, mlc = None, None, None
-            seq = cp.BeamLimitingDevicePositionSequence
-            for ps in seq:
-                if ps.RTBeamLimitingDeviceType == "X":
-                    xjaws = np.array(seq[1].LeafJawPositions, dtype=np.single)
-                if ps.RTBeamLimitingDeviceType == "Y":
-                    yjaws = np.array(seq[1].LeafJawPositions, dtype=np.single)
-                if ps.RTBeamLimitingDeviceType == "MLCX":
-                    mlc = np.array(seq[2].LeafJawPositions, dtype=np.single)
-            return xjaws, yjaws, mlc
-
-        for beam in ds.BeamSequence:
-            if beam.TreatmentDeliveryType != "TREATMENT":
-                continue
-            imrt_beam = IMRTBeam()
-            for cp in beam.ControlPointSequence:
-                if cp.ControlPointIndex == 0:
-                    imrt_beam.dicom_energy_label = cp.NominalBeamEnergy
-                    imrt_beam.iso = np.array([cp.IsocenterPosition[0], cp.IsocenterPosition[1],
-                                              cp.IsocenterPosition[2]], dtype=np.single)
-                    cumulative_mu = 0.0
-                    ca = cp.BeamLimitingDeviceAngle
-                    ta = cp.PatientSupportAngle
-                    ga = cp.GantryAngle
-                    xjaws, yjaws, mlc = limits(cp)
-                    continue
-                mu = cp.CumulativeMetersetWeight - cumulative_mu
-                if mlc is not None:
-                    mlc = np.reshape(mlc, (2, bm.n_mlc_pairs))
-                else:
-                    mlc = np.zeros((2, bm.n_mlc_pairs), dtype=np.single)
-                    if xjaws is not None:
-                        mlc[0, :] = xjaws[0]
-                        mlc[1, :] = xjaws[1]
-                    if yjaws is not None:
-                        for i in range(bm.n_mlc_pairs):
-                            if bm.mlc_offsets[i] < yjaws[0] or bm.mlc_offsets[i] > yjaws[1]:
-                                mlc[0, i] = 0.0
-                                mlc[1, i] = 0.0
-                mlc = np.transpose(np.array(np.vstack((mlc, bm.mlc_offsets.reshape(1, -1),
-                                                       bm.mlc_widths.reshape(1, -1))), dtype=np.single))
-                imrt_beam.addControlPoint(IMRTControlPoint(imrt_beam.iso, mu * total_mu, mlc, ga, ca, ta,
-                                                           xjaws, yjaws))
-                if "GantryAngle" in cp:
-                    ga = cp.GantryAngle
-                if "BeamLimitingDevicePositionSequence" in cp:
-                    xjaws, yjaws, mlc = limits(cp)
-                cumulative_mu = cp.CumulativeMetersetWeight
-            self.addBeam(imrt_beam)
+        for item in ds.BeamSequence:
+            if item.TreatmentDeliveryType == "TREATMENT":
+                self.addBeam(_beam_from_control_points(item.ControlPointSequence, self.beam_models[0], meterset))
 
     def addSquareField(self, dicom_energy_label='6', dimx=10, dimy=10, mu=100, gantry_angle=0.0,
                        collimator_angle=0.0, table_angle=0.0):
@@ -975,6 +922,82 @@ class IMRTPlan(Plan):
         cp = IMRTControlPoint(iso, mu, mlc, gantry_angle, collimator_angle, table_angle, jawx, jawy)
         imrt_beam.addControlPoint(cp)
         self.addBeam(imrt_beam)
+
+
+def _limiting_devices(cp):
+    """{device type: positions} of a control point's Beam Limiting Device
+    Position Sequence, read where the reference reads them
+    (plan_imrt.py:298-306): an X or a Y device takes the positions of the
+    sequence's item 1 and an MLCX device those of item 2, whatever those
+    items' own types."""
+    seq = cp.BeamLimitingDevicePositionSequence
+    item_of = {"X": 1, "Y": 1, "MLCX": 2}
+    kinds = [ps.RTBeamLimitingDeviceType for ps in seq]
+    return {k: np.array(seq[item_of[k]].LeafJawPositions, dtype=np.single) for k in item_of if k in kinds}
+
+
+class _Segment:
+    """Machine state between two control points: angles, beam limiting
+    devices and the cumulative meterset weight reached."""
+
+    def __init__(self, cp):
+        self.ga = cp.GantryAngle
+        self.ca = cp.BeamLimitingDeviceAngle
+        self.ta = cp.PatientSupportAngle
+        self.weight = 0.0
+        self._devices(cp)
+
+    def _devices(self, cp):
+        self.dev = _limiting_devices(cp)
+        self.used = False
+
+    def aperture(self, model):
+        """Rows (x1, x2, y offset, width) of the leaves in force, or of the
+        jaws when there is no MLC: the pairs span the X jaws (closed without
+        them) and close outside the Y jaws."""
+        n = model.n_mlc_pairs
+        if self.used:
+            # the reference keeps the rows it built in place of the leaf
+            # positions, so a second segment without new device positions
+            # fails reshaping them, as it does there
+            np.reshape(np.zeros(4 * n, np.single), (2, n))
+        self.used = True
+        if "MLCX" in self.dev:
+            banks = np.reshape(self.dev["MLCX"], (2, n))
+            return aperture_rows(model, banks[0], banks[1], np.ones(n, bool))
+        x, y = self.dev.get("X"), self.dev.get("Y")
+        off = np.asarray(model.mlc_offsets)
+        inside = np.ones(n, bool) if y is None else ~((off < y[0]) | (off > y[1]))
+        if x is None:
+            return aperture_rows(model, 0.0, 0.0, np.zeros(n, bool))
+        return aperture_rows(model, x[0], x[1], inside)
+
+    def advance(self, cp):
+        if "GantryAngle" in cp:
+            self.ga = cp.GantryAngle
+        if "BeamLimitingDevicePositionSequence" in cp:
+            self._devices(cp)
+        self.weight = cp.CumulativeMetersetWeight
+
+
+def _beam_from_control_points(sequence, model, meterset):
+    """An IMRTBeam from an RT Plan control point sequence: control point 0
+    sets the energy, isocentre, angles and devices; every later control point
+    closes a segment (MU = weight step x meterset) with the state in force
+    before it, then applies its own gantry angle and devices."""
+    beam = IMRTBeam()
+    seg = None
+    for cp in sequence:
+        if cp.ControlPointIndex == 0:
+            beam.dicom_energy_label = cp.NominalBeamEnergy
+            beam.iso = np.array([cp.IsocenterPosition[a] for a in range(3)], dtype=np.single)
+            seg = _Segment(cp)
+            continue
+        mu = (cp.CumulativeMetersetWeight - seg.weight) * meterset
+        beam.addControlPoint(IMRTControlPoint(beam.iso, mu, seg.aperture(model), seg.ga, seg.ca, seg.ta,
+                                              seg.dev.get("X"), seg.dev.get("Y")))
+        seg.advance(cp)
+    return beam
 
 
 class IMRTDoseGrid(DoseGrid):
