@@ -721,14 +721,30 @@ extern "C" int dh_sur_dose_batch_dev(const dh_sur_net *net, const dh_grid *grid,
         DH_REQUIRE(dilate >= 0 && dilate <= 2, "dilate must be 0, 1 or 2");
         for (int i = 0; i < n_cp; i++)
             DH_REQUIRE(d_terma[i] && d_electron[i] && d_dose_out[i], "null control point volume");
-        {
-            std::vector<const float *> o(d_dose_out, d_dose_out + n_cp);
-            std::sort(o.begin(), o.end());
-            DH_REQUIRE(std::adjacent_find(o.begin(), o.end()) == o.end(),
-                       "control points need distinct output volumes");
-        }
         dh_device_enter(dh_sur_net_gpu(net));
-        nvox_of(grid);
+        const size_t bytes = sizeof(float) * nvox_of(grid);
+        {
+            // outputs are written while other points' inputs are still read
+            // (and from two streams): no output may overlap another output
+            // or any input volume
+            std::vector<std::pair<uintptr_t, int>> r;   // (start, 1 = output)
+            for (int i = 0; i < n_cp; i++) {
+                r.push_back({(uintptr_t)d_dose_out[i], 1});
+                r.push_back({(uintptr_t)d_terma[i], 0});
+                r.push_back({(uintptr_t)d_electron[i], 0});
+            }
+            r.push_back({(uintptr_t)d_density, 0});
+            std::sort(r.begin(), r.end());
+            uintptr_t end_out = 0, end_any = 0;
+            for (const auto &e : r) {
+                // an output starting inside any earlier range, or any range
+                // starting inside an earlier output, overlaps
+                DH_REQUIRE(!(e.second == 1 && e.first < end_any) && !(e.first < end_out),
+                           "output volumes must not overlap each other or the input volumes");
+                end_any = std::max(end_any, e.first + bytes);
+                if (e.second == 1) end_out = std::max(end_out, e.first + bytes);
+            }
+        }
         std::vector<dh_ccc_args> cas;
         std::vector<float> dirs(3 * (size_t)n_cp);
         cas.reserve(n_cp);

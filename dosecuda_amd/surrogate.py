@@ -631,6 +631,12 @@ class SurrogateCCCS:
         for v in (*termas, *electrons, *outs):
             if v.shape != density.shape or v.dtype != torch.float32 or not v.is_contiguous():
                 raise ValueError("volumes must be contiguous float32 of the density's shape")
+        nbytes = density.numel() * 4
+        ins = [(v.data_ptr(), v.data_ptr() + nbytes) for v in (*termas, *electrons, density)]
+        outr = [(v.data_ptr(), v.data_ptr() + nbytes) for v in outs]
+        for k, (a, b) in enumerate(outr):
+            if any(a < d and c < b for c, d in ins) or any(a < d and c < b for c, d in outr[k + 1:]):
+                raise ValueError("output volumes must not overlap each other or the input volumes")
         arr = (_lib.PhotonCP * n)(*[_lib.photon_cp(*cp) for cp in cps])
         P = lambda vs: (C.c_void_p * n)(*[v.data_ptr() for v in vs])
         keep: list = []

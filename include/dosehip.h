@@ -314,7 +314,8 @@ int dh_sur_dose_dev(const dh_sur_net *net, const dh_grid *grid, const dh_photon_
 /* dh_sur_dose_dev for n_cp control points of one grid and density (per-point
  * TERMA, electron and output volumes, arrays of device pointers): the points
  * share network launches, up to 8 per batch, and each dose equals the
- * single-point call's bit for bit.  No reference counterpart (the reference
+ * single-point call's bit for bit.  Output volumes must not overlap one
+ * another or any input volume (DH_E_INVALID).  No reference counterpart (the reference
  * runs one control point per photon_dose_cuda call, plan_imrt.py:150-153). */
 int dh_sur_dose_batch_dev(const dh_sur_net *net, const dh_grid *grid, int32_t n_cp,
                           const dh_photon_cp *cps, const dh_photon_model *model,
@@ -323,7 +324,10 @@ int dh_sur_dose_batch_dev(const dh_sur_net *net, const dh_grid *grid, int32_t n_
                           float *const *d_dose_out, void *stream);
 /* Lattice and fallback voxel counts of the last checked surrogate dose run
  * by dh_sur_dose_dev / dh_photon_dose_surrogate_dev from this thread on
- * gpu_id (synchronises the device). */
+ * gpu_id (synchronises the device).  After dh_sur_dose_batch_dev they are
+ * those of the batch's last point checked on the caller's stream (the last
+ * even-numbered point; odd-numbered points are checked on a side stream
+ * with buffers of their own). */
 int dh_sur_counts(int gpu_id, int32_t *lattice, int32_t *fallback);
 /* dh_photon_dose_dev with the cone superposition replaced by the checked
  * surrogate (ray trace and TERMA as in the plain path). */

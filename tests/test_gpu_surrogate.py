@@ -502,6 +502,20 @@ def test_dose_batch_equals_single(photon_model, shape, n_cp):
         sur.dose_batch(termas[:1], rho, els, s, np.zeros(3), cps)
     with pytest.raises(ValueError):
         sur.dose_batch(termas[:2], rho, els[:2], s, np.zeros(3), cps[:2], outs=[got[0], got[0]])
+    # an output over an input volume is refused, by the wrapper and by the C ABI
+    with pytest.raises(ValueError, match="overlap"):
+        sur.dose_batch(termas[:2], rho, els[:2], s, np.zeros(3), cps[:2], outs=[got[0], termas[1]])
+    import ctypes as C
+    from dosecuda_amd import _lib
+    keep: list = []
+    m = engine.photon_args(pm, keep)
+    arr = (_lib.PhotonCP * 2)(*[_lib.photon_cp(*cp) for cp in cps[:2]])
+    P = lambda vs: (C.c_void_p * 2)(*[v.data_ptr() for v in vs])
+    for outs_bad in ([got[0], els[0]], [got[0], rho], [got[1], got[1]]):
+        rc = _lib.load().dh_sur_dose_batch_dev(sur.native.handle, _lib.grid(rho.shape, s, np.zeros(3)), 2, arr, m,
+                                               P(termas[:2]), _lib.ptr(rho), P(els[:2]), 3, 0.01, 1, P(outs_bad),
+                                               C.c_void_p(t.cuda.current_stream().cuda_stream))
+        assert rc == _lib.DH_E_INVALID and "overlap" in _lib.last_error()
     # the batch (with its side stream) captured as one HIP graph; replays
     # interleaved with eager work must reproduce the eager doses
     outs = [t.empty_like(rho) for _ in cps]
