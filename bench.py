@@ -211,6 +211,98 @@ def cpu_baseline_cccs(st, budget_s):
                     f"z-planes ({n_s} of {n_act} active voxels), scaled by active voxels"
 
 
+def _slab_mask(shape, planes):
+    nz = shape[0]
+    m = np.zeros(shape, np.uint8)
+    k0 = max(0, nz // 2 - planes // 2)
+    m[k0:k0 + planes] = 1
+    return m
+
+
+def cpu_baseline_cccs_1t(st, budget_s):
+    """The CCCS baseline on one host thread: ray trace, TERMA and the cone
+    superposition timed on a slab of z-planes through the field (TERMA of
+    the other voxels, which the cones sample, from the multi-thread run) and
+    scaled by voxel and active-voxel counts to ms/beam."""
+    from oracle import oracle as O
+    pm, c, dens = st["pm"], st["case"], st["dens"]
+    ga = st["angles"][0]
+    g = O.geom(dens.shape, c.spacing, c.origin, c.iso, ga, 0.0, 0.0, np.float32(pm.primary_source_distance))
+    raw, _ = O.raytrace(g, dens, float(np.float32(0.0012)))
+    tr, el = O.terma(g, pm, st["mlc"], raw.astype(np.float32))
+    t32, e32 = tr.astype(np.float32), el.astype(np.float32)
+    active = ~(t32 <= 0.01)
+    n, n_act = dens.size, int(active.sum())
+    nthr = O.get_threads()
+    O.set_threads(1)
+    try:
+        planes = 1
+        while True:
+            mask = _slab_mask(dens.shape, planes)
+            t0 = time.perf_counter()
+            O.raytrace(g, dens, float(np.float32(0.0012)), mask=mask)
+            t1 = time.perf_counter()
+            O.terma(g, pm, st["mlc"], raw.astype(np.float32), mask=mask)
+            t2 = time.perf_counter()
+            O.ccc(g, pm, 100.0, t32, dens, e32, mask)
+            t3 = time.perf_counter()
+            if t3 - t0 > budget_s / 3 or planes >= dens.shape[0]:
+                break
+            planes = min(dens.shape[0], planes * 2)
+    finally:
+        O.set_threads(nthr)
+    nv = int(mask.sum())
+    na = int((active & (mask > 0)).sum())
+    ms = ((t1 - t0 + t2 - t1) / nv * n + (t3 - t2) / max(na, 1) * n_act) * 1000.0
+    return ms, f"one thread, beam 0 of 7: ray trace, TERMA and CCCS on {planes} z-planes ({nv} voxels, {na} " \
+               f"active), scaled to the {dens.shape} grid ({n} voxels, {n_act} active)"
+
+
+def cpu_baseline_impt_1t(st, budget_s):
+    """The IMPT baseline on one host thread: ray trace on a slab of z-planes
+    scaled by voxels, smoothing on the full grid, the pencil beam on a
+    sample of spots scaled by voxel-spot pairs."""
+    from oracle import oracle as O
+    m, c = st["m"], st["case"]
+    rlsp = st["rlsp"]
+    D = (np.float32(m.VSADX) + np.float32(m.VSADY)) / np.float32(2.0)
+    g = O.geom(rlsp.shape, c.spacing, c.origin, c.iso, 0.0, 0.0, 0.0, D)
+    raw, _ = O.raytrace(g, rlsp, 0.0)
+    wet32 = O.smooth(g, raw.astype(np.float32)).astype(np.float32)
+    total_pairs = st["plan"].pairs(wet32)
+    nthr = O.get_threads()
+    O.set_threads(1)
+    try:
+        planes = 1
+        while True:
+            mask = _slab_mask(rlsp.shape, planes)
+            t0 = time.perf_counter()
+            O.raytrace(g, rlsp, 0.0, mask=mask)
+            dt = time.perf_counter() - t0
+            if dt > budget_s / 6 or planes >= rlsp.shape[0]:
+                break
+            planes = min(rlsp.shape[0], planes * 2)
+        t_rt = dt / int(mask.sum()) * rlsp.size
+        t0 = time.perf_counter()
+        O.smooth(g, raw.astype(np.float32))
+        t_sm = time.perf_counter() - t0
+        spots = st["spots"]
+        done, t_p = 0, 0.0
+        for q in range(0, len(spots), 97):
+            t1 = time.perf_counter()
+            _, pairs = O.pencil(g, m.VSADX, m.VSADY, wet32, spots[q:q + 1], m.divergence_params, m.lut_depths,
+                                m.lut_sigmas, m.lut_idds)
+            t_p += time.perf_counter() - t1
+            done += pairs
+            if t_p > budget_s / 3:
+                break
+    finally:
+        O.set_threads(nthr)
+    t_field = t_rt + t_sm + t_p * total_pairs / max(done, 1)
+    return total_pairs / t_field / 1e9, f"one thread: ray trace on {planes} z-planes scaled by voxels, smoothing " \
+                                         f"on the full grid, pencil beam on {done} of {total_pairs} pairs"
+
+
 def cpu_baseline_impt(st, budget_s):
     from oracle import oracle as O
     m, c = st["m"], st["case"]
@@ -402,6 +494,9 @@ def main():
     res = {}
     cpu = {}
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    if not a.no_cpu:
+        from oracle import oracle as O
+        threads = O.get_threads()          # the oracle's OpenMP threads (the cores it uses)
 
     if a.workload in ("all", "impt"):
         log("setting up IMPT 100^3 field")
@@ -435,6 +530,9 @@ def main():
             v, sample = cpu_baseline_impt(st, a.cpu_seconds / 2)
             cpu["impt"] = dict(value=v, unit="Gvoxel-spot/s", sample=sample)
             log(f"IMPT CPU oracle: {v:.4f} Gvoxel-spot/s")
+            v1, sample1 = cpu_baseline_impt_1t(st, a.cpu_seconds)
+            cpu["impt"]["single_thread"] = dict(value=v1, unit="Gvoxel-spot/s", cores=1, sample=sample1)
+            log(f"IMPT CPU oracle, one thread: {v1:.5f} Gvoxel-spot/s")
             res["impt"]["oracle_error"] = oracle_errors_impt(st)
             log(f"IMPT vs FP64 oracle: {res['impt']['oracle_error']}")
         st["plan"].close()
@@ -499,6 +597,9 @@ def main():
             v, sample = cpu_baseline_cccs(st, a.cpu_seconds)
             cpu["cccs"] = dict(value=v, unit="ms/beam", sample=sample)
             log(f"CCCS CPU oracle: {v:.1f} ms/beam")
+            v1, sample1 = cpu_baseline_cccs_1t(st, a.cpu_seconds)
+            cpu["cccs"]["single_thread"] = dict(value=v1, unit="ms/beam", cores=1, sample=sample1)
+            log(f"CCCS CPU oracle, one thread: {v1:.0f} ms/beam")
             res["cccs"]["oracle_error"] = oracle_errors_cccs(torch, st)
             log(f"CCCS vs FP64 oracle: max {res['cccs']['oracle_error']['max_rel']:.2e} "
                 f"rms {res['cccs']['oracle_error']['rms_rel']:.2e}")
@@ -661,7 +762,8 @@ def main():
                    "beams": 7, "grid": [128, 128, 128], "spacing_mm": 2.5},
         "roofline": cc["roofline"] if cc else None,
         "cpu_baseline": ({"value": cpu["cccs"]["value"], "unit": "ms/beam", "cores": threads,
-                          "kind": "port", "sample": cpu["cccs"]["sample"]} if "cccs" in cpu else None),
+                          "kind": "port", "sample": cpu["cccs"]["sample"],
+                          "single_thread": cpu["cccs"].get("single_thread")} if "cccs" in cpu else None),
         "oracle_error": {k: ({kk: vv for kk, vv in res[k]["oracle_error"].items() if kk != "per_beam"}
                              if res.get(k) and "oracle_error" in res[k] else None)
                          for k in ("impt", "cccs", "influence")},

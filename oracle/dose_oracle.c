@@ -202,14 +202,22 @@ double orc_sample(const float *v, int nx, int ny, int nz,
 /* rayTraceKernel, CudaClasses.cu:180-228.  Returns the WET in cm     */
 /* and (optionally) the number of samples on each ray.                */
 /* ------------------------------------------------------------------ */
-void orc_raytrace(const orc_geom *g, const float *rho, double border,
-                  double *wet_out, int32_t *nsamp_out)
+/* `mask` (optional): only voxels with mask[v] != 0 are traced (the others
+ * get 0); the CPU baseline times a slab of voxels this way. */
+void orc_raytrace_m(const orc_geom *g, const float *rho, double border,
+                    double *wet_out, int32_t *nsamp_out, const uint8_t *mask)
 {
     const int nx = g->nx, ny = g->ny, nz = g->nz;
     #pragma omp parallel for schedule(dynamic, 4) collapse(2)
     for (int k = 0; k < nz; k++)
         for (int j = 0; j < ny; j++)
             for (int i = 0; i < nx; i++) {
+                if (mask && !mask[(size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * (size_t)k)]) {
+                    size_t v0 = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * (size_t)k);
+                    wet_out[v0] = 0.0;
+                    if (nsamp_out) nsamp_out[v0] = 0;
+                    continue;
+                }
                 float p[3], u[3], q[3], t[3];
                 vox_xyz_f(g, i, j, k, p);
                 uvec_f(g, p, u);
@@ -229,6 +237,12 @@ void orc_raytrace(const orc_geom *g, const float *rho, double border,
                 wet_out[v] = -0.05 + 0.1 * sum;
                 if (nsamp_out) nsamp_out[v] = n;
             }
+}
+
+void orc_raytrace(const orc_geom *g, const float *rho, double border,
+                  double *wet_out, int32_t *nsamp_out)
+{
+    orc_raytrace_m(g, rho, border, wet_out, nsamp_out, NULL);
 }
 
 /* ------------------------------------------------------------------ */
@@ -439,8 +453,8 @@ static double head_transmission_d(const orc_photon_model *m, const float *mlc, i
 }
 
 /* termaKernel, IMRTClasses.cu:209-250 (+ offAxisFactors 152-171). */
-void orc_terma(const orc_geom *g, const orc_photon_model *m, const float *mlc, int n_pairs,
-               const float *wet, double *terma_out, double *electron_out)
+void orc_terma_m(const orc_geom *g, const orc_photon_model *m, const float *mlc, int n_pairs,
+                 const float *wet, double *terma_out, double *electron_out, const uint8_t *mask)
 {
     const int nx = g->nx, ny = g->ny, nz = g->nz;
     #pragma omp parallel for schedule(dynamic, 2) collapse(2)
@@ -448,6 +462,7 @@ void orc_terma(const orc_geom *g, const orc_photon_model *m, const float *mlc, i
         for (int j = 0; j < ny; j++)
             for (int i = 0; i < nx; i++) {
                 size_t v = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * (size_t)k);
+                if (mask && !mask[v]) { terma_out[v] = 0.0; electron_out[v] = 0.0; continue; }
                 float pf[3];
                 vox_xyz_f(g, i, j, k, pf);
                 double p[3] = { pf[0], pf[1], pf[2] }, h[3];
@@ -480,6 +495,17 @@ void orc_terma(const orc_geom *g, const orc_photon_model *m, const float *mlc, i
                 electron_out[v] = m->esw * (0.4 + 0.3 * tr) * el * Tp;
             }
 }
+
+void orc_terma(const orc_geom *g, const orc_photon_model *m, const float *mlc, int n_pairs,
+               const float *wet, double *terma_out, double *electron_out)
+{
+    orc_terma_m(g, m, mlc, n_pairs, wet, terma_out, electron_out, NULL);
+}
+
+/* OpenMP threads of the oracle's parallel loops (the CPU baseline times one
+ * thread and all host cores). */
+void orc_set_threads(int n);
+int orc_get_threads(void);
 
 /* kernelTilt, IMRTClasses.cu:173-203 (FP64). */
 static void kernel_tilt_d(const orc_geom *g, const double p[3], double d[3])
@@ -606,3 +632,12 @@ int orc_cone_steps(float L0, float spacing_mm)
     for (float L = L0; L >= 0.0f; L = L - sp) n++;
     return n;
 }
+
+#ifdef _OPENMP
+#include <omp.h>
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int orc_get_threads(void) { return omp_get_max_threads(); }
+#else
+void orc_set_threads(int n) { (void)n; }
+int orc_get_threads(void) { return 1; }
+#endif

@@ -73,6 +73,10 @@ def load(path: str):
                              C.c_double, C.c_double]
     L.orc_sample.restype = C.c_double
     L.orc_raytrace.argtypes = [C.POINTER(Geom), P, C.c_double, P, P]
+    L.orc_raytrace_m.argtypes = [C.POINTER(Geom), P, C.c_double, P, P, P]
+    L.orc_terma_m.argtypes = [C.POINTER(Geom), C.POINTER(PhotonModel), P, C.c_int, P, P, P, P]
+    L.orc_set_threads.argtypes = [C.c_int]
+    L.orc_get_threads.restype = C.c_int
     L.orc_smooth.argtypes = [C.POINTER(Geom), P, P]
     L.orc_make_layers.argtypes = [P, C.c_int, C.c_int, P, P, P]
     L.orc_make_layers.restype = C.c_int
@@ -178,12 +182,22 @@ def sample(vol, t, border=0.0):
                             float(border))
 
 
-def raytrace(g: Geom, rho, border=0.0):
+def raytrace(g: Geom, rho, border=0.0, mask=None):
     rho = _f32(rho)
     out = np.empty(rho.shape, np.float64)
     ns = np.empty(rho.shape, np.int32)
-    lib().orc_raytrace(C.byref(g), _p(rho), float(border), _p(out), _p(ns))
+    m8 = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+    lib().orc_raytrace_m(C.byref(g), _p(rho), float(border), _p(out), _p(ns), _p(m8))
     return out, ns
+
+
+def set_threads(n: int) -> None:
+    """OpenMP threads of the oracle's loops."""
+    lib().orc_set_threads(int(n))
+
+
+def get_threads() -> int:
+    return int(lib().orc_get_threads())
 
 
 def smooth(g: Geom, wet32):
@@ -257,13 +271,14 @@ def photon_model(m) -> tuple:
     return pm, keep
 
 
-def terma(g: Geom, model, mlc_rows, wet32):
+def terma(g: Geom, model, mlc_rows, wet32, mask=None):
     pm, keep = photon_model(model)
     mlc = _f32(np.asarray(mlc_rows)[:, :4])
     wet32 = _f32(wet32)
     t = np.empty(wet32.shape, np.float64)
     e = np.empty(wet32.shape, np.float64)
-    lib().orc_terma(C.byref(g), C.byref(pm), _p(mlc), mlc.shape[0], _p(wet32), _p(t), _p(e))
+    m8 = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+    lib().orc_terma_m(C.byref(g), C.byref(pm), _p(mlc), mlc.shape[0], _p(wet32), _p(t), _p(e), _p(m8))
     del keep
     return t, e
 
