@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libdosehip.so")
 
-SOURCES = ["impt.hip", "imrt.hip", "hu.hip", "influence.hip", "surrogate.hip", "scheduler.hip", "tuner.hip", "noalloc.hip", "runtime.cpp", "api.cpp"]
+SOURCES = ["impt.hip", "imrt.hip", "hu.hip", "influence.hip", "surrogate.hip", "scheduler.hip", "tuner.hip", "noalloc.hip", "host_tables.cpp", "runtime.cpp", "api.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -52,6 +52,29 @@ def build(verbose: bool = False) -> str:
         raise RuntimeError("hipcc failed building libdosehip.so")
     os.replace(tmp, LIB)
     return LIB
+
+
+def build_sanitize_host(verbose: bool = False) -> str:
+    """CPU-only executable of the GPU-free host set-up (host_tables.cpp) and
+    its driver (sanitize_host.cpp), built with AddressSanitizer and
+    UndefinedBehaviorSanitizer on the host side only; written to
+    dosecuda_amd/_build/sanitize_host.  Returns its path."""
+    out_dir = os.path.join(HERE, "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    exe = os.path.join(out_dir, "sanitize_host")
+    cmd = [hipcc(), "--offload-arch=gfx950", "-O1", "-g", "-std=c++17", "-ffp-contract=off",
+           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+           "-fno-omit-frame-pointer",
+           "-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+           "-Xarch_host", "-fno-sanitize-recover=all",
+           "-o", exe, os.path.join(CSRC, "host_tables.cpp"), os.path.join(CSRC, "sanitize_host.cpp")]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed building the host sanitizer check")
+    return exe
 
 
 if __name__ == "__main__":

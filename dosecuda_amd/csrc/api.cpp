@@ -180,70 +180,6 @@ extern "C" int dh_stage_raytrace_dev(int gpu_id, const dh_grid *grid, const floa
 // ---------------------------------------------------------------------------
 // proton: spots, layers and the pencil-beam dose
 // ---------------------------------------------------------------------------
-// make_spot_array + importLayers (dosemodule.cu:211-225, IMPTClasses.cu:29-67).
-// The sort is stable, so spots of one layer keep the caller's order (the
-// reference's std::sort leaves that order unspecified).
-dh_proton_tables dh_build_proton_tables(const dh_proton_beam *beam, const dh_proton_lut *lut,
-                                        const float *spot_rows, int32_t n_spots)
-{
-    DH_REQUIRE(beam && lut, "null argument");
-    DH_REQUIRE(n_spots >= 0, "negative spot count");
-    DH_REQUIRE(n_spots == 0 || spot_rows, "null spot array");
-    DH_REQUIRE(lut->n_energies >= 0, "negative energy count");
-    DH_REQUIRE(lut->n_energies == 0 ||
-                   (lut->divergence_params && lut->depths && lut->sigmas && lut->idds),
-               "null beam-model table");
-    DH_REQUIRE(lut->row_stride >= DH_LUT_LENGTH || lut->n_energies == 0,
-               "lookup tables need at least 400 columns");
-    struct sp { float x, y, mu; int id, row; };
-    std::vector<sp> s((size_t)n_spots);
-    for (int32_t q = 0; q < n_spots; q++) {
-        s[q].x = spot_rows[4 * q];
-        s[q].y = spot_rows[4 * q + 1];
-        s[q].mu = spot_rows[4 * q + 2];
-        s[q].id = static_cast<int>(spot_rows[4 * q + 3]);
-        s[q].row = q;
-    }
-    std::stable_sort(s.begin(), s.end(), [](const sp &a, const sp &b) { return a.id < b.id; });
-
-    dh_proton_tables t;
-    const float vx = (float)beam->vsadx, vy = (float)beam->vsady;
-    t.spots.resize((size_t)8 * s.size());
-    t.order.resize(s.size());
-    for (size_t q = 0; q < s.size(); q++) {
-        t.order[q] = s[q].row;
-        // unit spot-axis direction (x, y components), IMPTClasses.cu:125
-        float tx = s[q].x / vx, ty = s[q].y / vy;
-        float n = (float)(1.0 / std::sqrt((double)tx * tx + (double)ty * ty + 1.0));
-        float *r = &t.spots[8 * q];
-        r[0] = s[q].x; r[1] = s[q].y; r[2] = tx * n; r[3] = ty * n;
-        r[4] = s[q].mu; r[5] = 0.0f; r[6] = 0.0f; r[7] = 0.0f;
-    }
-    int start = 0;
-    for (int e = 0; e < lut->n_energies; e++) {
-        int end = start;
-        while (end < n_spots && s[end].id == e) end++;
-        int cnt = end - start;
-        if (!cnt) continue;
-        const float *dvp = lut->divergence_params + (size_t)5 * e;   // dvp_len = 5
-        dh_layer_k L;
-        L.start = start; L.n = cnt;
-        L.r80 = dvp[1];
-        L.cut = 1.1f * dvp[1];                                        // IMPTClasses.cu:224
-        L.A = dvp[2]; L.B = dvp[3]; L.C = dvp[4]; L.pad_ = 0.0f;
-        t.layers.push_back(L);
-        const size_t off = (size_t)e * lut->row_stride;
-        for (int q = 0; q < DH_LUT_LENGTH; q++) {
-            float idd = lut->idds[off + q];
-            if (!(idd >= 0.0f) || !std::isfinite(idd)) t.fold = false;
-            t.luts.push_back(make_float4(lut->depths[off + q], lut->sigmas[off + q], idd, 0.0f));
-        }
-        start += cnt;
-    }
-    t.n_layers = (int)t.layers.size();
-    return t;
-}
-
 // Device copy of the proton tables; stays alive for the plan's lifetime.
 struct dh_proton_plan {
     int dev = 0;
@@ -375,124 +311,6 @@ extern "C" int dh_proton_spot(int gpu_id, const dh_grid *grid, const dh_proton_b
 // ---------------------------------------------------------------------------
 // photon: TERMA + CCCS
 // ---------------------------------------------------------------------------
-static dh_geom photon_geom(const dh_grid *grid, const dh_photon_cp *cp, const dh_photon_model *m)
-{
-    return dh_make_geom(*grid, cp->iso, cp->ga, cp->ta, cp->ca, (float)m->primary_source_distance);
-}
-
-static void check_model(const dh_photon_model *m)
-{
-    DH_REQUIRE(m, "null photon model");
-    DH_REQUIRE(m->n_profile > 0 && m->n_profile <= DH_MAX_PROFILE, "profile must have 1..32 points");
-    DH_REQUIRE(m->n_spectral >= 0 && m->n_spectral <= DH_MAX_SPECTRAL, "spectrum must have at most 16 energies");
-    DH_REQUIRE(m->profile_radius && m->profile_intensities && m->profile_softening, "null profile");
-    DH_REQUIRE(m->n_spectral == 0 || (m->spectrum_attenuation_coefficients &&
-               m->spectrum_primary_weights && m->spectrum_scatter_weights), "null spectrum");
-}
-
-static dh_terma_args make_terma_args(const dh_grid *grid, const dh_photon_cp *cp,
-                                     const dh_photon_model *m, const float *mlc_rows, int32_t n_pairs)
-{
-    check_model(m);
-    DH_REQUIRE(n_pairs >= 0 && n_pairs <= DH_MAX_PAIRS, "at most 128 MLC leaf pairs are supported");
-    DH_REQUIRE(n_pairs == 0 || mlc_rows, "null MLC array");
-    dh_terma_args a;
-    std::memset(&a, 0, sizeof(a));
-    a.g = photon_geom(grid, cp, m);
-    const float sad = (float)m->primary_source_distance, dm = (float)m->mlc_distance;
-    const float ms = (sad - dm) / sad;                       // IMRTClasses.cu:122
-    int q = 0;
-    for (int i = 0; i < n_pairs; i++) {
-        const float *r = mlc_rows + 4 * i;
-        // closed pairs (x1 == x2) contribute exactly 0 * ey in FP32
-        if (r[0] == r[1]) continue;
-        a.mlc[q++] = make_float4(r[0] * ms, r[1] * ms, (r[2] - 0.5f * r[3]) * ms,
-                                 (r[2] + 0.5f * r[3]) * ms);
-    }
-    // Consecutive open pairs with equal leaf tips whose y ranges meet edge to
-    // edge become one rectangle: the head transmission sums such a run as
-    // ex * (erf(y_top) - erf(y_bottom)) / 2 (the per-pair terms telescope),
-    // which needs only the run's outer edges.  A rectangular field is one.
-    int nr = 0;
-    for (int i = 0; i < q; i++) {
-        if (nr > 0 && a.mlc[i].x == a.mlc[nr - 1].x && a.mlc[i].y == a.mlc[nr - 1].y &&
-            a.mlc[i].z == a.mlc[nr - 1].w)
-            a.mlc[nr - 1].w = a.mlc[i].w;
-        else
-            a.mlc[nr++] = a.mlc[i];
-    }
-    q = nr;
-    a.n_pairs = q;
-    a.sorted = 1;
-    for (int i = 1; i < q; i++)
-        if (!(a.mlc[i].z >= a.mlc[i - 1].z && a.mlc[i].w >= a.mlc[i - 1].w)) a.sorted = 0;
-    a.sad = sad;
-    a.ssd_s = (float)m->scatter_source_distance;
-    a.inv_p = 1.0f / ((float)m->primary_source_size * sqrtf(2.f));     // IMRTClasses.cu:123-124
-    a.inv_s_src = 1.0f / ((float)m->scatter_source_size * sqrtf(2.f));
-    a.d_mlc = dm;
-    a.ssw = (float)m->scatter_source_weight;
-    a.el_att = (float)m->electron_attenuation;
-    a.esw = (float)m->electron_source_weight;
-    a.el_norm_a = expf(-a.el_att * (float)m->electron_fitted_dmax);
-    a.el_norm_b = 1.0f - a.el_norm_a;
-    a.n_prof = m->n_profile;
-    for (int i = 0; i < m->n_profile; i++) {
-        a.prof_r[i] = m->profile_radius[i];
-        a.prof_i[i] = m->profile_intensities[i];
-        a.prof_s[i] = m->profile_softening[i];
-    }
-    a.n_spec = m->n_spectral;
-    for (int i = 0; i < m->n_spectral; i++) {
-        a.mu_e[i] = m->spectrum_attenuation_coefficients[i];
-        a.pw[i] = m->spectrum_primary_weights[i];
-        a.sw[i] = m->spectrum_scatter_weights[i];
-    }
-    return a;
-}
-
-static dh_ccc_args make_ccc_args(const dh_grid *grid, const dh_photon_cp *cp, const dh_photon_model *m)
-{
-    check_model(m);
-    dh_ccc_args a;
-    std::memset(&a, 0, sizeof(a));
-    a.g = photon_geom(grid, cp, m);
-    a.p = dh_make_pad(a.g.nx, a.g.ny, a.g.nz);
-    dh_uvec(a.g, 0.0f, 0.0f, 0.0f, a.ax, a.ay, a.az);          // IMRTClasses.cu:177-179
-    a.sp = a.g.s / 10.0f;                                      // IMRTClasses.cu:278
-    a.mu_cal = (float)m->mu_calibration;
-    a.mu = (float)cp->mu;
-    for (int r = 0; r < 6; r++) {
-        const float *K = m->kernel + 6 * r;
-        const float th = K[0] * DH_PI_F / 180.0f;
-        const float sth = sinf(th), cth = cosf(th);
-        const float A = K[1], am = K[2], B = K[3], bm = K[4];
-        a.L0[r] = K[5];
-        a.a[r] = am;
-        a.b[r] = bm;
-        a.c1[r] = sth * (A / (am * am));
-        a.c2p[r] = sth * (B / bm) * a.sp;
-        a.bp[r] = bm * a.sp;
-        a.a2[r] = -am * a.sp * DH_LOG2E_F;
-        int ns = 0;
-        for (float L = K[5]; L >= 0.0f; L = L - a.sp) {
-            ns++;
-            DH_REQUIRE(ns < 100000, "cone ray_length / spacing gives too many steps");
-        }
-        a.nsteps[r] = ns;
-        for (int az = 0; az < 12; az++) {
-            const float ang = (float)az * DH_PI_F / 6.0f;      // IMRTClasses.cu:286-288
-            const float sx = sinf(ang), cx = cosf(ang);
-            float dx, dy, dz;
-            dh_head_to_img(a.g, sth * cx, sth * sx, cth, dx, dy, dz);
-            a.dir[3 * (r * 12 + az) + 0] = dx;
-            a.dir[3 * (r * 12 + az) + 1] = dy;
-            a.dir[3 * (r * 12 + az) + 2] = dz;
-        }
-    }
-    return a;
-}
-
 // Unit vector source -> isocentre of a control point in volume axes (the
 // beam direction channels of the surrogate input).
 static void sur_direction(const dh_photon_cp *cp, float dir[3])
@@ -523,8 +341,8 @@ static void photon_device(int dev, const dh_grid *grid, const dh_photon_cp *cp,
 {
     size_t n = nvox_of(grid);
     DH_REQUIRE(cp && d_density && d_dose, "null argument");
-    dh_terma_args ta = make_terma_args(grid, cp, m, mlc_rows, n_pairs);
-    dh_ccc_args ca = make_ccc_args(grid, cp, m);
+    dh_terma_args ta = dh_make_terma_args(grid, cp, m, mlc_rows, n_pairs);
+    dh_ccc_args ca = dh_make_ccc_args(grid, cp, m);
     dh_pad p = ca.p;
     float4 *vpad = (float4 *)dh_scratch(dev, DH_SLOT_VOL_PAD, sizeof(float4) * (size_t)p.pxy * p.pz);
     float *wet = d_wet ? d_wet : (float *)dh_scratch(dev, DH_SLOT_WET_RAW, sizeof(float) * n);
@@ -599,7 +417,7 @@ extern "C" int dh_stage_terma_dev(int gpu_id, const dh_grid *grid, const dh_phot
         dh_device_enter(gpu_id);
         nvox_of(grid);
         DH_REQUIRE(cp && d_wet && d_terma_out && d_electron_out, "null argument");
-        dh_terma_args ta = make_terma_args(grid, cp, model, mlc_rows, n_pairs);
+        dh_terma_args ta = dh_make_terma_args(grid, cp, model, mlc_rows, n_pairs);
         dh_launch_terma(ta, d_wet, d_terma_out, d_electron_out, (hipStream_t)stream);
     });
 }
@@ -613,7 +431,7 @@ extern "C" int dh_stage_ccc_dev(int gpu_id, const dh_grid *grid, const dh_photon
         dh_device_enter(gpu_id);
         size_t n = nvox_of(grid);
         DH_REQUIRE(cp && d_terma && d_density && d_electron && d_dose_out, "null argument");
-        dh_ccc_args ca = make_ccc_args(grid, cp, model);
+        dh_ccc_args ca = dh_make_ccc_args(grid, cp, model);
         dh_pad p = ca.p;
         void *trp = dh_scratch(gpu_id, DH_SLOT_TR_PAD, dh_tr_pad_bytes(ca));
         int *act = (int *)dh_scratch(gpu_id, DH_SLOT_ACTIVE, sizeof(int) * n);
@@ -632,7 +450,7 @@ extern "C" int dh_stage_ccc_list_dev(int gpu_id, const dh_grid *grid, const dh_p
         size_t nv = nvox_of(grid);
         DH_REQUIRE(cp && d_terma && d_density && d_electron && d_dose_out, "null argument");
         DH_REQUIRE(n >= 0 && (size_t)n <= nv && (n == 0 || d_list), "bad voxel list");
-        dh_ccc_args ca = make_ccc_args(grid, cp, model);
+        dh_ccc_args ca = dh_make_ccc_args(grid, cp, model);
         dh_pad p = ca.p;
         void *trp = dh_scratch(gpu_id, DH_SLOT_TR_PAD, dh_tr_pad_bytes(ca));
         int *cnt = (int *)dh_scratch(gpu_id, DH_SLOT_COUNTER, 256);
@@ -665,7 +483,7 @@ extern "C" int dh_sur_check_dev(int gpu_id, const dh_grid *grid, const dh_photon
         DH_REQUIRE(stride >= 1 && stride <= 64, "lattice stride must be 1..64");
         DH_REQUIRE(work_len >= dh_sur_check_work_len(grid->nz, grid->ny, grid->nx, stride),
                    "work buffer too small (dh_sur_check_work_size)");
-        dh_ccc_args ca = make_ccc_args(grid, cp, model);
+        dh_ccc_args ca = dh_make_ccc_args(grid, cp, model);
         dh_pad p = ca.p;
         void *trp = dh_scratch(gpu_id, DH_SLOT_TR_PAD, dh_tr_pad_bytes(ca));
         dh_launch_sur_check(ca, d_terma, d_density, d_electron, d_dose, d_dmax, stride, tol, dilate, d_work,
@@ -700,7 +518,7 @@ extern "C" int dh_sur_dose_dev(const dh_sur_net *net, const dh_grid *grid, const
         DH_REQUIRE(dilate >= 0 && dilate <= 2, "dilate must be 0, 1 or 2");
         dh_device_enter(dh_sur_net_gpu(net));
         nvox_of(grid);
-        dh_ccc_args ca = make_ccc_args(grid, cp, model);
+        dh_ccc_args ca = dh_make_ccc_args(grid, cp, model);
         float dir[3];
         sur_direction(cp, dir);
         dh_launch_sur_dose(net, ca, d_terma, d_density, d_electron, dir, stride, tol, dilate, d_dose_out,
@@ -749,7 +567,7 @@ extern "C" int dh_sur_dose_batch_dev(const dh_sur_net *net, const dh_grid *grid,
         std::vector<float> dirs(3 * (size_t)n_cp);
         cas.reserve(n_cp);
         for (int i = 0; i < n_cp; i++) {
-            cas.push_back(make_ccc_args(grid, &cps[i], model));
+            cas.push_back(dh_make_ccc_args(grid, &cps[i], model));
             sur_direction(&cps[i], &dirs[3 * (size_t)i]);
         }
         dh_launch_sur_dose_batch(net, cas.data(), n_cp, d_terma, d_density, d_electron,
@@ -821,8 +639,8 @@ extern "C" int dh_autotune_photon(int gpu_id, const dh_grid *grid, const dh_phot
         hipStream_t s = (hipStream_t)stream;
         const size_t n = nvox_of(grid);
         DH_REQUIRE(cp && d_density, "null argument");
-        dh_terma_args ta = make_terma_args(grid, cp, model, mlc_rows, n_pairs);
-        dh_ccc_args ca = make_ccc_args(grid, cp, model);
+        dh_terma_args ta = dh_make_terma_args(grid, cp, model, mlc_rows, n_pairs);
+        dh_ccc_args ca = dh_make_ccc_args(grid, cp, model);
         const dh_pad p = ca.p;
         float4 *vpad = (float4 *)dh_scratch(gpu_id, DH_SLOT_VOL_PAD, sizeof(float4) * (size_t)p.pxy * p.pz);
         float *wref = (float *)dh_scratch(gpu_id, DH_SLOT_WET_RAW, sizeof(float) * n);

@@ -278,3 +278,21 @@ dh_proton_tables dh_build_proton_tables(const dh_proton_beam *beam, const dh_pro
 // CudaClasses.cu:17-51, IMRTClasses.cu:30-39).
 dh_geom dh_make_geom(const dh_grid &grid, const float iso[3], double gantry, double couch,
                      double collimator, float src_dist);
+
+// Photon control-point set-up (host only): geometry, model checks, TERMA and
+// cone-superposition kernel arguments.
+dh_geom dh_photon_geom(const dh_grid *grid, const dh_photon_cp *cp, const dh_photon_model *m);
+void dh_check_photon_model(const dh_photon_model *m);
+dh_terma_args dh_make_terma_args(const dh_grid *grid, const dh_photon_cp *cp, const dh_photon_model *m,
+                                 const float *mlc_rows, int32_t n_pairs);
+dh_ccc_args dh_make_ccc_args(const dh_grid *grid, const dh_photon_cp *cp, const dh_photon_model *m);
+
+// Influence-matrix table bounds of one layer (host only): per-layer maxima
+// of the total sigma and the dose factor over WET in [0, 1.1 R80] and source
+// distances in [dist_min, dist_max], the depth-resolved bounds per 1 mm cell
+// {sigma_t max, P max, sigma_ht max, H max}, and the radius beyond which
+// those put every voxel below a threshold.
+void dh_infl_layer_bounds(const dh_layer_k &L, const float4 *lut, float dist_min, float dist_max,
+                          float &sig_max, float &dose_max);
+std::vector<float4> dh_infl_depth_cells(const dh_layer_k &L, const float4 *lut, float dist_min, float dist_max);
+float dh_infl_depth_radius(const std::vector<float4> &cells, float thr);

@@ -543,118 +543,6 @@ struct dh_influence {
     int64_t fp_fallbacks = 0;
 };
 
-// Conservative per-layer bounds on the total sigma and on the dose factor
-// P + H over every voxel the layer can reach (w in [0, 1.1 R80], source
-// distance within the volume's range), from the tables themselves.
-static void layer_bounds(const dh_layer_k &L, const float4 *lut, float dist_min, float dist_max,
-                         float &sig_max, float &dose_max)
-{
-    const float r80 = L.r80;
-    float s_ms_max = -1e30f, s_ms_min = 1e30f, idd_max = 0.0f;
-    for (int q = 0; q < DH_LUT_LENGTH; q++) {
-        if (lut[q].x > L.cut + 1.0f && q > 0) break;
-        s_ms_max = std::max(s_ms_max, lut[q].y);
-        s_ms_min = std::min(s_ms_min, lut[q].y);
-        idd_max = std::max(idd_max, lut[q].z);
-    }
-    float sa_max = -1e30f, sa_min = 1e30f;
-    for (int q = 0; q <= 64; q++) {
-        const float w = L.cut * q / 64.0f;
-        for (int e = 0; e < 2; e++) {
-            const float d = (e ? dist_max : dist_min) - w + 0.7f * r80;
-            const float sa = L.A * d * d + L.B * d + L.C;
-            sa_max = std::max(sa_max, sa);
-            sa_min = std::min(sa_min, sa);
-        }
-    }
-    float sh_max = 0.0f;
-    for (int q = 0; q <= 256; q++) {
-        const float wc = std::min(std::max(L.cut * q / 256.0f, 0.1f), r80 - 0.1f);
-        const float t5 = wc - r80 - 5.0f;
-        float sh = 2.85f + 0.0014f * r80 * logf(wc + 3.0f) + 0.06f * wc - 7.4e-5f * wc * wc
-                 - 0.22f * r80 / (t5 * t5);
-        sh_max = std::max(sh_max, sh);
-    }
-    const float st_max = std::max(0.0f, sa_max + s_ms_max);
-    const float st_min = std::max(1e-3f, sa_min + s_ms_min);
-    sig_max = 1.05f * std::sqrt(st_max * st_max + sh_max * sh_max) + 0.5f;
-    // P + H <= idd / (2 pi st^2) (both normalisations are at most that)
-    dose_max = 1.05f * idd_max / (2.0f * DH_PI_F * st_min * st_min);
-}
-
-// Depth-resolved table bounds of one layer: for each 1 mm cell [d_q, d_q+1]
-// of the depth table within the range cut, bounds over every WET in the
-// cell and every source distance in [dist_min, dist_max] of
-// {sigma_t max, P max, sigma_ht max, H max} (the formulas of
-// dh_pencil_factors: IDD and sigma_ms interpolate linearly, so their extremes
-// are at the cell ends; sigma_air is a quadratic in its argument; the halo
-// forms are sampled at the cell ends and middle and widened by 2%).
-static std::vector<float4> depth_cells(const dh_layer_k &L, const float4 *lut, float dist_min, float dist_max)
-{
-    std::vector<float4> out;
-    const float r80 = L.r80;
-    auto halo = [&](float w, float &sh, float &hw) {
-        const float wc = std::min(std::max(w, 0.1f), r80 - 0.1f);
-        const float t5 = wc - r80 - 5.0f, r3 = r80 + 3.0f;
-        sh = 2.85f + 0.0014f * r80 * logf(wc + 3.0f) + 0.06f * wc - 7.4e-5f * wc * wc - 0.22f * r80 / (t5 * t5);
-        hw = 0.052f * logf(1.13f + wc / (11.2f - 0.023f * r80)) +
-             0.35f * (0.0017f * r80 * r80 - r80) / (r3 * r3 - wc * wc) - 1.61e-9f * wc * r3 * r3;
-        sh = std::max(sh, 0.0f);
-        hw = std::min(std::max(hw, 0.0f), 0.9f);
-    };
-    for (int q = 0; q + 1 < DH_LUT_LENGTH; q++) {
-        // the first cell also covers the small negative WET of voxels next
-        // to the upstream face (-0.05 cm + one sample), which read the first
-        // table entry
-        const float w0 = q == 0 ? std::min(lut[0].x, 0.0f) - 1.0f : lut[q].x;
-        const float w1 = std::min(lut[q + 1].x, L.cut);
-        if (q > 0 && w0 > L.cut) break;
-        if (w1 < w0) continue;
-        const float idd = std::max(std::max(lut[q].z, lut[q + 1].z), 0.0f);
-        const float sm0 = std::min(lut[q].y, lut[q + 1].y), sm1 = std::max(lut[q].y, lut[q + 1].y);
-        const float a0 = dist_min - w1 + 0.7f * r80, a1 = dist_max - w0 + 0.7f * r80;
-        auto sa = [&](float d) { return (L.A * d + L.B) * d + L.C; };
-        float sa0 = std::min(sa(a0), sa(a1)), sa1 = std::max(sa(a0), sa(a1));
-        if (L.A != 0.0f) {
-            const float dv = -L.B / (2.0f * L.A);
-            if (dv > a0 && dv < a1) { sa0 = std::min(sa0, sa(dv)); sa1 = std::max(sa1, sa(dv)); }
-        }
-        const float st_lo = sa0 + sm0, st_hi = sa1 + sm1;
-        // |sigma_t| range (P uses sigma_t^2); a range through 0 has no bound
-        const float at_min = (st_lo <= 0.0f && st_hi >= 0.0f) ? 0.0f : std::min(std::fabs(st_lo), std::fabs(st_hi));
-        const float at_max = 1.01f * std::max(std::fabs(st_lo), std::fabs(st_hi));
-        float sh_lo = 1e30f, sh_hi = 0.0f, hw_lo = 1.0f, hw_hi = 0.0f;
-        for (int e = 0; e < 3; e++) {
-            float sh, hw;
-            halo(e == 0 ? w0 : e == 1 ? w1 : 0.5f * (w0 + w1), sh, hw);
-            sh_lo = std::min(sh_lo, sh); sh_hi = std::max(sh_hi, sh);
-            hw_lo = std::min(hw_lo, hw); hw_hi = std::max(hw_hi, hw);
-        }
-        sh_lo *= 0.98f; sh_hi *= 1.02f;
-        hw_lo = std::max(0.0f, hw_lo - 0.005f); hw_hi = std::min(0.9f, hw_hi + 0.005f);
-        const float sht_min = std::sqrt(at_min * at_min + sh_lo * sh_lo);
-        const float sht_max = 1.01f * std::sqrt(at_max * at_max + sh_hi * sh_hi);
-        const float P = at_min > 0.0f ? 1.02f * (1.0f - hw_lo) * idd / (2.0f * DH_PI_F * at_min * at_min) : INFINITY;
-        const float H = sht_min > 0.0f ? 1.02f * hw_hi * idd / (2.0f * DH_PI_F * sht_min * sht_min) : INFINITY;
-        out.push_back(make_float4(at_max, P, sht_max, H));
-    }
-    return out;
-}
-
-// Radius beyond which every voxel of the layer, whatever its WET, has a dose
-// below thr: both Gaussian terms below thr / 2 in every depth cell.
-static float depth_radius(const std::vector<float4> &cells, float thr)
-{
-    if (!(thr > 0.0f)) return INFINITY;
-    float R = 0.0f;
-    for (const float4 &c : cells) {
-        if (!std::isfinite(c.y) || !std::isfinite(c.w)) return INFINITY;
-        if (2.0f * c.y > thr) R = std::max(R, c.x * std::sqrt(2.0f * std::log(2.0f * c.y / thr)));
-        if (2.0f * c.w > thr) R = std::max(R, c.z * std::sqrt(2.0f * std::log(2.0f * c.w / thr)));
-    }
-    return 1.002f * R;
-}
-
 dh_influence *dh_influence_make(int dev, const dh_geom &g, float vsadx, float vsady,
                                 const std::vector<dh_layer_k> &layers, const std::vector<float4> &luts,
                                 const std::vector<float> &spots, int n_spots, int S, int C, float rel)
@@ -680,10 +568,10 @@ dh_influence *dh_influence_make(int dev, const dh_geom &g, float vsadx, float vs
     f->spot_layer.assign(n_spots, -1);
     for (size_t l = 0; l < layers.size(); l++) {
         float sm, dm;
-        layer_bounds(layers[l], &luts[l * DH_LUT_LENGTH], dmin, dmax, sm, dm);
+        dh_infl_layer_bounds(layers[l], &luts[l * DH_LUT_LENGTH], dmin, dmax, sm, dm);
         f->sig_max.push_back(sm);
         f->dose_max.push_back(dm);
-        f->cells.push_back(depth_cells(layers[l], &luts[l * DH_LUT_LENGTH], dmin, dmax));
+        f->cells.push_back(dh_infl_depth_cells(layers[l], &luts[l * DH_LUT_LENGTH], dmin, dmax));
         f->lthr.push_back(INFINITY);
         // blocks of S spatially adjacent spots: sort the layer's spots by a
         // Morton key of their position on a 4 mm grid
@@ -882,7 +770,7 @@ static void launch(dh_influence *f, const std::vector<dh_infl_block> &blocks, co
 // bounds P + H <= dose_max and sigma_t, sigma_ht <= sig_max.
 static float table_radius(const dh_influence *f, int l, float thr)
 {
-    return depth_radius(f->cells[l], thr);
+    return dh_infl_depth_radius(f->cells[l], thr);
 }
 
 // Divergence factors 1 - hz / VSAD over the volume's head depths.

@@ -1,5 +1,5 @@
-// runtime.cpp -- per-device state (stream, scratch arena), launch timing and
-// the host-side beam set-up.
+// runtime.cpp -- per-device state (stream, scratch arena), the tuning table
+// and launch timing.
 #include "dh_runtime.h"
 
 #include <cmath>
@@ -289,30 +289,4 @@ extern "C" int dh_prof_get(const char *kernel, double *total_ms, int64_t *launch
     if (total_ms) *total_ms = tot;
     if (launches) *launches = n;
     return DH_OK;
-}
-
-// ---------------------------------------------------------------------------
-// beam set-up, FP32 as the reference host code computes it
-// ---------------------------------------------------------------------------
-dh_geom dh_make_geom(const dh_grid &grid, const float iso[3], double gantry, double couch,
-                     double collimator, float src_dist)
-{
-    dh_geom g;
-    g.nx = grid.nx; g.ny = grid.ny; g.nz = grid.nz;
-    g.s = grid.spacing;                 // CudaDose(dims, spacing), dosemodule.cu:146
-    g.inv_s = 1.0f / grid.spacing;      // CudaClasses.cuh:101
-    for (int a = 0; a < 3; a++) g.iso[a] = iso[a] - grid.origin[a];   // dosemodule.cu:160-164
-    float adj = fmodf((float)(gantry + 180.0), 360.0f);               // dosemodule.cu:152
-    float ga = adj * DH_PI_F / 180.0f;                                  // CudaClasses.cu:27
-    float ta = (float)couch * DH_PI_F / 180.0f;
-    float ca = (float)collimator * DH_PI_F / 180.0f;                   // IMRTClasses.cu:35
-    g.singa = sinf(ga); g.cosga = cosf(ga);
-    g.sinta = sinf(ta); g.costa = cosf(ta);
-    g.sinca = sinf(ca); g.cosca = cosf(ca);
-    float xg = -src_dist * g.singa;                                     // CudaClasses.cu:36-49
-    float yg = src_dist * g.cosga;
-    g.src[0] = xg * g.costa;
-    g.src[1] = yg;
-    g.src[2] = -xg * g.sinta;
-    return g;
 }
