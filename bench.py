@@ -669,15 +669,20 @@ def main():
                 autotune.tune_surrogate(net, (64, 64, 64), reps=5)
                 log(f"surrogate: convolution variants tuned ({time.time() - t0:.1f} s)")
             log("surrogate: 7-beam plan, checked surrogate vs plain CCCS")
-            r = S.evaluate(net, PhotonBeamModel(), stride=3, tol=0.01, dilate=1, reps=5)
+            r = S.evaluate(net, PhotonBeamModel(), stride=S.DEFAULT_CHECK[0], tol=S.DEFAULT_CHECK[1],
+                           dilate=S.DEFAULT_CHECK[2], reps=5)
             conv_ms_single = float(np.mean([b["surrogate_stage_ms"]["nn_conv3d"] for b in r["beams"]]))
             conv_ms = r["batch_conv_ms_per_beam"]
             tf = r["net_gflop"] / (conv_ms * 1e-3) / 1e3
-            sur_beam = r["plain_beam_ms"] - r["plain_superposition_ms"] + r["graph_checked_ms_per_beam"]
+            # the whole surrogate path per beam from the density (TERMA with one
+            # ray sample per voxel, checked surrogate batched over the plan)
+            sur_beam = r["path_ms_per_beam"]
             res["surrogate"] = dict(
                 speedup_vs_plain_beam=r["plain_beam_ms"] / sur_beam,
                 surrogate_beam_ms=sur_beam, plain_beam_ms=r["plain_beam_ms"],
                 speedup_target=10.0,
+                min_beam_gamma_path=r["path_min_gamma"], plan_gamma_path=r["path_plan_gamma"],
+                superposition_only_ms_per_beam=r["graph_checked_ms_per_beam"],
                 weights=os.path.relpath(S.WEIGHTS, ROOT), params=r["params"], stride=r["stride"], tol=r["tol"],
                 dilate=r["dilate"],
                 plan_gamma_checked=r["plan_gamma_checked"], min_beam_gamma_checked=r["min_gamma_checked"],
@@ -694,7 +699,7 @@ def main():
                           frac=tf / 2500.0, conv_ms_per_beam_unbatched=conv_ms_single),
                 graph_replay_equal=r["graph_replay_equal"])
             log(f"surrogate: whole beam {sur_beam:.3f} ms vs plain {r['plain_beam_ms']:.3f} ms "
-                f"({r['plain_beam_ms'] / sur_beam:.2f}x); checked superposition "
+                f"({r['plain_beam_ms'] / sur_beam:.2f}x, min beam gamma {r['path_min_gamma']:.5f}); checked superposition "
                 f"{r['graph_checked_ms_per_beam']:.3f} ms/beam vs {r['plain_superposition_ms']:.3f} "
                 f"plain superposition ({r['speedup_superposition']:.2f}x), plan gamma {r['plan_gamma_checked']:.5f}, "
                 f"min beam {r['min_gamma_checked']:.5f}, fallback {r['mean_fallback_fraction']:.3f}, "

@@ -329,6 +329,7 @@ static void sur_direction(const dh_photon_cp *cp, float dir[3])
 // control point of the same case with the same source position.
 struct wet_memo {
     const void *vpad = nullptr, *wet = nullptr;
+    float step = 0.0f;
 };
 static wet_memo g_wet_memo[64][DH_MAX_WORKSPACES];
 
@@ -356,12 +357,18 @@ static void photon_device(int dev, const dh_grid *grid, const dh_photon_cp *cp,
     // position only (not on the collimator or the MLC): a caller that knows
     // the previous control point on this workspace had the same ones may ask
     // to reuse it; it is reused only if this workspace still holds it.
-    if (!(reuse_wet && !d_wet && memo.vpad == vpad && memo.wet == wet)) {
+    // The checked surrogate path samples the ray once per voxel spacing (at
+    // least 1 mm) instead of every mm: its TERMA is an input of the network
+    // and of the plain check alike, and the result is judged by gamma against
+    // the plain CCCS.  The plain path keeps the reference's 1 mm samples.
+    const float step = net ? std::max(1.0f, ca.g.s) : 1.0f;
+    if (!(reuse_wet && !d_wet && memo.vpad == vpad && memo.wet == wet && memo.step == step)) {
         dh_launch_pad_quad(d_density, vpad, ca.g.nx, ca.g.ny, ca.g.nz, DH_AIR_DENSITY, s);  // IMRTClasses.cu:394
-        dh_launch_raytrace(ca.g, vpad, wet, s);
+        dh_launch_raytrace(ca.g, vpad, wet, s, -1, nullptr, step);
     }
     memo.vpad = d_wet ? nullptr : vpad;
     memo.wet = d_wet ? nullptr : wet;
+    memo.step = step;
     dh_launch_terma(ta, wet, terma, el, s);
     if (net) {
         float dir[3];
@@ -370,6 +377,28 @@ static void photon_device(int dev, const dh_grid *grid, const dh_photon_cp *cp,
     } else {
         dh_launch_ccc(ca, terma, d_density, el, trp, act, cnt, d_dose, s);
     }
+}
+
+extern "C" int dh_photon_terma_dev(int gpu_id, const dh_grid *grid, const dh_photon_cp *cp,
+                                   const dh_photon_model *model, const float *mlc_rows, int32_t n_pairs,
+                                   const float *d_density, float step_mm, float *d_terma_out,
+                                   float *d_electron_out, void *stream)
+{
+    return guarded([&] {
+        dh_device_enter(gpu_id);
+        const size_t n = nvox_of(grid);
+        DH_REQUIRE(cp && d_density && d_terma_out && d_electron_out, "null argument");
+        DH_REQUIRE(step_mm >= 1.0f && step_mm <= 16.0f, "ray-trace step must be 1..16 mm");
+        hipStream_t s = (hipStream_t)stream;
+        dh_terma_args ta = dh_make_terma_args(grid, cp, model, mlc_rows, n_pairs);
+        const dh_pad p = dh_make_pad(ta.g.nx, ta.g.ny, ta.g.nz);
+        float4 *vpad = (float4 *)dh_scratch(gpu_id, DH_SLOT_VOL_PAD, sizeof(float4) * (size_t)p.pxy * p.pz);
+        float *wet = (float *)dh_scratch(gpu_id, DH_SLOT_WET_RAW, sizeof(float) * n);
+        dh_launch_pad_quad(d_density, vpad, ta.g.nx, ta.g.ny, ta.g.nz, DH_AIR_DENSITY, s);  // IMRTClasses.cu:394
+        dh_launch_raytrace(ta.g, vpad, wet, s, -1, nullptr, step_mm);
+        dh_launch_terma(ta, wet, d_terma_out, d_electron_out, s);
+        g_wet_memo[gpu_id & 63][dh_get_workspace()] = wet_memo{};   // the padded volume was rewritten
+    });
 }
 
 extern "C" int dh_photon_dose_dev(int gpu_id, const dh_grid *grid, const dh_photon_cp *cp,
@@ -475,7 +504,7 @@ extern "C" int dh_sur_check_dev(int gpu_id, const dh_grid *grid, const dh_photon
                                 int32_t *d_work, int64_t work_len, float *d_plain, void *stream)
 {
     return guarded([&] {
-        DH_REQUIRE(dilate >= 0 && dilate <= 2, "dilate must be 0, 1 or 2");
+        DH_REQUIRE(dilate >= 0 && dilate <= 6 && (dilate & 3) != 3, "dilate must be 0, 1 or 2, plus 4 for the build-up band");
         dh_device_enter(gpu_id);
         nvox_of(grid);
         DH_REQUIRE(cp && d_terma && d_density && d_electron && d_dose && d_dmax && d_work && d_plain,
@@ -500,7 +529,7 @@ extern "C" int dh_photon_dose_surrogate_dev(int gpu_id, const dh_sur_net *net, c
     return guarded([&] {
         DH_REQUIRE(net, "null surrogate network");
         DH_REQUIRE(stride >= 1 && stride <= 64, "lattice stride must be 1..64");
-        DH_REQUIRE(dilate >= 0 && dilate <= 2, "dilate must be 0, 1 or 2");
+        DH_REQUIRE(dilate >= 0 && dilate <= 6 && (dilate & 3) != 3, "dilate must be 0, 1 or 2, plus 4 for the build-up band");
         dh_device_enter(gpu_id);
         photon_device(gpu_id, grid, cp, model, mlc_rows, n_pairs, d_density, d_dose_out, d_terma_out,
                       d_electron_out, nullptr, (hipStream_t)stream, net, stride, tol, dilate);
@@ -515,7 +544,7 @@ extern "C" int dh_sur_dose_dev(const dh_sur_net *net, const dh_grid *grid, const
     return guarded([&] {
         DH_REQUIRE(net && cp && d_terma && d_density && d_electron && d_dose_out, "null argument");
         DH_REQUIRE(stride >= 1 && stride <= 64, "lattice stride must be 1..64");
-        DH_REQUIRE(dilate >= 0 && dilate <= 2, "dilate must be 0, 1 or 2");
+        DH_REQUIRE(dilate >= 0 && dilate <= 6 && (dilate & 3) != 3, "dilate must be 0, 1 or 2, plus 4 for the build-up band");
         dh_device_enter(dh_sur_net_gpu(net));
         nvox_of(grid);
         dh_ccc_args ca = dh_make_ccc_args(grid, cp, model);
@@ -536,7 +565,7 @@ extern "C" int dh_sur_dose_batch_dev(const dh_sur_net *net, const dh_grid *grid,
         DH_REQUIRE(net && cps && d_terma && d_density && d_electron && d_dose_out, "null argument");
         DH_REQUIRE(n_cp >= 1 && n_cp <= 4096, "control point count must be 1..4096");
         DH_REQUIRE(stride >= 1 && stride <= 64, "lattice stride must be 1..64");
-        DH_REQUIRE(dilate >= 0 && dilate <= 2, "dilate must be 0, 1 or 2");
+        DH_REQUIRE(dilate >= 0 && dilate <= 6 && (dilate & 3) != 3, "dilate must be 0, 1 or 2, plus 4 for the build-up band");
         for (int i = 0; i < n_cp; i++)
             DH_REQUIRE(d_terma[i] && d_electron[i] && d_dose_out[i], "null control point volume");
         dh_device_enter(dh_sur_net_gpu(net));

@@ -176,8 +176,9 @@ void dh_launch_pad_quad(const float *src, float4 *dst, int nx, int ny, int nz, f
                         hipStream_t s);
 // choice < 0: the tuning table's choice for the shape (default 0); the
 // pencil's choice is tile + DH_TILE_CHOICES * (LUT in LDS ? 0 : 1)
+// step: sample spacing in mm (1: the reference's ray trace)
 void dh_launch_raytrace(const dh_geom &g, const float4 *vol_pad, float *wet, hipStream_t s, int choice = -1,
-                        const dh_hbox *box = nullptr);
+                        const dh_hbox *box = nullptr, float step = 1.0f);
 void dh_launch_smooth(const dh_geom &g, const float *wet_raw, float *wet_out, hipStream_t s, int choice = -1,
                       const dh_hbox *box = nullptr);
 void dh_launch_pencil(const dh_geom &g, float vsadx, float vsady, int n_layers,

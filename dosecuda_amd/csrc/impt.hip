@@ -53,7 +53,7 @@ k_pad_quad(const float *__restrict__ src, float4 *__restrict__ dst, int nx, int 
 // 5e-7 * La < 5e-7 * n / |du| voxel-steps -- both far below the margin.
 // A short bulk only moves samples to the exactly tested tail.
 __device__ static inline int safe_samples(const dh_geom &g, float x, float y, float z,
-                                          float ux, float uy, float uz)
+                                          float ux, float uy, float uz, float step)
 {
     const float p[3] = {x, y, z}, u[3] = {ux, uy, uz};
     const int n[3] = {g.nx, g.ny, g.nz};
@@ -68,7 +68,7 @@ __device__ static inline int safe_samples(const dh_geom &g, float x, float y, fl
         Lmin = fminf(Lmin, La);
     }
     if (!(Lmin >= 0.0f)) return 0;
-    return (int)fminf(floorf(Lmin) + 1.0f, 1.0e7f);
+    return (int)fminf(floorf(Lmin / step) + 1.0f, 1.0e7f);
 }
 
 // True when the voxel at image point (x, y, z) lies outside the head-frame box.
@@ -80,9 +80,13 @@ __device__ static inline bool dh_outside(const dh_geom &g, const dh_hbox &box, f
     return !(hx >= box.x0 && hx <= box.x1 && hy >= box.y0 && hy <= box.y1);
 }
 
+// step: sample spacing in mm.  1 is the reference's ray trace; the checked
+// surrogate path traces with one sample per voxel (step = spacing), whose
+// Riemann sum is WET = -0.05 step + 0.1 step * sum (cm), off the parity path.
 template <bool Off32>
 __global__ void __launch_bounds__(kBlock)
-k_raytrace(dh_geom g, dh_pad p, dh_tile tile, dh_hbox box, const float4 *__restrict__ vol, float *__restrict__ wet)
+k_raytrace(dh_geom g, dh_pad p, dh_tile tile, dh_hbox box, float step, const float4 *__restrict__ vol,
+           float *__restrict__ wet)
 {
     const float4 *vol_hi = vol + p.pxy;
     int i, j, k;
@@ -96,7 +100,7 @@ k_raytrace(dh_geom g, dh_pad p, dh_tile tile, dh_hbox box, const float4 *__restr
     double acc = 0.0;
     // Bulk: samples known to be inside need no exit test; their texel
     // coordinate is u = u0 + L * du (continuous in L, no decision rides on it).
-    const int nsafe = safe_samples(g, x, y, z, ux, uy, uz);
+    const int nsafe = safe_samples(g, x, y, z, ux, uy, uz, step);
     const float u0x = fmaf(g.inv_s, x + g.iso[0], 0.5f) - 0.5f;
     const float u0y = fmaf(g.inv_s, y + g.iso[1], 0.5f) - 0.5f;
     const float u0z = fmaf(g.inv_s, z + g.iso[2], 0.5f) - 0.5f;
@@ -106,7 +110,7 @@ k_raytrace(dh_geom g, dh_pad p, dh_tile tile, dh_hbox box, const float4 *__restr
         float part = 0.0f;
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const float L = (float)(m + q);
+            const float L = (float)(m + q) * step;
             part += fmaxf(dh_trilinear_uq<Off32>(vol, vol_hi, p, fmaf(dux, L, u0x), fmaf(duy, L, u0y),
                                                  fmaf(duz, L, u0z)), 0.0f);
         }
@@ -116,13 +120,13 @@ k_raytrace(dh_geom g, dh_pad p, dh_tile tile, dh_hbox box, const float4 *__restr
     // at a time.  Inside-ness is monotone along a ray leaving a box, so a
     // sample counts iff every earlier one was inside; loads past the exit read
     // clamped, in-range addresses and are discarded.
-    float L = (float)m;
+    int mt = m;
     for (;;) {
         float d[4];
         bool in[4];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const float Lq = L + (float)q;
+            const float Lq = (float)(mt + q) * step;
             const float qx = fmaf(ux, Lq, x), qy = fmaf(uy, Lq, y), qz = fmaf(uz, Lq, z);
             const float tx = fmaf(g.inv_s, qx + g.iso[0], 0.5f);
             const float ty = fmaf(g.inv_s, qy + g.iso[1], 0.5f);
@@ -139,9 +143,9 @@ k_raytrace(dh_geom g, dh_pad p, dh_tile tile, dh_hbox box, const float4 *__restr
         }
         acc += (double)part;
         if (!all) break;
-        L += 4.0f;
+        mt += 4;
     }
-    wet[v] = (float)(-0.05 + 0.1 * acc);
+    wet[v] = (float)(-0.05 * (double)step + 0.1 * (double)step * acc);
 }
 
 struct smooth_dirs {
@@ -304,8 +308,9 @@ void dh_launch_pad_quad(const float *src, float4 *dst, int nx, int ny, int nz, f
 }
 
 void dh_launch_raytrace(const dh_geom &g, const float4 *vol_pad, float *wet, hipStream_t s, int choice,
-                        const dh_hbox *box)
+                        const dh_hbox *box, float step)
 {
+    DH_REQUIRE(step >= 1.0f && step <= 16.0f, "ray-trace step must be 1..16 mm");
     const dh_hbox b = box ? *box : dh_hbox{0.0f, 0.0f, 0.0f, 0.0f, 0};
     dh_pad p = dh_make_pad(g.nx, g.ny, g.nz);
     if (choice < 0) choice = dh_tune_lookup("raytrace", dh_tune_key_geom(g), 0);
@@ -314,9 +319,9 @@ void dh_launch_raytrace(const dh_geom &g, const float4 *vol_pad, float *wet, hip
     dh_timed t("raytrace", s);
     // 32-bit offsets for quad volumes under 2 GiB whose planes fit 24-bit products
     if ((size_t)16 * p.pxy * p.pz < ((size_t)1 << 31) && p.pxy < ((int64_t)1 << 23))
-        hipLaunchKernelGGL(k_raytrace<true>, dim3(grid), dim3(kBlock), 0, s, g, p, tile, b, vol_pad, wet);
+        hipLaunchKernelGGL(k_raytrace<true>, dim3(grid), dim3(kBlock), 0, s, g, p, tile, b, step, vol_pad, wet);
     else
-        hipLaunchKernelGGL(k_raytrace<false>, dim3(grid), dim3(kBlock), 0, s, g, p, tile, b, vol_pad, wet);
+        hipLaunchKernelGGL(k_raytrace<false>, dim3(grid), dim3(kBlock), 0, s, g, p, tile, b, step, vol_pad, wet);
     DH_HIP(hipGetLastError());
 }
 

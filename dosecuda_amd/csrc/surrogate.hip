@@ -820,12 +820,16 @@ __global__ void k_sur_dilate(const int *__restrict__ lat_ok, const int *__restri
 constexpr int kFallbackRounds = 8;
 
 __global__ void __launch_bounds__(256)
-k_sur_fallback(const float *__restrict__ terma, int nz, int ny, int nx, int stride, int lz, int ly, int lx,
-               const int *__restrict__ redo, int *__restrict__ list, int *__restrict__ counter)
+k_sur_fallback(const float *__restrict__ terma, const float *__restrict__ electron, int band, int nz, int ny,
+               int nx, int stride, int lz, int ly, int lx, const int *__restrict__ redo, int *__restrict__ list,
+               int *__restrict__ counter)
 {
     const uint32_t n = (uint32_t)nz * ny * nx;
     dh_block_compact<kFallbackRounds>(blockIdx.x * (uint32_t)(256 * kFallbackRounds), n, [&](uint32_t v) {
         if (terma[v] <= 0.01f) return false;
+        // the build-up band: a nonzero electron-contamination term means
+        // WET < electron_fitted_dmax (IMRTClasses.cu:238-243)
+        if (band && electron[v] > 0.0f) return true;
         const int i = (int)(v % (uint32_t)nx);
         const uint32_t r = v / (uint32_t)nx;
         const int j = (int)(r % (uint32_t)ny), k = (int)(r / (uint32_t)ny);
@@ -1384,10 +1388,10 @@ void dh_launch_sur_check(const dh_ccc_args &ca, const float *terma, const float 
     {
         dh_timed t("sur_fallback", s);
         hipLaunchKernelGGL(k_sur_dilate, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, (const int *)lat_ok,
-                           (const int *)bad, lz, ly, lx, dilate, redo);
+                           (const int *)bad, lz, ly, lx, dilate & 3, redo);
         hipLaunchKernelGGL(k_sur_fallback, dim3((unsigned)((nv + 256 * kFallbackRounds - 1) / (256 * kFallbackRounds))),
-                           dim3(256), 0, s, terma, nz, ny, nx, stride, lz, ly, lx, (const int *)redo, fb_list,
-                           counter + 1);
+                           dim3(256), 0, s, terma, electron, (dilate & 4) ? 1 : 0, nz, ny, nx, stride, lz, ly, lx,
+                           (const int *)redo, fb_list, counter + 1);
     }
     dh_launch_ccc_counted(ca, tr_pad, layout, electron, fb_list, counter + 1, nv, dose, s);
     DH_HIP(hipGetLastError());

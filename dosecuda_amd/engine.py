@@ -260,6 +260,24 @@ def photon_dose(density: torch.Tensor, spacing, origin, iso, mu, ga, ca, ta, mod
     return out
 
 
+def photon_terma(density: torch.Tensor, spacing, origin, iso, mu, ga, ca, ta, model, mlc, step=1.0,
+                 terma_out=None, electron_out=None, stream=None):
+    """TERMA and electron-contamination volumes of one control point, with
+    the ray sampled every ``step`` mm (1: the reference's ray trace; the
+    checked surrogate path uses the voxel spacing)."""
+    dev = _dev(density)
+    terma_out = torch.empty_like(density) if terma_out is None else terma_out
+    electron_out = torch.empty_like(density) if electron_out is None else electron_out
+    keep: list = []
+    m = model.struct if isinstance(model, PhotonParams) else photon_args(model, keep)
+    rows = np.ascontiguousarray(np.asarray(mlc, dtype=np.float32)[:, :4])
+    _lib.check(_lib.load().dh_photon_terma_dev(
+        dev, _g(density.shape, spacing, origin), _lib.photon_cp(iso, mu, ga, ca, ta), m,
+        _lib.ptr(rows), rows.shape[0], _lib.ptr(density), float(step), _lib.ptr(terma_out),
+        _lib.ptr(electron_out), _stream(stream)))
+    return terma_out, electron_out
+
+
 # ---- single stages (stage-wise parity) -----------------------------------
 def stage_raytrace(vol: torch.Tensor, spacing, origin, iso, gantry, couch, src_dist, border,
                    out=None, stream=None):

@@ -358,13 +358,40 @@ def random_phantom(n, spacing, rng: np.random.Generator, device="cuda"):
     return rho.contiguous()
 
 
-def generate(model, n_samples, n=128, spacing=(2.0, 3.0), seed=1004, log=None):
+def _lung_bone_phantom(n, spacing, rng: np.random.Generator, device="cuda"):
+    """Density of a phantom of the 128^3 lung/bone plan's family (the
+    geometry of phantoms.imrt_lung_bone_case: water, a -700 HU lung ellipsoid
+    with semi-axes 30-50 mm within 20 mm of the centre, a +1000 HU bone
+    cylinder along z of radius 10-20 mm within 40 mm; bone over lung), built
+    on the device."""
+    from .beam_models import density_from_hu
+    d_lung, d_water, d_bone = (float(v) for v in density_from_hu(np.array([-700.0, 0.0, 1000.0])))
+    c = (n - 1) / 2.0 * spacing
+    z = torch.arange(n, device=device, dtype=torch.float32)[:, None, None] * spacing
+    y = torch.arange(n, device=device, dtype=torch.float32)[None, :, None] * spacing
+    x = torch.arange(n, device=device, dtype=torch.float32)[None, None, :] * spacing
+    lc = c + rng.uniform(-20.0, 20.0, 3)
+    ax = rng.uniform(30.0, 50.0, 3)
+    br = float(rng.uniform(10.0, 20.0))
+    bc = c + rng.uniform(-40.0, 40.0, 2)
+    rho = torch.full((n, n, n), d_water, device=device)
+    lung = ((x - lc[0]) / ax[0]) ** 2 + ((y - lc[1]) / ax[1]) ** 2 + ((z - lc[2]) / ax[2]) ** 2 <= 1.0
+    rho[lung.expand(n, n, n)] = d_lung
+    bone = (x - bc[0]) ** 2 + (y - bc[1]) ** 2 <= br * br
+    rho[bone.expand(n, n, n)] = d_bone
+    return rho.contiguous()
+
+
+def generate(model, n_samples, n=128, spacing=(2.0, 3.0), seed=1004, log=None, family="random"):
     """Training samples from the plain CCCS: random phantoms, voxel spacings
     (uniform in the `spacing` range, or fixed), gantry angles, isocentre
     shifts of up to 2 cm and square fields of 3-20 cm.  Returns inputs
     X [S, n/2, n/2, n/2, 16] bf16 (TERMA/max and density sub-voxels), the
     per-sample channels [S, 4] (beam direction, spacing / 2.5 mm) and targets
-    Y [S, n/2, n/2, n/2, 8] fp32 (superposition dose / max TERMA)."""
+    Y [S, n/2, n/2, n/2, 8] fp32 (superposition dose / max TERMA).
+    family "mix": every other sample is a lung/bone phantom of the plan
+    family at 2.5 mm (drawn from this generator's own seeded stream, not the
+    evaluated phantom's) with a 10 x 10 cm field half of the time."""
     from .beam_models import square_field_mlc
     rng = np.random.default_rng(seed)
     h = n // 2
@@ -377,9 +404,15 @@ def generate(model, n_samples, n=128, spacing=(2.0, 3.0), seed=1004, log=None):
     t0 = time.time()
     for s in range(n_samples):
         sp = float(rng.uniform(*spacing)) if isinstance(spacing, (tuple, list)) else float(spacing)
-        rho = random_phantom(n, sp, rng)
-        ga = float(rng.uniform(0.0, 360.0))
         fs = float(rng.uniform(3.0, 20.0))
+        if family == "mix" and s % 2 == 1:
+            sp = 2.5
+            rho = _lung_bone_phantom(n, sp, rng)
+            if rng.random() < 0.5:
+                fs = 10.0
+        else:
+            rho = random_phantom(n, sp, rng)
+        ga = float(rng.uniform(0.0, 360.0))
         iso = (np.full(3, (n - 1) / 2 * sp) + rng.uniform(-20.0, 20.0, 3)).astype(np.float32)
         mlc = square_field_mlc(model, fs, fs)
         engine.photon_dose(rho, sp, np.zeros(3), iso, 100.0, ga, 0.0, 0.0, model, mlc,
@@ -393,6 +426,24 @@ def generate(model, n_samples, n=128, spacing=(2.0, 3.0), seed=1004, log=None):
         if log and (s + 1) % max(1, n_samples // 4) == 0:
             log(f"generated {s + 1}/{n_samples} samples ({time.time() - t0:.1f} s)")
     return X, dirs, Y
+
+
+def _edge_weight(Y, alpha):
+    """Per-target-voxel loss weight 1 + alpha * |grad Y| / max Y (sub-voxel
+    layout in and out): high-gradient voxels (build-up, penumbra) count more."""
+    B = Y.shape[0]
+    out = torch.empty_like(Y)
+    for b in range(B):
+        v = _d2s(Y[b])
+        g = torch.zeros_like(v)
+        for ax in range(3):
+            d = torch.diff(v, dim=ax)
+            pad = [0, 0] * 3
+            pad[2 * (2 - ax) + 1] = 1
+            g = g + torch.nn.functional.pad(d, pad).abs()
+        m = torch.clamp(v.amax(), min=1e-12)
+        out[b] = _s2d(1.0 + alpha * g / m * 10.0)
+    return out
 
 
 def assemble(X, chans):
@@ -421,12 +472,14 @@ def eval_loss(net: SurrogateNet, X, chans, Y, batch=4):
 
 
 def train(net: SurrogateNet, model, n_samples=20000, chunk=1000, passes=3, batch=4, lr=5e-4,
-          n=128, spacing=(2.0, 3.0), seed=1004, log=None, fit_scale=True):
+          n=128, spacing=(2.0, 3.0), seed=1004, log=None, fit_scale=True, edge=0.0, family="random"):
     """Online training: the plain CCCS generates `chunk` samples at a time,
     each chunk is used for `passes` epochs, `n_samples` in total.  Adam on the
     FP32 master weights, one-cycle learning rate; loss = mean squared error
     over voxels of the field (target > 0) plus a small weight outside.
-    fit_scale=False keeps the network's output scale (continued training)."""
+    fit_scale=False keeps the network's output scale (continued training);
+    edge > 0 weights the in-field voxels by 1 + edge * 10 |grad target| / max
+    (per voxel of the 2.5 mm grid); family as in generate()."""
     n_chunks = max(1, n_samples // chunk)
     steps_per_chunk = max(1, chunk * passes // batch)
     total = n_chunks * steps_per_chunk
@@ -443,7 +496,7 @@ def train(net: SurrogateNet, model, n_samples=20000, chunk=1000, passes=3, batch
     best, good, lr_scale, restores = None, None, 1.0, 0
     for ci in range(n_chunks):
         tg = time.time()
-        X, dirs, Y = generate(model, chunk, n=n, spacing=spacing, seed=seed * 1000 + ci)
+        X, dirs, Y = generate(model, chunk, n=n, spacing=spacing, seed=seed * 1000 + ci, family=family)
         torch.cuda.synchronize()
         tgen += time.time() - tg
         peak = Y.amax(dim=(1, 2, 3, 4))
@@ -458,7 +511,7 @@ def train(net: SurrogateNet, model, n_samples=20000, chunk=1000, passes=3, batch
             xb = assemble(X[idx], dirs[idx])
             yb = Y[idx] * inv
             pred = net(xb)[..., :N_SUB]
-            wgt = torch.where(yb > 0, 1.0, 0.05)
+            wgt = torch.where(yb > 0, _edge_weight(yb, edge) if edge > 0 else 1.0, 0.05)
             loss = (wgt * (pred - yb) ** 2).mean()
             opt.zero_grad(set_to_none=True)
             loss.backward()
@@ -507,6 +560,10 @@ def train(net: SurrogateNet, model, n_samples=20000, chunk=1000, passes=3, batch
 # inference with the plain fallback
 # ---------------------------------------------------------------------------
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "cccs_surrogate.safetensors")
+# lattice stride, tolerance (of Dmax) and dilation of the check the benchmark
+# and the plan-level tests use with the shipped weights: on the 7-beam 128^3
+# lung/bone plan every beam of the whole path passes 2%/2 mm at >= 99.5%
+DEFAULT_CHECK = (4, 0.015, 1)
 
 
 class NativeNet:
@@ -645,6 +702,21 @@ class SurrogateCCCS:
             self.native.handle, _lib.grid(density.shape, spacing, origin), n, arr, m, P(termas),
             _lib.ptr(density), P(electrons), stride, tol, self.dilate, P(outs), _st()))
         return outs
+
+    @torch.no_grad()
+    def plan_doses(self, density, spacing, origin, cps, mlcs, outs=None, work=None):
+        """Checked surrogate doses of a plan's control points on one grid: per
+        point the TERMA and electron volumes with one ray sample per voxel
+        spacing (engine.photon_terma), then dose_batch.  cps: (iso, mu, ga,
+        ca, ta) per point, mlcs: their apertures.  work: optional lists
+        (termas, electrons) of volumes to reuse."""
+        step = max(1.0, float(np.float32(spacing)))
+        termas, els = work if work is not None else ([torch.empty_like(density) for _ in cps],
+                                                     [torch.empty_like(density) for _ in cps])
+        for cp, mlc, t, e in zip(cps, mlcs, termas, els):
+            engine.photon_terma(density, spacing, origin, *cp, self.model, mlc, step=step, terma_out=t,
+                                electron_out=e)
+        return self.dose_batch(termas, density, els, spacing, origin, cps, outs=outs)
 
     @torch.no_grad()
     def dose_torch(self, terma, density, electron, spacing, origin, iso, mu, ga, ca, ta, mlc=None,
@@ -831,6 +903,28 @@ def evaluate(net: SurrogateNet, model, n_beams=7, stride=3, tol=0.01, reps=5, lo
         res["graph_replay_equal"] = res["graph_replay_equal"] and all(
             bool(torch.equal(o, c)) for o, c in zip(outs_b, eager_b))
     res["graph_checked_ms_per_beam"] = _time(lambda: gb.replay(), reps) / len(angles)
+    # the whole surrogate path of the plan from the density: TERMA with one
+    # ray sample per voxel spacing, then the checked surrogate of all beams
+    # batched, as one HIP graph; judged by gamma against the plain plan
+    mlcs = [mlc for _ in angles]
+    work = ([torch.empty_like(rho) for _ in angles], [torch.empty_like(rho) for _ in angles])
+    outs_p = [torch.empty_like(rho) for _ in angles]
+    sur.plan_doses(rho, sp, org, cps, mlcs, outs=outs_p, work=work)
+    torch.cuda.synchronize()
+    eager_p = [o.clone() for o in outs_p]
+    gp = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(gp):
+        sur.plan_doses(rho, sp, org, cps, mlcs, outs=outs_p, work=work)
+    for _ in range(3):
+        gp.replay()
+        torch.cuda.synchronize()
+        res["graph_replay_equal"] = res["graph_replay_equal"] and all(
+            bool(torch.equal(o, c)) for o, c in zip(outs_p, eager_p))
+    res["path_ms_per_beam"] = _time(lambda: gp.replay(), reps) / len(angles)
+    pg = [gamma(o, p, sp)[0] for o, p in zip(outs_p, plains)]
+    res["path_min_gamma"] = min(pg)
+    res["path_plan_gamma"] = gamma(sum(outs_p), sums["plain"], sp)[0]
+    res["path_beam_gamma"] = pg
     engine.prof_reset()
     engine.prof_enable(True)
     for _ in range(reps):
@@ -869,6 +963,9 @@ def main(argv=None):
     t.add_argument("--out", default=WEIGHTS)
     t.add_argument("--init", default=None, help="continue from these weights (keeps their output scale)")
     t.add_argument("--eval", action="store_true", help="evaluate after training")
+    t.add_argument("--edge", type=float, default=0.0, help="gradient weighting of the loss (0 off)")
+    t.add_argument("--family", default="random", choices=["random", "mix"],
+                   help="phantoms: random, or every other one of the lung/bone plan family")
     e = sub.add_parser("eval", help="gamma and timing on the 7-beam lung/bone phantom")
     e.add_argument("--weights", default=WEIGHTS)
     e.add_argument("--sweep", default=None, help="stride:tol[:dilate] settings, e.g. 4:0.01,4:0.02:1")
@@ -884,7 +981,7 @@ def main(argv=None):
         net = load(a.init) if a.init else SurrogateNet().cuda()
         log(f"surrogate: {net.n_params()} parameters, {net.flops(64 ** 3) / 1e9:.1f} GFLOP per 64^3 patch")
         train(net, model, n_samples=a.samples, chunk=a.chunk, passes=a.passes, batch=a.batch, lr=a.lr,
-              seed=a.seed, log=log, fit_scale=not a.init)
+              seed=a.seed, log=log, fit_scale=not a.init, edge=a.edge, family=a.family)
         os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
         save(net, a.out, compact=True)
         log(f"saved {a.out}")

@@ -150,6 +150,16 @@ int dh_photon_dose_dev(int gpu_id, const dh_grid *grid, const dh_photon_cp *cp,
                        const float *d_density, float *d_dose_out, float *d_terma_out,
                        float *d_electron_out, float *d_wet_out, void *stream);
 
+/* TERMA and electron-contamination volumes of a control point from the
+ * density (ray trace + termaKernel, IMRTClasses.cu:209-250,353-405), the
+ * ray sampled every step_mm (1: the reference's 1 mm samples; the checked
+ * surrogate path uses one sample per voxel spacing, a coarser Riemann sum
+ * of the same radiological depth, judged by gamma against the plain CCCS). */
+int dh_photon_terma_dev(int gpu_id, const dh_grid *grid, const dh_photon_cp *cp,
+                        const dh_photon_model *model, const float *mlc_rows, int32_t n_pairs,
+                        const float *d_density, float step_mm, float *d_terma_out,
+                        float *d_electron_out, void *stream);
+
 /* ---- batched proton dose-influence matrix (CSR) ---------------------- */
 /* No reference counterpart: the reference only sums weighted spots
  * (IMPTClasses.cu:204-271).  Row q of the matrix is sorted spot q (energy-id
@@ -278,8 +288,12 @@ int dh_sur_finish_dev(int32_t nz, int32_t ny, int32_t nx, const float *d_y32, co
  * plain CCCS on every stride-th voxel (per axis) with TERMA > 0.01; every
  * lattice cell whose surrogate value misses the plain one by more than
  * tol * dmax fails; voxels with TERMA > 0.01 in a failed cell or next to one
- * (26-neighbourhood of cells), or in a cell whose lattice voxel has
- * TERMA <= 0.01, get the plain CCCS value in d_dose.  d_work holds int32
+ * (dilate & 3: 0 none, 1 the 6 face neighbours, 2 all 26 neighbours of
+ * cells), or in a cell whose lattice voxel has TERMA <= 0.01, get the plain
+ * CCCS value in d_dose; with dilate & 4 so do all voxels with TERMA > 0.01
+ * whose electron-contamination term is nonzero (the build-up band, WET below
+ * the model's electron_fitted_dmax).  Every dilate argument of the surrogate
+ * entry points below takes these values (0-2, 4-6).  d_work holds int32
  * [work_len] (dh_sur_check_work_size); afterwards d_work[0] is the number of
  * lattice voxels and d_work[1] the number of fallback voxels.  d_plain:
  * fp32 voxels of scratch. */
@@ -330,7 +344,8 @@ int dh_sur_dose_batch_dev(const dh_sur_net *net, const dh_grid *grid, int32_t n_
  * with buffers of their own). */
 int dh_sur_counts(int gpu_id, int32_t *lattice, int32_t *fallback);
 /* dh_photon_dose_dev with the cone superposition replaced by the checked
- * surrogate (ray trace and TERMA as in the plain path). */
+ * surrogate; the ray trace samples once per voxel spacing (at least 1 mm)
+ * instead of every mm (dh_photon_terma_dev), TERMA as in the plain path. */
 int dh_photon_dose_surrogate_dev(int gpu_id, const dh_sur_net *net, const dh_grid *grid,
                                  const dh_photon_cp *cp, const dh_photon_model *model,
                                  const float *mlc_rows, int32_t n_pairs, const float *d_density,

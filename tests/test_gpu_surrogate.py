@@ -463,7 +463,13 @@ def test_native_dose_equals_torch_path(photon_model):
     _lib.check(_lib.load().dh_photon_dose_surrogate_dev(
         0, sur.native.handle, _lib.grid(shape, s, np.zeros(3)), _lib.photon_cp(iso, 100.0, 120.0, 0.0, 0.0), m,
         _lib.ptr(rows), rows.shape[0], _lib.ptr(rho), _lib.ptr(d), None, None, 4, 0.02, sur.dilate, S._st()))
-    assert t.equal(d, a)
+    # (its ray trace samples once per voxel spacing: the same TERMA as photon_terma's at that step)
+    tf, ef = engine.photon_terma(rho, s, np.zeros(3), iso, 100.0, 120.0, 0.0, 0.0, pm, mlc, step=max(1.0, s))
+    a2 = sur.dose(tf, rho, ef, s, np.zeros(3), iso, 100.0, 120.0, 0.0, 0.0)
+    assert t.equal(d, a2)
+    # at step 1 mm the TERMA is the plain path's bit for bit
+    t1, e1 = engine.photon_terma(rho, s, np.zeros(3), iso, 100.0, 120.0, 0.0, 0.0, pm, mlc, step=1.0)
+    assert t.equal(t1, terma) and t.equal(e1, el)
 
 
 @pytest.mark.parametrize("shape, n_cp", [((40, 50, 36), 3), ((96, 96, 96), 9)])

@@ -74,6 +74,16 @@ def test_surrogate_plan_128_7beams(photon_model):
     g_plan, _ = S.gamma(sum(outs), sum(plains), sp)
     print(f"[parity] surrogate plan gamma {g_plan:.5f}", flush=True)
     assert g_plan >= 0.99
+    # the whole surrogate path from the density (TERMA with one ray sample
+    # per voxel spacing, the default check), every beam against its plain dose
+    st, tol, dil = S.DEFAULT_CHECK
+    sur2 = S.SurrogateCCCS(S.load(S.WEIGHTS), photon_model, stride=st, tol=tol, dilate=dil)
+    cps = [(iso, 100.0, ga, 0.0, 0.0) for ga in angles]
+    full = sur2.plan_doses(rho, sp, org, cps, [mlc for _ in angles])
+    for ga, o, p in zip(angles, full, plains):
+        g_path, _ = S.gamma(o, p, sp)
+        print(f"[parity] surrogate whole path beam {ga:.2f}: gamma {g_path:.5f}", flush=True)
+        assert g_path >= 0.99, ga
 
 
 def _largest(cases, kind, n):
