@@ -712,7 +712,8 @@ def main():
         cases = phantoms.mixed_cases(pm, xm, seed=1005, n_cases=a.mixed_cases)
         nat = S.NativeNet(S.load(S.WEIGHTS)) if os.path.exists(S.WEIGHTS) else None
         log(f"mixed: {len(cases)} cases, surrogate {'on' if nat else 'off (no weights)'}")
-        W = mixed.MixedWorkload(cases, pm, xm, surrogate=nat, streams=4, stride=3, tol=0.01, dilate=1, log=log)
+        st, tol, dil = S.DEFAULT_CHECK
+        W = mixed.MixedWorkload(cases, pm, xm, surrogate=nat, streams=4, stride=st, tol=tol, dilate=dil, log=log)
         cnt = W.counts()
         log(f"mixed: set up in {W.setup_s:.1f} s: {cnt}")
         t0 = time.time()

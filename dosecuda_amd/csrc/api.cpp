@@ -357,14 +357,17 @@ static void photon_device(int dev, const dh_grid *grid, const dh_photon_cp *cp,
     // position only (not on the collimator or the MLC): a caller that knows
     // the previous control point on this workspace had the same ones may ask
     // to reuse it; it is reused only if this workspace still holds it.
-    // The checked surrogate path samples the ray once per voxel spacing (at
-    // least 1 mm) instead of every mm: its TERMA is an input of the network
-    // and of the plain check alike, and the result is judged by gamma against
-    // the plain CCCS.  The plain path keeps the reference's 1 mm samples.
-    const float step = net ? std::max(1.0f, ca.g.s) : 1.0f;
+    // The checked surrogate path takes its radiological depth from fan-line
+    // integrals on a beam's-eye-view grid (step 0 here), or where that grid
+    // does not apply from a ray trace with one sample per voxel spacing: its
+    // TERMA is an input of the network and of the plain check alike, and the
+    // result is judged by gamma against the plain CCCS.  The plain path keeps
+    // the reference's ray trace with 1 mm samples.
+    float step = net ? 0.0f : 1.0f;
     if (!(reuse_wet && !d_wet && memo.vpad == vpad && memo.wet == wet && memo.step == step)) {
         dh_launch_pad_quad(d_density, vpad, ca.g.nx, ca.g.ny, ca.g.nz, DH_AIR_DENSITY, s);  // IMRTClasses.cu:394
-        dh_launch_raytrace(ca.g, vpad, wet, s, -1, nullptr, step);
+        if (step == 0.0f && !dh_launch_bev_wet(dev, ca.g, vpad, wet, s)) step = std::max(1.0f, ca.g.s);
+        if (step > 0.0f) dh_launch_raytrace(ca.g, vpad, wet, s, -1, nullptr, step);
     }
     memo.vpad = d_wet ? nullptr : vpad;
     memo.wet = d_wet ? nullptr : wet;
@@ -388,14 +391,16 @@ extern "C" int dh_photon_terma_dev(int gpu_id, const dh_grid *grid, const dh_pho
         dh_device_enter(gpu_id);
         const size_t n = nvox_of(grid);
         DH_REQUIRE(cp && d_density && d_terma_out && d_electron_out, "null argument");
-        DH_REQUIRE(step_mm >= 1.0f && step_mm <= 16.0f, "ray-trace step must be 1..16 mm");
+        DH_REQUIRE(step_mm == 0.0f || (step_mm >= 1.0f && step_mm <= 16.0f),
+                   "ray-trace step must be 0 (beam's-eye-view grid) or 1..16 mm");
         hipStream_t s = (hipStream_t)stream;
         dh_terma_args ta = dh_make_terma_args(grid, cp, model, mlc_rows, n_pairs);
         const dh_pad p = dh_make_pad(ta.g.nx, ta.g.ny, ta.g.nz);
         float4 *vpad = (float4 *)dh_scratch(gpu_id, DH_SLOT_VOL_PAD, sizeof(float4) * (size_t)p.pxy * p.pz);
         float *wet = (float *)dh_scratch(gpu_id, DH_SLOT_WET_RAW, sizeof(float) * n);
         dh_launch_pad_quad(d_density, vpad, ta.g.nx, ta.g.ny, ta.g.nz, DH_AIR_DENSITY, s);  // IMRTClasses.cu:394
-        dh_launch_raytrace(ta.g, vpad, wet, s, -1, nullptr, step_mm);
+        if (step_mm == 0.0f && !dh_launch_bev_wet(gpu_id, ta.g, vpad, wet, s)) step_mm = std::max(1.0f, ta.g.s);
+        if (step_mm > 0.0f) dh_launch_raytrace(ta.g, vpad, wet, s, -1, nullptr, step_mm);
         dh_launch_terma(ta, wet, d_terma_out, d_electron_out, s);
         g_wet_memo[gpu_id & 63][dh_get_workspace()] = wet_memo{};   // the padded volume was rewritten
     });

@@ -84,6 +84,7 @@ enum dh_slot {
     DH_SLOT_NN_WORK2,      // second check set (batched points on the side stream)
     DH_SLOT_NN_PLAIN2,
     DH_SLOT_TR_PAD2,
+    DH_SLOT_BEV,           // beam's-eye-view depth table of the checked surrogate path
     DH_SLOT_COUNT_
 };
 
@@ -184,6 +185,11 @@ void dh_launch_smooth(const dh_geom &g, const float *wet_raw, float *wet_out, hi
 void dh_launch_pencil(const dh_geom &g, float vsadx, float vsady, int n_layers,
                       const dh_layer_k *d_layers, const float4 *d_luts, const float *d_spots,
                       bool fold, const float *wet, float *dose, hipStream_t s, int choice = -1);
+// Radiological depth (cm) of every voxel from fan-line integrals on a
+// beam's-eye-view grid (one density sample per line step of one voxel
+// spacing, trapezoid rule, trilinear lookup); false (nothing launched) when
+// the source is not well outside the volume or the table would be too large.
+bool dh_launch_bev_wet(int dev, const dh_geom &g, const float4 *vol_pad, float *wet, hipStream_t s);
 void dh_launch_terma(const dh_terma_args &a, const float *wet, float *terma, float *electron,
                      hipStream_t s);
 // Sampling layout of the cone superposition's {TERMA, density} volume:

@@ -9,6 +9,9 @@
 // workgroup), so the cone kernel's waves are full of working lanes.
 #include "dh_runtime.h"
 
+#include <algorithm>
+#include <cmath>
+
 namespace {
 
 constexpr int kBlock = 256;
@@ -465,6 +468,140 @@ k_ccc_finish(float mu_cal, float mu, const float *__restrict__ electron, const i
     }
 }
 
+// ---- radiological depth on a beam's-eye-view grid (checked surrogate path) --
+// Fan lines from the source through an a x b lattice on the isocentre plane
+// (the plane through the isocentre normal to the central axis); along each
+// line the density is sampled every h mm of distance t from the source and
+// integrated with the trapezoid rule (zero outside the volume), so that
+// table(a, b, k) is the line integral from the source to distance t0 + k h.
+// The table is stored in 8 (a) x 4 (k) tiles of 128 bytes, tile rows along a
+// for each (b, k / 4): voxels next to each other in the image map to nearby
+// (a, k) whatever the beam angle, so a wave's lookups touch few cache lines.
+// A voxel's depth is that table interpolated trilinearly at its own (a, b, t):
+// one sample per line step instead of one per voxel per step, off the parity
+// path (the result is judged by gamma against the plain CCCS).
+struct bev_grid {
+    float e1[3], e2[3], c[3];    // lateral unit vectors and central axis (image frame)
+    float sad;                   // |source - isocentre|
+    float a0, b0, da;            // lattice origin and pitch on the isocentre plane (mm)
+    float t0, h;                 // first distance from the source and step (mm)
+    int na, nb, nt;
+    int nat, ntt;                // tiles along a (8 lines) and k (4 steps)
+};
+
+__device__ static inline size_t bev_index(const bev_grid &b, int ia, int ib, int ik)
+{
+    return (((size_t)ib * b.ntt + (ik >> 2)) * b.nat + (ia >> 3)) * 32 + (ik & 3) * 8 + (ia & 7);
+}
+
+// A workgroup owns 64 consecutive lines (one per lane); its kBevSeg waves
+// take consecutive segments of kBevLen steps of those lines.  Each wave keeps
+// its segment's running trapezoid sums in registers, the segment totals give
+// every segment its offset through LDS, and lines longer than kBevSeg
+// segments carry their integral into the next round of segments.
+constexpr int kBevSeg = 8, kBevLen = 32;
+
+template <bool Off32>
+__global__ void __launch_bounds__(64 * kBevSeg)
+k_bev_prefix(dh_geom g, dh_pad p, bev_grid b, const float4 *__restrict__ vol, float *__restrict__ table)
+{
+    __shared__ float s_tot[kBevSeg][64];
+    const int nl = b.na * b.nb;
+    const int lane = threadIdx.x & 63, seg = threadIdx.x >> 6;
+    const int l = blockIdx.x * 64 + lane;
+    const bool line = l < nl;
+    const int lc = line ? l : nl - 1;
+    const int ia = lc % b.na, ib = lc / b.na;
+    const float4 *vol_hi = vol + p.pxy;
+    const float pa = fmaf((float)ia, b.da, b.a0), pb = fmaf((float)ib, b.da, b.b0);
+    float dx = fmaf(pa, b.e1[0], pb * b.e2[0]) - g.src[0];
+    float dy = fmaf(pa, b.e1[1], pb * b.e2[1]) - g.src[1];
+    float dz = fmaf(pa, b.e1[2], pb * b.e2[2]) - g.src[2];
+    const float inv = rsqrtf(fmaf(dx, dx, fmaf(dy, dy, dz * dz)));
+    dx *= inv; dy *= inv; dz *= inv;
+    // texel coordinate of the point at distance t: T0 + t dT
+    const float T0x = fmaf(g.inv_s, g.src[0] + g.iso[0], 0.5f), dTx = g.inv_s * dx;
+    const float T0y = fmaf(g.inv_s, g.src[1] + g.iso[1], 0.5f), dTy = g.inv_s * dy;
+    const float T0z = fmaf(g.inv_s, g.src[2] + g.iso[2], 0.5f), dTz = g.inv_s * dz;
+    const float fnx = (float)g.nx, fny = (float)g.ny, fnz = (float)g.nz;
+    auto sample = [&](int k) -> float {
+        const float t = fmaf((float)k, b.h, b.t0);
+        const float tx = fmaf(dTx, t, T0x), ty = fmaf(dTy, t, T0y), tz = fmaf(dTz, t, T0z);
+        float rho = 0.0f;
+        if (k >= 0 && k < b.nt && tx >= 0.0f && tx < fnx && ty >= 0.0f && ty < fny && tz >= 0.0f && tz < fnz)
+            rho = fmaxf(dh_trilinear_q<Off32>(vol, vol_hi, p, g.nx, g.ny, g.nz, tx, ty, tz), 0.0f);
+        return rho;
+    };
+    const float hh = 0.5f * b.h;
+    float carry = 0.0f;
+    for (int base = 0; base < b.nt; base += kBevSeg * kBevLen) {
+        // step k adds the trapezoid over (k - 1, k)
+        const int k0 = base + seg * kBevLen;
+        float run[kBevLen];
+        float prev = sample(k0 - 1), acc = 0.0f;
+#pragma unroll
+        for (int q = 0; q < kBevLen; q++) {
+            const float rho = sample(k0 + q);
+            acc = fmaf(hh, prev + rho, acc);
+            prev = rho;
+            run[q] = acc;
+        }
+        s_tot[seg][lane] = acc;
+        __syncthreads();
+        float off = carry, all = carry;
+#pragma unroll
+        for (int q = 0; q < kBevSeg; q++) {
+            const float v = s_tot[q][lane];
+            off += q < seg ? v : 0.0f;
+            all += v;
+        }
+        __syncthreads();
+        carry = all;
+        if (line) {
+#pragma unroll
+            for (int q = 0; q < kBevLen; q++)
+                if (k0 + q < b.nt) table[bev_index(b, ia, ib, k0 + q)] = off + run[q];
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_bev_lookup(dh_geom g, bev_grid b, const float *__restrict__ table, float *__restrict__ wet)
+{
+    const uint32_t nvox = (uint32_t)g.nx * g.ny * g.nz;
+    const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+    if (v >= nvox) return;
+    const int i = (int)(v % (uint32_t)g.nx);
+    const uint32_t r = v / (uint32_t)g.nx;
+    float x, y, z;
+    dh_vox_xyz(g, i, (int)(r % (uint32_t)g.ny), (int)(r / (uint32_t)g.ny), x, y, z);
+    const float wx = x - g.src[0], wy = y - g.src[1], wz = z - g.src[2];
+    const float wc = fmaf(wx, b.c[0], fmaf(wy, b.c[1], wz * b.c[2]));
+    const float lam = b.sad / wc;     // the host checked wc > 0 for the whole volume
+    const float a = lam * fmaf(wx, b.e1[0], fmaf(wy, b.e1[1], wz * b.e1[2]));
+    const float bb = lam * fmaf(wx, b.e2[0], fmaf(wy, b.e2[1], wz * b.e2[2]));
+    const float t = sqrtf(fmaf(wx, wx, fmaf(wy, wy, wz * wz)));
+    // lattice coordinates, clamped so that both corners of each axis exist
+    const float fa = fminf(fmaxf((a - b.a0) / b.da, 0.0f), (float)b.na - 1.001f);
+    const float fb = fminf(fmaxf((bb - b.b0) / b.da, 0.0f), (float)b.nb - 1.001f);
+    const float fk = fminf(fmaxf((t - b.t0) / b.h, 0.0f), (float)b.nt - 1.001f);
+    const int ia = (int)fa, ib = (int)fb, ik = (int)fk;
+    const float ua = fa - (float)ia, ub = fb - (float)ib, uk = fk - (float)ik;
+    float q[2][2][2];                // [k][b][a] corners
+#pragma unroll
+    for (int dk = 0; dk < 2; dk++)
+#pragma unroll
+        for (int db = 0; db < 2; db++)
+#pragma unroll
+            for (int da = 0; da < 2; da++) q[dk][db][da] = table[bev_index(b, ia + da, ib + db, ik + dk)];
+    const float c00 = fmaf(ua, q[0][0][1] - q[0][0][0], q[0][0][0]);
+    const float c10 = fmaf(ua, q[0][1][1] - q[0][1][0], q[0][1][0]);
+    const float c01 = fmaf(ua, q[1][0][1] - q[1][0][0], q[1][0][0]);
+    const float c11 = fmaf(ua, q[1][1][1] - q[1][1][0], q[1][1][0]);
+    const float d0 = fmaf(ub, c10 - c00, c00), d1 = fmaf(ub, c11 - c01, c01);
+    wet[v] = 0.1f * fmaf(uk, d1 - d0, d0);      // mm of unit density -> cm
+}
+
 }  // namespace
 
 void dh_launch_terma(const dh_terma_args &a, const float *wet, float *terma, float *electron,
@@ -474,6 +611,85 @@ void dh_launch_terma(const dh_terma_args &a, const float *wet, float *terma, flo
     dh_timed t("terma", s);
     hipLaunchKernelGGL(k_terma, dim3(grid_for(n)), dim3(kBlock), 0, s, a, wet, terma, electron);
     DH_HIP(hipGetLastError());
+}
+
+bool dh_launch_bev_wet(int dev, const dh_geom &g, const float4 *vol_pad, float *wet, hipStream_t s)
+{
+    const double S[3] = {g.src[0], g.src[1], g.src[2]};
+    const double sad = std::sqrt(S[0] * S[0] + S[1] * S[1] + S[2] * S[2]);
+    if (!(sad > 1.0)) return false;
+    const double c[3] = {-S[0] / sad, -S[1] / sad, -S[2] / sad};
+    // lateral basis: c x (the axis least aligned with c), then c x e1
+    int ax = 0;
+    for (int q = 1; q < 3; q++)
+        if (std::fabs(c[q]) < std::fabs(c[ax])) ax = q;
+    double ref[3] = {0.0, 0.0, 0.0};
+    ref[ax] = 1.0;
+    double e1[3] = {c[1] * ref[2] - c[2] * ref[1], c[2] * ref[0] - c[0] * ref[2], c[0] * ref[1] - c[1] * ref[0]};
+    const double n1 = std::sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
+    for (double &e : e1) e /= n1;
+    const double e2[3] = {c[1] * e1[2] - c[2] * e1[1], c[2] * e1[0] - c[0] * e1[2], c[0] * e1[1] - c[1] * e1[0]};
+    // the volume's box (voxel centres +- half a voxel, relative to the isocentre)
+    const int n[3] = {g.nx, g.ny, g.nz};
+    double lo[3], hi[3];
+    for (int q = 0; q < 3; q++) {
+        lo[q] = -0.5 * g.s - g.iso[q];
+        hi[q] = ((double)n[q] - 0.5) * g.s - g.iso[q];
+    }
+    double amin = 1e300, amax = -1e300, bmin = 1e300, bmax = -1e300, t1 = 0.0;
+    for (int corner = 0; corner < 8; corner++) {
+        double w[3];
+        for (int q = 0; q < 3; q++) w[q] = ((corner >> q) & 1 ? hi[q] : lo[q]) - S[q];
+        const double wc = w[0] * c[0] + w[1] * c[1] + w[2] * c[2];
+        // every point of the volume must lie well in front of the source
+        if (!(wc > 0.05 * sad)) return false;
+        const double lam = sad / wc;
+        const double a = lam * (w[0] * e1[0] + w[1] * e1[1] + w[2] * e1[2]);
+        const double b = lam * (w[0] * e2[0] + w[1] * e2[1] + w[2] * e2[2]);
+        amin = std::min(amin, a); amax = std::max(amax, a);
+        bmin = std::min(bmin, b); bmax = std::max(bmax, b);
+        t1 = std::max(t1, std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]));
+    }
+    double d2 = 0.0;                         // distance from the source to the box
+    for (int q = 0; q < 3; q++) {
+        const double cq = std::min(std::max(S[q], lo[q]), hi[q]);
+        d2 += (S[q] - cq) * (S[q] - cq);
+    }
+    bev_grid b;
+    b.h = g.s;
+    b.da = (float)(g.s * sad / t1);          // line pitch <= one voxel everywhere in the volume
+    b.a0 = (float)(amin - 2.0 * b.da);
+    b.b0 = (float)(bmin - 2.0 * b.da);
+    b.na = (int)std::ceil((amax - amin) / b.da) + 5;
+    b.nb = (int)std::ceil((bmax - bmin) / b.da) + 5;
+    b.t0 = (float)std::max(0.0, std::sqrt(d2) - 2.0 * b.h);
+    b.nt = (int)std::ceil((t1 - b.t0) / b.h) + 3;
+    b.nat = (b.na + 7) / 8;
+    b.ntt = (b.nt + 3) / 4;
+    const double cells = (double)b.nat * 8 * b.nb * b.ntt * 4;
+    if (!(cells < (double)(1u << 28)) || (double)b.na * b.nb > (double)(1u << 30)) return false;
+    for (int q = 0; q < 3; q++) {
+        b.e1[q] = (float)e1[q];
+        b.e2[q] = (float)e2[q];
+        b.c[q] = (float)c[q];
+    }
+    b.sad = (float)sad;
+    float *table = (float *)dh_scratch(dev, DH_SLOT_BEV, sizeof(float) * (size_t)cells);
+    const dh_pad p = dh_make_pad(g.nx, g.ny, g.nz);
+    const size_t nl = (size_t)b.na * b.nb, nvox = (size_t)g.nx * g.ny * g.nz;
+    {
+        dh_timed t("bev_prefix", s);
+        const unsigned lgrid = (unsigned)((nl + 63) / 64);
+        if ((size_t)16 * p.pxy * p.pz < ((size_t)1 << 31) && p.pxy < ((int64_t)1 << 23))
+            hipLaunchKernelGGL(k_bev_prefix<true>, dim3(lgrid), dim3(64 * kBevSeg), 0, s, g, p, b, vol_pad, table);
+        else
+            hipLaunchKernelGGL(k_bev_prefix<false>, dim3(lgrid), dim3(64 * kBevSeg), 0, s, g, p, b, vol_pad, table);
+        DH_HIP(hipGetLastError());
+    }
+    dh_timed t("bev_lookup", s);
+    hipLaunchKernelGGL(k_bev_lookup, dim3(grid_for(nvox)), dim3(kBlock), 0, s, g, b, table, wet);
+    DH_HIP(hipGetLastError());
+    return true;
 }
 
 int dh_ccc_layout(const dh_ccc_args &a, int choice)

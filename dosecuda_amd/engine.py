@@ -263,8 +263,9 @@ def photon_dose(density: torch.Tensor, spacing, origin, iso, mu, ga, ca, ta, mod
 def photon_terma(density: torch.Tensor, spacing, origin, iso, mu, ga, ca, ta, model, mlc, step=1.0,
                  terma_out=None, electron_out=None, stream=None):
     """TERMA and electron-contamination volumes of one control point, with
-    the ray sampled every ``step`` mm (1: the reference's ray trace; the
-    checked surrogate path uses the voxel spacing)."""
+    the radiological depth from a ray trace sampled every ``step`` mm (1: the
+    reference's ray trace), or with ``step`` 0 from fan-line integrals on a
+    beam's-eye-view grid (the checked surrogate path's depth)."""
     dev = _dev(density)
     terma_out = torch.empty_like(density) if terma_out is None else terma_out
     electron_out = torch.empty_like(density) if electron_out is None else electron_out

@@ -704,13 +704,13 @@ class SurrogateCCCS:
         return outs
 
     @torch.no_grad()
-    def plan_doses(self, density, spacing, origin, cps, mlcs, outs=None, work=None):
+    def plan_doses(self, density, spacing, origin, cps, mlcs, outs=None, work=None, step=0.0):
         """Checked surrogate doses of a plan's control points on one grid: per
-        point the TERMA and electron volumes with one ray sample per voxel
-        spacing (engine.photon_terma), then dose_batch.  cps: (iso, mu, ga,
-        ca, ta) per point, mlcs: their apertures.  work: optional lists
-        (termas, electrons) of volumes to reuse."""
-        step = max(1.0, float(np.float32(spacing)))
+        point the TERMA and electron volumes (engine.photon_terma; step 0: the
+        depth from the beam's-eye-view grid, > 0: a ray trace sampled every
+        step mm), then dose_batch.  cps: (iso, mu, ga, ca, ta) per point,
+        mlcs: their apertures.  work: optional lists (termas, electrons) of
+        volumes to reuse."""
         termas, els = work if work is not None else ([torch.empty_like(density) for _ in cps],
                                                      [torch.empty_like(density) for _ in cps])
         for cp, mlc, t, e in zip(cps, mlcs, termas, els):
@@ -925,6 +925,26 @@ def evaluate(net: SurrogateNet, model, n_beams=7, stride=3, tol=0.01, reps=5, lo
     res["path_min_gamma"] = min(pg)
     res["path_plan_gamma"] = gamma(sum(outs_p), sums["plain"], sp)[0]
     res["path_beam_gamma"] = pg
+    # the same path with the depth from a ray trace sampled once per voxel spacing
+    step = max(1.0, float(np.float32(sp)))
+    gs = torch.cuda.CUDAGraph()
+    sur.plan_doses(rho, sp, org, cps, mlcs, outs=outs_p, work=work, step=step)
+    torch.cuda.synchronize()
+    with torch.cuda.graph(gs):
+        sur.plan_doses(rho, sp, org, cps, mlcs, outs=outs_p, work=work, step=step)
+    gs.replay()
+    torch.cuda.synchronize()
+    res["path_raytrace_ms_per_beam"] = _time(lambda: gs.replay(), reps) / len(angles)
+    res["path_raytrace_min_gamma"] = min(gamma(o, p, sp)[0] for o, p in zip(outs_p, plains))
+    engine.prof_reset()
+    engine.prof_enable(True)
+    for _ in range(reps):
+        sur.plan_doses(rho, sp, org, cps, mlcs, outs=outs_p, work=work)
+    torch.cuda.synchronize()
+    engine.prof_enable(False)
+    res["path_stage_ms_per_beam"] = {k: engine.prof_get(k)[0] / reps / len(angles)
+                                     for k in ("pad", "bev_prefix", "bev_lookup", "raytrace", "terma", "nn_conv3d", "ccc_list",
+                                               "sur_pack", "sur_finish", "sur_lattice", "sur_fallback")}
     engine.prof_reset()
     engine.prof_enable(True)
     for _ in range(reps):

@@ -151,10 +151,14 @@ int dh_photon_dose_dev(int gpu_id, const dh_grid *grid, const dh_photon_cp *cp,
                        float *d_electron_out, float *d_wet_out, void *stream);
 
 /* TERMA and electron-contamination volumes of a control point from the
- * density (ray trace + termaKernel, IMRTClasses.cu:209-250,353-405), the
- * ray sampled every step_mm (1: the reference's 1 mm samples; the checked
- * surrogate path uses one sample per voxel spacing, a coarser Riemann sum
- * of the same radiological depth, judged by gamma against the plain CCCS). */
+ * density (ray trace + termaKernel, IMRTClasses.cu:209-250,353-405).  The
+ * radiological depth: step_mm 1..16, a ray trace sampled every step_mm (1:
+ * the reference's 1 mm samples); step_mm 0, the checked surrogate path's
+ * depth, fan-line integrals from the source on a beam's-eye-view grid (one
+ * density sample per line per voxel spacing, trapezoid rule) interpolated
+ * at each voxel, judged by gamma against the plain CCCS (where the source is
+ * not well outside the volume it falls back to the ray trace at one sample
+ * per voxel spacing). */
 int dh_photon_terma_dev(int gpu_id, const dh_grid *grid, const dh_photon_cp *cp,
                         const dh_photon_model *model, const float *mlc_rows, int32_t n_pairs,
                         const float *d_density, float step_mm, float *d_terma_out,
@@ -344,8 +348,8 @@ int dh_sur_dose_batch_dev(const dh_sur_net *net, const dh_grid *grid, int32_t n_
  * with buffers of their own). */
 int dh_sur_counts(int gpu_id, int32_t *lattice, int32_t *fallback);
 /* dh_photon_dose_dev with the cone superposition replaced by the checked
- * surrogate; the ray trace samples once per voxel spacing (at least 1 mm)
- * instead of every mm (dh_photon_terma_dev), TERMA as in the plain path. */
+ * surrogate; the radiological depth comes from the beam's-eye-view grid
+ * (dh_photon_terma_dev at step_mm 0), TERMA as in the plain path. */
 int dh_photon_dose_surrogate_dev(int gpu_id, const dh_sur_net *net, const dh_grid *grid,
                                  const dh_photon_cp *cp, const dh_photon_model *model,
                                  const float *mlc_rows, int32_t n_pairs, const float *d_density,

@@ -100,7 +100,8 @@ def test_mixed_largest_cases_graph_and_oracle(proton_model, photon_model):
     sub = [cases[i] for i in pick]
     print(f"[case] mixed largest: {[(i, c.kind, c.shape) for i, c in zip(pick, sub)]}", flush=True)
     nat = S.NativeNet(S.load(S.WEIGHTS)) if os.path.exists(S.WEIGHTS) else None
-    W = mixed.MixedWorkload(sub, pm, xm, surrogate=nat, streams=4, stride=3, tol=0.01, dilate=1)
+    st, tol, dil = S.DEFAULT_CHECK
+    W = mixed.MixedWorkload(sub, pm, xm, surrogate=nat, streams=4, stride=st, tol=tol, dilate=dil)
     W.run(graph=False)
     t.cuda.synchronize()
     eager = [(it["dose"].clone() if it["kind"] == "imrt" else [f["val"][:f["nnz"]].clone() for f in it["fields"]])
