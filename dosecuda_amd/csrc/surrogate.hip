@@ -55,7 +55,8 @@ __global__ void __launch_bounds__(64 * KS)
 k_conv3d(conv_args a)
 {
     extern __shared__ float red[];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    // the wave index is uniform: keep it (and the K schedule below) scalar
+    const int lane = threadIdx.x & 63, wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nblk_co = a.cout_pad / (32 * NCO);
     const int cob = (int)(blockIdx.x % (unsigned)nblk_co);
     const int64_t vb = blockIdx.x / (unsigned)nblk_co;
@@ -83,24 +84,51 @@ k_conv3d(conv_args a)
 #pragma unroll
         for (int r = 0; r < 16; r++) acc[j][r] = 0.0f;
 
-    // fragments of K step ki: A (this lane's voxel, 8 channels) and NCO B
-    auto fetch = [&](int ki, uint4 &av, uint4 *bv) {
-        const int tp = ki / nsl;
-        const int c0 = (ki - tp * nsl) * 16;
-        const int dz = tp / kk - rad, dy = (tp / a.ksize) % a.ksize - rad, dx = tp % a.ksize - rad;
-        const int zz = z + dz, yy = y + dy, xx = x + dx;
+    // K step ki is (tap tp, 16-channel slice sl) with ki = tp * nsl + sl.
+    // The fetches below run through k0, k0 + 1, ..., k1 - 1 in order, so the
+    // tap offsets, slice and weight offset advance incrementally (scalar
+    // state) instead of being divided out of ki at every step.
+    // (element offsets fit 32 bits: make_conv bounds every tensor by 2^31)
+    int f_sl, f_dx, f_dy, f_dz;
+    uint32_t f_wtap;
+    {
+        const int tp = k0 / nsl;
+        f_sl = k0 - tp * nsl;
+        f_dz = tp / kk - rad;
+        f_dy = (tp / a.ksize) % a.ksize - rad;
+        f_dx = tp % a.ksize - rad;
+        f_wtap = (uint32_t)tp * (uint32_t)(a.cout_pad * a.cin);
+    }
+    const uint32_t wlane = (uint32_t)co0 * (uint32_t)a.cin + (uint32_t)(kh * 8);
+    const uint32_t wtap_step = (uint32_t)(a.cout_pad * a.cin);
+    // fragments of the next K step: A (this lane's voxel, 8 channels) and NCO B
+    auto fetch = [&](uint4 &av, uint4 *bv) {
+        const int c0 = f_sl * 16;
+        const int zz = z + f_dz, yy = y + f_dy, xx = x + f_dx;
         const bool ok = vin && zz >= 0 && zz < a.D && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
         av = make_uint4(0, 0, 0, 0);
         if (c0 < a.ca) {
-            const int64_t src = (((int64_t)n * Da + (zz >> a.up)) * Ha + (yy >> a.up)) * Wa + (xx >> a.up);
-            if (ok) av = *(const uint4 *)(a.xa + src * a.ca + c0 + kh * 8);
+            const uint32_t src = (((uint32_t)n * Da + (uint32_t)(zz >> a.up)) * Ha + (uint32_t)(yy >> a.up)) * Wa +
+                                 (uint32_t)(xx >> a.up);
+            if (ok) av = *(const uint4 *)(a.xa + (src * (uint32_t)a.ca + (uint32_t)(c0 + kh * 8)));
         } else {
-            const int64_t src = (((int64_t)n * a.D + zz) * a.H + yy) * a.W + xx;
-            if (ok) av = *(const uint4 *)(a.xb + src * a.cb + (c0 - a.ca) + kh * 8);
+            const uint32_t src = (((uint32_t)n * a.D + (uint32_t)zz) * a.H + (uint32_t)yy) * a.W + (uint32_t)xx;
+            if (ok) av = *(const uint4 *)(a.xb + (src * (uint32_t)a.cb + (uint32_t)(c0 - a.ca + kh * 8)));
         }
-        const uint16_t *wp = a.w + ((int64_t)tp * a.cout_pad + co0) * a.cin + c0 + kh * 8;
+        const uint16_t *wp = a.w + (f_wtap + wlane + (uint32_t)c0);
 #pragma unroll
-        for (int j = 0; j < NCO; j++) bv[j] = *(const uint4 *)(wp + (int64_t)j * 32 * a.cin);
+        for (int j = 0; j < NCO; j++) bv[j] = *(const uint4 *)(wp + (uint32_t)(j * 32 * a.cin));
+        if (++f_sl == nsl) {
+            f_sl = 0;
+            f_wtap += wtap_step;
+            if (++f_dx > rad) {
+                f_dx = -rad;
+                if (++f_dy > rad) {
+                    f_dy = -rad;
+                    ++f_dz;
+                }
+            }
+        }
     };
 
     // A ring of PF K steps in flight: step ki's slot is refilled with step
@@ -111,7 +139,7 @@ k_conv3d(conv_args a)
         uint4 av[PF], bv[PF][NCO];
 #pragma unroll
         for (int p = 0; p < PF; p++)
-            if (k0 + p < k1) fetch(k0 + p, av[p], bv[p]);
+            if (k0 + p < k1) fetch(av[p], bv[p]);
         for (int kb = k0; kb < k1; kb += PF) {
 #pragma unroll
             for (int p = 0; p < PF; p++) {
@@ -125,7 +153,7 @@ k_conv3d(conv_args a)
                         __builtin_memcpy(&B, &bv[p][j], 16);
                         acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, B, acc[j], 0, 0, 0);
                     }
-                    if (ki + PF < k1) fetch(ki + PF, av[p], bv[p]);
+                    if (ki + PF < k1) fetch(av[p], bv[p]);
                 }
             }
         }
@@ -1000,6 +1028,9 @@ conv_args make_conv(const void *xa, int ca, int up, const void *xb, int cb, int 
     DH_REQUIRE(!up || (d % 2 == 0 && h % 2 == 0 && w % 2 == 0), "upsampled input needs even sizes");
     DH_REQUIRE(cout > 0 && cout_pad >= cout && cout_pad % 32 == 0, "bad output channel padding");
     DH_REQUIRE(n > 0 && d > 0 && h > 0 && w > 0, "bad shape");
+    DH_REQUIRE((int64_t)n * d * h * w * std::max(ca + cb, cout_pad) < ((int64_t)1 << 31) &&
+                   (int64_t)27 * cout_pad * (ca + cb) < ((int64_t)1 << 31),
+               "convolution tensors must hold fewer than 2^31 elements");
     conv_args a;
     a.xa = (const uint16_t *)xa; a.xb = (const uint16_t *)xb; a.ca = ca; a.cb = cb; a.up = up;
     a.w = (const uint16_t *)wt; a.bias = bias;
