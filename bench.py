@@ -35,6 +35,11 @@ HBM_PEAK_GBS = 8000.0
 # scaled by a 256 MiB copy calibration (factors 2.0 and 1.0); the commands and
 # per-kernel figures are in profiles/hbm_traffic_cccs128_7beams.md
 CCC_TRAFFIC_BYTES = (2.0 * 111437 + 32833) * 1024
+# HBM bytes per influence fill launch (mean over the 3 fields' fills) from
+# `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE` runs of this bench with
+# the same calibration; commands and per-kernel figures in
+# profiles/kernel_stats_influence_3fields.md
+INFL_FILL_TRAFFIC_BYTES = (2.0 * 638586 + 2110335) * 1024
 
 
 def parse():
@@ -631,7 +636,11 @@ def main():
                                 fill_ms=fill_ms, csr_bytes=8 * st["nnz"] + 8 * (n_rows + 1),
                                 kernels_ms={k: v[0] / max(v[1], 1) for k, v in kt.items()},
                                 roofline=dict(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS,
-                                              unit="GB/s", frac=ach / HBM_PEAK_GBS, traffic=None))
+                                              unit="GB/s", frac=ach / HBM_PEAK_GBS,
+                                              traffic=INFL_FILL_TRAFFIC_BYTES,
+                                              traffic_source="profiled constant: rocprofv3 --pmc FETCH_SIZE / "
+                                                             "WRITE_SIZE of the fill, calibrated by a 256 MiB copy "
+                                                             "(profiles/kernel_stats_influence_3fields.md)"))
         sweep = []
         for f in st["infl"]:
             c, rad = f.stats()
