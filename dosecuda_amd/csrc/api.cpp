@@ -333,6 +333,14 @@ struct wet_memo {
 };
 static wet_memo g_wet_memo[64][DH_MAX_WORKSPACES];
 
+// Depth (cm) below which the beam's-eye-view depth is replaced by the 1 mm
+// ray trace: the electron-contamination term is non-zero and steep there
+// (it reaches zero at the model's fitted dmax); 0.5 cm of margin.
+static float bev_exact_depth(const dh_photon_model *m)
+{
+    return (float)(std::max(0.0, m->electron_fitted_dmax) + 0.5);
+}
+
 static void photon_device(int dev, const dh_grid *grid, const dh_photon_cp *cp,
                           const dh_photon_model *m, const float *mlc_rows, int32_t n_pairs,
                           const float *d_density, float *d_dose, float *d_terma,
@@ -366,7 +374,7 @@ static void photon_device(int dev, const dh_grid *grid, const dh_photon_cp *cp,
     float step = net ? 0.0f : 1.0f;
     if (!(reuse_wet && !d_wet && memo.vpad == vpad && memo.wet == wet && memo.step == step)) {
         dh_launch_pad_quad(d_density, vpad, ca.g.nx, ca.g.ny, ca.g.nz, DH_AIR_DENSITY, s);  // IMRTClasses.cu:394
-        if (step == 0.0f && !dh_launch_bev_wet(dev, ca.g, vpad, wet, s)) step = std::max(1.0f, ca.g.s);
+        if (step == 0.0f && !dh_launch_bev_wet(dev, ca.g, vpad, bev_exact_depth(m), wet, s)) step = std::max(1.0f, ca.g.s);
         if (step > 0.0f) dh_launch_raytrace(ca.g, vpad, wet, s, -1, nullptr, step);
     }
     memo.vpad = d_wet ? nullptr : vpad;
@@ -399,7 +407,7 @@ extern "C" int dh_photon_terma_dev(int gpu_id, const dh_grid *grid, const dh_pho
         float4 *vpad = (float4 *)dh_scratch(gpu_id, DH_SLOT_VOL_PAD, sizeof(float4) * (size_t)p.pxy * p.pz);
         float *wet = (float *)dh_scratch(gpu_id, DH_SLOT_WET_RAW, sizeof(float) * n);
         dh_launch_pad_quad(d_density, vpad, ta.g.nx, ta.g.ny, ta.g.nz, DH_AIR_DENSITY, s);  // IMRTClasses.cu:394
-        if (step_mm == 0.0f && !dh_launch_bev_wet(gpu_id, ta.g, vpad, wet, s)) step_mm = std::max(1.0f, ta.g.s);
+        if (step_mm == 0.0f && !dh_launch_bev_wet(gpu_id, ta.g, vpad, bev_exact_depth(model), wet, s)) step_mm = std::max(1.0f, ta.g.s);
         if (step_mm > 0.0f) dh_launch_raytrace(ta.g, vpad, wet, s, -1, nullptr, step_mm);
         dh_launch_terma(ta, wet, d_terma_out, d_electron_out, s);
         g_wet_memo[gpu_id & 63][dh_get_workspace()] = wet_memo{};   // the padded volume was rewritten

@@ -189,7 +189,11 @@ void dh_launch_pencil(const dh_geom &g, float vsadx, float vsady, int n_layers,
 // beam's-eye-view grid (one density sample per line step of one voxel
 // spacing, trapezoid rule, trilinear lookup); false (nothing launched) when
 // the source is not well outside the volume or the table would be too large.
-bool dh_launch_bev_wet(int dev, const dh_geom &g, const float4 *vol_pad, float *wet, hipStream_t s);
+// Radiological depth (cm) of every voxel from fan-line integrals on a
+// beam's-eye-view grid; voxels shallower than w_exact_cm take the 1 mm ray
+// trace.  Returns false (nothing launched) where the grid does not apply.
+bool dh_launch_bev_wet(int dev, const dh_geom &g, const float4 *vol_pad, float w_exact_cm, float *wet,
+                       hipStream_t s);
 void dh_launch_terma(const dh_terma_args &a, const float *wet, float *terma, float *electron,
                      hipStream_t s);
 // Sampling layout of the cone superposition's {TERMA, density} volume:

@@ -479,7 +479,13 @@ k_ccc_finish(float mu_cal, float mu, const float *__restrict__ electron, const i
 // (a, k) whatever the beam angle, so a wave's lookups touch few cache lines.
 // A voxel's depth is that table interpolated trilinearly at its own (a, b, t):
 // one sample per line step instead of one per voxel per step, off the parity
-// path (the result is judged by gamma against the plain CCCS).
+// path (the result is judged by gamma against the plain CCCS).  An entry whose
+// sample point lies outside the box of voxel centres is stored as
+// -1 - integral; a lattice line takes part in a voxel's lateral interpolation
+// only if one of its two samples around the voxel's distance lies inside that
+// box, so voxels next to a face parallel to the beam are not averaged with
+// lines that run outside the volume, or inside its last half voxel where the
+// density blends with the border.
 struct bev_grid {
     float e1[3], e2[3], c[3];    // lateral unit vectors and central axis (image frame)
     float sad;                   // |source - isocentre|
@@ -524,12 +530,17 @@ k_bev_prefix(dh_geom g, dh_pad p, bev_grid b, const float4 *__restrict__ vol, fl
     const float T0y = fmaf(g.inv_s, g.src[1] + g.iso[1], 0.5f), dTy = g.inv_s * dy;
     const float T0z = fmaf(g.inv_s, g.src[2] + g.iso[2], 0.5f), dTz = g.inv_s * dz;
     const float fnx = (float)g.nx, fny = (float)g.ny, fnz = (float)g.nz;
-    auto sample = [&](int k) -> float {
+    auto sample = [&](int k, bool &in) -> float {
         const float t = fmaf((float)k, b.h, b.t0);
         const float tx = fmaf(dTx, t, T0x), ty = fmaf(dTy, t, T0y), tz = fmaf(dTz, t, T0z);
         float rho = 0.0f;
+        // density over the volume's box (texel 0 .. n, the reference's inside
+        // test, border blend included); the flag marks points inside the box
+        // of voxel centres (texel 0.5 .. n - 0.5)
         if (k >= 0 && k < b.nt && tx >= 0.0f && tx < fnx && ty >= 0.0f && ty < fny && tz >= 0.0f && tz < fnz)
             rho = fmaxf(dh_trilinear_q<Off32>(vol, vol_hi, p, g.nx, g.ny, g.nz, tx, ty, tz), 0.0f);
+        in = k >= 0 && k < b.nt && tx >= 0.5f && tx <= fnx - 0.5f && ty >= 0.5f && ty <= fny - 0.5f &&
+             tz >= 0.5f && tz <= fnz - 0.5f;
         return rho;
     };
     const float hh = 0.5f * b.h;
@@ -538,10 +549,13 @@ k_bev_prefix(dh_geom g, dh_pad p, bev_grid b, const float4 *__restrict__ vol, fl
         // step k adds the trapezoid over (k - 1, k)
         const int k0 = base + seg * kBevLen;
         float run[kBevLen];
-        float prev = sample(k0 - 1), acc = 0.0f;
+        uint32_t inside = 0;             // bit q: the sample of step k0 + q lies among the voxel centres
+        bool in;
+        float prev = sample(k0 - 1, in), acc = 0.0f;
 #pragma unroll
         for (int q = 0; q < kBevLen; q++) {
-            const float rho = sample(k0 + q);
+            const float rho = sample(k0 + q, in);
+            inside |= (uint32_t)in << q;
             acc = fmaf(hh, prev + rho, acc);
             prev = rho;
             run[q] = acc;
@@ -560,13 +574,24 @@ k_bev_prefix(dh_geom g, dh_pad p, bev_grid b, const float4 *__restrict__ vol, fl
         if (line) {
 #pragma unroll
             for (int q = 0; q < kBevLen; q++)
-                if (k0 + q < b.nt) table[bev_index(b, ia, ib, k0 + q)] = off + run[q];
+                if (k0 + q < b.nt) {
+                    const float v = off + run[q];
+                    table[bev_index(b, ia, ib, k0 + q)] = (inside >> q) & 1u ? v : -1.0f - v;
+                }
         }
     }
 }
 
+// Voxels whose interpolated depth is below w_exact cm (where the electron
+// contamination term, which falls to zero at the model's fitted dmax, is
+// large and steep in the depth) take the reference's own ray trace instead:
+// 1 mm samples from the voxel towards the source until the sample leaves the
+// volume.  These rays are short, so the exact depth of the build-up region
+// costs a small fraction of a whole-volume trace.
+template <bool Off32>
 __global__ void __launch_bounds__(kBlock)
-k_bev_lookup(dh_geom g, bev_grid b, const float *__restrict__ table, float *__restrict__ wet)
+k_bev_lookup(dh_geom g, dh_pad p, bev_grid b, const float *__restrict__ table, const float4 *__restrict__ vol,
+             float w_exact, float *__restrict__ wet)
 {
     const uint32_t nvox = (uint32_t)g.nx * g.ny * g.nz;
     const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
@@ -587,19 +612,51 @@ k_bev_lookup(dh_geom g, bev_grid b, const float *__restrict__ table, float *__re
     const float fk = fminf(fmaxf((t - b.t0) / b.h, 0.0f), (float)b.nt - 1.001f);
     const int ia = (int)fa, ib = (int)fb, ik = (int)fk;
     const float ua = fa - (float)ia, ub = fb - (float)ib, uk = fk - (float)ik;
-    float q[2][2][2];                // [k][b][a] corners
+    // per lattice line (a, b corner): the integral at the voxel's distance
+    // (linear in k), and whether the line is inside the volume there
+    float sum_in = 0.0f, w_in = 0.0f, sum_all = 0.0f;
 #pragma unroll
-    for (int dk = 0; dk < 2; dk++)
+    for (int db = 0; db < 2; db++)
 #pragma unroll
-        for (int db = 0; db < 2; db++)
-#pragma unroll
-            for (int da = 0; da < 2; da++) q[dk][db][da] = table[bev_index(b, ia + da, ib + db, ik + dk)];
-    const float c00 = fmaf(ua, q[0][0][1] - q[0][0][0], q[0][0][0]);
-    const float c10 = fmaf(ua, q[0][1][1] - q[0][1][0], q[0][1][0]);
-    const float c01 = fmaf(ua, q[1][0][1] - q[1][0][0], q[1][0][0]);
-    const float c11 = fmaf(ua, q[1][1][1] - q[1][1][0], q[1][1][0]);
-    const float d0 = fmaf(ub, c10 - c00, c00), d1 = fmaf(ub, c11 - c01, c01);
-    wet[v] = 0.1f * fmaf(uk, d1 - d0, d0);      // mm of unit density -> cm
+        for (int da = 0; da < 2; da++) {
+            const float r0 = table[bev_index(b, ia + da, ib + db, ik)];
+            const float r1 = table[bev_index(b, ia + da, ib + db, ik + 1)];
+            const float v0 = r0 >= 0.0f ? r0 : -1.0f - r0, v1 = r1 >= 0.0f ? r1 : -1.0f - r1;
+            const float val = fmaf(uk, v1 - v0, v0);
+            const float w = (da ? ua : 1.0f - ua) * (db ? ub : 1.0f - ub);
+            sum_all = fmaf(w, val, sum_all);
+            if (r0 >= 0.0f || r1 >= 0.0f) {
+                sum_in = fmaf(w, val, sum_in);
+                w_in += w;
+            }
+        }
+    // lines that miss the volume at this distance do not count, unless
+    // (nearly) all four do
+    const float d = w_in > 1e-3f ? sum_in / w_in : sum_all;
+    float w = 0.1f * d;                  // mm of unit density -> cm
+    if (w < w_exact) {
+        const float4 *vol_hi = vol + p.pxy;
+        float ux, uy, uz;
+        dh_uvec(g, x, y, z, ux, uy, uz);
+        const float fnx = (float)g.nx, fny = (float)g.ny, fnz = (float)g.nz;
+        double acc = 0.0;
+        float part = 0.0f;
+        for (int m = 0;; m++) {
+            const float L = (float)m;
+            const float tx = fmaf(g.inv_s, fmaf(ux, L, x) + g.iso[0], 0.5f);
+            const float ty = fmaf(g.inv_s, fmaf(uy, L, y) + g.iso[1], 0.5f);
+            const float tz = fmaf(g.inv_s, fmaf(uz, L, z) + g.iso[2], 0.5f);
+            if (!(tx >= 0.0f && tx < fnx && ty >= 0.0f && ty < fny && tz >= 0.0f && tz < fnz)) break;
+            part += fmaxf(dh_trilinear_q<Off32>(vol, vol_hi, p, g.nx, g.ny, g.nz, tx, ty, tz), 0.0f);
+            if ((m & 3) == 3) {
+                acc += (double)part;
+                part = 0.0f;
+            }
+        }
+        acc += (double)part;
+        w = (float)(-0.05 + 0.1 * acc);
+    }
+    wet[v] = w;
 }
 
 }  // namespace
@@ -613,7 +670,8 @@ void dh_launch_terma(const dh_terma_args &a, const float *wet, float *terma, flo
     DH_HIP(hipGetLastError());
 }
 
-bool dh_launch_bev_wet(int dev, const dh_geom &g, const float4 *vol_pad, float *wet, hipStream_t s)
+bool dh_launch_bev_wet(int dev, const dh_geom &g, const float4 *vol_pad, float w_exact_cm, float *wet,
+                       hipStream_t s)
 {
     const double S[3] = {g.src[0], g.src[1], g.src[2]};
     const double sad = std::sqrt(S[0] * S[0] + S[1] * S[1] + S[2] * S[2]);
@@ -677,17 +735,23 @@ bool dh_launch_bev_wet(int dev, const dh_geom &g, const float4 *vol_pad, float *
     float *table = (float *)dh_scratch(dev, DH_SLOT_BEV, sizeof(float) * (size_t)cells);
     const dh_pad p = dh_make_pad(g.nx, g.ny, g.nz);
     const size_t nl = (size_t)b.na * b.nb, nvox = (size_t)g.nx * g.ny * g.nz;
+    const bool off32 = (size_t)16 * p.pxy * p.pz < ((size_t)1 << 31) && p.pxy < ((int64_t)1 << 23);
     {
         dh_timed t("bev_prefix", s);
         const unsigned lgrid = (unsigned)((nl + 63) / 64);
-        if ((size_t)16 * p.pxy * p.pz < ((size_t)1 << 31) && p.pxy < ((int64_t)1 << 23))
+        if (off32)
             hipLaunchKernelGGL(k_bev_prefix<true>, dim3(lgrid), dim3(64 * kBevSeg), 0, s, g, p, b, vol_pad, table);
         else
             hipLaunchKernelGGL(k_bev_prefix<false>, dim3(lgrid), dim3(64 * kBevSeg), 0, s, g, p, b, vol_pad, table);
         DH_HIP(hipGetLastError());
     }
     dh_timed t("bev_lookup", s);
-    hipLaunchKernelGGL(k_bev_lookup, dim3(grid_for(nvox)), dim3(kBlock), 0, s, g, b, table, wet);
+    if (off32)
+        hipLaunchKernelGGL(k_bev_lookup<true>, dim3(grid_for(nvox)), dim3(kBlock), 0, s, g, p, b, table, vol_pad,
+                           w_exact_cm, wet);
+    else
+        hipLaunchKernelGGL(k_bev_lookup<false>, dim3(grid_for(nvox)), dim3(kBlock), 0, s, g, p, b, table, vol_pad,
+                           w_exact_cm, wet);
     DH_HIP(hipGetLastError());
     return true;
 }

@@ -545,11 +545,14 @@ def test_bev_depth_terma_close_to_raytrace(photon_model, ga, ca, ta):
     """The checked surrogate path's radiological depth (fan-line integrals on
     a beam's-eye-view grid: photon_terma at step 0) gives a TERMA close to the
     reference ray trace's (step 1 mm): on water within 1% of the local value
-    wherever TERMA exceeds 1% of its maximum, two voxels or more inside the
-    volume's faces (next to a side face the lattice lines interpolated
-    include lines that miss the volume), on a 72^3 lung/bone phantom within
-    1.5% of the maximum everywhere and 0.1% on average; replaying the step-0
-    TERMA from a HIP graph gives the eager result."""
+    wherever TERMA exceeds 1% of its maximum two voxels or more inside the
+    volume's faces, and within 1.5% on the faces too wherever it exceeds 10%
+    of its maximum (lattice lines that miss the volume's voxel centres at a
+    face voxel's distance are left out of its interpolation, so face voxels
+    are extrapolated from the lines inside; depths below the electron term's
+    reach come from the 1 mm ray trace); on a 72^3 lung/bone
+    phantom within 1.5% of the maximum everywhere and 0.1% on average;
+    replaying the step-0 TERMA from a HIP graph gives the eager result."""
     t = _torch()
     from dosecuda_amd import engine, phantoms
     from dosecuda_amd.beam_models import square_field_mlc
@@ -570,11 +573,13 @@ def test_bev_depth_terma_close_to_raytrace(photon_model, ga, ca, ta):
         inner[2:-2, 2:-2, 2:-2] = True
         mi = m & inner
         rel = float((dev_abs[mi] / ref[mi]).max())
+        m10 = ref > 0.1 * tmax
+        rel_faces = float((dev_abs[m10] / ref[m10]).max())
         mx, mean = float(dev_abs.max()) / tmax, float(dev_abs[m].mean()) / tmax
-        print(f"[parity] bev depth {name}: max rel {rel:.2e}, max {mx:.2e} / mean {mean:.2e} of max TERMA",
-              flush=True)
+        print(f"[parity] bev depth {name}: max rel {rel:.2e} (with the faces, above 10% of max {rel_faces:.2e}), "
+              f"max {mx:.2e} / mean {mean:.2e} of max TERMA", flush=True)
         if name == "water":
-            assert rel <= 0.01
+            assert rel <= 0.01 and rel_faces <= 0.015
         else:
             assert mx <= 0.015 and mean <= 1e-3
     eager = got.clone()
