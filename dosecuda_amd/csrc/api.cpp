@@ -341,6 +341,37 @@ static float bev_exact_depth(const dh_photon_model *m)
     return (float)(std::max(0.0, m->electron_fitted_dmax) + 0.5);
 }
 
+// Box of the open leaf pairs on the isocentre plane (head frame, mm), widened
+// by 15 mm: beyond it the primary transmission, and with it the electron
+// term, is negligible (the leaf edges' erf profiles have the primary source
+// size, 1 mm, as their width).  An empty box when every pair is closed.
+static void bev_field_box(const dh_terma_args &ta, float box[4])
+{
+    const float ms = (ta.sad - ta.d_mlc) / ta.sad;
+    box[0] = box[2] = 1.0f;
+    box[1] = box[3] = -1.0f;
+    if (ta.n_pairs == 0 || !(ms > 0.0f)) return;
+    float x0 = 1e30f, x1 = -1e30f, y0 = 1e30f, y1 = -1e30f;
+    for (int i = 0; i < ta.n_pairs; i++) {
+        const float4 r = ta.mlc[i];
+        x0 = std::min(x0, std::min(r.x, r.y) / ms);
+        x1 = std::max(x1, std::max(r.x, r.y) / ms);
+        y0 = std::min(y0, std::min(r.z, r.w) / ms);
+        y1 = std::max(y1, std::max(r.z, r.w) / ms);
+    }
+    const float margin = 15.0f;
+    box[0] = x0 - margin; box[1] = x1 + margin;
+    box[2] = y0 - margin; box[3] = y1 + margin;
+}
+
+static bool bev_wet(int dev, const dh_terma_args &ta, const dh_geom &g, const float4 *vpad, const dh_photon_model *m,
+                    float *wet, hipStream_t s)
+{
+    float box[4];
+    bev_field_box(ta, box);
+    return dh_launch_bev_wet(dev, g, vpad, bev_exact_depth(m), box, ta.sad, wet, s);
+}
+
 static void photon_device(int dev, const dh_grid *grid, const dh_photon_cp *cp,
                           const dh_photon_model *m, const float *mlc_rows, int32_t n_pairs,
                           const float *d_density, float *d_dose, float *d_terma,
@@ -374,7 +405,7 @@ static void photon_device(int dev, const dh_grid *grid, const dh_photon_cp *cp,
     float step = net ? 0.0f : 1.0f;
     if (!(reuse_wet && !d_wet && memo.vpad == vpad && memo.wet == wet && memo.step == step)) {
         dh_launch_pad_quad(d_density, vpad, ca.g.nx, ca.g.ny, ca.g.nz, DH_AIR_DENSITY, s);  // IMRTClasses.cu:394
-        if (step == 0.0f && !dh_launch_bev_wet(dev, ca.g, vpad, bev_exact_depth(m), wet, s)) step = std::max(1.0f, ca.g.s);
+        if (step == 0.0f && !bev_wet(dev, ta, ca.g, vpad, m, wet, s)) step = std::max(1.0f, ca.g.s);
         if (step > 0.0f) dh_launch_raytrace(ca.g, vpad, wet, s, -1, nullptr, step);
     }
     memo.vpad = d_wet ? nullptr : vpad;
@@ -407,7 +438,7 @@ extern "C" int dh_photon_terma_dev(int gpu_id, const dh_grid *grid, const dh_pho
         float4 *vpad = (float4 *)dh_scratch(gpu_id, DH_SLOT_VOL_PAD, sizeof(float4) * (size_t)p.pxy * p.pz);
         float *wet = (float *)dh_scratch(gpu_id, DH_SLOT_WET_RAW, sizeof(float) * n);
         dh_launch_pad_quad(d_density, vpad, ta.g.nx, ta.g.ny, ta.g.nz, DH_AIR_DENSITY, s);  // IMRTClasses.cu:394
-        if (step_mm == 0.0f && !dh_launch_bev_wet(gpu_id, ta.g, vpad, bev_exact_depth(model), wet, s)) step_mm = std::max(1.0f, ta.g.s);
+        if (step_mm == 0.0f && !bev_wet(gpu_id, ta, ta.g, vpad, model, wet, s)) step_mm = std::max(1.0f, ta.g.s);
         if (step_mm > 0.0f) dh_launch_raytrace(ta.g, vpad, wet, s, -1, nullptr, step_mm);
         dh_launch_terma(ta, wet, d_terma_out, d_electron_out, s);
         g_wet_memo[gpu_id & 63][dh_get_workspace()] = wet_memo{};   // the padded volume was rewritten
@@ -517,7 +548,7 @@ extern "C" int dh_sur_check_dev(int gpu_id, const dh_grid *grid, const dh_photon
                                 int32_t *d_work, int64_t work_len, float *d_plain, void *stream)
 {
     return guarded([&] {
-        DH_REQUIRE(dilate >= 0 && dilate <= 6 && (dilate & 3) != 3, "dilate must be 0, 1 or 2, plus 4 for the build-up band");
+        dh_check_dilate_word(dilate);
         dh_device_enter(gpu_id);
         nvox_of(grid);
         DH_REQUIRE(cp && d_terma && d_density && d_electron && d_dose && d_dmax && d_work && d_plain,
@@ -542,7 +573,7 @@ extern "C" int dh_photon_dose_surrogate_dev(int gpu_id, const dh_sur_net *net, c
     return guarded([&] {
         DH_REQUIRE(net, "null surrogate network");
         DH_REQUIRE(stride >= 1 && stride <= 64, "lattice stride must be 1..64");
-        DH_REQUIRE(dilate >= 0 && dilate <= 6 && (dilate & 3) != 3, "dilate must be 0, 1 or 2, plus 4 for the build-up band");
+        dh_check_dilate_word(dilate);
         dh_device_enter(gpu_id);
         photon_device(gpu_id, grid, cp, model, mlc_rows, n_pairs, d_density, d_dose_out, d_terma_out,
                       d_electron_out, nullptr, (hipStream_t)stream, net, stride, tol, dilate);
@@ -557,7 +588,7 @@ extern "C" int dh_sur_dose_dev(const dh_sur_net *net, const dh_grid *grid, const
     return guarded([&] {
         DH_REQUIRE(net && cp && d_terma && d_density && d_electron && d_dose_out, "null argument");
         DH_REQUIRE(stride >= 1 && stride <= 64, "lattice stride must be 1..64");
-        DH_REQUIRE(dilate >= 0 && dilate <= 6 && (dilate & 3) != 3, "dilate must be 0, 1 or 2, plus 4 for the build-up band");
+        dh_check_dilate_word(dilate);
         dh_device_enter(dh_sur_net_gpu(net));
         nvox_of(grid);
         dh_ccc_args ca = dh_make_ccc_args(grid, cp, model);
@@ -578,7 +609,7 @@ extern "C" int dh_sur_dose_batch_dev(const dh_sur_net *net, const dh_grid *grid,
         DH_REQUIRE(net && cps && d_terma && d_density && d_electron && d_dose_out, "null argument");
         DH_REQUIRE(n_cp >= 1 && n_cp <= 4096, "control point count must be 1..4096");
         DH_REQUIRE(stride >= 1 && stride <= 64, "lattice stride must be 1..64");
-        DH_REQUIRE(dilate >= 0 && dilate <= 6 && (dilate & 3) != 3, "dilate must be 0, 1 or 2, plus 4 for the build-up band");
+        dh_check_dilate_word(dilate);
         for (int i = 0; i < n_cp; i++)
             DH_REQUIRE(d_terma[i] && d_electron[i] && d_dose_out[i], "null control point volume");
         dh_device_enter(dh_sur_net_gpu(net));

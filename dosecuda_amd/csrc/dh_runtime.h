@@ -185,15 +185,26 @@ void dh_launch_smooth(const dh_geom &g, const float *wet_raw, float *wet_out, hi
 void dh_launch_pencil(const dh_geom &g, float vsadx, float vsady, int n_layers,
                       const dh_layer_k *d_layers, const float4 *d_luts, const float *d_spots,
                       bool fold, const float *wet, float *dose, hipStream_t s, int choice = -1);
+// The surrogate check's `dilate` word: bits 0-1 the neighbourhood of a
+// failed lattice cell that is recomputed too (0 none, 1 the 6 face
+// neighbours, 2 all 26), bit 2 the build-up band (every voxel with a
+// nonzero electron term is recomputed).
+inline void dh_check_dilate_word(int dilate)
+{
+    DH_REQUIRE(dilate >= 0 && dilate <= 6 && (dilate & 3) != 3, "dilate must be 0, 1 or 2, plus 4 for the build-up band");
+}
+
 // Radiological depth (cm) of every voxel from fan-line integrals on a
 // beam's-eye-view grid (one density sample per line step of one voxel
 // spacing, trapezoid rule, trilinear lookup); false (nothing launched) when
 // the source is not well outside the volume or the table would be too large.
 // Radiological depth (cm) of every voxel from fan-line integrals on a
-// beam's-eye-view grid; voxels shallower than w_exact_cm take the 1 mm ray
-// trace.  Returns false (nothing launched) where the grid does not apply.
-bool dh_launch_bev_wet(int dev, const dh_geom &g, const float4 *vol_pad, float w_exact_cm, float *wet,
-                       hipStream_t s);
+// beam's-eye-view grid; voxels shallower than w_exact_cm whose projection on
+// the isocentre plane (source distance sad) lies in field_box {x0, x1, y0, y1}
+// (head frame, mm; null: none) take the 1 mm ray trace.  Returns false
+// (nothing launched) where the grid does not apply.
+bool dh_launch_bev_wet(int dev, const dh_geom &g, const float4 *vol_pad, float w_exact_cm, const float *field_box,
+                       float sad, float *wet, hipStream_t s);
 void dh_launch_terma(const dh_terma_args &a, const float *wet, float *terma, float *electron,
                      hipStream_t s);
 // Sampling layout of the cone superposition's {TERMA, density} volume:

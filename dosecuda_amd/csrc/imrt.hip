@@ -362,11 +362,17 @@ k_ccc_compact(dh_geom g, dh_pad p, const void *__restrict__ tr, int *__restrict_
 constexpr int kVoxPerItem = 64;
 constexpr int kGroups = 12;
 
-template <bool Quad, bool Off32>
+// Rows: for short voxel lists (the surrogate's check), whose launches are too
+// small to fill the chip, an item is one zenith row of one azimuth (six times
+// more, six times shorter items); each row's result goes to its own partial
+// and k_ccc_finish<true> adds an azimuth's six rows in the same compensated
+// order as the azimuth items do, so the dose is bit for bit the same.
+template <bool Quad, bool Off32, bool Rows = false>
 __global__ void __launch_bounds__(kBlock)
 k_ccc(dh_ccc_args a, const void *__restrict__ tr, const int *__restrict__ list, const int *__restrict__ counter,
       float *__restrict__ part, uint32_t pstride)
 {
+    constexpr int kItems = Rows ? 6 * kGroups : kGroups;
     const dh_geom &g = a.g;
     const tr_bases B = tr_make_bases<Quad>(tr, a.p);
     const int n = *counter;
@@ -377,12 +383,13 @@ k_ccc(dh_ccc_args a, const void *__restrict__ tr, const int *__restrict__ list, 
     // XCD's gathers stay in one slab of the volume (its own L2).
     const int wave = dh_xcd_block(blockIdx.x, gridDim.x) * (kBlock / 64) + (threadIdx.x >> 6);
     const int nwaves = gridDim.x * (kBlock / 64);
-    const int nitem = (n + kVoxPerItem - 1) / kVoxPerItem * kGroups;
+    const int nitem = (n + kVoxPerItem - 1) / kVoxPerItem * kItems;
     const int c0 = (int)((int64_t)nitem * wave / nwaves);
     const int c1 = (int)((int64_t)nitem * (wave + 1) / nwaves);
     for (int item = c0; item < c1; item++) {
-        const int grp = item % kGroups;
-        const int base = item / kGroups * kVoxPerItem;
+        const int slot = item % kItems;          // Rows: row * 12 + azimuth
+        const int grp = slot % kGroups;
+        const int base = item / kItems * kVoxPerItem;
         const int idx = base + lane;
         const bool valid = idx < n;
         const uint32_t v = (uint32_t)list[valid ? idx : base];
@@ -404,7 +411,8 @@ k_ccc(dh_ccc_args a, const void *__restrict__ tr, const int *__restrict__ list, 
         const float kz = a.ax * uy - a.ay * ux;
         const float inv1c = 1.0f / (1.0f + c);
         float ds = 0.0f, dcmp = 0.0f;
-        for (int row = 0; row < 6; row++) {
+        const int row0 = Rows ? slot / kGroups : 0, row1 = Rows ? row0 + 1 : 6;
+        for (int row = row0; row < row1; row++) {
             const float c1 = a.c1[row], c2p = a.c2p[row], bp = a.bp[row], a2 = a.a2[row];
             const float am = a.a[row], bm = a.b[row], L0 = a.L0[row];
             const int ns = a.nsteps[row];
@@ -440,15 +448,19 @@ k_ccc(dh_ccc_args a, const void *__restrict__ tr, const int *__restrict__ list, 
                     Rs = fmaf(rho, fmaf(T, c2p, -bp * Rs), Rs);               // Rs (1 - bD) + T D c2
                     L = L - a.sp;
                 }
-                dh_neumaier(ds, dcmp, fmaf(am, Rp, bm * Rs));
+                if (Rows)
+                    ds = fmaf(am, Rp, bm * Rs);
+                else
+                    dh_neumaier(ds, dcmp, fmaf(am, Rp, bm * Rs));
             }
         }
-        if (valid) part[(size_t)grp * pstride + idx] = ds + dcmp;
+        if (valid) part[(size_t)slot * pstride + idx] = ds + dcmp;
     }
 }
 
 // Dose of the listed voxels from the 12 azimuth sums, the electron term and the
 // MU calibration (IMRTClasses.cu:338-346).
+template <bool Rows = false>
 __global__ void __launch_bounds__(kBlock)
 k_ccc_finish(float mu_cal, float mu, const float *__restrict__ electron, const int *__restrict__ list,
              const int *__restrict__ counter, const float *__restrict__ part, uint32_t pstride,
@@ -458,8 +470,18 @@ k_ccc_finish(float mu_cal, float mu, const float *__restrict__ electron, const i
     for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < n; idx += gridDim.x * kBlock) {
         const uint32_t v = (uint32_t)list[idx];
         float p[kGroups];
+        if (Rows) {
 #pragma unroll
-        for (int q = 0; q < kGroups; q++) p[q] = part[(size_t)q * pstride + idx];
+            for (int q = 0; q < kGroups; q++) {
+                float ds = 0.0f, dcmp = 0.0f;
+#pragma unroll
+                for (int row = 0; row < 6; row++) dh_neumaier(ds, dcmp, part[(size_t)(row * kGroups + q) * pstride + idx]);
+                p[q] = ds + dcmp;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < kGroups; q++) p[q] = part[(size_t)q * pstride + idx];
+        }
         const float s0 = (p[0] + p[1]) + (p[2] + p[3]);
         const float s1 = (p[4] + p[5]) + (p[6] + p[7]);
         const float s2 = (p[8] + p[9]) + (p[10] + p[11]);
@@ -584,14 +606,17 @@ k_bev_prefix(dh_geom g, dh_pad p, bev_grid b, const float4 *__restrict__ vol, fl
 
 // Voxels whose interpolated depth is below w_exact cm (where the electron
 // contamination term, which falls to zero at the model's fitted dmax, is
-// large and steep in the depth) take the reference's own ray trace instead:
-// 1 mm samples from the voxel towards the source until the sample leaves the
+// large and steep in the depth) and whose projection onto the isocentre
+// plane lies in the field box fb (head frame, mm; the electron term is
+// proportional to the primary transmission, which vanishes a few mm outside
+// the open leaf pairs) take the reference's own ray trace instead: 1 mm
+// samples from the voxel towards the source until the sample leaves the
 // volume.  These rays are short, so the exact depth of the build-up region
 // costs a small fraction of a whole-volume trace.
 template <bool Off32>
 __global__ void __launch_bounds__(kBlock)
 k_bev_lookup(dh_geom g, dh_pad p, bev_grid b, const float *__restrict__ table, const float4 *__restrict__ vol,
-             float w_exact, float *__restrict__ wet)
+             float w_exact, float4 fbox, float sad_h, float *__restrict__ wet)
 {
     const uint32_t nvox = (uint32_t)g.nx * g.ny * g.nz;
     const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
@@ -634,7 +659,15 @@ k_bev_lookup(dh_geom g, dh_pad p, bev_grid b, const float *__restrict__ table, c
     // (nearly) all four do
     const float d = w_in > 1e-3f ? sum_in / w_in : sum_all;
     float w = 0.1f * d;                  // mm of unit density -> cm
-    if (w < w_exact) {
+    bool exact = w < w_exact;
+    if (exact) {
+        float hx, hy, hz;
+        dh_img_to_head(g, x, y, z, hx, hy, hz);
+        const float mag = sad_h / (sad_h - hz);
+        const float xi = hx * mag, yi = hy * mag;
+        exact = xi >= fbox.x && xi <= fbox.y && yi >= fbox.z && yi <= fbox.w;
+    }
+    if (exact) {
         const float4 *vol_hi = vol + p.pxy;
         float ux, uy, uz;
         dh_uvec(g, x, y, z, ux, uy, uz);
@@ -670,9 +703,11 @@ void dh_launch_terma(const dh_terma_args &a, const float *wet, float *terma, flo
     DH_HIP(hipGetLastError());
 }
 
-bool dh_launch_bev_wet(int dev, const dh_geom &g, const float4 *vol_pad, float w_exact_cm, float *wet,
-                       hipStream_t s)
+bool dh_launch_bev_wet(int dev, const dh_geom &g, const float4 *vol_pad, float w_exact_cm, const float *field_box,
+                       float sad_field, float *wet, hipStream_t s)
 {
+    const float4 fbox = field_box ? make_float4(field_box[0], field_box[1], field_box[2], field_box[3])
+                                : make_float4(1.0f, -1.0f, 1.0f, -1.0f);
     const double S[3] = {g.src[0], g.src[1], g.src[2]};
     const double sad = std::sqrt(S[0] * S[0] + S[1] * S[1] + S[2] * S[2]);
     if (!(sad > 1.0)) return false;
@@ -748,10 +783,10 @@ bool dh_launch_bev_wet(int dev, const dh_geom &g, const float4 *vol_pad, float w
     dh_timed t("bev_lookup", s);
     if (off32)
         hipLaunchKernelGGL(k_bev_lookup<true>, dim3(grid_for(nvox)), dim3(kBlock), 0, s, g, p, b, table, vol_pad,
-                           w_exact_cm, wet);
+                           w_exact_cm, fbox, sad_field, wet);
     else
         hipLaunchKernelGGL(k_bev_lookup<false>, dim3(grid_for(nvox)), dim3(kBlock), 0, s, g, p, b, table, vol_pad,
-                           w_exact_cm, wet);
+                           w_exact_cm, fbox, sad_field, wet);
     DH_HIP(hipGetLastError());
     return true;
 }
@@ -785,14 +820,30 @@ void launch_k_ccc(const char *timer, const dh_ccc_args &a, int layout, int64_t m
                   const float *electron, const int *list, const int *counter, float *dose, hipStream_t s)
 {
     const bool o32 = ccc_off32(a);
-    const uint32_t pstride = (uint32_t)((size_t)a.g.nx * a.g.ny * a.g.nz);
-    DH_REQUIRE(max_n >= 0 && max_n <= (int64_t)pstride, "cone list longer than the volume");
+    const uint32_t nvox = (uint32_t)((size_t)a.g.nx * a.g.ny * a.g.nz);
+    DH_REQUIRE(max_n >= 0 && max_n <= (int64_t)nvox, "cone list longer than the volume");
     float *part = (float *)((char *)tr + tr_vol_bytes(a));
-    const int64_t items = (max_n + kVoxPerItem - 1) / kVoxPerItem * kGroups;
+    // lists of at most 1/8 of the volume take row items (72 partials per
+    // voxel fit the 12-per-voxel partial buffer; such launches are short)
+    const bool rows = max_n <= (int64_t)nvox / 8;
+    const uint32_t pstride = rows ? (uint32_t)max_n : nvox;
+    const int64_t items = (max_n + kVoxPerItem - 1) / kVoxPerItem * kGroups * (rows ? 6 : 1);
     const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((items + 3) / 4, 2048));
     {
         dh_timed t(timer, s);
-        if (layout && o32)
+        if (rows && layout && o32)
+            hipLaunchKernelGGL((k_ccc<true, true, true>), dim3(grid), dim3(kBlock), 0, s, a, tr, list, counter, part,
+                               pstride);
+        else if (rows && layout)
+            hipLaunchKernelGGL((k_ccc<true, false, true>), dim3(grid), dim3(kBlock), 0, s, a, tr, list, counter, part,
+                               pstride);
+        else if (rows && o32)
+            hipLaunchKernelGGL((k_ccc<false, true, true>), dim3(grid), dim3(kBlock), 0, s, a, tr, list, counter, part,
+                               pstride);
+        else if (rows)
+            hipLaunchKernelGGL((k_ccc<false, false, true>), dim3(grid), dim3(kBlock), 0, s, a, tr, list, counter, part,
+                               pstride);
+        else if (layout && o32)
             hipLaunchKernelGGL((k_ccc<true, true>), dim3(grid), dim3(kBlock), 0, s, a, tr, list, counter, part, pstride);
         else if (layout)
             hipLaunchKernelGGL((k_ccc<true, false>), dim3(grid), dim3(kBlock), 0, s, a, tr, list, counter, part,
@@ -807,8 +858,12 @@ void launch_k_ccc(const char *timer, const dh_ccc_args &a, int layout, int64_t m
     }
     const unsigned fgrid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((max_n + kBlock - 1) / kBlock, 4096));
     dh_timed t("ccc_finish", s);
-    hipLaunchKernelGGL(k_ccc_finish, dim3(fgrid), dim3(kBlock), 0, s, a.mu_cal, a.mu, electron, list, counter,
-                       (const float *)part, pstride, dose);
+    if (rows)
+        hipLaunchKernelGGL(k_ccc_finish<true>, dim3(fgrid), dim3(kBlock), 0, s, a.mu_cal, a.mu, electron, list,
+                           counter, (const float *)part, pstride, dose);
+    else
+        hipLaunchKernelGGL(k_ccc_finish<false>, dim3(fgrid), dim3(kBlock), 0, s, a.mu_cal, a.mu, electron, list,
+                           counter, (const float *)part, pstride, dose);
     DH_HIP(hipGetLastError());
 }
 }  // namespace

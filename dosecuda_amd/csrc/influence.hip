@@ -213,29 +213,56 @@ k_influence(dh_infl_args a)
             }
             __syncthreads();
             st_rows += nrow;
-            int cur = 0;                 // compacted row of the chunk's first voxel
+            int cur = 0;                 // compacted row of the next chunk's first voxel
+            // Voxel of this lane in the chunk at flat position c0: rows starting
+            // inside the chunk come from a uniform walk (usually 1-2 rows).
+            // The WET and factor loads of chunk n + 1 are issued before chunk n's
+            // spots are evaluated, so their latency overlaps that work.
+            auto locate = [&](int c0, int &ci, int &cj, int &ck) {
+                const int f = min(c0 + lane, total - 1);
+                int q = cur, my_q = cur;
+                for (int nq = cur + 1; nq < nrow; nq++) {
+                    const int e = s_rex[nq];
+                    if (e > c0 + 63) break;
+                    my_q = f >= e ? nq : my_q;
+                    q = nq;
+                }
+                cur = q;
+                ci = s_rlo[my_q] + (f - s_rex[my_q]);
+                const int rglob = (int)rb + s_rid[my_q];
+                ck = B.k0 + rglob / nj;
+                cj = B.j0 + rglob % nj;
+            };
+            const float4 *fplane = a.factors + (size_t)B.layer * g.nx * g.ny * g.nz;
+            int ni, nj_, nk_;
+            locate(0, ni, nj_, nk_);
+            uint32_t nv = (uint32_t)ni + (uint32_t)g.nx * ((uint32_t)nj_ + (uint32_t)g.ny * (uint32_t)nk_);
+            float nw = 0.0f;
+            float4 nfc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (MODE != INF_FILL || span_first <= pc) {
+                nw = a.wet[nv];
+                nfc = fplane[nv];
+            }
             for (int f0 = 0; f0 < total; f0 += 64, pc++) {
                 if (MODE == INF_FILL && pc > span_last) {
                     done = true;
                     break;
                 }
-                const int f = min(f0 + lane, total - 1);
+                const int i = ni, j = nj_, k = nk_;
+                const uint32_t v = nv;
                 const bool inb = f0 + lane < total;
-                // rows starting inside this chunk (a uniform walk; usually 1-2)
-                int q = cur, my_q = cur;
-                for (int nq = cur + 1; nq < nrow; nq++) {
-                    const int e = s_rex[nq];
-                    if (e > f0 + 63) break;
-                    my_q = f >= e ? nq : my_q;
-                    q = nq;
+                const float w = nw * 10.0f;
+                // factors of this (voxel, layer), written by the bounds pass
+                const float4 fc = nfc;
+                if (f0 + 64 < total) {
+                    locate(f0 + 64, ni, nj_, nk_);
+                    nv = (uint32_t)ni + (uint32_t)g.nx * ((uint32_t)nj_ + (uint32_t)g.ny * (uint32_t)nk_);
+                    if (MODE != INF_FILL || span_first <= pc + 1) {
+                        nw = a.wet[nv];
+                        nfc = fplane[nv];
+                    }
                 }
-                cur = q;
                 if (MODE == INF_FILL && pc < span_first) continue;
-                const int i = s_rlo[my_q] + (f - s_rex[my_q]);
-                const int rglob = (int)rb + s_rid[my_q];
-                const int k = B.k0 + rglob / nj, j = B.j0 + rglob % nj;
-                const uint32_t v = (uint32_t)i + (uint32_t)g.nx * ((uint32_t)j + (uint32_t)g.ny * (uint32_t)k);
-                const float w = a.wet[v] * 10.0f;
                 const bool act = inb && !(w > L.cut);
                 st_chunk++;
                 st_visit += __popcll(__builtin_amdgcn_ballot_w64(inb));
@@ -244,8 +271,6 @@ k_influence(dh_infl_args a)
                 float px, py, pz, hx, hy, hz;
                 dh_vox_xyz(g, i, j, k, px, py, pz);
                 dh_img_to_head(g, px, py, pz, hx, hy, hz);
-                // factors of this (voxel, layer), written by the bounds pass
-                const float4 fc = a.factors[(size_t)B.layer * g.nx * g.ny * g.nz + v];
                 const float cx = 1.0f - hz / a.vsadx, cy = 1.0f - hz / a.vsady;
 #pragma unroll
                 for (int s = 0; s < S; s++) {

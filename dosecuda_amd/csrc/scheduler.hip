@@ -326,7 +326,7 @@ extern "C" int dh_sched_add_photon_scaled(dh_sched *s, int32_t case_id, const dh
     return dh_guarded([&] {
         DH_REQUIRE(std::isfinite(scale), "the dose scale must be finite");
         DH_REQUIRE(cp && model, "null argument");
-        DH_REQUIRE(dilate >= 0 && dilate <= 6 && (dilate & 3) != 3, "dilate must be 0, 1 or 2, plus 4 for the build-up band");
+        dh_check_dilate_word(dilate);
         DH_REQUIRE(n_pairs >= 0 && n_pairs <= DH_MAX_PAIRS && (n_pairs == 0 || mlc_rows), "bad MLC");
         DH_REQUIRE(!net || (stride >= 1 && stride <= 64), "lattice stride must be 1..64");
         DH_REQUIRE(s && case_id >= 0 && case_id < (int32_t)s->cases.size(), "unknown case");
