@@ -362,20 +362,24 @@ k_ccc_compact(dh_geom g, dh_pad p, const void *__restrict__ tr, int *__restrict_
 constexpr int kVoxPerItem = 64;
 constexpr int kGroups = 12;
 
-// Rows: for short voxel lists (the surrogate's check), whose launches are too
-// small to fill the chip, an item is one zenith row of one azimuth (six times
-// more, six times shorter items); each row's result goes to its own partial
-// and k_ccc_finish<true> adds an azimuth's six rows in the same compensated
-// order as the azimuth items do, so the dose is bit for bit the same.
-template <bool Quad, bool Off32, bool Rows = false>
+// Row items: for voxel lists of at most `cap` voxels (the surrogate's check
+// lists, whose launches are short), an item is one zenith row of one azimuth
+// (six times more, six times shorter items, so the list's work spreads over
+// the chip); each row's result goes to its own partial (stride cap) and
+// k_ccc_finish adds an azimuth's six rows in the same compensated order as
+// an azimuth item does, so the dose is bit for bit the same.  The choice is
+// made on the device from the list's length (uniform over the grid).
+template <bool Quad, bool Off32, bool RowItems = false>
 __global__ void __launch_bounds__(kBlock)
 k_ccc(dh_ccc_args a, const void *__restrict__ tr, const int *__restrict__ list, const int *__restrict__ counter,
-      float *__restrict__ part, uint32_t pstride)
+      float *__restrict__ part, uint32_t pstride_az, uint32_t cap)
 {
-    constexpr int kItems = Rows ? 6 * kGroups : kGroups;
     const dh_geom &g = a.g;
     const tr_bases B = tr_make_bases<Quad>(tr, a.p);
     const int n = *counter;
+    const bool Rows = RowItems && (uint32_t)n <= cap;
+    const int kItems = Rows ? 6 * kGroups : kGroups;
+    const uint32_t pstride = Rows ? cap : pstride_az;
     const float fnx = (float)g.nx, fny = (float)g.ny, fnz = (float)g.nz;
     const int lane = __lane_id();
     // Each wave takes one contiguous run of work items (a chunk's 4 groups are
@@ -460,13 +464,14 @@ k_ccc(dh_ccc_args a, const void *__restrict__ tr, const int *__restrict__ list, 
 
 // Dose of the listed voxels from the 12 azimuth sums, the electron term and the
 // MU calibration (IMRTClasses.cu:338-346).
-template <bool Rows = false>
 __global__ void __launch_bounds__(kBlock)
 k_ccc_finish(float mu_cal, float mu, const float *__restrict__ electron, const int *__restrict__ list,
-             const int *__restrict__ counter, const float *__restrict__ part, uint32_t pstride,
+             const int *__restrict__ counter, const float *__restrict__ part, uint32_t pstride_az, uint32_t cap,
              float *__restrict__ dose)
 {
     const int n = *counter;
+    const bool Rows = (uint32_t)n <= cap;
+    const uint32_t pstride = Rows ? cap : pstride_az;
     for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < n; idx += gridDim.x * kBlock) {
         const uint32_t v = (uint32_t)list[idx];
         float p[kGroups];
@@ -527,7 +532,7 @@ __device__ static inline size_t bev_index(const bev_grid &b, int ia, int ib, int
 // its segment's running trapezoid sums in registers, the segment totals give
 // every segment its offset through LDS, and lines longer than kBevSeg
 // segments carry their integral into the next round of segments.
-constexpr int kBevSeg = 8, kBevLen = 32;
+constexpr int kBevSeg = 16, kBevLen = 16;
 
 template <bool Off32>
 __global__ void __launch_bounds__(64 * kBevSeg)
@@ -817,53 +822,55 @@ bool ccc_off32(const dh_ccc_args &a)
 // Cone kernel + finish over the first *counter entries of list (at most
 // max_n, which sizes the grids).
 void launch_k_ccc(const char *timer, const dh_ccc_args &a, int layout, int64_t max_n, const void *tr,
-                  const float *electron, const int *list, const int *counter, float *dose, hipStream_t s)
+                  const float *electron, const int *list, const int *counter, float *dose, hipStream_t s,
+                  bool row_items)
 {
     const bool o32 = ccc_off32(a);
     const uint32_t nvox = (uint32_t)((size_t)a.g.nx * a.g.ny * a.g.nz);
     DH_REQUIRE(max_n >= 0 && max_n <= (int64_t)nvox, "cone list longer than the volume");
     float *part = (float *)((char *)tr + tr_vol_bytes(a));
-    // lists of at most 1/8 of the volume take row items (72 partials per
-    // voxel fit the 12-per-voxel partial buffer; such launches are short)
-    const bool rows = max_n <= (int64_t)nvox / 8;
-    const uint32_t pstride = rows ? (uint32_t)max_n : nvox;
-    const int64_t items = (max_n + kVoxPerItem - 1) / kVoxPerItem * kGroups * (rows ? 6 : 1);
+    // lists of at most 16384 voxels take row items: their azimuth items
+    // (12 per 64 voxels, at most 3072) would leave most of the chip's waves
+    // idle, while longer lists fill it either way and the row items' six
+    // times more partials only cost traffic there (measured on the checked
+    // surrogate's lattice and fallback lists); 72 partials per voxel must fit
+    // the 12-per-voxel partial buffer (at most nvox / 6 voxels)
+    const uint32_t cap = row_items ? (uint32_t)std::min<int64_t>(std::min<int64_t>(max_n, nvox / 6), 16384) : 0u;
+    const int64_t items = (max_n + kVoxPerItem - 1) / kVoxPerItem * kGroups +
+                          (cap ? ((int64_t)cap + kVoxPerItem - 1) / kVoxPerItem * kGroups * 5 : 0);
     const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((items + 3) / 4, 2048));
     {
         dh_timed t(timer, s);
-        if (rows && layout && o32)
+        if (cap && layout && o32)
             hipLaunchKernelGGL((k_ccc<true, true, true>), dim3(grid), dim3(kBlock), 0, s, a, tr, list, counter, part,
-                               pstride);
-        else if (rows && layout)
+                               nvox, cap);
+        else if (cap && layout)
             hipLaunchKernelGGL((k_ccc<true, false, true>), dim3(grid), dim3(kBlock), 0, s, a, tr, list, counter, part,
-                               pstride);
-        else if (rows && o32)
+                               nvox, cap);
+        else if (cap && o32)
             hipLaunchKernelGGL((k_ccc<false, true, true>), dim3(grid), dim3(kBlock), 0, s, a, tr, list, counter, part,
-                               pstride);
-        else if (rows)
+                               nvox, cap);
+        else if (cap)
             hipLaunchKernelGGL((k_ccc<false, false, true>), dim3(grid), dim3(kBlock), 0, s, a, tr, list, counter, part,
-                               pstride);
+                               nvox, cap);
         else if (layout && o32)
-            hipLaunchKernelGGL((k_ccc<true, true>), dim3(grid), dim3(kBlock), 0, s, a, tr, list, counter, part, pstride);
+            hipLaunchKernelGGL((k_ccc<true, true>), dim3(grid), dim3(kBlock), 0, s, a, tr, list, counter, part, nvox,
+                               cap);
         else if (layout)
-            hipLaunchKernelGGL((k_ccc<true, false>), dim3(grid), dim3(kBlock), 0, s, a, tr, list, counter, part,
-                               pstride);
+            hipLaunchKernelGGL((k_ccc<true, false>), dim3(grid), dim3(kBlock), 0, s, a, tr, list, counter, part, nvox,
+                               cap);
         else if (o32)
-            hipLaunchKernelGGL((k_ccc<false, true>), dim3(grid), dim3(kBlock), 0, s, a, tr, list, counter, part,
-                               pstride);
+            hipLaunchKernelGGL((k_ccc<false, true>), dim3(grid), dim3(kBlock), 0, s, a, tr, list, counter, part, nvox,
+                               cap);
         else
-            hipLaunchKernelGGL((k_ccc<false, false>), dim3(grid), dim3(kBlock), 0, s, a, tr, list, counter, part,
-                               pstride);
+            hipLaunchKernelGGL((k_ccc<false, false>), dim3(grid), dim3(kBlock), 0, s, a, tr, list, counter, part, nvox,
+                               cap);
         DH_HIP(hipGetLastError());
     }
     const unsigned fgrid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((max_n + kBlock - 1) / kBlock, 4096));
     dh_timed t("ccc_finish", s);
-    if (rows)
-        hipLaunchKernelGGL(k_ccc_finish<true>, dim3(fgrid), dim3(kBlock), 0, s, a.mu_cal, a.mu, electron, list,
-                           counter, (const float *)part, pstride, dose);
-    else
-        hipLaunchKernelGGL(k_ccc_finish<false>, dim3(fgrid), dim3(kBlock), 0, s, a.mu_cal, a.mu, electron, list,
-                           counter, (const float *)part, pstride, dose);
+    hipLaunchKernelGGL(k_ccc_finish, dim3(fgrid), dim3(kBlock), 0, s, a.mu_cal, a.mu, electron, list, counter,
+                       (const float *)part, nvox, cap, dose);
     DH_HIP(hipGetLastError());
 }
 }  // namespace
@@ -888,7 +895,7 @@ void dh_launch_ccc_counted(const dh_ccc_args &a, const void *tr_pad, int layout,
                            const int *list, const int *counter, int64_t max_n, float *dose, hipStream_t s)
 {
     if (max_n <= 0) return;
-    launch_k_ccc("ccc_list", a, layout, max_n, tr_pad, electron, list, counter, dose, s);
+    launch_k_ccc("ccc_list", a, layout, max_n, tr_pad, electron, list, counter, dose, s, true);
 }
 
 void dh_launch_ccc(const dh_ccc_args &a, const float *terma, const float *density,
@@ -911,7 +918,7 @@ void dh_launch_ccc(const dh_ccc_args &a, const float *terma, const float *densit
                                (const void *)tr_pad, active, counter, dose);
         DH_HIP(hipGetLastError());
     }
-    launch_k_ccc("ccc", a, layout, (int64_t)n, tr_pad, electron, active, counter, dose, s);
+    launch_k_ccc("ccc", a, layout, (int64_t)n, tr_pad, electron, active, counter, dose, s, false);
 }
 
 void dh_launch_ccc_list(const dh_ccc_args &a, const float *terma, const float *density,
