@@ -128,7 +128,11 @@ def setup_influence(torch, tune=True):
             # required of every candidate); the matrix below reads the choice
             t = autotune.tune_influence(w, rlsp.shape, case.spacing, case.origin, case.iso, g, 0.0,
                                         m, spots, reps=2)
-            st["tuning"].append(t["best"])
+            # voxel tiles of the footprint ray trace and smoothing (identical WET)
+            tw = autotune.tune_influence_wet(d, rlsp.shape, case.spacing, case.origin, case.iso, g, 0.0,
+                                             m, spots, reps=2)
+            st["tuning"].append(dict(t["best"], raytrace_box=tw["raytrace_box"]["best"],
+                                     smooth_box=tw["smooth_box"]["best"]))
         f = engine.InfluenceMatrix(0, rlsp.shape, case.spacing, case.origin, case.iso, g, 0.0, m, spots)
         st["infl"].append(f)
         st["wets"].append(f.wet(d, out=w))

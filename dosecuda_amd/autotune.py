@@ -155,6 +155,56 @@ def tune_surrogate(net, cells, reps=5):
     return out
 
 
+def geom_key(shape, gantry, couch, src_dist):
+    """The library's tuning key of a beam geometry: "nz,ny,nx,axis", axis the
+    image axis of the source position's largest component (FP32, as the
+    library computes it)."""
+    f = np.float32
+    pi = f(np.pi)
+    ga = f(np.fmod(f(gantry) + f(180.0), f(360.0))) * pi / f(180.0)
+    ta = f(couch) * pi / f(180.0)
+    xg = -f(src_dist) * np.sin(ga, dtype=f)
+    a = [abs(xg * np.cos(ta, dtype=f)), abs(f(src_dist) * np.cos(ga, dtype=f)), abs(-xg * np.sin(ta, dtype=f))]
+    axis = 0 if (a[0] >= a[1] and a[0] >= a[2]) else (1 if a[1] >= a[2] else 2)
+    return f"{int(shape[0])},{int(shape[1])},{int(shape[2])},{axis}"
+
+
+def tune_influence_wet(rlsp: torch.Tensor, shape, spacing, origin, iso, gantry, couch, model, spots,
+                       choices=(5, 6, 3, 0), reps=3):
+    """Voxel tiles of an influence matrix's footprint ray trace and smoothing
+    ("raytrace_box", "smooth_box"): times InfluenceMatrix.wet() with each
+    choice, requires the identical WET of every one and keeps the fastest."""
+    f = engine.InfluenceMatrix(0, shape, spacing, origin, iso, gantry, couch, model, spots)
+    key = geom_key(shape, gantry, couch, (float(np.float32(model.VSADX)) + float(np.float32(model.VSADY))) / 2.0)
+    out = {}
+    try:
+        w = torch.empty_like(rlsp)
+        ref = None
+        for kernel in ("raytrace_box", "smooth_box"):
+            times = {}
+            for c in choices:
+                set_choice(kernel, key, c)
+                f.wet(rlsp, out=w)
+                torch.cuda.synchronize()
+                if ref is None:
+                    ref = w.clone()
+                elif not torch.equal(ref, w):
+                    raise RuntimeError(f"{kernel} tile {TILES[c]} changes the WET")
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                for _ in range(reps):
+                    f.wet(rlsp, out=w)
+                e1.record()
+                torch.cuda.synchronize()
+                times[c] = e0.elapsed_time(e1) / reps
+            best = min(times, key=times.get)
+            set_choice(kernel, key, best, times[best])
+            out[kernel] = dict(best=TILES[best], ms={TILES[c]: t for c, t in times.items()})
+    finally:
+        f.close()
+    return out
+
+
 def tune_influence(rlsp_wet: torch.Tensor, shape, spacing, origin, iso, gantry, couch, model, spots,
                    candidates=((4, 32), (8, 16), (8, 32), (8, 64), (16, 16), (16, 32), (16, 64)), reps=3):
     """Spots per workgroup and row chunks of an influence matrix: builds the

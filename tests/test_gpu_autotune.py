@@ -78,6 +78,31 @@ def test_tune_influence(proton_model):
     autotune.clear()
 
 
+@pytest.mark.parametrize("gantry", [0.0, 120.0])
+def test_tune_influence_wet(proton_model, gantry):
+    """The footprint WET's tile tuner stores its fastest tiles under the
+    beam's geometry key and every tile gives the identical WET (the tuner
+    raises otherwise); the key names the image axis of the source's largest
+    component, as the library's own key does (y at gantry 0, x at 120)."""
+    t = _torch()
+    from dosecuda_amd import autotune, engine
+    m = proton_model
+    shape, s = (30, 36, 32), 2.5
+    rlsp = t.ones(shape, device="cuda")
+    rlsp[8:14, 10:20, 5:25] = 1.6
+    iso = np.array([(shape[2] - 1) / 2 * s, (shape[1] - 1) / 2 * s, (shape[0] - 1) / 2 * s], np.float32)
+    rng = np.random.default_rng(5)
+    spots = np.column_stack([rng.uniform(-15, 15, (40, 2)), np.ones(40), np.repeat([25, 33], 20)]).astype(np.float32)
+    autotune.clear()
+    print(f"[case] tune influence wet gantry={gantry}", flush=True)
+    r = autotune.tune_influence_wet(rlsp, shape, s, np.zeros(3), iso, gantry, 0.0, m, spots, reps=1)
+    key = autotune.geom_key(shape, gantry, 0.0, (float(np.float32(m.VSADX)) + float(np.float32(m.VSADY))) / 2.0)
+    for kernel in ("raytrace_box", "smooth_box"):
+        assert autotune.TILES[autotune.get(kernel, key)] == r[kernel]["best"]
+    assert key == "30,36,32," + ("1" if gantry == 0.0 else "0")
+    autotune.clear()
+
+
 def test_tune_photon(photon_model):
     t = _torch()
     from dosecuda_amd import autotune, engine
