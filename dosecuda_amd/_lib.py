@@ -86,6 +86,8 @@ SIGNATURES = {
                            C.c_int32, _P, _P, _P, _P, _P, _P],
     "dh_photon_terma_dev": [_I, C.POINTER(Grid), C.POINTER(PhotonCP), C.POINTER(PhotonModel), _P,
                             C.c_int32, _P, C.c_float, _P, _P, _P],
+    "dh_photon_terma_batch_dev": [_I, C.POINTER(Grid), C.c_int32, C.POINTER(PhotonCP), C.POINTER(PhotonModel),
+                                  _P, _P, _P, C.c_float, _P, _P, _P],
     "dh_hu_convert_dev": [_I, _P, C.c_int32, C.c_int64, _P, _P, C.c_int32, _P, _P],
     "dh_stage_raytrace_dev": [_I, C.POINTER(Grid), _P, C.c_double, C.c_double, C.c_float,
                               C.c_float, _P, _P, _P],

@@ -365,11 +365,11 @@ static void bev_field_box(const dh_terma_args &ta, float box[4])
 }
 
 static bool bev_wet(int dev, const dh_terma_args &ta, const dh_geom &g, const float4 *vpad, const dh_photon_model *m,
-                    float *wet, hipStream_t s)
+                    float *wet, hipStream_t s, int table_slot = DH_SLOT_BEV)
 {
     float box[4];
     bev_field_box(ta, box);
-    return dh_launch_bev_wet(dev, g, vpad, bev_exact_depth(m), box, ta.sad, wet, s);
+    return dh_launch_bev_wet(dev, g, vpad, bev_exact_depth(m), box, ta.sad, wet, s, table_slot);
 }
 
 static void photon_device(int dev, const dh_grid *grid, const dh_photon_cp *cp,
@@ -441,6 +441,63 @@ extern "C" int dh_photon_terma_dev(int gpu_id, const dh_grid *grid, const dh_pho
         if (step_mm == 0.0f && !bev_wet(gpu_id, ta, ta.g, vpad, model, wet, s)) step_mm = std::max(1.0f, ta.g.s);
         if (step_mm > 0.0f) dh_launch_raytrace(ta.g, vpad, wet, s, -1, nullptr, step_mm);
         dh_launch_terma(ta, wet, d_terma_out, d_electron_out, s);
+        g_wet_memo[gpu_id & 63][dh_get_workspace()] = wet_memo{};   // the padded volume was rewritten
+    });
+}
+
+extern "C" int dh_photon_terma_batch_dev(int gpu_id, const dh_grid *grid, int32_t n, const dh_photon_cp *cps,
+                                         const dh_photon_model *model, const float *const *mlc_rows,
+                                         const int32_t *n_pairs, const float *d_density, float step_mm,
+                                         float *const *d_terma_out, float *const *d_electron_out, void *stream)
+{
+    return guarded([&] {
+        dh_device_enter(gpu_id);
+        DH_REQUIRE(n >= 1 && cps && model && mlc_rows && n_pairs && d_density && d_terma_out && d_electron_out,
+                   "null argument");
+        DH_REQUIRE(step_mm == 0.0f || (step_mm >= 1.0f && step_mm <= 16.0f),
+                   "ray-trace step must be 0 (beam's-eye-view grid) or 1..16 mm");
+        const size_t nb = sizeof(float) * nvox_of(grid);
+        std::vector<std::pair<uintptr_t, uintptr_t>> outs;
+        for (int i = 0; i < n; i++) {
+            DH_REQUIRE(d_terma_out[i] && d_electron_out[i], "null output volume");
+            outs.push_back({(uintptr_t)d_terma_out[i], (uintptr_t)d_terma_out[i] + nb});
+            outs.push_back({(uintptr_t)d_electron_out[i], (uintptr_t)d_electron_out[i] + nb});
+        }
+        const uintptr_t d0 = (uintptr_t)d_density, d1 = d0 + nb;
+        for (size_t a = 0; a < outs.size(); a++) {
+            DH_REQUIRE(!(outs[a].first < d1 && d0 < outs[a].second), "output volumes must not overlap the density");
+            for (size_t b = a + 1; b < outs.size(); b++)
+                DH_REQUIRE(!(outs[a].first < outs[b].second && outs[b].first < outs[a].second),
+                           "output volumes must not overlap each other");
+        }
+        std::vector<dh_terma_args> ta;
+        for (int i = 0; i < n; i++) ta.push_back(dh_make_terma_args(grid, &cps[i], model, mlc_rows[i], n_pairs[i]));
+        hipStream_t s = (hipStream_t)stream;
+        const dh_pad p = dh_make_pad(ta[0].g.nx, ta[0].g.ny, ta[0].g.nz);
+        float4 *vpad = (float4 *)dh_scratch(gpu_id, DH_SLOT_VOL_PAD, sizeof(float4) * (size_t)p.pxy * p.pz);
+        float *wet[2] = {(float *)dh_scratch(gpu_id, DH_SLOT_WET_RAW, nb),
+                         n > 1 ? (float *)dh_scratch(gpu_id, DH_SLOT_WET_RAW2, nb) : nullptr};
+        // the padded density once; the points alternate between the caller's
+        // stream and the side stream, each with its own depth table and volume
+        dh_launch_pad_quad(d_density, vpad, ta[0].g.nx, ta[0].g.ny, ta[0].g.nz, DH_AIR_DENSITY, s);  // IMRTClasses.cu:394
+        dh_side *side = n > 1 ? &dh_side_of(gpu_id) : nullptr;
+        if (side) {
+            DH_HIP(hipEventRecord(side->fork, s));
+            DH_HIP(hipStreamWaitEvent(side->st, side->fork, 0));
+        }
+        for (int i = 0; i < n; i++) {
+            const int k = side ? (i & 1) : 0;
+            hipStream_t st = k ? side->st : s;
+            float step = step_mm;
+            if (step == 0.0f && !bev_wet(gpu_id, ta[i], ta[i].g, vpad, model, wet[k], st, k ? DH_SLOT_BEV2 : DH_SLOT_BEV))
+                step = std::max(1.0f, ta[i].g.s);
+            if (step > 0.0f) dh_launch_raytrace(ta[i].g, vpad, wet[k], st, -1, nullptr, step);
+            dh_launch_terma(ta[i], wet[k], d_terma_out[i], d_electron_out[i], st);
+        }
+        if (side) {
+            DH_HIP(hipEventRecord(side->join, side->st));
+            DH_HIP(hipStreamWaitEvent(s, side->join, 0));
+        }
         g_wet_memo[gpu_id & 63][dh_get_workspace()] = wet_memo{};   // the padded volume was rewritten
     });
 }

@@ -60,6 +60,14 @@ constexpr int DH_MAX_WORKSPACES = 33;   // 0 = library default, 1.. scheduler st
 void *dh_scratch(int dev, int slot, size_t bytes);
 void dh_set_workspace(int ws);
 int dh_get_workspace();
+
+// A second stream per device (default priority) with fork / join events, for
+// entry points that spread a batch over the caller's stream and this one.
+struct dh_side {
+    hipStream_t st = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+};
+dh_side &dh_side_of(int dev);
 int dh_current_device();
 
 enum dh_slot {
@@ -85,6 +93,8 @@ enum dh_slot {
     DH_SLOT_NN_PLAIN2,
     DH_SLOT_TR_PAD2,
     DH_SLOT_BEV,           // beam's-eye-view depth table of the checked surrogate path
+    DH_SLOT_BEV2,          // second table and depth volume (batched points on the side stream)
+    DH_SLOT_WET_RAW2,
     DH_SLOT_COUNT_
 };
 
@@ -204,7 +214,7 @@ inline void dh_check_dilate_word(int dilate)
 // (head frame, mm; null: none) take the 1 mm ray trace.  Returns false
 // (nothing launched) where the grid does not apply.
 bool dh_launch_bev_wet(int dev, const dh_geom &g, const float4 *vol_pad, float w_exact_cm, const float *field_box,
-                       float sad, float *wet, hipStream_t s);
+                       float sad, float *wet, hipStream_t s, int table_slot = DH_SLOT_BEV);
 void dh_launch_terma(const dh_terma_args &a, const float *wet, float *terma, float *electron,
                      hipStream_t s);
 // Sampling layout of the cone superposition's {TERMA, density} volume:

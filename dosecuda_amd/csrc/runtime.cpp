@@ -77,6 +77,21 @@ hipStream_t dh_stream(int dev)
     return g_dev[dev].stream;
 }
 
+dh_side &dh_side_of(int dev)
+{
+    static std::mutex mu;
+    static dh_side side[kMaxDevices];
+    DH_REQUIRE(dev >= 0 && dev < kMaxDevices, "device index out of range");
+    std::lock_guard<std::mutex> lk(mu);
+    dh_side &sd = side[dev];
+    if (!sd.st) {
+        DH_HIP(hipStreamCreateWithFlags(&sd.st, hipStreamNonBlocking));
+        DH_HIP(hipEventCreateWithFlags(&sd.fork, hipEventDisableTiming));
+        DH_HIP(hipEventCreateWithFlags(&sd.join, hipEventDisableTiming));
+    }
+    return sd;
+}
+
 void dh_set_workspace(int ws)
 {
     if (ws < 0 || ws >= DH_MAX_WORKSPACES) throw dh_error(DH_E_INVALID, "workspace index out of range");

@@ -1276,28 +1276,6 @@ extern "C" int dh_sur_net_forward_dev(const dh_sur_net *net, const void *d_x, in
     });
 }
 
-// Side stream (default priority) and its fork / join events, per device:
-// batched points alternate between the caller's stream and this one.
-struct sur_side {
-    hipStream_t st = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
-};
-
-static sur_side &sur_side_of(int dev)
-{
-    static std::mutex mu;
-    static sur_side side[64];
-    DH_REQUIRE(dev >= 0 && dev < 64, "device index out of range");
-    std::lock_guard<std::mutex> lk(mu);
-    sur_side &sd = side[dev];
-    if (!sd.st) {
-        DH_HIP(hipStreamCreateWithFlags(&sd.st, hipStreamNonBlocking));
-        DH_HIP(hipEventCreateWithFlags(&sd.fork, hipEventDisableTiming));
-        DH_HIP(hipEventCreateWithFlags(&sd.join, hipEventDisableTiming));
-    }
-    return sd;
-}
-
 // Checked surrogate doses of ncp control points on one grid (one density,
 // per-point TERMA and electron volumes): each point is packed into its slot
 // of one batch and the network runs once per batch of up to kSurBatch
@@ -1332,12 +1310,12 @@ void dh_launch_sur_dose_batch(const dh_sur_net *net, const dh_ccc_args *cas, int
     int *work[2] = {(int *)dh_scratch(dev, DH_SLOT_NN_WORK, sizeof(int) * (size_t)wl), nullptr};
     float *plain[2] = {(float *)dh_scratch(dev, DH_SLOT_NN_PLAIN, nvb), nullptr};
     void *trp[2] = {dh_scratch(dev, DH_SLOT_TR_PAD, dh_tr_pad_bytes(cas[0])), nullptr};
-    sur_side *side = nullptr;
+    dh_side *side = nullptr;
     if (two) {
         work[1] = (int *)dh_scratch(dev, DH_SLOT_NN_WORK2, sizeof(int) * (size_t)wl);
         plain[1] = (float *)dh_scratch(dev, DH_SLOT_NN_PLAIN2, nvb);
         trp[1] = dh_scratch(dev, DH_SLOT_TR_PAD2, dh_tr_pad_bytes(cas[0]));
-        side = &sur_side_of(dev);
+        side = &dh_side_of(dev);
     }
     for (int c0 = 0; c0 < ncp; c0 += nb) {
         const int n = std::min(nb, ncp - c0);

@@ -279,6 +279,37 @@ def photon_terma(density: torch.Tensor, spacing, origin, iso, mu, ga, ca, ta, mo
     return terma_out, electron_out
 
 
+def photon_terma_batch(density: torch.Tensor, spacing, origin, cps, model, mlcs, step=1.0, terma_outs=None,
+                       electron_outs=None, stream=None):
+    """photon_terma of several control points of one grid and density
+    (cps: (iso, mu, ga, ca, ta) per point, mlcs: their apertures) in one call
+    of the library: the density is padded once and the points' depth and
+    TERMA launches alternate between the stream and a side stream.  Each
+    output equals photon_terma of that point bit for bit."""
+    dev = _dev(density)
+    n = len(cps)
+    if n == 0 or len(mlcs) != n:
+        raise ValueError("need one aperture per control point")
+    terma_outs = [torch.empty_like(density) for _ in cps] if terma_outs is None else list(terma_outs)
+    electron_outs = [torch.empty_like(density) for _ in cps] if electron_outs is None else list(electron_outs)
+    if len(terma_outs) != n or len(electron_outs) != n:
+        raise ValueError("need one TERMA and one electron volume per control point")
+    for v in (*terma_outs, *electron_outs):
+        if v.shape != density.shape or v.dtype != torch.float32 or not v.is_contiguous():
+            raise ValueError("volumes must be contiguous float32 of the density's shape")
+    keep: list = []
+    m = model.struct if isinstance(model, PhotonParams) else photon_args(model, keep)
+    rows = [np.ascontiguousarray(np.asarray(mlc, dtype=np.float32)[:, :4]) for mlc in mlcs]
+    arr = (_lib.PhotonCP * n)(*[_lib.photon_cp(*cp) for cp in cps])
+    P = lambda vs: (C.c_void_p * n)(*[v.data_ptr() for v in vs])
+    R = (C.c_void_p * n)(*[r.ctypes.data for r in rows])
+    npairs = (C.c_int32 * n)(*[r.shape[0] for r in rows])
+    _lib.check(_lib.load().dh_photon_terma_batch_dev(
+        dev, _g(density.shape, spacing, origin), n, arr, m, R, npairs, _lib.ptr(density), float(step),
+        P(terma_outs), P(electron_outs), _stream(stream)))
+    return terma_outs, electron_outs
+
+
 # ---- single stages (stage-wise parity) -----------------------------------
 def stage_raytrace(vol: torch.Tensor, spacing, origin, iso, gantry, couch, src_dist, border,
                    out=None, stream=None):

@@ -705,17 +705,16 @@ class SurrogateCCCS:
 
     @torch.no_grad()
     def plan_doses(self, density, spacing, origin, cps, mlcs, outs=None, work=None, step=0.0):
-        """Checked surrogate doses of a plan's control points on one grid: per
-        point the TERMA and electron volumes (engine.photon_terma; step 0: the
-        depth from the beam's-eye-view grid, > 0: a ray trace sampled every
-        step mm), then dose_batch.  cps: (iso, mu, ga, ca, ta) per point,
-        mlcs: their apertures.  work: optional lists (termas, electrons) of
-        volumes to reuse."""
+        """Checked surrogate doses of a plan's control points on one grid: the
+        points' TERMA and electron volumes (engine.photon_terma_batch; step
+        0: the depth from the beam's-eye-view grid, > 0: a ray trace sampled
+        every step mm), then dose_batch.  cps: (iso, mu, ga, ca, ta) per
+        point, mlcs: their apertures.  work: optional lists (termas,
+        electrons) of volumes to reuse."""
         termas, els = work if work is not None else ([torch.empty_like(density) for _ in cps],
                                                      [torch.empty_like(density) for _ in cps])
-        for cp, mlc, t, e in zip(cps, mlcs, termas, els):
-            engine.photon_terma(density, spacing, origin, *cp, self.model, mlc, step=step, terma_out=t,
-                                electron_out=e)
+        engine.photon_terma_batch(density, spacing, origin, cps, self.model, mlcs, step=step,
+                                  terma_outs=termas, electron_outs=els)
         return self.dose_batch(termas, density, els, spacing, origin, cps, outs=outs)
 
     @torch.no_grad()

@@ -164,6 +164,19 @@ int dh_photon_terma_dev(int gpu_id, const dh_grid *grid, const dh_photon_cp *cp,
                         const float *d_density, float step_mm, float *d_terma_out,
                         float *d_electron_out, void *stream);
 
+/* dh_photon_terma_dev for n control points of one grid and density (the
+ * per-point TERMA of IMRTClasses.cu:353-405, batched; no reference
+ * counterpart as a batch): the density is padded once and the points
+ * alternate between the caller's stream and a side stream, each with its own
+ * depth buffers, so their depth and TERMA launches overlap.  mlc_rows[i] /
+ * n_pairs[i] are point i's aperture.  Each output equals dh_photon_terma_dev
+ * of that point bit for bit.  Output volumes may not overlap each other or
+ * the density. */
+int dh_photon_terma_batch_dev(int gpu_id, const dh_grid *grid, int32_t n, const dh_photon_cp *cps,
+                              const dh_photon_model *model, const float *const *mlc_rows,
+                              const int32_t *n_pairs, const float *d_density, float step_mm,
+                              float *const *d_terma_out, float *const *d_electron_out, void *stream);
+
 /* ---- batched proton dose-influence matrix (CSR) ---------------------- */
 /* No reference counterpart: the reference only sums weighted spots
  * (IMPTClasses.cu:204-271).  Row q of the matrix is sorted spot q (energy-id

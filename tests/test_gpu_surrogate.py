@@ -601,3 +601,41 @@ def test_bev_depth_terma_close_to_raytrace(photon_model, ga, ca, ta):
         g.replay()
         t.cuda.synchronize()
         assert t.equal(out_t, eager)
+
+
+@pytest.mark.parametrize("step", [0.0, 2.5])
+def test_terma_batch_equals_single(photon_model, step):
+    """photon_terma_batch (density padded once, points alternating over two
+    streams with their own depth buffers) gives every point's TERMA and
+    electron volumes bit for bit as photon_terma of that point alone, eager
+    and from a HIP graph replayed between eager work; outputs that overlap
+    each other or the density are refused."""
+    t = _torch()
+    from dosecuda_amd import engine, phantoms, _lib
+    from dosecuda_amd.beam_models import square_field_mlc
+    pm = photon_model
+    case = phantoms.imrt_lung_bone_case(seed=1002, n=56, spacing=2.5)
+    rho = t.from_numpy(phantoms.to_density(case)).cuda()
+    mlcs = [square_field_mlc(pm, 8.0, 6.0), square_field_mlc(pm, 4.0, 10.0), square_field_mlc(pm, 10.0, 10.0)]
+    cps = [(case.iso, 100.0, 0.0, 0.0, 0.0), (case.iso, 80.0, 110.0, 15.0, 0.0), (case.iso, 50.0, 250.0, 0.0, 20.0)]
+    print(f"[case] terma batch step={step}", flush=True)
+    ref = [engine.photon_terma(rho, case.spacing, case.origin, *cp, pm, m, step=step) for cp, m in zip(cps, mlcs)]
+    ts, es = engine.photon_terma_batch(rho, case.spacing, case.origin, cps, pm, mlcs, step=step)
+    for i, (rt, re) in enumerate(ref):
+        assert t.equal(ts[i], rt) and t.equal(es[i], re), f"point {i}"
+    g = t.cuda.CUDAGraph()
+    with t.cuda.graph(g):
+        engine.photon_terma_batch(rho, case.spacing, case.origin, cps, pm, mlcs, step=step, terma_outs=ts,
+                                  electron_outs=es)
+    for r in range(3):
+        for v in (*ts, *es):
+            v.fill_(-1.0)
+        g.replay()
+        t.cuda.synchronize()
+        for i, (rt, re) in enumerate(ref):
+            assert t.equal(ts[i], rt) and t.equal(es[i], re), f"replay {r}, point {i}"
+        engine.photon_terma(rho, case.spacing, case.origin, *cps[0], pm, mlcs[0], step=step)
+    for bad in ((ts[0], ts[0]), (rho, es[1])):
+        with pytest.raises(ValueError, match="overlap"):
+            engine.photon_terma_batch(rho, case.spacing, case.origin, cps[:2], pm, mlcs[:2], step=step,
+                                      terma_outs=[ts[0], bad[0]], electron_outs=[es[0], bad[1]])
