@@ -11,6 +11,7 @@
 // fragment is one 16-byte load.  Padding is "same" (zero outside).
 #include "dh_runtime.h"
 
+#include <cstring>
 #include <mutex>
 
 #include <hip/hip_bf16.h>
@@ -661,8 +662,7 @@ __device__ static inline void block_max_to(float m, float *out)
 }
 
 // max of a non-negative volume (16-byte loads when the base allows)
-__global__ void __launch_bounds__(256)
-k_max_nonneg(const float *__restrict__ v, int64_t n, float *__restrict__ out)
+__device__ static inline void max_nonneg(const float *__restrict__ v, int64_t n, float *__restrict__ out)
 {
     float m = 0.0f;
     const int64_t n4 = ((uintptr_t)v & 15) == 0 ? n / 4 : 0;
@@ -674,6 +674,27 @@ k_max_nonneg(const float *__restrict__ v, int64_t n, float *__restrict__ out)
     }
     for (int64_t i = 4 * n4 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += step) m = fmaxf(m, v[i]);
     block_max_to(m, out);
+}
+
+__global__ void __launch_bounds__(256)
+k_max_nonneg(const float *__restrict__ v, int64_t n, float *__restrict__ out)
+{
+    max_nonneg(v, n, out);
+}
+
+// Per-point pointers of a batch of control points (blockIdx.y = point).
+constexpr int kPackBatchMax = 8;
+struct sur_pack_batch {
+    const float *terma[kPackBatchMax];
+    float *tmax[kPackBatchMax];
+    uint16_t *x[kPackBatchMax];
+    float dir[kPackBatchMax][3];
+};
+
+__global__ void __launch_bounds__(256)
+k_max_nonneg_batch(sur_pack_batch b, int64_t n)
+{
+    max_nonneg(b.terma[blockIdx.y], n, b.tmax[blockIdx.y]);
 }
 
 __global__ void k_fill_f32(float *p, int n, float v)
@@ -692,10 +713,9 @@ __global__ void k_fill_i32(int *p, int n, int v)
 // 0-7 TERMA / max, 8-15 density of the 2x2x2 sub-voxels (sub = 4 dz + 2 dy +
 // dx), 16-18 the beam direction, 19 the spacing channel, 20-31 zero.  One
 // thread writes one cell's 32 channels (four 16-byte stores).
-__global__ void __launch_bounds__(256)
-k_sur_pack(const float *__restrict__ terma, const float *__restrict__ dens, int nz, int ny,
-           int nx, int cz, int cy, int cx, const float *__restrict__ tmax, float d0,
-           float d1, float d2, float spc, uint16_t *__restrict__ out)
+__device__ static inline void sur_pack(const float *__restrict__ terma, const float *__restrict__ dens, int nz,
+                                       int ny, int nx, int cz, int cy, int cx, const float *__restrict__ tmax,
+                                       float d0, float d1, float d2, float spc, uint16_t *__restrict__ out)
 {
     const int64_t n = (int64_t)cz * cy * cx;
     const float scale = fmaxf(*tmax, 1e-12f);
@@ -726,6 +746,23 @@ k_sur_pack(const float *__restrict__ terma, const float *__restrict__ dens, int 
         o[2] = cst;
         o[3] = make_uint4(0, 0, 0, 0);
     }
+}
+
+__global__ void __launch_bounds__(256)
+k_sur_pack(const float *__restrict__ terma, const float *__restrict__ dens, int nz, int ny,
+           int nx, int cz, int cy, int cx, const float *__restrict__ tmax, float d0,
+           float d1, float d2, float spc, uint16_t *__restrict__ out)
+{
+    sur_pack(terma, dens, nz, ny, nx, cz, cy, cx, tmax, d0, d1, d2, spc, out);
+}
+
+__global__ void __launch_bounds__(256)
+k_sur_pack_batch(sur_pack_batch b, const float *__restrict__ dens, int nz, int ny, int nx, int cz, int cy, int cx,
+                 float spc)
+{
+    const int q = blockIdx.y;
+    sur_pack(b.terma[q], dens, nz, ny, nx, cz, cy, cx, b.tmax[q], b.dir[q][0], b.dir[q][1], b.dir[q][2], spc,
+             b.x[q]);
 }
 
 // Surrogate dose: the network's sub-voxel outputs (fp32 [cells][16],
@@ -1088,13 +1125,43 @@ void launch_pack(int nz, int ny, int nx, const float *terma, const float *densit
     DH_HIP(hipGetLastError());
 }
 
+// The packed inputs of n points of one grid and density (n <= 8) in one
+// launch per step: both scalars of every point zeroed ({max TERMA, max
+// dose} pairs in scal), the TERMA maxima, the packing.  Each point's result
+// equals launch_pack's.
+void launch_pack_batch(int nz, int ny, int nx, int n, const float *const *terma, const float *density,
+                       const float (*dirs)[3], float spacing, uint16_t *x, int64_t x_stride, float *scal,
+                       hipStream_t s)
+{
+    DH_REQUIRE(n >= 1 && n <= kPackBatchMax, "pack batch of 1..8 points");
+    const int64_t nv = (int64_t)nz * ny * nx;
+    const int cz = ((nz + 15) / 16) * 8, cy = ((ny + 15) / 16) * 8, cx = ((nx + 15) / 16) * 8;
+    sur_pack_batch b;
+    std::memset(&b, 0, sizeof(b));
+    for (int i = 0; i < n; i++) {
+        b.terma[i] = terma[i];
+        b.tmax[i] = scal + 2 * i;
+        b.x[i] = x + (size_t)i * x_stride;
+        for (int c = 0; c < 3; c++) b.dir[i][c] = dirs[i][c];
+    }
+    hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, s, scal, 2 * n, 0.0f);
+    hipLaunchKernelGGL(k_max_nonneg_batch, dim3((unsigned)std::min<int64_t>((nv / 4 + 255) / 256 + 1, 256), n),
+                       dim3(256), 0, s, b, nv);
+    const int64_t items = (int64_t)cz * cy * cx;
+    dh_timed t("sur_pack", s);
+    hipLaunchKernelGGL(k_sur_pack_batch, dim3((unsigned)std::min<int64_t>((items + 255) / 256, 8192), n), dim3(256), 0,
+                       s, b, density, nz, ny, nx, cz, cy, cx, spacing / kSpRef);
+    DH_HIP(hipGetLastError());
+}
+
+// zero_dmax false: the caller has zeroed *dmax (launch_pack_batch does)
 void launch_finish(int nz, int ny, int nx, const float *y32, const float *tmax, float out_scale,
                    const float *terma, const float *electron, float mu_scale, float *dose, float *dmax,
-                   hipStream_t s)
+                   hipStream_t s, bool zero_dmax = true)
 {
     const int64_t n = (int64_t)nz * ny * nx;
     const int cy = ((ny + 15) / 16) * 8, cx = ((nx + 15) / 16) * 8;
-    hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, s, dmax, 1, 0.0f);
+    if (zero_dmax) hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, s, dmax, 1, 0.0f);
     dh_timed t("sur_finish", s);
     const int64_t ncell = (int64_t)((nz + 1) / 2) * ((ny + 1) / 2) * ((nx + 1) / 2);
     hipLaunchKernelGGL(k_sur_finish, dim3((unsigned)std::min<int64_t>((ncell + 255) / 256, 512)), dim3(256), 0, s,
@@ -1156,6 +1223,7 @@ namespace {
 constexpr int kSurLayers = 15;
 constexpr int kSurOut = 16;      // head output channels (0-7 used)
 constexpr int kSurBatch = 8;     // control points per network batch
+static_assert(kSurBatch <= kPackBatchMax, "a network batch must fit one packing launch");
 
 // activation elements (bf16) of one forward over n samples of cz x cy x cx cells
 int64_t sur_act_elems(const dh_sur_net *net, int n, int cz, int cy, int cx)
@@ -1319,9 +1387,7 @@ void dh_launch_sur_dose_batch(const dh_sur_net *net, const dh_ccc_args *cas, int
     }
     for (int c0 = 0; c0 < ncp; c0 += nb) {
         const int n = std::min(nb, ncp - c0);
-        for (int i = 0; i < n; i++)
-            launch_pack(nz, ny, nx, terma[c0 + i], density, dirs[c0 + i], g.s, x + (size_t)i * cells * net->in_ch,
-                        scal + 2 * i, s);
+        launch_pack_batch(nz, ny, nx, n, terma + c0, density, dirs + c0, g.s, x, cells * net->in_ch, scal, s);
         sur_forward(net, x, n, cz, cy, cx, y32, ws, s);
         if (two && n > 1) {
             DH_HIP(hipEventRecord(side->fork, s));
@@ -1332,7 +1398,8 @@ void dh_launch_sur_dose_batch(const dh_sur_net *net, const dh_ccc_args *cas, int
             hipStream_t st = k ? side->st : s;
             const dh_ccc_args &ca = cas[c0 + i];
             launch_finish(nz, ny, nx, y32 + (size_t)i * cells * kSurOut, scal + 2 * i, net->out_scale,
-                          terma[c0 + i], electron[c0 + i], ca.mu_cal * ca.mu, dose[c0 + i], scal + 2 * i + 1, st);
+                          terma[c0 + i], electron[c0 + i], ca.mu_cal * ca.mu, dose[c0 + i], scal + 2 * i + 1, st,
+                          false);
             dh_launch_sur_check(ca, terma[c0 + i], density, electron[c0 + i], dose[c0 + i], scal + 2 * i + 1,
                                 stride, tol, dilate, work[k], plain[k], trp[k], st);
         }
