@@ -39,7 +39,7 @@ CCC_TRAFFIC_BYTES = (2.0 * 111437 + 32833) * 1024
 # `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE` runs of this bench with
 # the same calibration; commands and per-kernel figures in
 # profiles/kernel_stats_influence_3fields.md
-INFL_FILL_TRAFFIC_BYTES = (2.0 * 638586 + 2110335) * 1024
+INFL_FILL_TRAFFIC_BYTES = (2.0 * 639669 + 1829090) * 1024
 
 
 def parse():
