@@ -20,7 +20,10 @@ import torch
 
 from . import _lib, engine
 
-TILES = ["linear", "64x4x1", "32x8x1", "16x16x1", "32x4x2", "16x4x4", "8x8x4", "64x2x2"]
+# voxel tiles of the ray trace, smoothing and pencil launches (the library's
+# choice codes); "/y": the same bricks numbered y fastest
+TILES = ["linear", "64x4x1", "32x8x1", "16x16x1", "32x4x2", "16x4x4", "8x8x4", "64x2x2",
+         "16x16x1/y", "16x4x4/y", "8x8x4/y", "32x4x2/y"]
 
 
 def arch(device=0) -> str:
@@ -170,7 +173,7 @@ def geom_key(shape, gantry, couch, src_dist):
 
 
 def tune_influence_wet(rlsp: torch.Tensor, shape, spacing, origin, iso, gantry, couch, model, spots,
-                       choices=(5, 6, 3, 0), reps=3):
+                       choices=(5, 6, 3, 9, 10, 8), reps=3):
     """Voxel tiles of an influence matrix's footprint ray trace and smoothing
     ("raytrace_box", "smooth_box"): times InfluenceMatrix.wet() with each
     choice, requires the identical WET of every one and keeps the fastest."""

@@ -147,9 +147,12 @@ __device__ static inline int dh_xcd_block(int b, int nb)
 
 // Voxels of a 256-thread workgroup: linear runs along x (tx == 0, the
 // voxel index is b * 256 + thread) or a tx x ty x tz brick (product 256),
-// bricks numbered x fastest.  Returns whether the voxel is in the volume.
+// bricks numbered x fastest (yfirst = 0) or y fastest (yfirst = 1; then x,
+// then z).  The numbering decides which bricks run at the same time: for rays
+// along y, y-fastest bricks share their rays' columns, so their working set
+// stays within one XCD's L2.  Returns whether the voxel is in the volume.
 struct dh_tile {
-    int tx, ty, tz, ntx, nty;
+    int tx, ty, tz, ntx, nty, yfirst;
 };
 
 __device__ static inline bool dh_tile_voxel(const dh_tile &t, int b, int tid, int nx, int ny, int nz, int &i,
@@ -163,7 +166,18 @@ __device__ static inline bool dh_tile_voxel(const dh_tile &t, int b, int tid, in
         k = (int)(r / (uint32_t)ny);
         return v < (uint32_t)nx * ny * nz;
     }
-    const int bx = b % t.ntx, r = b / t.ntx, by = r % t.nty, bz = r / t.nty;
+    int bx, by, bz;
+    if (t.yfirst) {
+        by = b % t.nty;
+        const int r = b / t.nty;
+        bx = r % t.ntx;
+        bz = r / t.ntx;
+    } else {
+        bx = b % t.ntx;
+        const int r = b / t.ntx;
+        by = r % t.nty;
+        bz = r / t.nty;
+    }
     i = bx * t.tx + tid % t.tx;
     j = by * t.ty + (tid / t.tx) % t.ty;
     k = bz * t.tz + tid / (t.tx * t.ty);

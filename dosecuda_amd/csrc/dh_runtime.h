@@ -116,7 +116,7 @@ dh_geom dh_proton_geom(const dh_grid *grid, const dh_proton_beam *beam);
 
 // Voxel tiles of the voxel-parallel kernels: choice 0 = linear runs along x,
 // 1.. = bricks (see dh_tile); grid receives the workgroup count.
-constexpr int DH_TILE_CHOICES = 8;
+constexpr int DH_TILE_CHOICES = 12;
 dh_tile dh_make_tile(int choice, int nx, int ny, int nz, unsigned *grid);
 
 // Launch timing: when enabled, launches are bracketed by HIP events on the

@@ -179,13 +179,16 @@ std::string dh_tune_key_geom(const dh_geom &g)
 
 dh_tile dh_make_tile(int choice, int nx, int ny, int nz, unsigned *grid)
 {
-    static const int shapes[DH_TILE_CHOICES][3] = {{0, 0, 0},  {64, 4, 1}, {32, 8, 1}, {16, 16, 1},
-                                                   {32, 4, 2}, {16, 4, 4}, {8, 8, 4},  {64, 2, 2}};
+    // choices 8-11: bricks of choices 3, 5, 6 and 4 numbered y fastest
+    static const int shapes[DH_TILE_CHOICES][4] = {{0, 0, 0, 0},  {64, 4, 1, 0}, {32, 8, 1, 0}, {16, 16, 1, 0},
+                                                   {32, 4, 2, 0}, {16, 4, 4, 0}, {8, 8, 4, 0},  {64, 2, 2, 0},
+                                                   {16, 16, 1, 1}, {16, 4, 4, 1}, {8, 8, 4, 1}, {32, 4, 2, 1}};
     DH_REQUIRE(choice >= 0 && choice < DH_TILE_CHOICES, "tile choice out of range");
     dh_tile t;
     t.tx = shapes[choice][0];
     t.ty = shapes[choice][1];
     t.tz = shapes[choice][2];
+    t.yfirst = shapes[choice][3];
     if (t.tx == 0) {
         t.ntx = t.nty = 0;
         *grid = (unsigned)(((size_t)nx * ny * nz + 255) / 256);

@@ -440,19 +440,24 @@ int dh_tune_count(int32_t *n);
 int dh_tune_entry(int32_t index, char *kernel, int32_t kernel_cap, char *key, int32_t key_cap,
                   int32_t *choice, double *ms);
 int dh_tune_clear(void);
-/* Ray-trace and smoothing voxel tiles (8 choices each) for a proton field;
- * ms arrays hold 8 entries. */
+/* Voxel tile codes of the ray trace, smoothing and pencil launches (12):
+ * 0 linear runs along x; 1-7 bricks 64x4x1, 32x8x1, 16x16x1, 32x4x2, 16x4x4,
+ * 8x8x4, 64x2x2 numbered x fastest; 8-11 bricks 16x16x1, 16x4x4, 8x8x4,
+ * 32x4x2 numbered y fastest.  Every code gives the same result bit for bit.
+ * Ray-trace and smoothing voxel tiles (12 choices each) for a proton field;
+ * ms arrays hold 12 entries. */
 int dh_autotune_proton_wet(int gpu_id, const dh_grid *grid, const dh_proton_beam *beam,
                            const float *d_rlsp, int32_t reps, int32_t *best_raytrace,
                            int32_t *best_smooth, float *ms_raytrace, float *ms_smooth, void *stream);
-/* Photon control point: ray-trace voxel tile (8 choices) and the cone
+/* Photon control point: ray-trace voxel tile (12 choices) and the cone
  * superposition's {TERMA, density} sampling layout (0 float2 pairs, 1 float4
- * quads); ms_raytrace holds 8 entries, ms_ccc 2. */
+ * quads); ms_raytrace holds 12 entries, ms_ccc 2. */
 int dh_autotune_photon(int gpu_id, const dh_grid *grid, const dh_photon_cp *cp,
                        const dh_photon_model *model, const float *mlc_rows, int32_t n_pairs,
                        const float *d_density, int32_t reps, int32_t *best_raytrace,
                        int32_t *best_layout, float *ms_raytrace, float *ms_ccc, void *stream);
-/* Pencil-beam voxel tile x LUT staging (LDS or global): 16 choices. */
+/* Pencil-beam voxel tile x LUT staging (LDS or global): 24 choices (tile +
+ * 12 when the LUT is read from global memory). */
 int dh_autotune_pencil(dh_proton_plan *plan, const float *d_wet, int32_t reps, int32_t *best,
                        float *ms, void *stream);
 /* Convolution variant of one layer shape (codes: 1000 + nco for the LDS
