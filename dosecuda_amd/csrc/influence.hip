@@ -500,6 +500,112 @@ __global__ void k_zero_words(unsigned *p, size_t n)
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) p[t] = 0u;
 }
 
+__global__ void k_fill_words(unsigned *p, size_t n, unsigned v)
+{
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) p[t] = v;
+}
+
+// Per spot, on the device (no host round trip between the passes): the
+// sweep radius and threshold from the spot's peak (bits in peak_bits) and its
+// layer's maxima bnd[4 l + {0..3}] = {sigma_t, sigma_ht, P, H}.
+// mode 0 (PEAK sweep; peak_bits = the axis lower bound pa): threshold pa;
+// beyond the radius both Gaussian terms of every voxel are below pa / 2, so
+// no voxel there can exceed pa (3 sigma_ht + s when the axis met no voxel).
+// mode 1 (COUNT / FILL; peak_bits = the exact peak): threshold rel * peak;
+// beyond the radius both terms are below thr / 2; the layer's smallest
+// threshold goes to lthr_bits (float bits, atomicMin; thresholds > 0).
+__global__ void __launch_bounds__(256)
+k_spot_radius(int n, int mode, const int *__restrict__ spot_layer, const unsigned *__restrict__ peak_bits,
+              const float *__restrict__ bnd, float rel, float s, float *__restrict__ thr, float *__restrict__ radius,
+              unsigned *__restrict__ lthr_bits)
+{
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= n) return;
+    const int l = spot_layer[q];
+    const float v = __uint_as_float(peak_bits[q]);
+    if (mode == 0) {
+        thr[q] = v;
+        if (l < 0) {
+            radius[q] = 0.0f;
+            return;
+        }
+        const float st = bnd[4 * l], sht = bnd[4 * l + 1], Pm = bnd[4 * l + 2], Hm = bnd[4 * l + 3];
+        if (!(v > 0.0f)) {
+            radius[q] = 3.0f * sht + s;
+            return;
+        }
+        const float rp = Pm > 0.5f * v ? st * sqrtf(2.0f * logf(2.0f * Pm / v)) : 0.0f;
+        const float rh = Hm > 0.5f * v ? sht * sqrtf(2.0f * logf(2.0f * Hm / v)) : 0.0f;
+        radius[q] = 1.001f * fmaxf(rp, rh) + s;
+    } else {
+        if (l < 0 || !(v > 0.0f)) {
+            thr[q] = INFINITY;
+            radius[q] = 0.0f;
+            return;
+        }
+        const float t = rel * v;
+        thr[q] = t;
+        const float st = bnd[4 * l], sht = bnd[4 * l + 1], Pm = bnd[4 * l + 2], Hm = bnd[4 * l + 3];
+        const float rp = Pm > 0.5f * t ? st * sqrtf(2.0f * logf(2.0f * Pm / t)) : 0.0f;
+        const float rh = Hm > 0.5f * t ? sht * sqrtf(2.0f * logf(2.0f * Hm / t)) : 0.0f;
+        radius[q] = fmaxf(rp, rh) + s;
+        atomicMin(&lthr_bits[l], __float_as_uint(t));
+    }
+}
+
+// The sweeps' work blocks from the spots' radii, on the device: block b's
+// lateral box (its spot axes over the volume's head depths, widened by the
+// largest radius) and the image rows the box's prism covers; written to
+// position pos[b] of the block list (the order the persistent queue hands
+// them out in).
+struct blk_consts {
+    float cmin_x, cmax_x, cmin_y, cmax_y, hz_min, hz_max;
+};
+
+__global__ void __launch_bounds__(256)
+k_make_blocks(dh_geom g, int nb, int S, blk_consts k, const int *__restrict__ block_layer,
+              const int *__restrict__ block_n, const int *__restrict__ block_spots, const float *__restrict__ spots,
+              const float *__restrict__ radius, const int *__restrict__ pos, dh_infl_block *__restrict__ out)
+{
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= nb) return;
+    dh_infl_block B;
+    B.layer = block_layer[b];
+    B.first = b;
+    B.n = block_n[b];
+    B.pad_ = 0;
+    float x0 = 1e30f, x1 = -1e30f, y0 = 1e30f, y1 = -1e30f;
+    float e0 = 1e30f, e1 = -1e30f, f0 = 1e30f, f1 = -1e30f, Rb = 0.0f;
+    for (int q = 0; q < S; q++) {
+        const int id = block_spots[b * S + q];
+        if (id < 0) continue;
+        const float sx = spots[8 * id], sy = spots[8 * id + 1];
+        const float R = radius[id] * 1.002f + 0.5f * g.s;
+        const float ax0 = fminf(sx * k.cmin_x, sx * k.cmax_x), ax1 = fmaxf(sx * k.cmin_x, sx * k.cmax_x);
+        const float ay0 = fminf(sy * k.cmin_y, sy * k.cmax_y), ay1 = fmaxf(sy * k.cmin_y, sy * k.cmax_y);
+        x0 = fminf(x0, ax0 - R); x1 = fmaxf(x1, ax1 + R);
+        y0 = fminf(y0, ay0 - R); y1 = fmaxf(y1, ay1 + R);
+        e0 = fminf(e0, ax0); e1 = fmaxf(e1, ax1);
+        f0 = fminf(f0, ay0); f1 = fmaxf(f1, ay1);
+        Rb = fmaxf(Rb, R);
+    }
+    B.bx0 = x0; B.bx1 = x1; B.by0 = y0; B.by1 = y1;
+    B.ex0 = e0; B.ex1 = e1; B.ey0 = f0; B.ey1 = f1; B.R = Rb;
+    B.pad1_ = B.pad2_ = B.pad3_ = 0.0f;
+    float kmin = 1e30f, kmax = -1e30f, jmin = 1e30f, jmax = -1e30f;
+    for (int c = 0; c < 8; c++) {
+        float px, py, pz;
+        dh_head_to_img(g, (c & 1) ? x1 : x0, (c & 2) ? y1 : y0, (c & 4) ? k.hz_max : k.hz_min, px, py, pz);
+        const float fj = (py + g.iso[1]) / g.s, fk = (pz + g.iso[2]) / g.s;
+        jmin = fminf(jmin, fj); jmax = fmaxf(jmax, fj);
+        kmin = fminf(kmin, fk); kmax = fmaxf(kmax, fk);
+    }
+    B.j0 = max(0, (int)floorf(jmin) - 1); B.j1 = min(g.ny - 1, (int)ceilf(jmax) + 1);
+    B.k0 = max(0, (int)floorf(kmin) - 1); B.k1 = min(g.nz - 1, (int)ceilf(kmax) + 1);
+    if (B.j0 > B.j1 || B.k0 > B.k1) { B.j0 = 0; B.j1 = -1; B.k0 = 0; B.k1 = 0; }
+    out[pos[b]] = B;
+}
+
 // Smallest box in the head frame: min/max over the volume corners of the
 // head coordinates (for the divergence scaling of the spot axes).
 static void head_depth_range(const dh_geom &g, float &hz_min, float &hz_max)
@@ -552,13 +658,18 @@ struct dh_influence {
     unsigned long long *d_stats = nullptr;   // 4 counters per pass
     float4 *d_factors = nullptr;             // [layer][voxel]
     unsigned *d_row_bits = nullptr;          // [layer][row][4] per-row maxima (float bits)
+    float *d_bnd = nullptr;                  // [layer][4] maxima of sigma_t, sigma_ht, P, H
+    float *d_radius = nullptr;               // per spot: sweep radius of the last pass set up
+    unsigned *d_lthr = nullptr;              // per layer: smallest threshold (float bits)
+    int *d_block_layer = nullptr, *d_block_n = nullptr;
+    int *d_block_pos = nullptr;              // per block: its position in the queue order
     std::vector<float> peak;         // host copy after the peak pass
     std::vector<float> radius;       // truncation radius per spot (COUNT = FILL)
-    std::vector<float> bnd;          // per layer: sigma_t, sigma_ht, P, H maxima
     std::vector<int> order;          // caller's spot row of each matrix row
     std::vector<int64_t> offsets;    // host copy of the exclusive scan
     int64_t nnz = -1;
     int fill_blocks = -1;            // blocks of the prepared fill pass in d_blocks (-1: none)
+    bool counted_blocks = false;     // d_blocks holds the COUNT pass's blocks
     // footprint-limited WET (dh_influence_wet): the inner head-frame box
     // where the WET is exact, the RLSP it came from and the buffer it went to
     bool fp_on = false;
@@ -567,6 +678,8 @@ struct dh_influence {
     dh_hbox fp_inner{0.0f, 0.0f, 0.0f, 0.0f, 0};
     int64_t fp_fallbacks = 0;
 };
+
+static std::vector<dh_infl_block> make_blocks(const dh_influence *f, const std::vector<float> &r);
 
 dh_influence *dh_influence_make(int dev, const dh_geom &g, float vsadx, float vsady,
                                 const std::vector<dh_layer_k> &layers, const std::vector<float4> &luts,
@@ -648,6 +761,25 @@ dh_influence *dh_influence_make(int dev, const dh_geom &g, float vsadx, float vs
                                             (size_t)g.nx * g.ny * g.nz));
         DH_HIP(hipMalloc(&f->d_active, sizeof(int2) * std::max<size_t>(layers.size(), 1) * g.ny * g.nz));
         DH_HIP(hipMalloc(&f->d_row_bits, sizeof(unsigned) * 4 * std::max<size_t>(layers.size(), 1) * g.ny * g.nz));
+        DH_HIP(hipMalloc(&f->d_bnd, sizeof(float) * 4 * std::max<size_t>(layers.size(), 1)));
+        DH_HIP(hipMalloc(&f->d_radius, sizeof(float) * std::max(n_spots, 1)));
+        DH_HIP(hipMalloc(&f->d_lthr, sizeof(unsigned) * std::max<size_t>(layers.size(), 1)));
+        DH_HIP(hipMalloc(&f->d_block_layer, sizeof(int) * std::max<size_t>(nb, 1)));
+        DH_HIP(hipMalloc(&f->d_block_n, sizeof(int) * std::max<size_t>(nb, 1)));
+        DH_HIP(hipMalloc(&f->d_block_pos, sizeof(int) * std::max<size_t>(nb, 1)));
+        if (nb) {
+            DH_HIP(hipMemcpy(f->d_block_layer, f->block_layer.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
+            DH_HIP(hipMemcpy(f->d_block_n, f->block_n.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
+            // the queue order: largest estimated work first (block boxes at a
+            // radius of 3 sigma_max of the layer); every pass keeps this order
+            std::vector<float> r_est(n_spots, 0.0f);
+            for (int q = 0; q < n_spots; q++)
+                if (f->spot_layer[q] >= 0) r_est[q] = 3.0f * f->sig_max[f->spot_layer[q]] + g.s;
+            const std::vector<dh_infl_block> est = make_blocks(f, r_est);
+            std::vector<int> pos(nb);
+            for (size_t i = 0; i < nb; i++) pos[est[i].first] = (int)i;
+            DH_HIP(hipMemcpy(f->d_block_pos, pos.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
+        }
         DH_HIP(hipMalloc(&f->d_spot_layer, sizeof(int) * std::max(n_spots, 1)));
         if (n_spots)
             DH_HIP(hipMemcpy(f->d_spot_layer, f->spot_layer.data(), sizeof(int) * n_spots, hipMemcpyHostToDevice));
@@ -665,7 +797,8 @@ void dh_influence_free(dh_influence *f)
     (void)hipSetDevice(f->dev);
     void *ps[] = {f->d_layers, f->d_luts, f->d_spots, f->d_block_spots, f->d_blocks, f->d_thr,
                   f->d_peak, f->d_counts, f->d_spans, f->d_offsets, f->d_queue, f->d_active, f->d_stats,
-                  f->d_factors, f->d_spot_layer, f->d_row_bits};
+                  f->d_factors, f->d_spot_layer, f->d_row_bits, f->d_bnd, f->d_radius, f->d_lthr,
+                  f->d_block_layer, f->d_block_n, f->d_block_pos};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     delete f;
@@ -780,14 +913,42 @@ static void launch_kernels(dh_influence *f, int n_blocks, const float *d_wet, ui
     DH_HIP(hipGetLastError());
 }
 
-template <int MODE>
-static void launch(dh_influence *f, const std::vector<dh_infl_block> &blocks, const float *d_wet,
-                   uint32_t *d_col, float *d_val, hipStream_t s)
+// The work blocks of the next pass from the spots' radii in d_radius, on
+// the device (stream-ordered: no host copy, no synchronisation).
+static void device_blocks(dh_influence *f, hipStream_t s)
 {
-    DH_HIP(hipMemcpyAsync(f->d_blocks, blocks.data(), sizeof(dh_infl_block) * blocks.size(),
-                          hipMemcpyHostToDevice, s));
-    f->fill_blocks = -1;      // d_blocks no longer holds the prepared fill blocks
-    launch_kernels<MODE>(f, (int)blocks.size(), d_wet, d_col, d_val, s);
+    const int nb = (int)f->block_layer.size();
+    f->fill_blocks = -1;
+    f->counted_blocks = false;
+    if (!nb) return;
+    float hz_min, hz_max;
+    head_depth_range(f->g, hz_min, hz_max);
+    blk_consts k;
+    k.cmin_x = std::min(1.0f - hz_max / f->vsadx, 1.0f - hz_min / f->vsadx);
+    k.cmax_x = std::max(1.0f - hz_max / f->vsadx, 1.0f - hz_min / f->vsadx);
+    k.cmin_y = std::min(1.0f - hz_max / f->vsady, 1.0f - hz_min / f->vsady);
+    k.cmax_y = std::max(1.0f - hz_max / f->vsady, 1.0f - hz_min / f->vsady);
+    k.hz_min = hz_min;
+    k.hz_max = hz_max;
+    hipLaunchKernelGGL(k_make_blocks, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, f->g, nb, f->S, k,
+                       f->d_block_layer, f->d_block_n, f->d_block_spots, f->d_spots, f->d_radius, f->d_block_pos,
+                       f->d_blocks);
+    DH_HIP(hipGetLastError());
+}
+
+// Per-spot radii and thresholds of a pass on the device (k_spot_radius).
+static void device_radius(dh_influence *f, int mode, hipStream_t s)
+{
+    const int n = f->n_spots;
+    if (!n) return;
+    if (mode == 1) {
+        const size_t nl = std::max<size_t>(f->layers.size(), 1);
+        hipLaunchKernelGGL(k_fill_words, dim3(1), dim3(256), 0, s, f->d_lthr, nl, 0x7f800000u);
+    }
+    hipLaunchKernelGGL(k_spot_radius, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, mode, f->d_spot_layer,
+                       (const unsigned *)f->d_peak, (const float *)f->d_bnd, f->rel, f->g.s, f->d_thr, f->d_radius,
+                       f->d_lthr);
+    DH_HIP(hipGetLastError());
 }
 
 // Radius beyond which every voxel of layer l, whatever its WET, has a dose
@@ -924,7 +1085,7 @@ static void wet_bounds(dh_influence *f, const float *d_wet, hipStream_t s)
     const size_t nvox = (size_t)f->g.nx * f->g.ny * f->g.nz;
     const unsigned nblk = (unsigned)((nvox + 255) / 256);
     float *d_part = (float *)dh_scratch(f->dev, DH_SLOT_TMP0, sizeof(float) * 4 * nl * nblk);
-    float *d_b = (float *)dh_scratch(f->dev, DH_SLOT_TMP1, sizeof(float) * 4 * nl);
+    float *d_b = f->d_bnd;
     {
         dh_timed t("influence_bounds", s);
         const size_t nrb = 4 * nl * (size_t)f->g.ny * f->g.nz;
@@ -936,9 +1097,6 @@ static void wet_bounds(dh_influence *f, const float *d_wet, hipStream_t s)
         hipLaunchKernelGGL(k_reduce_bounds, dim3((unsigned)(4 * nl)), dim3(256), 0, s, d_part, (int)nblk, d_b);
         DH_HIP(hipGetLastError());
     }
-    f->bnd.assign(4 * nl, 0.0f);
-    DH_HIP(hipMemcpyAsync(f->bnd.data(), d_b, sizeof(float) * 4 * nl, hipMemcpyDeviceToHost, s));
-    DH_HIP(hipStreamSynchronize(s));
 }
 
 int64_t dh_influence_count(dh_influence *f, const float *d_wet, int64_t base, hipStream_t s)
@@ -949,13 +1107,15 @@ int64_t dh_influence_count(dh_influence *f, const float *d_wet, int64_t base, hi
     hipLaunchKernelGGL(k_zero_words, dim3(64), dim3(256), 0, s, f->d_peak, (size_t)std::max(n, 1));
     // Peak in two steps.  k_axis_peak evaluates the voxels nearest to each
     // spot axis; their maximum pk_a is a value of the row, so a lower bound of
-    // the peak.  At lateral distance d every voxel's dose is
-    // at most Pmax e^(-d^2 / 2 st_max^2) + Hmax e^(-d^2 / 2 sht_max^2) (the
-    // layer's maxima), so beyond the radius where both terms are below pk_a / 2
-    // no voxel can exceed pk_a: the sweep covers that radius only, and the
-    // maximum of both is the exact peak.  A spot whose axis met no active
-    // voxel keeps the generous radius 3 sht_max + s.
-    std::vector<float> r(n, 0.0f);
+    // the peak.  At lateral distance d every voxel's dose is at most
+    // Pmax e^(-d^2 / 2 st_max^2) + Hmax e^(-d^2 / 2 sht_max^2) (the layer's
+    // maxima), so beyond the radius where both terms are below pk_a / 2 no
+    // voxel can exceed pk_a: the PEAK sweep covers that radius only, and the
+    // maximum of both is the exact peak.  Radii, thresholds and the sweeps'
+    // blocks are computed on the device between the passes, so the bounds,
+    // peak and count passes of a field run without a host synchronisation;
+    // the one synchronisation brings back the counts (the caller sizes its
+    // arrays with them), the thresholds and the radii.
     if (n) {
         float hz0, hz1;
         head_depth_range(f->g, hz0, hz1);
@@ -964,51 +1124,26 @@ int64_t dh_influence_count(dh_influence *f, const float *d_wet, int64_t base, hi
                            f->d_spot_layer, f->d_layers, f->d_spots, d_wet, f->d_factors, hz0, hz1, f->d_peak);
         DH_HIP(hipGetLastError());
     }
+    device_radius(f, 0, s);
+    device_blocks(f, s);
+    launch_kernels<INF_PEAK>(f, (int)f->block_layer.size(), d_wet, nullptr, nullptr, s);
+    device_radius(f, 1, s);
+    device_blocks(f, s);
+    launch_kernels<INF_COUNT>(f, (int)f->block_layer.size(), d_wet, nullptr, nullptr, s);
+    std::vector<int> cnt((size_t)n * f->C);
     std::vector<unsigned> pk(std::max(n, 1));
-    DH_HIP(hipMemcpyAsync(pk.data(), f->d_peak, sizeof(unsigned) * n, hipMemcpyDeviceToHost, s));
-    DH_HIP(hipStreamSynchronize(s));
-    for (int q = 0; q < n; q++) {
-        const int l = f->spot_layer[q];
-        if (l < 0) { r[q] = 0.0f; continue; }
-        float pa;
-        std::memcpy(&pa, &pk[q], 4);
-        const float st = f->bnd[4 * l], sht = f->bnd[4 * l + 1];
-        const float Pm = f->bnd[4 * l + 2], Hm = f->bnd[4 * l + 3];
-        if (!(pa > 0.0f)) {
-            r[q] = 3.0f * sht + f->g.s;
-            continue;
-        }
-        const float rp = Pm > 0.5f * pa ? st * std::sqrt(2.0f * std::log(2.0f * Pm / pa)) : 0.0f;
-        const float rh = Hm > 0.5f * pa ? sht * std::sqrt(2.0f * std::log(2.0f * Hm / pa)) : 0.0f;
-        r[q] = 1.001f * std::max(rp, rh) + f->g.s;
+    std::vector<float> thr(std::max(n, 1)), r(std::max(n, 1));
+    if (n) {
+        DH_HIP(hipMemcpyAsync(cnt.data(), f->d_counts, sizeof(int) * cnt.size(), hipMemcpyDeviceToHost, s));
+        DH_HIP(hipMemcpyAsync(pk.data(), f->d_peak, sizeof(unsigned) * n, hipMemcpyDeviceToHost, s));
+        DH_HIP(hipMemcpyAsync(thr.data(), f->d_thr, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+        DH_HIP(hipMemcpyAsync(r.data(), f->d_radius, sizeof(float) * n, hipMemcpyDeviceToHost, s));
     }
-    // the PEAK sweep's per-row radius threshold: each spot's peak lower bound
-    {
-        std::vector<float> pa(std::max(n, 1), 0.0f);
-        for (int q = 0; q < n; q++) std::memcpy(&pa[q], &pk[q], 4);
-        DH_HIP(hipMemcpyAsync(f->d_thr, pa.data(), sizeof(float) * n, hipMemcpyHostToDevice, s));
-        DH_HIP(hipStreamSynchronize(s));
-    }
-    launch<INF_PEAK>(f, make_blocks(f, r), d_wet, nullptr, nullptr, s);
-    DH_HIP(hipMemcpyAsync(pk.data(), f->d_peak, sizeof(unsigned) * n, hipMemcpyDeviceToHost, s));
     DH_HIP(hipStreamSynchronize(s));
+    thr.resize(n);
+    r.resize(n);
     f->peak.assign(n, 0.0f);
-    std::vector<float> thr(n, INFINITY);
-    for (int q = 0; q < n; q++) {
-        float v;
-        std::memcpy(&v, &pk[q], 4);
-        f->peak[q] = v;
-        const int l = f->spot_layer[q];
-        if (l < 0 || !(v > 0.0f)) { r[q] = 0.0f; continue; }
-        thr[q] = f->rel * v;
-        // beyond R both Gaussian terms are below thr / 2:
-        // Pmax e^(-R^2 / 2 st^2) <= thr/2 and Hmax e^(-R^2 / 2 sht^2) <= thr/2
-        const float st = f->bnd[4 * l], sht = f->bnd[4 * l + 1];
-        const float Pm = f->bnd[4 * l + 2], Hm = f->bnd[4 * l + 3];
-        const float rp = Pm > 0.5f * thr[q] ? st * std::sqrt(2.0f * std::log(2.0f * Pm / thr[q])) : 0.0f;
-        const float rh = Hm > 0.5f * thr[q] ? sht * std::sqrt(2.0f * std::log(2.0f * Hm / thr[q])) : 0.0f;
-        r[q] = std::max(rp, rh) + f->g.s;
-    }
+    for (int q = 0; q < n; q++) std::memcpy(&f->peak[q], &pk[q], 4);
     for (size_t l = 0; l < f->lthr.size(); l++) f->lthr[l] = INFINITY;
     for (int q = 0; q < n; q++)
         if (f->spot_layer[q] >= 0) f->lthr[f->spot_layer[q]] = std::min(f->lthr[f->spot_layer[q]], thr[q]);
@@ -1019,11 +1154,6 @@ int64_t dh_influence_count(dh_influence *f, const float *d_wet, int64_t base, hi
         field_wet(f, f->fp_rlsp, const_cast<float *>(d_wet), nullptr, s);
         return dh_influence_count(f, d_wet, base, s);
     }
-    DH_HIP(hipMemcpyAsync(f->d_thr, thr.data(), sizeof(float) * n, hipMemcpyHostToDevice, s));
-    launch<INF_COUNT>(f, make_blocks(f, r), d_wet, nullptr, nullptr, s);
-    std::vector<int> cnt((size_t)n * f->C);
-    if (n) DH_HIP(hipMemcpyAsync(cnt.data(), f->d_counts, sizeof(int) * cnt.size(), hipMemcpyDeviceToHost, s));
-    DH_HIP(hipStreamSynchronize(s));
     f->offsets.assign(cnt.size() + 1, 0);
     int64_t acc = base;
     for (int q = 0; q < n; q++) {
@@ -1036,7 +1166,19 @@ int64_t dh_influence_count(dh_influence *f, const float *d_wet, int64_t base, hi
     f->offsets[cnt.size()] = acc;
     f->nnz = acc - base;
     f->radius = r;
+    f->counted_blocks = true;      // d_blocks holds the COUNT pass's blocks, d_radius its radii
     return f->nnz;
+}
+
+// The COUNT pass's blocks, rebuilt on the device from its radii unless
+// d_blocks still holds them.
+static void count_blocks(dh_influence *f, hipStream_t s)
+{
+    if (f->counted_blocks) return;
+    const int n = f->n_spots;
+    if (n) DH_HIP(hipMemcpyAsync(f->d_radius, f->radius.data(), sizeof(float) * n, hipMemcpyHostToDevice, s));
+    device_blocks(f, s);
+    f->counted_blocks = true;
 }
 
 void dh_influence_fill(dh_influence *f, const float *d_wet, uint32_t *d_col, float *d_val, hipStream_t s)
@@ -1044,10 +1186,11 @@ void dh_influence_fill(dh_influence *f, const float *d_wet, uint32_t *d_col, flo
     DH_REQUIRE(f->nnz >= 0, "dh_influence_count must run before dh_influence_fill");
     const size_t n = f->offsets.size();
     if (n) DH_HIP(hipMemcpyAsync(f->d_offsets, f->offsets.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, s));
-    // the same boxes as the COUNT pass, so every entry lands in its slot;
+    // the same blocks as the COUNT pass, so every entry lands in its slot;
     // the active rows and factor planes of the COUNT pass are reused: the
     // caller passes the same WET to count and fill
-    launch<INF_FILL>(f, make_blocks(f, f->radius), d_wet, d_col, d_val, s);
+    count_blocks(f, s);
+    launch_kernels<INF_FILL>(f, (int)f->block_layer.size(), d_wet, d_col, d_val, s);
 }
 
 void dh_influence_fill_prepare(dh_influence *f, hipStream_t s)
@@ -1055,11 +1198,9 @@ void dh_influence_fill_prepare(dh_influence *f, hipStream_t s)
     DH_REQUIRE(f->nnz >= 0, "dh_influence_count must run before the fill is prepared");
     const size_t n = f->offsets.size();
     if (n) DH_HIP(hipMemcpyAsync(f->d_offsets, f->offsets.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, s));
-    std::vector<dh_infl_block> blocks = make_blocks(f, f->radius);
-    DH_HIP(hipMemcpyAsync(f->d_blocks, blocks.data(), sizeof(dh_infl_block) * blocks.size(),
-                          hipMemcpyHostToDevice, s));
+    count_blocks(f, s);
     DH_HIP(hipStreamSynchronize(s));
-    f->fill_blocks = (int)blocks.size();
+    f->fill_blocks = (int)f->block_layer.size();
 }
 
 void dh_influence_fill_enqueue(dh_influence *f, const float *d_wet, uint32_t *d_col, float *d_val, hipStream_t s)
