@@ -51,7 +51,9 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=15.0,
                    help="approximate budget of the CPU baseline sample")
     p.add_argument("--workload", default="all",
-                   choices=["all", "cccs", "impt", "influence", "surrogate", "mixed"])
+                   choices=["all", "cccs", "impt", "influence", "surrogate", "surrogate_path", "mixed"],
+                   help="surrogate_path (not part of all): only the whole checked surrogate path of the "
+                        "7-beam plan as a replayed HIP graph, --steps replays, for kernel traces")
     p.add_argument("--influence-steps", type=int, default=3)
     p.add_argument("--no-tune", action="store_true",
                    help="skip the autotuners (launch shapes from the cache or the defaults)")
@@ -718,6 +720,14 @@ def main():
                 f"min beam {r['min_gamma_checked']:.5f}, fallback {r['mean_fallback_fraction']:.3f}, "
                 f"MFMA {tf:.0f} TFLOP/s")
 
+    if a.workload == "surrogate_path":
+        from dosecuda_amd import surrogate as S
+        from dosecuda_amd.beam_models import PhotonBeamModel
+        r = S.time_plan_path(S.load(S.WEIGHTS), PhotonBeamModel(), reps=a.steps)
+        res["surrogate_path"] = r
+        log(f"surrogate path: {r['ms_per_beam']:.4f} ms/beam over {a.steps} replays, min beam gamma "
+            f"{r['min_gamma']:.5f}")
+
     if a.workload in ("all", "mixed"):
         from dosecuda_amd import mixed, phantoms, surrogate as S
         from dosecuda_amd.beam_models import PhotonBeamModel, ProtonBeamModel
@@ -794,6 +804,7 @@ def main():
         "cccs_detail": cc,
         "influence": res.get("influence"),
         "surrogate": res.get("surrogate"),
+        "surrogate_path": res.get("surrogate_path"),
         "mixed": res.get("mixed"),
         "autotuned": not a.no_tune,
     }

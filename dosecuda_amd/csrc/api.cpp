@@ -614,10 +614,8 @@ extern "C" int dh_sur_check_dev(int gpu_id, const dh_grid *grid, const dh_photon
         DH_REQUIRE(work_len >= dh_sur_check_work_len(grid->nz, grid->ny, grid->nx, stride),
                    "work buffer too small (dh_sur_check_work_size)");
         dh_ccc_args ca = dh_make_ccc_args(grid, cp, model);
-        dh_pad p = ca.p;
-        void *trp = dh_scratch(gpu_id, DH_SLOT_TR_PAD, dh_tr_pad_bytes(ca));
-        dh_launch_sur_check(ca, d_terma, d_density, d_electron, d_dose, d_dmax, stride, tol, dilate, d_work,
-                            d_plain, trp, (hipStream_t)stream);
+        dh_launch_sur_check(gpu_id, ca, d_terma, d_density, d_electron, d_dose, d_dmax, stride, tol, dilate, d_work,
+                            d_plain, (hipStream_t)stream);
     });
 }
 
@@ -672,9 +670,8 @@ extern "C" int dh_sur_dose_batch_dev(const dh_sur_net *net, const dh_grid *grid,
         dh_device_enter(dh_sur_net_gpu(net));
         const size_t bytes = sizeof(float) * nvox_of(grid);
         {
-            // outputs are written while other points' inputs are still read
-            // (and from two streams): no output may overlap another output
-            // or any input volume
+            // outputs are written while other points' inputs are still
+            // read: no output may overlap another output or any input volume
             std::vector<std::pair<uintptr_t, int>> r;   // (start, 1 = output)
             for (int i = 0; i < n_cp; i++) {
                 r.push_back({(uintptr_t)d_dose_out[i], 1});

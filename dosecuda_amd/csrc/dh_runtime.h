@@ -89,9 +89,7 @@ enum dh_slot {
     DH_SLOT_NN_WORK,       // surrogate check: counters, flags, lists
     DH_SLOT_NN_PLAIN,      // surrogate check: plain values on the lattice
     DH_SLOT_NN_SCAL,       // surrogate scalars (max TERMA, max dose)
-    DH_SLOT_NN_WORK2,      // second check set (batched points on the side stream)
-    DH_SLOT_NN_PLAIN2,
-    DH_SLOT_TR_PAD2,
+    DH_SLOT_CHK_TR,        // surrogate check: per point packed {TERMA, density} and cone partials
     DH_SLOT_BEV,           // beam's-eye-view depth table of the checked surrogate path
     DH_SLOT_BEV2,          // second table and depth volume (batched points on the side stream)
     DH_SLOT_WET_RAW2,
@@ -171,7 +169,44 @@ struct dh_ccc_args {
     int nsteps[6];
     float L0[6], a[6], b[6], c1[6], c2p[6], bp[6], a2[6];
     float dir[72 * 3];              // cone directions in image space
+    float hdir[72 * 3];             // the same in the head frame (before dh_head_to_img)
 };
+
+// Cone superposition of up to DH_CCC_MULTI_MAX control points' voxel lists on
+// one grid in one launch (k_ccc_multi): the model's row parameters and
+// head-frame cone directions once, then per point its geometry, MU and
+// buffers.  tr: the point's packed {TERMA, density} (pairs layout, written by
+// dh_launch_pack_tr_multi from terma and the shared density); part: 12 floats
+// per voxel of the grid; out: the dose of the listed voxels.
+constexpr int DH_CCC_MULTI_MAX = 8;
+struct dh_ccc_multi {
+    dh_pad p;
+    float sp, mu_cal;
+    int nsteps[6];
+    float L0[6], a[6], b[6], c1[6], c2p[6], bp[6], a2[6];
+    float hdir[72 * 3];
+    uint32_t nvox, cap;
+    int n;
+    dh_geom g[DH_CCC_MULTI_MAX];
+    float mu[DH_CCC_MULTI_MAX];
+    const void *tr[DH_CCC_MULTI_MAX];
+    const float *terma[DH_CCC_MULTI_MAX];
+    const float *electron[DH_CCC_MULTI_MAX];
+    const int *list[DH_CCC_MULTI_MAX];
+    const int *counter[DH_CCC_MULTI_MAX];
+    float *part[DH_CCC_MULTI_MAX];
+    float *out[DH_CCC_MULTI_MAX];
+};
+// the shared fields from cas[0] (every point must share grid and model) and
+// each point's geometry and MU; buffers are left null
+void dh_ccc_multi_init(dh_ccc_multi &m, const dh_ccc_args *cas, int n);
+// bytes of one point's packed volume and of its partial sums
+size_t dh_ccc_multi_tr_bytes(const dh_ccc_args &a);
+size_t dh_ccc_multi_part_bytes(const dh_ccc_args &a);
+void dh_launch_pack_tr_multi(const dh_ccc_multi &m, const float *density, hipStream_t s);
+// cone kernel + finish over each point's first *counter[q] list entries (at
+// most max_n per point, which sizes the grids)
+void dh_launch_ccc_multi(const char *timer, dh_ccc_multi m, int64_t max_n, hipStream_t s);
 
 // Lateral box in the head (beam's-eye-view) frame: voxels whose head x, y lie
 // outside [x0, x1] x [y0, y1] are skipped by the ray trace (nothing written)
@@ -287,9 +322,11 @@ void dh_launch_sur_dose_batch(const dh_sur_net *net, const dh_ccc_args *cas, int
                               const float *const *terma, const float *density, const float *const *electron,
                               const float (*dirs)[3], int stride, float tol, int dilate, float *const *dose,
                               hipStream_t s);
-void dh_launch_sur_check(const dh_ccc_args &a, const float *terma, const float *density,
+// Lattice check and plain fallback of one surrogate dose with the caller's
+// work (dh_sur_check_work_len ints) and plain (one float per voxel) buffers.
+void dh_launch_sur_check(int dev, const dh_ccc_args &a, const float *terma, const float *density,
                          const float *electron, float *dose, const float *dmax, int stride,
-                         float tol, int dilate, int *work, float *plain, void *tr_pad, hipStream_t s);
+                         float tol, int dilate, int *work, float *plain, hipStream_t s);
 
 // A proton field's spots sorted by energy id (stable), its layers
 // (importLayers) and one 400-entry LUT row per layer.  spots holds 8 floats

@@ -208,7 +208,11 @@ dh_ccc_args dh_make_ccc_args(const dh_grid *grid, const dh_photon_cp *cp, const 
             const float ang = (float)az * DH_PI_F / 6.0f;      // IMRTClasses.cu:286-288
             const float sx = sinf(ang), cx = cosf(ang);
             float dx, dy, dz;
-            dh_head_to_img(a.g, sth * cx, sth * sx, cth, dx, dy, dz);
+            const float hx = sth * cx, hy = sth * sx;
+            a.hdir[3 * (r * 12 + az) + 0] = hx;
+            a.hdir[3 * (r * 12 + az) + 1] = hy;
+            a.hdir[3 * (r * 12 + az) + 2] = cth;
+            dh_head_to_img(a.g, hx, hy, cth, dx, dy, dz);
             a.dir[3 * (r * 12 + az) + 0] = dx;
             a.dir[3 * (r * 12 + az) + 1] = dy;
             a.dir[3 * (r * 12 + az) + 2] = dz;

@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 namespace {
 
@@ -201,9 +202,8 @@ __device__ static inline float2 tr_at(const float *__restrict__ terma, const flo
 }
 
 template <bool Quad>
-__global__ void __launch_bounds__(kBlock)
-k_pack_tr(const float *__restrict__ terma, const float *__restrict__ dens, int nx, int ny, int nz,
-          dh_pad p, void *__restrict__ outv)
+__device__ static inline void pack_tr_body(const float *__restrict__ terma, const float *__restrict__ dens, int nx,
+                                           int ny, int nz, const dh_pad &p, void *__restrict__ outv)
 {
     const uint32_t n = (uint32_t)(p.pxy * p.pz);
     for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < n; t += gridDim.x * kBlock) {
@@ -222,6 +222,14 @@ k_pack_tr(const float *__restrict__ terma, const float *__restrict__ dens, int n
             o[1] = make_float4(a.y, b.y - a.y, c.y, d.y - c.y);
         }
     }
+}
+
+template <bool Quad>
+__global__ void __launch_bounds__(kBlock)
+k_pack_tr(const float *__restrict__ terma, const float *__restrict__ dens, int nx, int ny, int nz,
+          dh_pad p, void *__restrict__ outv)
+{
+    pack_tr_body<Quad>(terma, dens, nx, ny, nz, p, outv);
 }
 
 // Bytes per padded element of a layout.
@@ -369,6 +377,79 @@ constexpr int kGroups = 12;
 // k_ccc_finish adds an azimuth's six rows in the same compensated order as
 // an azimuth item does, so the dose is bit for bit the same.  The choice is
 // made on the device from the list's length (uniform over the grid).
+// One work item for the voxel v (one lane of the item's 64): azimuth grp,
+// zenith rows row0 .. row1 - 1, the rows' results summed with Neumaier
+// compensation (rows: a single row, returned as is).  R supplies the rows'
+// kernel parameters (nsteps, L0, a, b, c1, c2p, bp, a2) and sp; dirf(cone,
+// dx, dy, dz) the cone direction in image space; (ax, ay, az) is the unit
+// vector from the isocentre to the source.
+template <bool Quad, bool Off32, class R, class Dir>
+__device__ static inline float ccc_item(const R &a, const dh_geom &g, const dh_pad &p, const tr_bases &B, float ax,
+                                        float ay, float az, uint32_t v, int grp, int row0, int row1, bool rows,
+                                        Dir dirf)
+{
+    const float fnx = (float)g.nx, fny = (float)g.ny, fnz = (float)g.nz;
+    const int i = (int)(v % (uint32_t)g.nx);
+    const uint32_t r = v / (uint32_t)g.nx;
+    const int j = (int)(r % (uint32_t)g.ny);
+    const int k = (int)(r / (uint32_t)g.ny);
+    float x, y, z, ux, uy, uz;
+    dh_vox_xyz(g, i, j, k, x, y, z);
+    dh_uvec(g, x, y, z, ux, uy, uz);
+    // texel-space origin of the voxel (u = t - 0.5)
+    const float ucx = fmaf(g.inv_s, x + g.iso[0], 0.5f) - 0.5f;
+    const float ucy = fmaf(g.inv_s, y + g.iso[1], 0.5f) - 0.5f;
+    const float ucz = fmaf(g.inv_s, z + g.iso[2], 0.5f) - 0.5f;
+    // Rodrigues tilt taking the iso->source axis onto u
+    const float c = ax * ux + ay * uy + az * uz;
+    const float kx = ay * uz - az * uy;
+    const float ky = az * ux - ax * uz;
+    const float kz = ax * uy - ay * ux;
+    const float inv1c = 1.0f / (1.0f + c);
+    float ds = 0.0f, dcmp = 0.0f;
+    for (int row = row0; row < row1; row++) {
+        const float c1 = a.c1[row], c2p = a.c2p[row], bp = a.bp[row], a2 = a.a2[row];
+        const float am = a.a[row], bm = a.b[row], L0 = a.L0[row];
+        const int ns = a.nsteps[row];
+        {
+            float dx0, dy0, dz0;
+            dirf(row * 12 + grp, dx0, dy0, dz0);
+            const float sc = (kx * dx0 + ky * dy0 + kz * dz0) * inv1c;
+            const float dx = (ky * dz0 - kz * dy0) + dx0 * c + kx * sc;
+            const float dy = (kz * dx0 - kx * dz0) + dy0 * c + ky * sc;
+            const float dz = (kx * dy0 - ky * dx0) + dz0 * c + kz * sc;
+            const float tdx = dx * g.inv_s, tdy = dy * g.inv_s, tdz = dz * g.inv_s;
+            float Rp = 0.0f, Rs = 0.0f;
+            float L = L0;
+            for (int st = 0; st < ns; st++) {
+                const float Lm = L * 10.0f;
+                float qx = __builtin_amdgcn_fmed3f(fmaf(tdx, Lm, ucx), -1.0f, fnx);
+                float qy = __builtin_amdgcn_fmed3f(fmaf(tdy, Lm, ucy), -1.0f, fny);
+                float qz = __builtin_amdgcn_fmed3f(fmaf(tdz, Lm, ucz), -1.0f, fnz);
+                const float fx = floorf(qx), fy = floorf(qy), fz = floorf(qz);
+                const float wx = qx - fx, wy = qy - fy, wz = qz - fz;
+                float t00, t10, t01, t11, r00, r10, r01, r11;
+                tr_xrows<Quad>(B, tr_offset<Quad, Off32>(p, (int)fx, (int)fy, (int)fz), wx, t00, t10, t01, t11, r00,
+                               r10, r01, r11);
+                const float t0 = fmaf(wy, t10 - t00, t00), t1 = fmaf(wy, t11 - t01, t01);
+                const float r0 = fmaf(wy, r10 - r00, r00), r1 = fmaf(wy, r11 - r01, r01);
+                const float T = fmaf(wz, t1 - t0, t0);
+                const float rho = fmaxf(fmaf(wz, r1 - r0, r0), DH_AIR_DENSITY);
+                const float ex = __builtin_amdgcn_exp2f(a2 * rho);       // exp(-a * rho * sp)
+                const float K = T * c1;
+                Rp = fmaf(Rp - K, ex, K);                                  // Rp e + K (1 - e)
+                Rs = fmaf(rho, fmaf(T, c2p, -bp * Rs), Rs);               // Rs (1 - bD) + T D c2
+                L = L - a.sp;
+            }
+            if (rows)
+                ds = fmaf(am, Rp, bm * Rs);
+            else
+                dh_neumaier(ds, dcmp, fmaf(am, Rp, bm * Rs));
+        }
+    }
+    return ds + dcmp;
+}
+
 template <bool Quad, bool Off32, bool RowItems = false>
 __global__ void __launch_bounds__(kBlock)
 k_ccc(dh_ccc_args a, const void *__restrict__ tr, const int *__restrict__ list, const int *__restrict__ counter,
@@ -380,7 +461,6 @@ k_ccc(dh_ccc_args a, const void *__restrict__ tr, const int *__restrict__ list, 
     const bool Rows = RowItems && (uint32_t)n <= cap;
     const int kItems = Rows ? 6 * kGroups : kGroups;
     const uint32_t pstride = Rows ? cap : pstride_az;
-    const float fnx = (float)g.nx, fny = (float)g.ny, fnz = (float)g.nz;
     const int lane = __lane_id();
     // Each wave takes one contiguous run of work items (a chunk's 4 groups are
     // neighbours), and the waves of one XCD take neighbouring runs, so an
@@ -397,79 +477,83 @@ k_ccc(dh_ccc_args a, const void *__restrict__ tr, const int *__restrict__ list, 
         const int idx = base + lane;
         const bool valid = idx < n;
         const uint32_t v = (uint32_t)list[valid ? idx : base];
-        const int i = (int)(v % (uint32_t)g.nx);
-        const uint32_t r = v / (uint32_t)g.nx;
-        const int j = (int)(r % (uint32_t)g.ny);
-        const int k = (int)(r / (uint32_t)g.ny);
-        float x, y, z, ux, uy, uz;
-        dh_vox_xyz(g, i, j, k, x, y, z);
-        dh_uvec(g, x, y, z, ux, uy, uz);
-        // texel-space origin of the voxel (u = t - 0.5)
-        const float ucx = fmaf(g.inv_s, x + g.iso[0], 0.5f) - 0.5f;
-        const float ucy = fmaf(g.inv_s, y + g.iso[1], 0.5f) - 0.5f;
-        const float ucz = fmaf(g.inv_s, z + g.iso[2], 0.5f) - 0.5f;
-        // Rodrigues tilt taking the iso->source axis onto u
-        const float c = a.ax * ux + a.ay * uy + a.az * uz;
-        const float kx = a.ay * uz - a.az * uy;
-        const float ky = a.az * ux - a.ax * uz;
-        const float kz = a.ax * uy - a.ay * ux;
-        const float inv1c = 1.0f / (1.0f + c);
-        float ds = 0.0f, dcmp = 0.0f;
         const int row0 = Rows ? slot / kGroups : 0, row1 = Rows ? row0 + 1 : 6;
-        for (int row = row0; row < row1; row++) {
-            const float c1 = a.c1[row], c2p = a.c2p[row], bp = a.bp[row], a2 = a.a2[row];
-            const float am = a.a[row], bm = a.b[row], L0 = a.L0[row];
-            const int ns = a.nsteps[row];
-            {
-                const int cone = row * 12 + grp;
-                const float dx0 = a.dir[3 * cone + 0];
-                const float dy0 = a.dir[3 * cone + 1];
-                const float dz0 = a.dir[3 * cone + 2];
-                const float sc = (kx * dx0 + ky * dy0 + kz * dz0) * inv1c;
-                const float dx = (ky * dz0 - kz * dy0) + dx0 * c + kx * sc;
-                const float dy = (kz * dx0 - kx * dz0) + dy0 * c + ky * sc;
-                const float dz = (kx * dy0 - ky * dx0) + dz0 * c + kz * sc;
-                const float tdx = dx * g.inv_s, tdy = dy * g.inv_s, tdz = dz * g.inv_s;
-                float Rp = 0.0f, Rs = 0.0f;
-                float L = L0;
-                for (int st = 0; st < ns; st++) {
-                    const float Lm = L * 10.0f;
-                    float qx = __builtin_amdgcn_fmed3f(fmaf(tdx, Lm, ucx), -1.0f, fnx);
-                    float qy = __builtin_amdgcn_fmed3f(fmaf(tdy, Lm, ucy), -1.0f, fny);
-                    float qz = __builtin_amdgcn_fmed3f(fmaf(tdz, Lm, ucz), -1.0f, fnz);
-                    const float fx = floorf(qx), fy = floorf(qy), fz = floorf(qz);
-                    const float wx = qx - fx, wy = qy - fy, wz = qz - fz;
-                    float t00, t10, t01, t11, r00, r10, r01, r11;
-                    tr_xrows<Quad>(B, tr_offset<Quad, Off32>(a.p, (int)fx, (int)fy, (int)fz), wx, t00, t10, t01,
-                                   t11, r00, r10, r01, r11);
-                    const float t0 = fmaf(wy, t10 - t00, t00), t1 = fmaf(wy, t11 - t01, t01);
-                    const float r0 = fmaf(wy, r10 - r00, r00), r1 = fmaf(wy, r11 - r01, r01);
-                    const float T = fmaf(wz, t1 - t0, t0);
-                    const float rho = fmaxf(fmaf(wz, r1 - r0, r0), DH_AIR_DENSITY);
-                    const float ex = __builtin_amdgcn_exp2f(a2 * rho);       // exp(-a * rho * sp)
-                    const float K = T * c1;
-                    Rp = fmaf(Rp - K, ex, K);                                  // Rp e + K (1 - e)
-                    Rs = fmaf(rho, fmaf(T, c2p, -bp * Rs), Rs);               // Rs (1 - bD) + T D c2
-                    L = L - a.sp;
-                }
-                if (Rows)
-                    ds = fmaf(am, Rp, bm * Rs);
-                else
-                    dh_neumaier(ds, dcmp, fmaf(am, Rp, bm * Rs));
-            }
+        const float res = ccc_item<Quad, Off32>(a, g, a.p, B, a.ax, a.ay, a.az, v, grp, row0, row1, Rows,
+                                                [&](int cone, float &dx, float &dy, float &dz) {
+                                                    dx = a.dir[3 * cone + 0];
+                                                    dy = a.dir[3 * cone + 1];
+                                                    dz = a.dir[3 * cone + 2];
+                                                });
+        if (valid) part[(size_t)slot * pstride + idx] = res;
+    }
+}
+
+// The cone superposition of several control points' voxel lists (one grid,
+// pairs layout) in one launch: the points' work items (as in k_ccc, each
+// point taking row items when its list has at most cap voxels) are laid end
+// to end and the waves take them in rounds, so short lists share the chip
+// instead of leaving it idle.  A point's axis and cone directions
+// are formed from its geometry with the host's FP32 (dh_uvec, dh_head_to_img
+// of the head-frame directions), so each voxel's value equals k_ccc's.
+template <bool Off32>
+__global__ void __launch_bounds__(kBlock)
+k_ccc_multi(dh_ccc_multi m)
+{
+    __shared__ int s_first[DH_CCC_MULTI_MAX + 1], s_n[DH_CCC_MULTI_MAX];
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int q = 0; q < m.n; q++) {
+            const int nq = *m.counter[q];
+            s_first[q] = tot;
+            s_n[q] = nq;
+            tot += (nq + kVoxPerItem - 1) / kVoxPerItem * ((uint32_t)nq <= m.cap ? 6 * kGroups : kGroups);
         }
-        if (valid) part[(size_t)slot * pstride + idx] = ds + dcmp;
+        s_first[m.n] = tot;
+    }
+    __syncthreads();
+    const int lane = __lane_id();
+    const int wave = dh_xcd_block(blockIdx.x, gridDim.x) * (kBlock / 64) + (threadIdx.x >> 6);
+    const int nwaves = gridDim.x * (kBlock / 64);
+    const int nitem = s_first[m.n];
+    // rounds of nwaves consecutive items, each XCD taking a contiguous part
+    // of a round: at any time the chip works on one stretch of one point's
+    // list, as a launch per point would, so the gathers of an XCD stay in
+    // one slab of one point's volume (its L2)
+    int q = 0;
+    for (int item = wave; item < nitem; item += nwaves) {
+        while (item >= s_first[q + 1]) q++;
+        // the point is uniform over the wave: keep its arguments in scalars
+        q = __builtin_amdgcn_readfirstlane(q);
+        const int n = __builtin_amdgcn_readfirstlane(s_n[q]);
+        const bool rows = (uint32_t)n <= m.cap;
+        const int kItems = rows ? 6 * kGroups : kGroups;
+        const uint32_t pstride = rows ? m.cap : m.nvox;
+        const int it = item - __builtin_amdgcn_readfirstlane(s_first[q]);
+        const int slot = it % kItems;
+        const int grp = slot % kGroups;
+        const int base = it / kItems * kVoxPerItem;
+        const int idx = base + lane;
+        const bool valid = idx < n;
+        const dh_geom g = m.g[q];
+        const uint32_t v = (uint32_t)m.list[q][valid ? idx : base];
+        float ax, ay, az;
+        dh_uvec(g, 0.0f, 0.0f, 0.0f, ax, ay, az);
+        const int row0 = rows ? slot / kGroups : 0, row1 = rows ? row0 + 1 : 6;
+        const float res = ccc_item<false, Off32>(m, g, m.p, tr_make_bases<false>(m.tr[q], m.p), ax, ay, az, v, grp,
+                                                 row0, row1, rows, [&](int cone, float &dx, float &dy, float &dz) {
+                                                     dh_head_to_img(g, m.hdir[3 * cone + 0], m.hdir[3 * cone + 1],
+                                                                    m.hdir[3 * cone + 2], dx, dy, dz);
+                                                 });
+        if (valid) m.part[q][(size_t)slot * pstride + idx] = res;
     }
 }
 
 // Dose of the listed voxels from the 12 azimuth sums, the electron term and the
 // MU calibration (IMRTClasses.cu:338-346).
-__global__ void __launch_bounds__(kBlock)
-k_ccc_finish(float mu_cal, float mu, const float *__restrict__ electron, const int *__restrict__ list,
-             const int *__restrict__ counter, const float *__restrict__ part, uint32_t pstride_az, uint32_t cap,
-             float *__restrict__ dose)
+__device__ static inline void ccc_finish(float mu_cal, float mu, const float *__restrict__ electron,
+                                         const int *__restrict__ list, int n, const float *__restrict__ part,
+                                         uint32_t pstride_az, uint32_t cap, float *__restrict__ dose)
 {
-    const int n = *counter;
     const bool Rows = (uint32_t)n <= cap;
     const uint32_t pstride = Rows ? cap : pstride_az;
     for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < n; idx += gridDim.x * kBlock) {
@@ -493,6 +577,31 @@ k_ccc_finish(float mu_cal, float mu, const float *__restrict__ electron, const i
         const float dv = ((s0 + s1) + s2) + electron[v];
         dose[v] = (!isnan(dv) && dv >= 0.0f) ? mu_cal * dv * mu : 0.0f;
     }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_ccc_finish(float mu_cal, float mu, const float *__restrict__ electron, const int *__restrict__ list,
+             const int *__restrict__ counter, const float *__restrict__ part, uint32_t pstride_az, uint32_t cap,
+             float *__restrict__ dose)
+{
+    ccc_finish(mu_cal, mu, electron, list, *counter, part, pstride_az, cap, dose);
+}
+
+// k_ccc_finish of each point of a k_ccc_multi launch (blockIdx.y = point).
+__global__ void __launch_bounds__(kBlock)
+k_ccc_finish_multi(dh_ccc_multi m)
+{
+    const int q = blockIdx.y;
+    ccc_finish(m.mu_cal, m.mu[q], m.electron[q], m.list[q], *m.counter[q], m.part[q], m.nvox, m.cap, m.out[q]);
+}
+
+// k_pack_tr (pairs layout) of each point's TERMA with the shared density
+// (blockIdx.y = point).
+__global__ void __launch_bounds__(kBlock)
+k_pack_tr_multi(dh_ccc_multi m, const float *__restrict__ dens)
+{
+    const int q = blockIdx.y;
+    pack_tr_body<false>(m.terma[q], dens, m.g[q].nx, m.g[q].ny, m.g[q].nz, m.p, (void *)m.tr[q]);
 }
 
 // ---- radiological depth on a beam's-eye-view grid (checked surrogate path) --
@@ -896,6 +1005,71 @@ void dh_launch_ccc_counted(const dh_ccc_args &a, const void *tr_pad, int layout,
 {
     if (max_n <= 0) return;
     launch_k_ccc("ccc_list", a, layout, max_n, tr_pad, electron, list, counter, dose, s, true);
+}
+
+void dh_ccc_multi_init(dh_ccc_multi &m, const dh_ccc_args *cas, int n)
+{
+    DH_REQUIRE(n >= 1 && n <= DH_CCC_MULTI_MAX, "cone launch of 1..8 control points");
+    std::memset(&m, 0, sizeof(m));
+    const dh_ccc_args &a = cas[0];
+    m.p = a.p;
+    m.sp = a.sp;
+    m.mu_cal = a.mu_cal;
+    for (int r = 0; r < 6; r++) {
+        m.nsteps[r] = a.nsteps[r];
+        m.L0[r] = a.L0[r]; m.a[r] = a.a[r]; m.b[r] = a.b[r];
+        m.c1[r] = a.c1[r]; m.c2p[r] = a.c2p[r]; m.bp[r] = a.bp[r]; m.a2[r] = a.a2[r];
+    }
+    std::memcpy(m.hdir, a.hdir, sizeof(m.hdir));
+    m.nvox = (uint32_t)((size_t)a.g.nx * a.g.ny * a.g.nz);
+    m.n = n;
+    for (int q = 0; q < n; q++) {
+        DH_REQUIRE(cas[q].g.nx == a.g.nx && cas[q].g.ny == a.g.ny && cas[q].g.nz == a.g.nz && cas[q].sp == a.sp &&
+                   cas[q].mu_cal == a.mu_cal, "the points of a cone launch must share grid and model");
+        m.g[q] = cas[q].g;
+        m.mu[q] = cas[q].mu;
+    }
+}
+
+size_t dh_ccc_multi_tr_bytes(const dh_ccc_args &a) { return ((size_t)8 * a.p.pxy * a.p.pz + 255) & ~(size_t)255; }
+size_t dh_ccc_multi_part_bytes(const dh_ccc_args &a)
+{
+    return sizeof(float) * kGroups * (size_t)a.g.nx * a.g.ny * a.g.nz;
+}
+
+void dh_launch_pack_tr_multi(const dh_ccc_multi &m, const float *density, hipStream_t s)
+{
+    const size_t np = (size_t)m.p.pxy * m.p.pz;
+    const unsigned grid = (unsigned)std::min<size_t>(grid_for(np), 4096);
+    dh_timed t("pack_tr", s);
+    hipLaunchKernelGGL(k_pack_tr_multi, dim3(grid, m.n), dim3(kBlock), 0, s, m, density);
+    DH_HIP(hipGetLastError());
+}
+
+void dh_launch_ccc_multi(const char *timer, dh_ccc_multi m, int64_t max_n, hipStream_t s)
+{
+    if (max_n <= 0) return;
+    DH_REQUIRE(max_n <= (int64_t)m.nvox, "cone list longer than the volume");
+    // 32-bit offsets as for k_ccc (the pairs volume is a quarter of the size
+    // that test admits)
+    const bool o32 = (size_t)32 * m.p.pxy * m.p.pz < ((size_t)1 << 31) && m.p.pxy < ((int64_t)1 << 23);
+    // row items for lists of at most 16384 voxels, as launch_k_ccc
+    m.cap = (uint32_t)std::min<int64_t>(std::min<int64_t>(max_n, m.nvox / 6), 16384);
+    const int64_t items = m.n * ((max_n + kVoxPerItem - 1) / kVoxPerItem * kGroups +
+                                 ((int64_t)m.cap + kVoxPerItem - 1) / kVoxPerItem * kGroups * 5);
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((items + 3) / 4, 2048));
+    {
+        dh_timed t(timer, s);
+        if (o32)
+            hipLaunchKernelGGL(k_ccc_multi<true>, dim3(grid), dim3(kBlock), 0, s, m);
+        else
+            hipLaunchKernelGGL(k_ccc_multi<false>, dim3(grid), dim3(kBlock), 0, s, m);
+        DH_HIP(hipGetLastError());
+    }
+    const unsigned fgrid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((max_n + kBlock - 1) / kBlock, 4096));
+    dh_timed t("ccc_finish", s);
+    hipLaunchKernelGGL(k_ccc_finish_multi, dim3(fgrid, m.n), dim3(kBlock), 0, s, m);
+    DH_HIP(hipGetLastError());
 }
 
 void dh_launch_ccc(const dh_ccc_args &a, const float *terma, const float *density,

@@ -657,7 +657,9 @@ __device__ static inline void block_max_to(float m, float *out)
     __syncthreads();
     if (threadIdx.x == 0) {
         const float b = fmaxf(fmaxf(s_m[0], s_m[1]), fmaxf(s_m[2], s_m[3]));
-        if (b > 0.0f) atomicMax((unsigned int *)out, __float_as_uint(b));
+        // the maximum only grows, so a value read at any time bounds it from
+        // below: a workgroup whose maximum does not exceed it has nothing to add
+        if (b > 0.0f && b > *(volatile float *)out) atomicMax((unsigned int *)out, __float_as_uint(b));
     }
 }
 
@@ -770,11 +772,11 @@ k_sur_pack_batch(sur_pack_batch b, const float *__restrict__ dens, int nz, int n
 // network's output scale, plus the electron term, times MU calibration;
 // zero where TERMA <= 0.01 or the value is not positive.  Also the dose max.
 // One thread per cell of the volume (two 16-byte loads of its outputs).
-__global__ void __launch_bounds__(256)
-k_sur_finish(const float *__restrict__ y32, int cy, int cx, const float *__restrict__ tmax,
-             float out_scale, const float *__restrict__ terma, const float *__restrict__ electron,
-             float mu_scale, int nz, int ny, int nx, float *__restrict__ dose,
-             float *__restrict__ dmax)
+__device__ static inline void sur_finish(const float *__restrict__ y32, int cy, int cx,
+                                         const float *__restrict__ tmax, float out_scale,
+                                         const float *__restrict__ terma, const float *__restrict__ electron,
+                                         float mu_scale, int nz, int ny, int nx, float *__restrict__ dose,
+                                         float *__restrict__ dmax)
 {
     const int vz = (nz + 1) / 2, vy = (ny + 1) / 2, vx = (nx + 1) / 2;
     const int64_t n = (int64_t)vz * vy * vx;
@@ -802,6 +804,89 @@ k_sur_finish(const float *__restrict__ y32, int cy, int cx, const float *__restr
     block_max_to(m, dmax);
 }
 
+__global__ void __launch_bounds__(256)
+k_sur_finish(const float *__restrict__ y32, int cy, int cx, const float *__restrict__ tmax,
+             float out_scale, const float *__restrict__ terma, const float *__restrict__ electron,
+             float mu_scale, int nz, int ny, int nx, float *__restrict__ dose,
+             float *__restrict__ dmax)
+{
+    sur_finish(y32, cy, cx, tmax, out_scale, terma, electron, mu_scale, nz, ny, nx, dose, dmax);
+}
+
+// Per-point buffers of the batched check (blockIdx.y = point).  work: int32
+// {lattice count, fallback count, 2 spare}, then per lattice cell lat_ok,
+// bad and redo, then the lattice list (one per cell) and the fallback list
+// (one per voxel).
+struct sur_check_batch {
+    const float *y32[kPackBatchMax], *tmax[kPackBatchMax];
+    float mu_scale[kPackBatchMax];
+    const float *terma[kPackBatchMax], *electron[kPackBatchMax];
+    float *dose[kPackBatchMax], *dmax[kPackBatchMax], *plain[kPackBatchMax];
+    int *work[kPackBatchMax];
+};
+
+__device__ static inline int *chk_lat_ok(int *w) { return w + 4; }
+__device__ static inline int *chk_bad(int *w, int64_t nc) { return w + 4 + nc; }
+__device__ static inline int *chk_redo(int *w, int64_t nc) { return w + 4 + 2 * nc; }
+__device__ static inline int *chk_lat_list(int *w, int64_t nc) { return w + 4 + 3 * nc; }
+__device__ static inline int *chk_fb_list(int *w, int64_t nc) { return w + 4 + 4 * nc; }
+
+// The surrogate dose of each point of a batch (network outputs y32 at
+// cells * 16 floats per point), as k_sur_finish: workgroups take rows of
+// cells (ck, cj) of their point (blockIdx.y) in turn, staging 64 cells'
+// eight sub-voxel outputs in LDS at a time so that the four voxel rows are
+// read and written with consecutive lanes on consecutive voxels.  Block (0, 0) of each point
+// also zeroes the point's check counters.
+__global__ void __launch_bounds__(256)
+k_sur_finish_batch(sur_check_batch b, int cy, int cx, float out_scale, int nz, int ny, int nx)
+{
+    __shared__ float s_y[64][9];
+    const int q = blockIdx.y;
+    const int vy = (ny + 1) / 2, nrow = ((nz + 1) / 2) * vy;
+    if (blockIdx.x == 0 && threadIdx.x < 4) b.work[q][threadIdx.x] = 0;
+    const float *__restrict__ y32 = b.y32[q];
+    const float *__restrict__ terma = b.terma[q];
+    const float *__restrict__ electron = b.electron[q];
+    float *__restrict__ dose = b.dose[q];
+    const float s = fmaxf(*b.tmax[q], 1e-12f) * out_scale;
+    const float mu_scale = b.mu_scale[q];
+    const int t = threadIdx.x, row = t >> 6, lane = t & 63;
+    const int dz = row >> 1, dy = row & 1;
+    float m = 0.0f;
+    for (int cr = blockIdx.x; cr < nrow; cr += gridDim.x) {
+    const int ck = cr / vy, cj = cr % vy;
+    const int k = 2 * ck + dz, j = 2 * cj + dy;
+    const bool in_row = k < nz && j < ny;
+    const int64_t vrow = ((int64_t)k * ny + j) * nx;
+    const int64_t crow = ((int64_t)ck * cy + cj) * cx;
+    for (int c0 = 0; 2 * c0 < nx; c0 += 64) {
+        {
+            // 64 cells x 8 outputs: four lanes per cell, two floats each
+            const int cell = t >> 2, part = t & 3;
+            float2 y = make_float2(0.0f, 0.0f);
+            if (c0 + cell < cx) y = *(const float2 *)(y32 + (crow + c0 + cell) * 16 + 2 * part);
+            s_y[cell][2 * part] = y.x;
+            s_y[cell][2 * part + 1] = y.y;
+        }
+        __syncthreads();
+        if (in_row) {
+            for (int xx = lane; xx < 128; xx += 64) {
+                const int x = 2 * c0 + xx;
+                if (x >= nx) break;
+                const int64_t v = vrow + x;
+                const float sup = s_y[xx >> 1][4 * dz + 2 * dy + (x & 1)] * s;
+                const float d = (sup + electron[v]) * mu_scale;
+                const float o = (terma[v] > 0.01f && d > 0.0f) ? d : 0.0f;
+                dose[v] = o;
+                m = fmaxf(m, o);
+            }
+        }
+        __syncthreads();
+    }
+    }
+    block_max_to(m, b.dmax[q]);
+}
+
 // block-aggregated append of the flagged items of this block to list
 __device__ static inline void block_append(bool flag, int item, int *__restrict__ list, int *__restrict__ counter)
 {
@@ -820,11 +905,23 @@ __device__ static inline void block_append(bool flag, int item, int *__restrict_
     if (flag) list[s_base + s_cnt[wave] + __popcll(mk & ((1ull << lane) - 1ull))] = item;
 }
 
-// lattice voxels (every stride-th voxel along each axis) with TERMA > 0.01
-__global__ void __launch_bounds__(256)
-k_sur_lattice(const float *__restrict__ terma, int nz, int ny, int nx, int stride, int lz, int ly, int lx,
-              int *__restrict__ lat_ok, int *__restrict__ list, int *__restrict__ counter)
+// The check voxel of each lattice cell (stride^3 voxels from its low corner):
+// the low corner when its TERMA > 0.01, else the cell's centre voxel (offset
+// stride / 2 on each axis) when that lies in the volume with TERMA > 0.01, so
+// cells across the field edge are checked rather than all recomputed.
+// lat_ok: 1 corner, 2 centre, 0 no check voxel.
+__device__ static inline int64_t sur_check_voxel(int code, int ck, int cj, int ci, int stride, int ny, int nx)
 {
+    const int h = code == 2 ? stride / 2 : 0;
+    return ((int64_t)(ck * stride + h) * ny + (cj * stride + h)) * nx + (ci * stride + h);
+}
+
+__global__ void __launch_bounds__(256)
+k_sur_lattice(sur_check_batch bt, int nz, int ny, int nx, int stride, int lz, int ly, int lx)
+{
+    const int q = blockIdx.y;
+    const float *__restrict__ terma = bt.terma[q];
+    int *w = bt.work[q];
     const int64_t nc = (int64_t)lz * ly * lx;
     const int64_t c = blockIdx.x * (int64_t)256 + threadIdx.x;
     bool act = false;
@@ -833,37 +930,48 @@ k_sur_lattice(const float *__restrict__ terma, int nz, int ny, int nx, int strid
         const int ci = (int)(c % lx);
         const int64_t r = c / lx;
         const int cj = (int)(r % ly), ck = (int)(r / ly);
-        v = (int)(((int64_t)ck * stride * ny + (int64_t)cj * stride) * nx + (int64_t)ci * stride);
-        act = !(terma[v] <= 0.01f);
-        lat_ok[c] = act ? 1 : 0;
+        const int h = stride / 2;
+        int code = 0;
+        if (!(terma[sur_check_voxel(1, ck, cj, ci, stride, ny, nx)] <= 0.01f)) code = 1;
+        else if (h > 0 && ck * stride + h < nz && cj * stride + h < ny && ci * stride + h < nx &&
+                 !(terma[sur_check_voxel(2, ck, cj, ci, stride, ny, nx)] <= 0.01f)) code = 2;
+        act = code != 0;
+        v = (int)sur_check_voxel(code ? code : 1, ck, cj, ci, stride, ny, nx);
+        chk_lat_ok(w)[c] = code;
     }
-    block_append(act, v, list, counter);
+    block_append(act, v, chk_lat_list(w, nc), w);
 }
 
-// per lattice cell: the surrogate missed the plain value by more than tol * max
-__global__ void k_sur_flag(const int *__restrict__ lat_ok, const int *__restrict__ list,
-                           const int *__restrict__ counter, const float *__restrict__ dose,
-                           const float *__restrict__ plain, const float *__restrict__ dmax, float tol,
-                           int nz, int ny, int nx, int stride, int lz, int ly, int lx, int *__restrict__ bad)
+// per lattice cell: the surrogate missed the plain value at the cell's check
+// voxel by more than tol * max
+__global__ void __launch_bounds__(256)
+k_sur_flag(sur_check_batch bt, float tol, int ny, int nx, int stride, int lz, int ly, int lx)
 {
+    const int q = blockIdx.y;
+    int *w = bt.work[q];
     const int64_t nc = (int64_t)lz * ly * lx;
     const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (c >= nc) return;
     const int ci = (int)(c % lx);
     const int64_t r = c / lx;
     const int cj = (int)(r % ly), ck = (int)(r / ly);
-    const int64_t v = ((int64_t)ck * stride * ny + (int64_t)cj * stride) * nx + (int64_t)ci * stride;
-    const float thr = tol * fmaxf(*dmax, 1e-12f);
-    bad[c] = (lat_ok[c] && fabsf(dose[v] - plain[v]) > thr) ? 1 : 0;
+    const int code = chk_lat_ok(w)[c];
+    const float thr = tol * fmaxf(*bt.dmax[q], 1e-12f);
+    const int64_t v = sur_check_voxel(code ? code : 1, ck, cj, ci, stride, ny, nx);
+    chk_bad(w, nc)[c] = (code && fabsf(bt.dose[q][v] - bt.plain[q][v]) > thr) ? 1 : 0;
 }
 
-// per cell: recompute its voxels when its lattice voxel has TERMA <= 0.01
-// (cell not covered by the check) or the cell failed, or (dilate 1) one of
+// per cell: recompute its voxels when it has no check voxel (neither its
+// corner nor its centre has TERMA > 0.01) or the cell failed, or (dilate 1) one of
 // its 6 face neighbours or (dilate 2) one of its 26 neighbours failed
-__global__ void k_sur_dilate(const int *__restrict__ lat_ok, const int *__restrict__ bad, int lz, int ly, int lx,
-                             int dilate, int *__restrict__ redo)
+__global__ void __launch_bounds__(256)
+k_sur_dilate(sur_check_batch bt, int lz, int ly, int lx, int dilate)
 {
+    const int q = blockIdx.y;
+    int *w = bt.work[q];
     const int64_t nc = (int64_t)lz * ly * lx;
+    const int *__restrict__ lat_ok = chk_lat_ok(w);
+    const int *__restrict__ bad = chk_bad(w, nc);
     const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (c >= nc) return;
     const int ci = (int)(c % lx);
@@ -878,17 +986,21 @@ __global__ void k_sur_dilate(const int *__restrict__ lat_ok, const int *__restri
                 if (a < 0 || a >= lz || b < 0 || b >= ly || e < 0 || e >= lx) continue;
                 fb = bad[((int64_t)a * ly + b) * lx + e] != 0;
             }
-    redo[c] = fb ? 1 : 0;
+    chk_redo(w, nc)[c] = fb ? 1 : 0;
 }
 
 // voxels with TERMA > 0.01 in cells marked for recomputation, in voxel order
 constexpr int kFallbackRounds = 8;
 
 __global__ void __launch_bounds__(256)
-k_sur_fallback(const float *__restrict__ terma, const float *__restrict__ electron, int band, int nz, int ny,
-               int nx, int stride, int lz, int ly, int lx, const int *__restrict__ redo, int *__restrict__ list,
-               int *__restrict__ counter)
+k_sur_fallback(sur_check_batch bt, int band, int nz, int ny, int nx, int stride, int lz, int ly, int lx)
 {
+    const int q = blockIdx.y;
+    int *w = bt.work[q];
+    const int64_t nc = (int64_t)lz * ly * lx;
+    const float *__restrict__ terma = bt.terma[q];
+    const float *__restrict__ electron = bt.electron[q];
+    const int *__restrict__ redo = chk_redo(w, nc);
     const uint32_t n = (uint32_t)nz * ny * nx;
     dh_block_compact<kFallbackRounds>(blockIdx.x * (uint32_t)(256 * kFallbackRounds), n, [&](uint32_t v) {
         if (terma[v] <= 0.01f) return false;
@@ -899,7 +1011,7 @@ k_sur_fallback(const float *__restrict__ terma, const float *__restrict__ electr
         const uint32_t r = v / (uint32_t)nx;
         const int j = (int)(r % (uint32_t)ny), k = (int)(r / (uint32_t)ny);
         return redo[((int64_t)(k / stride) * ly + j / stride) * lx + i / stride] != 0;
-    }, list, counter);
+    }, chk_fb_list(w, nc), w + 1);
 }
 
 }  // namespace
@@ -1344,16 +1456,70 @@ extern "C" int dh_sur_net_forward_dev(const dh_sur_net *net, const void *d_x, in
     });
 }
 
+namespace {
+// offset (ints) of the last point's work buffer in the check slot after the
+// last dh_launch_sur_dose_batch of this thread, per device (dh_sur_counts)
+thread_local int64_t t_last_counts[64] = {};
+
+// Lattice check and plain fallback of the n (<= 8) points of b, whose
+// surrogate doses and dose maxima are final and whose counters are zero:
+// one launch per step for all points (pack, lattice, lattice cones, flags,
+// dilation, fallback list, fallback cones), the cone launches spreading all
+// points' lists over the chip.
+void sur_check_multi(int dev, const dh_ccc_args *cas, int n, const sur_check_batch &b, const float *density,
+                     int stride, float tol, int dilate, hipStream_t s)
+{
+    const dh_geom &g = cas[0].g;
+    const int nz = g.nz, ny = g.ny, nx = g.nx;
+    const int lz = (nz + stride - 1) / stride, ly = (ny + stride - 1) / stride, lx = (nx + stride - 1) / stride;
+    const int64_t nc = (int64_t)lz * ly * lx, nv = (int64_t)nz * ny * nx;
+    dh_ccc_multi m;
+    dh_ccc_multi_init(m, cas, n);
+    const size_t trb = dh_ccc_multi_tr_bytes(cas[0]), pb = dh_ccc_multi_part_bytes(cas[0]);
+    char *buf = (char *)dh_scratch(dev, DH_SLOT_CHK_TR, (trb + pb) * (size_t)n);
+    for (int q = 0; q < n; q++) {
+        m.tr[q] = buf + (trb + pb) * (size_t)q;
+        m.part[q] = (float *)(buf + (trb + pb) * (size_t)q + trb);
+        m.terma[q] = b.terma[q];
+        m.electron[q] = b.electron[q];
+    }
+    dh_launch_pack_tr_multi(m, density, s);
+    const unsigned cgrid = (unsigned)((nc + 255) / 256);
+    {
+        dh_timed t("sur_lattice", s);
+        hipLaunchKernelGGL(k_sur_lattice, dim3(cgrid, n), dim3(256), 0, s, b, nz, ny, nx, stride, lz, ly, lx);
+    }
+    for (int q = 0; q < n; q++) {
+        m.list[q] = b.work[q] + 4 + 3 * nc;
+        m.counter[q] = b.work[q];
+        m.out[q] = b.plain[q];
+    }
+    dh_launch_ccc_multi("ccc_list", m, nc, s);
+    hipLaunchKernelGGL(k_sur_flag, dim3(cgrid, n), dim3(256), 0, s, b, tol, ny, nx, stride, lz, ly, lx);
+    {
+        dh_timed t("sur_fallback", s);
+        hipLaunchKernelGGL(k_sur_dilate, dim3(cgrid, n), dim3(256), 0, s, b, lz, ly, lx, dilate & 3);
+        hipLaunchKernelGGL(k_sur_fallback, dim3((unsigned)((nv + 256 * kFallbackRounds - 1) / (256 * kFallbackRounds)), n),
+                           dim3(256), 0, s, b, (dilate & 4) ? 1 : 0, nz, ny, nx, stride, lz, ly, lx);
+    }
+    for (int q = 0; q < n; q++) {
+        m.list[q] = b.work[q] + 4 + 4 * nc;
+        m.counter[q] = b.work[q] + 1;
+        m.out[q] = b.dose[q];
+    }
+    dh_launch_ccc_multi("ccc_list", m, nv, s);
+    DH_HIP(hipGetLastError());
+}
+}  // namespace
+
 // Checked surrogate doses of ncp control points on one grid (one density,
 // per-point TERMA and electron volumes): each point is packed into its slot
 // of one batch and the network runs once per batch of up to kSurBatch
-// points.  The points' finish and lattice check then alternate between the
-// caller's stream and a side stream with a second set of check buffers, so
-// one point's lattice pass (a short, latency-bound cone launch) overlaps
-// another's fallback pass; the side stream joins the caller's before the
-// next batch reuses the network buffers.  Every layer uses the variant of
-// its one-sample shape and every point the same kernels on its own buffers,
-// so each dose equals the single-point result bit for bit.
+// points; then one finish launch and one launch per check step serve all
+// points of the batch (sur_check_multi), each point with buffers of its own.
+// Every layer uses the variant of its one-sample shape and every voxel the
+// same arithmetic whatever the batch, so each dose equals the single-point
+// result bit for bit.
 void dh_launch_sur_dose_batch(const dh_sur_net *net, const dh_ccc_args *cas, int ncp,
                               const float *const *terma, const float *density, const float *const *electron,
                               const float (*dirs)[3], int stride, float tol, int dilate, float *const *dose,
@@ -1367,47 +1533,59 @@ void dh_launch_sur_dose_batch(const dh_sur_net *net, const dh_ccc_args *cas, int
     const int64_t cells = (int64_t)cz * cy * cx;
     const int nb = std::min(ncp, kSurBatch);
     const int dev = net->gpu;
+    DH_REQUIRE(dev >= 0 && dev < 64, "device id out of range");
     uint16_t *x = (uint16_t *)dh_scratch(dev, DH_SLOT_NN_X, sizeof(uint16_t) * (size_t)(nb * cells * net->in_ch));
     uint16_t *ws = (uint16_t *)dh_scratch(dev, DH_SLOT_NN_ACT,
                                           sizeof(uint16_t) * (size_t)sur_act_elems(net, nb, cz, cy, cx));
     float *y32 = (float *)dh_scratch(dev, DH_SLOT_NN_Y, sizeof(float) * (size_t)(nb * cells * kSurOut));
     float *scal = (float *)dh_scratch(dev, DH_SLOT_NN_SCAL, 256);     // {max TERMA, max dose} per slot
     const int64_t wl = dh_sur_check_work_len(nz, ny, nx, stride);
-    const size_t nvb = sizeof(float) * (size_t)nz * ny * nx;
-    const bool two = ncp > 1;
-    int *work[2] = {(int *)dh_scratch(dev, DH_SLOT_NN_WORK, sizeof(int) * (size_t)wl), nullptr};
-    float *plain[2] = {(float *)dh_scratch(dev, DH_SLOT_NN_PLAIN, nvb), nullptr};
-    void *trp[2] = {dh_scratch(dev, DH_SLOT_TR_PAD, dh_tr_pad_bytes(cas[0])), nullptr};
-    dh_side *side = nullptr;
-    if (two) {
-        work[1] = (int *)dh_scratch(dev, DH_SLOT_NN_WORK2, sizeof(int) * (size_t)wl);
-        plain[1] = (float *)dh_scratch(dev, DH_SLOT_NN_PLAIN2, nvb);
-        trp[1] = dh_scratch(dev, DH_SLOT_TR_PAD2, dh_tr_pad_bytes(cas[0]));
-        side = &dh_side_of(dev);
-    }
+    const int64_t nv = (int64_t)nz * ny * nx;
+    int *work = (int *)dh_scratch(dev, DH_SLOT_NN_WORK, sizeof(int) * (size_t)(wl * nb));
+    float *plain = (float *)dh_scratch(dev, DH_SLOT_NN_PLAIN, sizeof(float) * (size_t)(nv * nb));
     for (int c0 = 0; c0 < ncp; c0 += nb) {
         const int n = std::min(nb, ncp - c0);
         launch_pack_batch(nz, ny, nx, n, terma + c0, density, dirs + c0, g.s, x, cells * net->in_ch, scal, s);
         sur_forward(net, x, n, cz, cy, cx, y32, ws, s);
-        if (two && n > 1) {
-            DH_HIP(hipEventRecord(side->fork, s));
-            DH_HIP(hipStreamWaitEvent(side->st, side->fork, 0));
-        }
+        sur_check_batch b;
+        std::memset(&b, 0, sizeof(b));
         for (int i = 0; i < n; i++) {
-            const int k = (two && n > 1) ? (i & 1) : 0;
-            hipStream_t st = k ? side->st : s;
             const dh_ccc_args &ca = cas[c0 + i];
-            launch_finish(nz, ny, nx, y32 + (size_t)i * cells * kSurOut, scal + 2 * i, net->out_scale,
-                          terma[c0 + i], electron[c0 + i], ca.mu_cal * ca.mu, dose[c0 + i], scal + 2 * i + 1, st,
-                          false);
-            dh_launch_sur_check(ca, terma[c0 + i], density, electron[c0 + i], dose[c0 + i], scal + 2 * i + 1,
-                                stride, tol, dilate, work[k], plain[k], trp[k], st);
+            b.y32[i] = y32 + (size_t)i * cells * kSurOut;
+            b.tmax[i] = scal + 2 * i;
+            b.mu_scale[i] = ca.mu_cal * ca.mu;
+            b.terma[i] = terma[c0 + i];
+            b.electron[i] = electron[c0 + i];
+            b.dose[i] = dose[c0 + i];
+            b.dmax[i] = scal + 2 * i + 1;
+            b.plain[i] = plain + (size_t)i * nv;
+            b.work[i] = work + (size_t)i * wl;
         }
-        if (two && n > 1) {
-            DH_HIP(hipEventRecord(side->join, side->st));
-            DH_HIP(hipStreamWaitEvent(s, side->join, 0));
+        {
+            dh_timed t("sur_finish", s);
+            hipLaunchKernelGGL(k_sur_finish_batch, dim3((unsigned)std::min((nz + 1) / 2 * ((ny + 1) / 2), 512), n),
+                               dim3(256), 0, s, b, cy, cx, net->out_scale, nz, ny, nx);
+            DH_HIP(hipGetLastError());
         }
+        sur_check_multi(dev, cas + c0, n, b, density, stride, tol, dilate, s);
+        t_last_counts[dev] = (int64_t)(n - 1) * wl;
     }
+}
+
+void dh_launch_sur_check(int dev, const dh_ccc_args &ca, const float *terma, const float *density,
+                         const float *electron, float *dose, const float *dmax, int stride, float tol, int dilate,
+                         int *work, float *plain, hipStream_t s)
+{
+    sur_check_batch b;
+    std::memset(&b, 0, sizeof(b));
+    b.terma[0] = terma;
+    b.electron[0] = electron;
+    b.dose[0] = dose;
+    b.dmax[0] = const_cast<float *>(dmax);     // read only by the check kernels
+    b.plain[0] = plain;
+    b.work[0] = work;
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, work, 4, 0);
+    sur_check_multi(dev, &ca, 1, b, density, stride, tol, dilate, s);
 }
 
 void dh_launch_sur_dose(const dh_sur_net *net, const dh_ccc_args &ca, const float *terma,
@@ -1423,7 +1601,9 @@ extern "C" int dh_sur_counts(int gpu_id, int32_t *lattice, int32_t *fallback)
     return dh_guarded([&] {
         DH_REQUIRE(lattice && fallback, "null argument");
         dh_device_enter(gpu_id);
-        int *work = (int *)dh_scratch(gpu_id, DH_SLOT_NN_WORK, 16);
+        DH_REQUIRE(gpu_id >= 0 && gpu_id < 64, "device id out of range");
+        const int64_t off = t_last_counts[gpu_id];
+        int *work = (int *)dh_scratch(gpu_id, DH_SLOT_NN_WORK, sizeof(int) * (size_t)(off + 4)) + off;
         int h[2] = {0, 0};
         DH_HIP(hipDeviceSynchronize());
         DH_HIP(hipMemcpy(h, work, sizeof(h), hipMemcpyDeviceToHost));
@@ -1437,40 +1617,6 @@ int64_t dh_sur_check_work_len(int nz, int ny, int nx, int stride)
     const int64_t nc = (int64_t)((nz + stride - 1) / stride) * ((ny + stride - 1) / stride) *
                        ((nx + stride - 1) / stride);
     return 4 + 4 * nc + (int64_t)nz * ny * nx;
-}
-
-void dh_launch_sur_check(const dh_ccc_args &ca, const float *terma, const float *density,
-                         const float *electron, float *dose, const float *dmax, int stride,
-                         float tol, int dilate, int *work, float *plain, void *tr_pad, hipStream_t s)
-{
-    const dh_geom &g = ca.g;
-    const int nz = g.nz, ny = g.ny, nx = g.nx;
-    const int lz = (nz + stride - 1) / stride, ly = (ny + stride - 1) / stride, lx = (nx + stride - 1) / stride;
-    const int64_t nc = (int64_t)lz * ly * lx, nv = (int64_t)nz * ny * nx;
-    int *counter = work, *lat_ok = work + 4, *bad = lat_ok + nc, *redo = bad + nc, *lat_list = redo + nc;
-    int *fb_list = lat_list + nc;
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, counter, 4, 0);
-    const int layout = dh_ccc_layout(ca, -1);
-    dh_launch_pack_tr(ca, terma, density, tr_pad, layout, s);
-    {
-        dh_timed t("sur_lattice", s);
-        hipLaunchKernelGGL(k_sur_lattice, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, terma, nz, ny, nx,
-                           stride, lz, ly, lx, lat_ok, lat_list, counter);
-    }
-    dh_launch_ccc_counted(ca, tr_pad, layout, electron, lat_list, counter, nc, plain, s);
-    hipLaunchKernelGGL(k_sur_flag, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, (const int *)lat_ok,
-                       (const int *)lat_list, (const int *)counter, (const float *)dose, (const float *)plain,
-                       dmax, tol, nz, ny, nx, stride, lz, ly, lx, bad);
-    {
-        dh_timed t("sur_fallback", s);
-        hipLaunchKernelGGL(k_sur_dilate, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, (const int *)lat_ok,
-                           (const int *)bad, lz, ly, lx, dilate & 3, redo);
-        hipLaunchKernelGGL(k_sur_fallback, dim3((unsigned)((nv + 256 * kFallbackRounds - 1) / (256 * kFallbackRounds))),
-                           dim3(256), 0, s, terma, electron, (dilate & 4) ? 1 : 0, nz, ny, nx, stride, lz, ly, lx,
-                           (const int *)redo, fb_list, counter + 1);
-    }
-    dh_launch_ccc_counted(ca, tr_pad, layout, electron, fb_list, counter + 1, nv, dose, s);
-    DH_HIP(hipGetLastError());
 }
 
 extern "C" int dh_nn_conv3d_wgrad_dev(const void *x, const void *dy, int32_t n, int32_t d,

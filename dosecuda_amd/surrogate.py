@@ -796,6 +796,40 @@ def _time(fn, reps):
     return e0.elapsed_time(e1) / reps
 
 
+def time_plan_path(net: SurrogateNet, model, n_beams=7, reps=10, check=None):
+    """The whole checked surrogate path of the 128^3 lung/bone plan (7
+    coplanar 10x10 cm beams): `plan_doses` from the density captured as one
+    HIP graph and replayed `reps` times; returns the milliseconds per beam and
+    each beam's gamma (2%/2 mm, 10% cut) against the plain CCCS.  Only this
+    path runs between the capture and the gamma checks, so a kernel trace of
+    a process that calls it shows the path's own launches."""
+    from .beam_models import square_field_mlc
+    from .phantoms import imrt_beams, imrt_lung_bone_case, to_density
+    st, tol, dil = check or DEFAULT_CHECK
+    case = imrt_lung_bone_case(seed=1002, n=128, spacing=2.5)
+    rho = torch.from_numpy(to_density(case)).cuda()
+    mlc = square_field_mlc(model, 10.0, 10.0)
+    sur = SurrogateCCCS(net, model, stride=st, tol=tol, dilate=dil)
+    angles = imrt_beams(n_beams)
+    cps = [(case.iso, 100.0, ga, 0.0, 0.0) for ga in angles]
+    mlcs = [mlc] * len(angles)
+    work = ([torch.empty_like(rho) for _ in angles], [torch.empty_like(rho) for _ in angles])
+    outs = [torch.empty_like(rho) for _ in angles]
+    sur.plan_doses(rho, case.spacing, case.origin, cps, mlcs, outs=outs, work=work)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        sur.plan_doses(rho, case.spacing, case.origin, cps, mlcs, outs=outs, work=work)
+    g.replay()
+    torch.cuda.synchronize()
+    ms = _time(lambda: g.replay(), reps) / len(angles)
+    plains = [engine.photon_dose(rho, case.spacing, case.origin, case.iso, 100.0, ga, 0.0, 0.0, model, mlc)
+              for ga in angles]
+    gm = [gamma(o, p, case.spacing)[0] for o, p in zip(outs, plains)]
+    return dict(ms_per_beam=ms, beam_gamma=gm, min_gamma=min(gm), beams=len(angles), reps=reps,
+                check=dict(stride=st, tol=tol, dilate=dil))
+
+
 def evaluate(net: SurrogateNet, model, n_beams=7, stride=3, tol=0.01, reps=5, log=None, dilate=1):
     """Per-beam and plan gamma (2%/2 mm, 10% cut) of the surrogate alone and
     of the checked surrogate against the plain CCCS on the 128^3 lung/bone

@@ -365,36 +365,59 @@ def test_pack_and_finish_match_torch(photon_model):
 
 
 def _torch_checked(pred, plain, terma, stride, tol, dilate=2):
-    """The lattice check and fallback restated with torch operations."""
+    """The lattice check and fallback restated with torch operations: each
+    cell of stride^3 voxels is checked at its low corner when that voxel is
+    active, else at its centre voxel (offset stride // 2) when that is in the
+    volume and active, else it has no check voxel and is recomputed."""
     import torch
     import torch.nn.functional as F
     active = ~(terma <= 0.01)
-    lat = torch.zeros_like(active)
-    lat[::stride, ::stride, ::stride] = True
+    nz, ny, nx = active.shape
+    L = [-(-n // stride) for n in active.shape]
+    h = stride // 2
+    cells = lambda v: v[::stride, ::stride, ::stride]
+    corner = cells(active)
+    centre = torch.zeros(L, dtype=torch.bool, device=active.device)
+    if h > 0:
+        ca = active[h::stride, h::stride, h::stride]
+        centre[:ca.shape[0], :ca.shape[1], :ca.shape[2]] = ca
+    centre &= ~corner
+    err = lambda v: cells(v)
     dmax = torch.clamp(pred.max(), min=1e-12)
-    bad = torch.zeros_like(active)
-    li = torch.nonzero((active & lat).view(-1)).view(-1)
-    bad.view(-1)[li] = (pred.view(-1)[li] - plain.view(-1)[li]).abs() > tol * dmax
-    pool = lambda v: F.max_pool3d(v.float()[None, None], kernel_size=stride, stride=stride, ceil_mode=True)
-    pb = pool(bad)
+    diff = (pred - plain).abs() > tol * dmax
+    dc = torch.zeros(L, dtype=torch.bool, device=active.device)
+    if h > 0:
+        e = diff[h::stride, h::stride, h::stride]
+        dc[:e.shape[0], :e.shape[1], :e.shape[2]] = e
+    bad = (corner & err(diff)) | (centre & dc)
+    checked = corner | centre
+    pb = bad.float()[None, None]
     if dilate == 2:
-        cells = F.max_pool3d(pb, kernel_size=3, stride=1, padding=1)
+        flag = F.max_pool3d(pb, kernel_size=3, stride=1, padding=1)
     elif dilate == 1:
         k = torch.zeros((1, 1, 3, 3, 3), device=pb.device)
         k[0, 0, 1, 1, :] = 1
         k[0, 0, 1, :, 1] = 1
         k[0, 0, :, 1, 1] = 1
-        cells = (F.conv3d(pb, k, padding=1) > 0).float()
+        flag = (F.conv3d(pb, k, padding=1) > 0).float()
     else:
-        cells = pb
-    cells = torch.maximum(cells, pool(active) * (1.0 - pool(active & lat)))
-    up = cells.repeat_interleave(stride, 2).repeat_interleave(stride, 3).repeat_interleave(stride, 4)
-    fb = (up[0, 0, :pred.shape[0], :pred.shape[1], :pred.shape[2]] > 0) & active
-    return torch.where(fb, plain, pred), int(li.numel()), int(fb.sum())
+        flag = pb
+    pool = lambda v: F.max_pool3d(v.float()[None, None], kernel_size=stride, stride=stride, ceil_mode=True)
+    flag = torch.maximum(flag, pool(active) * (~checked).float()[None, None])
+    up = flag.repeat_interleave(stride, 2).repeat_interleave(stride, 3).repeat_interleave(stride, 4)
+    fb = (up[0, 0, :nz, :ny, :nx] > 0) & active
+    return (torch.where(fb, plain, pred), int(checked.sum()), int(fb.sum()), int(centre.sum()),
+            int((centre & dc).sum()))
 
 
-@pytest.mark.parametrize("stride, dilate", [(3, 2), (4, 2), (4, 1), (4, 0)])
-def test_device_check_matches_torch(photon_model, stride, dilate):
+@pytest.mark.parametrize("stride, dilate, field", [(3, 2, 7.0), (4, 2, 7.0), (4, 1, 7.0), (4, 0, 7.0), (4, 1, 3.0),
+                                                   (3, 2, 3.0)])
+def test_device_check_matches_torch(photon_model, stride, dilate, field):
+    """The device check equals its torch restatement: lattice and fallback
+    counts and the checked dose bit for bit.  In a 7 cm field every voxel
+    has TERMA > 0.01; in a 3 cm field cells across the field edge have an
+    inactive corner and are checked at their centre voxel, and those of the
+    upper half are made to fail there."""
     t = _torch()
     from dosecuda_amd import engine, surrogate as S
     from dosecuda_amd.beam_models import square_field_mlc
@@ -405,12 +428,23 @@ def test_device_check_matches_torch(photon_model, stride, dilate):
     iso = np.array([(shape[2] - 1) / 2 * s, (shape[1] - 1) / 2 * s, (shape[0] - 1) / 2 * s], np.float32)
     terma, el = t.empty_like(rho), t.empty_like(rho)
     plain = engine.photon_dose(rho, s, np.zeros(3), iso, 100.0, 200.0, 0.0, 0.0, pm,
-                               square_field_mlc(pm, 7.0, 7.0), terma_out=terma, electron_out=el)
+                               square_field_mlc(pm, field, field), terma_out=terma, electron_out=el)
     pred = plain.clone()
     pred[10:18, 10:20, 12:22] *= 1.05          # fails a 1% check where the dose is high
     pred[25:35, 25:35, 25:35] *= 1.002         # passes
     pred[30:32, 2:6, 2:6] = 0.0                # isolated failures
-    want, n_lat, n_fb = _torch_checked(pred, plain, terma, stride, 0.01, dilate)
+    # cells of the upper half whose corner voxel is inactive (across the
+    # field edge, checked at their centre voxel) 2% of the maximum high
+    act = ~(terma <= 0.01)
+    edge = (~act[::stride, ::stride, ::stride]).repeat_interleave(stride, 0).repeat_interleave(stride, 1)
+    edge = edge.repeat_interleave(stride, 2)[:shape[0], :shape[1], :shape[2]] & act
+    edge[21:] = False
+    pred[edge] += 0.02 * float(plain.max())
+    want, n_lat, n_fb, n_centre, n_centre_bad = _torch_checked(pred, plain, terma, stride, 0.01, dilate)
+    if field < 5.0:
+        assert n_centre > n_centre_bad > 0
+    else:
+        assert n_centre == 0
     sur = S.SurrogateCCCS(S.SurrogateNet().cuda(), pm, stride=stride, dilate=dilate)
     b = sur._buffers(shape, terma.device, stride)
     b["dmax"].fill_(float(pred.max()))
@@ -425,7 +459,7 @@ def test_device_check_matches_torch(photon_model, stride, dilate):
                                             _lib.ptr(b["plain"]), S._st()))
     lat, fb = sur.counts(shape, stride=stride)
     print(f"[parity] device check stride {stride} dilate {dilate}: lattice {lat} fallback {fb} "
-          f"(torch {n_lat}, {n_fb})", flush=True)
+          f"(torch {n_lat}, {n_fb}, {n_centre} at cell centres)", flush=True)
     assert (lat, fb) == (n_lat, n_fb)
     assert 0 < fb < int((~(terma <= 0.01)).sum())
     assert t.equal(got, want)
