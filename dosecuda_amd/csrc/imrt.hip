@@ -491,47 +491,53 @@ k_ccc(dh_ccc_args a, const void *__restrict__ tr, const int *__restrict__ list, 
 // The cone superposition of several control points' voxel lists (one grid,
 // pairs layout) in one launch: the points' work items (as in k_ccc, each
 // point taking row items when its list has at most cap voxels) are laid end
-// to end and the waves take them in rounds, so short lists share the chip
-// instead of leaving it idle.  A point's axis and cone directions
+// to end, so short lists share the chip instead of leaving it idle.  A point's axis and cone directions
 // are formed from its geometry with the host's FP32 (dh_uvec, dh_head_to_img
 // of the head-frame directions), so each voxel's value equals k_ccc's.
 template <bool Off32>
 __global__ void __launch_bounds__(kBlock)
 k_ccc_multi(dh_ccc_multi m)
 {
-    __shared__ int s_first[DH_CCC_MULTI_MAX + 1], s_n[DH_CCC_MULTI_MAX];
+    // The dispatcher hands workgroup b to XCD b % 8.  XCD x takes the x-th
+    // eighth (slab) of every point's items, the points' slabs end to end,
+    // and its waves go through that sequence in rounds of consecutive items:
+    // an XCD's gathers stay in one slab of one point's volume at a time (its
+    // L2), as a launch per point with one slab per XCD would keep them.  The
+    // grid has at least 8 workgroups, so every XCD has waves.
+    __shared__ int s_first[DH_CCC_MULTI_MAX + 1], s_n[DH_CCC_MULTI_MAX], s_lo[DH_CCC_MULTI_MAX];
+    const int x = blockIdx.x % DH_NUM_XCD;
     if (threadIdx.x == 0) {
         int tot = 0;
         for (int q = 0; q < m.n; q++) {
             const int nq = *m.counter[q];
+            const int items = (nq + kVoxPerItem - 1) / kVoxPerItem * ((uint32_t)nq <= m.cap ? 6 * kGroups : kGroups);
+            const int lo = (int)((int64_t)items * x / DH_NUM_XCD), hi = (int)((int64_t)items * (x + 1) / DH_NUM_XCD);
             s_first[q] = tot;
             s_n[q] = nq;
-            tot += (nq + kVoxPerItem - 1) / kVoxPerItem * ((uint32_t)nq <= m.cap ? 6 * kGroups : kGroups);
+            s_lo[q] = lo;
+            tot += hi - lo;
         }
         s_first[m.n] = tot;
     }
     __syncthreads();
     const int lane = __lane_id();
-    const int wave = dh_xcd_block(blockIdx.x, gridDim.x) * (kBlock / 64) + (threadIdx.x >> 6);
-    const int nwaves = gridDim.x * (kBlock / 64);
-    const int nitem = s_first[m.n];
-    // rounds of nwaves consecutive items, each XCD taking a contiguous part
-    // of a round: at any time the chip works on one stretch of one point's
-    // list, as a launch per point would, so the gathers of an XCD stay in
-    // one slab of one point's volume (its L2)
+    const int nbx = (int)gridDim.x / DH_NUM_XCD + (x < (int)gridDim.x % DH_NUM_XCD ? 1 : 0);
+    const int wx = nbx * (kBlock / 64);                                   // waves on this XCD
+    const int lw = (int)(blockIdx.x / DH_NUM_XCD) * (kBlock / 64) + (threadIdx.x >> 6);
+    const int nseq = s_first[m.n];
     int q = 0;
-    for (int item = wave; item < nitem; item += nwaves) {
-        while (item >= s_first[q + 1]) q++;
+    for (int i = lw; i < nseq; i += wx) {
+        while (i >= s_first[q + 1]) q++;
         // the point is uniform over the wave: keep its arguments in scalars
         q = __builtin_amdgcn_readfirstlane(q);
+        const int item = __builtin_amdgcn_readfirstlane(s_lo[q] - s_first[q]) + i;
         const int n = __builtin_amdgcn_readfirstlane(s_n[q]);
         const bool rows = (uint32_t)n <= m.cap;
         const int kItems = rows ? 6 * kGroups : kGroups;
         const uint32_t pstride = rows ? m.cap : m.nvox;
-        const int it = item - __builtin_amdgcn_readfirstlane(s_first[q]);
-        const int slot = it % kItems;
+        const int slot = item % kItems;
         const int grp = slot % kGroups;
-        const int base = it / kItems * kVoxPerItem;
+        const int base = item / kItems * kVoxPerItem;
         const int idx = base + lane;
         const bool valid = idx < n;
         const dh_geom g = m.g[q];
@@ -1057,7 +1063,7 @@ void dh_launch_ccc_multi(const char *timer, dh_ccc_multi m, int64_t max_n, hipSt
     m.cap = (uint32_t)std::min<int64_t>(std::min<int64_t>(max_n, m.nvox / 6), 16384);
     const int64_t items = m.n * ((max_n + kVoxPerItem - 1) / kVoxPerItem * kGroups +
                                  ((int64_t)m.cap + kVoxPerItem - 1) / kVoxPerItem * kGroups * 5);
-    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((items + 3) / 4, 2048));
+    const unsigned grid = (unsigned)std::max<int64_t>(DH_NUM_XCD, std::min<int64_t>((items + 3) / 4, 2048));
     {
         dh_timed t(timer, s);
         if (o32)

@@ -25,11 +25,8 @@ namespace {
 __device__ static inline float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 __device__ static inline uint16_t f2bf(float f)
-{   // round to nearest even (NaN kept quiet)
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
+{   // round to nearest even: one v_cvt_pk_bf16_f32 on gfx950
+    return __builtin_bit_cast(uint16_t, (__bf16)f);
 }
 
 struct conv_args {
@@ -339,20 +336,25 @@ __global__ void __launch_bounds__(256)
 k_conv3d_c16(conv_args a)
 {
     constexpr int TZ = 2, BY = TY + 2, BX = 34, NB = (TZ + 2) * BY * BX;
+    // the two 8-channel planes, the second 128 bytes further on, so that the
+    // 16 lanes of a staging store (8 per plane) fall on different banks
+    constexpr int PL = NB * 8 + 64;
     constexpr int RPW = TZ * TY / 4;                 // rows of 32 voxels per wave
-    __shared__ __attribute__((aligned(16))) uint16_t sa[2 * NB * 8];   // [half][voxel][8]
+    __shared__ __attribute__((aligned(16))) uint16_t sa[2 * PL];   // [half][voxel][8]
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int col = lane & 15, kq = lane >> 4;
     const int half = kq & 1, tsub = kq >> 1;
     const int ntx = (a.W + 31) / 32, nty = (a.H + TY - 1) / TY, ntz = (a.D + TZ - 1) / TZ;
-    int64_t bi = blockIdx.x;
-    const int tx = (int)(bi % ntx); bi /= ntx;
-    const int ty = (int)(bi % nty); bi /= nty;
-    const int tz = (int)(bi % ntz);
-    const int n = (int)(bi / ntz);
+    uint32_t bi = blockIdx.x;
+    const int tx = (int)(bi % (uint32_t)ntx); bi /= (uint32_t)ntx;
+    const int ty = (int)(bi % (uint32_t)nty); bi /= (uint32_t)nty;
+    const int tz = (int)(bi % (uint32_t)ntz);
+    const int n = (int)(bi / (uint32_t)ntz);
     const int x0 = tx * 32, y0 = ty * TY, z0 = tz * TZ;
     const int Da = a.D >> a.up, Ha = a.H >> a.up, Wa = a.W >> a.up;
     const int nsl = a.cin >> 4;
+    // element indices in 32 bits: the host admits tensors of fewer than 2^31
+    // elements
 
     dh_f32x4 acc[RPW][2];
 #pragma unroll
@@ -377,29 +379,33 @@ k_conv3d_c16(conv_args a)
             stage[u] = make_uint4(0, 0, 0, 0);
             if (q < 2 * NB && gz >= 0 && gz < a.D && gy >= 0 && gy < a.H && gx >= 0 && gx < a.W) {
                 if (c0 < a.ca) {
-                    const int64_t src = (((int64_t)n * Da + (gz >> a.up)) * Ha + (gy >> a.up)) * Wa + (gx >> a.up);
-                    stage[u] = *(const uint4 *)(a.xa + src * a.ca + c0 + hf * 8);
+                    const uint32_t src = ((uint32_t)(n * Da + (gz >> a.up)) * (uint32_t)Ha + (uint32_t)(gy >> a.up)) *
+                                             (uint32_t)Wa + (uint32_t)(gx >> a.up);
+                    stage[u] = *(const uint4 *)(a.xa + (src * (uint32_t)a.ca + (uint32_t)(c0 + hf * 8)));
                 } else {
-                    const int64_t src = (((int64_t)n * a.D + gz) * a.H + gy) * a.W + gx;
-                    stage[u] = *(const uint4 *)(a.xb + src * a.cb + (c0 - a.ca) + hf * 8);
+                    const uint32_t src = ((uint32_t)(n * a.D + gz) * (uint32_t)a.H + (uint32_t)gy) * (uint32_t)a.W +
+                                         (uint32_t)gx;
+                    stage[u] = *(const uint4 *)(a.xb + (src * (uint32_t)a.cb + (uint32_t)(c0 - a.ca + hf * 8)));
                 }
             }
         }
 #pragma unroll
         for (int u = 0; u < NQ; u++) {
             const int q = tid + 256 * u;
-            if (q < 2 * NB) *(uint4 *)(sa + ((q & 1) * NB + (q >> 1)) * 8) = stage[u];
+            if (q < 2 * NB) *(uint4 *)(sa + (q & 1) * PL + (q >> 1) * 8) = stage[u];
         }
         dh_bf16x8 B[14];
+        const uint16_t *wl = a.w + ((uint32_t)col * (uint32_t)a.cin + (uint32_t)(c0 + half * 8));
+        const uint32_t wtap = (uint32_t)a.cout_pad * (uint32_t)a.cin;
 #pragma unroll
         for (int p = 0; p < 14; p++) {
             const int tp = 2 * p + tsub;
             uint4 bv = make_uint4(0, 0, 0, 0);
-            if (tp < 27 && col < a.cout)
-                bv = *(const uint4 *)(a.w + ((int64_t)tp * a.cout_pad + col) * a.cin + c0 + half * 8);
+            if (tp < 27 && col < a.cout) bv = *(const uint4 *)(wl + (uint32_t)tp * wtap);
             __builtin_memcpy(&B[p], &bv, 16);
         }
         __syncthreads();
+        const uint16_t *sh = sa + half * PL + col * 8;
 #pragma unroll
         for (int p = 0; p < 14; p++) {
             const int tp = min(2 * p + tsub, 26);
@@ -409,8 +415,8 @@ k_conv3d_c16(conv_args a)
                 const int rr = wid * RPW + m, rz = rr / TY, ry = rr % TY;
 #pragma unroll
                 for (int h2 = 0; h2 < 2; h2++) {
-                    const int vb = ((rz + dz) * BY + (ry + dy)) * BX + h2 * 16 + col + dx;
-                    const uint4 av = *(const uint4 *)(sa + (half * NB + vb) * 8);
+                    const int vb = ((rz + dz) * BY + (ry + dy)) * BX + h2 * 16 + dx;
+                    const uint4 av = *(const uint4 *)(sh + vb * 8);
                     dh_bf16x8 A;
                     __builtin_memcpy(&A, &av, 16);
                     acc[m][h2] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[p], acc[m][h2], 0, 0, 0);
@@ -421,22 +427,45 @@ k_conv3d_c16(conv_args a)
     }
     if (col >= a.cout) return;
     const float bsv = a.bias ? a.bias[col] : 0.0f;
+    const int nx = min(32, a.W - x0);                // valid voxels of the brick's rows
+    // the bias and the activation of every output first (branch-free), then
+    // the stores: the common case (whole rows of 32 voxels, 16 channels, bf16
+    // output only: the network's hidden layers) without per-element tests and
+    // with constant store offsets
+#pragma unroll
+    for (int m = 0; m < RPW; m++)
+#pragma unroll
+        for (int h2 = 0; h2 < 2; h2++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                float o = acc[m][h2][r] + bsv;
+                acc[m][h2][r] = a.act ? (o > 0.0f ? o : o * a.slope) : o;
+            }
+    const bool fast = nx == 32 && a.cout == 16 && a.y && !a.y32;
 #pragma unroll
     for (int m = 0; m < RPW; m++) {
         const int rr = wid * RPW + m;
         const int gz = z0 + rr / TY, gy = y0 + rr % TY;
         if (gz >= a.D || gy >= a.H) continue;
+        // element of (voxel x0 + 4 kq of this row, channel col)
+        const uint32_t e0 = (((uint32_t)(n * a.D + gz) * (uint32_t)a.H + (uint32_t)gy) * (uint32_t)a.W +
+                             (uint32_t)(x0 + 4 * kq)) * (uint32_t)a.cout + (uint32_t)col;
+        if (fast) {
+            uint16_t *yr = a.y + e0;
+#pragma unroll
+            for (int h2 = 0; h2 < 2; h2++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) yr[(16 * h2 + r) * 16] = f2bf(acc[m][h2][r]);
+            continue;
+        }
 #pragma unroll
         for (int h2 = 0; h2 < 2; h2++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int gx = x0 + 16 * h2 + 4 * kq + r;
-                if (gx >= a.W) continue;
-                const int64_t ov = (((int64_t)n * a.D + gz) * a.H + gy) * a.W + gx;
-                float o = acc[m][h2][r] + bsv;
-                if (a.act) o = o > 0.0f ? o : o * a.slope;
-                if (a.y) a.y[ov * a.cout + col] = f2bf(o);
-                if (a.y32) a.y32[ov * a.cout + col] = o;
+                if (16 * h2 + 4 * kq + r >= nx) continue;
+                const uint32_t off = (uint32_t)(16 * h2 + r) * (uint32_t)a.cout;
+                if (a.y) a.y[e0 + off] = f2bf(acc[m][h2][r]);
+                if (a.y32) a.y32[e0 + off] = acc[m][h2][r];
             }
     }
 }
@@ -1077,7 +1106,7 @@ void conv_pick(const conv_args &a, int &nco, int &ksplit)
     else if (code > 0) { nco = code / 100; ksplit = code % 100; }
     if (nco == 0 && ksplit == 0) {
         if (a.cout <= 16 && a.ksize == 1 && a.cb == 0 && a.up == 0) { nco = 1; ksplit = -3; }
-        else if (a.cout <= 16 && a.ksize == 3 && nvox >= 32768) { nco = 8; ksplit = -3; }
+        else if (a.cout <= 16 && a.ksize == 3 && nvox >= 32768) { nco = nvox >= (1 << 20) ? 4 : 8; ksplit = -3; }
         else if (a.ksize == 3 && nvox >= 32768 && a.cout_pad <= 64) ksplit = nvox < 131072 ? -2 : -1;
     }
     if (ksplit >= 0 && (nco <= 0 || ksplit <= 0)) conv_shape(nvox, a.cout_pad, ki, nco, ksplit);
@@ -1098,10 +1127,11 @@ void launch_conv3d(const conv_args &a, int nco, int ksplit, hipStream_t s)
     }
     if (nco == 0 && ksplit == 0) {
         // untuned shape: the variants the tuner picks for the surrogate's
-        // layers (16-output-channel kernels for <= 16 outputs; LDS bricks for
-        // large 3x3x3 layers, two-row bricks below 2^17 voxels)
+        // layers (16-output-channel kernels for <= 16 outputs, four-row bricks
+        // from 2^20 voxels (batches of points); LDS bricks for large 3x3x3
+        // layers, two-row bricks below 2^17 voxels)
         if (a.cout <= 16 && a.ksize == 1 && a.cb == 0 && a.up == 0) { nco = 1; ksplit = -3; }
-        else if (a.cout <= 16 && a.ksize == 3 && nvox >= 32768) { nco = 8; ksplit = -3; }
+        else if (a.cout <= 16 && a.ksize == 3 && nvox >= 32768) { nco = nvox >= (1 << 20) ? 4 : 8; ksplit = -3; }
         else if (a.ksize == 3 && nvox >= 32768 && a.cout_pad <= 64) ksplit = nvox < 131072 ? -2 : -1;
     }
     int c = nco, k = ksplit;
@@ -1346,7 +1376,10 @@ int64_t sur_act_elems(const dh_sur_net *net, int n, int cz, int cy, int cx)
 }
 
 // One layer over n samples with the variant picked for one sample, so every
-// sample's output equals that of a forward over it alone.
+// sample's output equals that of a forward over it alone.  The 16-output-
+// channel brick kernels are the exception: their 4- and 8-row forms add
+// every tile's taps and slices in the same order, so the batch takes the
+// rows picked for the batch's own shape.
 void sur_conv(const dh_sur_net *net, int li, const uint16_t *xa, const uint16_t *xb, int n, int d, int h, int w,
               void *y, float *y32, hipStream_t s)
 {
@@ -1356,6 +1389,11 @@ void sur_conv(const dh_sur_net *net, int li, const uint16_t *xa, const uint16_t 
     int nco, ksplit;
     conv_pick(a, nco, ksplit);
     a.N = n;
+    if (n > 1 && ksplit == -3 && a.ksize == 3) {
+        int nb, kb;
+        conv_pick(a, nb, kb);
+        if (kb == -3) nco = nb;
+    }
     launch_conv3d(a, nco, ksplit, s);
 }
 
@@ -1563,8 +1601,10 @@ void dh_launch_sur_dose_batch(const dh_sur_net *net, const dh_ccc_args *cas, int
         }
         {
             dh_timed t("sur_finish", s);
-            hipLaunchKernelGGL(k_sur_finish_batch, dim3((unsigned)std::min((nz + 1) / 2 * ((ny + 1) / 2), 512), n),
-                               dim3(256), 0, s, b, cy, cx, net->out_scale, nz, ny, nx);
+            // about 2048 workgroups over the batch (at least 512 per point)
+            const int fgrid = std::min((nz + 1) / 2 * ((ny + 1) / 2), std::max(512, 2048 / n));
+            hipLaunchKernelGGL(k_sur_finish_batch, dim3((unsigned)fgrid, n), dim3(256), 0, s, b, cy, cx,
+                               net->out_scale, nz, ny, nx);
             DH_HIP(hipGetLastError());
         }
         sur_check_multi(dev, cas + c0, n, b, density, stride, tol, dilate, s);

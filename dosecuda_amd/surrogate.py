@@ -14,11 +14,12 @@ hand-written BF16 MFMA kernel of libdosehip.so with FP32 accumulation; torch
 supplies memory, pooling/upsampling/concatenation glue and the Adam update
 of the FP32 master weights.
 
-Trust, on the timed path: the plain CCCS (`k_ccc`) is evaluated on a sparse
-lattice of voxels (every `stride`-th voxel along each axis among voxels with
-TERMA > 0.01); every lattice cell whose surrogate value differs from the
-plain value by more than `tol` of the maximum, together with its neighbour
-cells, is recomputed with the plain CCCS voxel by voxel.  The full plain
+Trust, on the timed path: the plain CCCS (`k_ccc`) is evaluated at one
+voxel of each lattice cell of `stride`^3 voxels (its low corner when that has
+TERMA > 0.01, else its centre voxel); every cell whose surrogate value there
+differs from the plain value by more than `tol` of the maximum, together with
+its neighbour cells, and every cell with neither voxel active is recomputed
+with the plain CCCS voxel by voxel.  The full plain
 CCCS and the 2%/2 mm gamma index score it, off the timed path.
 
 No reference counterpart: DoseCUDA computes only the plain CCCS

@@ -301,13 +301,15 @@ int dh_sur_pack_dev(int32_t nz, int32_t ny, int32_t nx, const float *d_terma,
 int dh_sur_finish_dev(int32_t nz, int32_t ny, int32_t nx, const float *d_y32, const float *d_tmax,
                       float out_scale, const float *d_terma, const float *d_electron,
                       float mu_scale, float *d_dose_out, float *d_dmax_out, void *stream);
-/* Plain-CCCS check and fallback of a surrogate dose, all on the device: the
- * plain CCCS on every stride-th voxel (per axis) with TERMA > 0.01; every
- * lattice cell whose surrogate value misses the plain one by more than
- * tol * dmax fails; voxels with TERMA > 0.01 in a failed cell or next to one
- * (dilate & 3: 0 none, 1 the 6 face neighbours, 2 all 26 neighbours of
- * cells), or in a cell whose lattice voxel has TERMA <= 0.01, get the plain
- * CCCS value in d_dose; with dilate & 4 so do all voxels with TERMA > 0.01
+/* Plain-CCCS check and fallback of a surrogate dose, all on the device: each
+ * lattice cell of stride^3 voxels is checked at one voxel, its low corner
+ * (every stride-th voxel per axis) when that has TERMA > 0.01, else its
+ * centre voxel (offset stride / 2 per axis) when that lies in the volume with
+ * TERMA > 0.01; the plain CCCS runs on those voxels and a cell whose
+ * surrogate value misses the plain one by more than tol * dmax fails; voxels
+ * with TERMA > 0.01 in a failed cell or next to one (dilate & 3: 0 none, 1
+ * the 6 face neighbours, 2 all 26 neighbours of cells), or in a cell with no
+ * check voxel, get the plain CCCS value in d_dose; with dilate & 4 so do all voxels with TERMA > 0.01
  * whose electron-contamination term is nonzero (the build-up band, WET below
  * the model's electron_fitted_dmax).  Every dilate argument of the surrogate
  * entry points below takes these values (0-2, 4-6).  d_work holds int32
@@ -353,12 +355,10 @@ int dh_sur_dose_batch_dev(const dh_sur_net *net, const dh_grid *grid, int32_t n_
                           const float *const *d_terma, const float *d_density,
                           const float *const *d_electron, int32_t stride, float tol, int32_t dilate,
                           float *const *d_dose_out, void *stream);
-/* Lattice and fallback voxel counts of the last checked surrogate dose run
- * by dh_sur_dose_dev / dh_photon_dose_surrogate_dev from this thread on
- * gpu_id (synchronises the device).  After dh_sur_dose_batch_dev they are
- * those of the batch's last point checked on the caller's stream (the last
- * even-numbered point; odd-numbered points are checked on a side stream
- * with buffers of their own). */
+/* Lattice (check voxel) and fallback voxel counts of the last checked
+ * surrogate dose run by dh_sur_dose_dev / dh_photon_dose_surrogate_dev from
+ * this thread on gpu_id (synchronises the device).  After
+ * dh_sur_dose_batch_dev they are those of the batch's last point. */
 int dh_sur_counts(int gpu_id, int32_t *lattice, int32_t *fallback);
 /* dh_photon_dose_dev with the cone superposition replaced by the checked
  * surrogate; the radiological depth comes from the beam's-eye-view grid
