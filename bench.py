@@ -681,7 +681,8 @@ def main():
             net = S.load(S.WEIGHTS)
             if not a.no_tune:
                 t0 = time.time()
-                autotune.tune_surrogate(net, (64, 64, 64), reps=5)
+                # the plan's 7 beams run through the network as one batch
+                autotune.tune_surrogate(net, (64, 64, 64), reps=5, batch=7)
                 log(f"surrogate: convolution variants tuned ({time.time() - t0:.1f} s)")
             log("surrogate: 7-beam plan, checked surrogate vs plain CCCS")
             r = S.evaluate(net, PhotonBeamModel(), stride=S.DEFAULT_CHECK[0], tol=S.DEFAULT_CHECK[1],
@@ -721,9 +722,12 @@ def main():
                 f"MFMA {tf:.0f} TFLOP/s")
 
     if a.workload == "surrogate_path":
-        from dosecuda_amd import surrogate as S
+        from dosecuda_amd import autotune, surrogate as S
         from dosecuda_amd.beam_models import PhotonBeamModel
-        r = S.time_plan_path(S.load(S.WEIGHTS), PhotonBeamModel(), reps=a.steps)
+        net = S.load(S.WEIGHTS)
+        if not a.no_tune:
+            autotune.tune_surrogate(net, (64, 64, 64), reps=5, batch=7)
+        r = S.time_plan_path(net, PhotonBeamModel(), reps=a.steps)
         res["surrogate_path"] = r
         log(f"surrogate path: {r['ms_per_beam']:.4f} ms/beam over {a.steps} replays, min beam gamma "
             f"{r['min_gamma']:.5f}")

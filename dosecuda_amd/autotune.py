@@ -151,10 +151,24 @@ def surrogate_layer_shapes(net, cells):
             s(1, c, b, 1, b), s(1, b, 0, 0, b), s(0, b, a, 1, a), s(0, a, 0, 0, a), s(0, a, 0, 0, 16, 1)]
 
 
-def tune_surrogate(net, cells, reps=5):
+def conv_key(n, d, h, w, ca, cb, up, cout, ksize):
+    """The library's tuning key of a convolution shape."""
+    return f"{n},{d},{h},{w},{ca},{cb},{1 if up else 0},{cout},{ksize}"
+
+
+def tune_surrogate(net, cells, reps=5, batch=1):
+    """Tunes every layer of the surrogate's forward pass over `cells`.  With
+    batch > 1 each layer is timed over `batch` samples and the fastest
+    variant is stored for that shape and for the one-sample shape too: a
+    batch runs every layer with its one-sample variant (so a batched dose
+    equals the single-point one bit for bit), and that variant is then the
+    batch's fastest."""
     out = {}
     for shp in dict.fromkeys(surrogate_layer_shapes(net, cells)):
-        out[",".join(map(str, shp))] = tune_conv3d(*shp, reps=reps)
+        r = tune_conv3d(batch, *shp[1:], reps=reps)
+        if batch > 1 and r["best"] > 0:
+            set_choice("conv3d", conv_key(1, *shp[1:]), r["best"])
+        out[",".join(map(str, shp))] = r
     return out
 
 

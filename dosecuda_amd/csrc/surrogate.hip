@@ -916,6 +916,87 @@ k_sur_finish_batch(sur_check_batch b, int cy, int cx, float out_scale, int nz, i
     block_max_to(m, b.dmax[q]);
 }
 
+// The network's output layer (k_conv1_c16: 1x1x1, at most 16 outputs, one
+// source, channels 0-7 the sub-voxel outputs) fused with the surrogate dose
+// of a batch of points (the network's cells of point q are cells q * cells
+// .. (q + 1) * cells - 1): each wave forms its 64 cells' outputs with the
+// layer's MFMAs, the workgroup's 256 cells' eight sub-voxel outputs go
+// through LDS, and the workgroup then forms the doses of those cells' voxels
+// as k_sur_finish does (consecutive lanes on consecutive voxels of a row) and
+// adds its maximum to the point's.  The values are exactly those of the
+// layer's fp32 output fed to k_sur_finish.  The first workgroup of each point
+// zeroes the point's check counters.
+__global__ void __launch_bounds__(256)
+k_conv1_finish(conv_args a, sur_check_batch b, float out_scale, int nz, int ny, int nx, int cy, int cx,
+               int64_t cells)
+{
+    __shared__ float s_y[256][9];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, col = lane & 15, kq = lane >> 4;
+    const int64_t cb0 = (int64_t)blockIdx.x * 256;          // first cell of the workgroup (all points)
+    const int q = (int)(cb0 / cells);
+    const int64_t v0 = cb0 + wid * 64;
+    dh_f32x4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) acc[t][r] = 0.0f;
+    for (int c0 = 0; c0 < a.cin; c0 += 32) {
+        const int ci = c0 + 8 * kq;
+        uint4 bv = make_uint4(0, 0, 0, 0);
+        if (ci < a.cin && col < a.cout) bv = *(const uint4 *)(a.w + (int64_t)col * a.cin + ci);
+        dh_bf16x8 B;
+        __builtin_memcpy(&B, &bv, 16);
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int64_t v = v0 + 16 * t + col;
+            uint4 av = make_uint4(0, 0, 0, 0);
+            if (ci < a.cin) av = *(const uint4 *)(a.xa + v * a.cin + ci);
+            dh_bf16x8 A;
+            __builtin_memcpy(&A, &av, 16);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[t], 0, 0, 0);
+        }
+    }
+    if (col < 8) {                       // the eight sub-voxel outputs
+        const float bsv = a.bias ? a.bias[col] : 0.0f;
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                float o = acc[t][r] + bsv;
+                if (a.act) o = o > 0.0f ? o : o * a.slope;
+                s_y[wid * 64 + 16 * t + 4 * kq + r][col] = o;
+            }
+    }
+    if (cb0 % cells == 0 && tid < 4) b.work[q][tid] = 0;
+    __syncthreads();
+    const float *__restrict__ terma = b.terma[q];
+    const float *__restrict__ electron = b.electron[q];
+    float *__restrict__ dose = b.dose[q];
+    const float s = fmaxf(*b.tmax[q], 1e-12f) * out_scale;
+    const float mu_scale = b.mu_scale[q];
+    // cell and voxel indices in 32 bits (a point's volume and cells number
+    // fewer than 2^31)
+    const uint32_t cl0 = (uint32_t)(cb0 - (int64_t)q * cells);    // first cell within the point
+    float m = 0.0f;
+    for (int idx = tid; idx < 2048; idx += 256) {
+        // voxel rows (dz, dy) outermost, then cells, then dx: consecutive
+        // lanes on consecutive voxels of a row
+        const int row = idx >> 9, cell = (idx & 511) >> 1, dx = idx & 1;
+        const int dz = row >> 1, dy = row & 1;
+        const uint32_t c = cl0 + (uint32_t)cell, cr = c / (uint32_t)cx;
+        const int ci = (int)(c - cr * (uint32_t)cx), cj = (int)(cr % (uint32_t)cy), ck = (int)(cr / (uint32_t)cy);
+        const int k = 2 * ck + dz, j = 2 * cj + dy, i = 2 * ci + dx;
+        if (k >= nz || j >= ny || i >= nx) continue;
+        const uint32_t v = ((uint32_t)k * (uint32_t)ny + (uint32_t)j) * (uint32_t)nx + (uint32_t)i;
+        const float sup = s_y[cell][4 * dz + 2 * dy + dx] * s;
+        const float d = (sup + electron[v]) * mu_scale;
+        const float o = (terma[v] > 0.01f && d > 0.0f) ? d : 0.0f;
+        dose[v] = o;
+        m = fmaxf(m, o);
+    }
+    block_max_to(m, b.dmax[q]);
+}
+
 // block-aggregated append of the flagged items of this block to list
 __device__ static inline void block_append(bool flag, int item, int *__restrict__ list, int *__restrict__ counter)
 {
@@ -1398,8 +1479,18 @@ void sur_conv(const dh_sur_net *net, int li, const uint16_t *xa, const uint16_t 
 }
 
 // U-Net forward: x [n][cz][cy][cx][in_ch] bf16 -> y32 [n][cz][cy][cx][16] fp32
-void sur_forward(const dh_sur_net *net, const uint16_t *x, int n, int cz, int cy, int cx, float *y32,
-                 uint16_t *ws, hipStream_t s)
+// With fin (the surrogate doses of the n points, finish arguments in fa):
+// the output layer writes the doses directly (k_conv1_finish) when its
+// picked variant is the 16-channel 1x1 kernel, whose arithmetic the fused
+// kernel repeats; otherwise the layer writes y32 and k_sur_finish_batch
+// follows.  Returns whether the doses were written.
+struct sur_finish_args {
+    float out_scale;
+    int nz, ny, nx;
+};
+bool sur_forward(const dh_sur_net *net, const uint16_t *x, int n, int cz, int cy, int cx, float *y32,
+                 uint16_t *ws, hipStream_t s, const sur_check_batch *fin = nullptr,
+                 const sur_finish_args *fa = nullptr)
 {
     const int64_t L0 = (int64_t)n * cz * cy * cx, L1 = L0 / 8, L2 = L0 / 64, L3 = L0 / 512;
     const int a = net->widths[0], b = net->widths[1], c = net->widths[2], d = net->widths[3];
@@ -1428,7 +1519,24 @@ void sur_forward(const dh_sur_net *net, const uint16_t *x, int n, int cz, int cy
     sur_conv(net, 11, B1, nullptr, n, z1, y1, x1, A1, nullptr, s);
     sur_conv(net, 12, A1, S0, n, cz, cy, cx, B0, nullptr, s);
     sur_conv(net, 13, B0, nullptr, n, cz, cy, cx, A0, nullptr, s);
+    if (fin) {
+        const dh_sur_layer &L = net->L[14];
+        conv_args ca = make_conv(A0, L.ca, L.up, nullptr, L.cb, 1, cz, cy, cx, L.w, L.b, L.cout, L.cout_pad, L.ksize,
+                                 L.act, net->slope, nullptr, y32);
+        int nco, ksplit;
+        conv_pick(ca, nco, ksplit);
+        const int64_t cells = (int64_t)cz * cy * cx;
+        if (nco == 1 && ksplit == -3 && cells % 256 == 0) {
+            ca.N = n;
+            dh_timed t("sur_finish", s);
+            hipLaunchKernelGGL(k_conv1_finish, dim3((unsigned)(n * cells / 256)), dim3(256), 0, s, ca, *fin,
+                               fa->out_scale, fa->nz, fa->ny, fa->nx, cy, cx, cells);
+            DH_HIP(hipGetLastError());
+            return true;
+        }
+    }
     sur_conv(net, 14, A0, nullptr, n, cz, cy, cx, nullptr, y32, s);
+    return false;
 }
 
 }  // namespace
@@ -1584,7 +1692,6 @@ void dh_launch_sur_dose_batch(const dh_sur_net *net, const dh_ccc_args *cas, int
     for (int c0 = 0; c0 < ncp; c0 += nb) {
         const int n = std::min(nb, ncp - c0);
         launch_pack_batch(nz, ny, nx, n, terma + c0, density, dirs + c0, g.s, x, cells * net->in_ch, scal, s);
-        sur_forward(net, x, n, cz, cy, cx, y32, ws, s);
         sur_check_batch b;
         std::memset(&b, 0, sizeof(b));
         for (int i = 0; i < n; i++) {
@@ -1599,7 +1706,8 @@ void dh_launch_sur_dose_batch(const dh_sur_net *net, const dh_ccc_args *cas, int
             b.plain[i] = plain + (size_t)i * nv;
             b.work[i] = work + (size_t)i * wl;
         }
-        {
+        const sur_finish_args fa{net->out_scale, nz, ny, nx};
+        if (!sur_forward(net, x, n, cz, cy, cx, y32, ws, s, &b, &fa)) {
             dh_timed t("sur_finish", s);
             // about 2048 workgroups over the batch (at least 512 per point)
             const int fgrid = std::min((nz + 1) / 2 * ((ny + 1) / 2), std::max(512, 2048 / n));
