@@ -502,6 +502,9 @@ def main():
         print(json.dumps({"error": "no GPU visible"}))
         sys.exit(1)
     _lib.load()
+    if a.no_tune:
+        from dosecuda_amd import autotune
+        log(f"tuning cache {autotune.cache_path()}: {autotune.load()} choices loaded")
     res = {}
     cpu = {}
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -775,6 +778,10 @@ def main():
             f"{res['mixed']['imrt_min_gamma']}, max IMPT CSR error {res['mixed']['impt_max_err_csr_vs_dense']}")
         W.close()
 
+    if not a.no_tune:
+        from dosecuda_amd import autotune
+        if autotune.entries():
+            log(f"tuning cache saved to {autotune.save()}")
     cc = res.get("cccs")
     line = {
         "metric": "IMPT Gvoxel-spot/s + CCCS ms/beam; surrogate vs plain at 2%/2mm gamma>=99%",
