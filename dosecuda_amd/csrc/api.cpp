@@ -365,11 +365,11 @@ static void bev_field_box(const dh_terma_args &ta, float box[4])
 }
 
 static bool bev_wet(int dev, const dh_terma_args &ta, const dh_geom &g, const float4 *vpad, const dh_photon_model *m,
-                    float *wet, hipStream_t s, int table_slot = DH_SLOT_BEV)
+                    float *wet, hipStream_t s, int table_slot = DH_SLOT_BEV, bool table_ready = false)
 {
     float box[4];
     bev_field_box(ta, box);
-    return dh_launch_bev_wet(dev, g, vpad, bev_exact_depth(m), box, ta.sad, wet, s, table_slot);
+    return dh_launch_bev_wet(dev, g, vpad, bev_exact_depth(m), box, ta.sad, wet, s, table_slot, table_ready);
 }
 
 static void photon_device(int dev, const dh_grid *grid, const dh_photon_cp *cp,
@@ -402,11 +402,16 @@ static void photon_device(int dev, const dh_grid *grid, const dh_photon_cp *cp,
     // TERMA is an input of the network and of the plain check alike, and the
     // result is judged by gamma against the plain CCCS.  The plain path keeps
     // the reference's ray trace with 1 mm samples.
+    // The beam's-eye-view depth also depends on the aperture (its exact
+    // shallow trace covers the open leaf pairs' box), so a reused one keeps
+    // only its fan-line table and runs its lookup again.
     float step = net ? 0.0f : 1.0f;
     if (!(reuse_wet && !d_wet && memo.vpad == vpad && memo.wet == wet && memo.step == step)) {
         dh_launch_pad_quad(d_density, vpad, ca.g.nx, ca.g.ny, ca.g.nz, DH_AIR_DENSITY, s);  // IMRTClasses.cu:394
         if (step == 0.0f && !bev_wet(dev, ta, ca.g, vpad, m, wet, s)) step = std::max(1.0f, ca.g.s);
         if (step > 0.0f) dh_launch_raytrace(ca.g, vpad, wet, s, -1, nullptr, step);
+    } else if (step == 0.0f) {
+        DH_REQUIRE(bev_wet(dev, ta, ca.g, vpad, m, wet, s, DH_SLOT_BEV, true), "depth table no longer applies");
     }
     memo.vpad = d_wet ? nullptr : vpad;
     memo.wet = d_wet ? nullptr : wet;
@@ -710,6 +715,61 @@ void dh_photon_enqueue(int dev, const dh_grid *grid, const dh_photon_cp *cp, con
 {
     photon_device(dev, grid, cp, m, mlc_rows, n_pairs, d_density, d_dose, nullptr, nullptr, nullptr, s, net,
                   stride, tol, dilate, reuse_wet);
+}
+
+void dh_photon_sur_group_enqueue(int dev, const dh_grid *grid, int n, const dh_photon_cp *cps,
+                                 const dh_photon_model *m, const float *const *mlc_rows, const int *n_pairs,
+                                 const float *d_density, float *const *d_dose, const dh_sur_net *net, int stride,
+                                 float tol, int dilate, hipStream_t s, bool reuse_first)
+{
+    DH_REQUIRE(net && n >= 1 && n <= kSurGroupMax, "surrogate group of 1..8 control points with a network");
+    const size_t nv = nvox_of(grid);
+    std::vector<dh_terma_args> ta;
+    std::vector<dh_ccc_args> cas;
+    std::vector<float> dirs(3 * (size_t)n);
+    for (int i = 0; i < n; i++) {
+        ta.push_back(dh_make_terma_args(grid, &cps[i], m, mlc_rows[i], n_pairs[i]));
+        cas.push_back(dh_make_ccc_args(grid, &cps[i], m));
+        sur_direction(&cps[i], &dirs[3 * (size_t)i]);
+    }
+    const dh_pad p = cas[0].p;
+    float4 *vpad = (float4 *)dh_scratch(dev, DH_SLOT_VOL_PAD, sizeof(float4) * (size_t)p.pxy * p.pz);
+    float *wet = (float *)dh_scratch(dev, DH_SLOT_WET_RAW, sizeof(float) * nv);
+    float *terma = (float *)dh_scratch(dev, DH_SLOT_TMP0, sizeof(float) * nv * (size_t)n);
+    float *el = (float *)dh_scratch(dev, DH_SLOT_TMP1, sizeof(float) * nv * (size_t)n);
+    wet_memo &memo = g_wet_memo[dev & 63][dh_get_workspace()];
+    std::vector<const float *> tp(n), ep(n);
+    // the padded density once per group (the caller's previous point on this
+    // workspace left it there when reuse_first)
+    if (!(reuse_first && memo.vpad == vpad))
+        dh_launch_pad_quad(d_density, vpad, cas[0].g.nx, cas[0].g.ny, cas[0].g.nz, DH_AIR_DENSITY, s);  // IMRTClasses.cu:394
+    for (int i = 0; i < n; i++) {
+        // each point's depth as photon_device forms it for the surrogate path
+        // (beam's-eye-view grid, else a ray trace at one sample per voxel
+        // spacing); when the source is the same as the previous point's on
+        // this workspace, its fan-line table is reused
+        const bool same = i > 0 ? (cps[i].iso[0] == cps[i - 1].iso[0] && cps[i].iso[1] == cps[i - 1].iso[1] &&
+                                   cps[i].iso[2] == cps[i - 1].iso[2] && cps[i].ga == cps[i - 1].ga &&
+                                   cps[i].ta == cps[i - 1].ta)
+                                : reuse_first;
+        float step = 0.0f;
+        if (!(same && memo.vpad == vpad && memo.wet == wet && memo.step == step)) {
+            if (!bev_wet(dev, ta[i], cas[i].g, vpad, m, wet, s)) step = std::max(1.0f, cas[i].g.s);
+            if (step > 0.0f) dh_launch_raytrace(cas[i].g, vpad, wet, s, -1, nullptr, step);
+        } else {
+            // same source: the fan-line table stands, the aperture-dependent
+            // lookup runs again
+            DH_REQUIRE(bev_wet(dev, ta[i], cas[i].g, vpad, m, wet, s, DH_SLOT_BEV, true), "depth table no longer applies");
+        }
+        memo.vpad = vpad;
+        memo.wet = wet;
+        memo.step = step;
+        tp[i] = terma + nv * (size_t)i;
+        ep[i] = el + nv * (size_t)i;
+        dh_launch_terma(ta[i], wet, terma + nv * (size_t)i, el + nv * (size_t)i, s);
+    }
+    dh_launch_sur_dose_batch(net, cas.data(), n, tp.data(), d_density, ep.data(),
+                             reinterpret_cast<const float (*)[3]>(dirs.data()), stride, tol, dilate, d_dose, s);
 }
 
 void dh_proton_wet_enqueue(int dev, const dh_grid *grid, const dh_proton_beam *beam, const float *d_rlsp,

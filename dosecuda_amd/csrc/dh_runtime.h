@@ -241,15 +241,17 @@ inline void dh_check_dilate_word(int dilate)
 
 // Radiological depth (cm) of every voxel from fan-line integrals on a
 // beam's-eye-view grid (one density sample per line step of one voxel
-// spacing, trapezoid rule, trilinear lookup); false (nothing launched) when
-// the source is not well outside the volume or the table would be too large.
-// Radiological depth (cm) of every voxel from fan-line integrals on a
-// beam's-eye-view grid; voxels shallower than w_exact_cm whose projection on
-// the isocentre plane (source distance sad) lies in field_box {x0, x1, y0, y1}
-// (head frame, mm; null: none) take the 1 mm ray trace.  Returns false
-// (nothing launched) where the grid does not apply.
+// spacing, trapezoid rule, trilinear lookup); voxels shallower than
+// w_exact_cm whose projection on the isocentre plane (source distance sad)
+// lies in field_box {x0, x1, y0, y1} (head frame, mm; null: none) take the
+// 1 mm ray trace.  The table depends on the density, the grid and the source
+// only; table_ready: table_slot already holds it for this geometry and
+// density, and only the (aperture-dependent) lookup runs.  Returns false
+// (nothing launched) when the source is not well outside the volume or the
+// table would be too large.
 bool dh_launch_bev_wet(int dev, const dh_geom &g, const float4 *vol_pad, float w_exact_cm, const float *field_box,
-                       float sad, float *wet, hipStream_t s, int table_slot = DH_SLOT_BEV);
+                       float sad, float *wet, hipStream_t s, int table_slot = DH_SLOT_BEV,
+                       bool table_ready = false);
 void dh_launch_terma(const dh_terma_args &a, const float *wet, float *terma, float *electron,
                      hipStream_t s);
 // Sampling layout of the cone superposition's {TERMA, density} volume:
@@ -296,6 +298,16 @@ void dh_photon_enqueue(int dev, const dh_grid *grid, const dh_photon_cp *cp, con
                        const float *mlc_rows, int n_pairs, const float *d_density, float *d_dose,
                        const dh_sur_net *net, int stride, float tol, int dilate, hipStream_t s,
                        bool reuse_wet = false);
+// Checked surrogate doses of n (1..kSurGroupMax) control points of one grid,
+// density, model and network: each point's depth (reused from the previous
+// point when the source is the same; reuse_first: from the caller's previous
+// point) and TERMA into the workspace, then one batched network and check.
+// Each dose equals dh_photon_enqueue's for that point bit for bit.
+constexpr int kSurGroupMax = 8;
+void dh_photon_sur_group_enqueue(int dev, const dh_grid *grid, int n, const dh_photon_cp *cps,
+                                 const dh_photon_model *m, const float *const *mlc_rows, const int *n_pairs,
+                                 const float *d_density, float *const *d_dose, const dh_sur_net *net, int stride,
+                                 float tol, int dilate, hipStream_t s, bool reuse_first);
 void dh_proton_wet_enqueue(int dev, const dh_grid *grid, const dh_proton_beam *beam, const float *d_rlsp,
                            float *d_wet, hipStream_t s);
 void dh_proton_plan_enqueue(dh_proton_plan *p, const float *d_wet, float *d_dose, hipStream_t s);

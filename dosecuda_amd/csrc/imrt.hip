@@ -824,7 +824,7 @@ void dh_launch_terma(const dh_terma_args &a, const float *wet, float *terma, flo
 }
 
 bool dh_launch_bev_wet(int dev, const dh_geom &g, const float4 *vol_pad, float w_exact_cm, const float *field_box,
-                       float sad_field, float *wet, hipStream_t s, int table_slot)
+                       float sad_field, float *wet, hipStream_t s, int table_slot, bool table_ready)
 {
     const float4 fbox = field_box ? make_float4(field_box[0], field_box[1], field_box[2], field_box[3])
                                 : make_float4(1.0f, -1.0f, 1.0f, -1.0f);
@@ -891,7 +891,7 @@ bool dh_launch_bev_wet(int dev, const dh_geom &g, const float4 *vol_pad, float w
     const dh_pad p = dh_make_pad(g.nx, g.ny, g.nz);
     const size_t nl = (size_t)b.na * b.nb, nvox = (size_t)g.nx * g.ny * g.nz;
     const bool off32 = (size_t)16 * p.pxy * p.pz < ((size_t)1 << 31) && p.pxy < ((int64_t)1 << 23);
-    {
+    if (!table_ready) {
         dh_timed t("bev_prefix", s);
         const unsigned lgrid = (unsigned)((nl + 63) / 64);
         if (off32)

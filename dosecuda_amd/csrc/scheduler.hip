@@ -13,6 +13,8 @@
 // (dosemodule.cu:95-538).
 #include "dh_runtime.h"
 
+#include <cstring>
+
 #include <cmath>
 
 #include <algorithm>
@@ -197,6 +199,62 @@ void enqueue_job(dh_sched *s, job &J, scase &C, hipStream_t st, bool timing, boo
     if (timing) DH_HIP(hipEventRecord(J.ev[1], st));
 }
 
+// The same beam model: every scalar field and table equal.
+bool same_model(const model_copy &a, const model_copy &b)
+{
+    dh_photon_model x = a.m, y = b.m;
+    for (dh_photon_model *m : {&x, &y}) {
+        m->profile_radius = m->profile_intensities = m->profile_softening = nullptr;
+        m->spectrum_attenuation_coefficients = m->spectrum_primary_weights = m->spectrum_scatter_weights = nullptr;
+    }
+    return std::memcmp(&x, &y, sizeof(x)) == 0 && a.pr == b.pr && a.pi == b.pi && a.ps == b.ps && a.mu == b.mu &&
+           a.pw == b.pw && a.sw == b.sw;
+}
+
+// Two checked-surrogate photon jobs of one case that can share a network
+// launch: the same network, check settings and beam model.
+bool same_sur_group(const job &a, const job &b)
+{
+    if (a.kind != J_PHOTON || b.kind != J_PHOTON || !a.net || a.net != b.net) return false;
+    if (a.stride != b.stride || a.tol != b.tol || a.dilate != b.dilate) return false;
+    return same_model(a.model, b.model);
+}
+
+// Up to kSurGroupMax consecutive surrogate photon jobs J[0..n) of case C as
+// one batched network and check (dh_photon_sur_group_enqueue), each dose
+// then added with its own scale.  The group's time is the first job's; the
+// others record empty intervals after it.
+void enqueue_sur_group(dh_sched *s, job *const *J, int n, scase &C, hipStream_t st, bool timing, bool reuse_first)
+{
+    if (timing) DH_HIP(hipEventRecord(J[0]->ev[0], st));
+    const int64_t nv = nvox(C.grid);
+    float *tmp = (float *)dh_scratch(s->gpu, DH_SLOT_OUT1, sizeof(float) * (size_t)(nv * n));
+    std::vector<dh_photon_cp> cps(n);
+    std::vector<const float *> mlc(n);
+    std::vector<int> np(n);
+    std::vector<float *> out(n);
+    for (int i = 0; i < n; i++) {
+        cps[i] = J[i]->cp;
+        mlc[i] = J[i]->mlc.data();
+        np[i] = (int)(J[i]->mlc.size() / 4);
+        out[i] = tmp + (size_t)nv * i;
+    }
+    const job &J0 = *J[0];
+    dh_photon_sur_group_enqueue(s->gpu, &C.grid, n, cps.data(), &J0.model.m, mlc.data(), np.data(), C.vol, out.data(),
+                                J0.net, J0.stride, J0.tol, J0.dilate, st, reuse_first);
+    for (int i = 0; i < n; i++)
+        hipLaunchKernelGGL(k_sched_accum, dim3(grid_of(nv)), dim3(256), 0, st, C.dose, (const float *)out[i], nv,
+                           J[i]->scale);
+    DH_HIP(hipGetLastError());
+    if (timing) {
+        DH_HIP(hipEventRecord(J[0]->ev[1], st));
+        for (int i = 1; i < n; i++) {
+            DH_HIP(hipEventRecord(J[i]->ev[0], st));
+            DH_HIP(hipEventRecord(J[i]->ev[1], st));
+        }
+    }
+}
+
 // all cases' work on the worker streams, after `fork` and before `join`
 void enqueue_batch(dh_sched *s, bool timing)
 {
@@ -211,12 +269,29 @@ void enqueue_batch(dh_sched *s, bool timing)
                 hipLaunchKernelGGL(k_sched_fill, dim3(grid_of(n)), dim3(256), 0, st, C.dose, n);
             }
             // a case's jobs run back to back on its stream and workspace, so a
-            // control point may reuse the ray trace of the one before it
+            // control point may reuse the ray trace of the one before it;
+            // consecutive checked-surrogate control points share network
+            // launches in groups of up to kSurGroupMax
             const job *prev = nullptr;
-            for (int j : C.jobs) {
-                job &J = *s->jobs[j];
-                enqueue_job(s, J, C, st, timing, prev && same_source(*prev, J));
+            for (size_t q = 0; q < C.jobs.size();) {
+                job &J = *s->jobs[C.jobs[q]];
+                const bool reuse = prev && same_source(*prev, J);
+                if (J.kind == J_PHOTON && J.net) {
+                    job *G[kSurGroupMax];
+                    int n = 0;
+                    while (q + n < C.jobs.size() && n < kSurGroupMax &&
+                           same_sur_group(J, *s->jobs[C.jobs[q + n]])) {
+                        G[n] = s->jobs[C.jobs[q + n]].get();
+                        n++;
+                    }
+                    enqueue_sur_group(s, G, n, C, st, timing, reuse);
+                    prev = G[n - 1];
+                    q += n;
+                    continue;
+                }
+                enqueue_job(s, J, C, st, timing, reuse);
                 prev = &J;
+                q++;
             }
         }
         for (size_t k = 0; k < s->streams.size(); k++) {

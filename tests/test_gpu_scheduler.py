@@ -211,3 +211,45 @@ def test_scheduler_surrogate_photon(photon_model):
         for d, r in zip(doses, refs):
             assert t.equal(d, r)
     sched.close()
+
+
+def test_scheduler_surrogate_groups(photon_model):
+    """Eleven checked-surrogate control points of one case (a group of 8 and
+    one of 3, runs of equal gantry angles sharing the depth, scaled doses)
+    equal the scaled sum of single-point surrogate doses bit for bit, eager
+    and as a graph."""
+    t = _torch()
+    from dosecuda_amd import _lib, engine, surrogate as S
+    from dosecuda_amd.beam_models import square_field_mlc
+    from dosecuda_amd.scheduler import Scheduler
+    pm = photon_model
+    net = S.SurrogateNet().cuda()
+    net.out_scale.fill_(1.5)
+    native = S.NativeNet(net)
+    rng = np.random.default_rng(7)
+    rho, s, iso, _ = _photon_cases(t, rng, n_cases=1)[0]
+    gantries = [0.0, 0.0, 0.0, 40.0, 40.0, 90.0, 90.0, 90.0, 200.0, 200.0, 300.0]
+    print(f"[case] scheduler surrogate groups of {len(gantries)} control points", flush=True)
+    sched = Scheduler(0, streams=1)
+    d = t.empty_like(rho)
+    cid = sched.add_case(rho, s, np.zeros(3), d)
+    ref = t.zeros_like(rho)
+    for q, ga in enumerate(gantries):
+        fs = 4.0 + (q % 4)
+        scale = float(np.float32(0.5 + 0.25 * q))
+        mlc = square_field_mlc(pm, fs, fs + 1.0)
+        sched.add_photon(cid, iso, 100.0, ga, 0.0, 0.0, pm, mlc, surrogate=native, stride=4, tol=0.02, dilate=1,
+                         scale=scale)
+        one = t.empty_like(rho)
+        keep: list = []
+        m = engine.photon_args(pm, keep)
+        rows = np.ascontiguousarray(np.asarray(mlc, dtype=np.float32)[:, :4])
+        _lib.check(_lib.load().dh_photon_dose_surrogate_dev(
+            0, native.handle, _lib.grid(rho.shape, s, np.zeros(3)), _lib.photon_cp(iso, 100.0, ga, 0.0, 0.0),
+            m, _lib.ptr(rows), rows.shape[0], _lib.ptr(rho), _lib.ptr(one), None, None, 4, 0.02, 1, S._st()))
+        ref = ref + one * scale
+    for graph in (False, True, True):
+        sched.run(graph=graph)
+        t.cuda.synchronize()
+        assert t.equal(d, ref)
+    sched.close()
