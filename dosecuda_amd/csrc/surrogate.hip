@@ -1188,7 +1188,7 @@ void conv_pick(const conv_args &a, int &nco, int &ksplit)
     if (nco == 0 && ksplit == 0) {
         if (a.cout <= 16 && a.ksize == 1 && a.cb == 0 && a.up == 0) { nco = 1; ksplit = -3; }
         else if (a.cout <= 16 && a.ksize == 3 && nvox >= 32768) { nco = nvox >= (1 << 20) ? 4 : 8; ksplit = -3; }
-        else if (a.ksize == 3 && nvox >= 32768 && a.cout_pad <= 64) ksplit = nvox < 131072 ? -2 : -1;
+        else if (a.ksize == 3 && nvox >= 4096 && a.cout_pad <= 64) { nco = 1; ksplit = -1; }
     }
     if (ksplit >= 0 && (nco <= 0 || ksplit <= 0)) conv_shape(nvox, a.cout_pad, ki, nco, ksplit);
 }
@@ -1208,12 +1208,12 @@ void launch_conv3d(const conv_args &a, int nco, int ksplit, hipStream_t s)
     }
     if (nco == 0 && ksplit == 0) {
         // untuned shape: the variants the tuner picks for the surrogate's
-        // layers (16-output-channel kernels for <= 16 outputs, four-row bricks
-        // from 2^20 voxels (batches of points); LDS bricks for large 3x3x3
-        // layers, two-row bricks below 2^17 voxels)
+        // layers timed over batches of points (16-output-channel kernels for
+        // <= 16 outputs, four-row bricks from 2^20 voxels; four-row LDS bricks
+        // for 3x3x3 layers of at most 64 outputs from 4096 voxels)
         if (a.cout <= 16 && a.ksize == 1 && a.cb == 0 && a.up == 0) { nco = 1; ksplit = -3; }
         else if (a.cout <= 16 && a.ksize == 3 && nvox >= 32768) { nco = nvox >= (1 << 20) ? 4 : 8; ksplit = -3; }
-        else if (a.ksize == 3 && nvox >= 32768 && a.cout_pad <= 64) ksplit = nvox < 131072 ? -2 : -1;
+        else if (a.ksize == 3 && nvox >= 4096 && a.cout_pad <= 64) { nco = 1; ksplit = -1; }
     }
     int c = nco, k = ksplit;
     dh_timed t("nn_conv3d", s);
