@@ -494,6 +494,9 @@ k_ccc(dh_ccc_args a, const void *__restrict__ tr, const int *__restrict__ list, 
 // to end, so short lists share the chip instead of leaving it idle.  A point's axis and cone directions
 // are formed from its geometry with the host's FP32 (dh_uvec, dh_head_to_img
 // of the head-frame directions), so each voxel's value equals k_ccc's.
+// the struct travels as a kernel argument (at most 4 KiB)
+static_assert(sizeof(dh_ccc_multi) <= 4096, "cone arguments of a multi-point launch exceed the kernel argument space");
+
 template <bool Off32>
 __global__ void __launch_bounds__(kBlock)
 k_ccc_multi(dh_ccc_multi m)

@@ -854,6 +854,8 @@ struct sur_check_batch {
     int *work[kPackBatchMax];
 };
 
+static_assert(sizeof(sur_check_batch) <= 4096, "check batch arguments exceed the kernel argument space");
+
 __device__ static inline int *chk_lat_ok(int *w) { return w + 4; }
 __device__ static inline int *chk_bad(int *w, int64_t nc) { return w + 4 + nc; }
 __device__ static inline int *chk_redo(int *w, int64_t nc) { return w + 4 + 2 * nc; }
