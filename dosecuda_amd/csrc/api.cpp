@@ -364,12 +364,15 @@ static void bev_field_box(const dh_terma_args &ta, float box[4])
     box[2] = y0 - margin; box[3] = y1 + margin;
 }
 
-static bool bev_wet(int dev, const dh_terma_args &ta, const dh_geom &g, const float4 *vpad, const dh_photon_model *m,
-                    float *wet, hipStream_t s, int table_slot = DH_SLOT_BEV, bool table_ready = false)
+// TERMA from the beam's-eye-view depth in one pass (dh_launch_bev_terma)
+static bool bev_terma(int dev, const dh_terma_args &ta, const float4 *vpad, const dh_photon_model *m, float *terma,
+                      float *electron, float *wet_out, hipStream_t s, int table_slot = DH_SLOT_BEV,
+                      bool table_ready = false)
 {
     float box[4];
     bev_field_box(ta, box);
-    return dh_launch_bev_wet(dev, g, vpad, bev_exact_depth(m), box, ta.sad, wet, s, table_slot, table_ready);
+    return dh_launch_bev_terma(dev, ta, vpad, bev_exact_depth(m), box, terma, electron, wet_out, s, table_slot,
+                               table_ready);
 }
 
 static void photon_device(int dev, const dh_grid *grid, const dh_photon_cp *cp,
@@ -405,18 +408,26 @@ static void photon_device(int dev, const dh_grid *grid, const dh_photon_cp *cp,
     // The beam's-eye-view depth also depends on the aperture (its exact
     // shallow trace covers the open leaf pairs' box), so a reused one keeps
     // only its fan-line table and runs its lookup again.
+    // On the beam's-eye-view path the depth goes straight into TERMA (one
+    // pass, written to d_wet only when the caller asked for it).
     float step = net ? 0.0f : 1.0f;
+    bool done = false;
     if (!(reuse_wet && !d_wet && memo.vpad == vpad && memo.wet == wet && memo.step == step)) {
         dh_launch_pad_quad(d_density, vpad, ca.g.nx, ca.g.ny, ca.g.nz, DH_AIR_DENSITY, s);  // IMRTClasses.cu:394
-        if (step == 0.0f && !bev_wet(dev, ta, ca.g, vpad, m, wet, s)) step = std::max(1.0f, ca.g.s);
+        if (step == 0.0f) {
+            done = bev_terma(dev, ta, vpad, m, terma, el, d_wet, s);
+            if (!done) step = std::max(1.0f, ca.g.s);
+        }
         if (step > 0.0f) dh_launch_raytrace(ca.g, vpad, wet, s, -1, nullptr, step);
     } else if (step == 0.0f) {
-        DH_REQUIRE(bev_wet(dev, ta, ca.g, vpad, m, wet, s, DH_SLOT_BEV, true), "depth table no longer applies");
+        DH_REQUIRE(bev_terma(dev, ta, vpad, m, terma, el, nullptr, s, DH_SLOT_BEV, true),
+                   "depth table no longer applies");
+        done = true;
     }
     memo.vpad = d_wet ? nullptr : vpad;
     memo.wet = d_wet ? nullptr : wet;
     memo.step = step;
-    dh_launch_terma(ta, wet, terma, el, s);
+    if (!done) dh_launch_terma(ta, wet, terma, el, s);
     if (net) {
         float dir[3];
         sur_direction(cp, dir);
@@ -443,9 +454,15 @@ extern "C" int dh_photon_terma_dev(int gpu_id, const dh_grid *grid, const dh_pho
         float4 *vpad = (float4 *)dh_scratch(gpu_id, DH_SLOT_VOL_PAD, sizeof(float4) * (size_t)p.pxy * p.pz);
         float *wet = (float *)dh_scratch(gpu_id, DH_SLOT_WET_RAW, sizeof(float) * n);
         dh_launch_pad_quad(d_density, vpad, ta.g.nx, ta.g.ny, ta.g.nz, DH_AIR_DENSITY, s);  // IMRTClasses.cu:394
-        if (step_mm == 0.0f && !bev_wet(gpu_id, ta, ta.g, vpad, model, wet, s)) step_mm = std::max(1.0f, ta.g.s);
-        if (step_mm > 0.0f) dh_launch_raytrace(ta.g, vpad, wet, s, -1, nullptr, step_mm);
-        dh_launch_terma(ta, wet, d_terma_out, d_electron_out, s);
+        bool done = false;
+        if (step_mm == 0.0f) {
+            done = bev_terma(gpu_id, ta, vpad, model, d_terma_out, d_electron_out, nullptr, s);
+            if (!done) step_mm = std::max(1.0f, ta.g.s);
+        }
+        if (!done) {
+            dh_launch_raytrace(ta.g, vpad, wet, s, -1, nullptr, step_mm);
+            dh_launch_terma(ta, wet, d_terma_out, d_electron_out, s);
+        }
         g_wet_memo[gpu_id & 63][dh_get_workspace()] = wet_memo{};   // the padded volume was rewritten
     });
 }
@@ -494,10 +511,16 @@ extern "C" int dh_photon_terma_batch_dev(int gpu_id, const dh_grid *grid, int32_
             const int k = side ? (i & 1) : 0;
             hipStream_t st = k ? side->st : s;
             float step = step_mm;
-            if (step == 0.0f && !bev_wet(gpu_id, ta[i], ta[i].g, vpad, model, wet[k], st, k ? DH_SLOT_BEV2 : DH_SLOT_BEV))
-                step = std::max(1.0f, ta[i].g.s);
-            if (step > 0.0f) dh_launch_raytrace(ta[i].g, vpad, wet[k], st, -1, nullptr, step);
-            dh_launch_terma(ta[i], wet[k], d_terma_out[i], d_electron_out[i], st);
+            bool done = false;
+            if (step == 0.0f) {
+                done = bev_terma(gpu_id, ta[i], vpad, model, d_terma_out[i], d_electron_out[i], nullptr, st,
+                                 k ? DH_SLOT_BEV2 : DH_SLOT_BEV);
+                if (!done) step = std::max(1.0f, ta[i].g.s);
+            }
+            if (!done) {
+                dh_launch_raytrace(ta[i].g, vpad, wet[k], st, -1, nullptr, step);
+                dh_launch_terma(ta[i], wet[k], d_terma_out[i], d_electron_out[i], st);
+            }
         }
         if (side) {
             DH_HIP(hipEventRecord(side->join, side->st));
@@ -753,20 +776,27 @@ void dh_photon_sur_group_enqueue(int dev, const dh_grid *grid, int n, const dh_p
                                    cps[i].ta == cps[i - 1].ta)
                                 : reuse_first;
         float step = 0.0f;
+        float *ti = terma + nv * (size_t)i, *ei = el + nv * (size_t)i;
+        bool done = false;
         if (!(same && memo.vpad == vpad && memo.wet == wet && memo.step == step)) {
-            if (!bev_wet(dev, ta[i], cas[i].g, vpad, m, wet, s)) step = std::max(1.0f, cas[i].g.s);
-            if (step > 0.0f) dh_launch_raytrace(cas[i].g, vpad, wet, s, -1, nullptr, step);
+            done = bev_terma(dev, ta[i], vpad, m, ti, ei, nullptr, s);
+            if (!done) {
+                step = std::max(1.0f, cas[i].g.s);
+                dh_launch_raytrace(cas[i].g, vpad, wet, s, -1, nullptr, step);
+            }
         } else {
             // same source: the fan-line table stands, the aperture-dependent
             // lookup runs again
-            DH_REQUIRE(bev_wet(dev, ta[i], cas[i].g, vpad, m, wet, s, DH_SLOT_BEV, true), "depth table no longer applies");
+            DH_REQUIRE(bev_terma(dev, ta[i], vpad, m, ti, ei, nullptr, s, DH_SLOT_BEV, true),
+                       "depth table no longer applies");
+            done = true;
         }
         memo.vpad = vpad;
         memo.wet = wet;
         memo.step = step;
-        tp[i] = terma + nv * (size_t)i;
-        ep[i] = el + nv * (size_t)i;
-        dh_launch_terma(ta[i], wet, terma + nv * (size_t)i, el + nv * (size_t)i, s);
+        tp[i] = ti;
+        ep[i] = ei;
+        if (!done) dh_launch_terma(ta[i], wet, ti, ei, s);
     }
     dh_launch_sur_dose_batch(net, cas.data(), n, tp.data(), d_density, ep.data(),
                              reinterpret_cast<const float (*)[3]>(dirs.data()), stride, tol, dilate, d_dose, s);

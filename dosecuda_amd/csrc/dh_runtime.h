@@ -239,19 +239,21 @@ inline void dh_check_dilate_word(int dilate)
     DH_REQUIRE(dilate >= 0 && dilate <= 6 && (dilate & 3) != 3, "dilate must be 0, 1 or 2, plus 4 for the build-up band");
 }
 
-// Radiological depth (cm) of every voxel from fan-line integrals on a
-// beam's-eye-view grid (one density sample per line step of one voxel
-// spacing, trapezoid rule, trilinear lookup); voxels shallower than
-// w_exact_cm whose projection on the isocentre plane (source distance sad)
-// lies in field_box {x0, x1, y0, y1} (head frame, mm; null: none) take the
-// 1 mm ray trace.  The table depends on the density, the grid and the source
-// only; table_ready: table_slot already holds it for this geometry and
-// density, and only the (aperture-dependent) lookup runs.  Returns false
-// (nothing launched) when the source is not well outside the volume or the
-// table would be too large.
-bool dh_launch_bev_wet(int dev, const dh_geom &g, const float4 *vol_pad, float w_exact_cm, const float *field_box,
-                       float sad, float *wet, hipStream_t s, int table_slot = DH_SLOT_BEV,
-                       bool table_ready = false);
+// TERMA and electron volumes (as dh_launch_terma) from the radiological depth
+// (cm) of every voxel by fan-line integrals on a beam's-eye-view grid (one
+// density sample per line step of one voxel spacing, trapezoid rule,
+// trilinear lookup), the depth formed per voxel and fed straight into TERMA;
+// voxels shallower than w_exact_cm whose projection on the isocentre plane
+// (source distance a.sad) lies in field_box {x0, x1, y0, y1} (head frame,
+// mm; null: none) take the 1 mm ray trace.  The table depends on the
+// density, the grid and the source only; table_ready: table_slot already
+// holds it for this geometry and density, and only the (aperture-dependent)
+// lookup and TERMA run.  wet_out, when not null, receives the depth.  Returns
+// false (nothing launched) when the source is not well outside the volume or
+// the table would be too large.
+bool dh_launch_bev_terma(int dev, const dh_terma_args &a, const float4 *vol_pad, float w_exact_cm,
+                         const float *field_box, float *terma, float *electron, float *wet_out, hipStream_t s,
+                         int table_slot = DH_SLOT_BEV, bool table_ready = false);
 void dh_launch_terma(const dh_terma_args &a, const float *wet, float *terma, float *electron,
                      hipStream_t s);
 // Sampling layout of the cone superposition's {TERMA, density} volume:

@@ -132,19 +132,12 @@ __device__ static inline float head_transmission(const dh_terma_args &a, float h
     return tr;
 }
 
-// TERMA and electron contamination (IMRTClasses.cu:209-250).
-__global__ void __launch_bounds__(kBlock)
-k_terma(dh_terma_args a, const float *__restrict__ wet, float *__restrict__ terma,
-        float *__restrict__ electron)
+// TERMA and electron contamination (IMRTClasses.cu:209-250) of voxel v =
+// (i, j, k) at radiological depth W (cm).
+__device__ static inline void terma_voxel(const dh_terma_args &a, uint32_t v, int i, int j, int k, float W,
+                                          float *__restrict__ terma, float *__restrict__ electron)
 {
     const dh_geom &g = a.g;
-    const uint32_t nvox = (uint32_t)g.nx * g.ny * g.nz;
-    const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
-    if (v >= nvox) return;
-    const int i = (int)(v % (uint32_t)g.nx);
-    const uint32_t r = v / (uint32_t)g.nx;
-    const int j = (int)(r % (uint32_t)g.ny);
-    const int k = (int)(r / (uint32_t)g.ny);
     float x, y, z, hx, hy, hz;
     dh_vox_xyz(g, i, j, k, x, y, z);
     dh_img_to_head(g, x, y, z, hx, hy, hz);
@@ -166,7 +159,6 @@ k_terma(dh_terma_args a, const float *__restrict__ wet, float *__restrict__ term
     }
     const float Tp = head_transmission(a, hx, hy, hz, a.sad, a.inv_p);
     const float Ts = head_transmission(a, hx, hy, hz, a.ssd_s, a.inv_s_src);
-    const float W = wet[v];
     const float tr = fminf(1.0f, Ts / Tp);
     const float ip = a.sad / dps, is = a.ssd_s / dss;
     const float fp = (1.0f - tr) * (ip * ip), fs = tr * is;
@@ -178,6 +170,19 @@ k_terma(dh_terma_args a, const float *__restrict__ wet, float *__restrict__ term
     const float el = fmaxf(0.0f, (expf(-a.el_att * W) - a.el_norm_a) / a.el_norm_b);
     terma[v] = oaf * ((1.0f - a.ssw) * Tp * t + a.ssw * Ts * t);
     electron[v] = a.esw * (0.4f + (0.3f * tr)) * el * Tp;
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_terma(dh_terma_args a, const float *__restrict__ wet, float *__restrict__ terma,
+        float *__restrict__ electron)
+{
+    const dh_geom &g = a.g;
+    const uint32_t nvox = (uint32_t)g.nx * g.ny * g.nz;
+    const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+    if (v >= nvox) return;
+    const int i = (int)(v % (uint32_t)g.nx);
+    const uint32_t r = v / (uint32_t)g.nx;
+    terma_voxel(a, v, i, (int)(r % (uint32_t)g.ny), (int)(r / (uint32_t)g.ny), wet[v], terma, electron);
 }
 
 // {TERMA, density} of the padded grid (border TERMA 0, density air) in one
@@ -737,17 +742,10 @@ k_bev_prefix(dh_geom g, dh_pad p, bev_grid b, const float4 *__restrict__ vol, fl
 // volume.  These rays are short, so the exact depth of the build-up region
 // costs a small fraction of a whole-volume trace.
 template <bool Off32>
-__global__ void __launch_bounds__(kBlock)
-k_bev_lookup(dh_geom g, dh_pad p, bev_grid b, const float *__restrict__ table, const float4 *__restrict__ vol,
-             float w_exact, float4 fbox, float sad_h, float *__restrict__ wet)
+__device__ static inline float bev_depth(const dh_geom &g, const dh_pad &p, const bev_grid &b,
+                                         const float *__restrict__ table, const float4 *__restrict__ vol,
+                                         float w_exact, const float4 &fbox, float sad_h, float x, float y, float z)
 {
-    const uint32_t nvox = (uint32_t)g.nx * g.ny * g.nz;
-    const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
-    if (v >= nvox) return;
-    const int i = (int)(v % (uint32_t)g.nx);
-    const uint32_t r = v / (uint32_t)g.nx;
-    float x, y, z;
-    dh_vox_xyz(g, i, (int)(r % (uint32_t)g.ny), (int)(r / (uint32_t)g.ny), x, y, z);
     const float wx = x - g.src[0], wy = y - g.src[1], wz = z - g.src[2];
     const float wc = fmaf(wx, b.c[0], fmaf(wy, b.c[1], wz * b.c[2]));
     const float lam = b.sad / wc;     // the host checked wc > 0 for the whole volume
@@ -812,7 +810,29 @@ k_bev_lookup(dh_geom g, dh_pad p, bev_grid b, const float *__restrict__ table, c
         acc += (double)part;
         w = (float)(-0.05 + 0.1 * acc);
     }
-    wet[v] = w;
+    return w;
+}
+
+// The depth of a voxel from the table (bev_depth) goes straight into its
+// TERMA (terma_voxel), written to wet_out too when that is not null.
+template <bool Off32>
+__global__ void __launch_bounds__(kBlock)
+k_terma_bev(dh_terma_args a, dh_pad p, bev_grid b, const float *__restrict__ table, const float4 *__restrict__ vol,
+            float w_exact, float4 fbox, float sad_h, float *__restrict__ wet_out, float *__restrict__ terma,
+            float *__restrict__ electron)
+{
+    const dh_geom &g = a.g;
+    const uint32_t nvox = (uint32_t)g.nx * g.ny * g.nz;
+    const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+    if (v >= nvox) return;
+    const int i = (int)(v % (uint32_t)g.nx);
+    const uint32_t r = v / (uint32_t)g.nx;
+    const int j = (int)(r % (uint32_t)g.ny), k = (int)(r / (uint32_t)g.ny);
+    float x, y, z;
+    dh_vox_xyz(g, i, j, k, x, y, z);
+    const float w = bev_depth<Off32>(g, p, b, table, vol, w_exact, fbox, sad_h, x, y, z);
+    if (wet_out) wet_out[v] = w;
+    terma_voxel(a, v, i, j, k, w, terma, electron);
 }
 
 }  // namespace
@@ -826,8 +846,20 @@ void dh_launch_terma(const dh_terma_args &a, const float *wet, float *terma, flo
     DH_HIP(hipGetLastError());
 }
 
-bool dh_launch_bev_wet(int dev, const dh_geom &g, const float4 *vol_pad, float w_exact_cm, const float *field_box,
-                       float sad_field, float *wet, hipStream_t s, int table_slot, bool table_ready)
+namespace {
+// The beam's-eye-view grid of a geometry and its table in table_slot (the
+// prefix launch unless table_ready); false (nothing launched) where the grid
+// does not apply.
+struct bev_setup {
+    bev_grid b;
+    float *table;
+    dh_pad p;
+    float4 fbox;
+    bool off32;
+    size_t nvox;
+};
+bool bev_table(int dev, const dh_geom &g, const float4 *vol_pad, const float *field_box, int table_slot,
+               bool table_ready, hipStream_t s, bev_setup &out)
 {
     const float4 fbox = field_box ? make_float4(field_box[0], field_box[1], field_box[2], field_box[3])
                                 : make_float4(1.0f, -1.0f, 1.0f, -1.0f);
@@ -903,13 +935,29 @@ bool dh_launch_bev_wet(int dev, const dh_geom &g, const float4 *vol_pad, float w
             hipLaunchKernelGGL(k_bev_prefix<false>, dim3(lgrid), dim3(64 * kBevSeg), 0, s, g, p, b, vol_pad, table);
         DH_HIP(hipGetLastError());
     }
-    dh_timed t("bev_lookup", s);
-    if (off32)
-        hipLaunchKernelGGL(k_bev_lookup<true>, dim3(grid_for(nvox)), dim3(kBlock), 0, s, g, p, b, table, vol_pad,
-                           w_exact_cm, fbox, sad_field, wet);
+    out.b = b;
+    out.table = table;
+    out.p = p;
+    out.fbox = fbox;
+    out.off32 = off32;
+    out.nvox = nvox;
+    return true;
+}
+}  // namespace
+
+bool dh_launch_bev_terma(int dev, const dh_terma_args &a, const float4 *vol_pad, float w_exact_cm,
+                         const float *field_box, float *terma, float *electron, float *wet_out, hipStream_t s,
+                         int table_slot, bool table_ready)
+{
+    bev_setup u;
+    if (!bev_table(dev, a.g, vol_pad, field_box, table_slot, table_ready, s, u)) return false;
+    dh_timed t("terma", s);
+    if (u.off32)
+        hipLaunchKernelGGL(k_terma_bev<true>, dim3(grid_for(u.nvox)), dim3(kBlock), 0, s, a, u.p, u.b, u.table,
+                           vol_pad, w_exact_cm, u.fbox, a.sad, wet_out, terma, electron);
     else
-        hipLaunchKernelGGL(k_bev_lookup<false>, dim3(grid_for(nvox)), dim3(kBlock), 0, s, g, p, b, table, vol_pad,
-                           w_exact_cm, fbox, sad_field, wet);
+        hipLaunchKernelGGL(k_terma_bev<false>, dim3(grid_for(u.nvox)), dim3(kBlock), 0, s, a, u.p, u.b, u.table,
+                           vol_pad, w_exact_cm, u.fbox, a.sad, wet_out, terma, electron);
     DH_HIP(hipGetLastError());
     return true;
 }

@@ -977,7 +977,7 @@ def evaluate(net: SurrogateNet, model, n_beams=7, stride=3, tol=0.01, reps=5, lo
     torch.cuda.synchronize()
     engine.prof_enable(False)
     res["path_stage_ms_per_beam"] = {k: engine.prof_get(k)[0] / reps / len(angles)
-                                     for k in ("pad", "bev_prefix", "bev_lookup", "raytrace", "terma", "nn_conv3d", "ccc_list",
+                                     for k in ("pad", "bev_prefix", "raytrace", "terma", "nn_conv3d", "ccc_list",
                                                "sur_pack", "sur_finish", "sur_lattice", "sur_fallback")}
     engine.prof_reset()
     engine.prof_enable(True)
