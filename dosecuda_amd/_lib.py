@@ -101,6 +101,7 @@ SIGNATURES = {
     "dh_influence_wet": [_P, _P, C.c_float, _P, _P],
     "dh_influence_footprint": [_P, _P, C.POINTER(C.c_int32), C.POINTER(C.c_int64)],
     "dh_influence_count": [_P, _P, C.c_int64, C.POINTER(C.c_int64), _P, _P],
+    "dh_influence_count_many": [_P, C.c_int32, _P, C.c_float, _P, C.c_int64, _P, _P, _P],
     "dh_influence_fill": [_P, _P, _P, _P, _P],
     "dh_influence_info": [_P, C.POINTER(C.c_int32), _P, _P],
     "dh_influence_destroy": [_P],

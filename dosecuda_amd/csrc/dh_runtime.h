@@ -56,7 +56,10 @@ hipStream_t dh_stream(int dev);
 // old allocation (so it must not happen during stream capture).  Work that
 // uses the same workspace from different streams at the same time races;
 // the scheduler gives each of its streams a workspace of its own.
-constexpr int DH_MAX_WORKSPACES = 33;   // 0 = library default, 1.. scheduler streams
+constexpr int DH_SCHED_WORKSPACES = 32;                      // 1..32: the scheduler's streams
+constexpr int DH_INFL_STREAMS = 4;                           // streams of dh_influence_count_many
+constexpr int DH_WS_INFLUENCE = 1 + DH_SCHED_WORKSPACES;     // their workspaces, 33..36
+constexpr int DH_MAX_WORKSPACES = DH_WS_INFLUENCE + DH_INFL_STREAMS;   // 0 = library default
 void *dh_scratch(int dev, int slot, size_t bytes);
 void dh_set_workspace(int ws);
 int dh_get_workspace();

@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -677,6 +678,10 @@ struct dh_influence {
     const float *fp_wet = nullptr;
     dh_hbox fp_inner{0.0f, 0.0f, 0.0f, 0.0f, 0};
     int64_t fp_fallbacks = 0;
+    // pinned host copies of the count's results (hipHostMalloc on first use)
+    int *h_cnt = nullptr;
+    unsigned *h_pk = nullptr;
+    float *h_thr = nullptr, *h_r = nullptr;
 };
 
 static std::vector<dh_infl_block> make_blocks(const dh_influence *f, const std::vector<float> &r);
@@ -801,6 +806,9 @@ void dh_influence_free(dh_influence *f)
                   f->d_block_layer, f->d_block_n, f->d_block_pos};
     for (void *p : ps)
         if (p) (void)hipFree(p);
+    void *hs[] = {f->h_cnt, f->h_pk, f->h_thr, f->h_r};
+    for (void *p : hs)
+        if (p) (void)hipHostFree(p);
     delete f;
 }
 
@@ -1064,8 +1072,7 @@ static bool footprint_holds(const dh_influence *f, const std::vector<float> &thr
     return true;
 }
 
-// PEAK + COUNT; returns nnz and fills the host offsets (exclusive scan over
-// (spot, chunk) in row-major order, starting at base).
+// Per layer and image row, the x-interval of voxels the layer reaches.
 static void active_rows(dh_influence *f, const float *d_wet, hipStream_t s)
 {
     const int nrow = f->g.ny * f->g.nz;
@@ -1099,9 +1106,18 @@ static void wet_bounds(dh_influence *f, const float *d_wet, hipStream_t s)
     }
 }
 
-int64_t dh_influence_count(dh_influence *f, const float *d_wet, int64_t base, hipStream_t s)
+// Enqueues the bounds, PEAK and COUNT passes of a field and the copies of
+// their results (counts, peaks, thresholds, radii) into the field's pinned
+// host buffers; nothing waits.
+static void count_enqueue(dh_influence *f, const float *d_wet, hipStream_t s)
 {
     const int n = f->n_spots;
+    if (!f->h_cnt) {
+        DH_HIP(hipHostMalloc((void **)&f->h_cnt, sizeof(int) * std::max<size_t>((size_t)n * f->C, 1), 0));
+        DH_HIP(hipHostMalloc((void **)&f->h_pk, sizeof(unsigned) * std::max(n, 1), 0));
+        DH_HIP(hipHostMalloc((void **)&f->h_thr, sizeof(float) * std::max(n, 1), 0));
+        DH_HIP(hipHostMalloc((void **)&f->h_r, sizeof(float) * std::max(n, 1), 0));
+    }
     active_rows(f, d_wet, s);
     wet_bounds(f, d_wet, s);
     hipLaunchKernelGGL(k_zero_words, dim3(64), dim3(256), 0, s, f->d_peak, (size_t)std::max(n, 1));
@@ -1130,20 +1146,27 @@ int64_t dh_influence_count(dh_influence *f, const float *d_wet, int64_t base, hi
     device_radius(f, 1, s);
     device_blocks(f, s);
     launch_kernels<INF_COUNT>(f, (int)f->block_layer.size(), d_wet, nullptr, nullptr, s);
-    std::vector<int> cnt((size_t)n * f->C);
-    std::vector<unsigned> pk(std::max(n, 1));
-    std::vector<float> thr(std::max(n, 1)), r(std::max(n, 1));
     if (n) {
-        DH_HIP(hipMemcpyAsync(cnt.data(), f->d_counts, sizeof(int) * cnt.size(), hipMemcpyDeviceToHost, s));
-        DH_HIP(hipMemcpyAsync(pk.data(), f->d_peak, sizeof(unsigned) * n, hipMemcpyDeviceToHost, s));
-        DH_HIP(hipMemcpyAsync(thr.data(), f->d_thr, sizeof(float) * n, hipMemcpyDeviceToHost, s));
-        DH_HIP(hipMemcpyAsync(r.data(), f->d_radius, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+        DH_HIP(hipMemcpyAsync(f->h_cnt, f->d_counts, sizeof(int) * (size_t)n * f->C, hipMemcpyDeviceToHost, s));
+        DH_HIP(hipMemcpyAsync(f->h_pk, f->d_peak, sizeof(unsigned) * n, hipMemcpyDeviceToHost, s));
+        DH_HIP(hipMemcpyAsync(f->h_thr, f->d_thr, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+        DH_HIP(hipMemcpyAsync(f->h_r, f->d_radius, sizeof(float) * n, hipMemcpyDeviceToHost, s));
     }
-    DH_HIP(hipStreamSynchronize(s));
-    thr.resize(n);
-    r.resize(n);
+}
+
+int64_t dh_influence_count(dh_influence *f, const float *d_wet, int64_t base, hipStream_t s);
+
+// After count_enqueue's stream has drained: peaks, per-layer thresholds,
+// the footprint check (a field whose spots reach past its footprint gets the
+// full-grid WET and a recount on s, synchronised there) and the entry
+// offsets (exclusive scan over (spot, chunk) in row-major order from base);
+// returns nnz.
+static int64_t count_finish(dh_influence *f, const float *d_wet, int64_t base, hipStream_t s)
+{
+    const int n = f->n_spots;
+    std::vector<float> thr(f->h_thr, f->h_thr + n), r(f->h_r, f->h_r + n);
     f->peak.assign(n, 0.0f);
-    for (int q = 0; q < n; q++) std::memcpy(&f->peak[q], &pk[q], 4);
+    for (int q = 0; q < n; q++) std::memcpy(&f->peak[q], &f->h_pk[q], 4);
     for (size_t l = 0; l < f->lthr.size(); l++) f->lthr[l] = INFINITY;
     for (int q = 0; q < n; q++)
         if (f->spot_layer[q] >= 0) f->lthr[f->spot_layer[q]] = std::min(f->lthr[f->spot_layer[q]], thr[q]);
@@ -1154,20 +1177,86 @@ int64_t dh_influence_count(dh_influence *f, const float *d_wet, int64_t base, hi
         field_wet(f, f->fp_rlsp, const_cast<float *>(d_wet), nullptr, s);
         return dh_influence_count(f, d_wet, base, s);
     }
-    f->offsets.assign(cnt.size() + 1, 0);
+    f->offsets.assign((size_t)n * f->C + 1, 0);
     int64_t acc = base;
     for (int q = 0; q < n; q++) {
         const int l = f->spot_layer[q];
         for (int c = 0; c < f->C; c++) {
             f->offsets[(size_t)q * f->C + c] = acc;
-            if (l >= 0) acc += cnt[(size_t)q * f->C + c];
+            if (l >= 0) acc += f->h_cnt[(size_t)q * f->C + c];
         }
     }
-    f->offsets[cnt.size()] = acc;
+    f->offsets[(size_t)n * f->C] = acc;
     f->nnz = acc - base;
     f->radius = r;
     f->counted_blocks = true;      // d_blocks holds the COUNT pass's blocks, d_radius its radii
     return f->nnz;
+}
+
+// PEAK + COUNT; returns nnz and fills the host offsets (exclusive scan over
+// (spot, chunk) in row-major order, starting at base).
+int64_t dh_influence_count(dh_influence *f, const float *d_wet, int64_t base, hipStream_t s)
+{
+    count_enqueue(f, d_wet, s);
+    DH_HIP(hipStreamSynchronize(s));
+    return count_finish(f, d_wet, base, s);
+}
+
+// Streams (default priority) of dh_influence_count_many per device, each
+// with a scratch workspace of its own, and the fork event.
+struct infl_pool {
+    hipStream_t st[DH_INFL_STREAMS] = {};
+    hipEvent_t fork = nullptr;
+};
+
+static infl_pool &infl_pool_of(int dev)
+{
+    static std::mutex mu;
+    static infl_pool pools[64];
+    DH_REQUIRE(dev >= 0 && dev < 64, "device index out of range");
+    std::lock_guard<std::mutex> lk(mu);
+    infl_pool &P = pools[dev];
+    if (!P.fork) {
+        for (hipStream_t &st : P.st) DH_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        DH_HIP(hipEventCreateWithFlags(&P.fork, hipEventDisableTiming));
+    }
+    return P;
+}
+
+// Footprint WET + COUNT of n fields of one device, field q on stream
+// q mod DH_INFL_STREAMS with that stream's workspace, so the fields' passes
+// overlap on the GPU; one host synchronisation for all.  Every field's
+// results equal those of dh_influence_wet + dh_influence_count on it alone;
+// field q's entries start after those of fields 0..q-1 (from base).
+void dh_influence_count_many(dh_influence *const *fs, int n, const float *d_rlsp, float margin_scale,
+                             float *const *d_wet, int64_t base, int64_t *nnz, hipStream_t cs)
+{
+    if (n <= 0) return;
+    const int dev = fs[0]->dev;
+    infl_pool &P = infl_pool_of(dev);
+    DH_HIP(hipEventRecord(P.fork, cs));
+    const int used = std::min(n, DH_INFL_STREAMS);
+    for (int k = 0; k < used; k++) DH_HIP(hipStreamWaitEvent(P.st[k], P.fork, 0));
+    const int ws0 = dh_get_workspace();
+    try {
+        for (int q = 0; q < n; q++) {
+            const int k = q % DH_INFL_STREAMS;
+            dh_set_workspace(DH_WS_INFLUENCE + k);
+            footprint_wet(fs[q], d_rlsp, margin_scale, d_wet[q], P.st[k]);
+            count_enqueue(fs[q], d_wet[q], P.st[k]);
+        }
+    } catch (...) {
+        dh_set_workspace(ws0);
+        for (int k = 0; k < used; k++) (void)hipStreamSynchronize(P.st[k]);
+        throw;
+    }
+    dh_set_workspace(ws0);
+    for (int k = 0; k < used; k++) DH_HIP(hipStreamSynchronize(P.st[k]));
+    int64_t b = base;
+    for (int q = 0; q < n; q++) {
+        nnz[q] = count_finish(fs[q], d_wet[q], b, cs);
+        b += nnz[q];
+    }
 }
 
 // The COUNT pass's blocks, rebuilt on the device from its radii unless
@@ -1275,6 +1364,33 @@ extern "C" int dh_influence_count(dh_influence *f, const float *d_wet, int64_t b
             for (int q = 0; q < f->n_spots; q++) row_ptr[q] = f->offsets[(size_t)q * f->C];
             row_ptr[f->n_spots] = base + z;
         }
+    });
+}
+
+extern "C" int dh_influence_count_many(dh_influence *const *fields, int32_t n, const float *d_rlsp,
+                                       float margin_scale, float *const *d_wet, int64_t base, int64_t *nnz,
+                                       int64_t *const *row_ptr, void *stream)
+{
+    return dh_guarded([&] {
+        DH_REQUIRE(n >= 0 && (n == 0 || (fields && d_rlsp && d_wet && nnz)), "null argument");
+        DH_REQUIRE(margin_scale >= 0.0f, "margin_scale must be >= 0");
+        if (n == 0) return;
+        for (int q = 0; q < n; q++) {
+            DH_REQUIRE(fields[q] && d_wet[q], "null field or WET buffer");
+            DH_REQUIRE(fields[q]->dev == fields[0]->dev, "all fields must be on one device");
+            for (int p = 0; p < q; p++)
+                DH_REQUIRE(fields[p] != fields[q] && d_wet[p] != d_wet[q],
+                           "fields and WET buffers must be distinct");
+        }
+        dh_device_enter(fields[0]->dev);
+        dh_influence_count_many(fields, n, d_rlsp, margin_scale, d_wet, base, nnz, (hipStream_t)stream);
+        if (row_ptr)
+            for (int q = 0; q < n; q++) {
+                const dh_influence *f = fields[q];
+                if (!row_ptr[q]) continue;
+                for (int r = 0; r < f->n_spots; r++) row_ptr[q][r] = f->offsets[(size_t)r * f->C];
+                row_ptr[q][f->n_spots] = f->offsets[(size_t)f->n_spots * f->C];
+            }
     });
 }
 

@@ -311,7 +311,7 @@ extern "C" int dh_sched_create(int gpu_id, int32_t n_streams, dh_sched **out)
 {
     return dh_guarded([&] {
         DH_REQUIRE(out, "null output handle");
-        DH_REQUIRE(n_streams >= 1 && n_streams < DH_MAX_WORKSPACES, "1..32 streams");
+        DH_REQUIRE(n_streams >= 1 && n_streams <= DH_SCHED_WORKSPACES, "1..32 streams");
         dh_device_enter(gpu_id);
         std::unique_ptr<dh_sched> s(new dh_sched());
         s->gpu = gpu_id;

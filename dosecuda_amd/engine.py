@@ -19,6 +19,18 @@ def _stream(stream=None):
     return C.c_void_p(s.cuda_stream)
 
 
+_SIDE: dict = {}
+
+
+def _side_streams(device, n: int):
+    """n side streams (default priority) of a device, created once and reused."""
+    key = torch.device(device).index or 0
+    have = _SIDE.setdefault(key, [])
+    while len(have) < n:
+        have.append(torch.cuda.Stream(device=key))
+    return have[:n]
+
+
 def _dev(t: torch.Tensor) -> int:
     if not (t.is_cuda and t.dtype == torch.float32 and t.is_contiguous()):
         raise ValueError("expected a contiguous float32 tensor on the GPU")
@@ -174,6 +186,50 @@ class InfluenceMatrix:
         self.nnz = int(nnz.value)
         self.row_ptr = rp
         return self.nnz
+
+    @staticmethod
+    def count_many(fields, rlsp: torch.Tensor, wets, base: int = 0, margin_scale: float = 1.0,
+                   stream=None):
+        """wet() and count() of several fields of one device at once (the
+        fields' passes overlap on library streams, one host synchronisation);
+        field q's entries follow those of fields 0..q-1 from `base`.  Returns
+        the per-field entry counts; each field's nnz and row_ptr are set as
+        count() sets them."""
+        n = len(fields)
+        if len(wets) != n:
+            raise ValueError("one WET buffer per field")
+        hs = (C.c_void_p * max(n, 1))(*[f._h for f in fields])
+        ws = (C.c_void_p * max(n, 1))(*[_lib.ptr(w) for w in wets])
+        nnz = np.zeros(max(n, 1), np.int64)
+        rps = [np.zeros(f.n_rows + 1, np.int64) for f in fields]
+        rpp = (C.c_void_p * max(n, 1))(*[_lib.ptr(r) for r in rps])
+        _lib.check(_lib.load().dh_influence_count_many(hs, n, _lib.ptr(rlsp), float(margin_scale), ws, int(base),
+                                                       _lib.ptr(nnz), rpp, _stream(stream)))
+        for q, f in enumerate(fields):
+            f.nnz = int(nnz[q])
+            f.row_ptr = rps[q]
+        return [int(x) for x in nnz[:n]]
+
+    @staticmethod
+    def fill_many(fields, wets, col: torch.Tensor, val: torch.Tensor, stream=None):
+        """fill() of several counted fields into one shared CSR, each on a
+        side stream of its own forked from `stream` (the current stream by
+        default) and joined back to it, so the fills overlap on the GPU; the
+        entries are those of the fills in turn (each field writes only its own
+        slots)."""
+        if len(wets) != len(fields):
+            raise ValueError("one WET buffer per field")
+        cur = stream if stream is not None else torch.cuda.current_stream()
+        side = _side_streams(cur.device, len(fields))
+        for s in side:
+            s.wait_stream(cur)
+        for f, w, s in zip(fields, wets, side):
+            f.fill(w, col, val, stream=s)
+            # keep the buffers alive until the side stream has used them
+            for t in (w, col, val):
+                t.record_stream(s)
+        for s in side:
+            cur.wait_stream(s)
 
     def fill(self, wet: torch.Tensor, col: torch.Tensor, val: torch.Tensor, stream=None):
         _lib.check(_lib.load().dh_influence_fill(self._h, _lib.ptr(wet), _lib.ptr(col),

@@ -210,6 +210,18 @@ int dh_influence_wet(dh_influence *f, const float *d_rlsp, float margin_scale, f
 int dh_influence_footprint(const dh_influence *f, float box[4], int32_t *active, int64_t *fallbacks);
 int dh_influence_count(dh_influence *f, const float *d_wet, int64_t base, int64_t *nnz,
                        int64_t *row_ptr, void *stream);
+/* dh_influence_wet (footprint WET from the shared d_rlsp into d_wet[q]) and
+ * dh_influence_count for n fields of one device at once: the fields' passes
+ * run on library streams of their own (four, with scratch workspaces of
+ * their own), overlapping on the GPU, after the work already on `stream`,
+ * with one host synchronisation for all.  Field q's entries start at base +
+ * nnz[0] + ... + nnz[q - 1]; nnz[q] and row_ptr[q] (n_rows + 1 entries,
+ * skipped when row_ptr or row_ptr[q] is null) equal what the single-field
+ * calls in that order give.  Fields and WET buffers must be distinct.  No
+ * reference counterpart (the reference builds no influence matrix). */
+int dh_influence_count_many(dh_influence *const *fields, int32_t n, const float *d_rlsp, float margin_scale,
+                            float *const *d_wet, int64_t base, int64_t *nnz, int64_t *const *row_ptr,
+                            void *stream);
 int dh_influence_fill(dh_influence *f, const float *d_wet, uint32_t *d_col, float *d_val,
                       void *stream);
 int dh_influence_info(const dh_influence *f, int32_t *n_rows, int32_t *row_spot, float *row_peak);
