@@ -185,9 +185,45 @@ __device__ static inline bool dh_tile_voxel(const dh_tile &t, int b, int tid, in
 }
 
 // Ordered compaction by a 256-thread workgroup over K rounds of 256
-// consecutive items starting at `base`: flag(v) is evaluated once per item
-// (coalesced), the flagged items are appended to `list` in item order with
-// one atomic per workgroup (K times fewer than one per round).
+// consecutive items starting at `base`: item u stands for the value map(u)
+// (negative: none), flag(map(u)) is evaluated once per item, and the flagged
+// values are appended to `list` in item order with one atomic per workgroup
+// (K times fewer than one per round).
+template <int K, class Map, class Flag>
+__device__ static inline void dh_block_compact_map(uint32_t base, uint32_t n, Map map, Flag flag,
+                                                   int *__restrict__ list, int *__restrict__ counter)
+{
+    __shared__ int s_cnt[K][4];
+    __shared__ int s_base;
+    const int lane = __lane_id(), wave = threadIdx.x >> 6;
+    uint64_t m[K];
+    int val[K];
+#pragma unroll
+    for (int q = 0; q < K; q++) {
+        const uint32_t u = base + (uint32_t)q * 256u + threadIdx.x;
+        val[q] = u < n ? map(u) : -1;
+        const bool f = val[q] >= 0 && flag((uint32_t)val[q]);
+        m[q] = __builtin_amdgcn_ballot_w64(f);
+        if (lane == 0) s_cnt[q][wave] = __popcll(m[q]);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int q = 0; q < K; q++)
+            for (int w = 0; w < 4; w++) {
+                const int c = s_cnt[q][w];
+                s_cnt[q][w] = tot;
+                tot += c;
+            }
+        s_base = tot ? atomicAdd(counter, tot) : 0;
+    }
+    __syncthreads();
+    const uint64_t below = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int q = 0; q < K; q++)
+        if ((m[q] >> lane) & 1ull) list[s_base + s_cnt[q][wave] + __popcll(m[q] & below)] = val[q];
+}
+
 template <int K, class Flag>
 __device__ static inline void dh_block_compact(uint32_t base, uint32_t n, Flag flag, int *__restrict__ list,
                                                int *__restrict__ counter)

@@ -1101,8 +1101,13 @@ k_sur_dilate(sur_check_batch bt, int lz, int ly, int lx, int dilate)
     chk_redo(w, nc)[c] = fb ? 1 : 0;
 }
 
-// voxels with TERMA > 0.01 in cells marked for recomputation, in voxel order
+// voxels with TERMA > 0.01 in cells marked for recomputation, listed brick
+// by brick: a workgroup walks one 16 x 16 x 8 brick (x fastest), so the 64
+// consecutive list entries of a cone work item lie close together in 3-D
+// (in voxel order they would span whole rows, often both edges of a field)
 constexpr int kFallbackRounds = 8;
+constexpr int kFbBx = 16, kFbBy = 16, kFbBz = 8;
+static_assert(kFbBx * kFbBy * kFbBz == 256 * kFallbackRounds, "one brick per workgroup");
 
 __global__ void __launch_bounds__(256)
 k_sur_fallback(sur_check_batch bt, int band, int nz, int ny, int nx, int stride, int lz, int ly, int lx)
@@ -1113,8 +1118,17 @@ k_sur_fallback(sur_check_batch bt, int band, int nz, int ny, int nx, int stride,
     const float *__restrict__ terma = bt.terma[q];
     const float *__restrict__ electron = bt.electron[q];
     const int *__restrict__ redo = chk_redo(w, nc);
-    const uint32_t n = (uint32_t)nz * ny * nx;
-    dh_block_compact<kFallbackRounds>(blockIdx.x * (uint32_t)(256 * kFallbackRounds), n, [&](uint32_t v) {
+    const int nbx = (nx + kFbBx - 1) / kFbBx, nby = (ny + kFbBy - 1) / kFbBy;
+    // brick blockIdx.x: its voxels are items blockIdx.x * 2048 + (0 .. 2047)
+    const int bx = (int)(blockIdx.x % (uint32_t)nbx), rb = (int)(blockIdx.x / (uint32_t)nbx);
+    const int by = rb % nby, bz = rb / nby;
+    const uint32_t base = blockIdx.x * (uint32_t)(256 * kFallbackRounds);
+    auto voxel = [&](uint32_t u) -> int {
+        const int w = (int)(u - base);
+        const int i = bx * kFbBx + w % kFbBx, j = by * kFbBy + (w / kFbBx) % kFbBy, k = bz * kFbBz + w / (kFbBx * kFbBy);
+        return i < nx && j < ny && k < nz ? i + nx * (j + ny * k) : -1;
+    };
+    dh_block_compact_map<kFallbackRounds>(base, 0xffffffffu, voxel, [&](uint32_t v) {
         if (terma[v] <= 0.01f) return false;
         // the build-up band: a nonzero electron-contamination term means
         // WET < electron_fitted_dmax (IMRTClasses.cu:238-243)
@@ -1647,8 +1661,10 @@ void sur_check_multi(int dev, const dh_ccc_args *cas, int n, const sur_check_bat
     {
         dh_timed t("sur_fallback", s);
         hipLaunchKernelGGL(k_sur_dilate, dim3(cgrid, n), dim3(256), 0, s, b, lz, ly, lx, dilate & 3);
-        hipLaunchKernelGGL(k_sur_fallback, dim3((unsigned)((nv + 256 * kFallbackRounds - 1) / (256 * kFallbackRounds)), n),
-                           dim3(256), 0, s, b, (dilate & 4) ? 1 : 0, nz, ny, nx, stride, lz, ly, lx);
+        const unsigned bricks = (unsigned)((nx + kFbBx - 1) / kFbBx) * (unsigned)((ny + kFbBy - 1) / kFbBy) *
+                                (unsigned)((nz + kFbBz - 1) / kFbBz);
+        hipLaunchKernelGGL(k_sur_fallback, dim3(bricks, n), dim3(256), 0, s, b, (dilate & 4) ? 1 : 0, nz, ny, nx,
+                           stride, lz, ly, lx);
     }
     for (int q = 0; q < n; q++) {
         m.list[q] = b.work[q] + 4 + 4 * nc;
