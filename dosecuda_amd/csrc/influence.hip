@@ -1372,7 +1372,8 @@ extern "C" int dh_influence_count_many(dh_influence *const *fields, int32_t n, c
                                        int64_t *const *row_ptr, void *stream)
 {
     return dh_guarded([&] {
-        DH_REQUIRE(n >= 0 && (n == 0 || (fields && d_rlsp && d_wet && nnz)), "null argument");
+        DH_REQUIRE(n >= 0, "field count must be >= 0");
+        DH_REQUIRE(n == 0 || (fields && d_rlsp && d_wet && nnz), "null argument");
         DH_REQUIRE(margin_scale >= 0.0f, "margin_scale must be >= 0");
         if (n == 0) return;
         for (int q = 0; q < n; q++) {

@@ -156,3 +156,23 @@ def test_hu_conversion_clamps():
     assert r.dtype == np.float32
     np.testing.assert_allclose(r[[0, 1, 2, 4]], [0.00113, 0.00113, 1.00653, 2.36825], rtol=1e-6)
     np.testing.assert_allclose(density_from_hu(hu)[2], 1.00653, rtol=1e-6)
+
+
+def test_influence_count_many_argument_errors():
+    """The multi-field count rejects bad arguments before touching a device
+    (so the checks run here without a GPU) and does nothing for no fields."""
+    L = _lib.load()
+    assert L.dh_influence_count_many(None, 0, None, 1.0, None, 0, None, None, None) == 0
+    assert L.dh_influence_count_many(None, -1, None, 1.0, None, 0, None, None, None) != 0
+    assert "field count" in _lib.last_error()
+    assert L.dh_influence_count_many(None, 2, None, 1.0, None, 0, None, None, None) != 0
+    assert "null argument" in _lib.last_error()
+    import ctypes as C
+    fields = (C.c_void_p * 1)(None)
+    wets = (C.c_void_p * 1)(None)
+    nnz = np.zeros(1, np.int64)
+    dummy = np.zeros(4, np.float32)
+    rc = L.dh_influence_count_many(fields, 1, _lib.ptr(dummy), 1.0, wets, 0, _lib.ptr(nnz), None, None)
+    assert rc != 0 and "null field or WET buffer" in _lib.last_error()
+    rc = L.dh_influence_count_many(fields, 1, _lib.ptr(dummy), -1.0, wets, 0, _lib.ptr(nnz), None, None)
+    assert rc != 0 and "margin_scale" in _lib.last_error()
